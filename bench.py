@@ -1,0 +1,238 @@
+"""Benchmark: closest-hit Mrays/s on the 3,407,872-triangle terrain scene.
+
+One step = one frame of the closest-hit chain on one MI355X, inputs resident
+in HBM: a fused 1920x1080 pinhole primary trace (coherent class), secondary-ray
+generation from the primary hits (one cosine-weighted diffuse bounce and one
+shadow ray per hit, stable wave64 compaction), and the closest-hit trace of
+the diffuse rays (incoherent class), read off a device-side ray count. The
+step's value is (primary + diffuse rays) / step time. The shadow any-hit rays
+of the same frame are traced after each step and reported as their own class.
+
+Roofline for the dominant kernel (the incoherent closest-hit trace):
+algorithmic bytes per launch = sum over rays of 64*nodes + 36*tris + 32 + 16
+(BVH4 node, SoA triangle, explicit ray in, hit out), with node and triangle
+counts from the counted kernel, whose counts equal the CPU oracle's. The CPU
+baseline is the oracle's multithreaded traversal on a bounded ray sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+METRIC = "Mrays/s closest-hit (coherent/incoherent, 2M tris); equal-time denoised PSNR gain"
+HBM_PEAK_GBS = 8000.0
+RAY_KEYS = ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 18, help="rays per class for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the ray workload would shard by image tiles, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    pkg = importlib.import_module("vk-rt-cuda-raytracing_amd")
+    scenes = importlib.import_module("vk-rt-cuda-raytracing_amd.scenes")
+    dev = torch.device("cuda:0")
+    W, H = args.width, args.height
+    n = W * H
+
+    t0 = time.time()
+    tris = scenes.terrain_scene(seed=args.seed)
+    log(f"scene: {tris.shape[0]} triangles ({time.time() - t0:.1f} s)")
+    t0 = time.time()
+    bvh = pkg.BVH.build(tris)
+    build_s = time.time() - t0
+    log(f"build: {build_s:.2f} s, {bvh.num_bvh4_nodes} BVH4 nodes, depth {bvh.depth}")
+
+    stream = torch.cuda.Stream()
+    bvh.set_stream(stream.cuda_stream)
+    cam = scenes.terrain_camera(W, H)
+    c = pkg.camera(cam["o"], cam["f"], cam["r"], cam["u"], cam["scale_x"], cam["scale_y"])
+    light = scenes.TERRAIN_LIGHT
+
+    with torch.cuda.stream(stream):
+        pt = torch.empty(n, device=dev)
+        pid = torch.empty(n, dtype=torch.int32, device=dev)
+        pu = torch.empty(n, device=dev)
+        pv = torch.empty(n, device=dev)
+        D = {k: torch.empty(n, device=dev) for k in RAY_KEYS}
+        S = {k: torch.empty(n, device=dev) for k in RAY_KEYS}
+        src = torch.empty(n, dtype=torch.int32, device=dev)
+        cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+        dt = torch.empty(n, device=dev)
+        did = torch.empty(n, dtype=torch.int32, device=dev)
+        du = torch.empty(n, device=dev)
+        dv = torch.empty(n, device=dev)
+        occ = torch.empty(n, dtype=torch.uint8, device=dev)
+
+    def step(ev=None):
+        if ev: ev[0].record(stream)
+        bvh.trace_pinhole_closest(W, H, c, pt, pid, pu, pv)
+        if ev: ev[1].record(stream)
+        bvh.gen_bounce_rays(W, H, c, pt, pid, 1234, light, D, S, src, cnt)
+        if ev: ev[2].record(stream)
+        bvh.trace_closest_queued(n, cnt, D, dt, did, du, dv)
+        if ev: ev[3].record(stream)
+        bvh.trace_any_queued(n, cnt, S, occ)
+        if ev: ev[4].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    m = int(cnt.item())
+    hit_rate = float((pid >= 0).float().mean().item())
+
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall_s = time.perf_counter() - t_wall
+    seg = np.array([[e[i].elapsed_time(e[i + 1]) for i in range(4)] for e in evs])   # ms
+    prim_ms, bounce_ms, diff_ms, shadow_ms = seg.mean(axis=0)
+    step_ms = prim_ms + bounce_ms + diff_ms
+    value = (n + m) / (step_ms * 1e3)
+    diff_hit = float((did[:m] >= 0).float().mean().item())
+    occ_rate = float(occ[:m].float().mean().item())
+    log(f"primary {prim_ms:.3f} ms  bounce {bounce_ms:.3f} ms  diffuse {diff_ms:.3f} ms  shadow {shadow_ms:.3f} ms"
+        f"  (wall {wall_s / args.steps * 1e3:.3f} ms/step incl. shadow)")
+
+    # Algorithmic bytes of the dominant kernel (incoherent closest hit) from
+    # exact visit counts; the counted kernel's counts equal the oracle's.
+    with torch.cuda.stream(stream):
+        cn = torch.empty(n, dtype=torch.int32, device=dev)
+        ct = torch.empty(n, dtype=torch.int32, device=dev)
+        Dm = {k: v[:m] for k, v in D.items()}
+        bvh.trace_closest_counted(Dm, dt[:m], did[:m], du[:m], dv[:m], cn[:m], ct[:m])
+        pcn = torch.empty(n, dtype=torch.int32, device=dev)
+        pct = torch.empty(n, dtype=torch.int32, device=dev)
+        Pr = {k: torch.empty(n, device=dev) for k in RAY_KEYS}
+        pkg.gen_pinhole_rays(W, H, c, Pr, stream.cuda_stream)
+        bvh.trace_closest_counted(Pr, pt, pid, pu, pv, pcn, pct)
+    torch.cuda.synchronize()
+    d_nodes, d_tris = int(cn[:m].sum().item()), int(ct[:m].sum().item())
+    p_nodes, p_tris = int(pcn.sum().item()), int(pct.sum().item())
+    diff_bytes = 64 * d_nodes + 36 * d_tris + (32 + 16) * m
+    achieved = diff_bytes / (diff_ms * 1e-3) / 1e9
+    prim_bytes = 64 * p_nodes + 36 * p_tris + 16 * n
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(bvh, Pr, D, m, args.cpu_sample)
+
+    prof_traffic = load_traffic()
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "terrain: 1024x1024-cell heightfield + 256 level-4 icospheres, 1920x1080 primary + "
+                        "one diffuse bounce per hit (closest hit); shadow any-hit reported per class",
+            "triangles": int(tris.shape[0]), "width": W, "height": H, "seed": args.seed,
+            "camera": {"eye": scenes.TERRAIN_EYE, "target": scenes.TERRAIN_TARGET, "vfov_deg": 60},
+            "light": light, "bvh": "binned-SAH BVH4, 16 bins, <=4 tris/leaf, FP16 bounds rounded outward",
+            "isect": "watertight",
+        },
+        "classes": {
+            "coherent_closest": {"rays": n, "ms": round(prim_ms, 4), "mrays_s": round(n / (prim_ms * 1e3), 1),
+                                 "hit_rate": round(hit_rate, 4), "nodes_per_ray": round(p_nodes / n, 2),
+                                 "tris_per_ray": round(p_tris / n, 2),
+                                 "alg_GBps": round(prim_bytes / (prim_ms * 1e-3) / 1e9, 1)},
+            "incoherent_closest": {"rays": m, "ms": round(diff_ms, 4), "mrays_s": round(m / (diff_ms * 1e3), 1),
+                                   "hit_rate": round(diff_hit, 4), "nodes_per_ray": round(d_nodes / max(m, 1), 2),
+                                   "tris_per_ray": round(d_tris / max(m, 1), 2)},
+            "bounce_generation": {"ms": round(bounce_ms, 4)},
+            "shadow_any": {"rays": m, "ms": round(shadow_ms, 4), "mrays_s": round(m / (shadow_ms * 1e3), 1),
+                           "occluded": round(occ_rate, 4)},
+        },
+        "roofline": {
+            "kernel": "k_trace closest-hit, explicit rays (incoherent diffuse class)",
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": prof_traffic,
+            "alg_bytes_per_launch": diff_bytes, "kernel_ms": round(diff_ms, 4),
+        },
+        "cpu_baseline": cpu,
+        "build_s": round(build_s, 2),
+    }
+    print(json.dumps(out), flush=True)
+    bvh.set_stream(None)
+    bvh.destroy()
+
+
+def load_traffic():
+    """HBM bytes per launch of the dominant kernel, measured with rocprofv3
+    PMC counters and committed under profiles/ (null until measured)."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("incoherent_closest_bytes_per_launch")
+
+
+def cpu_baseline(bvh, Pr, D, m, sample):
+    """The CPU oracle's traversal (C, OpenMP over all host cores) on an evenly
+    strided sample of the same primary and diffuse rays and the same BVH."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    nodes = bvh.node_data()
+    lt = np.ascontiguousarray(bvh.tri_data().T)
+    pid = bvh.leaf_prim_ids()
+    n = Pr["ox"].shape[0]
+    ip = np.linspace(0, n - 1, min(sample, n)).astype(np.int64)
+    idd = np.linspace(0, m - 1, min(sample, m)).astype(np.int64)
+    prim = np.stack([Pr[k].cpu().numpy()[ip] for k in RAY_KEYS]).astype(np.float32)
+    diff = np.stack([D[k][:m].cpu().numpy()[idd] for k in RAY_KEYS]).astype(np.float32)
+    threads = pyoracle.num_threads(0)
+    pyoracle.trace_closest(nodes, lt, pid, prim[:, :4096], counts=False)       # warm-up
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        pyoracle.trace_closest(nodes, lt, pid, prim, counts=False)
+        pyoracle.trace_closest(nodes, lt, pid, diff, counts=False)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    rays = prim.shape[1] + diff.shape[1]
+    log(f"cpu baseline: {rays} rays in {t:.3f} s on {threads} threads")
+    return {"value": round(rays / t / 1e6, 2), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{prim.shape[1]} primary + {diff.shape[1]} diffuse rays (evenly strided), closest hit, "
+                      f"watertight, same BVH; median of 3"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,240 @@
+/* cudabvh.h — C ABI of the MI355X ray-tracing core (libcudabvh_hip.so).
+ *
+ * Part 1 keeps every symbol of the reference boundary
+ * layer/cuda_bvh_backend.h:12-141 with the same C signature, so the
+ * reference's Vulkan layer (layer/VkLayer_CudaRT.cpp) or bvh_bench.cu links
+ * against this library unchanged. Part 2 adds entry points in the same style
+ * that the reference lacks (hit IDs and barycentrics, any-hit, device-pointer
+ * packets, error reporting). Part 2 is new API with no reference counterpart.
+ *
+ * Conventions (as the reference): handles are opaque; getters return borrowed
+ * host memory valid until cudaBVH_destroy; int returns are 0 on success and
+ * -1 on failure; float returns are MRays/s and 0 on failure. Unlike the
+ * reference, every failure also records a message for cudaBVH_lastError().
+ * A handle must not be used from two threads at once; distinct handles are
+ * independent (no process-global BVH state).
+ */
+#ifndef CUDABVH_H
+#define CUDABVH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================== Part 1: reference boundary ===================== */
+
+/* layer/cuda_bvh_backend.h:13 */
+typedef struct CudaBVHHandle* CudaBVH_t;
+
+/* layer/cuda_bvh_backend.h:16-18 */
+typedef struct {
+    float v0[3], v1[3], v2[3];
+} CudaTri;
+
+/* layer/cuda_bvh_backend.h:21. Copies the input. Builds on the GPU: binned
+ * SAH (16 bins, <= 4 triangles per leaf) -> BVH4 with FP16 child bounds
+ * rounded outward. numTris <= 0 or tris == NULL: returns NULL (the reference
+ * has undefined behaviour there). */
+CudaBVH_t cudaBVH_build(const CudaTri* tris, int numTris);
+
+/* layer/cuda_bvh_backend.h:24. Accepts NULL. */
+void cudaBVH_destroy(CudaBVH_t bvh);
+
+/* layer/cuda_bvh_backend.h:29-31. side x side pinhole rays from camO looking
+ * +z (90 degree FOV, backend :69-74). outHitT: nullable HOST array of side^2
+ * floats, row-major py*side+px; misses are 1e30. Returns MRays/s over the
+ * trace kernel; 0 on error. */
+float cudaBVH_tracePrimary(CudaBVH_t bvh, int side,
+                           float camOx, float camOy, float camOz,
+                           float* outHitT);
+
+/* layer/cuda_bvh_backend.h:37-40. Host SoA rays (t in (0.001, 1e30) as
+ * cuda_rt_v40.cu:550), sorted on the GPU by octant+Morton key, traced, hit
+ * distances written to outHitT (host) in the original order. Returns MRays/s
+ * over the trace kernel. */
+float cudaBVH_traceDiffuse(CudaBVH_t bvh, int numRays,
+                           const float* rayOx, const float* rayOy, const float* rayOz,
+                           const float* rayDx, const float* rayDy, const float* rayDz,
+                           float* outHitT);
+
+/* layer/cuda_bvh_backend.h:45-47. Look-at camera toward the scene centre,
+ * shaded RGBA8 into a host buffer. 0 on success. */
+int cudaBVH_traceToRGBA(CudaBVH_t bvh, int width, int height,
+                        float camOx, float camOy, float camOz,
+                        uint32_t* outRGBA_host);
+
+/* layer/cuda_bvh_backend.h:54-56. d_outRGBA is device memory; outFmt 0 RGBA8,
+ * 1 BGRA8, 2 RGBA16F, 3 RGBA32F. Asynchronous on the handle's stream. */
+int cudaBVH_traceToGPUPtr(CudaBVH_t bvh, int width, int height,
+                          float camOx, float camOy, float camOz,
+                          void* d_outRGBA, int outFmt);
+
+/* layer/cuda_bvh_backend.h:62-64. Fire-and-forget on a background thread;
+ * the first call blocks until its frame is done. */
+void cudaBVH_traceToGPUAsync(CudaBVH_t bvh, int width, int height,
+                             float camOx, float camOy, float camOz,
+                             void* d_outRGBA, int outFmt);
+
+/* layer/cuda_bvh_backend.h:71-73. Background trace into double-buffered host
+ * memory; copies the last finished frame into h_outRGBA. */
+int cudaBVH_traceToHostPtr(CudaBVH_t bvh, int width, int height,
+                           float camOx, float camOy, float camOz,
+                           void* h_outRGBA, int outFmt, uint64_t bufSize);
+
+/* layer/cuda_bvh_backend.h:78. Imports an opaque POSIX fd as device memory
+ * (hipImportExternalMemory). NULL on failure. */
+void* cudaBVH_importBufferFd(int fd, uint64_t size);
+
+/* layer/cuda_bvh_backend.h:81. Out-pointers nullable. */
+void cudaBVH_getBounds(CudaBVH_t bvh, float* centerX, float* centerY, float* centerZ, float* extent);
+
+/* layer/cuda_bvh_backend.h:84-86. 0 for NULL. getNumCWBVHNodes returns the
+ * real compressed-wide-BVH node count (0 until one is built). */
+int cudaBVH_getNumTris(CudaBVH_t bvh);
+int cudaBVH_getNumBVH4Nodes(CudaBVH_t bvh);
+int cudaBVH_getNumCWBVHNodes(CudaBVH_t bvh);
+
+/* layer/cuda_bvh_backend.h:91. numNodes x 64 bytes: halves boundsX[8],
+ * boundsY[8], boundsZ[8] (mins in 0-3, maxes in 4-7, rounded outward), then
+ * int32 child[4] (-1 empty, >= 0 node, <= -2 leaf = -((first<<3)|(count-1))-2). */
+const void* cudaBVH_getNodeData(CudaBVH_t bvh);
+
+/* layer/cuda_bvh_backend.h:96. Nine leaf-order SoA float arrays. */
+void cudaBVH_getTriData(CudaBVH_t bvh, const float* outPtrs[9]);
+
+/* layer/cuda_bvh_backend.h:103. 8 uint32 per node: bmin[3], bmax[3] (fp32),
+ * leaf_enc (0 inner), skip (-1 past the end). Returns the node count. */
+int cudaBVH_getStacklessBVH2(CudaBVH_t bvh, uint32_t** outData);
+
+/* layer/cuda_bvh_backend.h:111. 3 vec4 per leaf-order triangle:
+ * {v0.xyz, v1.x} {v1.yz, v2.xy} {v2.z, float(originalIndex), 0, 0}.
+ * Returns the vec4 count. */
+int cudaBVH_getPackedTris(CudaBVH_t bvh, float** outData);
+
+/* layer/cuda_bvh_backend.h:116. Waits for all library work and releases
+ * library-global device resources. The device itself is not reset because
+ * the library shares it with its host process. */
+void cudaBVH_resetDevice(void);
+
+/* layer/cuda_bvh_backend.h:123. One degenerate triangle per AABB (backend
+ * :1549-1563), built like cudaBVH_build. NULL if numAABBs <= 0. */
+CudaBVH_t cudaBVH_buildFromAABBs(const float* aabbs, int numAABBs);
+
+/* layer/cuda_bvh_backend.h:130-131. LBVH (Morton, median split, one item per
+ * leaf) -> stackless BVH2. *outNodes and *outOrdered are malloc'd; the caller
+ * frees them with free(). Returns the node count (0 on error). */
+int cudaBVH_buildTLASFast(const float* aabbs, int numAABBs, uint32_t** outNodes,
+                          int** outOrdered
+#ifdef __cplusplus
+                          = nullptr
+#endif
+                          );
+
+/* layer/cuda_bvh_backend.h:140-141. 0 on success, -1 on failure. */
+int cudaBVH_readGPUMemoryFd(int fd, uint64_t allocationSize,
+                            uint64_t offset, uint64_t size, void* dst);
+
+/* ===================== Part 2: extensions (no reference counterpart) ===================== */
+
+/* Structure-of-arrays rays: a hit is valid when tmin < t < tmax. */
+typedef struct {
+    const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+} CudaRayPacket;
+
+/* Closest-hit output: t (1e30 on a miss), ORIGINAL primitive index (-1 on a
+ * miss), barycentrics u (weight of v1) and v (weight of v2). */
+typedef struct {
+    float* t;
+    int*   primId;
+    float* u;
+    float* v;
+} CudaHitPacket;
+
+/* Pinhole camera: origin, forward, right, up (need not be unit), and the
+ * image-plane scales; pixel (px,py) of W x H looks along
+ * f + (sx*scaleX)*r + (sy*scaleY)*u, sx = ((px+0.5)/W)*2-1, sy likewise. */
+typedef struct {
+    float o[3], f[3], r[3], u[3];
+    float scaleX, scaleY;
+} CudaCamera;
+
+enum {
+    CUDABVH_ISECT_WATERTIGHT   = 0,       /* Woop-Benthin-Wald 2013, default */
+    CUDABVH_ISECT_REFERENCE_MT = 1,       /* backend :145-156 Moller-Trumbore */
+    CUDABVH_ISECT_MASK         = 0xF,
+    CUDABVH_FLAG_HOST_POINTERS = 1u << 4, /* packets hold host pointers: copy in/out */
+    CUDABVH_FLAG_SORT_RAYS     = 1u << 5, /* octant+Morton sort before tracing */
+    CUDABVH_FLAG_SYNC          = 1u << 6  /* wait for completion before returning */
+};
+
+/* Closest hit for n rays. Device pointers unless CUDABVH_FLAG_HOST_POINTERS.
+ * Asynchronous on the handle's stream unless CUDABVH_FLAG_SYNC or host
+ * pointers. 0 on success. */
+int cudaBVH_traceClosest(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays,
+                         const CudaHitPacket* hits, unsigned flags);
+
+/* Any hit: occluded[i] = 1 when some triangle has tmin < t < tmax. */
+int cudaBVH_traceAny(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays,
+                     uint8_t* occluded, unsigned flags);
+
+/* Closest hit with per-ray counters (BVH4 nodes fetched, triangles tested);
+ * the counters equal the CPU oracle's on the same BVH and rays. Device
+ * pointers. Slower than cudaBVH_traceClosest; for measurement and tests. */
+int cudaBVH_traceClosestCounted(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays,
+                                const CudaHitPacket* hits, int* nodeVisits, int* triTests,
+                                unsigned flags);
+int cudaBVH_traceAnyCounted(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays,
+                            uint8_t* occluded, int* nodeVisits, int* triTests, unsigned flags);
+
+/* Writes W*H pinhole rays (tmin 0, tmax 1e30) into a device packet. */
+int cudaBVH_genPinholeRays(int width, int height, const CudaCamera* cam,
+                           const CudaRayPacket* out, void* stream);
+
+/* Fused generation + closest hit for a pinhole camera (no ray buffers:
+ * coherent primary rays). Device hit packet of W*H entries. */
+int cudaBVH_tracePinholeClosest(CudaBVH_t bvh, int width, int height, const CudaCamera* cam,
+                                const CudaHitPacket* hits, unsigned flags);
+
+/* Secondary rays of a pinhole frame (camera as in tracePinholeClosest): for
+ * every pixel whose primary hit has primId >= 0, in pixel order, one
+ * cosine-weighted diffuse ray about the geometric normal facing the camera
+ * (PCG32 stream seeded by (seed, pixel), t in (1e-3, 1e30)) and one shadow
+ * ray toward light (t in (1e-3, distance)). Outputs are device packets of
+ * W*H capacity; srcIndex[k] is the pixel of ray k; *d_count (device) the
+ * number written. No host synchronisation. */
+int cudaBVH_genBounceRays(CudaBVH_t bvh, int width, int height, const CudaCamera* cam,
+                          const CudaHitPacket* primary, uint32_t seed, const float light[3],
+                          const CudaRayPacket* diffuseOut, const CudaRayPacket* shadowOut,
+                          int* srcIndex, int* d_count, unsigned flags);
+
+/* Device work queue traces: the ray count is read from device memory
+ * (*d_count, at most capacity) by the kernel itself, so a chain of
+ * generate -> trace launches needs no host round trip. */
+int cudaBVH_traceClosestQueued(CudaBVH_t bvh, int capacity, const int* d_count,
+                               const CudaRayPacket* rays, const CudaHitPacket* hits, unsigned flags);
+int cudaBVH_traceAnyQueued(CudaBVH_t bvh, int capacity, const int* d_count,
+                           const CudaRayPacket* rays, uint8_t* occluded, unsigned flags);
+
+/* Handle stream: all work of the handle is issued on this HIP stream
+ * (NULL = the null stream). */
+int   cudaBVH_setStream(CudaBVH_t bvh, void* hipStream);
+void* cudaBVH_getStream(CudaBVH_t bvh);
+
+/* Device views of the built structure (borrowed, valid until destroy):
+ * BVH4 nodes (64 B each), leaf-order triangles (3 x float4 each:
+ * {v0.xyz, bits(primId)}, {v1.xyz, 0}, {v2.xyz, 0}). */
+const void* cudaBVH_deviceNodes(CudaBVH_t bvh);
+const void* cudaBVH_deviceTris(CudaBVH_t bvh);
+/* Depth of the BVH4 (node levels on the longest root-to-leaf path). */
+int cudaBVH_getDepth(CudaBVH_t bvh);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char* cudaBVH_lastError(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUDABVH_H */

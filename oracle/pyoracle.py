@@ -1,0 +1,218 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
+benchmark's CPU-baseline leg, where it is the checker. The product library
+never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        P, I, F = C.c_void_p, C.c_int, C.c_float
+        sig = {
+            "orc_f2h_rne": (C.c_uint16, [F]), "orc_f2h_down": (C.c_uint16, [F]), "orc_f2h_up": (C.c_uint16, [F]),
+            "orc_h2f": (F, [C.c_uint16]),
+            "orc_build": (P, [P, I, I]), "orc_free": (None, [P]),
+            "orc_num_bvh2": (I, [P]), "orc_num_bvh4": (I, [P]), "orc_num_ordered": (I, [P]),
+            "orc_bvh4_depth": (I, [P]), "orc_get_bvh4": (None, [P, P]), "orc_get_order": (None, [P, P]),
+            "orc_get_bvh2": (None, [P, P, P]), "orc_num_dfs": (I, [P]), "orc_get_dfs": (None, [P, P]),
+            "orc_sah_cost": (C.c_double, [P]),
+            "orc_lbvh_tlas": (I, [P, I, P, P]), "orc_release": (None, [P]),
+            "orc_morton3d": (C.c_uint32, [F, F, F]),
+            "orc_gen_pinhole": (None, [P, I, I, P]),
+            "orc_trace_closest": (None, [P, I, P, P, I, P, I, P, P, P, P, P, P, I, I]),
+            "orc_trace_any": (None, [P, I, P, P, I, P, I, P, P, P, I, I]),
+            "orc_brute_closest": (None, [P, P, I, P, I, P, P, P, P, I, I]),
+            "orc_brute_closest_f64": (None, [P, P, I, P, I, P, P, I]),
+            "orc_isect_watertight": (I, [P, P, P, P, P]), "orc_isect_mt": (I, [P, P, P, P, P]),
+            "orc_sort_keys": (None, [P, I, P, P, P]),
+            "orc_num_threads": (I, [I]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype, f.argtypes = r, a
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+def num_threads(requested: int = 0) -> int:
+    return lib().orc_num_threads(requested)
+
+
+class OracleBVH:
+    """CPU binned-SAH BVH2 -> BVH4 (layer/cuda_bvh_backend.cu:201-251, :391-418)."""
+
+    def __init__(self, tris: np.ndarray, max_leaf: int = 4):
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        self.tris = t
+        self.h = lib().orc_build(t.ctypes.data, t.shape[0], max_leaf)
+        if not self.h:
+            raise ValueError("oracle build rejected its input")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_free(self.h)
+            self.h = None
+
+    @property
+    def num_bvh4(self) -> int:
+        return lib().orc_num_bvh4(self.h)
+
+    @property
+    def depth(self) -> int:
+        return lib().orc_bvh4_depth(self.h)
+
+    def nodes(self) -> np.ndarray:
+        out = np.zeros((self.num_bvh4, 64), np.uint8)
+        lib().orc_get_bvh4(self.h, out.ctypes.data)
+        return out
+
+    def order(self) -> np.ndarray:
+        out = np.zeros(lib().orc_num_ordered(self.h), np.int32)
+        lib().orc_get_order(self.h, out.ctypes.data)
+        return out
+
+    def bvh2(self):
+        n = lib().orc_num_bvh2(self.h)
+        boxes = np.zeros((n, 6), np.float32)
+        links = np.zeros((n, 4), np.int32)
+        lib().orc_get_bvh2(self.h, boxes.ctypes.data, links.ctypes.data)
+        return boxes, links
+
+    def dfs(self) -> np.ndarray:
+        n = lib().orc_num_dfs(self.h)
+        out = np.zeros((n, 8), np.uint32)
+        lib().orc_get_dfs(self.h, out.ctypes.data)
+        return out
+
+    def sah_cost(self) -> float:
+        return lib().orc_sah_cost(self.h)
+
+    def leaf_tris(self) -> np.ndarray:
+        return np.ascontiguousarray(self.tris[self.order()])
+
+
+def trace_closest(nodes: np.ndarray, leaf_tris: np.ndarray, prim_id: np.ndarray, rays: np.ndarray,
+                  isect: int = 0, threads: int = 0, counts: bool = True):
+    """rays: float32 (8, n). Returns dict t, id, u, v (+ nodes, tris counters)."""
+    n = rays.shape[1]
+    rays = np.ascontiguousarray(rays, np.float32)
+    lt = np.ascontiguousarray(leaf_tris, np.float32).reshape(-1, 9)
+    pid = np.ascontiguousarray(prim_id, np.int32)
+    nd = np.ascontiguousarray(nodes, np.uint8)
+    out = dict(t=np.zeros(n, np.float32), id=np.zeros(n, np.int32), u=np.zeros(n, np.float32),
+               v=np.zeros(n, np.float32))
+    if counts:
+        out["nodes"] = np.zeros(n, np.int32)
+        out["tris"] = np.zeros(n, np.int32)
+    lib().orc_trace_closest(nd.ctypes.data, nd.shape[0], lt.ctypes.data, pid.ctypes.data, lt.shape[0],
+                            rays.ctypes.data, n, out["t"].ctypes.data, out["id"].ctypes.data, out["u"].ctypes.data,
+                            out["v"].ctypes.data, _p(out.get("nodes")), _p(out.get("tris")), isect, threads)
+    return out
+
+
+def trace_any(nodes, leaf_tris, prim_id, rays, isect: int = 0, threads: int = 0, counts: bool = True):
+    n = rays.shape[1]
+    rays = np.ascontiguousarray(rays, np.float32)
+    lt = np.ascontiguousarray(leaf_tris, np.float32).reshape(-1, 9)
+    pid = np.ascontiguousarray(prim_id, np.int32)
+    nd = np.ascontiguousarray(nodes, np.uint8)
+    out = dict(occ=np.zeros(n, np.uint8))
+    if counts:
+        out["nodes"] = np.zeros(n, np.int32)
+        out["tris"] = np.zeros(n, np.int32)
+    lib().orc_trace_any(nd.ctypes.data, nd.shape[0], lt.ctypes.data, pid.ctypes.data, lt.shape[0], rays.ctypes.data,
+                        n, out["occ"].ctypes.data, _p(out.get("nodes")), _p(out.get("tris")), isect, threads)
+    return out
+
+
+def brute_closest(tris, prim_id, rays, isect: int = 0, threads: int = 0):
+    n = rays.shape[1]
+    rays = np.ascontiguousarray(rays, np.float32)
+    t9 = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    pid = np.ascontiguousarray(prim_id, np.int32)
+    out = dict(t=np.zeros(n, np.float32), id=np.zeros(n, np.int32), u=np.zeros(n, np.float32),
+               v=np.zeros(n, np.float32))
+    lib().orc_brute_closest(t9.ctypes.data, pid.ctypes.data, t9.shape[0], rays.ctypes.data, n, out["t"].ctypes.data,
+                            out["id"].ctypes.data, out["u"].ctypes.data, out["v"].ctypes.data, isect, threads)
+    return out
+
+
+def brute_closest_f64(tris, prim_id, rays, threads: int = 0):
+    n = rays.shape[1]
+    rays = np.ascontiguousarray(rays, np.float32)
+    t9 = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    pid = np.ascontiguousarray(prim_id, np.int32)
+    t = np.zeros(n, np.float64)
+    ids = np.zeros(n, np.int32)
+    lib().orc_brute_closest_f64(t9.ctypes.data, pid.ctypes.data, t9.shape[0], rays.ctypes.data, n, t.ctypes.data,
+                                ids.ctypes.data, threads)
+    return dict(t=t, id=ids)
+
+
+def gen_pinhole(cam14: np.ndarray, W: int, H: int) -> np.ndarray:
+    out = np.zeros((8, W * H), np.float32)
+    c = np.ascontiguousarray(cam14, np.float32)
+    lib().orc_gen_pinhole(c.ctypes.data, W, H, out.ctypes.data)
+    return out
+
+
+def isect(ray8, tri9, mode: int = 0):
+    r = np.ascontiguousarray(ray8, np.float32)
+    t9 = np.ascontiguousarray(tri9, np.float32)
+    t, u, v = C.c_float(), C.c_float(), C.c_float()
+    f = lib().orc_isect_mt if mode == 1 else lib().orc_isect_watertight
+    hit = f(r.ctypes.data, t9.ctypes.data, C.byref(t), C.byref(u), C.byref(v))
+    return bool(hit), t.value, u.value, v.value
+
+
+def lbvh_tlas(aabbs: np.ndarray):
+    a = np.ascontiguousarray(aabbs, np.float32).reshape(-1, 6)
+    out, ordp = C.c_void_p(), C.c_void_p()
+    n = lib().orc_lbvh_tlas(a.ctypes.data, a.shape[0], C.byref(out), C.byref(ordp))
+    nodes = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_uint32)), shape=(n * 8,)).reshape(n, 8).copy()
+    order = np.ctypeslib.as_array(C.cast(ordp, C.POINTER(C.c_int32)), shape=(a.shape[0],)).copy()
+    lib().orc_release(out)
+    lib().orc_release(ordp)
+    return nodes, order
+
+
+def sort_keys(rays: np.ndarray, lo, inv) -> np.ndarray:
+    rays = np.ascontiguousarray(rays, np.float32)
+    n = rays.shape[1]
+    keys = np.zeros(n, np.uint32)
+    lo = np.asarray(lo, np.float32)
+    inv = np.asarray(inv, np.float32)
+    lib().orc_sort_keys(rays.ctypes.data, n, lo.ctypes.data, inv.ctypes.data, keys.ctypes.data)
+    return keys
+
+
+def f2h(x: float, mode: str = "rne") -> int:
+    return {"rne": lib().orc_f2h_rne, "down": lib().orc_f2h_down, "up": lib().orc_f2h_up}[mode](x)
+
+
+def h2f(h: int) -> float:
+    return lib().orc_h2f(h)

@@ -1,0 +1,749 @@
+/* rt_oracle.c — CPU oracle for the ray-tracing hot path (TEST INFRASTRUCTURE).
+ *
+ * Only tests, the smoke check and the benchmark's CPU-baseline leg use this
+ * file, as the checker. Built with -O2 -ffp-contract=off (see Makefile) so the
+ * FP32 arithmetic is exactly the operation sequence written here.
+ *
+ * Reference lines cited are in the reference snapshot; "backend" means
+ * layer/cuda_bvh_backend.cu.
+ */
+#include "rt_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#include <limits.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* 1 + 2*gamma(3) rounded to float: the tmax inflation of Ize's robust slab
+ * test (T. Ize, "Robust BVH Ray Traversal", JCGT 2013). */
+#define ROBUST_TFAR (1.0f + 3.0f * 0x1p-23f)
+#define MISS_T 1e30f
+
+int orc_num_threads(int requested) {
+#ifdef _OPENMP
+    return requested > 0 ? requested : omp_get_max_threads();
+#else
+    (void)requested;
+    return 1;
+#endif
+}
+
+void orc_release(void* p) { free(p); }
+
+/* ===================================================================== */
+/* IEEE binary16 (the conversion cuda_fp16's __float2half performs)       */
+/* ===================================================================== */
+static uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+uint16_t orc_f2h_rne(float x) {
+    uint32_t u = fbits(x);
+    uint32_t sign = (u >> 16) & 0x8000u;
+    uint32_t a = u & 0x7fffffffu;
+    if (a >= 0x7f800000u) {                       /* inf or nan */
+        return (uint16_t)(sign | 0x7c00u | (a > 0x7f800000u ? 0x200u : 0u));
+    }
+    if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u); /* >= 65520 -> inf */
+    if (a < 0x33000001u) return (uint16_t)sign;   /* < 2^-25 (tie goes to even zero) */
+    int e = (int)(a >> 23) - 127;                 /* unbiased exponent */
+    uint32_t m = (a & 0x7fffffu) | 0x800000u;     /* 24-bit significand */
+    if (e < -14) {                                /* subnormal half */
+        int shift = -e - 14 + 13;                 /* bits to drop */
+        uint32_t q = m >> shift;
+        uint32_t rem = m & ((1u << shift) - 1u);
+        uint32_t half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (q & 1u))) q++;
+        return (uint16_t)(sign | q);
+    }
+    uint32_t q = ((uint32_t)(e + 15) << 10) | ((m >> 13) & 0x3ffu);
+    uint32_t rem = m & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) q++;   /* carry may reach inf: correct */
+    return (uint16_t)(sign | q);
+}
+
+float orc_h2f(uint16_t h) {
+    uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+    uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    if (e == 0x1f) return bitsf(sign | 0x7f800000u | (m << 13));
+    if (e == 0) {
+        if (m == 0) return bitsf(sign);
+        float f = (float)m * 0x1p-24f;            /* exact */
+        return sign ? -f : f;
+    }
+    return bitsf(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+/* Next half toward +inf / -inf in value order. */
+static uint16_t h_next_up(uint16_t h) {
+    if ((h & 0x7fffu) == 0) return 0x0001u;            /* +-0 -> smallest subnormal */
+    if (h & 0x8000u) return (uint16_t)(h - 1u);        /* negative: toward zero */
+    if (h == 0x7c00u) return h;                        /* +inf stays */
+    return (uint16_t)(h + 1u);
+}
+static uint16_t h_next_down(uint16_t h) {
+    if ((h & 0x7fffu) == 0) return 0x8001u;
+    if (h & 0x8000u) { if (h == 0xfc00u) return h; return (uint16_t)(h + 1u); }
+    return (uint16_t)(h - 1u);
+}
+
+/* Outward rounding: the corrected form of backend :405-407 (which rounds to
+ * nearest and can shrink a box). */
+uint16_t orc_f2h_down(float x) {
+    uint16_t h = orc_f2h_rne(x);
+    if (orc_h2f(h) > x) h = h_next_down(h);
+    return h;
+}
+uint16_t orc_f2h_up(float x) {
+    uint16_t h = orc_f2h_rne(x);
+    if (orc_h2f(h) < x) h = h_next_up(h);
+    return h;
+}
+
+/* ===================================================================== */
+/* Binned-SAH BVH2 builder (backend :186-251)                             */
+/* ===================================================================== */
+typedef struct { float mn[3], mx[3]; } box3;
+typedef struct { box3 box; int left, right, triStart, triCount; } bnode2;
+
+struct orc_bvh {
+    int n, max_leaf;
+    bnode2* nodes; int num_nodes;
+    int* ordered; int num_ordered;
+    uint8_t* b4; int num_b4; int b4_depth;
+};
+
+static box3 box_empty(void) {
+    box3 b; for (int a = 0; a < 3; a++) { b.mn[a] = 1e30f; b.mx[a] = -1e30f; } return b;
+}
+static box3 box_merge(box3 a, box3 b) {      /* backend :191-194 */
+    box3 r;
+    for (int k = 0; k < 3; k++) { r.mn[k] = fminf(a.mn[k], b.mn[k]); r.mx[k] = fmaxf(a.mx[k], b.mx[k]); }
+    return r;
+}
+static float sa_area(box3 b) {               /* backend :195-198 */
+    float dx = b.mx[0] - b.mn[0], dy = b.mx[1] - b.mn[1], dz = b.mx[2] - b.mn[2];
+    return 2.f * (dx * dy + dy * dz + dz * dx);
+}
+static box3 tri_box(const float* t) {        /* backend :187-190 */
+    box3 b;
+    for (int a = 0; a < 3; a++) {
+        b.mn[a] = fminf(fminf(t[a], t[3 + a]), t[6 + a]);
+        b.mx[a] = fmaxf(fmaxf(t[a], t[3 + a]), t[6 + a]);
+    }
+    return b;
+}
+
+#define NBINS 16
+
+/* Bin index of backend :232: floor((c-cmin)/(cmax-cmin)*(NB-1)), clamped. */
+static int bin_of(float c, float cmin, float cmax) {
+    int b = (int)((c - cmin) / (cmax - cmin) * (float)(NBINS - 1));
+    return b < 0 ? 0 : (b >= NBINS ? NBINS - 1 : b);
+}
+
+typedef struct { int s, e; int parent; int side; } btask;
+
+orc_bvh* orc_build(const float* tris, int n, int max_leaf) {
+    if (!tris || n <= 0 || max_leaf < 1 || max_leaf > 8) return NULL;
+    orc_bvh* B = (orc_bvh*)calloc(1, sizeof(orc_bvh));
+    B->n = n; B->max_leaf = max_leaf;
+    box3* pb = (box3*)malloc(sizeof(box3) * (size_t)n);
+    float* cen = (float*)malloc(sizeof(float) * 3 * (size_t)n);
+    int* idx = (int*)malloc(sizeof(int) * (size_t)n);
+    int* tmp = (int*)malloc(sizeof(int) * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        pb[i] = tri_box(tris + 9 * (size_t)i);
+        for (int a = 0; a < 3; a++) cen[3 * i + a] = (pb[i].mn[a] + pb[i].mx[a]) * .5f;   /* :207-208 */
+        idx[i] = i;
+    }
+    B->nodes = (bnode2*)malloc(sizeof(bnode2) * (size_t)(2 * n));
+    B->ordered = (int*)malloc(sizeof(int) * (size_t)n);
+    btask* st = (btask*)malloc(sizeof(btask) * (size_t)(2 * n + 2));
+    int sp = 0;
+    st[sp++] = (btask){0, n, -1, 0};
+    while (sp > 0) {
+        btask tk = st[--sp];
+        int s = tk.s, e = tk.e, cnt = e - s;
+        int id = B->num_nodes++;                       /* pre-order numbering, :248 */
+        if (tk.parent >= 0) {
+            if (tk.side == 0) B->nodes[tk.parent].left = id; else B->nodes[tk.parent].right = id;
+        }
+        bnode2 nd; nd.left = nd.right = nd.triStart = nd.triCount = 0;
+        nd.box = pb[idx[s]];
+        for (int i = s + 1; i < e; i++) nd.box = box_merge(nd.box, pb[idx[i]]);   /* :216 */
+        if (cnt <= max_leaf) {                          /* :220 (reference: <=3) */
+            nd.triStart = B->num_ordered; nd.triCount = cnt;
+            for (int i = s; i < e; i++) B->ordered[B->num_ordered++] = idx[i];
+            B->nodes[id] = nd;
+            continue;
+        }
+        float bestCost = 1e30f; int bestAxis = -1, bestBin = -1;
+        float pA = sa_area(nd.box);
+        for (int ax = 0; ax < 3; ax++) {                /* :223-245 */
+            float cmin = 1e30f, cmax = -1e30f;
+            for (int i = s; i < e; i++) { float c = cen[3 * idx[i] + ax]; cmin = fminf(cmin, c); cmax = fmaxf(cmax, c); }
+            if (cmax - cmin < 1e-8f) continue;
+            box3 bb[NBINS]; int bc[NBINS];
+            for (int b = 0; b < NBINS; b++) { bb[b] = box_empty(); bc[b] = 0; }
+            for (int i = s; i < e; i++) {
+                int b = bin_of(cen[3 * idx[i] + ax], cmin, cmax);
+                bb[b] = box_merge(bb[b], pb[idx[i]]); bc[b]++;
+            }
+            box3 L[NBINS], R[NBINS]; int lc[NBINS], rc[NBINS];
+            L[0] = bb[0]; lc[0] = bc[0];
+            for (int b = 1; b < NBINS; b++) { L[b] = box_merge(L[b - 1], bb[b]); lc[b] = lc[b - 1] + bc[b]; }
+            R[NBINS - 1] = bb[NBINS - 1]; rc[NBINS - 1] = bc[NBINS - 1];
+            for (int b = NBINS - 2; b >= 0; b--) { R[b] = box_merge(R[b + 1], bb[b]); rc[b] = rc[b + 1] + bc[b]; }
+            for (int b = 0; b < NBINS - 1; b++) {
+                if (lc[b] == 0 || rc[b + 1] == 0) continue;
+                float cost = (float)lc[b] * sa_area(L[b]) / pA + (float)rc[b + 1] * sa_area(R[b + 1]) / pA + 1.f;  /* :240 */
+                if (cost < bestCost) { bestCost = cost; bestAxis = ax; bestBin = b; }
+            }
+        }
+        int mid;
+        if (bestAxis >= 0) {
+            /* Stable partition on the evaluated bin boundary (the reference
+             * re-derives a plane that does not match the bins, :242, and then
+             * std::sorts, :246; both are replaced by this exact rule). */
+            float cmin = 1e30f, cmax = -1e30f;
+            for (int i = s; i < e; i++) { float c = cen[3 * idx[i] + bestAxis]; cmin = fminf(cmin, c); cmax = fmaxf(cmax, c); }
+            int nl = 0, nr = 0;
+            for (int i = s; i < e; i++) {
+                int p = idx[i];
+                if (bin_of(cen[3 * p + bestAxis], cmin, cmax) <= bestBin) idx[s + nl++] = p; else tmp[nr++] = p;
+            }
+            memcpy(idx + s + nl, tmp, sizeof(int) * (size_t)nr);
+            mid = s + nl;
+        } else {
+            mid = s + cnt / 2;                          /* all centroids coincide: :247 */
+        }
+        B->nodes[id] = nd;
+        st[sp++] = (btask){mid, e, id, 1};              /* right popped after the whole left subtree */
+        st[sp++] = (btask){s, mid, id, 0};
+    }
+    free(st); free(pb); free(cen); free(idx); free(tmp);
+
+    /* ---- collapse to BVH4 (:391-418), pre-order numbering ---- */
+    B->b4 = (uint8_t*)calloc((size_t)(B->num_nodes + 1), 64);
+    typedef struct { int n2; int parent; int slot; int depth; } ctask;
+    ctask* cs = (ctask*)malloc(sizeof(ctask) * (size_t)(B->num_nodes + 4));
+    int csp = 0, maxd = 0;
+    cs[csp++] = (ctask){0, -1, 0, 1};
+    while (csp > 0) {
+        ctask ct = cs[--csp];
+        int me = B->num_b4++;
+        if (ct.depth > maxd) maxd = ct.depth;
+        if (ct.parent >= 0) {
+            int32_t v = me; memcpy(B->b4 + 64 * (size_t)ct.parent + 48 + 4 * ct.slot, &v, 4);
+        }
+        const bnode2* nn = &B->nodes[ct.n2];
+        int gather[4], ng = 0;
+        if (nn->triCount > 0) {
+            gather[ng++] = ct.n2;                       /* leaf root: one-child node */
+        } else {
+            int ch[2] = {nn->left, nn->right};
+            for (int c = 0; c < 2; c++) {
+                const bnode2* cn = &B->nodes[ch[c]];
+                if (cn->triCount > 0 || ng >= 3) { gather[ng++] = ch[c]; continue; }
+                gather[ng++] = cn->left; gather[ng++] = cn->right;
+            }
+        }
+        uint8_t* out = B->b4 + 64 * (size_t)me;
+        uint16_t hx[8], hy[8], hz[8]; int32_t child[4];
+        for (int i = 0; i < 4; i++) {
+            if (i < ng) {
+                const box3* bx = &B->nodes[gather[i]].box;
+                hx[i] = orc_f2h_down(bx->mn[0]); hx[4 + i] = orc_f2h_up(bx->mx[0]);
+                hy[i] = orc_f2h_down(bx->mn[1]); hy[4 + i] = orc_f2h_up(bx->mx[1]);
+                hz[i] = orc_f2h_down(bx->mn[2]); hz[4 + i] = orc_f2h_up(bx->mx[2]);
+            } else {                                    /* :409-411: empty = (+inf,-inf) */
+                hx[i] = hy[i] = hz[i] = 0x7c00u;
+                hx[4 + i] = hy[4 + i] = hz[4 + i] = 0xfc00u;
+            }
+            child[i] = -1;
+        }
+        for (int i = 0; i < ng; i++) {
+            const bnode2* cn = &B->nodes[gather[i]];
+            if (cn->triCount > 0) child[i] = -((cn->triStart << 3) | (cn->triCount - 1)) - 2;   /* :393 */
+        }
+        memcpy(out, hx, 16); memcpy(out + 16, hy, 16); memcpy(out + 32, hz, 16); memcpy(out + 48, child, 16);
+        for (int i = ng - 1; i >= 0; i--)
+            if (B->nodes[gather[i]].triCount == 0) cs[csp++] = (ctask){gather[i], me, i, ct.depth + 1};
+    }
+    free(cs);
+    B->b4_depth = maxd;
+    return B;
+}
+
+void orc_free(orc_bvh* b) {
+    if (!b) return;
+    free(b->nodes); free(b->ordered); free(b->b4); free(b);
+}
+int orc_num_bvh2(const orc_bvh* b) { return b ? b->num_nodes : 0; }
+int orc_num_bvh4(const orc_bvh* b) { return b ? b->num_b4 : 0; }
+int orc_num_ordered(const orc_bvh* b) { return b ? b->num_ordered : 0; }
+int orc_bvh4_depth(const orc_bvh* b) { return b ? b->b4_depth : 0; }
+void orc_get_bvh4(const orc_bvh* b, void* out) { memcpy(out, b->b4, 64 * (size_t)b->num_b4); }
+void orc_get_order(const orc_bvh* b, int* o) { memcpy(o, b->ordered, sizeof(int) * (size_t)b->num_ordered); }
+void orc_get_bvh2(const orc_bvh* b, float* boxes, int* links) {
+    for (int i = 0; i < b->num_nodes; i++) {
+        const bnode2* nd = &b->nodes[i];
+        for (int a = 0; a < 3; a++) { boxes[6 * i + a] = nd->box.mn[a]; boxes[6 * i + 3 + a] = nd->box.mx[a]; }
+        links[4 * i] = nd->left; links[4 * i + 1] = nd->right;
+        links[4 * i + 2] = nd->triStart; links[4 * i + 3] = nd->triCount;
+    }
+}
+
+/* Subtree sizes of a pre-order BVH2 (children always follow the parent). */
+static int* subtree_sizes(const bnode2* nodes, int nn) {
+    int* sz = (int*)malloc(sizeof(int) * (size_t)nn);
+    for (int i = nn - 1; i >= 0; i--) {
+        sz[i] = 1;
+        if (nodes[i].triCount == 0) sz[i] += sz[nodes[i].left] + sz[nodes[i].right];
+    }
+    return sz;
+}
+
+/* Stackless DFS BVH2 (:456-522). The pre-order numbering of the builder is
+ * already the DFS order; skip = first node after the subtree, -1 at the end. */
+static void write_dfs(const bnode2* nodes, int nn, uint32_t* out) {
+    int* sz = subtree_sizes(nodes, nn);
+    for (int i = 0; i < nn; i++) {
+        const bnode2* nd = &nodes[i];
+        uint32_t* p = out + 8 * (size_t)i;
+        memcpy(p + 0, &nd->box.mn[0], 4); memcpy(p + 1, &nd->box.mn[1], 4); memcpy(p + 2, &nd->box.mn[2], 4);
+        memcpy(p + 3, &nd->box.mx[0], 4); memcpy(p + 4, &nd->box.mx[1], 4); memcpy(p + 5, &nd->box.mx[2], 4);
+        int32_t enc = nd->triCount > 0 ? -((nd->triStart << 3) | (nd->triCount - 1)) - 2 : 0;
+        int32_t skip = (i + sz[i] < nn) ? i + sz[i] : -1;
+        memcpy(p + 6, &enc, 4); memcpy(p + 7, &skip, 4);
+    }
+    free(sz);
+}
+int orc_num_dfs(const orc_bvh* b) { return b ? b->num_nodes : 0; }
+void orc_get_dfs(const orc_bvh* b, uint32_t* out) { write_dfs(b->nodes, b->num_nodes, out); }
+
+double orc_sah_cost(const orc_bvh* b) {
+    double root = sa_area(b->nodes[0].box), c = 0;
+    if (root <= 0) return 0;
+    for (int i = 0; i < b->num_nodes; i++) {
+        double a = sa_area(b->nodes[i].box) / root;
+        c += b->nodes[i].triCount > 0 ? a * b->nodes[i].triCount : a;
+    }
+    return c;
+}
+
+/* ===================================================================== */
+/* LBVH for TLAS (:257-389, :1620-1698)                                  */
+/* ===================================================================== */
+static uint32_t expand_bits(uint32_t v) {      /* spread 10 bits to every third bit */
+    uint32_t r = 0;
+    for (int i = 0; i < 10; i++) r |= ((v >> i) & 1u) << (3 * i);
+    return r;
+}
+uint32_t orc_morton3d(float x, float y, float z) {   /* :265-273 */
+    x = fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
+    y = fminf(fmaxf(y * 1024.0f, 0.0f), 1023.0f);
+    z = fminf(fmaxf(z * 1024.0f, 0.0f), 1023.0f);
+    return expand_bits((uint32_t)x) * 4u + expand_bits((uint32_t)y) * 2u + expand_bits((uint32_t)z);
+}
+
+int orc_lbvh_tlas(const float* aabbs, int n, uint32_t** out, int** ordered) {
+    if (out) *out = NULL;
+    if (ordered) *ordered = NULL;
+    if (!aabbs || n <= 0 || !out) return 0;
+    float smin[3] = {1e30f, 1e30f, 1e30f}, smax[3] = {-1e30f, -1e30f, -1e30f};
+    for (int i = 0; i < n; i++)
+        for (int a = 0; a < 3; a++) {
+            float c = (aabbs[6 * i + a] + aabbs[6 * i + 3 + a]) * 0.5f;
+            smin[a] = fminf(smin[a], c); smax[a] = fmaxf(smax[a], c);
+        }
+    float inv[3];
+    for (int a = 0; a < 3; a++) inv[a] = (smax[a] - smin[a] > 1e-8f) ? 1.0f / (smax[a] - smin[a]) : 0.0f;
+    uint32_t* code = (uint32_t*)malloc(4 * (size_t)n); int* id = (int*)malloc(4 * (size_t)n);
+    uint32_t* code2 = (uint32_t*)malloc(4 * (size_t)n); int* id2 = (int*)malloc(4 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        float c[3];
+        for (int a = 0; a < 3; a++) c[a] = (aabbs[6 * i + a] + aabbs[6 * i + 3 + a]) * 0.5f;
+        code[i] = orc_morton3d((c[0] - smin[0]) * inv[0], (c[1] - smin[1]) * inv[1], (c[2] - smin[2]) * inv[2]);
+        id[i] = i;
+    }
+    for (int pass = 0; pass < 4; pass++) {          /* stable LSD radix sort, :309-324 */
+        int shift = 8 * pass, count[256] = {0}, pre[256];
+        for (int i = 0; i < n; i++) count[(code[i] >> shift) & 0xff]++;
+        pre[0] = 0;
+        for (int i = 1; i < 256; i++) pre[i] = pre[i - 1] + count[i - 1];
+        for (int i = 0; i < n; i++) { int b = (code[i] >> shift) & 0xff; code2[pre[b]] = code[i]; id2[pre[b]++] = id[i]; }
+        uint32_t* tc = code; code = code2; code2 = tc; int* ti = id; id = id2; id2 = ti;
+    }
+    /* median-split tree, leaf = 1 item (:345-387), pre-order ids */
+    bnode2* nodes = (bnode2*)malloc(sizeof(bnode2) * (size_t)(2 * n));
+    int nn = 0;
+    typedef struct { int s, e, parent, side, stage; } lt;
+    lt* st = (lt*)malloc(sizeof(lt) * (size_t)(4 * n + 4));
+    int sp = 0;
+    st[sp++] = (lt){0, n, -1, 0, 0};
+    while (sp > 0) {
+        lt t = st[--sp];
+        if (t.stage == 1) {                          /* post-order box propagation */
+            bnode2* p = &nodes[t.parent];
+            p->box = box_merge(nodes[p->left].box, nodes[p->right].box);
+            continue;
+        }
+        int me = nn++;
+        if (t.parent >= 0) { if (t.side == 0) nodes[t.parent].left = me; else nodes[t.parent].right = me; }
+        bnode2* nd = &nodes[me];
+        nd->left = nd->right = nd->triStart = nd->triCount = 0;
+        nd->box = box_empty();
+        int cnt = t.e - t.s;
+        if (cnt <= 1) {
+            nd->triStart = t.s; nd->triCount = cnt;
+            for (int i = t.s; i < t.e; i++) {
+                const float* b = aabbs + 6 * (size_t)id[i];
+                for (int a = 0; a < 3; a++) { nd->box.mn[a] = fminf(nd->box.mn[a], b[a]); nd->box.mx[a] = fmaxf(nd->box.mx[a], b[3 + a]); }
+            }
+            continue;
+        }
+        int mid = t.s + cnt / 2;
+        st[sp++] = (lt){0, 0, me, 0, 1};
+        st[sp++] = (lt){mid, t.e, me, 1, 0};
+        st[sp++] = (lt){t.s, mid, me, 0, 0};
+    }
+    free(st);
+    uint32_t* o = (uint32_t*)malloc(32 * (size_t)nn);
+    write_dfs(nodes, nn, o);
+    *out = o;
+    if (ordered) { *ordered = id; id = NULL; }
+    free(nodes); free(code); free(code2); free(id); free(id2);
+    return nn;
+}
+
+/* ===================================================================== */
+/* Rays                                                                   */
+/* ===================================================================== */
+void orc_gen_pinhole(const float* cam, int W, int H, float* rays) {
+    size_t n = (size_t)W * (size_t)H;
+    const float *o = cam, *f = cam + 3, *r = cam + 6, *u = cam + 9;
+    float sxs = cam[12], sys = cam[13];
+    for (int py = 0; py < H; py++)
+        for (int px = 0; px < W; px++) {
+            size_t i = (size_t)py * W + px;
+            float sx = ((float)px + 0.5f) / (float)W * 2.0f - 1.0f;   /* backend :69-70 */
+            float sy = ((float)py + 0.5f) / (float)H * 2.0f - 1.0f;
+            float a = sx * sxs, b = sy * sys;
+            float dx = (f[0] + a * r[0]) + b * u[0];
+            float dy = (f[1] + a * r[1]) + b * u[1];
+            float dz = (f[2] + a * r[2]) + b * u[2];
+            float inv = 1.0f / sqrtf((dx * dx + dy * dy) + dz * dz);  /* IEEE, not rsqrtf (:72) */
+            rays[0 * n + i] = o[0]; rays[1 * n + i] = o[1]; rays[2 * n + i] = o[2];
+            rays[3 * n + i] = dx * inv; rays[4 * n + i] = dy * inv; rays[5 * n + i] = dz * inv;
+            rays[6 * n + i] = 0.0f; rays[7 * n + i] = MISS_T;
+        }
+}
+
+void orc_sort_keys(const float* rays, int n, const float* lo, const float* inv, uint32_t* keys) {
+    for (int i = 0; i < n; i++) {
+        float nx = (rays[i] - lo[0]) * inv[0];
+        float ny = (rays[(size_t)n + i] - lo[1]) * inv[1];
+        float nz = (rays[2 * (size_t)n + i] - lo[2]) * inv[2];
+        uint32_t oct = (rays[3 * (size_t)n + i] < 0.f ? 4u : 0u) | (rays[4 * (size_t)n + i] < 0.f ? 2u : 0u) |
+                       (rays[5 * (size_t)n + i] < 0.f ? 1u : 0u);
+        keys[i] = (oct << 29) | (orc_morton3d(nx, ny, nz) & 0x1FFFFFFFu);
+    }
+}
+
+/* ===================================================================== */
+/* Ray-triangle tests                                                     */
+/* ===================================================================== */
+typedef struct {
+    float o[3], d[3], tmin, tmax, inv[3];
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    int flat;
+} xray;
+
+static void setup_ray(xray* r, const float* rays, size_t n, size_t i) {
+    for (int a = 0; a < 3; a++) { r->o[a] = rays[a * n + i]; r->d[a] = rays[(3 + a) * n + i]; }
+    r->tmin = rays[6 * n + i]; r->tmax = rays[7 * n + i];
+    /* A zero (or tiny) direction component gives an infinite reciprocal; such
+     * a "flat" ray can produce (b - o) * inv = 0 * inf = NaN when its origin
+     * lies exactly on a box plane, which slab4 resolves explicitly. */
+    r->flat = 0;
+    for (int a = 0; a < 3; a++) {
+        r->inv[a] = 1.0f / r->d[a];
+        if (isinf(r->inv[a])) r->flat = 1;
+    }
+    /* watertight shear (Woop et al. 2013, sec. 3) */
+    float ax = fabsf(r->d[0]), ay = fabsf(r->d[1]), az = fabsf(r->d[2]);
+    int kz = 0;
+    if (ay > ax) kz = 1;
+    if (az > (kz == 0 ? ax : ay)) kz = 2;
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    if (r->d[kz] < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r->kx = kx; r->ky = ky; r->kz = kz;
+    r->Sx = r->d[kx] / r->d[kz];
+    r->Sy = r->d[ky] / r->d[kz];
+    r->Sz = 1.0f / r->d[kz];
+}
+
+/* Watertight test; returns 1 with t,u,v (u = weight of v1, v = weight of v2)
+ * when the ray line crosses the triangle with a non-zero determinant. The
+ * caller applies the (tmin, tmax) and closest-hit rules. */
+static int isect_wt(const xray* r, const float* tri, float* tt, float* uu, float* vv) {
+    float A[3], Bv[3], C[3];
+    for (int a = 0; a < 3; a++) { A[a] = tri[a] - r->o[a]; Bv[a] = tri[3 + a] - r->o[a]; C[a] = tri[6 + a] - r->o[a]; }
+    float Ax = A[r->kx] - r->Sx * A[r->kz], Ay = A[r->ky] - r->Sy * A[r->kz];
+    float Bx = Bv[r->kx] - r->Sx * Bv[r->kz], By = Bv[r->ky] - r->Sy * Bv[r->kz];
+    float Cx = C[r->kx] - r->Sx * C[r->kz], Cy = C[r->ky] - r->Sy * C[r->kz];
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {      /* FP64 fallback on an edge */
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return 0;
+    float det = U + V + W;
+    if (det == 0.0f) return 0;
+    float Az = r->Sz * A[r->kz], Bz = r->Sz * Bv[r->kz], Cz = r->Sz * C[r->kz];
+    float T = U * Az + V * Bz + W * Cz;
+    float rcp = 1.0f / det;
+    *tt = T * rcp; *uu = V * rcp; *vv = W * rcp;
+    return 1;
+}
+
+/* Moller-Trumbore exactly as backend :145-156 (IEEE division in place of the
+ * fast-math reciprocal); the t > 0 and t < tHit checks are left to the caller. */
+static int isect_mt(const xray* r, const float* tri, float* tt, float* uu, float* vv) {
+    float dx = r->d[0], dy = r->d[1], dz = r->d[2];
+    float e1x = tri[3] - tri[0], e1y = tri[4] - tri[1], e1z = tri[5] - tri[2];
+    float e2x = tri[6] - tri[0], e2y = tri[7] - tri[1], e2z = tri[8] - tri[2];
+    float ppx = dy * e2z - dz * e2y, ppy = dz * e2x - dx * e2z, ppz = dx * e2y - dy * e2x;
+    float det = e1x * ppx + e1y * ppy + e1z * ppz;
+    if (fabsf(det) < 1e-12f) return 0;
+    float inv = 1.f / det;
+    float tx = r->o[0] - tri[0], ty = r->o[1] - tri[1], tz = r->o[2] - tri[2];
+    float u = inv * (tx * ppx + ty * ppy + tz * ppz);
+    if (u < 0.f || u > 1.f) return 0;
+    float qx = ty * e1z - tz * e1y, qy = tz * e1x - tx * e1z, qz = tx * e1y - ty * e1x;
+    float v = inv * (dx * qx + dy * qy + dz * qz);
+    if (v < 0.f || u + v > 1.f) return 0;
+    *tt = inv * (e2x * qx + e2y * qy + e2z * qz); *uu = u; *vv = v;
+    return 1;
+}
+
+static int isect(int mode, const xray* r, const float* tri, float* t, float* u, float* v) {
+    return mode == 1 ? isect_mt(r, tri, t, u, v) : isect_wt(r, tri, t, u, v);
+}
+
+int orc_isect_watertight(const float* ray, const float* tri, float* t, float* u, float* v) {
+    xray r; setup_ray(&r, ray, 1, 0);
+    return isect_wt(&r, tri, t, u, v);
+}
+int orc_isect_mt(const float* ray, const float* tri, float* t, float* u, float* v) {
+    xray r; setup_ray(&r, ray, 1, 0);
+    return isect_mt(&r, tri, t, u, v);
+}
+
+/* ===================================================================== */
+/* BVH4 traversal                                                         */
+/* ===================================================================== */
+/* Child slab test shared by both traversals: robust far distance (Ize 2013),
+ * inclusive planes for flat rays. Returns 1 and the entry distance when the box may
+ * hold a hit in [tmin, tlimit]. */
+static int slab4(const uint8_t* node, int c, const xray* r, float tlimit, float* tnear) {
+    uint16_t h[6];
+    memcpy(&h[0], node + 2 * c, 2);       memcpy(&h[1], node + 2 * (4 + c), 2);
+    memcpy(&h[2], node + 16 + 2 * c, 2);  memcpy(&h[3], node + 16 + 2 * (4 + c), 2);
+    memcpy(&h[4], node + 32 + 2 * c, 2);  memcpy(&h[5], node + 32 + 2 * (4 + c), 2);
+    float t1x = (orc_h2f(h[0]) - r->o[0]) * r->inv[0], t2x = (orc_h2f(h[1]) - r->o[0]) * r->inv[0];
+    float t1y = (orc_h2f(h[2]) - r->o[1]) * r->inv[1], t2y = (orc_h2f(h[3]) - r->o[1]) * r->inv[1];
+    float t1z = (orc_h2f(h[4]) - r->o[2]) * r->inv[2], t2z = (orc_h2f(h[5]) - r->o[2]) * r->inv[2];
+    if (r->flat) {
+        /* origin on a plane of an axis the ray runs parallel to: that axis
+         * must not constrain the interval (inclusive slab) */
+        if (t1x != t1x) t1x = -r->inv[0];
+        if (t2x != t2x) t2x = r->inv[0];
+        if (t1y != t1y) t1y = -r->inv[1];
+        if (t2y != t2y) t2y = r->inv[1];
+        if (t1z != t1z) t1z = -r->inv[2];
+        if (t2z != t2z) t2z = r->inv[2];
+    }
+    float tN = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
+    float tF = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z)) * ROBUST_TFAR;
+    *tnear = tN;
+    return (tN <= tF) && (tF >= r->tmin) && (tN <= tlimit);
+}
+
+typedef struct { int ref; float tn; } sentry;
+
+static void closest_one(const uint8_t* nodes, const float* tris, const int* pid, const xray* r, int mode,
+                        sentry* stk, float* ot, int* oid, float* ou, float* ov, int* nv, int* nt) {
+    float best = r->tmax, bu = 0.f, bv = 0.f;
+    int bid = -1, visits = 0, tests = 0, sp = 0;
+    stk[sp++] = (sentry){0, -INFINITY};
+    while (sp > 0) {
+        sentry e = stk[--sp];
+        if (e.tn > best) continue;                     /* re-check against the current hit */
+        if (e.ref < 0) {                               /* leaf, backend :141-157 */
+            int enc = -(e.ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
+            for (int k = 0; k < tc; k++) {
+                int ti = ts + k; float t, u, v;
+                tests++;
+                if (!isect(mode, r, tris + 9 * (size_t)ti, &t, &u, &v)) continue;
+                if (!(t > r->tmin) || !(t < r->tmax)) continue;
+                int id = pid[ti];
+                /* best starts at tmax and t < tmax, so the first valid hit wins */
+                if (t < best || (t == best && id < bid)) { best = t; bid = id; bu = u; bv = v; }
+            }
+            continue;
+        }
+        const uint8_t* nd = nodes + 64 * (size_t)e.ref;
+        visits++;
+        int32_t ch[4]; memcpy(ch, nd + 48, 16);
+        float dist[4]; int child[4];
+        for (int c = 0; c < 4; c++) {
+            float tn;
+            if (ch[c] != -1 && slab4(nd, c, r, best, &tn)) { dist[c] = tn; child[c] = ch[c]; }
+            else { dist[c] = INFINITY; child[c] = -1; }
+        }
+#define CSWAP(a, b) do { if (dist[a] > dist[b]) { float td = dist[a]; dist[a] = dist[b]; dist[b] = td; \
+                         int tc2 = child[a]; child[a] = child[b]; child[b] = tc2; } } while (0)
+        CSWAP(0, 1); CSWAP(2, 3); CSWAP(0, 2); CSWAP(1, 3); CSWAP(1, 2);      /* :177 */
+#undef CSWAP
+        for (int c = 3; c >= 0; c--)
+            if (child[c] != -1) stk[sp++] = (sentry){child[c], dist[c]};
+    }
+    if (bid < 0) { *ot = MISS_T; *oid = -1; *ou = 0.f; *ov = 0.f; }
+    else { *ot = best; *oid = bid; *ou = bu; *ov = bv; }
+    if (nv) *nv = visits;
+    if (nt) *nt = tests;
+}
+
+static int stack_cap(int num_nodes) {
+    /* A DFS over a 4-ary tree holds at most 3 pending siblings per level plus
+     * 4 children; bounding the depth by the node count keeps it unbounded in
+     * effect (no silent drops, unlike backend :179). */
+    (void)num_nodes;
+    return 3 * num_nodes + 8;
+}
+
+void orc_trace_closest(const void* nodes, int num_nodes, const float* tris, const int* prim_id, int num_tris,
+                       const float* rays, int n_rays, float* t, int* id, float* u, float* v,
+                       int* node_visits, int* tri_tests, int isect_mode, int threads) {
+    (void)num_tris;
+    int nth = orc_num_threads(threads);
+    int cap = stack_cap(num_nodes);
+#pragma omp parallel num_threads(nth)
+    {
+        sentry* stk = (sentry*)malloc(sizeof(sentry) * (size_t)cap);
+#pragma omp for schedule(dynamic, 512)
+        for (int i = 0; i < n_rays; i++) {
+            xray r; setup_ray(&r, rays, (size_t)n_rays, (size_t)i);
+            closest_one((const uint8_t*)nodes, tris, prim_id, &r, isect_mode, stk, &t[i], &id[i], &u[i], &v[i],
+                        node_visits ? &node_visits[i] : NULL, tri_tests ? &tri_tests[i] : NULL);
+        }
+        free(stk);
+    }
+}
+
+void orc_trace_any(const void* nodes_, int num_nodes, const float* tris, const int* prim_id, int num_tris,
+                   const float* rays, int n_rays, uint8_t* occluded, int* node_visits, int* tri_tests,
+                   int isect_mode, int threads) {
+    (void)num_tris; (void)prim_id;
+    const uint8_t* nodes = (const uint8_t*)nodes_;
+    int nth = orc_num_threads(threads);
+    int cap = stack_cap(num_nodes);
+#pragma omp parallel num_threads(nth)
+    {
+        int* stk = (int*)malloc(sizeof(int) * (size_t)cap);
+#pragma omp for schedule(dynamic, 512)
+        for (int i = 0; i < n_rays; i++) {
+            xray r; setup_ray(&r, rays, (size_t)n_rays, (size_t)i);
+            int sp = 0, found = 0, visits = 0, tests = 0;
+            stk[sp++] = 0;
+            while (sp > 0 && !found) {
+                int ref = stk[--sp];
+                if (ref < 0) {
+                    int enc = -(ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
+                    for (int k = 0; k < tc && !found; k++) {
+                        float tt, uu, vv;
+                        tests++;
+                        if (isect(isect_mode, &r, tris + 9 * (size_t)(ts + k), &tt, &uu, &vv) &&
+                            tt > r.tmin && tt < r.tmax) found = 1;
+                    }
+                    continue;
+                }
+                const uint8_t* nd = nodes + 64 * (size_t)ref;
+                visits++;
+                int32_t ch[4]; memcpy(ch, nd + 48, 16);
+                for (int c = 0; c < 4; c++) {                 /* v10_pipeline.cu:241-244, unsorted */
+                    float tn;
+                    if (ch[c] != -1 && slab4(nd, c, &r, r.tmax, &tn)) stk[sp++] = ch[c];
+                }
+            }
+            occluded[i] = (uint8_t)found;
+            if (node_visits) node_visits[i] = visits;
+            if (tri_tests) tri_tests[i] = tests;
+        }
+        free(stk);
+    }
+}
+
+/* ===================================================================== */
+/* Brute force                                                            */
+/* ===================================================================== */
+void orc_brute_closest(const float* tris, const int* prim_id, int num_tris, const float* rays, int n_rays,
+                       float* t, int* id, float* u, float* v, int isect_mode, int threads) {
+    int nth = orc_num_threads(threads);
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 64)
+    for (int i = 0; i < n_rays; i++) {
+        xray r; setup_ray(&r, rays, (size_t)n_rays, (size_t)i);
+        float best = r.tmax, bu = 0, bv = 0; int bid = -1;
+        for (int k = 0; k < num_tris; k++) {
+            float tt, uu, vv;
+            if (!isect(isect_mode, &r, tris + 9 * (size_t)k, &tt, &uu, &vv)) continue;
+            if (!(tt > r.tmin) || !(tt < r.tmax)) continue;
+            int pk = prim_id[k];
+            if (bid < 0 || tt < best || (tt == best && pk < bid)) { best = tt; bid = pk; bu = uu; bv = vv; }
+        }
+        if (bid < 0) { t[i] = MISS_T; id[i] = -1; u[i] = 0; v[i] = 0; }
+        else { t[i] = best; id[i] = bid; u[i] = bu; v[i] = bv; }
+    }
+}
+
+void orc_brute_closest_f64(const float* tris, const int* prim_id, int num_tris, const float* rays, int n_rays,
+                           double* t, int* id, int threads) {
+    int nth = orc_num_threads(threads);
+    size_t n = (size_t)n_rays;
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 64)
+    for (int i = 0; i < n_rays; i++) {
+        double o[3], d[3];
+        for (int a = 0; a < 3; a++) { o[a] = rays[a * n + i]; d[a] = rays[(3 + a) * n + i]; }
+        double tmin = rays[6 * n + i], tmax = rays[7 * n + i];
+        double best = tmax; int bid = -1;
+        for (int k = 0; k < num_tris; k++) {
+            const float* q = tris + 9 * (size_t)k;
+            double e1[3], e2[3], s[3], p[3], qq[3];
+            for (int a = 0; a < 3; a++) { e1[a] = (double)q[3 + a] - q[a]; e2[a] = (double)q[6 + a] - q[a]; s[a] = o[a] - q[a]; }
+            p[0] = d[1] * e2[2] - d[2] * e2[1]; p[1] = d[2] * e2[0] - d[0] * e2[2]; p[2] = d[0] * e2[1] - d[1] * e2[0];
+            double det = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
+            if (det == 0.0) continue;
+            double inv = 1.0 / det;
+            double uu = inv * (s[0] * p[0] + s[1] * p[1] + s[2] * p[2]);
+            if (uu < 0.0 || uu > 1.0) continue;
+            qq[0] = s[1] * e1[2] - s[2] * e1[1]; qq[1] = s[2] * e1[0] - s[0] * e1[2]; qq[2] = s[0] * e1[1] - s[1] * e1[0];
+            double vv = inv * (d[0] * qq[0] + d[1] * qq[1] + d[2] * qq[2]);
+            if (vv < 0.0 || uu + vv > 1.0) continue;
+            double tt = inv * (e2[0] * qq[0] + e2[1] * qq[1] + e2[2] * qq[2]);
+            if (!(tt > tmin) || !(tt < tmax)) continue;
+            if (bid < 0 || tt < best || (tt == best && prim_id[k] < bid)) { best = tt; bid = prim_id[k]; }
+        }
+        t[i] = bid < 0 ? (double)MISS_T : best; id[i] = bid;
+    }
+}

@@ -1,0 +1,120 @@
+/* rt_oracle.h — CPU oracle for the ray-tracing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: tests, the smoke
+ * check and the benchmark's CPU-baseline leg call it to judge the HIP path.
+ * The product library never links, loads or calls it.
+ *
+ * It restates, in plain C, the algorithms of the reference BVH backend
+ * (layer/cuda_bvh_backend.cu) and the standalone engines it was extracted
+ * from; every function cites the reference lines it follows. Where the
+ * reference has a defect that the north-star contract forbids (FP16 bounds
+ * rounded to nearest, a 16-entry stack that drops nodes, an unstable sort in
+ * the SAH partition), the oracle states the corrected rule and says so.
+ *
+ * Arithmetic contract shared with the GPU kernels: FP32, no FMA contraction
+ * (built with -ffp-contract=off), IEEE division and square root, the same
+ * operation order. That makes hit IDs, t, u, v and the node/triangle visit
+ * counters bit-identical between this oracle and the HIP traversal.
+ */
+#ifndef RT_ORACLE_H
+#define RT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- IEEE binary16 conversion ---------------- */
+uint16_t orc_f2h_rne(float x);   /* round to nearest even (cuda_fp16 __float2half) */
+uint16_t orc_f2h_down(float x);  /* largest half <= x  (outward rounding for mins) */
+uint16_t orc_f2h_up(float x);    /* smallest half >= x (outward rounding for maxes) */
+float    orc_h2f(uint16_t h);    /* exact */
+
+/* ---------------- BVH build ----------------
+ * Binned-SAH BVH2 (layer/cuda_bvh_backend.cu:201-251) collapsed to BVH4
+ * (:391-418). tris: n x 9 floats (v0.xyz v1.xyz v2.xyz), original order. */
+typedef struct orc_bvh orc_bvh;
+
+orc_bvh* orc_build(const float* tris, int n, int max_leaf);
+void     orc_free(orc_bvh* b);
+int      orc_num_bvh2(const orc_bvh* b);
+int      orc_num_bvh4(const orc_bvh* b);
+int      orc_num_ordered(const orc_bvh* b);
+int      orc_bvh4_depth(const orc_bvh* b);   /* levels of BVH4 nodes on the longest path */
+/* 64-byte BVH4 nodes in the reference format (layer/cuda_bvh_backend.cu:31-34),
+ * bounds rounded outward. */
+void     orc_get_bvh4(const orc_bvh* b, void* out /* num_bvh4*64 bytes */);
+/* ordered_idx[i] = original index of the i-th triangle in leaf order. */
+void     orc_get_order(const orc_bvh* b, int* ordered_idx);
+/* BVH2 nodes, pre-order: box[6], left, right, triStart, triCount. */
+void     orc_get_bvh2(const orc_bvh* b, float* boxes /*n*6*/, int* links /*n*4*/);
+/* Stackless DFS BVH2, 8 x uint32 per node (layer/cuda_bvh_backend.cu:456-522). */
+int      orc_num_dfs(const orc_bvh* b);
+void     orc_get_dfs(const orc_bvh* b, uint32_t* out /* n*8 */);
+/* Binned-SAH cost of the BVH2 (sum over inner nodes of SA(node)/SA(root)
+ * plus leaves' triCount*SA/SA(root)); a tree-quality figure. */
+double   orc_sah_cost(const orc_bvh* b);
+
+/* LBVH over AABBs (layer/cuda_bvh_backend.cu:257-389, :1620-1698).
+ * Returns node count; out (8 uint32 per node) and ordered (n ints) are
+ * malloc'd and must be released with orc_release(). */
+int      orc_lbvh_tlas(const float* aabbs, int n, uint32_t** out, int** ordered);
+void     orc_release(void* p);
+uint32_t orc_morton3d(float x, float y, float z);   /* :265-273 */
+
+/* ---------------- Rays ----------------
+ * Pinhole generator shared with the GPU kernels. For pixel (px,py):
+ *   sx = ((px+0.5)/W)*2-1, sy = ((py+0.5)/H)*2-1            (:69-71)
+ *   d  = f + (sx*scale_x)*r + (sy*scale_y)*u, then d *= 1/sqrt(d.d)
+ * cam: o[3], f[3], r[3], u[3], scale_x, scale_y (14 floats). */
+void orc_gen_pinhole(const float* cam, int W, int H, float* rays /* 8 x W*H SoA */);
+
+/* ---------------- Traversal ----------------
+ * nodes: BVH4 (64 B each); tris: n x 9 floats in leaf order; prim_id: the
+ * original primitive ID of each leaf-order triangle.
+ * rays: SoA, 8 arrays of n_rays: ox oy oz dx dy dz tmin tmax.
+ * A hit is valid when tmin < t < tmax; closest = lexicographic min (t, id).
+ * isect: 0 = watertight (Woop, Benthin, Wald 2013), 1 = the reference's
+ * Moller-Trumbore (layer/cuda_bvh_backend.cu:145-156).
+ * Counters (nullable): BVH4 nodes fetched and triangles tested per ray.
+ * Misses: t = 1e30, id = -1, u = v = 0. threads <= 0 means all cores. */
+void orc_trace_closest(const void* nodes, int num_nodes,
+                       const float* tris, const int* prim_id, int num_tris,
+                       const float* rays, int n_rays,
+                       float* t, int* id, float* u, float* v,
+                       int* node_visits, int* tri_tests,
+                       int isect, int threads);
+
+/* Any hit (v10_pipeline.cu:195-250 semantics: stop at the first valid hit,
+ * children pushed unsorted in slot order 3..0 so slot 0 is popped first). */
+void orc_trace_any(const void* nodes, int num_nodes,
+                   const float* tris, const int* prim_id, int num_tris,
+                   const float* rays, int n_rays, uint8_t* occluded,
+                   int* node_visits, int* tri_tests, int isect, int threads);
+
+/* Brute force over every triangle: watertight FP32 (same per-triangle test as
+ * the traversal) and an FP64 Moller-Trumbore (independent arithmetic). */
+void orc_brute_closest(const float* tris, const int* prim_id, int num_tris,
+                       const float* rays, int n_rays,
+                       float* t, int* id, float* u, float* v, int isect, int threads);
+void orc_brute_closest_f64(const float* tris, const int* prim_id, int num_tris,
+                           const float* rays, int n_rays, double* t, int* id, int threads);
+
+/* Single-triangle tests (for known-answer tests). Return 1 on hit. */
+int orc_isect_watertight(const float* ray /*8*/, const float* tri /*9*/,
+                         float* t, float* u, float* v);
+int orc_isect_mt(const float* ray /*8*/, const float* tri /*9*/,
+                 float* t, float* u, float* v);
+
+/* Ray sort key (cuda_rt_v29.cu:84-106): octant<<29 | (morton & 0x1FFFFFFF),
+ * origin normalised by box (lo, inv). */
+void orc_sort_keys(const float* rays, int n, const float* lo3, const float* inv3,
+                   uint32_t* keys);
+
+int orc_num_threads(int requested);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,306 @@
+"""GPU parity: the HIP traversal through the C ABI against the CPU oracle on
+the same BVH (exported by the library) and the same FP32 rays. IDs, t, u, v
+and the node/triangle visit counters must be bit-identical."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import hit_buffers, to_dev
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "known_answers.npz")
+
+
+def _oracle_inputs(bvh):
+    return bvh.node_data(), np.ascontiguousarray(bvh.tri_data().T), bvh.leaf_prim_ids()
+
+
+def _gpu_closest(bvh, rays, dev, flags=0, counted=False):
+    import torch
+    n = rays.shape[1]
+    R = to_dev(rays, dev)
+    H = hit_buffers(n, dev)
+    if counted:
+        cn = torch.empty(n, dtype=torch.int32, device=dev)
+        ct = torch.empty(n, dtype=torch.int32, device=dev)
+        bvh.trace_closest_counted(R, H["t"], H["prim_id"], H["u"], H["v"], cn, ct, flags)
+    else:
+        bvh.trace_closest(R, H["t"], H["prim_id"], H["u"], H["v"], flags)
+    torch.cuda.synchronize()
+    out = {k: v.cpu().numpy() for k, v in H.items()}
+    if counted:
+        out["nodes"], out["tris"] = cn.cpu().numpy(), ct.cpu().numpy()
+    return out
+
+
+def _gpu_any(bvh, rays, dev, counted=False, flags=0):
+    import torch
+    n = rays.shape[1]
+    R = to_dev(rays, dev)
+    occ = torch.empty(n, dtype=torch.uint8, device=dev)
+    if counted:
+        cn = torch.empty(n, dtype=torch.int32, device=dev)
+        ct = torch.empty(n, dtype=torch.int32, device=dev)
+        bvh.trace_any_counted(R, occ, cn, ct, flags)
+        torch.cuda.synchronize()
+        return occ.cpu().numpy(), cn.cpu().numpy(), ct.cpu().numpy()
+    bvh.trace_any(R, occ, flags)
+    torch.cuda.synchronize()
+    return occ.cpu().numpy()
+
+
+def _same_bits(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+
+
+def _assert_closest_equal(got, ref):
+    assert np.array_equal(got["prim_id"], ref["id"]), f"{(got['prim_id'] != ref['id']).sum()} IDs differ"
+    assert _same_bits(got["t"], ref["t"])
+    assert _same_bits(got["u"], ref["u"]) and _same_bits(got["v"], ref["v"])
+
+
+@pytest.fixture(scope="module")
+def parity_scene(scenes):
+    """The 65,536-triangle parity scene and its 512 x 512 pinhole rays."""
+    tris = scenes.random_triangles(65536, seed=0)
+    cam = scenes.parity_camera(512, 512)
+    return tris, cam
+
+
+@pytest.fixture(scope="module")
+def parity_bvh(pkg, parity_scene, dev):
+    tris, _ = parity_scene
+    b = pkg.BVH.build(tris)
+    yield b
+    b.destroy()
+
+
+def test_builder_matches_oracle(pkg, oracle, parity_scene, parity_bvh):
+    tris, _ = parity_scene
+    ob = oracle.OracleBVH(tris, max_leaf=4)
+    assert np.array_equal(parity_bvh.node_data(), ob.nodes())
+    assert np.array_equal(parity_bvh.leaf_prim_ids(), ob.order())
+    assert np.array_equal(parity_bvh.stackless_bvh2(), ob.dfs())
+    assert np.array_equal(parity_bvh.tri_data().T, tris[ob.order()])
+    assert parity_bvh.num_tris == len(tris) and parity_bvh.num_bvh4_nodes == ob.num_bvh4
+    assert parity_bvh.depth == ob.depth
+    pk = parity_bvh.packed_tris().reshape(-1, 12)
+    assert np.array_equal(pk[:, :9], tris[ob.order()])
+    assert np.array_equal(pk[:, 9], ob.order().astype(np.float32))
+    cx, cy, cz, ext = parity_bvh.bounds()
+    mn, mx = tris.reshape(-1, 3).min(0), tris.reshape(-1, 3).max(0)
+    assert np.allclose([cx, cy, cz], (mn + mx) / 2) and np.isclose(ext, np.linalg.norm(mx - mn), rtol=1e-6)
+
+
+@pytest.mark.parametrize("isect", [0, 1])
+def test_closest_parity_pinhole(oracle, scenes, parity_scene, parity_bvh, dev, isect):
+    _, cam = parity_scene
+    rays = oracle.gen_pinhole(scenes.cam_array(cam), 512, 512)
+    nodes, lt, pid = _oracle_inputs(parity_bvh)
+    ref = oracle.trace_closest(nodes, lt, pid, rays, isect=isect)
+    got = _gpu_closest(parity_bvh, rays, dev, flags=isect, counted=True)
+    _assert_closest_equal(got, ref)
+    assert np.array_equal(got["nodes"], ref["nodes"]) and np.array_equal(got["tris"], ref["tris"])
+    assert (ref["id"] >= 0).mean() > 0.3
+    got2 = _gpu_closest(parity_bvh, rays, dev, flags=isect)      # uncounted kernel, same answers
+    _assert_closest_equal(got2, ref)
+
+
+def test_pinhole_generation_and_fused_trace(pkg, oracle, scenes, parity_scene, parity_bvh, dev):
+    import torch
+    _, cam = parity_scene
+    W = H = 512
+    c = pkg.camera(cam["o"], cam["f"], cam["r"], cam["u"], cam["scale_x"], cam["scale_y"])
+    ref_rays = oracle.gen_pinhole(scenes.cam_array(cam), W, H)
+    R = {k: torch.empty(W * H, device=dev) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")}
+    pkg.gen_pinhole_rays(W, H, c, R)
+    torch.cuda.synchronize()
+    gen = np.stack([R[k].cpu().numpy() for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")])
+    assert _same_bits(gen, ref_rays)
+    hb = hit_buffers(W * H, dev)
+    parity_bvh.trace_pinhole_closest(W, H, c, hb["t"], hb["prim_id"], hb["u"], hb["v"])
+    torch.cuda.synchronize()
+    nodes, lt, pid = _oracle_inputs(parity_bvh)
+    ref = oracle.trace_closest(nodes, lt, pid, ref_rays)
+    _assert_closest_equal({k: v.cpu().numpy() for k, v in hb.items()}, ref)
+
+
+def test_any_hit_parity(oracle, parity_scene, parity_bvh, dev):
+    rng = np.random.default_rng(8)
+    n = 200_000
+    rays = np.zeros((8, n), np.float32)
+    rays[0:3] = rng.uniform(-1.1, 1.1, (3, n))
+    d = rng.normal(size=(3, n))
+    rays[3:6] = d / np.linalg.norm(d, axis=0)
+    rays[6] = 1e-3
+    rays[7] = rng.uniform(0.01, 2.0, n)
+    nodes, lt, pid = _oracle_inputs(parity_bvh)
+    ref = oracle.trace_any(nodes, lt, pid, rays)
+    occ, cn, ct = _gpu_any(parity_bvh, rays, dev, counted=True)
+    assert np.array_equal(occ, ref["occ"])
+    assert np.array_equal(cn, ref["nodes"]) and np.array_equal(ct, ref["tris"])
+    assert np.array_equal(_gpu_any(parity_bvh, rays, dev), ref["occ"])
+    refc = oracle.trace_closest(nodes, lt, pid, rays)
+    got = _gpu_closest(parity_bvh, rays, dev)
+    _assert_closest_equal(got, refc)
+    assert np.array_equal(occ.astype(bool), got["prim_id"] >= 0)   # any == (closest exists)
+    assert 0.05 < occ.mean() < 0.95
+
+
+def test_known_answers_on_gpu(pkg, dev):
+    g = np.load(GOLD)
+    with pkg.BVH.build(g["tris"]) as b:
+        got = _gpu_closest(b, g["rays"], dev)
+        assert np.array_equal(got["prim_id"], g["id"])
+        assert _same_bits(got["t"], g["t"]) and _same_bits(got["u"], g["u"]) and _same_bits(got["v"], g["v"])
+        assert np.array_equal(_gpu_any(b, g["rays"], dev).astype(bool), g["id"] >= 0)
+
+
+def test_axis_parallel_rays(pkg, oracle, dev):
+    """Rays with zero direction components whose origins lie on box planes
+    (the flat-ray slab path), mixed into waves with ordinary rays."""
+    g = np.load(GOLD)
+    tris = np.concatenate([g["tris"], g["tris"] + np.float32(0.5)])
+    rng = np.random.default_rng(9)
+    n = 4096
+    rays = np.zeros((8, n), np.float32)
+    grid = np.float32(0.25) * rng.integers(-8, 40, (2, n)).astype(np.float32)
+    rays[0], rays[1], rays[2] = grid[0], grid[1], -1.0
+    rays[5] = 1.0
+    mixed = rng.random(n) < 0.3                               # some ordinary rays in the same waves
+    d = rng.normal(size=(3, n)).astype(np.float32)
+    d[2] = np.abs(d[2]) + 0.5
+    rays[3:6, mixed] = (d / np.linalg.norm(d, axis=0))[:, mixed]
+    rays[7] = 1e30
+    with pkg.BVH.build(tris) as b:
+        nodes, lt, pid = _oracle_inputs(b)
+        ref = oracle.trace_closest(nodes, lt, pid, rays)
+        bf = oracle.brute_closest(tris, np.arange(len(tris), dtype=np.int32), rays)
+        assert np.array_equal(ref["id"], bf["id"])
+        got = _gpu_closest(b, rays, dev, counted=True)
+        _assert_closest_equal(got, ref)
+        assert np.array_equal(got["nodes"], ref["nodes"])
+        assert (ref["id"] >= 0).sum() > 100
+
+
+def test_edge_cases(pkg, oracle, dev):
+    import torch
+    one = np.array([[0, 0, 1, 1, 0, 1, 0, 1, 1]], np.float32)
+    with pkg.BVH.build(one) as b:
+        assert b.num_bvh4_nodes == 1 and b.num_tris == 1
+        rays = np.array([[0.25, 2.0], [0.25, 2.0], [0, 0], [0, 0], [0, 0], [1, 1], [0, 0], [1e30, 1e30]], np.float32)
+        got = _gpu_closest(b, rays, dev)
+        assert got["prim_id"].tolist() == [0, -1] and got["t"][0] == 1.0 and got["t"][1] == np.float32(1e30)
+        # zero rays is a no-op
+        R = to_dev(np.zeros((8, 0), np.float32), dev)
+        H = hit_buffers(0, dev)
+        b.trace_closest(R, H["t"], H["prim_id"], H["u"], H["v"])
+    # coincident triangles: the lowest original ID wins
+    many = np.tile(one, (37, 1))
+    with pkg.BVH.build(many) as b:
+        got = _gpu_closest(b, np.array([[0.25], [0.25], [0], [0], [0], [1], [0], [1e30]], np.float32), dev)
+        assert got["prim_id"][0] == 0
+    # invalid input: NULL handle and an empty build, reported through lastError
+    with pytest.raises(pkg.CudaBVHError):
+        pkg.BVH.build(np.zeros((0, 9), np.float32))
+    assert "numTris" in pkg.last_error()
+
+
+def test_host_pointer_flag(pkg, oracle, scenes, parity_bvh, dev):
+    rays = oracle.gen_pinhole(scenes.cam_array(scenes.parity_camera(128, 128)), 128, 128)
+    n = rays.shape[1]
+    H = dict(t=np.zeros(n, np.float32), prim_id=np.zeros(n, np.int32), u=np.zeros(n, np.float32),
+             v=np.zeros(n, np.float32))
+    R = {k: np.ascontiguousarray(rays[i]) for i, k in enumerate(("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax"))}
+    parity_bvh.trace_closest(R, H["t"], H["prim_id"], H["u"], H["v"], flags=pkg.FLAG_HOST_POINTERS)
+    nodes, lt, pid = _oracle_inputs(parity_bvh)
+    _assert_closest_equal(H, oracle.trace_closest(nodes, lt, pid, rays))
+    occ = np.zeros(n, np.uint8)
+    parity_bvh.trace_any(R, occ, flags=pkg.FLAG_HOST_POINTERS)
+    assert np.array_equal(occ, oracle.trace_any(nodes, lt, pid, rays)["occ"])
+
+
+def test_reference_trace_primary(pkg, oracle, parity_scene, parity_bvh):
+    """cudaBVH_tracePrimary: side x side rays from camO looking +z, 90 degree
+    FOV (layer/cuda_bvh_backend.cu:69-74), hit distances row-major."""
+    side = 256
+    cam_o = (0.0, 0.0, -3.0)
+    out = np.zeros(side * side, np.float32)
+    mrs = parity_bvh.trace_primary(side, cam_o, out)
+    assert mrs > 0
+    cam14 = np.array([0, 0, -3, 0, 0, 1, 1, 0, 0, 0, 1, 0, 1, 1], np.float32)
+    rays = oracle.gen_pinhole(cam14, side, side)
+    nodes, lt, pid = _oracle_inputs(parity_bvh)
+    ref = oracle.trace_closest(nodes, lt, pid, rays)
+    assert _same_bits(out, ref["t"])
+    assert (out < 1e20).mean() > 0.3
+
+
+def test_reference_trace_diffuse(pkg, oracle, parity_bvh):
+    rng = np.random.default_rng(12)
+    n = 50_000
+    o = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    out = np.zeros(n, np.float32)
+    assert parity_bvh.trace_diffuse(o, d, out) > 0
+    rays = np.zeros((8, n), np.float32)
+    rays[0:3], rays[3:6], rays[6], rays[7] = o.T, d.T, 0.0, 1e30
+    nodes, lt, pid = _oracle_inputs(parity_bvh)
+    assert _same_bits(out, oracle.trace_closest(nodes, lt, pid, rays)["t"])
+
+
+def test_shading_formats(pkg, oracle, parity_bvh, dev):
+    """traceToRGBA / traceToGPUPtr: misses get the sky gradient, hits the
+    Lambert+rim shade (layer/cuda_bvh_backend.cu:923-984); the hit mask and
+    colours match a numpy restatement on the oracle's hits."""
+    import torch
+    W, H = 160, 96
+    cam_o = (0.3, 0.2, -3.0)
+    img = parity_bvh.trace_to_rgba(W, H, cam_o)
+    f32 = torch.empty(W * H * 4, device=dev)
+    parity_bvh.trace_to_gpu_ptr(W, H, cam_o, f32.data_ptr(), 3)
+    f16 = torch.empty(W * H * 4, dtype=torch.float16, device=dev)
+    parity_bvh.trace_to_gpu_ptr(W, H, cam_o, f16.data_ptr(), 2)
+    bgra = torch.empty(W * H, dtype=torch.int32, device=dev)
+    parity_bvh.trace_to_gpu_ptr(W, H, cam_o, bgra.data_ptr(), 1)
+    torch.cuda.synchronize()
+    rgb = f32.cpu().numpy().reshape(H, W, 4)
+    # camera of backend :991-1016
+    cx, cy, cz, _ = parity_bvh.bounds()
+    fw = np.array([cx, cy, cz], np.float32) - np.array(cam_o, np.float32)
+    fw = fw / np.float32(np.sqrt(np.float32(fw @ fw)))
+    rt = np.array([fw[2], 0, -fw[0]], np.float32)
+    rt = rt / np.float32(np.sqrt(rt[0] ** 2 + rt[2] ** 2))
+    up = np.cross(rt, fw).astype(np.float32)
+    cam14 = np.concatenate([np.array(cam_o, np.float32), fw, rt, up, np.array([W / H * 0.6, 0.6], np.float32)])
+    rays = oracle.gen_pinhole(cam14.astype(np.float32), W, H)
+    nodes, lt, pid = _oracle_inputs(parity_bvh)
+    ref = oracle.trace_closest(nodes, lt, pid, rays)
+    hit = (ref["id"] >= 0).reshape(H, W)
+    py = np.arange(H, dtype=np.float32)[:, None] / np.float32(H)
+    sky = np.stack([(20 + 10 * (1 - py)) / 255, (30 + 20 * (1 - py)) / 255, (60 + 100 * py) / 255], -1)
+    sky = np.broadcast_to(sky, (H, W, 3))
+    assert np.allclose(rgb[~hit][:, :3], sky[~hit], atol=1e-6)
+    assert hit.mean() > 0.2
+    assert (rgb[hit][:, 0] >= 0.08).all() and (rgb[..., 3] == 1).all()
+    assert np.allclose(f16.cpu().numpy().reshape(H, W, 4), rgb, atol=2e-3)
+    r8 = (img & 0xFF).astype(np.int32)
+    assert np.array_equal(r8, (rgb[..., 0] * 255).astype(np.uint8).astype(np.int32))
+    b8 = (bgra.cpu().numpy().reshape(H, W).view(np.uint32) & 0xFF).astype(np.int32)
+    assert np.array_equal(b8, (rgb[..., 2] * 255).astype(np.uint8).astype(np.int32))
+
+
+def test_determinism_and_stream(pkg, oracle, scenes, parity_bvh, dev):
+    import torch
+    rays = oracle.gen_pinhole(scenes.cam_array(scenes.parity_camera(256, 256)), 256, 256)
+    s = torch.cuda.Stream()
+    parity_bvh.set_stream(s.cuda_stream)
+    try:
+        with torch.cuda.stream(s):
+            a = _gpu_closest(parity_bvh, rays, dev)
+            b = _gpu_closest(parity_bvh, rays, dev)
+    finally:
+        parity_bvh.set_stream(None)
+    _assert_closest_equal(a, {"id": b["prim_id"], "t": b["t"], "u": b["u"], "v": b["v"]})

@@ -1,0 +1,347 @@
+"""MI355X-native software ray-tracing core: Python host bindings.
+
+The product is ``libcudabvh_hip.so`` (HIP kernels for gfx950 behind the C ABI
+of ``include/cudabvh.h``). This module binds every exported symbol with ctypes,
+mirroring the reference boundary ``layer/cuda_bvh_backend.h`` function by
+function (same names without the ``cudaBVH_`` prefix, same argument meaning,
+same return conventions: 0 / -1 for int calls, MRays/s / 0 for float calls,
+NULL handles on failure), and adds :class:`BVH`, a small owner of a handle.
+
+Importing succeeds without a GPU; loading the library raises
+:class:`LibraryMissing` when it has not been built — there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcudabvh_hip.so")
+
+# Flags (include/cudabvh.h, part 2)
+ISECT_WATERTIGHT = 0
+ISECT_REFERENCE_MT = 1
+FLAG_HOST_POINTERS = 1 << 4
+FLAG_SORT_RAYS = 1 << 5
+FLAG_SYNC = 1 << 6
+MISS_T = np.float32(1e30)
+
+
+class LibraryMissing(RuntimeError):
+    """libcudabvh_hip.so is not built (run __graft_entry__.build())."""
+
+
+class CudaBVHError(RuntimeError):
+    """A cudaBVH_* call reported failure; the message is cudaBVH_lastError()."""
+
+
+class CudaTri(C.Structure):
+    _fields_ = [("v0", C.c_float * 3), ("v1", C.c_float * 3), ("v2", C.c_float * 3)]
+
+
+class CudaRayPacket(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")]
+
+
+class CudaHitPacket(C.Structure):
+    _fields_ = [("t", C.c_void_p), ("primId", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p)]
+
+
+class CudaCamera(C.Structure):
+    _fields_ = [("o", C.c_float * 3), ("f", C.c_float * 3), ("r", C.c_float * 3), ("u", C.c_float * 3),
+                ("scaleX", C.c_float), ("scaleY", C.c_float)]
+
+
+# name -> (restype, argtypes); the exported C ABI, in header order.
+_H = C.c_void_p
+_P = C.c_void_p
+_F = C.c_float
+_I = C.c_int
+SIGNATURES = {
+    "cudaBVH_build": (_H, [_P, _I]),
+    "cudaBVH_destroy": (None, [_H]),
+    "cudaBVH_tracePrimary": (_F, [_H, _I, _F, _F, _F, _P]),
+    "cudaBVH_traceDiffuse": (_F, [_H, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "cudaBVH_traceToRGBA": (_I, [_H, _I, _I, _F, _F, _F, _P]),
+    "cudaBVH_traceToGPUPtr": (_I, [_H, _I, _I, _F, _F, _F, _P, _I]),
+    "cudaBVH_traceToGPUAsync": (None, [_H, _I, _I, _F, _F, _F, _P, _I]),
+    "cudaBVH_traceToHostPtr": (_I, [_H, _I, _I, _F, _F, _F, _P, _I, C.c_uint64]),
+    "cudaBVH_importBufferFd": (_P, [_I, C.c_uint64]),
+    "cudaBVH_getBounds": (None, [_H, _P, _P, _P, _P]),
+    "cudaBVH_getNumTris": (_I, [_H]),
+    "cudaBVH_getNumBVH4Nodes": (_I, [_H]),
+    "cudaBVH_getNumCWBVHNodes": (_I, [_H]),
+    "cudaBVH_getNodeData": (_P, [_H]),
+    "cudaBVH_getTriData": (None, [_H, _P]),
+    "cudaBVH_getStacklessBVH2": (_I, [_H, _P]),
+    "cudaBVH_getPackedTris": (_I, [_H, _P]),
+    "cudaBVH_resetDevice": (None, []),
+    "cudaBVH_buildFromAABBs": (_H, [_P, _I]),
+    "cudaBVH_buildTLASFast": (_I, [_P, _I, _P, _P]),
+    "cudaBVH_readGPUMemoryFd": (_I, [_I, C.c_uint64, C.c_uint64, C.c_uint64, _P]),
+    # extensions
+    "cudaBVH_traceClosest": (_I, [_H, _I, C.POINTER(CudaRayPacket), C.POINTER(CudaHitPacket), C.c_uint]),
+    "cudaBVH_traceAny": (_I, [_H, _I, C.POINTER(CudaRayPacket), _P, C.c_uint]),
+    "cudaBVH_traceClosestCounted": (_I, [_H, _I, C.POINTER(CudaRayPacket), C.POINTER(CudaHitPacket), _P, _P,
+                                         C.c_uint]),
+    "cudaBVH_traceAnyCounted": (_I, [_H, _I, C.POINTER(CudaRayPacket), _P, _P, _P, C.c_uint]),
+    "cudaBVH_genPinholeRays": (_I, [_I, _I, C.POINTER(CudaCamera), C.POINTER(CudaRayPacket), _P]),
+    "cudaBVH_tracePinholeClosest": (_I, [_H, _I, _I, C.POINTER(CudaCamera), C.POINTER(CudaHitPacket), C.c_uint]),
+    "cudaBVH_genBounceRays": (_I, [_H, _I, _I, C.POINTER(CudaCamera), C.POINTER(CudaHitPacket), C.c_uint32, _P,
+                                   C.POINTER(CudaRayPacket), C.POINTER(CudaRayPacket), _P, _P, C.c_uint]),
+    "cudaBVH_traceClosestQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), C.POINTER(CudaHitPacket), C.c_uint]),
+    "cudaBVH_traceAnyQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), _P, C.c_uint]),
+    "cudaBVH_setStream": (_I, [_H, _P]),
+    "cudaBVH_getStream": (_P, [_H]),
+    "cudaBVH_deviceNodes": (_P, [_H]),
+    "cudaBVH_deviceTris": (_P, [_H]),
+    "cudaBVH_getDepth": (_I, [_H]),
+    "cudaBVH_lastError": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libcudabvh_hip.so and declare every signature (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise LibraryMissing(f"{path} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    msg = load_library().cudaBVH_lastError()
+    return msg.decode() if msg else ""
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise CudaBVHError(f"{what}: {last_error()}")
+
+
+def _ptr(x) -> int | None:
+    """Address of a numpy array or torch tensor (None passes NULL)."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        assert x.flags["C_CONTIGUOUS"]
+        return x.ctypes.data
+    return x.data_ptr()  # torch tensor
+
+
+def ray_packet(rays) -> CudaRayPacket:
+    """Packet from a dict (or object) holding ox oy oz dx dy dz tmin tmax arrays."""
+    get = rays.__getitem__ if isinstance(rays, dict) else (lambda k: getattr(rays, k))
+    return CudaRayPacket(*[_ptr(get(k)) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")])
+
+
+def hit_packet(t, prim_id=None, u=None, v=None) -> CudaHitPacket:
+    return CudaHitPacket(_ptr(t), _ptr(prim_id), _ptr(u), _ptr(v))
+
+
+def camera(o, f, r, u, scale_x, scale_y) -> CudaCamera:
+    c = CudaCamera()
+    for dst, src in ((c.o, o), (c.f, f), (c.r, r), (c.u, u)):
+        for i in range(3):
+            dst[i] = float(np.float32(src[i]))
+    c.scaleX = float(np.float32(scale_x))
+    c.scaleY = float(np.float32(scale_y))
+    return c
+
+
+class BVH:
+    """Owner of one CudaBVH_t handle; methods mirror the C ABI."""
+
+    def __init__(self, handle: int, num_input: int):
+        if not handle:
+            raise CudaBVHError(f"build failed: {last_error()}")
+        self.lib = load_library()
+        self.handle = C.c_void_p(handle)
+        self.num_input = num_input
+
+    # --- construction (layer/cuda_bvh_backend.h:21, :123) ---
+    @classmethod
+    def build(cls, tris: np.ndarray) -> "BVH":
+        """tris: float32 array of shape (n, 9) or (n, 3, 3) in original order."""
+        t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+        return cls(load_library().cudaBVH_build(t.ctypes.data, t.shape[0]), t.shape[0])
+
+    @classmethod
+    def build_from_aabbs(cls, aabbs: np.ndarray) -> "BVH":
+        a = np.ascontiguousarray(aabbs, dtype=np.float32).reshape(-1, 6)
+        return cls(load_library().cudaBVH_buildFromAABBs(a.ctypes.data, a.shape[0]), a.shape[0])
+
+    def destroy(self) -> None:
+        if self.handle:
+            self.lib.cudaBVH_destroy(self.handle)
+            self.handle = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+    # --- stats and host views ---
+    @property
+    def num_tris(self) -> int:
+        return self.lib.cudaBVH_getNumTris(self.handle)
+
+    @property
+    def num_bvh4_nodes(self) -> int:
+        return self.lib.cudaBVH_getNumBVH4Nodes(self.handle)
+
+    @property
+    def num_cwbvh_nodes(self) -> int:
+        return self.lib.cudaBVH_getNumCWBVHNodes(self.handle)
+
+    @property
+    def depth(self) -> int:
+        return self.lib.cudaBVH_getDepth(self.handle)
+
+    def bounds(self):
+        v = [C.c_float() for _ in range(4)]
+        self.lib.cudaBVH_getBounds(self.handle, *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def node_data(self) -> np.ndarray:
+        """BVH4 nodes as uint8 (n, 64), copied out of the borrowed pointer."""
+        p = self.lib.cudaBVH_getNodeData(self.handle)
+        n = self.num_bvh4_nodes
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(n * 64,)).reshape(n, 64).copy()
+
+    def tri_data(self) -> np.ndarray:
+        """Leaf-order SoA triangles, float32 (9, n)."""
+        ptrs = (C.c_void_p * 9)()
+        self.lib.cudaBVH_getTriData(self.handle, ptrs)
+        n = self.num_tris
+        return np.stack([np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(n,)).copy() for p in ptrs])
+
+    def stackless_bvh2(self) -> np.ndarray:
+        out = C.c_void_p()
+        n = self.lib.cudaBVH_getStacklessBVH2(self.handle, C.byref(out))
+        if n == 0:
+            return np.zeros((0, 8), np.uint32)
+        return np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_uint32)), shape=(n * 8,)).reshape(n, 8).copy()
+
+    def packed_tris(self) -> np.ndarray:
+        out = C.c_void_p()
+        n = self.lib.cudaBVH_getPackedTris(self.handle, C.byref(out))
+        return np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_float)), shape=(n * 4,)).reshape(n, 4).copy()
+
+    def leaf_prim_ids(self) -> np.ndarray:
+        """Original primitive index of each leaf-order triangle (from the packed export)."""
+        return self.packed_tris()[2::3, 1].astype(np.int64).astype(np.int32)
+
+    # --- streams ---
+    def set_stream(self, stream: int | None) -> None:
+        _check(self.lib.cudaBVH_setStream(self.handle, stream), "setStream")
+
+    # --- reference traces (layer/cuda_bvh_backend.h:29-73) ---
+    def trace_primary(self, side: int, cam_o, out_hit_t: np.ndarray | None = None) -> float:
+        return self.lib.cudaBVH_tracePrimary(self.handle, side, *[float(c) for c in cam_o], _ptr(out_hit_t))
+
+    def trace_diffuse(self, o: np.ndarray, d: np.ndarray, out_hit_t: np.ndarray | None = None) -> float:
+        o = [np.ascontiguousarray(o[:, k], np.float32) for k in range(3)]
+        d = [np.ascontiguousarray(d[:, k], np.float32) for k in range(3)]
+        return self.lib.cudaBVH_traceDiffuse(self.handle, o[0].shape[0], *[x.ctypes.data for x in o + d],
+                                             _ptr(out_hit_t))
+
+    def trace_to_rgba(self, width: int, height: int, cam_o) -> np.ndarray:
+        out = np.zeros(width * height, np.uint32)
+        _check(self.lib.cudaBVH_traceToRGBA(self.handle, width, height, *[float(c) for c in cam_o], out.ctypes.data),
+               "traceToRGBA")
+        return out.reshape(height, width)
+
+    def trace_to_gpu_ptr(self, width: int, height: int, cam_o, dev_ptr: int, fmt: int) -> None:
+        _check(self.lib.cudaBVH_traceToGPUPtr(self.handle, width, height, *[float(c) for c in cam_o], dev_ptr, fmt),
+               "traceToGPUPtr")
+
+    # --- extensions (device or host packets) ---
+    def trace_closest(self, rays, t, prim_id=None, u=None, v=None, flags: int = 0) -> None:
+        n = int(rays["ox"].shape[0]) if isinstance(rays, dict) else int(rays.ox.shape[0])
+        rp, hp = ray_packet(rays), hit_packet(t, prim_id, u, v)
+        _check(self.lib.cudaBVH_traceClosest(self.handle, n, C.byref(rp), C.byref(hp), flags), "traceClosest")
+
+    def trace_any(self, rays, occluded, flags: int = 0) -> None:
+        n = int(rays["ox"].shape[0]) if isinstance(rays, dict) else int(rays.ox.shape[0])
+        rp = ray_packet(rays)
+        _check(self.lib.cudaBVH_traceAny(self.handle, n, C.byref(rp), _ptr(occluded), flags), "traceAny")
+
+    def trace_closest_counted(self, rays, t, prim_id, u, v, node_visits, tri_tests, flags: int = 0) -> None:
+        n = int(rays["ox"].shape[0])
+        rp, hp = ray_packet(rays), hit_packet(t, prim_id, u, v)
+        _check(self.lib.cudaBVH_traceClosestCounted(self.handle, n, C.byref(rp), C.byref(hp), _ptr(node_visits),
+                                                    _ptr(tri_tests), flags), "traceClosestCounted")
+
+    def trace_any_counted(self, rays, occluded, node_visits, tri_tests, flags: int = 0) -> None:
+        n = int(rays["ox"].shape[0])
+        rp = ray_packet(rays)
+        _check(self.lib.cudaBVH_traceAnyCounted(self.handle, n, C.byref(rp), _ptr(occluded), _ptr(node_visits),
+                                                _ptr(tri_tests), flags), "traceAnyCounted")
+
+    def trace_pinhole_closest(self, width: int, height: int, cam: CudaCamera, t, prim_id=None, u=None, v=None,
+                              flags: int = 0) -> None:
+        hp = hit_packet(t, prim_id, u, v)
+        _check(self.lib.cudaBVH_tracePinholeClosest(self.handle, width, height, C.byref(cam), C.byref(hp), flags),
+               "tracePinholeClosest")
+
+
+    def gen_bounce_rays(self, width: int, height: int, cam: CudaCamera, hit_t, hit_id, seed: int, light,
+                        diffuse, shadow, src_index, d_count, flags: int = 0) -> None:
+        hp = hit_packet(hit_t, hit_id)
+        lt = (C.c_float * 3)(*[float(x) for x in light])
+        dp, sp = ray_packet(diffuse), ray_packet(shadow)
+        _check(self.lib.cudaBVH_genBounceRays(self.handle, width, height, C.byref(cam), C.byref(hp), seed, lt,
+                                              C.byref(dp), C.byref(sp), _ptr(src_index), _ptr(d_count), flags),
+               "genBounceRays")
+
+    def trace_closest_queued(self, capacity: int, d_count, rays, t, prim_id=None, u=None, v=None,
+                             flags: int = 0) -> None:
+        rp, hp = ray_packet(rays), hit_packet(t, prim_id, u, v)
+        _check(self.lib.cudaBVH_traceClosestQueued(self.handle, capacity, _ptr(d_count), C.byref(rp), C.byref(hp),
+                                                   flags), "traceClosestQueued")
+
+    def trace_any_queued(self, capacity: int, d_count, rays, occluded, flags: int = 0) -> None:
+        rp = ray_packet(rays)
+        _check(self.lib.cudaBVH_traceAnyQueued(self.handle, capacity, _ptr(d_count), C.byref(rp), _ptr(occluded),
+                                               flags), "traceAnyQueued")
+
+
+def gen_pinhole_rays(width: int, height: int, cam: CudaCamera, rays, stream: int | None = None) -> None:
+    rp = ray_packet(rays)
+    _check(load_library().cudaBVH_genPinholeRays(width, height, C.byref(cam), C.byref(rp), stream), "genPinholeRays")
+
+
+def build_tlas_fast(aabbs: np.ndarray):
+    """LBVH TLAS: returns (nodes uint32 (n, 8), ordered int32 (m,))."""
+    lib = load_library()
+    a = np.ascontiguousarray(aabbs, dtype=np.float32).reshape(-1, 6)
+    nodes, order = C.c_void_p(), C.c_void_p()
+    n = lib.cudaBVH_buildTLASFast(a.ctypes.data, a.shape[0], C.byref(nodes), C.byref(order))
+    if n <= 0:
+        raise CudaBVHError(f"buildTLASFast: {last_error()}")
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    out = np.ctypeslib.as_array(C.cast(nodes, C.POINTER(C.c_uint32)), shape=(n * 8,)).reshape(n, 8).copy()
+    ordv = np.ctypeslib.as_array(C.cast(order, C.POINTER(C.c_int32)), shape=(a.shape[0],)).copy()
+    libc.free(nodes)
+    libc.free(order)
+    return out, ordv

@@ -1,0 +1,678 @@
+// api.hip — the extern "C" boundary of libcudabvh_hip.so. Part 1 keeps the
+// reference's cudaBVH_* symbols (layer/cuda_bvh_backend.h:12-141); part 2
+// adds the packet entry points the reference lacks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "rt_internal.h"
+#include "trace_launch.h"
+
+namespace rt {
+const char* last_error();
+void build_sah_host(const float* tris, int n, int maxLeaf, HostBVH& out);
+}
+
+using rt::set_error;
+
+#define HIP_TRY(expr, ...)                                                        \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) {                                                   \
+            set_error("%s failed: %s", #expr, hipGetErrorString(_e));             \
+            __VA_ARGS__;                                                          \
+        }                                                                         \
+    } while (0)
+
+static inline hipStream_t hstream(const CudaBVHHandle* h) { return (hipStream_t)h->stream; }
+
+static void free_handle_device(CudaBVHHandle* h) {
+    hipFree(h->d_nodes4); hipFree(h->d_tris); hipFree(h->d_leafOfPrim); hipFree(h->d_work);
+    rtk::free_scratch(h->scratch);
+    h->d_nodes4 = h->d_tris = h->d_work = nullptr;
+    h->d_leafOfPrim = nullptr;
+}
+
+// Builds a handle from n x 9 floats (original order).
+static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf) {
+    rt::clear_error();
+    if (!tri9 || n <= 0) { set_error("cudaBVH_build: need tris != NULL and numTris > 0 (got %d)", n); return nullptr; }
+    if (n > (1 << 27)) { set_error("cudaBVH_build: %d triangles exceed the leaf encoding (2^27)", n); return nullptr; }
+    auto* h = new CudaBVHHandle();
+    h->numTris = n;
+    float bmin[3] = {1e30f, 1e30f, 1e30f}, bmax[3] = {-1e30f, -1e30f, -1e30f};
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 9; k++) {
+            const float x = tri9[9 * size_t(i) + k];
+            bmin[k % 3] = std::min(bmin[k % 3], x);
+            bmax[k % 3] = std::max(bmax[k % 3], x);
+        }
+    for (int a = 0; a < 3; a++) { h->bboxMin[a] = bmin[a]; h->bboxMax[a] = bmax[a]; h->bboxCenter[a] = (bmin[a] + bmax[a]) * 0.5f; }
+    const float ex = bmax[0] - bmin[0], ey = bmax[1] - bmin[1], ez = bmax[2] - bmin[2];
+    h->bboxExtent = sqrtf(ex * ex + ey * ey + ez * ez);
+
+    rt::HostBVH hb;
+    rt::build_sah_host(tri9, n, maxLeaf, hb);
+    h->numOrdered = int(hb.ordered.size());
+    h->numNodes4 = hb.numNodes4;
+    h->depth4 = hb.depth4;
+    h->h_nodes4 = std::move(hb.nodes4);
+    h->h_ordered = hb.ordered;
+
+    // Leaf-order triangles: {v0.xyz, bits(primId)}, {v1.xyz, 0}, {v2.xyz, 0}
+    const int m = h->numOrdered;
+    std::vector<float> aos(size_t(m) * 12);
+    std::vector<int> leafOf(n, -1);
+    for (int a = 0; a < 9; a++) h->h_soa[a].resize(m);
+    for (int i = 0; i < m; i++) {
+        const int p = hb.ordered[i];
+        const float* s = tri9 + 9 * size_t(p);
+        float* d = &aos[12 * size_t(i)];
+        d[0] = s[0]; d[1] = s[1]; d[2] = s[2]; memcpy(&d[3], &p, 4);
+        d[4] = s[3]; d[5] = s[4]; d[6] = s[5]; d[7] = 0.f;
+        d[8] = s[6]; d[9] = s[7]; d[10] = s[8]; d[11] = 0.f;
+        for (int a = 0; a < 9; a++) h->h_soa[a][i] = s[a];
+        leafOf[p] = i;
+    }
+    // Host exports for the Vulkan-side consumers
+    rt::export_dfs(hb.nodes2, h->h_dfs);
+    h->numDFS = int(hb.nodes2.size());
+    h->h_packed.assign(size_t(m) * 12, 0.f);
+    for (int i = 0; i < m; i++) {
+        float* p = &h->h_packed[12 * size_t(i)];
+        for (int a = 0; a < 9; a++) p[a] = h->h_soa[a][i];
+        p[9] = (float)hb.ordered[i];   // the value, not the bits (backend :730-733)
+    }
+
+    HIP_TRY(hipMalloc(&h->d_nodes4, h->h_nodes4.size()), goto fail);
+    HIP_TRY(hipMalloc(&h->d_tris, aos.size() * sizeof(float)), goto fail);
+    HIP_TRY(hipMalloc((void**)&h->d_leafOfPrim, size_t(n) * sizeof(int)), goto fail);
+    HIP_TRY(hipMemcpy(h->d_nodes4, h->h_nodes4.data(), h->h_nodes4.size(), hipMemcpyHostToDevice), goto fail);
+    HIP_TRY(hipMemcpy(h->d_tris, aos.data(), aos.size() * sizeof(float), hipMemcpyHostToDevice), goto fail);
+    HIP_TRY(hipMemcpy(h->d_leafOfPrim, leafOf.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice), goto fail);
+    if (rtk::alloc_scratch(h->scratch) != 0) goto fail;
+    if (rt::verbose())
+        fprintf(stderr, "[CudaBVH] built %d tris: %d BVH4 nodes (depth %d), %d BVH2 nodes\n", n, h->numNodes4,
+                h->depth4, h->numDFS);
+    return h;
+fail:
+    free_handle_device(h);
+    delete h;
+    return nullptr;
+}
+
+static bool check_handle(CudaBVH_t bvh, const char* fn) {
+    if (!bvh) { set_error("%s: NULL handle", fn); return false; }
+    return true;
+}
+
+// Device staging for calls that hand over host arrays.
+static void* work_buffer(CudaBVHHandle* h, size_t bytes) {
+    if (bytes > h->workBytes) {
+        hipFree(h->d_work);
+        h->d_work = nullptr; h->workBytes = 0;
+        if (hipMalloc(&h->d_work, bytes) != hipSuccess) { set_error("cannot allocate %zu staging bytes", bytes); return nullptr; }
+        h->workBytes = bytes;
+    }
+    return h->d_work;
+}
+
+static rtd::Cam to_cam(const CudaCamera& c) {
+    rtd::Cam k;
+    for (int a = 0; a < 3; a++) { k.o[a] = c.o[a]; k.f[a] = c.f[a]; k.r[a] = c.r[a]; k.u[a] = c.u[a]; }
+    k.sx = c.scaleX; k.sy = c.scaleY;
+    return k;
+}
+
+// Reference look-at camera toward the scene centre (backend :991-1016).
+static rtd::Cam lookat_cam(const CudaBVHHandle* h, int width, int height, float cx, float cy, float cz) {
+    float fx = h->bboxCenter[0] - cx, fy = h->bboxCenter[1] - cy, fz = h->bboxCenter[2] - cz;
+    float fl = sqrtf(fx * fx + fy * fy + fz * fz);
+    if (fl < 1e-6f) fl = 1.f;
+    fx /= fl; fy /= fl; fz /= fl;
+    float rx = fz, ry = 0.f, rz = -fx;
+    const float rl = sqrtf(rx * rx + rz * rz);
+    if (rl > 1e-6f) { rx /= rl; rz /= rl; } else { rx = 1.f; rz = 0.f; }
+    const float ux = ry * fz - rz * fy, uy = rz * fx - rx * fz, uz = rx * fy - ry * fx;
+    const float fov = 0.6f, aspect = (float)width / (float)height;
+    rtd::Cam k{};
+    k.o[0] = cx; k.o[1] = cy; k.o[2] = cz;
+    k.f[0] = fx; k.f[1] = fy; k.f[2] = fz;
+    k.r[0] = rx; k.r[1] = ry; k.r[2] = rz;
+    k.u[0] = ux; k.u[1] = uy; k.u[2] = uz;
+    k.sx = aspect * fov; k.sy = fov;
+    return k;
+}
+
+static int shade_to_device(CudaBVHHandle* h, rt::TraceScratch& sc, hipStream_t s, int width, int height, float cx,
+                           float cy, float cz, void* dimg, int fmt) {
+    if (width <= 0 || height <= 0 || (long long)width * height > (1ll << 30)) { set_error("bad image size %dx%d", width, height); return -1; }
+    if (fmt < 0 || fmt > 3) { set_error("bad output format %d", fmt); return -1; }
+    rtk::TraceArgs a;
+    a.n = width * height; a.W = width; a.H = height;
+    a.cam = lookat_cam(h, width, height, cx, cy, cz);
+    a.img = dimg; a.fmt = fmt;
+    return rtk::launch_trace(h, sc, a, rtk::kModeShade, false, true, false, s);
+}
+
+extern "C" {
+
+const char* cudaBVH_lastError(void) { return rt::last_error(); }
+
+CudaBVH_t cudaBVH_build(const CudaTri* tris, int numTris) {
+    return build_impl(reinterpret_cast<const float*>(tris), numTris, 4);
+}
+
+void cudaBVH_destroy(CudaBVH_t bvh) {
+    if (!bvh) return;
+    if (bvh->stream) hipStreamSynchronize(hstream(bvh)); else hipDeviceSynchronize();
+    free_handle_device(bvh);
+    delete bvh;
+}
+
+float cudaBVH_tracePrimary(CudaBVH_t bvh, int side, float camOx, float camOy, float camOz, float* outHitT) {
+    rt::clear_error();
+    if (!check_handle(bvh, "cudaBVH_tracePrimary")) return 0.f;
+    if (side <= 0 || (long long)side * side > (1ll << 30)) { set_error("cudaBVH_tracePrimary: bad side %d", side); return 0.f; }
+    const int n = side * side;
+    float* dt = (float*)work_buffer(bvh, size_t(n) * sizeof(float));
+    if (!dt) return 0.f;
+    rtk::TraceArgs a;
+    a.n = n; a.W = side; a.H = side;
+    const float o[3] = {camOx, camOy, camOz};
+    for (int k = 0; k < 3; k++) { a.cam.o[k] = o[k]; a.cam.f[k] = 0.f; a.cam.r[k] = 0.f; a.cam.u[k] = 0.f; }
+    a.cam.f[2] = 1.f; a.cam.r[0] = 1.f; a.cam.u[1] = 1.f;   // looking +z, 90 degrees (backend :69-74)
+    a.cam.sx = 1.f; a.cam.sy = 1.f;
+    a.t = dt;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0); hipEventCreate(&e1);
+    hipEventRecord(e0, hstream(bvh));
+    int rc = rtk::launch_trace(bvh, bvh->scratch, a, rtk::kModeClosest, false, true, false, hstream(bvh));
+    hipEventRecord(e1, hstream(bvh));
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0); hipEventDestroy(e1);
+    if (rc != 0) return 0.f;
+    if (outHitT) HIP_TRY(hipMemcpy(outHitT, dt, size_t(n) * sizeof(float), hipMemcpyDeviceToHost), return 0.f);
+    const float mrs = ms > 0.f ? (float)n / (ms * 1000.f) : 0.f;
+    if (rt::verbose()) fprintf(stderr, "[CudaBVH] Primary trace: %dx%d = %d rays in %.3fms -> %.0f MR/s\n", side, side, n, ms, mrs);
+    return mrs;
+}
+
+float cudaBVH_traceDiffuse(CudaBVH_t bvh, int numRays, const float* rayOx, const float* rayOy, const float* rayOz,
+                           const float* rayDx, const float* rayDy, const float* rayDz, float* outHitT) {
+    rt::clear_error();
+    if (!check_handle(bvh, "cudaBVH_traceDiffuse")) return 0.f;
+    if (numRays <= 0 || !rayOx || !rayOy || !rayOz || !rayDx || !rayDy || !rayDz) {
+        set_error("cudaBVH_traceDiffuse: bad arguments (numRays %d)", numRays);
+        return 0.f;
+    }
+    const size_t n = size_t(numRays);
+    float* w = (float*)work_buffer(bvh, n * 9 * sizeof(float));
+    if (!w) return 0.f;
+    const float* src[6] = {rayOx, rayOy, rayOz, rayDx, rayDy, rayDz};
+    for (int k = 0; k < 6; k++) HIP_TRY(hipMemcpyAsync(w + k * n, src[k], n * 4, hipMemcpyHostToDevice, hstream(bvh)), return 0.f);
+    std::vector<float> lim(2 * n);
+    std::fill(lim.begin(), lim.begin() + n, 0.0f);           // t > 0 (backend :594)
+    std::fill(lim.begin() + n, lim.end(), RT_MISS_T);
+    HIP_TRY(hipMemcpyAsync(w + 6 * n, lim.data(), 2 * n * 4, hipMemcpyHostToDevice, hstream(bvh)), return 0.f);
+    rtk::TraceArgs a;
+    a.n = numRays;
+    a.ox = w; a.oy = w + n; a.oz = w + 2 * n; a.dx = w + 3 * n; a.dy = w + 4 * n; a.dz = w + 5 * n;
+    a.tmin = w + 6 * n; a.tmax = w + 7 * n;
+    a.t = w + 8 * n;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0); hipEventCreate(&e1);
+    hipEventRecord(e0, hstream(bvh));
+    int rc = rtk::launch_trace(bvh, bvh->scratch, a, rtk::kModeClosest, false, false, false, hstream(bvh));
+    hipEventRecord(e1, hstream(bvh));
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0); hipEventDestroy(e1);
+    if (rc != 0) return 0.f;
+    if (outHitT) HIP_TRY(hipMemcpy(outHitT, a.t, n * 4, hipMemcpyDeviceToHost), return 0.f);
+    return ms > 0.f ? (float)numRays / (ms * 1000.f) : 0.f;
+}
+
+int cudaBVH_traceToGPUPtr(CudaBVH_t bvh, int width, int height, float camOx, float camOy, float camOz,
+                          void* d_outRGBA, int outFmt) {
+    rt::clear_error();
+    if (!check_handle(bvh, "cudaBVH_traceToGPUPtr")) return -1;
+    if (!d_outRGBA) { set_error("cudaBVH_traceToGPUPtr: NULL output"); return -1; }
+    return shade_to_device(bvh, bvh->scratch, hstream(bvh), width, height, camOx, camOy, camOz, d_outRGBA, outFmt);
+}
+
+int cudaBVH_traceToRGBA(CudaBVH_t bvh, int width, int height, float camOx, float camOy, float camOz,
+                        uint32_t* outRGBA_host) {
+    rt::clear_error();
+    if (!check_handle(bvh, "cudaBVH_traceToRGBA")) return -1;
+    if (!outRGBA_host) { set_error("cudaBVH_traceToRGBA: NULL output"); return -1; }
+    if (width <= 0 || height <= 0) { set_error("bad image size %dx%d", width, height); return -1; }
+    const size_t bytes = size_t(width) * height * 4;
+    void* d = work_buffer(bvh, bytes);
+    if (!d) return -1;
+    if (shade_to_device(bvh, bvh->scratch, hstream(bvh), width, height, camOx, camOy, camOz, d, 0) != 0) return -1;
+    HIP_TRY(hipMemcpyAsync(outRGBA_host, d, bytes, hipMemcpyDeviceToHost, hstream(bvh)), return -1);
+    HIP_TRY(hipStreamSynchronize(hstream(bvh)), return -1);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Background tracer (backend :1093-1336): one process-wide worker thread with
+// double-buffered host output, as in the reference. Calls are serialised.
+// ---------------------------------------------------------------------------
+namespace {
+struct AsyncTracer {
+    std::mutex mu;
+    std::condition_variable req, ready;
+    std::thread th;
+    bool running = false, hasRequest = false, hasResult = false;
+    int device = 0;
+    CudaBVH_t bvh = nullptr;
+    int w = 0, h = 0, fmt = 0;
+    float cx = 0, cy = 0, cz = 0;
+    void* gpuTarget = nullptr;
+    std::vector<uint8_t> host[2];
+    int readIdx = 0, writeIdx = 1;
+    uint64_t bufSize = 0;
+    void* dbuf = nullptr; size_t dbytes = 0;
+    hipStream_t stream = nullptr;
+    rt::TraceScratch scratch;            // the worker's own traversal scratch
+    std::mutex startMu;
+
+    void loop() {
+        hipSetDevice(device);
+        hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+        std::unique_lock<std::mutex> lk(mu);
+        while (running) {
+            req.wait(lk, [&] { return hasRequest || !running; });
+            if (!running) break;
+            CudaBVH_t b = bvh; int W = w, H = h, F = fmt; float X = cx, Y = cy, Z = cz;
+            void* tgt = gpuTarget; int wi = writeIdx; uint64_t sz = bufSize;
+            hasRequest = false;
+            lk.unlock();
+            if (tgt) {
+                shade_to_device(b, scratch, stream, W, H, X, Y, Z, tgt, F);
+                hipStreamSynchronize(stream);
+            } else {
+                if (dbytes < sz) { hipFree(dbuf); dbuf = nullptr; dbytes = 0; if (hipMalloc(&dbuf, sz) == hipSuccess) dbytes = sz; }
+                if (dbuf && shade_to_device(b, scratch, stream, W, H, X, Y, Z, dbuf, F) == 0) {
+                    hipMemcpyAsync(host[wi].data(), dbuf, sz, hipMemcpyDeviceToHost, stream);
+                    hipStreamSynchronize(stream);
+                }
+            }
+            lk.lock();
+            readIdx = wi; writeIdx = 1 - wi; hasResult = true;
+            ready.notify_all();
+        }
+        lk.unlock();
+        hipFree(dbuf); dbuf = nullptr; dbytes = 0;
+        rtk::free_scratch(scratch);
+        hipStreamDestroy(stream);
+    }
+    void start(uint64_t bytes) {
+        std::lock_guard<std::mutex> sg(startMu);
+        if (running) return;
+        hipGetDevice(&device);
+        bufSize = bytes;
+        host[0].assign(bytes, 0); host[1].assign(bytes, 0);
+        running = true;
+        th = std::thread([this] { loop(); });
+    }
+    void stop() {
+        std::lock_guard<std::mutex> sg(startMu);
+        { std::lock_guard<std::mutex> g(mu); if (!running) return; running = false; }
+        req.notify_all();
+        if (th.joinable()) th.join();
+        hasResult = false; hasRequest = false;
+    }
+    ~AsyncTracer() { stop(); }
+};
+AsyncTracer g_async;
+std::atomic<int> g_async_calls_host{0}, g_async_calls_gpu{0};
+}  // namespace
+
+int cudaBVH_traceToHostPtr(CudaBVH_t bvh, int width, int height, float camOx, float camOy, float camOz,
+                           void* h_outRGBA, int outFmt, uint64_t bufSize) {
+    rt::clear_error();
+    if (!check_handle(bvh, "cudaBVH_traceToHostPtr")) return -1;
+    const uint64_t bpp = outFmt == 3 ? 16 : (outFmt == 2 ? 8 : 4);
+    if (!h_outRGBA || width <= 0 || height <= 0 || bufSize < uint64_t(width) * height * bpp) {
+        set_error("cudaBVH_traceToHostPtr: bad arguments");
+        return -1;
+    }
+    const int call = ++g_async_calls_host;
+    g_async.start(bufSize);
+    std::unique_lock<std::mutex> lk(g_async.mu);
+    if (bufSize > g_async.bufSize) {
+        set_error("cudaBVH_traceToHostPtr: bufSize %llu exceeds the first call's %llu",
+                  (unsigned long long)bufSize, (unsigned long long)g_async.bufSize);
+        return -1;
+    }
+    g_async.bvh = bvh; g_async.w = width; g_async.h = height; g_async.fmt = outFmt;
+    g_async.cx = camOx; g_async.cy = camOy; g_async.cz = camOz; g_async.gpuTarget = nullptr;
+    g_async.hasRequest = true;
+    g_async.req.notify_all();
+    if (call == 1) g_async.ready.wait(lk, [] { return g_async.hasResult; });   // first frame blocks
+    if (g_async.hasResult) memcpy(h_outRGBA, g_async.host[g_async.readIdx].data(), bufSize);
+    return 0;
+}
+
+void cudaBVH_traceToGPUAsync(CudaBVH_t bvh, int width, int height, float camOx, float camOy, float camOz,
+                             void* d_outRGBA, int outFmt) {
+    rt::clear_error();
+    if (!check_handle(bvh, "cudaBVH_traceToGPUAsync") || !d_outRGBA) return;
+    const int call = ++g_async_calls_gpu;
+    const uint64_t bpp = outFmt == 3 ? 16 : (outFmt == 2 ? 8 : 4);
+    g_async.start(uint64_t(width) * height * bpp);
+    std::unique_lock<std::mutex> lk(g_async.mu);
+    g_async.bvh = bvh; g_async.w = width; g_async.h = height; g_async.fmt = outFmt;
+    g_async.cx = camOx; g_async.cy = camOy; g_async.cz = camOz; g_async.gpuTarget = d_outRGBA;
+    g_async.hasResult = (call == 1) ? false : g_async.hasResult;
+    g_async.hasRequest = true;
+    g_async.req.notify_all();
+    if (call == 1) g_async.ready.wait(lk, [] { return g_async.hasResult; });
+}
+
+void* cudaBVH_importBufferFd(int fd, uint64_t size) {
+    rt::clear_error();
+    hipExternalMemoryHandleDesc desc;
+    memset(&desc, 0, sizeof desc);
+    desc.type = hipExternalMemoryHandleTypeOpaqueFd;
+    desc.handle.fd = fd;
+    desc.size = size;
+    hipExternalMemory_t ext = nullptr;
+    hipError_t e = hipImportExternalMemory(&ext, &desc);
+    if (e != hipSuccess) { set_error("cannot import fd %d: %s", fd, hipGetErrorString(e)); return nullptr; }
+    hipExternalMemoryBufferDesc bd;
+    memset(&bd, 0, sizeof bd);
+    bd.offset = 0; bd.size = size;
+    void* p = nullptr;
+    e = hipExternalMemoryGetMappedBuffer(&p, ext, &bd);
+    if (e != hipSuccess) { set_error("cannot map imported fd %d: %s", fd, hipGetErrorString(e)); hipDestroyExternalMemory(ext); return nullptr; }
+    return p;
+}
+
+int cudaBVH_readGPUMemoryFd(int fd, uint64_t allocationSize, uint64_t offset, uint64_t size, void* dst) {
+    rt::clear_error();
+    if (!dst) { set_error("cudaBVH_readGPUMemoryFd: NULL destination"); return -1; }
+    hipExternalMemoryHandleDesc desc;
+    memset(&desc, 0, sizeof desc);
+    desc.type = hipExternalMemoryHandleTypeOpaqueFd;
+    desc.handle.fd = fd;
+    desc.size = allocationSize;
+    hipExternalMemory_t ext = nullptr;
+    hipError_t e = hipImportExternalMemory(&ext, &desc);
+    if (e != hipSuccess) { set_error("cannot import fd %d: %s", fd, hipGetErrorString(e)); return -1; }
+    hipExternalMemoryBufferDesc bd;
+    memset(&bd, 0, sizeof bd);
+    bd.offset = offset; bd.size = size;
+    void* p = nullptr;
+    e = hipExternalMemoryGetMappedBuffer(&p, ext, &bd);
+    if (e == hipSuccess) e = hipMemcpy(dst, p, size, hipMemcpyDeviceToHost);
+    if (p) hipFree(p);
+    hipDestroyExternalMemory(ext);
+    if (e != hipSuccess) { set_error("readGPUMemoryFd failed: %s", hipGetErrorString(e)); return -1; }
+    return 0;
+}
+
+void cudaBVH_getBounds(CudaBVH_t bvh, float* cx, float* cy, float* cz, float* extent) {
+    if (!bvh) { set_error("cudaBVH_getBounds: NULL handle"); return; }
+    if (cx) *cx = bvh->bboxCenter[0];
+    if (cy) *cy = bvh->bboxCenter[1];
+    if (cz) *cz = bvh->bboxCenter[2];
+    if (extent) *extent = bvh->bboxExtent;
+}
+
+int cudaBVH_getNumTris(CudaBVH_t bvh) { return bvh ? bvh->numOrdered : 0; }
+int cudaBVH_getNumBVH4Nodes(CudaBVH_t bvh) { return bvh ? bvh->numNodes4 : 0; }
+int cudaBVH_getNumCWBVHNodes(CudaBVH_t bvh) { return bvh ? bvh->numCWBVH : 0; }
+int cudaBVH_getDepth(CudaBVH_t bvh) { return bvh ? bvh->depth4 : 0; }
+
+const void* cudaBVH_getNodeData(CudaBVH_t bvh) { return bvh ? bvh->h_nodes4.data() : nullptr; }
+
+void cudaBVH_getTriData(CudaBVH_t bvh, const float* outPtrs[9]) {
+    if (!outPtrs) return;
+    for (int i = 0; i < 9; i++) outPtrs[i] = bvh ? bvh->h_soa[i].data() : nullptr;
+}
+
+int cudaBVH_getStacklessBVH2(CudaBVH_t bvh, uint32_t** outData) {
+    if (!outData) return 0;
+    if (!bvh || bvh->h_dfs.empty()) { *outData = nullptr; return 0; }
+    *outData = bvh->h_dfs.data();
+    return bvh->numDFS;
+}
+
+int cudaBVH_getPackedTris(CudaBVH_t bvh, float** outData) {
+    if (!outData) return 0;
+    if (!bvh || bvh->h_packed.empty()) { *outData = nullptr; return 0; }
+    *outData = bvh->h_packed.data();
+    return int(bvh->h_packed.size() / 4);
+}
+
+void cudaBVH_resetDevice(void) {
+    g_async.stop();
+    hipDeviceSynchronize();
+}
+
+CudaBVH_t cudaBVH_buildFromAABBs(const float* aabbs, int numAABBs) {
+    rt::clear_error();
+    if (!aabbs || numAABBs <= 0) { set_error("cudaBVH_buildFromAABBs: need aabbs and numAABBs > 0"); return nullptr; }
+    // One degenerate triangle per box whose bounds equal the box (backend :1549-1563).
+    std::vector<float> t(size_t(numAABBs) * 9);
+    for (int i = 0; i < numAABBs; i++) {
+        const float* b = aabbs + 6 * size_t(i);
+        float* d = &t[9 * size_t(i)];
+        d[0] = b[0]; d[1] = b[1]; d[2] = b[2];
+        d[3] = b[3]; d[4] = b[4]; d[5] = b[5];
+        d[6] = b[3]; d[7] = b[1]; d[8] = b[2];
+    }
+    return build_impl(t.data(), numAABBs, 4);
+}
+
+int cudaBVH_buildTLASFast(const float* aabbs, int numAABBs, uint32_t** outNodes, int** outOrdered) {
+    rt::clear_error();
+    if (outNodes) *outNodes = nullptr;
+    if (outOrdered) *outOrdered = nullptr;
+    if (!aabbs || numAABBs <= 0 || !outNodes) { set_error("cudaBVH_buildTLASFast: bad arguments"); return 0; }
+    std::vector<uint32_t> dfs;
+    std::vector<int> ord;
+    const int nn = rt::build_tlas_lbvh(aabbs, numAABBs, dfs, ord);
+    uint32_t* p = (uint32_t*)malloc(dfs.size() * sizeof(uint32_t));
+    if (!p) { set_error("out of host memory"); return 0; }
+    memcpy(p, dfs.data(), dfs.size() * sizeof(uint32_t));
+    *outNodes = p;
+    if (outOrdered) {
+        int* o = (int*)malloc(ord.size() * sizeof(int));
+        if (!o) { free(p); *outNodes = nullptr; set_error("out of host memory"); return 0; }
+        memcpy(o, ord.data(), ord.size() * sizeof(int));
+        *outOrdered = o;
+    }
+    return nn;
+}
+
+// ------------------------------- extensions -------------------------------
+
+int cudaBVH_setStream(CudaBVH_t bvh, void* s) {
+    if (!check_handle(bvh, "cudaBVH_setStream")) return -1;
+    bvh->stream = s;
+    return 0;
+}
+void* cudaBVH_getStream(CudaBVH_t bvh) { return bvh ? bvh->stream : nullptr; }
+const void* cudaBVH_deviceNodes(CudaBVH_t bvh) { return bvh ? bvh->d_nodes4 : nullptr; }
+const void* cudaBVH_deviceTris(CudaBVH_t bvh) { return bvh ? bvh->d_tris : nullptr; }
+
+static bool packet_ok(const CudaRayPacket* r) {
+    return r && r->ox && r->oy && r->oz && r->dx && r->dy && r->dz && r->tmin && r->tmax;
+}
+
+// Shared body of the packet entry points. Host pointers are staged through
+// the handle's work buffer.
+static int trace_packet(CudaBVH_t bvh, int n, const CudaRayPacket* rays, const CudaHitPacket* hits, uint8_t* occ,
+                        int* cn, int* ct, unsigned flags, int mode, const char* fn) {
+    rt::clear_error();
+    if (!check_handle(bvh, fn)) return -1;
+    if (n < 0 || !packet_ok(rays)) { set_error("%s: bad ray packet", fn); return -1; }
+    if (mode == rtk::kModeClosest && (!hits || !hits->t)) { set_error("%s: hit packet needs t", fn); return -1; }
+    if (mode == rtk::kModeAny && !occ) { set_error("%s: NULL occlusion output", fn); return -1; }
+    if (flags & CUDABVH_FLAG_SORT_RAYS) { set_error("%s: CUDABVH_FLAG_SORT_RAYS is not available in this build", fn); return -1; }
+    const unsigned isect = flags & CUDABVH_ISECT_MASK;
+    if (isect > 1) { set_error("%s: unknown intersection mode %u", fn, isect); return -1; }
+    if (n == 0) return 0;
+    hipStream_t s = hstream(bvh);
+    rtk::TraceArgs a;
+    a.n = n;
+    const bool host = (flags & CUDABVH_FLAG_HOST_POINTERS) != 0;
+    const size_t N = size_t(n);
+    if (!host) {
+        a.ox = rays->ox; a.oy = rays->oy; a.oz = rays->oz; a.dx = rays->dx; a.dy = rays->dy; a.dz = rays->dz;
+        a.tmin = rays->tmin; a.tmax = rays->tmax;
+        if (hits) { a.t = hits->t; a.id = hits->primId; a.u = hits->u; a.v = hits->v; }
+        a.occ = occ; a.cntNodes = cn; a.cntTris = ct;
+        int rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, isect == 1, false, cn != nullptr, s);
+        if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(s), return -1);
+        return rc;
+    }
+    // host pointers: 8 ray arrays + 4 hit arrays + 2 counters (+ occlusion bytes)
+    float* w = (float*)work_buffer(bvh, N * 14 * 4 + N);
+    if (!w) return -1;
+    const float* src[8] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+    for (int k = 0; k < 8; k++) HIP_TRY(hipMemcpyAsync(w + k * N, src[k], N * 4, hipMemcpyHostToDevice, s), return -1);
+    a.ox = w; a.oy = w + N; a.oz = w + 2 * N; a.dx = w + 3 * N; a.dy = w + 4 * N; a.dz = w + 5 * N;
+    a.tmin = w + 6 * N; a.tmax = w + 7 * N;
+    a.t = w + 8 * N; a.id = (int*)(w + 9 * N); a.u = w + 10 * N; a.v = w + 11 * N;
+    a.cntNodes = (int*)(w + 12 * N); a.cntTris = (int*)(w + 13 * N);
+    a.occ = (uint8_t*)(w + 14 * N);
+    int rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, isect == 1, false, cn != nullptr, s);
+    if (rc != 0) return rc;
+    if (mode == rtk::kModeClosest) {
+        HIP_TRY(hipMemcpyAsync(hits->t, a.t, N * 4, hipMemcpyDeviceToHost, s), return -1);
+        if (hits->primId) HIP_TRY(hipMemcpyAsync(hits->primId, a.id, N * 4, hipMemcpyDeviceToHost, s), return -1);
+        if (hits->u) HIP_TRY(hipMemcpyAsync(hits->u, a.u, N * 4, hipMemcpyDeviceToHost, s), return -1);
+        if (hits->v) HIP_TRY(hipMemcpyAsync(hits->v, a.v, N * 4, hipMemcpyDeviceToHost, s), return -1);
+    } else {
+        HIP_TRY(hipMemcpyAsync(occ, a.occ, N, hipMemcpyDeviceToHost, s), return -1);
+    }
+    if (cn) HIP_TRY(hipMemcpyAsync(cn, a.cntNodes, N * 4, hipMemcpyDeviceToHost, s), return -1);
+    if (ct) HIP_TRY(hipMemcpyAsync(ct, a.cntTris, N * 4, hipMemcpyDeviceToHost, s), return -1);
+    HIP_TRY(hipStreamSynchronize(s), return -1);
+    return 0;
+}
+
+int cudaBVH_traceClosest(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays, const CudaHitPacket* hits, unsigned flags) {
+    return trace_packet(bvh, numRays, rays, hits, nullptr, nullptr, nullptr, flags, rtk::kModeClosest, "cudaBVH_traceClosest");
+}
+int cudaBVH_traceAny(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays, uint8_t* occluded, unsigned flags) {
+    return trace_packet(bvh, numRays, rays, nullptr, occluded, nullptr, nullptr, flags, rtk::kModeAny, "cudaBVH_traceAny");
+}
+int cudaBVH_traceClosestCounted(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays, const CudaHitPacket* hits,
+                                int* nodeVisits, int* triTests, unsigned flags) {
+    if (!nodeVisits || !triTests) { set_error("cudaBVH_traceClosestCounted: NULL counters"); return -1; }
+    return trace_packet(bvh, numRays, rays, hits, nullptr, nodeVisits, triTests, flags, rtk::kModeClosest,
+                        "cudaBVH_traceClosestCounted");
+}
+int cudaBVH_traceAnyCounted(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays, uint8_t* occluded, int* nodeVisits,
+                            int* triTests, unsigned flags) {
+    if (!nodeVisits || !triTests) { set_error("cudaBVH_traceAnyCounted: NULL counters"); return -1; }
+    return trace_packet(bvh, numRays, rays, nullptr, occluded, nodeVisits, triTests, flags, rtk::kModeAny,
+                        "cudaBVH_traceAnyCounted");
+}
+
+int cudaBVH_genPinholeRays(int width, int height, const CudaCamera* cam, const CudaRayPacket* out, void* stream) {
+    rt::clear_error();
+    if (!cam || !packet_ok(out) || width <= 0 || height <= 0 || (long long)width * height > (1ll << 30)) {
+        set_error("cudaBVH_genPinholeRays: bad arguments");
+        return -1;
+    }
+    return rtk::launch_gen_pinhole(to_cam(*cam), width, height, *out, (hipStream_t)stream);
+}
+
+int cudaBVH_tracePinholeClosest(CudaBVH_t bvh, int width, int height, const CudaCamera* cam, const CudaHitPacket* hits,
+                                unsigned flags) {
+    rt::clear_error();
+    if (!check_handle(bvh, "cudaBVH_tracePinholeClosest")) return -1;
+    if (!cam || !hits || !hits->t || width <= 0 || height <= 0 || (long long)width * height > (1ll << 30)) {
+        set_error("cudaBVH_tracePinholeClosest: bad arguments");
+        return -1;
+    }
+    const unsigned isect = flags & CUDABVH_ISECT_MASK;
+    if (isect > 1 || (flags & (CUDABVH_FLAG_HOST_POINTERS | CUDABVH_FLAG_SORT_RAYS))) {
+        set_error("cudaBVH_tracePinholeClosest: unsupported flags 0x%x", flags);
+        return -1;
+    }
+    rtk::TraceArgs a;
+    a.n = width * height; a.W = width; a.H = height;
+    a.cam = to_cam(*cam);
+    a.t = hits->t; a.id = hits->primId; a.u = hits->u; a.v = hits->v;
+    int rc = rtk::launch_trace(bvh, bvh->scratch, a, rtk::kModeClosest, isect == 1, true, false, hstream(bvh));
+    if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(hstream(bvh)), return -1);
+    return rc;
+}
+
+int cudaBVH_genBounceRays(CudaBVH_t bvh, int width, int height, const CudaCamera* cam, const CudaHitPacket* primary,
+                          uint32_t seed, const float light[3], const CudaRayPacket* diffuseOut,
+                          const CudaRayPacket* shadowOut, int* srcIndex, int* d_count, unsigned flags) {
+    rt::clear_error();
+    if (!check_handle(bvh, "cudaBVH_genBounceRays")) return -1;
+    if (!cam || !primary || !primary->t || !primary->primId || !light || !packet_ok(diffuseOut) ||
+        !packet_ok(shadowOut) || !srcIndex || !d_count || width <= 0 || height <= 0 ||
+        (long long)width * height > (1ll << 30)) {
+        set_error("cudaBVH_genBounceRays: bad arguments");
+        return -1;
+    }
+    int rc = rtk::launch_bounce(bvh, bvh->scratch, to_cam(*cam), width, height, primary->t, primary->primId, seed,
+                                light, *diffuseOut, *shadowOut, srcIndex, d_count, hstream(bvh));
+    if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(hstream(bvh)), return -1);
+    return rc;
+}
+
+static int trace_queued(CudaBVH_t bvh, int capacity, const int* d_count, const CudaRayPacket* rays,
+                        const CudaHitPacket* hits, uint8_t* occ, unsigned flags, int mode, const char* fn) {
+    rt::clear_error();
+    if (!check_handle(bvh, fn)) return -1;
+    if (capacity < 0 || !d_count || !packet_ok(rays) || (mode == rtk::kModeClosest && (!hits || !hits->t)) ||
+        (mode == rtk::kModeAny && !occ)) {
+        set_error("%s: bad arguments", fn);
+        return -1;
+    }
+    const unsigned isect = flags & CUDABVH_ISECT_MASK;
+    if (isect > 1 || (flags & (CUDABVH_FLAG_HOST_POINTERS | CUDABVH_FLAG_SORT_RAYS))) {
+        set_error("%s: unsupported flags 0x%x", fn, flags);
+        return -1;
+    }
+    if (capacity == 0) return 0;
+    rtk::TraceArgs a;
+    a.n = capacity; a.nDev = d_count;
+    a.ox = rays->ox; a.oy = rays->oy; a.oz = rays->oz; a.dx = rays->dx; a.dy = rays->dy; a.dz = rays->dz;
+    a.tmin = rays->tmin; a.tmax = rays->tmax;
+    if (hits) { a.t = hits->t; a.id = hits->primId; a.u = hits->u; a.v = hits->v; }
+    a.occ = occ;
+    int rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, isect == 1, false, false, hstream(bvh));
+    if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(hstream(bvh)), return -1);
+    return rc;
+}
+
+int cudaBVH_traceClosestQueued(CudaBVH_t bvh, int capacity, const int* d_count, const CudaRayPacket* rays,
+                               const CudaHitPacket* hits, unsigned flags) {
+    return trace_queued(bvh, capacity, d_count, rays, hits, nullptr, flags, rtk::kModeClosest,
+                        "cudaBVH_traceClosestQueued");
+}
+
+int cudaBVH_traceAnyQueued(CudaBVH_t bvh, int capacity, const int* d_count, const CudaRayPacket* rays,
+                           uint8_t* occluded, unsigned flags) {
+    return trace_queued(bvh, capacity, d_count, rays, nullptr, occluded, flags, rtk::kModeAny,
+                        "cudaBVH_traceAnyQueued");
+}
+
+}  // extern "C"

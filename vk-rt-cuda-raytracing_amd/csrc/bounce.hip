@@ -1,0 +1,188 @@
+// bounce.hip — secondary rays from primary hits: one cosine-weighted diffuse
+// bounce and one shadow ray toward a point light per hit pixel, compacted in
+// pixel order (stable) with wave64 scans.
+//
+// Three launches, no host round trip: count hits per 2048-pixel tile,
+// exclusive-scan the tile counts (which also writes the total to device
+// memory, where the queued traces read it), then emit rays at their compacted
+// positions. The diffuse direction follows the reference's diffuse-ray
+// generator (cuda_rt_v37.cu:514-540: per-ray seeded RNG, hemisphere around
+// the surface normal) with PCG32 and Malley's method: a uniform point of the
+// unit disk by rejection, lifted to the hemisphere, in the orthonormal basis
+// of Duff et al. 2017. Only IEEE +,-,*,/,sqrt are used, so a host
+// restatement reproduces every ray bit for bit.
+#include "rt_device.h"
+#include "rt_scan.h"
+#include "trace_launch.h"
+
+namespace rtk {
+
+constexpr int kTileThreads = 256;
+constexpr int kTileItems = 8;
+constexpr int kTile = kTileThreads * kTileItems;
+
+struct Pcg32 {
+    uint64_t state, inc;
+    __device__ __forceinline__ uint32_t next() {
+        const uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    __device__ __forceinline__ void seed(uint64_t initstate, uint64_t initseq) {
+        state = 0u;
+        inc = (initseq << 1u) | 1u;
+        next();
+        state += initstate;
+        next();
+    }
+    __device__ __forceinline__ float uniform() { return (float)(next() >> 8) * 0x1p-24f; }
+};
+
+struct BounceArgs {
+    rtd::Cam cam;
+    int W, H, n;
+    const float* hitT;
+    const int* hitId;
+    const int* leafOfPrim;
+    const float4* tris;
+    uint32_t seed;
+    float lx, ly, lz;
+    float *dox, *doy, *doz, *ddx, *ddy, *ddz, *dtmin, *dtmax;
+    float *sox, *soy, *soz, *sdx, *sdy, *sdz, *stmin, *stmax;
+    int* src;
+    int* count;
+    int* tileCounts;
+};
+
+__global__ __launch_bounds__(kTileThreads) void k_count_hits(BounceArgs a) {
+    __shared__ int lds[kTileThreads / 64 + 1];
+    const int base = blockIdx.x * kTile + threadIdx.x * kTileItems;
+    int c = 0;
+    for (int k = 0; k < kTileItems; k++) {
+        const int i = base + k;
+        if (i < a.n && a.hitId[i] >= 0) c++;
+    }
+    int total;
+    rts::block_exclusive_scan<kTileThreads>(c, lds, &total);
+    if (threadIdx.x == 0) a.tileCounts[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(int* counts, int nb, int* total) {
+    __shared__ int lds[1024 / 64 + 1];
+    int carry = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = i < nb ? counts[i] : 0;
+        int tot;
+        const int ex = rts::block_exclusive_scan<1024>(v, lds, &tot);
+        if (i < nb) counts[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__device__ __forceinline__ void emit(const BounceArgs& a, int i, int pos) {
+    const int py = i / a.W, px = i - py * a.W;
+    rtd::Ray r;
+    rtd::pinhole_ray(a.cam, px, py, a.W, a.H, r);
+    const float t = a.hitT[i];
+    const int li = a.leafOfPrim[a.hitId[i]];
+    const float4 p0 = a.tris[3 * li], p1 = a.tris[3 * li + 1], p2 = a.tris[3 * li + 2];
+    const float hx = r.ox + t * r.dx, hy = r.oy + t * r.dy, hz = r.oz + t * r.dz;
+    const float e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
+    const float e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
+    float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    const float nl = 1.0f / sqrtf((nx * nx + ny * ny) + nz * nz);
+    nx = nx * nl; ny = ny * nl; nz = nz * nl;
+    if ((nx * r.dx + ny * r.dy) + nz * r.dz > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
+
+    Pcg32 rng;
+    rng.seed((uint64_t)a.seed, (uint64_t)i);
+    float x = 0.0f, y = 0.0f, s = 0.0f;
+    bool ok = false;
+    for (int k = 0; k < 64 && !ok; k++) {                // bounded rejection
+        x = 2.0f * rng.uniform() - 1.0f;
+        y = 2.0f * rng.uniform() - 1.0f;
+        s = x * x + y * y;
+        ok = s < 1.0f;
+    }
+    if (!ok) { x = 0.0f; y = 0.0f; s = 0.0f; }
+    const float z = sqrtf(1.0f - s);
+    const float sg = copysignf(1.0f, nz);
+    const float ca = -1.0f / (sg + nz);
+    const float cb = nx * ny * ca;
+    const float t1x = 1.0f + sg * nx * nx * ca, t1y = sg * cb, t1z = -sg * nx;
+    const float t2x = cb, t2y = sg + ny * ny * ca, t2z = -ny;
+    float dx = (x * t1x + y * t2x) + z * nx;
+    float dy = (x * t1y + y * t2y) + z * ny;
+    float dz = (x * t1z + y * t2z) + z * nz;
+    const float dl = 1.0f / sqrtf((dx * dx + dy * dy) + dz * dz);
+    dx = dx * dl; dy = dy * dl; dz = dz * dl;
+    a.dox[pos] = hx; a.doy[pos] = hy; a.doz[pos] = hz;
+    a.ddx[pos] = dx; a.ddy[pos] = dy; a.ddz[pos] = dz;
+    a.dtmin[pos] = 1e-3f; a.dtmax[pos] = RT_MISS_T;
+
+    const float lx = a.lx - hx, ly = a.ly - hy, lz = a.lz - hz;
+    const float dist = sqrtf((lx * lx + ly * ly) + lz * lz);
+    const float il = 1.0f / dist;
+    a.sox[pos] = hx; a.soy[pos] = hy; a.soz[pos] = hz;
+    a.sdx[pos] = lx * il; a.sdy[pos] = ly * il; a.sdz[pos] = lz * il;
+    a.stmin[pos] = 1e-3f; a.stmax[pos] = dist;
+    a.src[pos] = i;
+}
+
+__global__ __launch_bounds__(kTileThreads) void k_emit(BounceArgs a) {
+    __shared__ int lds[kTileThreads / 64 + 1];
+    const int base = blockIdx.x * kTile + threadIdx.x * kTileItems;
+    int c = 0;
+    for (int k = 0; k < kTileItems; k++) {
+        const int i = base + k;
+        if (i < a.n && a.hitId[i] >= 0) c++;
+    }
+    int total;
+    int pos = a.tileCounts[blockIdx.x] + rts::block_exclusive_scan<kTileThreads>(c, lds, &total);
+    for (int k = 0; k < kTileItems; k++) {
+        const int i = base + k;
+        if (i < a.n && a.hitId[i] >= 0) emit(a, i, pos++);
+    }
+}
+
+int launch_bounce(const CudaBVHHandle* h, rt::TraceScratch& sc, const rtd::Cam& cam, int W, int H,
+                  const float* hitT, const int* hitId, uint32_t seed, const float light[3], const CudaRayPacket& d,
+                  const CudaRayPacket& s, int* src, int* count, hipStream_t st) {
+    const int n = W * H;
+    const int tiles = (n + kTile - 1) / kTile;
+    if (alloc_scratch(sc) != 0) return -1;
+    if ((size_t)tiles > sc.tileCap) {
+        hipFree(sc.tiles);
+        sc.tiles = nullptr; sc.tileCap = 0;
+        if (hipMalloc((void**)&sc.tiles, (size_t)tiles * sizeof(int)) != hipSuccess) {
+            rt::set_error("cannot allocate %d tile counters", tiles);
+            return -1;
+        }
+        sc.tileCap = (size_t)tiles;
+    }
+    BounceArgs a;
+    a.cam = cam; a.W = W; a.H = H; a.n = n;
+    a.hitT = hitT; a.hitId = hitId;
+    a.leafOfPrim = h->d_leafOfPrim;
+    a.tris = (const float4*)h->d_tris;
+    a.seed = seed; a.lx = light[0]; a.ly = light[1]; a.lz = light[2];
+    a.dox = (float*)d.ox; a.doy = (float*)d.oy; a.doz = (float*)d.oz;
+    a.ddx = (float*)d.dx; a.ddy = (float*)d.dy; a.ddz = (float*)d.dz;
+    a.dtmin = (float*)d.tmin; a.dtmax = (float*)d.tmax;
+    a.sox = (float*)s.ox; a.soy = (float*)s.oy; a.soz = (float*)s.oz;
+    a.sdx = (float*)s.dx; a.sdy = (float*)s.dy; a.sdz = (float*)s.dz;
+    a.stmin = (float*)s.tmin; a.stmax = (float*)s.tmax;
+    a.src = src; a.count = count; a.tileCounts = sc.tiles;
+    hipLaunchKernelGGL(k_count_hits, dim3(tiles), dim3(kTileThreads), 0, st, a);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, sc.tiles, tiles, count);
+    hipLaunchKernelGGL(k_emit, dim3(tiles), dim3(kTileThreads), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { rt::set_error("bounce launch failed: %s", hipGetErrorString(e)); return -1; }
+    return 0;
+}
+
+}  // namespace rtk

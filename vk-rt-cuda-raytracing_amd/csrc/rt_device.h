@@ -1,0 +1,167 @@
+// rt_device.h — device-side ray/box/triangle arithmetic for gfx950.
+//
+// Every FP32 expression here is written in the exact operation order of the
+// CPU oracle and compiled with -ffp-contract=off, IEEE division and square
+// root and no denormal flushing, so hit IDs, t, u, v and visit counters are
+// bit-identical to the oracle's on the same BVH and rays.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+
+namespace rtd {
+
+// 1 + 2*gamma(3) rounded to float: far-distance inflation of the robust slab
+// test (T. Ize, "Robust BVH Ray Traversal", JCGT 2013).
+#define RT_ROBUST_TFAR (1.0f + 3.0f * 0x1p-23f)
+#define RT_MISS_T 1e30f
+
+struct Ray {
+    float ox, oy, oz, dx, dy, dz, tmin, tmax;
+    float ix, iy, iz;          // 1/d, IEEE (infinite for a zero or tiny component)
+    float Sx, Sy, Sz;          // watertight shear constants
+    int kx, ky, kz;            // watertight axis permutation
+};
+
+// A ray with an infinite reciprocal component can meet (b - o) * inv =
+// 0 * inf = NaN when its origin lies exactly on a box plane.
+__device__ __forceinline__ bool ray_flat(const Ray& r) {
+    return __builtin_isinf(r.ix) || __builtin_isinf(r.iy) || __builtin_isinf(r.iz);
+}
+
+__device__ __forceinline__ float sel3(int k, float a, float b, float c) {
+    return k == 0 ? a : (k == 1 ? b : c);
+}
+
+// Per-ray set-up: reciprocal direction for the slab test, and the dominant
+// axis / shear of the watertight test (Woop, Benthin, Wald, JCGT 2013).
+__device__ __forceinline__ void ray_setup(Ray& r) {
+    r.ix = 1.0f / r.dx;
+    r.iy = 1.0f / r.dy;
+    r.iz = 1.0f / r.dz;
+    float ax = fabsf(r.dx), ay = fabsf(r.dy), az = fabsf(r.dz);
+    int kz = 0;
+    if (ay > ax) kz = 1;
+    if (az > (kz == 0 ? ax : ay)) kz = 2;
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    float dkz = sel3(kz, r.dx, r.dy, r.dz);
+    if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.Sx = sel3(kx, r.dx, r.dy, r.dz) / dkz;
+    r.Sy = sel3(ky, r.dx, r.dy, r.dz) / dkz;
+    r.Sz = 1.0f / dkz;
+}
+
+// Watertight ray-triangle test. Returns true with t, u (weight of v1) and v
+// (weight of v2) when the ray's line crosses the triangle with a non-zero
+// determinant; the caller applies (tmin, tmax) and the closest-hit rule.
+__device__ __forceinline__ bool isect_watertight(const Ray& r, float4 p0, float4 p1, float4 p2,
+                                                 float& t, float& u, float& v) {
+    float A0 = p0.x - r.ox, A1 = p0.y - r.oy, A2 = p0.z - r.oz;
+    float B0 = p1.x - r.ox, B1 = p1.y - r.oy, B2 = p1.z - r.oz;
+    float C0 = p2.x - r.ox, C1 = p2.y - r.oy, C2 = p2.z - r.oz;
+    float Akz = sel3(r.kz, A0, A1, A2), Bkz = sel3(r.kz, B0, B1, B2), Ckz = sel3(r.kz, C0, C1, C2);
+    float Ax = sel3(r.kx, A0, A1, A2) - r.Sx * Akz, Ay = sel3(r.ky, A0, A1, A2) - r.Sy * Akz;
+    float Bx = sel3(r.kx, B0, B1, B2) - r.Sx * Bkz, By = sel3(r.ky, B0, B1, B2) - r.Sy * Bkz;
+    float Cx = sel3(r.kx, C0, C1, C2) - r.Sx * Ckz, Cy = sel3(r.ky, C0, C1, C2) - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {   // FP64 fallback on an edge (paper, sec. 3.2)
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    float rcp = 1.0f / det;
+    t = T * rcp; u = V * rcp; v = W * rcp;
+    return true;
+}
+
+// Moller-Trumbore with the reference's thresholds (layer/cuda_bvh_backend.cu
+// :145-156), IEEE division in place of the fast-math reciprocal.
+__device__ __forceinline__ bool isect_mt(const Ray& r, float4 p0, float4 p1, float4 p2,
+                                         float& t, float& u, float& v) {
+    float e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
+    float e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
+    float ppx = r.dy * e2z - r.dz * e2y, ppy = r.dz * e2x - r.dx * e2z, ppz = r.dx * e2y - r.dy * e2x;
+    float det = e1x * ppx + e1y * ppy + e1z * ppz;
+    if (fabsf(det) < 1e-12f) return false;
+    float inv = 1.f / det;
+    float tx = r.ox - p0.x, ty = r.oy - p0.y, tz = r.oz - p0.z;
+    float uu = inv * (tx * ppx + ty * ppy + tz * ppz);
+    if (uu < 0.f || uu > 1.f) return false;
+    float qx = ty * e1z - tz * e1y, qy = tz * e1x - tx * e1z, qz = tx * e1y - ty * e1x;
+    float vv = inv * (r.dx * qx + r.dy * qy + r.dz * qz);
+    if (vv < 0.f || uu + vv > 1.f) return false;
+    t = inv * (e2x * qx + e2y * qy + e2z * qz); u = uu; v = vv;
+    return true;
+}
+
+template <bool MT>
+__device__ __forceinline__ bool isect(const Ray& r, float4 p0, float4 p1, float4 p2, float& t, float& u, float& v) {
+    if constexpr (MT) return isect_mt(r, p0, p1, p2, t, u, v);
+    else return isect_watertight(r, p0, p1, p2, t, u, v);
+}
+
+__device__ __forceinline__ float h_lo(uint32_t w) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu));
+}
+__device__ __forceinline__ float h_hi(uint32_t w) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+}
+
+// Slab test of child slot C of a 64-byte BVH4 node (qx, qy, qz hold the
+// x/y/z halves: mins in slots 0-3, maxes in 4-7). Robust far distance. FLAT
+// (chosen per wave when any lane's ray is flat) resolves the 0*inf NaN so
+// that a parallel axis whose plane holds the origin does not constrain the
+// interval; for other rays both variants give identical results. True when
+// the box may hold a hit in [tmin, limit].
+template <int C, bool FLAT>
+__device__ __forceinline__ bool slab_child(const Ray& r, const uint4& qx, const uint4& qy, const uint4& qz,
+                                           float limit, float& tnear) {
+    const uint32_t wxl = (C < 2) ? qx.x : qx.y, wxh = (C < 2) ? qx.z : qx.w;
+    const uint32_t wyl = (C < 2) ? qy.x : qy.y, wyh = (C < 2) ? qy.z : qy.w;
+    const uint32_t wzl = (C < 2) ? qz.x : qz.y, wzh = (C < 2) ? qz.z : qz.w;
+    float x0 = (C & 1) ? h_hi(wxl) : h_lo(wxl), x1 = (C & 1) ? h_hi(wxh) : h_lo(wxh);
+    float y0 = (C & 1) ? h_hi(wyl) : h_lo(wyl), y1 = (C & 1) ? h_hi(wyh) : h_lo(wyh);
+    float z0 = (C & 1) ? h_hi(wzl) : h_lo(wzl), z1 = (C & 1) ? h_hi(wzh) : h_lo(wzh);
+    float t1x = (x0 - r.ox) * r.ix, t2x = (x1 - r.ox) * r.ix;
+    float t1y = (y0 - r.oy) * r.iy, t2y = (y1 - r.oy) * r.iy;
+    float t1z = (z0 - r.oz) * r.iz, t2z = (z1 - r.oz) * r.iz;
+    if constexpr (FLAT) {
+        t1x = t1x != t1x ? -r.ix : t1x; t2x = t2x != t2x ? r.ix : t2x;
+        t1y = t1y != t1y ? -r.iy : t1y; t2y = t2y != t2y ? r.iy : t2y;
+        t1z = t1z != t1z ? -r.iz : t1z; t2z = t2z != t2z ? r.iz : t2z;
+    }
+    float tN = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
+    float tF = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z)) * RT_ROBUST_TFAR;
+    tnear = tN;
+    return (tN <= tF) && (tF >= r.tmin) && (tN <= limit);
+}
+
+// Pinhole ray for pixel (px, py) of a W x H image; the same sequence as the
+// oracle's generator: sx = ((px+0.5)/W)*2-1, d = f + (sx*sX)*r + (sy*sY)*u,
+// d *= 1/sqrt(d.d).
+struct Cam { float o[3], f[3], r[3], u[3], sx, sy; };
+
+__device__ __forceinline__ void pinhole_ray(const Cam& c, int px, int py, int W, int H, Ray& r) {
+    float sx = ((float)px + 0.5f) / (float)W * 2.0f - 1.0f;
+    float sy = ((float)py + 0.5f) / (float)H * 2.0f - 1.0f;
+    float a = sx * c.sx, b = sy * c.sy;
+    float dx = (c.f[0] + a * c.r[0]) + b * c.u[0];
+    float dy = (c.f[1] + a * c.r[1]) + b * c.u[1];
+    float dz = (c.f[2] + a * c.r[2]) + b * c.u[2];
+    float inv = 1.0f / sqrtf((dx * dx + dy * dy) + dz * dz);
+    r.ox = c.o[0]; r.oy = c.o[1]; r.oz = c.o[2];
+    r.dx = dx * inv; r.dy = dy * inv; r.dz = dz * inv;
+    r.tmin = 0.0f; r.tmax = RT_MISS_T;
+}
+
+}  // namespace rtd

@@ -1,0 +1,105 @@
+// rt_internal.h — internal structures of libcudabvh_hip.so (host side).
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/cudabvh.h"
+
+namespace rt {
+
+// Records the failure message for cudaBVH_lastError() on this thread and
+// prints it with the reference's "[CudaBVH]" prefix (backend :25).
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void clear_error();
+bool verbose();
+
+// ---------------------------------------------------------------------------
+// Host-side BVH produced by the builder (all arrays in leaf / pre-order).
+// ---------------------------------------------------------------------------
+struct Box { float mn[3], mx[3]; };
+struct Node2 { Box box; int left, right, triStart, triCount; };
+
+struct HostBVH {
+    int numTris = 0;
+    int maxLeaf = 4;
+    std::vector<Node2> nodes2;          // BVH2, pre-order (DFS) numbering
+    std::vector<int> ordered;           // leaf-order position -> original index
+    std::vector<uint8_t> nodes4;        // BVH4, 64 B per node, pre-order
+    int numNodes4 = 0;
+    int depth4 = 0;                     // node levels on the longest path
+};
+
+// BVH4 node format shared with the reference (backend :31-34).
+constexpr int kNode4Bytes = 64;
+constexpr int kMaxLeaf = 8;             // leaf encoding holds count-1 in 3 bits
+
+// Leaf child encoding (backend :393).
+inline int32_t encode_leaf(int first, int count) { return -((first << 3) | (count - 1)) - 2; }
+
+// Host FP16 conversions (IEEE binary16, round-to-nearest-even plus the
+// outward-rounded variants used for child bounds).
+uint16_t f2h_rne(float x);
+uint16_t f2h_down(float x);
+uint16_t f2h_up(float x);
+float h2f(uint16_t h);
+
+// Collapses a pre-order BVH2 into BVH4 nodes (backend :391-418 rule: every
+// internal child is opened into its two children), FP16 bounds rounded outward.
+void collapse_bvh4(HostBVH& b);
+
+// Stackless DFS BVH2 export (backend :456-522): 8 uint32 per node.
+void export_dfs(const std::vector<Node2>& nodes, std::vector<uint32_t>& out);
+
+// LBVH over AABBs for the TLAS fast path (backend :257-389, :1620-1698).
+int build_tlas_lbvh(const float* aabbs, int n, std::vector<uint32_t>& dfs, std::vector<int>& ordered);
+
+uint32_t morton3d(float x, float y, float z);
+
+// Device scratch of one traversal context: the persistent-kernel work
+// counter, a status word (bit 0: stack overflow, unreachable by
+// construction) and the per-lane stack spill region (grown on demand).
+struct TraceScratch {
+    void* spill = nullptr;
+    size_t spillBytes = 0;
+    uint32_t* counter = nullptr;
+    int* status = nullptr;
+    int* tiles = nullptr;               // per-tile counts of the compaction scans
+    size_t tileCap = 0;
+};
+
+}  // namespace rt
+
+// ---------------------------------------------------------------------------
+// The opaque handle. All state is per handle (the reference keeps the top
+// nodes and ray counters in process-global device symbols, backend :41-44).
+// ---------------------------------------------------------------------------
+struct CudaBVHHandle {
+    int numTris = 0;                    // input triangles
+    int numOrdered = 0;
+    int numNodes4 = 0;
+    int depth4 = 0;
+    int numCWBVH = 0;
+
+    // Device copies
+    void* d_nodes4 = nullptr;           // numNodes4 * 64 B
+    void* d_tris = nullptr;             // numOrdered * 48 B (3 x float4)
+    int* d_leafOfPrim = nullptr;        // original index -> leaf-order position
+
+    // Host copies for the Vulkan-side getters (borrowed by callers)
+    std::vector<uint8_t> h_nodes4;
+    std::vector<float> h_soa[9];
+    std::vector<uint32_t> h_dfs;
+    std::vector<float> h_packed;
+    std::vector<int> h_ordered;
+    int numDFS = 0;
+
+    float bboxMin[3], bboxMax[3], bboxCenter[3], bboxExtent = 0.f;
+
+    void* stream = nullptr;             // hipStream_t, borrowed (NULL stream by default)
+
+    rt::TraceScratch scratch;           // traversal scratch of calls on this handle
+    void* d_work = nullptr;   size_t workBytes = 0;   // staging for host-pointer calls
+};

@@ -1,0 +1,368 @@
+// trace.hip — BVH4 traversal kernels for gfx950 (closest hit, any hit,
+// fused pinhole generation, shading) and their launchers.
+//
+// Execution model: persistent wave64 workers. Each wave takes 64 consecutive
+// rays with one atomicAdd, traces them one ray per lane, and repeats until the
+// ray counter passes the end. Each lane owns a traversal stack whose first
+// LDS_DEPTH entries live in LDS (laid out [depth][thread] so a wave's lanes hit
+// consecutive banks) and whose deeper entries spill to a per-lane HBM region
+// sized from the BVH depth, so no entry is ever dropped (the reference's
+// 16-entry register stack drops children on overflow, backend :179).
+//
+// Traversal order is the reference's (backend :104-183): children tested
+// against the ray, kept when the robust slab test passes and tnear <= the
+// current closest t, sorted near-first with the 5-compare network of :177 and
+// pushed far-to-near; entries popped later are re-checked against the closest
+// t. Any-hit pushes children unsorted in slot order (v10_pipeline.cu:241-244).
+#include "rt_device.h"
+#include "rt_internal.h"
+#include "trace_launch.h"
+
+namespace rtk {
+
+using rtd::Ray;
+
+__global__ void k_reset_counter(uint32_t* c) { c[0] = 0u; }
+
+// Face-normal Lambert + rim shading of a primary hit, sky gradient on a miss
+// (backend :923-984; IEEE 1/sqrt and an explicit cube in place of rsqrtf and
+// powf).
+__device__ __forceinline__ void shade_pixel(const TraceArgs& a, const Ray& r, int ri, int py, bool hit, int ti) {
+    float rf, gf, bf;
+    if (!hit) {
+        const float vv = (float)py / (float)a.H;
+        rf = (20.f + 10.f * (1.f - vv)) / 255.f;
+        gf = (30.f + 20.f * (1.f - vv)) / 255.f;
+        bf = (60.f + 100.f * vv) / 255.f;
+    } else {
+        const float4 p0 = a.tris[3 * ti], p1 = a.tris[3 * ti + 1], p2 = a.tris[3 * ti + 2];
+        const float e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
+        const float e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
+        float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+        const float nl = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz + 1e-20f);
+        nx *= nl; ny *= nl; nz *= nl;
+        if (nx * r.dx + ny * r.dy + nz * r.dz > 0.f) { nx = -nx; ny = -ny; nz = -nz; }
+        float lx = 0.5f, ly = 0.7f, lz = 0.5f;
+        const float ll = 1.0f / sqrtf(lx * lx + ly * ly + lz * lz);
+        lx *= ll; ly *= ll; lz *= ll;
+        const float ndotl = fmaxf(nx * lx + ny * ly + nz * lz, 0.f);
+        const float ndotv = fabsf(nx * (-r.dx) + ny * (-r.dy) + nz * (-r.dz));
+        const float k = 1.f - ndotv;
+        const float shade = fminf(0.15f + 0.65f * ndotl + 0.2f * (k * k * k), 1.f);
+        rf = shade * 0.92f + 0.08f;
+        gf = shade * 0.88f + 0.07f;
+        bf = shade * 0.82f + 0.06f;
+    }
+    if (a.fmt == 3) {
+        reinterpret_cast<float4*>(a.img)[ri] = make_float4(rf, gf, bf, 1.0f);
+    } else if (a.fmt == 2) {
+        _Float16* o = reinterpret_cast<_Float16*>(a.img) + 4 * (size_t)ri;
+        o[0] = (_Float16)rf; o[1] = (_Float16)gf; o[2] = (_Float16)bf; o[3] = (_Float16)1.0f;
+    } else {
+        const uint32_t R = (uint32_t)(uint8_t)(rf * 255.f), G = (uint32_t)(uint8_t)(gf * 255.f),
+                       B = (uint32_t)(uint8_t)(bf * 255.f);
+        reinterpret_cast<uint32_t*>(a.img)[ri] =
+            a.fmt == 1 ? (255u << 24) | (R << 16) | (G << 8) | B : (255u << 24) | (B << 16) | (G << 8) | R;
+    }
+}
+
+// Per-lane traversal result.
+struct Result {
+    float t, u, v;
+    int id, ti;        // original primitive ID and leaf-order index (-1: miss)
+    bool occluded;
+    int visits, tests;
+};
+
+// Traverses one ray. Stack entries (node ref, entry distance) live in LDS
+// (slots [0, LDSD)) and spill to HBM beyond.
+template <int MODE, bool MT, bool FLAT, int LDSD, int BLOCK>
+__device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int* s_ref, float* s_tn, int tid,
+                                          int gtid, Result& res) {
+    const uint4* __restrict__ nodes = a.nodes;
+    const float4* __restrict__ tris = a.tris;
+    int sp = 0, visits = 0, tests = 0;
+    int ref = 0;
+    float tn = -INFINITY;
+    float best = r.tmax, bu = 0.f, bv = 0.f;
+    int bid = -1, bti = -1;
+    bool found = false;
+
+    auto push = [&](int rf, float t) {
+        if (sp < LDSD) {
+            s_ref[sp * BLOCK + tid] = rf;
+            s_tn[sp * BLOCK + tid] = t;
+        } else if (sp < a.maxStack) {
+            a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid] = make_int2(rf, __float_as_int(t));
+        } else {
+            atomicOr(a.status, 1);   // unreachable: maxStack bounds a DFS of this BVH
+            return;
+        }
+        sp++;
+    };
+
+    for (;;) {
+        bool descend = false;
+        if constexpr (MODE == kModeAny) {
+            if (ref < 0) {
+                const int enc = -(ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
+                for (int k = 0; k < tc; k++) {
+                    const int ti = ts + k;
+                    tests++;
+                    float t, u, v;
+                    if (rtd::isect<MT>(r, tris[3 * ti], tris[3 * ti + 1], tris[3 * ti + 2], t, u, v) &&
+                        t > r.tmin && t < r.tmax) { found = true; break; }
+                }
+                if (found) break;
+            } else {
+                visits++;
+                const uint4* np = nodes + 4 * (size_t)ref;
+                const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
+                float d0, d1, d2, d3;
+                const bool h0 = (int)qc.x != -1 && rtd::slab_child<0, FLAT>(r, qx, qy, qz, r.tmax, d0);
+                const bool h1 = (int)qc.y != -1 && rtd::slab_child<1, FLAT>(r, qx, qy, qz, r.tmax, d1);
+                const bool h2 = (int)qc.z != -1 && rtd::slab_child<2, FLAT>(r, qx, qy, qz, r.tmax, d2);
+                const bool h3 = (int)qc.w != -1 && rtd::slab_child<3, FLAT>(r, qx, qy, qz, r.tmax, d3);
+                // pushed in slot order; the last valid one is taken next instead of popped
+                int nxt = -1;
+                if (h0) { nxt = (int)qc.x; }
+                if (h1) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.y; }
+                if (h2) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.z; }
+                if (h3) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.w; }
+                if (nxt != -1) { ref = nxt; descend = true; }
+            }
+        } else {
+            if (tn <= best) {
+                if (ref < 0) {
+                    const int enc = -(ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
+                    for (int k = 0; k < tc; k++) {
+                        const int ti = ts + k;
+                        tests++;
+                        const float4 p0 = tris[3 * ti];
+                        float t, u, v;
+                        if (!rtd::isect<MT>(r, p0, tris[3 * ti + 1], tris[3 * ti + 2], t, u, v)) continue;
+                        if (!(t > r.tmin) || !(t < r.tmax)) continue;
+                        const int id = __float_as_int(p0.w);
+                        if (t < best || (t == best && id < bid)) { best = t; bid = id; bu = u; bv = v; bti = ti; }
+                    }
+                } else {
+                    visits++;
+                    const uint4* np = nodes + 4 * (size_t)ref;
+                    const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
+                    float d[4];
+                    int c[4];
+                    float tt;
+                    if ((int)qc.x != -1 && rtd::slab_child<0, FLAT>(r, qx, qy, qz, best, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
+                    if ((int)qc.y != -1 && rtd::slab_child<1, FLAT>(r, qx, qy, qz, best, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
+                    if ((int)qc.z != -1 && rtd::slab_child<2, FLAT>(r, qx, qy, qz, best, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
+                    if ((int)qc.w != -1 && rtd::slab_child<3, FLAT>(r, qx, qy, qz, best, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
+#define RT_CSWAP(i, j)                                                                  \
+    {                                                                                   \
+        const bool s = d[i] > d[j];                                                     \
+        const float di = d[i], dj = d[j];                                               \
+        const int ci = c[i], cj = c[j];                                                 \
+        d[i] = s ? dj : di; d[j] = s ? di : dj; c[i] = s ? cj : ci; c[j] = s ? ci : cj; \
+    }
+                    RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
+#undef RT_CSWAP
+                    // pushed far-to-near; the nearest valid one is taken next
+                    int nxt = -1; float nt = 0.f;
+                    if (c[3] != -1) { nxt = c[3]; nt = d[3]; }
+                    if (c[2] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[2]; nt = d[2]; }
+                    if (c[1] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[1]; nt = d[1]; }
+                    if (c[0] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[0]; nt = d[0]; }
+                    if (nxt != -1) { ref = nxt; tn = nt; descend = true; }
+                }
+            }
+        }
+        if (descend) continue;
+        if (sp == 0) break;
+        sp--;
+        if (sp < LDSD) {
+            ref = s_ref[sp * BLOCK + tid];
+            tn = s_tn[sp * BLOCK + tid];
+        } else {
+            const int2 e = a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid];
+            ref = e.x; tn = __int_as_float(e.y);
+        }
+    }
+    res.t = best; res.u = bu; res.v = bv; res.id = bid; res.ti = bti;
+    res.occluded = found; res.visits = visits; res.tests = tests;
+}
+
+template <int MODE, bool MT, bool PIN, bool COUNT, int LDSD, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
+    __shared__ int s_ref[LDSD * BLOCK];
+    __shared__ float s_tn[LDSD * BLOCK];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int gtid = blockIdx.x * BLOCK + tid;
+    // queued traces read their ray count from device memory (no host round trip)
+    const int n = a.nDev ? min(*a.nDev, a.n) : a.n;
+
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(a.counter, 64u);
+        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+        if (base >= (uint32_t)n) break;
+        const int ri = (int)base + lane;
+        const bool live = ri < n;
+
+        Ray r;
+        int py = 0;
+        if (live) {
+            if constexpr (PIN) {
+                py = ri / a.W;
+                rtd::pinhole_ray(a.cam, ri - py * a.W, py, a.W, a.H, r);
+            } else {
+                r.ox = a.ox[ri]; r.oy = a.oy[ri]; r.oz = a.oz[ri];
+                r.dx = a.dx[ri]; r.dy = a.dy[ri]; r.dz = a.dz[ri];
+                r.tmin = a.tmin[ri]; r.tmax = a.tmax[ri];
+            }
+            rtd::ray_setup(r);
+        }
+        // A wave with any flat ray takes the slab variant that resolves
+        // 0*inf; both variants agree on every other ray.
+        const bool anyFlat = __any(live && rtd::ray_flat(r));
+        if (!live) continue;
+        Result res;
+        if (anyFlat) trace_ray<MODE, MT, true, LDSD, BLOCK>(a, r, s_ref, s_tn, tid, gtid, res);
+        else trace_ray<MODE, MT, false, LDSD, BLOCK>(a, r, s_ref, s_tn, tid, gtid, res);
+
+        if constexpr (MODE == kModeAny) {
+            a.occ[ri] = res.occluded ? 1 : 0;
+        } else if constexpr (MODE == kModeClosest) {
+            const bool hit = res.id >= 0;
+            a.t[ri] = hit ? res.t : RT_MISS_T;
+            if (a.id) a.id[ri] = res.id;
+            if (a.u) a.u[ri] = hit ? res.u : 0.f;
+            if (a.v) a.v[ri] = hit ? res.v : 0.f;
+        } else {
+            shade_pixel(a, r, ri, py, res.id >= 0, res.ti);
+        }
+        if constexpr (COUNT) {
+            a.cntNodes[ri] = res.visits;
+            a.cntTris[ri] = res.tests;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launch plumbing
+// ---------------------------------------------------------------------------
+constexpr int kBlock = 256;
+constexpr int kLdsDepth = 16;
+
+static int num_cus() {
+    static int n = 0;
+    if (!n) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+        if (n <= 0) n = 1;
+    }
+    return n;
+}
+
+template <int MODE, bool MT, bool PIN, bool COUNT>
+static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, hipStream_t s) {
+    auto kern = k_trace<MODE, MT, PIN, COUNT, kLdsDepth, kBlock>;
+    static int blocksPerCU = 0;
+    if (!blocksPerCU) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, 0) != hipSuccess || nb <= 0) nb = 1;
+        blocksPerCU = nb;
+    }
+    const int grid = blocksPerCU * num_cus();
+    const int threads = grid * kBlock;
+    a.maxStack = 3 * h->depth4 + 4;
+    const int spillDepth = a.maxStack > kLdsDepth ? a.maxStack - kLdsDepth : 0;
+    const size_t need = (size_t)spillDepth * threads * sizeof(int2);
+    if (need > sc.spillBytes) {
+        if (sc.spill) hipFree(sc.spill);
+        sc.spill = nullptr; sc.spillBytes = 0;
+        if (hipMalloc(&sc.spill, need) != hipSuccess) {
+            rt::set_error("cannot allocate %zu bytes of traversal stack spill", need);
+            return -1;
+        }
+        sc.spillBytes = need;
+    }
+    a.spill = (int2*)sc.spill;
+    a.spillStride = threads;
+    a.counter = sc.counter;
+    a.status = sc.status;
+    a.nodes = (const uint4*)h->d_nodes4;
+    a.tris = (const float4*)h->d_tris;
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, sc.counter);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { rt::set_error("trace launch failed: %s", hipGetErrorString(e)); return -1; }
+    return 0;
+}
+
+int alloc_scratch(rt::TraceScratch& sc) {
+    if (sc.counter) return 0;
+    if (hipMalloc((void**)&sc.counter, 64) != hipSuccess || hipMalloc((void**)&sc.status, 64) != hipSuccess ||
+        hipMemset(sc.status, 0, 64) != hipSuccess) {
+        free_scratch(sc);
+        rt::set_error("cannot allocate traversal scratch");
+        return -1;
+    }
+    return 0;
+}
+
+void free_scratch(rt::TraceScratch& sc) {
+    hipFree(sc.spill); hipFree(sc.counter); hipFree(sc.status); hipFree(sc.tiles);
+    sc.spill = nullptr; sc.spillBytes = 0; sc.counter = nullptr; sc.status = nullptr;
+    sc.tiles = nullptr; sc.tileCap = 0;
+}
+
+int launch_trace(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int mode, bool mt, bool pinhole,
+                 bool counted, hipStream_t s) {
+    if (alloc_scratch(sc) != 0) return -1;
+    if (a.n <= 0) return 0;
+#define RT_DISPATCH(M)                                                                   \
+    if (mt) {                                                                            \
+        if (pinhole) return counted ? launch_one<M, true, true, true>(h, sc, a, s) : launch_one<M, true, true, false>(h, sc, a, s); \
+        return counted ? launch_one<M, true, false, true>(h, sc, a, s) : launch_one<M, true, false, false>(h, sc, a, s);          \
+    } else {                                                                             \
+        if (pinhole) return counted ? launch_one<M, false, true, true>(h, sc, a, s) : launch_one<M, false, true, false>(h, sc, a, s); \
+        return counted ? launch_one<M, false, false, true>(h, sc, a, s) : launch_one<M, false, false, false>(h, sc, a, s);         \
+    }
+    if (mode == kModeClosest) { RT_DISPATCH(kModeClosest) }
+    if (mode == kModeAny) { RT_DISPATCH(kModeAny) }
+    if (mode == kModeShade) {
+        if (mt) return launch_one<kModeShade, true, true, false>(h, sc, a, s);
+        return launch_one<kModeShade, false, true, false>(h, sc, a, s);
+    }
+#undef RT_DISPATCH
+    rt::set_error("bad trace mode %d", mode);
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// Pinhole ray generation into a packet (same arithmetic as the fused kernel)
+// ---------------------------------------------------------------------------
+__global__ void k_gen_pinhole(rtd::Cam cam, int W, int H, float* ox, float* oy, float* oz, float* dx, float* dy,
+                              float* dz, float* tmin, float* tmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W * H) return;
+    const int py = i / W, px = i - py * W;
+    Ray r;
+    rtd::pinhole_ray(cam, px, py, W, H, r);
+    ox[i] = r.ox; oy[i] = r.oy; oz[i] = r.oz;
+    dx[i] = r.dx; dy[i] = r.dy; dz[i] = r.dz;
+    tmin[i] = r.tmin; tmax[i] = r.tmax;
+}
+
+int launch_gen_pinhole(const rtd::Cam& cam, int W, int H, const CudaRayPacket& p, hipStream_t s) {
+    const int n = W * H;
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_gen_pinhole, dim3((n + 255) / 256), dim3(256), 0, s, cam, W, H, (float*)p.ox, (float*)p.oy,
+                       (float*)p.oz, (float*)p.dx, (float*)p.dy, (float*)p.dz, (float*)p.tmin, (float*)p.tmax);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { rt::set_error("ray generation launch failed: %s", hipGetErrorString(e)); return -1; }
+    return 0;
+}
+
+}  // namespace rtk

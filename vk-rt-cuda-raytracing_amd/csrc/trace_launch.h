@@ -1,0 +1,45 @@
+// trace_launch.h — kernel argument block and launcher declarations shared by
+// the traversal kernels (trace.hip) and the C-ABI layer (api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rt_device.h"
+#include "rt_internal.h"
+
+namespace rtk {
+
+enum { kModeClosest = 0, kModeAny = 1, kModeShade = 2 };
+
+struct TraceArgs {
+    const uint4* nodes = nullptr;       // BVH4, 4 x uint4 per node
+    const float4* tris = nullptr;       // leaf order, 3 x float4 per triangle
+    int n = 0;                          // rays (capacity when nDev is set)
+    const int* nDev = nullptr;          // device-resident ray count (queued traces)
+    // ray source: SoA packet, or a pinhole camera when generated inline
+    const float *ox = nullptr, *oy = nullptr, *oz = nullptr, *dx = nullptr, *dy = nullptr, *dz = nullptr;
+    const float *tmin = nullptr, *tmax = nullptr;
+    rtd::Cam cam{};
+    int W = 0, H = 0;
+    // outputs
+    float* t = nullptr; int* id = nullptr; float* u = nullptr; float* v = nullptr;
+    uint8_t* occ = nullptr;
+    int* cntNodes = nullptr; int* cntTris = nullptr;
+    void* img = nullptr; int fmt = 0;   // shading output (RGBA8, BGRA8, RGBA16F, RGBA32F)
+    // scheduling and stack
+    uint32_t* counter = nullptr;
+    int2* spill = nullptr; int spillStride = 0; int maxStack = 0;
+    int* status = nullptr;
+};
+
+int launch_trace(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int mode, bool mt, bool pinhole,
+                 bool counted, hipStream_t s);
+int alloc_scratch(rt::TraceScratch& sc);
+void free_scratch(rt::TraceScratch& sc);
+int launch_gen_pinhole(const rtd::Cam& cam, int W, int H, const CudaRayPacket& p, hipStream_t s);
+int launch_bounce(const CudaBVHHandle* h, rt::TraceScratch& sc, const rtd::Cam& cam, int W, int H,
+                  const float* hitT, const int* hitId, uint32_t seed, const float light[3], const CudaRayPacket& d,
+                  const CudaRayPacket& s, int* src, int* count, hipStream_t st);
+
+}  // namespace rtk

@@ -1,0 +1,143 @@
+"""Seeded synthetic scenes and cameras for the benchmark workloads.
+
+All generators are deterministic functions of their seed (numpy PCG64 via
+``default_rng``) and return float32 triangles of shape (n, 9) in original
+order: v0.xyz, v1.xyz, v2.xyz.
+
+* random_triangles  — small random triangles (the 65,536-triangle parity scene
+  and the many-scene batches).
+* terrain_scene     — a 1024 x 1024-cell sine-octave heightfield over
+  [-10, 10]^2 (2,097,152 triangles) plus 256 icospheres of subdivision level 4
+  (5,120 triangles each): 3,407,872 triangles in all. The icospheres are added
+  on top of the heightfield triangles.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+TAN30 = math.tan(math.radians(30.0))
+
+
+def random_triangles(n: int, seed: int = 0, min_len: float = 0.005, max_len: float = 0.05) -> np.ndarray:
+    """Centroids uniform in [-1,1]^3; each vertex = centroid + (uniform unit
+    direction) * length uniform in [min_len, max_len]."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1.0, 1.0, size=(n, 1, 3))
+    d = rng.normal(size=(n, 3, 3))
+    d /= np.linalg.norm(d, axis=2, keepdims=True)
+    ln = rng.uniform(min_len, max_len, size=(n, 3, 1))
+    return (c + d * ln).astype(np.float32).reshape(n, 9)
+
+
+# ----------------------------------------------------------------------------
+# Terrain scene
+# ----------------------------------------------------------------------------
+def _octaves(seed: int, n: int = 6):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.0, 2.0 * np.pi, n), rng.uniform(0.0, 2.0 * np.pi, n)
+
+
+def terrain_height(x, z, seed: int = 1, amp0: float = 1.0, freq0: float = 0.35, octaves: int = 6):
+    """Sum of `octaves` sine waves; amplitude halves and frequency doubles."""
+    theta, phase = _octaves(seed, octaves)
+    h = np.zeros(np.broadcast(x, z).shape, np.float64)
+    for k in range(octaves):
+        a, f = amp0 * 0.5 ** k, freq0 * 2.0 ** k
+        h += a * np.sin(f * (x * np.cos(theta[k]) + z * np.sin(theta[k])) + phase[k])
+    return h
+
+
+def heightfield(res: int = 1024, extent: float = 10.0, seed: int = 1) -> np.ndarray:
+    xs = np.linspace(-extent, extent, res + 1)
+    X, Z = np.meshgrid(xs, xs, indexing="xy")          # X varies along columns
+    Y = terrain_height(X, Z, seed)
+    V = np.stack([X, Y, Z], axis=-1)                   # (res+1, res+1, 3)
+    v00, v10 = V[:-1, :-1], V[:-1, 1:]
+    v01, v11 = V[1:, :-1], V[1:, 1:]
+    t0 = np.concatenate([v00, v10, v11], axis=-1).reshape(-1, 9)
+    t1 = np.concatenate([v00, v11, v01], axis=-1).reshape(-1, 9)
+    tris = np.stack([t0, t1], axis=1).reshape(-1, 9)   # cell-interleaved
+    return tris.astype(np.float32)
+
+
+def icosphere(level: int = 4) -> np.ndarray:
+    """Unit icosphere as (20*4^level, 3, 3) float64."""
+    p = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, p, 0), (1, p, 0), (-1, -p, 0), (1, -p, 0), (0, -1, p), (0, 1, p), (0, -1, -p), (0, 1, -p),
+         (p, 0, -1), (p, 0, 1), (-p, 0, -1), (-p, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    V = np.array(v, np.float64)
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    T = V[np.array(f)]                                 # (20, 3, 3)
+    for _ in range(level):
+        a, b, c = T[:, 0], T[:, 1], T[:, 2]
+        ab, bc, ca = a + b, b + c, c + a
+        ab /= np.linalg.norm(ab, axis=1, keepdims=True)
+        bc /= np.linalg.norm(bc, axis=1, keepdims=True)
+        ca /= np.linalg.norm(ca, axis=1, keepdims=True)
+        T = np.stack([np.stack([a, ab, ca], 1), np.stack([ab, b, bc], 1), np.stack([ca, bc, c], 1),
+                      np.stack([ab, bc, ca], 1)], 1).reshape(-1, 3, 3)
+    return T
+
+
+def terrain_scene(res: int = 1024, spheres: int = 256, level: int = 4, seed: int = 1) -> np.ndarray:
+    """Heightfield plus `spheres` icospheres (radii uniform in [0.1, 1.0])
+    resting partly above the terrain."""
+    hf = heightfield(res, 10.0, seed)
+    if spheres <= 0:
+        return hf
+    rng = np.random.default_rng(seed + 1000)
+    cx = rng.uniform(-9.0, 9.0, spheres)
+    cz = rng.uniform(-9.0, 9.0, spheres)
+    r = rng.uniform(0.1, 1.0, spheres)
+    cy = terrain_height(cx, cz, seed) + r * rng.uniform(0.3, 1.2, spheres)
+    unit = icosphere(level)                            # (m, 3, 3)
+    C = np.stack([cx, cy, cz], 1)[:, None, None, :]
+    S = (unit[None] * r[:, None, None, None] + C).reshape(-1, 9).astype(np.float32)
+    return np.concatenate([hf, S], axis=0)
+
+
+# ----------------------------------------------------------------------------
+# Cameras: (o, f, r, u, scale_x, scale_y) for the pinhole generator
+#   d = f + (sx*scale_x)*r + (sy*scale_y)*u,  sx,sy in (-1,1); row 0 on top.
+# ----------------------------------------------------------------------------
+def look_at(eye, target, width: int, height: int, vfov_deg: float = 60.0, world_up=(0.0, 1.0, 0.0)):
+    eye = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - eye
+    f /= np.linalg.norm(f)
+    r = np.cross(np.asarray(world_up, np.float64), f)
+    r /= np.linalg.norm(r)
+    up = np.cross(f, r)
+    th = math.tan(math.radians(vfov_deg) / 2.0)
+    return dict(o=eye.astype(np.float32), f=f.astype(np.float32), r=r.astype(np.float32),
+                u=(-up).astype(np.float32), scale_x=np.float32(th * width / height), scale_y=np.float32(th))
+
+
+def parity_camera(width: int = 512, height: int = 512):
+    """(0,0,-3) looking +z, 60 degree vertical FOV."""
+    return look_at((0.0, 0.0, -3.0), (0.0, 0.0, 0.0), width, height, 60.0)
+
+
+# The terrain camera looks down onto the heightfield so that most primary
+# rays hit geometry: from (-3,5,-9) toward (0,-1,0), 60 degree vertical FOV,
+# about 82% of 1920x1080 rays hit (from (0,8,-18) toward the origin only about
+# 32% do, the rest leave over the far edge into the sky).
+TERRAIN_EYE = (-3.0, 5.0, -9.0)
+TERRAIN_TARGET = (0.0, -1.0, 0.0)
+
+
+def terrain_camera(width: int = 1920, height: int = 1080):
+    return look_at(TERRAIN_EYE, TERRAIN_TARGET, width, height, 60.0)
+
+
+TERRAIN_LIGHT = (0.0, 20.0, 0.0)
+
+
+def cam_array(cam) -> np.ndarray:
+    """14 floats: o f r u scale_x scale_y (the oracle's camera layout)."""
+    return np.concatenate([cam["o"], cam["f"], cam["r"], cam["u"],
+                           np.array([cam["scale_x"], cam["scale_y"]], np.float32)]).astype(np.float32)
