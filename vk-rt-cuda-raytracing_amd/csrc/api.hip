@@ -523,6 +523,7 @@ static int trace_packet(CudaBVH_t bvh, int n, const CudaRayPacket* rays, const C
                         int* cn, int* ct, unsigned flags, int mode, const char* fn) {
     rt::clear_error();
     if (!check_handle(bvh, fn)) return -1;
+    if (n == 0) return 0;                              // nothing to trace (buffers may be NULL)
     if (n < 0 || !packet_ok(rays)) { set_error("%s: bad ray packet", fn); return -1; }
     if (mode == rtk::kModeClosest && (!hits || !hits->t)) { set_error("%s: hit packet needs t", fn); return -1; }
     if (mode == rtk::kModeAny && !occ) { set_error("%s: NULL occlusion output", fn); return -1; }
