@@ -46,8 +46,9 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 18, help="rays per class for the CPU baseline")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 22, help="max rays per class for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch configuration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -105,6 +106,25 @@ def main():
         step()
     torch.cuda.synchronize()
     m = int(cnt.item())
+
+    # Autotuner (untimed): LDS stack depth and resident workgroups per CU for
+    # each ray class, measured on this frame's rays.
+    tuning = {}
+    if not args.no_tune:
+        with torch.cuda.stream(stream):
+            Dm = {k: v[:m] for k, v in D.items()}
+            Sm = {k: v[:m] for k, v in S.items()}
+            for name, cls, kw in (("coherent_closest", pkg.CLASS_COHERENT, dict(width=W, height=H, cam=c)),
+                                  ("incoherent_closest", pkg.CLASS_INCOHERENT, dict(rays=Dm)),
+                                  ("shadow_any", pkg.CLASS_ANY, dict(rays=Sm))):
+                best, table = bvh.autotune(cls, reps=3, **kw)
+                tuning[name] = {"lds_depth": best[0], "blocks_per_cu": best[1],
+                                "sweep_ms": {f"{d}/{b}": round(ms, 4) for d, b, ms in table}}
+                log(f"autotune {name}: lds {best[0]} blocks/CU {best[1] or 'max'}  "
+                    + " ".join(f"{d}/{b}:{ms:.3f}" for d, b, ms in table))
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
     hit_rate = float((pid >= 0).float().mean().item())
 
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
@@ -189,6 +209,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "build_s": round(build_s, 2),
+        "tuning": tuning,
     }
     print(json.dumps(out), flush=True)
     bvh.set_stream(None)
@@ -205,7 +226,7 @@ def load_traffic():
         return json.load(f).get("incoherent_closest_bytes_per_launch")
 
 
-def cpu_baseline(bvh, Pr, D, m, sample):
+def cpu_baseline(bvh, Pr, D, m, sample, budget_s=10.0):
     """The CPU oracle's traversal (C, OpenMP over all host cores) on an evenly
     strided sample of the same primary and diffuse rays and the same BVH."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -221,17 +242,18 @@ def cpu_baseline(bvh, Pr, D, m, sample):
     threads = pyoracle.num_threads(0)
     pyoracle.trace_closest(nodes, lt, pid, prim[:, :4096], counts=False)       # warm-up
     times = []
-    for _ in range(3):
+    t_all = time.perf_counter()
+    while len(times) < 3 or (time.perf_counter() - t_all < budget_s and len(times) < 50):
         t0 = time.perf_counter()
         pyoracle.trace_closest(nodes, lt, pid, prim, counts=False)
         pyoracle.trace_closest(nodes, lt, pid, diff, counts=False)
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     rays = prim.shape[1] + diff.shape[1]
-    log(f"cpu baseline: {rays} rays in {t:.3f} s on {threads} threads")
+    log(f"cpu baseline: {rays} rays in {t:.3f} s (median of {len(times)}) on {threads} threads")
     return {"value": round(rays / t / 1e6, 2), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{prim.shape[1]} primary + {diff.shape[1]} diffuse rays (evenly strided), closest hit, "
-                      f"watertight, same BVH; median of 3"}
+            "sample": f"{prim.shape[1]} primary + {diff.shape[1]} diffuse rays of the frame, closest hit, "
+                      f"watertight, same BVH; median of {len(times)} passes (~{budget_s:.0f} s of CPU work)"}
 
 
 if __name__ == "__main__":

@@ -217,6 +217,21 @@ int cudaBVH_traceClosestQueued(CudaBVH_t bvh, int capacity, const int* d_count,
 int cudaBVH_traceAnyQueued(CudaBVH_t bvh, int capacity, const int* d_count,
                            const CudaRayPacket* rays, uint8_t* occluded, unsigned flags);
 
+/* Launch configuration per ray class (0 coherent pinhole primaries, 1
+ * incoherent closest hit on explicit rays, 2 any hit): LDS stack depth per
+ * lane (4, 8, 12 or 16 entries; deeper entries spill to HBM) and resident
+ * workgroups per CU (0 = occupancy limit). */
+int cudaBVH_setTraceConfig(CudaBVH_t bvh, int rayClass, int ldsDepth, int blocksPerCU);
+int cudaBVH_getTraceConfig(CudaBVH_t bvh, int rayClass, int* ldsDepth, int* blocksPerCU);
+
+/* Autotuner: times every candidate configuration of one ray class on the
+ * caller's workload (coherent: a W x H pinhole frame; otherwise numRays
+ * device rays), reps timed launches each, and keeps the fastest in the
+ * handle. results (nullable, 3 floats per candidate: ldsDepth, blocksPerCU,
+ * ms) and *numResults report the sweep. */
+int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const CudaCamera* cam,
+                     int numRays, const CudaRayPacket* rays, int reps, float* results, int* numResults);
+
 /* Handle stream: all work of the handle is issued on this HIP stream
  * (NULL = the null stream). */
 int   cudaBVH_setStream(CudaBVH_t bvh, void* hipStream);

@@ -676,4 +676,90 @@ int cudaBVH_traceAnyQueued(CudaBVH_t bvh, int capacity, const int* d_count, cons
                         "cudaBVH_traceAnyQueued");
 }
 
+int cudaBVH_setTraceConfig(CudaBVH_t bvh, int rayClass, int ldsDepth, int blocksPerCU) {
+    rt::clear_error();
+    if (!check_handle(bvh, "cudaBVH_setTraceConfig")) return -1;
+    if (rayClass < 0 || rayClass >= rt::kNumClasses || (ldsDepth != 4 && ldsDepth != 8 && ldsDepth != 12 &&
+                                                         ldsDepth != 16) || blocksPerCU < 0) {
+        set_error("cudaBVH_setTraceConfig: bad class %d / ldsDepth %d / blocksPerCU %d", rayClass, ldsDepth,
+                  blocksPerCU);
+        return -1;
+    }
+    bvh->tune[rayClass].ldsDepth = ldsDepth;
+    bvh->tune[rayClass].blocksPerCU = blocksPerCU;
+    return 0;
+}
+
+int cudaBVH_getTraceConfig(CudaBVH_t bvh, int rayClass, int* ldsDepth, int* blocksPerCU) {
+    if (!check_handle(bvh, "cudaBVH_getTraceConfig") || rayClass < 0 || rayClass >= rt::kNumClasses) return -1;
+    if (ldsDepth) *ldsDepth = bvh->tune[rayClass].ldsDepth;
+    if (blocksPerCU) *blocksPerCU = bvh->tune[rayClass].blocksPerCU;
+    return 0;
+}
+
+// Autotuner: times every candidate launch configuration of one ray class on
+// the caller's workload (a pinhole frame for the coherent class, a device ray
+// packet otherwise) and keeps the fastest in the handle. results (nullable)
+// receives one row {ldsDepth, blocksPerCU, milliseconds} per candidate.
+int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const CudaCamera* cam, int numRays,
+                     const CudaRayPacket* rays, int reps, float* results, int* numResults) {
+    rt::clear_error();
+    if (!check_handle(bvh, "cudaBVH_autotune")) return -1;
+    const bool coherent = rayClass == rt::kClassCoherent;
+    if (rayClass < 0 || rayClass >= rt::kNumClasses || reps <= 0 || (coherent && (!cam || width <= 0 || height <= 0)) ||
+        (!coherent && (numRays <= 0 || !packet_ok(rays)))) {
+        set_error("cudaBVH_autotune: bad arguments");
+        return -1;
+    }
+    const int n = coherent ? width * height : numRays;
+    hipStream_t s = hstream(bvh);
+    void* out = nullptr;
+    HIP_TRY(hipMalloc(&out, size_t(n) * 16), return -1);
+    rtk::TraceArgs base;
+    base.n = n;
+    if (coherent) { base.W = width; base.H = height; base.cam = to_cam(*cam); }
+    else {
+        base.ox = rays->ox; base.oy = rays->oy; base.oz = rays->oz; base.dx = rays->dx; base.dy = rays->dy;
+        base.dz = rays->dz; base.tmin = rays->tmin; base.tmax = rays->tmax;
+    }
+    float* o = (float*)out;
+    if (rayClass == rt::kClassAny) base.occ = (uint8_t*)out;
+    else { base.t = o; base.id = (int*)(o + n); base.u = o + 2 * size_t(n); base.v = o + 3 * size_t(n); }
+    const int mode = rayClass == rt::kClassAny ? rtk::kModeAny : rtk::kModeClosest;
+    const int depths[4] = {4, 8, 12, 16};
+    const int bpcs[3] = {0, 4, 2};
+    const rt::TraceConfig saved = bvh->tune[rayClass];
+    rt::TraceConfig best = saved;
+    float bestMs = 1e30f;
+    int row = 0;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0); hipEventCreate(&e1);
+    int rc = 0;
+    for (int d : depths) {
+        for (int b : bpcs) {
+            bvh->tune[rayClass].ldsDepth = d;
+            bvh->tune[rayClass].blocksPerCU = b;
+            rtk::TraceArgs a = base;
+            rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, false, coherent, false, s);   // warm-up
+            hipEventRecord(e0, s);
+            for (int r = 0; r < reps && rc == 0; r++) { a = base; rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, false, coherent, false, s); }
+            hipEventRecord(e1, s);
+            hipEventSynchronize(e1);
+            if (rc != 0) break;
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            ms /= reps;
+            if (results) { results[3 * row] = (float)d; results[3 * row + 1] = (float)b; results[3 * row + 2] = ms; }
+            row++;
+            if (ms < bestMs) { bestMs = ms; best = bvh->tune[rayClass]; }
+        }
+        if (rc != 0) break;
+    }
+    hipEventDestroy(e0); hipEventDestroy(e1);
+    hipFree(out);
+    bvh->tune[rayClass] = rc == 0 ? best : saved;
+    if (numResults) *numResults = row;
+    return rc;
+}
+
 }  // extern "C"

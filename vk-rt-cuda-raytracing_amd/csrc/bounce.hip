@@ -56,12 +56,14 @@ struct BounceArgs {
     int* tileCounts;
 };
 
+// Pixel i of tile b, step k: b*kTile + k*kTileThreads + threadIdx.x, so the
+// lanes of a wave read and write consecutive pixels.
 __global__ __launch_bounds__(kTileThreads) void k_count_hits(BounceArgs a) {
     __shared__ int lds[kTileThreads / 64 + 1];
-    const int base = blockIdx.x * kTile + threadIdx.x * kTileItems;
+    const int base = blockIdx.x * kTile + threadIdx.x;
     int c = 0;
     for (int k = 0; k < kTileItems; k++) {
-        const int i = base + k;
+        const int i = base + k * kTileThreads;
         if (i < a.n && a.hitId[i] >= 0) c++;
     }
     int total;
@@ -133,19 +135,29 @@ __device__ __forceinline__ void emit(const BounceArgs& a, int i, int pos) {
     a.src[pos] = i;
 }
 
+// Stable compaction: per step, a ballot gives each hit lane its rank within
+// the wave, and the wave totals (in LDS) order the waves.
 __global__ __launch_bounds__(kTileThreads) void k_emit(BounceArgs a) {
-    __shared__ int lds[kTileThreads / 64 + 1];
-    const int base = blockIdx.x * kTile + threadIdx.x * kTileItems;
-    int c = 0;
+    constexpr int NW = kTileThreads / 64;
+    __shared__ int waveCount[NW];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int running = a.tileCounts[blockIdx.x];
     for (int k = 0; k < kTileItems; k++) {
-        const int i = base + k;
-        if (i < a.n && a.hitId[i] >= 0) c++;
-    }
-    int total;
-    int pos = a.tileCounts[blockIdx.x] + rts::block_exclusive_scan<kTileThreads>(c, lds, &total);
-    for (int k = 0; k < kTileItems; k++) {
-        const int i = base + k;
-        if (i < a.n && a.hitId[i] >= 0) emit(a, i, pos++);
+        const int i = blockIdx.x * kTile + k * kTileThreads + threadIdx.x;
+        const bool hit = i < a.n && a.hitId[i] >= 0;
+        const uint64_t m = __ballot(hit);
+        if (lane == 0) waveCount[w] = __popcll(m);
+        __syncthreads();
+        int before = 0, total = 0;
+#pragma unroll
+        for (int j = 0; j < NW; j++) {
+            const int cj = waveCount[j];
+            before += j < w ? cj : 0;
+            total += cj;
+        }
+        if (hit) emit(a, i, running + before + __popcll(m & ((1ull << lane) - 1ull)));
+        running += total;
+        __syncthreads();
     }
 }
 

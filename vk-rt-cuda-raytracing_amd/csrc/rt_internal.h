@@ -61,6 +61,15 @@ uint32_t morton3d(float x, float y, float z);
 // Device scratch of one traversal context: the persistent-kernel work
 // counter, a status word (bit 0: stack overflow, unreachable by
 // construction) and the per-lane stack spill region (grown on demand).
+// Launch configuration of one ray class, chosen by the autotuner: LDS stack
+// depth per lane (4, 8, 12 or 16 entries) and resident workgroups per CU
+// (0 = as many as occupancy allows).
+enum { kClassCoherent = 0, kClassIncoherent = 1, kClassAny = 2, kNumClasses = 3 };
+struct TraceConfig {
+    int ldsDepth = 16;
+    int blocksPerCU = 0;
+};
+
 struct TraceScratch {
     void* spill = nullptr;
     size_t spillBytes = 0;
@@ -101,5 +110,6 @@ struct CudaBVHHandle {
     void* stream = nullptr;             // hipStream_t, borrowed (NULL stream by default)
 
     rt::TraceScratch scratch;           // traversal scratch of calls on this handle
+    rt::TraceConfig tune[rt::kNumClasses];  // per ray class launch configuration
     void* d_work = nullptr;   size_t workBytes = 0;   // staging for host-pointer calls
 };

@@ -198,21 +198,31 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     const int lane = tid & 63;
     const int gtid = blockIdx.x * BLOCK + tid;
     // queued traces read their ray count from device memory (no host round trip)
-    const int n = a.nDev ? min(*a.nDev, a.n) : a.n;
+    // pinhole work items cover 8x8 pixel tiles, one tile per wave
+    const int n = PIN ? a.nTiles * 64 : (a.nDev ? min(*a.nDev, a.n) : a.n);
 
     for (;;) {
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(a.counter, 64u);
         base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
         if (base >= (uint32_t)n) break;
-        const int ri = (int)base + lane;
-        const bool live = ri < n;
+        int ri = (int)base + lane;
+        int py = 0;
+        bool live;
+        if constexpr (PIN) {
+            const int tile = (int)base >> 6;
+            const int ty = tile / a.tilesX, tx = tile - ty * a.tilesX;
+            const int px = tx * 8 + (lane & 7);
+            py = ty * 8 + (lane >> 3);
+            live = px < a.W && py < a.H;
+            ri = py * a.W + px;
+        } else {
+            live = ri < n;
+        }
 
         Ray r;
-        int py = 0;
         if (live) {
             if constexpr (PIN) {
-                py = ri / a.W;
                 rtd::pinhole_ray(a.cam, ri - py * a.W, py, a.W, a.H, r);
             } else {
                 r.ox = a.ox[ri]; r.oy = a.oy[ri]; r.oz = a.oz[ri];
@@ -251,7 +261,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
 // Launch plumbing
 // ---------------------------------------------------------------------------
 constexpr int kBlock = 256;
-constexpr int kLdsDepth = 16;
+constexpr int kDefaultLds = 16;
 
 static int num_cus() {
     static int n = 0;
@@ -264,19 +274,20 @@ static int num_cus() {
     return n;
 }
 
-template <int MODE, bool MT, bool PIN, bool COUNT>
-static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, hipStream_t s) {
-    auto kern = k_trace<MODE, MT, PIN, COUNT, kLdsDepth, kBlock>;
-    static int blocksPerCU = 0;
-    if (!blocksPerCU) {
+template <int MODE, bool MT, bool PIN, bool COUNT, int LDSD>
+static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int blocksPerCU, hipStream_t s) {
+    auto kern = k_trace<MODE, MT, PIN, COUNT, LDSD, kBlock>;
+    static int maxBlocks = 0;
+    if (!maxBlocks) {
         int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, 0) != hipSuccess || nb <= 0) nb = 1;
-        blocksPerCU = nb;
+        maxBlocks = nb;
     }
-    const int grid = blocksPerCU * num_cus();
+    const int bpc = (blocksPerCU > 0 && blocksPerCU < maxBlocks) ? blocksPerCU : maxBlocks;
+    const int grid = bpc * num_cus();
     const int threads = grid * kBlock;
     a.maxStack = 3 * h->depth4 + 4;
-    const int spillDepth = a.maxStack > kLdsDepth ? a.maxStack - kLdsDepth : 0;
+    const int spillDepth = a.maxStack > LDSD ? a.maxStack - LDSD : 0;
     const size_t need = (size_t)spillDepth * threads * sizeof(int2);
     if (need > sc.spillBytes) {
         if (sc.spill) hipFree(sc.spill);
@@ -286,6 +297,10 @@ static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a
             return -1;
         }
         sc.spillBytes = need;
+    }
+    if (PIN) {
+        a.tilesX = (a.W + 7) / 8;
+        a.nTiles = a.tilesX * ((a.H + 7) / 8);
     }
     a.spill = (int2*)sc.spill;
     a.spillStride = threads;
@@ -317,25 +332,44 @@ void free_scratch(rt::TraceScratch& sc) {
     sc.tiles = nullptr; sc.tileCap = 0;
 }
 
+// Tunable variants (LDS stack depth, resident blocks per CU) exist for the
+// watertight closest/any kernels; the counted, Moller-Trumbore and shading
+// kernels use the default configuration.
+template <int MODE, bool PIN>
+static int launch_tuned(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, const rt::TraceConfig& cfg,
+                        hipStream_t s) {
+    switch (cfg.ldsDepth) {
+        case 4: return launch_one<MODE, false, PIN, false, 4>(h, sc, a, cfg.blocksPerCU, s);
+        case 8: return launch_one<MODE, false, PIN, false, 8>(h, sc, a, cfg.blocksPerCU, s);
+        case 12: return launch_one<MODE, false, PIN, false, 12>(h, sc, a, cfg.blocksPerCU, s);
+        default: return launch_one<MODE, false, PIN, false, kDefaultLds>(h, sc, a, cfg.blocksPerCU, s);
+    }
+}
+
 int launch_trace(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int mode, bool mt, bool pinhole,
                  bool counted, hipStream_t s) {
     if (alloc_scratch(sc) != 0) return -1;
     if (a.n <= 0) return 0;
-#define RT_DISPATCH(M)                                                                   \
-    if (mt) {                                                                            \
-        if (pinhole) return counted ? launch_one<M, true, true, true>(h, sc, a, s) : launch_one<M, true, true, false>(h, sc, a, s); \
-        return counted ? launch_one<M, true, false, true>(h, sc, a, s) : launch_one<M, true, false, false>(h, sc, a, s);          \
-    } else {                                                                             \
-        if (pinhole) return counted ? launch_one<M, false, true, true>(h, sc, a, s) : launch_one<M, false, true, false>(h, sc, a, s); \
-        return counted ? launch_one<M, false, false, true>(h, sc, a, s) : launch_one<M, false, false, false>(h, sc, a, s);         \
+    const rt::TraceConfig& cfg = h->tune[mode == kModeAny ? rt::kClassAny : (pinhole ? rt::kClassCoherent : rt::kClassIncoherent)];
+    if (!mt && !counted) {
+        if (mode == kModeClosest) return pinhole ? launch_tuned<kModeClosest, true>(h, sc, a, cfg, s)
+                                                 : launch_tuned<kModeClosest, false>(h, sc, a, cfg, s);
+        if (mode == kModeAny) return pinhole ? launch_tuned<kModeAny, true>(h, sc, a, cfg, s)
+                                             : launch_tuned<kModeAny, false>(h, sc, a, cfg, s);
     }
-    if (mode == kModeClosest) { RT_DISPATCH(kModeClosest) }
-    if (mode == kModeAny) { RT_DISPATCH(kModeAny) }
-    if (mode == kModeShade) {
-        if (mt) return launch_one<kModeShade, true, true, false>(h, sc, a, s);
-        return launch_one<kModeShade, false, true, false>(h, sc, a, s);
+#define RT_L(M, MTv, P, C) launch_one<M, MTv, P, C, kDefaultLds>(h, sc, a, 0, s)
+    if (mode == kModeClosest) {
+        if (mt) return pinhole ? (counted ? RT_L(kModeClosest, true, true, true) : RT_L(kModeClosest, true, true, false))
+                               : (counted ? RT_L(kModeClosest, true, false, true) : RT_L(kModeClosest, true, false, false));
+        return pinhole ? RT_L(kModeClosest, false, true, true) : RT_L(kModeClosest, false, false, true);
     }
-#undef RT_DISPATCH
+    if (mode == kModeAny) {
+        if (mt) return pinhole ? (counted ? RT_L(kModeAny, true, true, true) : RT_L(kModeAny, true, true, false))
+                               : (counted ? RT_L(kModeAny, true, false, true) : RT_L(kModeAny, true, false, false));
+        return pinhole ? RT_L(kModeAny, false, true, true) : RT_L(kModeAny, false, false, true);
+    }
+    if (mode == kModeShade) return mt ? RT_L(kModeShade, true, true, false) : RT_L(kModeShade, false, true, false);
+#undef RT_L
     rt::set_error("bad trace mode %d", mode);
     return -1;
 }

@@ -22,6 +22,7 @@ struct TraceArgs {
     const float *tmin = nullptr, *tmax = nullptr;
     rtd::Cam cam{};
     int W = 0, H = 0;
+    int tilesX = 0, nTiles = 0;         // pinhole: 8x8 pixel tiles, one per wave
     // outputs
     float* t = nullptr; int* id = nullptr; float* u = nullptr; float* v = nullptr;
     uint8_t* occ = nullptr;
