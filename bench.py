@@ -118,10 +118,11 @@ def main():
                                   ("incoherent_closest", pkg.CLASS_INCOHERENT, dict(rays=Dm)),
                                   ("shadow_any", pkg.CLASS_ANY, dict(rays=Sm))):
                 best, table = bvh.autotune(cls, reps=3, **kw)
-                tuning[name] = {"lds_depth": best[0], "blocks_per_cu": best[1],
-                                "sweep_ms": {f"{d}/{b}": round(ms, 4) for d, b, ms in table}}
-                log(f"autotune {name}: lds {best[0]} blocks/CU {best[1] or 'max'}  "
-                    + " ".join(f"{d}/{b}:{ms:.3f}" for d, b, ms in table))
+                sched = ("batch", "while-while")
+                tuning[name] = {"schedule": sched[best[0]], "lds_depth": best[1], "blocks_per_cu": best[2],
+                                "sweep_ms": {f"{sched[sc]}/{d}/{b}": round(ms, 4) for sc, d, b, ms in table}}
+                log(f"autotune {name}: {sched[best[0]]} lds {best[1]} blocks/CU {best[2] or 'max'}  "
+                    + " ".join(f"{sc}/{d}/{b}:{ms:.3f}" for sc, d, b, ms in table))
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()

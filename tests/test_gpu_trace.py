@@ -309,3 +309,37 @@ def test_determinism_and_stream(pkg, oracle, scenes, parity_bvh, dev):
     finally:
         parity_bvh.set_stream(None)
     _assert_closest_equal(a, {"id": b["prim_id"], "t": b["t"], "u": b["u"], "v": b["v"]})
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("depth", [4, 8, 16])
+def test_every_launch_configuration(pkg, oracle, scenes, parity_bvh, dev, schedule, depth):
+    """Each wave schedule and LDS stack depth (4 spills most entries to HBM)
+    gives the oracle's answers for all three ray classes."""
+    import torch
+    W = H = 192
+    cam = scenes.parity_camera(W, H)
+    c = pkg.camera(cam["o"], cam["f"], cam["r"], cam["u"], cam["scale_x"], cam["scale_y"])
+    nodes, lt, pid = _oracle_inputs(parity_bvh)
+    rng = np.random.default_rng(30 + depth)
+    n = 60_000
+    rays = np.zeros((8, n), np.float32)
+    rays[0:3] = rng.uniform(-1.1, 1.1, (3, n))
+    d = rng.normal(size=(3, n))
+    rays[3:6] = d / np.linalg.norm(d, axis=0)
+    rays[6] = 1e-3
+    rays[7] = rng.uniform(0.05, 3.0, n)
+    try:
+        for cls in (pkg.CLASS_COHERENT, pkg.CLASS_INCOHERENT, pkg.CLASS_ANY):
+            parity_bvh.set_trace_config(cls, schedule, depth, 0 if depth != 8 else 2)
+        hb = hit_buffers(W * H, dev)
+        parity_bvh.trace_pinhole_closest(W, H, c, hb["t"], hb["prim_id"], hb["u"], hb["v"])
+        torch.cuda.synchronize()
+        ref = oracle.trace_closest(nodes, lt, pid, oracle.gen_pinhole(scenes.cam_array(cam), W, H))
+        _assert_closest_equal({k: v.cpu().numpy() for k, v in hb.items()}, ref)
+        _assert_closest_equal(_gpu_closest(parity_bvh, rays, dev), oracle.trace_closest(nodes, lt, pid, rays))
+        assert np.array_equal(_gpu_any(parity_bvh, rays, dev), oracle.trace_any(nodes, lt, pid, rays)["occ"])
+    finally:
+        parity_bvh.set_trace_config(pkg.CLASS_COHERENT, 0, 12, 0)
+        parity_bvh.set_trace_config(pkg.CLASS_INCOHERENT, 1, 8, 0)
+        parity_bvh.set_trace_config(pkg.CLASS_ANY, 0, 12, 0)

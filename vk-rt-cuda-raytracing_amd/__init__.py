@@ -20,8 +20,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcudabvh_hip.so")
 
-# Ray classes of the launch configuration / autotuner
+# Ray classes and wave schedules of the launch configuration / autotuner
 CLASS_COHERENT, CLASS_INCOHERENT, CLASS_ANY = 0, 1, 2
+SCHEDULE_BATCH, SCHEDULE_WHILE_WHILE = 0, 1
 
 # Flags (include/cudabvh.h, part 2)
 ISECT_WATERTIGHT = 0
@@ -96,8 +97,8 @@ SIGNATURES = {
                                    C.POINTER(CudaRayPacket), C.POINTER(CudaRayPacket), _P, _P, C.c_uint]),
     "cudaBVH_traceClosestQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), C.POINTER(CudaHitPacket), C.c_uint]),
     "cudaBVH_traceAnyQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), _P, C.c_uint]),
-    "cudaBVH_setTraceConfig": (_I, [_H, _I, _I, _I]),
-    "cudaBVH_getTraceConfig": (_I, [_H, _I, _P, _P]),
+    "cudaBVH_setTraceConfig": (_I, [_H, _I, _I, _I, _I]),
+    "cudaBVH_getTraceConfig": (_I, [_H, _I, _P, _P, _P]),
     "cudaBVH_autotune": (_I, [_H, _I, _I, _I, _P, _I, _P, _I, _P, _P]),
     "cudaBVH_setStream": (_I, [_H, _P]),
     "cudaBVH_getStream": (_P, [_H]),
@@ -331,26 +332,30 @@ class BVH:
                                                flags), "traceAnyQueued")
 
 
-    def set_trace_config(self, ray_class: int, lds_depth: int, blocks_per_cu: int = 0) -> None:
-        _check(self.lib.cudaBVH_setTraceConfig(self.handle, ray_class, lds_depth, blocks_per_cu), "setTraceConfig")
+    def set_trace_config(self, ray_class: int, schedule: int, lds_depth: int, blocks_per_cu: int = 0) -> None:
+        _check(self.lib.cudaBVH_setTraceConfig(self.handle, ray_class, schedule, lds_depth, blocks_per_cu),
+               "setTraceConfig")
 
     def trace_config(self, ray_class: int):
-        d, b = C.c_int(), C.c_int()
-        _check(self.lib.cudaBVH_getTraceConfig(self.handle, ray_class, C.byref(d), C.byref(b)), "getTraceConfig")
-        return d.value, b.value
+        """(schedule, lds_depth, blocks_per_cu) of a ray class."""
+        sc, d, b = C.c_int(), C.c_int(), C.c_int()
+        _check(self.lib.cudaBVH_getTraceConfig(self.handle, ray_class, C.byref(sc), C.byref(d), C.byref(b)),
+               "getTraceConfig")
+        return sc.value, d.value, b.value
 
     def autotune(self, ray_class: int, reps: int = 3, width: int = 0, height: int = 0, cam: CudaCamera | None = None,
                  rays=None):
         """Sweeps the launch configurations of one ray class; returns the
-        chosen (lds_depth, blocks_per_cu) and the table of timings."""
-        res = (C.c_float * (3 * 64))()
+        chosen (schedule, lds_depth, blocks_per_cu) and the table of timings."""
+        res = (C.c_float * 96)()
         nres = C.c_int()
         n = 0 if rays is None else int(rays["ox"].shape[0])
         rp = ray_packet(rays) if rays is not None else None
         _check(self.lib.cudaBVH_autotune(self.handle, ray_class, width, height, C.byref(cam) if cam else None, n,
                                          C.byref(rp) if rp is not None else None, reps, res, C.byref(nres)),
                "autotune")
-        table = [(int(res[3 * i]), int(res[3 * i + 1]), float(res[3 * i + 2])) for i in range(nres.value)]
+        table = [(int(res[4 * i]), int(res[4 * i + 1]), int(res[4 * i + 2]), float(res[4 * i + 3]))
+                 for i in range(nres.value)]
         return self.trace_config(ray_class), table
 
 

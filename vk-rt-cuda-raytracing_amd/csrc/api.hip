@@ -100,6 +100,10 @@ static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf) {
     HIP_TRY(hipMemcpy(h->d_tris, aos.data(), aos.size() * sizeof(float), hipMemcpyHostToDevice), goto fail);
     HIP_TRY(hipMemcpy(h->d_leafOfPrim, leafOf.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice), goto fail);
     if (rtk::alloc_scratch(h->scratch) != 0) goto fail;
+    h->tune[rt::kClassIncoherent].schedule = rt::kScheduleWhileWhile;
+    h->tune[rt::kClassIncoherent].ldsDepth = 8;
+    h->tune[rt::kClassCoherent].ldsDepth = 12;
+    h->tune[rt::kClassAny].ldsDepth = 12;
     if (rt::verbose())
         fprintf(stderr, "[CudaBVH] built %d tris: %d BVH4 nodes (depth %d), %d BVH2 nodes\n", n, h->numNodes4,
                 h->depth4, h->numDFS);
@@ -676,22 +680,25 @@ int cudaBVH_traceAnyQueued(CudaBVH_t bvh, int capacity, const int* d_count, cons
                         "cudaBVH_traceAnyQueued");
 }
 
-int cudaBVH_setTraceConfig(CudaBVH_t bvh, int rayClass, int ldsDepth, int blocksPerCU) {
+int cudaBVH_setTraceConfig(CudaBVH_t bvh, int rayClass, int schedule, int ldsDepth, int blocksPerCU) {
     rt::clear_error();
     if (!check_handle(bvh, "cudaBVH_setTraceConfig")) return -1;
-    if (rayClass < 0 || rayClass >= rt::kNumClasses || (ldsDepth != 4 && ldsDepth != 8 && ldsDepth != 12 &&
-                                                         ldsDepth != 16) || blocksPerCU < 0) {
-        set_error("cudaBVH_setTraceConfig: bad class %d / ldsDepth %d / blocksPerCU %d", rayClass, ldsDepth,
-                  blocksPerCU);
+    if (rayClass < 0 || rayClass >= rt::kNumClasses || (schedule != rt::kScheduleBatch &&
+        schedule != rt::kScheduleWhileWhile) || (ldsDepth != 4 && ldsDepth != 8 && ldsDepth != 12 &&
+        ldsDepth != 16) || blocksPerCU < 0) {
+        set_error("cudaBVH_setTraceConfig: bad class %d / schedule %d / ldsDepth %d / blocksPerCU %d", rayClass,
+                  schedule, ldsDepth, blocksPerCU);
         return -1;
     }
+    bvh->tune[rayClass].schedule = schedule;
     bvh->tune[rayClass].ldsDepth = ldsDepth;
     bvh->tune[rayClass].blocksPerCU = blocksPerCU;
     return 0;
 }
 
-int cudaBVH_getTraceConfig(CudaBVH_t bvh, int rayClass, int* ldsDepth, int* blocksPerCU) {
+int cudaBVH_getTraceConfig(CudaBVH_t bvh, int rayClass, int* schedule, int* ldsDepth, int* blocksPerCU) {
     if (!check_handle(bvh, "cudaBVH_getTraceConfig") || rayClass < 0 || rayClass >= rt::kNumClasses) return -1;
+    if (schedule) *schedule = bvh->tune[rayClass].schedule;
     if (ldsDepth) *ldsDepth = bvh->tune[rayClass].ldsDepth;
     if (blocksPerCU) *blocksPerCU = bvh->tune[rayClass].blocksPerCU;
     return 0;
@@ -699,8 +706,9 @@ int cudaBVH_getTraceConfig(CudaBVH_t bvh, int rayClass, int* ldsDepth, int* bloc
 
 // Autotuner: times every candidate launch configuration of one ray class on
 // the caller's workload (a pinhole frame for the coherent class, a device ray
-// packet otherwise) and keeps the fastest in the handle. results (nullable)
-// receives one row {ldsDepth, blocksPerCU, milliseconds} per candidate.
+// packet otherwise) and keeps the fastest in the handle. results (nullable,
+// room for 4 floats x 24 candidates) receives {schedule, ldsDepth,
+// blocksPerCU, milliseconds} per candidate.
 int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const CudaCamera* cam, int numRays,
                      const CudaRayPacket* rays, int reps, float* results, int* numResults) {
     rt::clear_error();
@@ -726,32 +734,42 @@ int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const C
     if (rayClass == rt::kClassAny) base.occ = (uint8_t*)out;
     else { base.t = o; base.id = (int*)(o + n); base.u = o + 2 * size_t(n); base.v = o + 3 * size_t(n); }
     const int mode = rayClass == rt::kClassAny ? rtk::kModeAny : rtk::kModeClosest;
+    const int schedules[2] = {rt::kScheduleBatch, rt::kScheduleWhileWhile};
     const int depths[4] = {4, 8, 12, 16};
     const int bpcs[3] = {0, 4, 2};
     const rt::TraceConfig saved = bvh->tune[rayClass];
     rt::TraceConfig best = saved;
     float bestMs = 1e30f;
-    int row = 0;
+    int row = 0, rc = 0;
     hipEvent_t e0, e1;
     hipEventCreate(&e0); hipEventCreate(&e1);
-    int rc = 0;
-    for (int d : depths) {
-        for (int b : bpcs) {
-            bvh->tune[rayClass].ldsDepth = d;
-            bvh->tune[rayClass].blocksPerCU = b;
-            rtk::TraceArgs a = base;
-            rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, false, coherent, false, s);   // warm-up
-            hipEventRecord(e0, s);
-            for (int r = 0; r < reps && rc == 0; r++) { a = base; rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, false, coherent, false, s); }
-            hipEventRecord(e1, s);
-            hipEventSynchronize(e1);
+    for (int sch : schedules) {
+        for (int d : depths) {
+            for (int b : bpcs) {
+                bvh->tune[rayClass].schedule = sch;
+                bvh->tune[rayClass].ldsDepth = d;
+                bvh->tune[rayClass].blocksPerCU = b;
+                rtk::TraceArgs a = base;
+                rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, false, coherent, false, s);   // warm-up
+                hipEventRecord(e0, s);
+                for (int r = 0; r < reps && rc == 0; r++) {
+                    a = base;
+                    rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, false, coherent, false, s);
+                }
+                hipEventRecord(e1, s);
+                hipEventSynchronize(e1);
+                if (rc != 0) break;
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, e0, e1);
+                ms /= reps;
+                if (results) {
+                    results[4 * row] = (float)sch; results[4 * row + 1] = (float)d;
+                    results[4 * row + 2] = (float)b; results[4 * row + 3] = ms;
+                }
+                row++;
+                if (ms < bestMs) { bestMs = ms; best = bvh->tune[rayClass]; }
+            }
             if (rc != 0) break;
-            float ms = 0.f;
-            hipEventElapsedTime(&ms, e0, e1);
-            ms /= reps;
-            if (results) { results[3 * row] = (float)d; results[3 * row + 1] = (float)b; results[3 * row + 2] = ms; }
-            row++;
-            if (ms < bestMs) { bestMs = ms; best = bvh->tune[rayClass]; }
         }
         if (rc != 0) break;
     }

@@ -66,6 +66,60 @@ __device__ __forceinline__ void shade_pixel(const TraceArgs& a, const Ray& r, in
     }
 }
 
+// Closest-hit node visit: tests the four children with the current closest
+// distance as limit, orders the hits near-first with the 5-compare network of
+// backend :177, pushes all but the nearest (far-to-near) and returns the
+// nearest in (ref, tn); ref = -1 when no child is hit.
+template <bool FLAT, class Push>
+__device__ __forceinline__ void visit_closest(const uint4* __restrict__ nodes, const Ray& r, float best, int& ref,
+                                              float& tn, Push&& push) {
+    const uint4* np = nodes + 4 * (size_t)ref;
+    const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
+    float d[4];
+    int c[4];
+    float tt;
+    if ((int)qc.x != -1 && rtd::slab_child<0, FLAT>(r, qx, qy, qz, best, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
+    if ((int)qc.y != -1 && rtd::slab_child<1, FLAT>(r, qx, qy, qz, best, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
+    if ((int)qc.z != -1 && rtd::slab_child<2, FLAT>(r, qx, qy, qz, best, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
+    if ((int)qc.w != -1 && rtd::slab_child<3, FLAT>(r, qx, qy, qz, best, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
+#define RT_CSWAP(i, j)                                                                  \
+    {                                                                                   \
+        const bool s = d[i] > d[j];                                                     \
+        const float di = d[i], dj = d[j];                                               \
+        const int ci = c[i], cj = c[j];                                                 \
+        d[i] = s ? dj : di; d[j] = s ? di : dj; c[i] = s ? cj : ci; c[j] = s ? ci : cj; \
+    }
+    RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
+#undef RT_CSWAP
+    int nxt = -1;
+    float nt = 0.f;
+    if (c[3] != -1) { nxt = c[3]; nt = d[3]; }
+    if (c[2] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[2]; nt = d[2]; }
+    if (c[1] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[1]; nt = d[1]; }
+    if (c[0] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[0]; nt = d[0]; }
+    ref = nxt;
+    tn = nt;
+}
+
+// Any-hit node visit: children kept against tmax, pushed in slot order; the
+// last one kept is taken next (v10_pipeline.cu:241-244 pops it first).
+template <bool FLAT, class Push>
+__device__ __forceinline__ void visit_any(const uint4* __restrict__ nodes, const Ray& r, int& ref, Push&& push) {
+    const uint4* np = nodes + 4 * (size_t)ref;
+    const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
+    float d0, d1, d2, d3;
+    const bool h0 = (int)qc.x != -1 && rtd::slab_child<0, FLAT>(r, qx, qy, qz, r.tmax, d0);
+    const bool h1 = (int)qc.y != -1 && rtd::slab_child<1, FLAT>(r, qx, qy, qz, r.tmax, d1);
+    const bool h2 = (int)qc.z != -1 && rtd::slab_child<2, FLAT>(r, qx, qy, qz, r.tmax, d2);
+    const bool h3 = (int)qc.w != -1 && rtd::slab_child<3, FLAT>(r, qx, qy, qz, r.tmax, d3);
+    int nxt = -1;
+    if (h0) { nxt = (int)qc.x; }
+    if (h1) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.y; }
+    if (h2) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.z; }
+    if (h3) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.w; }
+    ref = nxt;
+}
+
 // Per-lane traversal result.
 struct Result {
     float t, u, v;
@@ -74,7 +128,8 @@ struct Result {
     int visits, tests;
 };
 
-// Traverses one ray. Stack entries (node ref, entry distance) live in LDS
+// Batch schedule: traverses one ray in one loop (node and leaf steps share
+// iterations). Stack entries (node ref, entry distance) live in LDS
 // (slots [0, LDSD)) and spill to HBM beyond.
 template <int MODE, bool MT, bool FLAT, int LDSD, int BLOCK>
 __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int* s_ref, float* s_tn, int tid,
@@ -190,8 +245,11 @@ __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int*
     res.occluded = found; res.visits = visits; res.tests = tests;
 }
 
+// Batch schedule: each wave takes 64 rays (one 8x8 pixel tile for pinhole
+// rays) with one atomicAdd and runs until all 64 are done. Best for coherent
+// rays, whose lanes stay in step.
 template <int MODE, bool MT, bool PIN, bool COUNT, int LDSD, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
+__global__ __launch_bounds__(BLOCK) void k_trace_batch(TraceArgs a) {
     __shared__ int s_ref[LDSD * BLOCK];
     __shared__ float s_tn[LDSD * BLOCK];
     const int tid = threadIdx.x;
@@ -257,6 +315,192 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     }
 }
 
+
+// Persistent traversal kernel, while-while with dynamic ray fetch.
+//
+// Each wave loops: (1) lanes without a ray take new ones — one atomicAdd per
+// wave for all idle lanes, ranks from the ballot mask — so a wave stays full
+// while rays of very different lengths finish; (2) node phase: every lane
+// pops/culls its stack and visits internal nodes, in a wave-uniform loop that
+// ends when no lane is at an internal node (each lane then waits at a leaf or
+// has finished); (3) leaf phase: all lanes at a leaf test its triangles
+// together; (4) finished rays write their result. Node and leaf work never
+// share an iteration, so their instruction streams do not serialise against
+// each other. Every ray still follows the reference's depth-first order
+// (pop, re-check against the closest t, visit), so its hits and its node /
+// triangle counters are exactly those of a one-ray-at-a-time traversal.
+template <int MODE, bool MT, bool PIN, bool COUNT, int LDSD, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace_ww(TraceArgs a) {
+    __shared__ int s_ref[LDSD * BLOCK];
+    __shared__ float s_tn[LDSD * BLOCK];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int gtid = blockIdx.x * BLOCK + tid;
+    const uint4* __restrict__ nodes = a.nodes;
+    const float4* __restrict__ tris = a.tris;
+    // queued traces read their ray count from device memory (no host round
+    // trip); pinhole work items cover 8x8 pixel tiles, 64 items per tile
+    const int n = PIN ? a.nTiles * 64 : (a.nDev ? min(*a.nDev, a.n) : a.n);
+
+    Ray r;
+    int ri = 0, py = 0;
+    bool has = false;          // lane holds a ray
+    bool exhausted = false;    // wave-uniform: the work counter has passed n
+    int poolNext = 0, poolEnd = 0;   // wave-uniform: reserved, not yet taken work items
+    bool flat = false, found = false;
+    int sp = 0, ref = -1, visits = 0, tests = 0, bid = -1, bti = -1;
+    float tn = 0.f, best = 0.f, bu = 0.f, bv = 0.f;
+
+    auto push = [&](int rf, float t) {
+        if (sp < LDSD) {
+            s_ref[sp * BLOCK + tid] = rf;
+            s_tn[sp * BLOCK + tid] = t;
+        } else if (sp < a.maxStack) {
+            a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid] = make_int2(rf, __float_as_int(t));
+        } else {
+            atomicOr(a.status, 1);   // unreachable: maxStack bounds a DFS of this BVH
+            return;
+        }
+        sp++;
+    };
+
+    for (;;) {
+        // ---- (1) refill idle lanes from the wave's pool of reserved work
+        // items; the pool is refilled 64 items at a time with one atomicAdd ----
+        if (!exhausted || poolNext < poolEnd) {
+            const uint64_t idle = __ballot(!has);
+            if (idle) {
+                const int cnt = __popcll(idle);
+                const int avail = poolEnd - poolNext;
+                int fresh = n;                               // n: no fresh batch
+                if (cnt > avail && !exhausted) {
+                    uint32_t b = 0;
+                    if (lane == 0) b = atomicAdd(a.counter, 64u);
+                    fresh = min(__builtin_amdgcn_readfirstlane((int)b), n);
+                    if (fresh + 64 >= n) exhausted = true;
+                }
+                const int k = __popcll(idle & ((1ull << lane) - 1ull));
+                int w = n;                                   // n: no item
+                if (k < avail) w = poolNext + k;
+                else if (fresh < n) w = fresh + (k - avail);
+                if (cnt > avail) {
+                    if (fresh < n) { poolNext = fresh + (cnt - avail); poolEnd = fresh + 64; }
+                    else { poolNext = 0; poolEnd = 0; }
+                } else {
+                    poolNext += cnt;
+                }
+                if (!has) {
+                    bool live = w < n;
+                    if constexpr (PIN) {
+                        const int tile = w >> 6, k = w & 63;
+                        const int ty = tile / a.tilesX, tx = tile - ty * a.tilesX;
+                        const int px = tx * 8 + (k & 7);
+                        py = ty * 8 + (k >> 3);
+                        live = live && px < a.W && py < a.H;
+                        ri = py * a.W + px;
+                        if (live) rtd::pinhole_ray(a.cam, px, py, a.W, a.H, r);
+                    } else {
+                        ri = w;
+                        if (live) {
+                            r.ox = a.ox[ri]; r.oy = a.oy[ri]; r.oz = a.oz[ri];
+                            r.dx = a.dx[ri]; r.dy = a.dy[ri]; r.dz = a.dz[ri];
+                            r.tmin = a.tmin[ri]; r.tmax = a.tmax[ri];
+                        }
+                    }
+                    if (live) {
+                        rtd::ray_setup(r);
+                        flat = rtd::ray_flat(r);
+                        has = true;
+                        sp = 0; ref = 0; tn = -INFINITY;
+                        best = r.tmax; bu = 0.f; bv = 0.f; bid = -1; bti = -1;
+                        visits = 0; tests = 0; found = false;
+                    }
+                }
+            }
+        }
+        if (!__any(has)) {
+            if (exhausted && poolNext >= poolEnd) break;
+            continue;
+        }
+        // A wave with any flat ray takes the slab variant that resolves 0*inf;
+        // both variants agree on every other ray.
+        const bool waveFlat = __any(has && flat);
+
+        // ---- (2) node phase ----
+        bool fin = false;
+        for (;;) {
+            if (has && !fin) {
+                // pop until an entry survives the closest-t re-check
+                while (ref == -1 || (MODE != kModeAny && tn > best)) {
+                    if (sp == 0) { fin = true; break; }
+                    sp--;
+                    if (sp < LDSD) {
+                        ref = s_ref[sp * BLOCK + tid];
+                        tn = s_tn[sp * BLOCK + tid];
+                    } else {
+                        const int2 e = a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid];
+                        ref = e.x; tn = __int_as_float(e.y);
+                    }
+                }
+            }
+            const bool atNode = has && !fin && ref >= 0;
+            if (!__any(atNode)) break;
+            if (atNode) {
+                visits++;
+                if constexpr (MODE == kModeAny) {
+                    if (waveFlat) visit_any<true>(nodes, r, ref, push);
+                    else visit_any<false>(nodes, r, ref, push);
+                } else {
+                    if (waveFlat) visit_closest<true>(nodes, r, best, ref, tn, push);
+                    else visit_closest<false>(nodes, r, best, ref, tn, push);
+                }
+            }
+        }
+
+        // ---- (3) leaf phase ----
+        if (has && !fin) {
+            const int enc = -(ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
+            for (int k = 0; k < tc; k++) {
+                const int ti = ts + k;
+                tests++;
+                const float4 p0 = tris[3 * ti];
+                float t, u, v;
+                if (!rtd::isect<MT>(r, p0, tris[3 * ti + 1], tris[3 * ti + 2], t, u, v)) continue;
+                if (!(t > r.tmin) || !(t < r.tmax)) continue;
+                if constexpr (MODE == kModeAny) {
+                    found = true;
+                    fin = true;
+                    break;
+                } else {
+                    const int id = __float_as_int(p0.w);
+                    if (t < best || (t == best && id < bid)) { best = t; bid = id; bu = u; bv = v; bti = ti; }
+                }
+            }
+            ref = -1;
+        }
+
+        // ---- (4) retire finished rays ----
+        if (has && fin) {
+            if constexpr (MODE == kModeAny) {
+                a.occ[ri] = found ? 1 : 0;
+            } else if constexpr (MODE == kModeClosest) {
+                const bool hit = bid >= 0;
+                a.t[ri] = hit ? best : RT_MISS_T;
+                if (a.id) a.id[ri] = bid;
+                if (a.u) a.u[ri] = hit ? bu : 0.f;
+                if (a.v) a.v[ri] = hit ? bv : 0.f;
+            } else {
+                shade_pixel(a, r, ri, py, bid >= 0, bti);
+            }
+            if constexpr (COUNT) {
+                a.cntNodes[ri] = visits;
+                a.cntTris[ri] = tests;
+            }
+            has = false;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Launch plumbing
 // ---------------------------------------------------------------------------
@@ -274,9 +518,9 @@ static int num_cus() {
     return n;
 }
 
-template <int MODE, bool MT, bool PIN, bool COUNT, int LDSD>
+template <int MODE, bool MT, bool PIN, bool COUNT, int LDSD, bool WW>
 static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int blocksPerCU, hipStream_t s) {
-    auto kern = k_trace<MODE, MT, PIN, COUNT, LDSD, kBlock>;
+    auto kern = WW ? k_trace_ww<MODE, MT, PIN, COUNT, LDSD, kBlock> : k_trace_batch<MODE, MT, PIN, COUNT, LDSD, kBlock>;
     static int maxBlocks = 0;
     if (!maxBlocks) {
         int nb = 0;
@@ -335,15 +579,22 @@ void free_scratch(rt::TraceScratch& sc) {
 // Tunable variants (LDS stack depth, resident blocks per CU) exist for the
 // watertight closest/any kernels; the counted, Moller-Trumbore and shading
 // kernels use the default configuration.
+template <int MODE, bool PIN, bool WW>
+static int launch_depth(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, const rt::TraceConfig& cfg,
+                        hipStream_t s) {
+    switch (cfg.ldsDepth) {
+        case 4: return launch_one<MODE, false, PIN, false, 4, WW>(h, sc, a, cfg.blocksPerCU, s);
+        case 8: return launch_one<MODE, false, PIN, false, 8, WW>(h, sc, a, cfg.blocksPerCU, s);
+        case 12: return launch_one<MODE, false, PIN, false, 12, WW>(h, sc, a, cfg.blocksPerCU, s);
+        default: return launch_one<MODE, false, PIN, false, kDefaultLds, WW>(h, sc, a, cfg.blocksPerCU, s);
+    }
+}
+
 template <int MODE, bool PIN>
 static int launch_tuned(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, const rt::TraceConfig& cfg,
                         hipStream_t s) {
-    switch (cfg.ldsDepth) {
-        case 4: return launch_one<MODE, false, PIN, false, 4>(h, sc, a, cfg.blocksPerCU, s);
-        case 8: return launch_one<MODE, false, PIN, false, 8>(h, sc, a, cfg.blocksPerCU, s);
-        case 12: return launch_one<MODE, false, PIN, false, 12>(h, sc, a, cfg.blocksPerCU, s);
-        default: return launch_one<MODE, false, PIN, false, kDefaultLds>(h, sc, a, cfg.blocksPerCU, s);
-    }
+    return cfg.schedule == rt::kScheduleWhileWhile ? launch_depth<MODE, PIN, true>(h, sc, a, cfg, s)
+                                                   : launch_depth<MODE, PIN, false>(h, sc, a, cfg, s);
 }
 
 int launch_trace(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int mode, bool mt, bool pinhole,
@@ -357,7 +608,7 @@ int launch_trace(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int
         if (mode == kModeAny) return pinhole ? launch_tuned<kModeAny, true>(h, sc, a, cfg, s)
                                              : launch_tuned<kModeAny, false>(h, sc, a, cfg, s);
     }
-#define RT_L(M, MTv, P, C) launch_one<M, MTv, P, C, kDefaultLds>(h, sc, a, 0, s)
+#define RT_L(M, MTv, P, C) launch_one<M, MTv, P, C, kDefaultLds, !P>(h, sc, a, 0, s)
     if (mode == kModeClosest) {
         if (mt) return pinhole ? (counted ? RT_L(kModeClosest, true, true, true) : RT_L(kModeClosest, true, true, false))
                                : (counted ? RT_L(kModeClosest, true, false, true) : RT_L(kModeClosest, true, false, false));
