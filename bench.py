@@ -118,7 +118,7 @@ def main():
                                   ("incoherent_closest", pkg.CLASS_INCOHERENT, dict(rays=Dm)),
                                   ("shadow_any", pkg.CLASS_ANY, dict(rays=Sm))):
                 best, table = bvh.autotune(cls, reps=3, **kw)
-                sched = ("batch", "while-while")
+                sched = ("batch", "while-while", "packet")
                 tuning[name] = {"schedule": sched[best[0]], "lds_depth": best[1], "blocks_per_cu": best[2],
                                 "sweep_ms": {f"{sched[sc]}/{d}/{b}": round(ms, 4) for sc, d, b, ms in table}}
                 log(f"autotune {name}: {sched[best[0]]} lds {best[1]} blocks/CU {best[2] or 'max'}  "

@@ -220,7 +220,9 @@ int cudaBVH_traceAnyQueued(CudaBVH_t bvh, int capacity, const int* d_count,
 /* Launch configuration per ray class (0 coherent pinhole primaries, 1
  * incoherent closest hit on explicit rays, 2 any hit): the wave schedule
  * (0 batch: a wave keeps 64 rays until all finish; 1 while-while: separate
- * node and leaf phases, idle lanes refilled from a wave-local work pool),
+ * node and leaf phases, idle lanes refilled from a wave-local work pool;
+ * 2 packet, coherent class only: one 8x8 pixel tile per wave sharing one
+ * traversal stack),
  * LDS stack depth per lane (4, 8, 12 or 16 entries; deeper entries spill to
  * HBM) and resident workgroups per CU (0 = occupancy limit). */
 int cudaBVH_setTraceConfig(CudaBVH_t bvh, int rayClass, int schedule, int ldsDepth, int blocksPerCU);
@@ -229,7 +231,7 @@ int cudaBVH_getTraceConfig(CudaBVH_t bvh, int rayClass, int* schedule, int* ldsD
 /* Autotuner: times every candidate configuration of one ray class on the
  * caller's workload (coherent: a W x H pinhole frame; otherwise numRays
  * device rays), reps timed launches each, and keeps the fastest in the
- * handle. results (nullable, room for 96 floats: schedule, ldsDepth,
+ * handle. results (nullable, room for 144 floats: schedule, ldsDepth,
  * blocksPerCU, ms per candidate) and *numResults report the sweep. */
 int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const CudaCamera* cam,
                      int numRays, const CudaRayPacket* rays, int reps, float* results, int* numResults);

@@ -311,7 +311,7 @@ def test_determinism_and_stream(pkg, oracle, scenes, parity_bvh, dev):
     _assert_closest_equal(a, {"id": b["prim_id"], "t": b["t"], "u": b["u"], "v": b["v"]})
 
 
-@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("schedule", [0, 1, 2])
 @pytest.mark.parametrize("depth", [4, 8, 16])
 def test_every_launch_configuration(pkg, oracle, scenes, parity_bvh, dev, schedule, depth):
     """Each wave schedule and LDS stack depth (4 spills most entries to HBM)
@@ -331,7 +331,9 @@ def test_every_launch_configuration(pkg, oracle, scenes, parity_bvh, dev, schedu
     rays[7] = rng.uniform(0.05, 3.0, n)
     try:
         for cls in (pkg.CLASS_COHERENT, pkg.CLASS_INCOHERENT, pkg.CLASS_ANY):
-            parity_bvh.set_trace_config(cls, schedule, depth, 0 if depth != 8 else 2)
+            # the packet schedule exists for the coherent class only
+            sch = schedule if (schedule < 2 or cls == pkg.CLASS_COHERENT) else 0
+            parity_bvh.set_trace_config(cls, sch, depth, 0 if depth != 8 else 2)
         hb = hit_buffers(W * H, dev)
         parity_bvh.trace_pinhole_closest(W, H, c, hb["t"], hb["prim_id"], hb["u"], hb["v"])
         torch.cuda.synchronize()

@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(HERE, "libcudabvh_hip.so")
 
 # Ray classes and wave schedules of the launch configuration / autotuner
 CLASS_COHERENT, CLASS_INCOHERENT, CLASS_ANY = 0, 1, 2
-SCHEDULE_BATCH, SCHEDULE_WHILE_WHILE = 0, 1
+SCHEDULE_BATCH, SCHEDULE_WHILE_WHILE, SCHEDULE_PACKET = 0, 1, 2
 
 # Flags (include/cudabvh.h, part 2)
 ISECT_WATERTIGHT = 0
@@ -347,7 +347,7 @@ class BVH:
                  rays=None):
         """Sweeps the launch configurations of one ray class; returns the
         chosen (schedule, lds_depth, blocks_per_cu) and the table of timings."""
-        res = (C.c_float * 96)()
+        res = (C.c_float * 144)()
         nres = C.c_int()
         n = 0 if rays is None else int(rays["ox"].shape[0])
         rp = ray_packet(rays) if rays is not None else None

@@ -683,8 +683,9 @@ int cudaBVH_traceAnyQueued(CudaBVH_t bvh, int capacity, const int* d_count, cons
 int cudaBVH_setTraceConfig(CudaBVH_t bvh, int rayClass, int schedule, int ldsDepth, int blocksPerCU) {
     rt::clear_error();
     if (!check_handle(bvh, "cudaBVH_setTraceConfig")) return -1;
-    if (rayClass < 0 || rayClass >= rt::kNumClasses || (schedule != rt::kScheduleBatch &&
-        schedule != rt::kScheduleWhileWhile) || (ldsDepth != 4 && ldsDepth != 8 && ldsDepth != 12 &&
+    if (rayClass < 0 || rayClass >= rt::kNumClasses || schedule < rt::kScheduleBatch ||
+        schedule > rt::kSchedulePacket || (schedule == rt::kSchedulePacket && rayClass != rt::kClassCoherent) ||
+        (ldsDepth != 4 && ldsDepth != 8 && ldsDepth != 12 &&
         ldsDepth != 16) || blocksPerCU < 0) {
         set_error("cudaBVH_setTraceConfig: bad class %d / schedule %d / ldsDepth %d / blocksPerCU %d", rayClass,
                   schedule, ldsDepth, blocksPerCU);
@@ -707,7 +708,7 @@ int cudaBVH_getTraceConfig(CudaBVH_t bvh, int rayClass, int* schedule, int* ldsD
 // Autotuner: times every candidate launch configuration of one ray class on
 // the caller's workload (a pinhole frame for the coherent class, a device ray
 // packet otherwise) and keeps the fastest in the handle. results (nullable,
-// room for 4 floats x 24 candidates) receives {schedule, ldsDepth,
+// room for 4 floats x 36 candidates) receives {schedule, ldsDepth,
 // blocksPerCU, milliseconds} per candidate.
 int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const CudaCamera* cam, int numRays,
                      const CudaRayPacket* rays, int reps, float* results, int* numResults) {
@@ -734,7 +735,7 @@ int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const C
     if (rayClass == rt::kClassAny) base.occ = (uint8_t*)out;
     else { base.t = o; base.id = (int*)(o + n); base.u = o + 2 * size_t(n); base.v = o + 3 * size_t(n); }
     const int mode = rayClass == rt::kClassAny ? rtk::kModeAny : rtk::kModeClosest;
-    const int schedules[2] = {rt::kScheduleBatch, rt::kScheduleWhileWhile};
+    const int schedules[3] = {rt::kScheduleBatch, rt::kScheduleWhileWhile, rt::kSchedulePacket};
     const int depths[4] = {4, 8, 12, 16};
     const int bpcs[3] = {0, 4, 2};
     const rt::TraceConfig saved = bvh->tune[rayClass];
@@ -744,6 +745,7 @@ int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const C
     hipEvent_t e0, e1;
     hipEventCreate(&e0); hipEventCreate(&e1);
     for (int sch : schedules) {
+        if (sch == rt::kSchedulePacket && !coherent) continue;
         for (int d : depths) {
             for (int b : bpcs) {
                 bvh->tune[rayClass].schedule = sch;

@@ -63,11 +63,12 @@ uint32_t morton3d(float x, float y, float z);
 // construction) and the per-lane stack spill region (grown on demand).
 // Launch configuration of one ray class, chosen by the autotuner: the wave
 // schedule (batch: a wave keeps 64 rays until all finish; while-while: node
-// and leaf phases with idle lanes refilled from a wave-local work pool), the
+// and leaf phases with idle lanes refilled from a wave-local work pool;
+// packet: one shared stack per wave of coherent pinhole rays), the
 // LDS stack depth per lane (4, 8, 12 or 16 entries) and resident workgroups
 // per CU (0 = as many as occupancy allows).
 enum { kClassCoherent = 0, kClassIncoherent = 1, kClassAny = 2, kNumClasses = 3 };
-enum { kScheduleBatch = 0, kScheduleWhileWhile = 1 };
+enum { kScheduleBatch = 0, kScheduleWhileWhile = 1, kSchedulePacket = 2 };
 struct TraceConfig {
     int schedule = kScheduleBatch;
     int ldsDepth = 16;

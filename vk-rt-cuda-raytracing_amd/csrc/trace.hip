@@ -501,6 +501,159 @@ __global__ __launch_bounds__(BLOCK) void k_trace_ww(TraceArgs a) {
     }
 }
 
+// Wave-packet schedule for coherent pinhole rays: one wave per 8x8 pixel
+// tile shares one traversal stack. The node or leaf on top is wave-uniform,
+// so it is fetched with scalar loads and every lane runs the same
+// instructions; each stack slot keeps a per-lane entry distance, and a slot
+// whose box no lane still needs (entry beyond every lane's closest t) is
+// dropped without a fetch. Children are ordered near-first by the entry
+// distance of the lowest lane that hits them. A lane accepts a triangle hit
+// by the same rule as the per-ray traversal (tmin < t < tmax, lexicographic
+// (t, id) minimum), so a lane's answer equals the per-ray answer.
+template <int MODE, bool MT, int LDSD, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace_packet(TraceArgs a) {
+    constexpr int NW = BLOCK / 64;
+    __shared__ int s_ref[NW][LDSD];
+    __shared__ float s_tn[NW][LDSD][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int gwave = blockIdx.x * NW + w;
+    // Read-only structure seen through the constant address space, so that
+    // wave-uniform node and triangle reads become scalar loads.
+    typedef __attribute__((address_space(4))) const unsigned int cu32;
+    typedef __attribute__((address_space(4))) const float cf32;
+    const cu32* nodes = (const cu32*)a.nodes;
+    const cf32* tris = (const cf32*)a.tris;
+    const int n = a.nTiles * 64;
+
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(a.counter, 64u);
+        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+        if (base >= (uint32_t)n) break;
+        const int tile = (int)base >> 6;
+        const int ty = tile / a.tilesX, tx = tile - ty * a.tilesX;
+        const int px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
+        const bool live = px < a.W && py < a.H;
+        const int ri = py * a.W + px;
+        Ray r;
+        rtd::pinhole_ray(a.cam, live ? px : 0, live ? py : 0, a.W, a.H, r);
+        rtd::ray_setup(r);
+        const bool waveFlat = __any(live && rtd::ray_flat(r));
+        // a dead lane keeps best = -inf: it culls every box and accepts nothing
+        float best = live ? r.tmax : -INFINITY, bu = 0.f, bv = 0.f;
+        int bid = -1, bti = -1;
+        int sp = 0, ref = 0;
+        float tn = live ? -INFINITY : INFINITY;
+
+        for (;;) {
+            if (__any(tn <= best)) {
+                if (ref >= 0) {
+                    const cu32* np = nodes + 16 * (size_t)ref;
+                    const uint4 qx = make_uint4(np[0], np[1], np[2], np[3]);
+                    const uint4 qy = make_uint4(np[4], np[5], np[6], np[7]);
+                    const uint4 qz = make_uint4(np[8], np[9], np[10], np[11]);
+                    const uint4 qc = make_uint4(np[12], np[13], np[14], np[15]);
+                    const bool need = tn <= best;
+                    float d[4];
+                    int c[4];
+                    uint64_t m[4];
+                    float tt;
+                    bool h;
+#define RT_PCHILD(K, CH)                                                                                \
+    h = need && (int)CH != -1 &&                                                                        \
+        (waveFlat ? rtd::slab_child<K, true>(r, qx, qy, qz, best, tt)                                   \
+                  : rtd::slab_child<K, false>(r, qx, qy, qz, best, tt));                                \
+    d[K] = h ? tt : INFINITY;                                                                           \
+    m[K] = __ballot(h);                                                                                 \
+    c[K] = (int)CH;
+                    RT_PCHILD(0, qc.x) RT_PCHILD(1, qc.y) RT_PCHILD(2, qc.z) RT_PCHILD(3, qc.w)
+#undef RT_PCHILD
+                    // wave-uniform keys: entry distance of the lowest hitting lane
+                    float key[4];
+                    int slot[4] = {0, 1, 2, 3};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int l = m[k] ? (int)__builtin_ctzll(m[k]) : 0;
+                        key[k] = m[k] ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d[k]), l)) : INFINITY;
+                    }
+#define RT_USWAP(i, j)                                                                          \
+    {                                                                                           \
+        if (key[i] > key[j]) { float tk = key[i]; key[i] = key[j]; key[j] = tk;                 \
+                               int ts = slot[i]; slot[i] = slot[j]; slot[j] = ts; }             \
+    }
+                    RT_USWAP(0, 1) RT_USWAP(2, 3) RT_USWAP(0, 2) RT_USWAP(1, 3) RT_USWAP(1, 2)
+#undef RT_USWAP
+                    int nxt = -1;
+                    float ntn = INFINITY;
+#pragma unroll
+                    for (int k = 3; k >= 0; k--) {
+                        const int sl = slot[k];
+                        const uint64_t mk = sl == 0 ? m[0] : sl == 1 ? m[1] : sl == 2 ? m[2] : m[3];
+                        if (mk) {
+                            const float dk = sl == 0 ? d[0] : sl == 1 ? d[1] : sl == 2 ? d[2] : d[3];
+                            const int ck = sl == 0 ? c[0] : sl == 1 ? c[1] : sl == 2 ? c[2] : c[3];
+                            if (nxt != -1) {
+                                if (sp < LDSD) {
+                                    if (lane == 0) s_ref[w][sp] = nxt;
+                                    s_tn[w][sp][lane] = ntn;
+                                } else if (sp < a.maxStack) {
+                                    const size_t e = ((size_t)(sp - LDSD) * a.spillStride + gwave) * 64 + lane;
+                                    a.spill[e] = make_int2(nxt, __float_as_int(ntn));
+                                } else if (lane == 0) {
+                                    atomicOr(a.status, 1);
+                                }
+                                sp = sp < a.maxStack ? sp + 1 : sp;
+                            }
+                            nxt = ck;
+                            ntn = dk;
+                        }
+                    }
+                    if (nxt != -1) {
+                        ref = __builtin_amdgcn_readfirstlane(nxt);
+                        tn = ntn;
+                        continue;
+                    }
+                } else {
+                    const int enc = -(ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
+                    for (int k = 0; k < tc; k++) {
+                        const int ti = ts + k;
+                        const cf32* tp = tris + 12 * (size_t)ti;
+                        const float4 p0 = make_float4(tp[0], tp[1], tp[2], tp[3]);
+                        const float4 p1 = make_float4(tp[4], tp[5], tp[6], tp[7]);
+                        const float4 p2 = make_float4(tp[8], tp[9], tp[10], tp[11]);
+                        float t, u, v;
+                        if (!rtd::isect<MT>(r, p0, p1, p2, t, u, v)) continue;
+                        if (!(t > r.tmin) || !(t < r.tmax) || !(t <= best)) continue;
+                        const int id = __float_as_int(p0.w);
+                        if (t < best || id < bid) { best = t; bid = id; bu = u; bv = v; bti = ti; }
+                    }
+                }
+            }
+            if (sp == 0) break;
+            sp--;
+            if (sp < LDSD) {
+                ref = __builtin_amdgcn_readfirstlane(s_ref[w][sp]);
+                tn = s_tn[w][sp][lane];
+            } else {
+                const int2 e = a.spill[((size_t)(sp - LDSD) * a.spillStride + gwave) * 64 + lane];
+                ref = __builtin_amdgcn_readfirstlane(e.x);
+                tn = __int_as_float(e.y);
+            }
+        }
+        if (live) {
+            if constexpr (MODE == kModeClosest) {
+                const bool hit = bid >= 0;
+                a.t[ri] = hit ? best : RT_MISS_T;
+                if (a.id) a.id[ri] = bid;
+                if (a.u) a.u[ri] = hit ? bu : 0.f;
+                if (a.v) a.v[ri] = hit ? bv : 0.f;
+            } else {
+                shade_pixel(a, r, ri, py, bid >= 0, bti);
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Launch plumbing
 // ---------------------------------------------------------------------------
@@ -518,10 +671,12 @@ static int num_cus() {
     return n;
 }
 
-template <int MODE, bool MT, bool PIN, bool COUNT, int LDSD, bool WW>
-static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int blocksPerCU, hipStream_t s) {
-    auto kern = WW ? k_trace_ww<MODE, MT, PIN, COUNT, LDSD, kBlock> : k_trace_batch<MODE, MT, PIN, COUNT, LDSD, kBlock>;
-    static int maxBlocks = 0;
+// Shared launch path: grid = resident workgroups per CU x CUs (capped by the
+// kernel's occupancy), per-lane spill region of (maxStack - LDSD) entries.
+// Wave-packet kernels index the spill region per wave (spillStride = waves).
+template <class K>
+static int launch_kernel(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, K kern, int ldsd, bool perWave,
+                         int blocksPerCU, int& maxBlocks, hipStream_t s) {
     if (!maxBlocks) {
         int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, 0) != hipSuccess || nb <= 0) nb = 1;
@@ -531,7 +686,7 @@ static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a
     const int grid = bpc * num_cus();
     const int threads = grid * kBlock;
     a.maxStack = 3 * h->depth4 + 4;
-    const int spillDepth = a.maxStack > LDSD ? a.maxStack - LDSD : 0;
+    const int spillDepth = a.maxStack > ldsd ? a.maxStack - ldsd : 0;
     const size_t need = (size_t)spillDepth * threads * sizeof(int2);
     if (need > sc.spillBytes) {
         if (sc.spill) hipFree(sc.spill);
@@ -542,12 +697,12 @@ static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a
         }
         sc.spillBytes = need;
     }
-    if (PIN) {
+    if (a.W > 0) {
         a.tilesX = (a.W + 7) / 8;
         a.nTiles = a.tilesX * ((a.H + 7) / 8);
     }
     a.spill = (int2*)sc.spill;
-    a.spillStride = threads;
+    a.spillStride = perWave ? threads / 64 : threads;
     a.counter = sc.counter;
     a.status = sc.status;
     a.nodes = (const uint4*)h->d_nodes4;
@@ -557,6 +712,19 @@ static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { rt::set_error("trace launch failed: %s", hipGetErrorString(e)); return -1; }
     return 0;
+}
+
+template <int MODE, bool MT, bool PIN, bool COUNT, int LDSD, bool WW>
+static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int blocksPerCU, hipStream_t s) {
+    static int maxBlocks = 0;
+    auto kern = WW ? k_trace_ww<MODE, MT, PIN, COUNT, LDSD, kBlock> : k_trace_batch<MODE, MT, PIN, COUNT, LDSD, kBlock>;
+    return launch_kernel(h, sc, a, kern, LDSD, false, blocksPerCU, maxBlocks, s);
+}
+
+template <int MODE, bool MT, int LDSD>
+static int launch_packet(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int blocksPerCU, hipStream_t s) {
+    static int maxBlocks = 0;
+    return launch_kernel(h, sc, a, k_trace_packet<MODE, MT, LDSD, kBlock>, LDSD, true, blocksPerCU, maxBlocks, s);
 }
 
 int alloc_scratch(rt::TraceScratch& sc) {
@@ -593,6 +761,16 @@ static int launch_depth(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs&
 template <int MODE, bool PIN>
 static int launch_tuned(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, const rt::TraceConfig& cfg,
                         hipStream_t s) {
+    if constexpr (PIN && MODE == kModeClosest) {
+        if (cfg.schedule == rt::kSchedulePacket) {
+            switch (cfg.ldsDepth) {
+                case 4: return launch_packet<MODE, false, 4>(h, sc, a, cfg.blocksPerCU, s);
+                case 8: return launch_packet<MODE, false, 8>(h, sc, a, cfg.blocksPerCU, s);
+                case 12: return launch_packet<MODE, false, 12>(h, sc, a, cfg.blocksPerCU, s);
+                default: return launch_packet<MODE, false, 16>(h, sc, a, cfg.blocksPerCU, s);
+            }
+        }
+    }
     return cfg.schedule == rt::kScheduleWhileWhile ? launch_depth<MODE, PIN, true>(h, sc, a, cfg, s)
                                                    : launch_depth<MODE, PIN, false>(h, sc, a, cfg, s);
 }
@@ -619,7 +797,8 @@ int launch_trace(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int
                                : (counted ? RT_L(kModeAny, true, false, true) : RT_L(kModeAny, true, false, false));
         return pinhole ? RT_L(kModeAny, false, true, true) : RT_L(kModeAny, false, false, true);
     }
-    if (mode == kModeShade) return mt ? RT_L(kModeShade, true, true, false) : RT_L(kModeShade, false, true, false);
+    if (mode == kModeShade) return mt ? launch_packet<kModeShade, true, 16>(h, sc, a, 0, s)
+                                      : launch_packet<kModeShade, false, 16>(h, sc, a, 0, s);
 #undef RT_L
     rt::set_error("bad trace mode %d", mode);
     return -1;
