@@ -144,6 +144,27 @@ def main():
     log(f"primary {prim_ms:.3f} ms  bounce {bounce_ms:.3f} ms  diffuse {diff_ms:.3f} ms  shadow {shadow_ms:.3f} ms"
         f"  (wall {wall_s / args.steps * 1e3:.3f} ms/step incl. shadow)")
 
+    # Incoherent class with the ray sort (octant + Morton keys, GPU radix
+    # sort, gather, trace, scatter back) for comparison; reported, not in value.
+    sorted_cls = {}
+    with torch.cuda.stream(stream):
+        Dm = {k: v[:m] for k, v in D.items()}
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        for _ in range(2):
+            bvh.trace_closest(Dm, dt[:m], did[:m], du[:m], dv[:m], flags=pkg.FLAG_SORT_RAYS)
+        e[0].record(stream)
+        for _ in range(5):
+            bvh.trace_closest(Dm, dt[:m], did[:m], du[:m], dv[:m], flags=pkg.FLAG_SORT_RAYS)
+        e[1].record(stream)
+        for _ in range(5):
+            bvh.trace_closest(Dm, dt[:m], did[:m], du[:m], dv[:m])
+        e[2].record(stream)
+    torch.cuda.synchronize()
+    srt_ms, uns_ms = e[0].elapsed_time(e[1]) / 5, e[1].elapsed_time(e[2]) / 5
+    sorted_cls = {"rays": m, "sort_and_trace_ms": round(srt_ms, 4), "unsorted_trace_ms": round(uns_ms, 4),
+                  "mrays_s_incl_sort": round(m / (srt_ms * 1e3), 1)}
+    log(f"incoherent with ray sort: {srt_ms:.3f} ms (sort+trace) vs {uns_ms:.3f} ms unsorted")
+
     # Algorithmic bytes of the dominant kernel (incoherent closest hit) from
     # exact visit counts; the counted kernel's counts equal the oracle's.
     with torch.cuda.stream(stream):
@@ -198,6 +219,7 @@ def main():
                                    "hit_rate": round(diff_hit, 4), "nodes_per_ray": round(d_nodes / max(m, 1), 2),
                                    "tris_per_ray": round(d_tris / max(m, 1), 2)},
             "bounce_generation": {"ms": round(bounce_ms, 4)},
+            "incoherent_closest_sorted": sorted_cls,
             "shadow_any": {"rays": m, "ms": round(shadow_ms, 4), "mrays_s": round(m / (shadow_ms * 1e3), 1),
                            "occluded": round(occ_rate, 4)},
         },

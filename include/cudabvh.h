@@ -50,10 +50,10 @@ float cudaBVH_tracePrimary(CudaBVH_t bvh, int side,
                            float camOx, float camOy, float camOz,
                            float* outHitT);
 
-/* layer/cuda_bvh_backend.h:37-40. Host SoA rays (t in (0.001, 1e30) as
- * cuda_rt_v40.cu:550), sorted on the GPU by octant+Morton key, traced, hit
- * distances written to outHitT (host) in the original order. Returns MRays/s
- * over the trace kernel. */
+/* layer/cuda_bvh_backend.h:37-40. Host SoA rays (t in (0, 1e30) as backend
+ * :594), sorted on the GPU by octant+Morton key, traced, hit distances
+ * written to outHitT (host) in the original order. Returns MRays/s over the
+ * sort and trace kernels. */
 float cudaBVH_traceDiffuse(CudaBVH_t bvh, int numRays,
                            const float* rayOx, const float* rayOy, const float* rayOz,
                            const float* rayDx, const float* rayDy, const float* rayDz,
@@ -235,6 +235,12 @@ int cudaBVH_getTraceConfig(CudaBVH_t bvh, int rayClass, int* schedule, int* ldsD
  * blocksPerCU, ms per candidate) and *numResults report the sweep. */
 int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const CudaCamera* cam,
                      int numRays, const CudaRayPacket* rays, int reps, float* results, int* numResults);
+
+/* Ray sort of the incoherent path (device arrays): key = direction octant
+ * << 29 | low 29 bits of the 30-bit Morton code of the origin normalised by
+ * the scene box (cuda_rt_v29.cu:84-106); stable LSD radix sort. keysOut
+ * receives the sorted keys, permOut the ray index of each sorted slot. */
+int cudaBVH_sortRays(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays, uint32_t* keysOut, int* permOut);
 
 /* Handle stream: all work of the handle is issued on this HIP stream
  * (NULL = the null stream). */

@@ -128,11 +128,13 @@ static float sa_area(box3 b) {               /* backend :195-198 */
     float dx = b.mx[0] - b.mn[0], dy = b.mx[1] - b.mn[1], dz = b.mx[2] - b.mn[2];
     return 2.f * (dx * dy + dy * dz + dz * dx);
 }
-static box3 tri_box(const float* t) {        /* backend :187-190 */
+/* Triangle box (backend :187-190); "+ 0.0f" turns -0 into +0, so that the
+ * box never depends on which of two equal zeros a min/max returns. */
+static box3 tri_box(const float* t) {
     box3 b;
     for (int a = 0; a < 3; a++) {
-        b.mn[a] = fminf(fminf(t[a], t[3 + a]), t[6 + a]);
-        b.mx[a] = fmaxf(fmaxf(t[a], t[3 + a]), t[6 + a]);
+        b.mn[a] = fminf(fminf(t[a], t[3 + a]), t[6 + a]) + 0.0f;
+        b.mx[a] = fmaxf(fmaxf(t[a], t[3 + a]), t[6 + a]) + 0.0f;
     }
     return b;
 }

@@ -100,6 +100,7 @@ SIGNATURES = {
     "cudaBVH_setTraceConfig": (_I, [_H, _I, _I, _I, _I]),
     "cudaBVH_getTraceConfig": (_I, [_H, _I, _P, _P, _P]),
     "cudaBVH_autotune": (_I, [_H, _I, _I, _I, _P, _I, _P, _I, _P, _P]),
+    "cudaBVH_sortRays": (_I, [_H, _I, C.POINTER(CudaRayPacket), _P, _P]),
     "cudaBVH_setStream": (_I, [_H, _P]),
     "cudaBVH_getStream": (_P, [_H]),
     "cudaBVH_deviceNodes": (_P, [_H]),
@@ -331,6 +332,11 @@ class BVH:
         _check(self.lib.cudaBVH_traceAnyQueued(self.handle, capacity, _ptr(d_count), C.byref(rp), _ptr(occluded),
                                                flags), "traceAnyQueued")
 
+
+    def sort_rays(self, rays, keys_out, perm_out) -> None:
+        n = int(rays["ox"].shape[0])
+        rp = ray_packet(rays)
+        _check(self.lib.cudaBVH_sortRays(self.handle, n, C.byref(rp), _ptr(keys_out), _ptr(perm_out)), "sortRays")
 
     def set_trace_config(self, ray_class: int, schedule: int, lds_depth: int, blocks_per_cu: int = 0) -> None:
         _check(self.lib.cudaBVH_setTraceConfig(self.handle, ray_class, schedule, lds_depth, blocks_per_cu),

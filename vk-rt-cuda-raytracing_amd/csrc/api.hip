@@ -19,7 +19,6 @@
 
 namespace rt {
 const char* last_error();
-void build_sah_host(const float* tris, int n, int maxLeaf, HostBVH& out);
 }
 
 using rt::set_error;
@@ -36,7 +35,8 @@ using rt::set_error;
 static inline hipStream_t hstream(const CudaBVHHandle* h) { return (hipStream_t)h->stream; }
 
 static void free_handle_device(CudaBVHHandle* h) {
-    hipFree(h->d_nodes4); hipFree(h->d_tris); hipFree(h->d_leafOfPrim); hipFree(h->d_work);
+    hipFree(h->d_nodes4); hipFree(h->d_tris); hipFree(h->d_leafOfPrim); hipFree(h->d_work); hipFree(h->d_sortWs);
+    h->d_sortWs = nullptr; h->sortWsBytes = 0;
     rtk::free_scratch(h->scratch);
     h->d_nodes4 = h->d_tris = h->d_work = nullptr;
     h->d_leafOfPrim = nullptr;
@@ -60,45 +60,47 @@ static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf) {
     const float ex = bmax[0] - bmin[0], ey = bmax[1] - bmin[1], ez = bmax[2] - bmin[2];
     h->bboxExtent = sqrtf(ex * ex + ey * ey + ez * ez);
 
-    rt::HostBVH hb;
-    rt::build_sah_host(tri9, n, maxLeaf, hb);
-    h->numOrdered = int(hb.ordered.size());
-    h->numNodes4 = hb.numNodes4;
-    h->depth4 = hb.depth4;
-    h->h_nodes4 = std::move(hb.nodes4);
-    h->h_ordered = hb.ordered;
-
-    // Leaf-order triangles: {v0.xyz, bits(primId)}, {v1.xyz, 0}, {v2.xyz, 0}
-    const int m = h->numOrdered;
-    std::vector<float> aos(size_t(m) * 12);
-    std::vector<int> leafOf(n, -1);
-    for (int a = 0; a < 9; a++) h->h_soa[a].resize(m);
-    for (int i = 0; i < m; i++) {
-        const int p = hb.ordered[i];
-        const float* s = tri9 + 9 * size_t(p);
-        float* d = &aos[12 * size_t(i)];
-        d[0] = s[0]; d[1] = s[1]; d[2] = s[2]; memcpy(&d[3], &p, 4);
-        d[4] = s[3]; d[5] = s[4]; d[6] = s[5]; d[7] = 0.f;
-        d[8] = s[6]; d[9] = s[7]; d[10] = s[8]; d[11] = 0.f;
-        for (int a = 0; a < 9; a++) h->h_soa[a][i] = s[a];
-        leafOf[p] = i;
+    {
+        // Device build: upload the triangles once, build, keep the device
+        // structure; host copies serve the reference's getters.
+        hipStream_t s = hstream(h);
+        float* d_in = nullptr;
+        rt::GpuBVH g;
+        HIP_TRY(hipMalloc((void**)&d_in, size_t(n) * 36), goto fail);
+        HIP_TRY(hipMemcpyAsync(d_in, tri9, size_t(n) * 36, hipMemcpyHostToDevice, s), hipFree(d_in); goto fail);
+        if (rt::gpu_build(d_in, n, maxLeaf, s, g) != 0) {
+            hipFree(d_in); hipFree(g.nodes2); hipFree(g.nodes4); hipFree(g.tris); hipFree(g.leafOf); hipFree(g.ordered);
+            goto fail;
+        }
+        hipFree(d_in);
+        h->d_nodes4 = g.nodes4;
+        h->d_tris = g.tris;
+        h->d_leafOfPrim = g.leafOf;
+        h->numOrdered = n;
+        h->numNodes4 = g.numNodes4;
+        h->depth4 = g.depth4;
+        h->h_nodes4.resize(size_t(g.numNodes4) * 64);
+        h->h_ordered.resize(n);
+        std::vector<rt::Node2> nodes2(g.numNodes2);
+        hipError_t e = hipMemcpy(h->h_nodes4.data(), g.nodes4, h->h_nodes4.size(), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(h->h_ordered.data(), g.ordered, size_t(n) * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(nodes2.data(), g.nodes2, nodes2.size() * sizeof(rt::Node2), hipMemcpyDeviceToHost);
+        hipFree(g.nodes2); hipFree(g.ordered);
+        if (e != hipSuccess) { set_error("copying the built BVH back failed: %s", hipGetErrorString(e)); goto fail; }
+        // Host exports for the Vulkan-side consumers (backend :656-735)
+        const int m = n;
+        for (int a = 0; a < 9; a++) h->h_soa[a].resize(m);
+        h->h_packed.assign(size_t(m) * 12, 0.f);
+        for (int i = 0; i < m; i++) {
+            const int p = h->h_ordered[i];
+            const float* src = tri9 + 9 * size_t(p);
+            float* pk = &h->h_packed[12 * size_t(i)];
+            for (int a = 0; a < 9; a++) { h->h_soa[a][i] = src[a]; pk[a] = src[a]; }
+            pk[9] = (float)p;   // the value, not the bits (backend :730-733)
+        }
+        rt::export_dfs(nodes2, h->h_dfs);
+        h->numDFS = int(nodes2.size());
     }
-    // Host exports for the Vulkan-side consumers
-    rt::export_dfs(hb.nodes2, h->h_dfs);
-    h->numDFS = int(hb.nodes2.size());
-    h->h_packed.assign(size_t(m) * 12, 0.f);
-    for (int i = 0; i < m; i++) {
-        float* p = &h->h_packed[12 * size_t(i)];
-        for (int a = 0; a < 9; a++) p[a] = h->h_soa[a][i];
-        p[9] = (float)hb.ordered[i];   // the value, not the bits (backend :730-733)
-    }
-
-    HIP_TRY(hipMalloc(&h->d_nodes4, h->h_nodes4.size()), goto fail);
-    HIP_TRY(hipMalloc(&h->d_tris, aos.size() * sizeof(float)), goto fail);
-    HIP_TRY(hipMalloc((void**)&h->d_leafOfPrim, size_t(n) * sizeof(int)), goto fail);
-    HIP_TRY(hipMemcpy(h->d_nodes4, h->h_nodes4.data(), h->h_nodes4.size(), hipMemcpyHostToDevice), goto fail);
-    HIP_TRY(hipMemcpy(h->d_tris, aos.data(), aos.size() * sizeof(float), hipMemcpyHostToDevice), goto fail);
-    HIP_TRY(hipMemcpy(h->d_leafOfPrim, leafOf.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice), goto fail);
     if (rtk::alloc_scratch(h->scratch) != 0) goto fail;
     h->tune[rt::kClassIncoherent].schedule = rt::kScheduleWhileWhile;
     h->tune[rt::kClassIncoherent].ldsDepth = 8;
@@ -128,6 +130,17 @@ static void* work_buffer(CudaBVHHandle* h, size_t bytes) {
         h->workBytes = bytes;
     }
     return h->d_work;
+}
+
+static void* sort_workspace(CudaBVHHandle* h, int n) {
+    const size_t bytes = rtk::sort_scratch_bytes(n);
+    if (bytes > h->sortWsBytes) {
+        hipFree(h->d_sortWs);
+        h->d_sortWs = nullptr; h->sortWsBytes = 0;
+        if (hipMalloc(&h->d_sortWs, bytes) != hipSuccess) { set_error("cannot allocate %zu sort bytes", bytes); return nullptr; }
+        h->sortWsBytes = bytes;
+    }
+    return h->d_sortWs;
 }
 
 static rtd::Cam to_cam(const CudaCamera& c) {
@@ -230,22 +243,24 @@ float cudaBVH_traceDiffuse(CudaBVH_t bvh, int numRays, const float* rayOx, const
     std::fill(lim.begin(), lim.begin() + n, 0.0f);           // t > 0 (backend :594)
     std::fill(lim.begin() + n, lim.end(), RT_MISS_T);
     HIP_TRY(hipMemcpyAsync(w + 6 * n, lim.data(), 2 * n * 4, hipMemcpyHostToDevice, hstream(bvh)), return 0.f);
-    rtk::TraceArgs a;
-    a.n = numRays;
-    a.ox = w; a.oy = w + n; a.oz = w + 2 * n; a.dx = w + 3 * n; a.dy = w + 4 * n; a.dz = w + 5 * n;
-    a.tmin = w + 6 * n; a.tmax = w + 7 * n;
-    a.t = w + 8 * n;
+    // Rays are sorted on the GPU by octant + Morton code of the origin before
+    // the trace (the reference sorts by Morton code on the host, backend
+    // :1387-1433); hits come back in the caller's order.
+    void* ws = sort_workspace(bvh, numRays);
+    if (!ws) return 0.f;
+    CudaRayPacket rp{w, w + n, w + 2 * n, w + 3 * n, w + 4 * n, w + 5 * n, w + 6 * n, w + 7 * n};
+    CudaHitPacket hp{w + 8 * n, nullptr, nullptr, nullptr};
     hipEvent_t e0, e1;
     hipEventCreate(&e0); hipEventCreate(&e1);
     hipEventRecord(e0, hstream(bvh));
-    int rc = rtk::launch_trace(bvh, bvh->scratch, a, rtk::kModeClosest, false, false, false, hstream(bvh));
+    int rc = rtk::trace_sorted(bvh, bvh->scratch, numRays, rp, &hp, nullptr, rtk::kModeClosest, false, ws, hstream(bvh));
     hipEventRecord(e1, hstream(bvh));
     hipEventSynchronize(e1);
     float ms = 0.f;
     hipEventElapsedTime(&ms, e0, e1);
     hipEventDestroy(e0); hipEventDestroy(e1);
     if (rc != 0) return 0.f;
-    if (outHitT) HIP_TRY(hipMemcpy(outHitT, a.t, n * 4, hipMemcpyDeviceToHost), return 0.f);
+    if (outHitT) HIP_TRY(hipMemcpy(outHitT, w + 8 * n, n * 4, hipMemcpyDeviceToHost), return 0.f);
     return ms > 0.f ? (float)numRays / (ms * 1000.f) : 0.f;
 }
 
@@ -531,7 +546,8 @@ static int trace_packet(CudaBVH_t bvh, int n, const CudaRayPacket* rays, const C
     if (n < 0 || !packet_ok(rays)) { set_error("%s: bad ray packet", fn); return -1; }
     if (mode == rtk::kModeClosest && (!hits || !hits->t)) { set_error("%s: hit packet needs t", fn); return -1; }
     if (mode == rtk::kModeAny && !occ) { set_error("%s: NULL occlusion output", fn); return -1; }
-    if (flags & CUDABVH_FLAG_SORT_RAYS) { set_error("%s: CUDABVH_FLAG_SORT_RAYS is not available in this build", fn); return -1; }
+    const bool sorted = (flags & CUDABVH_FLAG_SORT_RAYS) != 0;
+    if (sorted && cn) { set_error("%s: counted traces do not sort", fn); return -1; }
     const unsigned isect = flags & CUDABVH_ISECT_MASK;
     if (isect > 1) { set_error("%s: unknown intersection mode %u", fn, isect); return -1; }
     if (n == 0) return 0;
@@ -545,7 +561,14 @@ static int trace_packet(CudaBVH_t bvh, int n, const CudaRayPacket* rays, const C
         a.tmin = rays->tmin; a.tmax = rays->tmax;
         if (hits) { a.t = hits->t; a.id = hits->primId; a.u = hits->u; a.v = hits->v; }
         a.occ = occ; a.cntNodes = cn; a.cntTris = ct;
-        int rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, isect == 1, false, cn != nullptr, s);
+        int rc;
+        if (sorted) {
+            void* ws = sort_workspace(bvh, n);
+            if (!ws) return -1;
+            rc = rtk::trace_sorted(bvh, bvh->scratch, n, *rays, hits, occ, mode, isect == 1, ws, s);
+        } else {
+            rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, isect == 1, false, cn != nullptr, s);
+        }
         if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(s), return -1);
         return rc;
     }
@@ -559,7 +582,16 @@ static int trace_packet(CudaBVH_t bvh, int n, const CudaRayPacket* rays, const C
     a.t = w + 8 * N; a.id = (int*)(w + 9 * N); a.u = w + 10 * N; a.v = w + 11 * N;
     a.cntNodes = (int*)(w + 12 * N); a.cntTris = (int*)(w + 13 * N);
     a.occ = (uint8_t*)(w + 14 * N);
-    int rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, isect == 1, false, cn != nullptr, s);
+    int rc;
+    if (sorted) {
+        void* ws = sort_workspace(bvh, n);
+        if (!ws) return -1;
+        CudaRayPacket rp{a.ox, a.oy, a.oz, a.dx, a.dy, a.dz, a.tmin, a.tmax};
+        CudaHitPacket hp{a.t, a.id, a.u, a.v};
+        rc = rtk::trace_sorted(bvh, bvh->scratch, n, rp, &hp, a.occ, mode, isect == 1, ws, s);
+    } else {
+        rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, isect == 1, false, cn != nullptr, s);
+    }
     if (rc != 0) return rc;
     if (mode == rtk::kModeClosest) {
         HIP_TRY(hipMemcpyAsync(hits->t, a.t, N * 4, hipMemcpyDeviceToHost, s), return -1);
@@ -780,6 +812,15 @@ int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const C
     bvh->tune[rayClass] = rc == 0 ? best : saved;
     if (numResults) *numResults = row;
     return rc;
+}
+
+int cudaBVH_sortRays(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays, uint32_t* keysOut, int* permOut) {
+    rt::clear_error();
+    if (!check_handle(bvh, "cudaBVH_sortRays")) return -1;
+    if (numRays <= 0 || !packet_ok(rays) || !keysOut || !permOut) { set_error("cudaBVH_sortRays: bad arguments"); return -1; }
+    void* ws = sort_workspace(bvh, numRays);
+    if (!ws) return -1;
+    return rtk::sort_rays(bvh, numRays, *rays, keysOut, permOut, ws, hstream(bvh));
 }
 
 }  // extern "C"

@@ -1,0 +1,671 @@
+// build.hip — binned-SAH BVH2 -> BVH4 build on the GPU.
+//
+// The algorithm is the reference's host builder (layer/cuda_bvh_backend.cu
+// :201-251: 16 centroid bins per axis, bin = floor((c-cmin)/(cmax-cmin)*15),
+// cost = nL*SA(L)/SA(P) + nR*SA(R)/SA(P) + 1, leaves of <= maxLeaf
+// triangles) with two rules made exact: the partition is taken on the
+// evaluated bin boundary and is stable, and a node whose centroids coincide on
+// every axis splits at the middle of its range. Under those rules the tree is
+// a pure function of the input, and this level-synchronous GPU build yields
+// bit for bit the tree of a recursive host build:
+//
+//  * per level, every primitive of an unfinished node is binned on the three
+//    axes with atomics (box min/max as order-preserving integers, which makes
+//    the result independent of atomic order; LDS-privatised when a workgroup's
+//    chunk lies in one node);
+//  * one thread per node evaluates the 45 SAH candidates in the host order;
+//  * a device-wide exclusive scan of the go-left flags gives every primitive
+//    its stable position in the child range;
+//  * child boxes and centroid bounds are min/max reductions over the moved
+//    primitives;
+//  * finally nodes are renumbered in depth-first pre-order (subtree sizes
+//    bottom-up, offsets top-down), and BVH4 nodes are written for the internal
+//    BVH2 nodes at even depth (backend :391-418: every internal child is
+//    opened into its two children), FP16 bounds rounded outward.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "rt_internal.h"
+#include "rt_scan.h"
+
+namespace rtb {
+
+constexpr int kBins = 16;
+constexpr int kBinInts = 7;                 // box min xyz, max xyz (ordered ints), count
+constexpr int kAxisInts = kBins * kBinInts;
+constexpr int kSlotInts = 3 * kAxisInts;
+constexpr int kThreads = 256;
+constexpr int kChunk = 2048;                // primitives per workgroup in the binning passes
+
+__device__ __forceinline__ int f2o(float f) {
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7fffffff;
+}
+__device__ __forceinline__ float o2f(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7fffffff); }
+constexpr int kOrdPosInf = 0x7f800000;      // f2o(+inf)
+constexpr int kOrdNegInf = (int)0x807fffff; // f2o(-inf)
+
+__device__ __forceinline__ float area(float x0, float y0, float z0, float x1, float y1, float z1) {
+    const float dx = x1 - x0, dy = y1 - y0, dz = z1 - z0;
+    return 2.f * (dx * dy + dy * dz + dz * dx);
+}
+
+__device__ __forceinline__ int bin_of(float c, float cmin, float cmax) {
+    int b = (int)((c - cmin) / (cmax - cmin) * (float)(kBins - 1));
+    return b < 0 ? 0 : (b >= kBins ? kBins - 1 : b);
+}
+
+struct Build {
+    int n, maxLeaf;
+    const float* pbox;      // [n][6] primitive boxes, original index
+    const float4* cen;      // [n] centroids
+    int* idx;               // [n] primitive at position
+    int* pnode;             // [n] node of position
+    int* idx2;
+    int* pnode2;
+    int* flag;              // [n] go-left flag
+    int* rank;              // [n] exclusive scan of flag
+    int* nbox;              // [cap][6] ordered ints
+    int* ncen;              // [cap][6] ordered ints (centroid bounds)
+    int* ns;                // [cap] range start
+    int* ncnt;              // [cap] primitive count
+    int* nleft;             // [cap] -1 for a leaf
+    int* nright;
+    int* ndepth;
+    int* nodeSlot;          // [cap] slot of an active node
+    int* act;               // [n] active nodes of the level
+    int* bins;              // [m][3][16][7]
+    int* sAxis;             // [m] split axis (-1: middle split)
+    int* sBin;
+    int* sNl;               // left count
+};
+
+// ---------------------------------------------------------------------------
+__global__ void k_prims(const float* __restrict__ tris, int n, float* pbox, float4* cen, int* idx, int* pnode) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* t = tris + 9 * (size_t)i;
+    float mn[3], mx[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        mn[a] = fminf(fminf(t[a], t[3 + a]), t[6 + a]) + 0.0f;   // + 0: -0 -> +0
+        mx[a] = fmaxf(fmaxf(t[a], t[3 + a]), t[6 + a]) + 0.0f;
+        pbox[6 * (size_t)i + a] = mn[a];
+        pbox[6 * (size_t)i + 3 + a] = mx[a];
+    }
+    cen[i] = make_float4((mn[0] + mx[0]) * .5f, (mn[1] + mx[1]) * .5f, (mn[2] + mx[2]) * .5f, 0.f);
+    idx[i] = i;
+    pnode[i] = 0;
+}
+
+__global__ void k_init_root(Build b) {
+    b.ns[0] = 0; b.ncnt[0] = b.n; b.nleft[0] = -1; b.nright[0] = -1; b.ndepth[0] = 0;
+    b.nodeSlot[0] = 0; b.act[0] = 0;
+}
+
+__global__ void k_init_nodes(int* nbox, int* ncen, int first, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const size_t o = 6 * (size_t)(first + i);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        nbox[o + a] = kOrdPosInf; nbox[o + 3 + a] = kOrdNegInf;
+        ncen[o + a] = kOrdPosInf; ncen[o + 3 + a] = kOrdNegInf;
+    }
+}
+
+// Box and centroid bounds of every node at or above levelBase, reduced over
+// its primitives (LDS first when the chunk lies in one node).
+__global__ __launch_bounds__(kThreads) void k_node_bounds(Build b, int levelBase) {
+    __shared__ int lb[12];
+    __shared__ int uni;
+    const int c0 = blockIdx.x * kChunk, c1 = min(c0 + kChunk, b.n);
+    if (threadIdx.x == 0) {
+        const int a = b.pnode[c0], z = b.pnode[c1 - 1];
+        uni = (a == z && a >= levelBase) ? a : -1;
+    }
+    if (threadIdx.x < 12) lb[threadIdx.x] = (threadIdx.x % 6) < 3 ? kOrdPosInf : kOrdNegInf;
+    __syncthreads();
+    const int u = uni;
+    for (int i = c0 + threadIdx.x; i < c1; i += kThreads) {
+        const int node = b.pnode[i];
+        if (node < levelBase) continue;
+        const int p = b.idx[i];
+        const float* pb = b.pbox + 6 * (size_t)p;
+        const float4 c = b.cen[p];
+        const float cv[3] = {c.x, c.y, c.z};
+        if (u >= 0) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                atomicMin(&lb[a], f2o(pb[a])); atomicMax(&lb[3 + a], f2o(pb[3 + a]));
+                atomicMin(&lb[6 + a], f2o(cv[a])); atomicMax(&lb[9 + a], f2o(cv[a]));
+            }
+        } else {
+            int* nb = b.nbox + 6 * (size_t)node;
+            int* nc = b.ncen + 6 * (size_t)node;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                atomicMin(&nb[a], f2o(pb[a])); atomicMax(&nb[3 + a], f2o(pb[3 + a]));
+                atomicMin(&nc[a], f2o(cv[a])); atomicMax(&nc[3 + a], f2o(cv[a]));
+            }
+        }
+    }
+    __syncthreads();
+    if (u >= 0 && threadIdx.x < 12) {
+        const int k = threadIdx.x;
+        int* dst = (k < 6 ? b.nbox : b.ncen) + 6 * (size_t)u + (k % 6);
+        if ((k % 6) < 3) atomicMin(dst, lb[k]); else atomicMax(dst, lb[k]);
+    }
+}
+
+__global__ void k_clear_bins(int* bins, int m) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m * kSlotInts) return;
+    const int f = i % kBinInts;
+    bins[i] = f < 3 ? kOrdPosInf : (f < 6 ? kOrdNegInf : 0);
+}
+
+__device__ __forceinline__ void bin_prim(const Build& b, int node, int p, int* dst /*[3][16][7]*/) {
+    const float* pb = b.pbox + 6 * (size_t)p;
+    const float4 c = b.cen[p];
+    const float cv[3] = {c.x, c.y, c.z};
+    const int* nc = b.ncen + 6 * (size_t)node;
+    int bx[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) bx[k] = f2o(pb[k]);
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        const float cmin = o2f(nc[ax]), cmax = o2f(nc[3 + ax]);
+        if (cmax - cmin < 1e-8f) continue;
+        int* d = dst + ax * kAxisInts + bin_of(cv[ax], cmin, cmax) * kBinInts;
+#pragma unroll
+        for (int k = 0; k < 3; k++) { atomicMin(&d[k], bx[k]); atomicMax(&d[3 + k], bx[3 + k]); }
+        atomicAdd(&d[6], 1);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_bin(Build b) {
+    __shared__ int lb[kSlotInts];
+    __shared__ int uni;
+    const int c0 = blockIdx.x * kChunk, c1 = min(c0 + kChunk, b.n);
+    if (threadIdx.x == 0) {
+        const int a = b.pnode[c0], z = b.pnode[c1 - 1];
+        uni = (a == z && b.ncnt[a] > b.maxLeaf) ? a : -1;
+    }
+    for (int i = threadIdx.x; i < kSlotInts; i += kThreads) {
+        const int f = i % kBinInts;
+        lb[i] = f < 3 ? kOrdPosInf : (f < 6 ? kOrdNegInf : 0);
+    }
+    __syncthreads();
+    const int u = uni;
+    for (int i = c0 + threadIdx.x; i < c1; i += kThreads) {
+        const int node = b.pnode[i];
+        if (b.ncnt[node] <= b.maxLeaf) continue;
+        if (u >= 0) bin_prim(b, node, b.idx[i], lb);
+        else bin_prim(b, node, b.idx[i], b.bins + (size_t)b.nodeSlot[node] * kSlotInts);
+    }
+    __syncthreads();
+    if (u >= 0) {
+        int* g = b.bins + (size_t)b.nodeSlot[u] * kSlotInts;
+        for (int i = threadIdx.x; i < kSlotInts; i += kThreads) {
+            const int f = i % kBinInts;
+            if (f == 6) { if (lb[i]) atomicAdd(&g[i], lb[i]); }
+            else if (f < 3) { if (lb[i] != kOrdPosInf) atomicMin(&g[i], lb[i]); }
+            else if (lb[i] != kOrdNegInf) atomicMax(&g[i], lb[i]);
+        }
+    }
+}
+
+// One thread per active node: SAH over 3 axes x 15 boundaries in the host
+// order (axis-major, strict <), then the two children.
+__global__ void k_sah(Build b, int m, int childBase) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= m) return;
+    const int node = b.act[slot];
+    const int* nb = b.nbox + 6 * (size_t)node;
+    const int* nc = b.ncen + 6 * (size_t)node;
+    const float pA = area(o2f(nb[0]), o2f(nb[1]), o2f(nb[2]), o2f(nb[3]), o2f(nb[4]), o2f(nb[5]));
+    const int cnt = b.ncnt[node];
+    float bestCost = 1e30f;
+    int bestAxis = -1, bestBin = -1, bestNl = 0;
+    const int* bins = b.bins + (size_t)slot * kSlotInts;
+    for (int ax = 0; ax < 3; ax++) {
+        const float cmin = o2f(nc[ax]), cmax = o2f(nc[3 + ax]);
+        if (cmax - cmin < 1e-8f) continue;
+        const int* B = bins + ax * kAxisInts;
+        float R[kBins][6];
+        int rc[kBins];
+        for (int k = 0; k < 6; k++) R[kBins - 1][k] = o2f(B[(kBins - 1) * kBinInts + k]);
+        rc[kBins - 1] = B[(kBins - 1) * kBinInts + 6];
+        for (int q = kBins - 2; q >= 0; q--) {
+            for (int k = 0; k < 3; k++) {
+                R[q][k] = fminf(R[q + 1][k], o2f(B[q * kBinInts + k]));
+                R[q][3 + k] = fmaxf(R[q + 1][3 + k], o2f(B[q * kBinInts + 3 + k]));
+            }
+            rc[q] = rc[q + 1] + B[q * kBinInts + 6];
+        }
+        float L[6];
+        for (int k = 0; k < 6; k++) L[k] = o2f(B[k]);
+        int lc = B[6];
+        for (int q = 0; q < kBins - 1; q++) {
+            if (q > 0) {
+                for (int k = 0; k < 3; k++) {
+                    L[k] = fminf(L[k], o2f(B[q * kBinInts + k]));
+                    L[3 + k] = fmaxf(L[3 + k], o2f(B[q * kBinInts + 3 + k]));
+                }
+                lc += B[q * kBinInts + 6];
+            }
+            if (lc == 0 || rc[q + 1] == 0) continue;
+            const float cost = (float)lc * area(L[0], L[1], L[2], L[3], L[4], L[5]) / pA +
+                               (float)rc[q + 1] * area(R[q + 1][0], R[q + 1][1], R[q + 1][2], R[q + 1][3],
+                                                        R[q + 1][4], R[q + 1][5]) / pA + 1.f;
+            if (cost < bestCost) { bestCost = cost; bestAxis = ax; bestBin = q; bestNl = lc; }
+        }
+    }
+    if (bestAxis < 0) bestNl = cnt / 2;                      // coincident centroids: middle split
+    b.sAxis[slot] = bestAxis;
+    b.sBin[slot] = bestBin;
+    b.sNl[slot] = bestNl;
+    const int L = childBase + 2 * slot, R = L + 1;
+    const int s = b.ns[node], d = b.ndepth[node] + 1;
+    b.nleft[node] = L; b.nright[node] = R;
+    b.ns[L] = s; b.ncnt[L] = bestNl; b.ns[R] = s + bestNl; b.ncnt[R] = cnt - bestNl;
+    b.ndepth[L] = d; b.ndepth[R] = d;
+    b.nleft[L] = b.nright[L] = b.nleft[R] = b.nright[R] = -1;
+}
+
+__global__ void k_flags(Build b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.n) return;
+    const int node = b.pnode[i];
+    int f = 0;
+    if (b.ncnt[node] > b.maxLeaf) {
+        const int slot = b.nodeSlot[node];
+        const int ax = b.sAxis[slot];
+        if (ax < 0) {
+            f = (i - b.ns[node]) < b.sNl[slot];
+        } else {
+            const float4 c = b.cen[b.idx[i]];
+            const float cv = ax == 0 ? c.x : (ax == 1 ? c.y : c.z);
+            const int* nc = b.ncen + 6 * (size_t)node;
+            f = bin_of(cv, o2f(nc[ax]), o2f(nc[3 + ax])) <= b.sBin[slot];
+        }
+    }
+    b.flag[i] = f;
+}
+
+__global__ void k_scatter(Build b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.n) return;
+    const int node = b.pnode[i];
+    if (b.ncnt[node] <= b.maxLeaf) {
+        b.idx2[i] = b.idx[i];
+        b.pnode2[i] = node;
+        return;
+    }
+    const int slot = b.nodeSlot[node];
+    const int s = b.ns[node], nl = b.sNl[slot];
+    const int lr = b.rank[i] - b.rank[s];
+    int pos, child;
+    if (b.flag[i]) { pos = s + lr; child = b.nleft[node]; }
+    else { pos = s + nl + ((i - s) - lr); child = b.nright[node]; }
+    b.idx2[pos] = b.idx[i];
+    b.pnode2[pos] = child;
+}
+
+__global__ void k_next_active(Build b, int first, int count, const int* rank, int* act2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int node = first + i;
+    if (b.ncnt[node] > b.maxLeaf) {
+        act2[rank[i]] = node;
+        b.nodeSlot[node] = rank[i];
+    }
+}
+
+__global__ void k_active_flags(const int* ncnt, int first, int count, int maxLeaf, int* f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) f[i] = ncnt[first + i] > maxLeaf;
+}
+
+// ------------------------------- pre-order renumbering -----------------------
+__global__ void k_sizes(const int* nleft, const int* nright, int* size, int first, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int v = first + i;
+    size[v] = nleft[v] < 0 ? 1 : 1 + size[nleft[v]] + size[nright[v]];
+}
+
+__global__ void k_preorder(const int* nleft, const int* nright, const int* size, int* pre, int first, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int v = first + i;
+    if (nleft[v] < 0) return;
+    pre[nleft[v]] = pre[v] + 1;
+    pre[nright[v]] = pre[v] + 1 + size[nleft[v]];
+}
+
+// Pre-order BVH2 (rt::Node2 layout: box[6], left, right, triStart, triCount)
+// and the head flags of the BVH4 collapse (internal nodes at even depth).
+__global__ void k_emit_bvh2(Build b, const int* pre, int count, rt::Node2* out, int* head, int* preDepth) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= count) return;
+    const int p = pre[v];
+    rt::Node2 nd;
+    for (int k = 0; k < 3; k++) {
+        nd.box.mn[k] = o2f(b.nbox[6 * (size_t)v + k]);
+        nd.box.mx[k] = o2f(b.nbox[6 * (size_t)v + 3 + k]);
+    }
+    const bool leaf = b.nleft[v] < 0;
+    nd.left = leaf ? 0 : pre[b.nleft[v]];
+    nd.right = leaf ? 0 : pre[b.nright[v]];
+    nd.triStart = leaf ? b.ns[v] : 0;
+    nd.triCount = leaf ? b.ncnt[v] : 0;
+    out[p] = nd;
+    head[p] = (!leaf && (b.ndepth[v] & 1) == 0) ? 1 : 0;
+    preDepth[p] = b.ndepth[v];
+}
+
+__device__ __forceinline__ uint16_t h_bits(_Float16 h) { return __builtin_bit_cast(uint16_t, h); }
+__device__ __forceinline__ float h_val(uint16_t u) { return (float)__builtin_bit_cast(_Float16, u); }
+
+// Largest half <= x / smallest half >= x: round to nearest even, then step
+// once when the rounding went the wrong way (the host rule of the oracle).
+__device__ __forceinline__ uint16_t f2h_down(float x) {
+    uint16_t h = h_bits((_Float16)x);
+    if (h_val(h) > x) {
+        if ((h & 0x7fffu) == 0) h = 0x8001u;
+        else if (h & 0x8000u) { if (h != 0xfc00u) h++; }
+        else h--;
+    }
+    return h;
+}
+__device__ __forceinline__ uint16_t f2h_up(float x) {
+    uint16_t h = h_bits((_Float16)x);
+    if (h_val(h) < x) {
+        if ((h & 0x7fffu) == 0) h = 0x0001u;
+        else if (h & 0x8000u) h--;
+        else if (h != 0x7c00u) h++;
+    }
+    return h;
+}
+
+__device__ __forceinline__ int32_t leaf_code(int first, int count) { return -((first << 3) | (count - 1)) - 2; }
+
+// One BVH4 node per head (pre-order position p with head[p]); b4id[p] is the
+// exclusive count of heads before p, i.e. the node's pre-order BVH4 index.
+__global__ void k_emit_bvh4(const rt::Node2* n2, const int* head, const int* b4id, const int* preDepth, int count,
+                            uint4* out, int* depth4) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= count || !head[p]) return;
+    const rt::Node2& nd = n2[p];
+    int gather[4], ng = 0;
+    const int ch[2] = {nd.left, nd.right};
+    for (int c = 0; c < 2; c++) {
+        const rt::Node2& cn = n2[ch[c]];
+        if (cn.triCount > 0) gather[ng++] = ch[c];
+        else { gather[ng++] = cn.left; gather[ng++] = cn.right; }
+    }
+    uint16_t hx[8], hy[8], hz[8];
+    int32_t child[4];
+    for (int i = 0; i < 4; i++) {
+        if (i < ng) {
+            const rt::Node2& g = n2[gather[i]];
+            hx[i] = f2h_down(g.box.mn[0]); hx[4 + i] = f2h_up(g.box.mx[0]);
+            hy[i] = f2h_down(g.box.mn[1]); hy[4 + i] = f2h_up(g.box.mx[1]);
+            hz[i] = f2h_down(g.box.mn[2]); hz[4 + i] = f2h_up(g.box.mx[2]);
+            child[i] = g.triCount > 0 ? leaf_code(g.triStart, g.triCount) : b4id[gather[i]];
+        } else {
+            hx[i] = hy[i] = hz[i] = 0x7c00u;
+            hx[4 + i] = hy[4 + i] = hz[4 + i] = 0xfc00u;
+            child[i] = -1;
+        }
+    }
+    uint4* o = out + 4 * (size_t)b4id[p];
+    o[0] = make_uint4(hx[0] | (uint32_t)hx[1] << 16, hx[2] | (uint32_t)hx[3] << 16, hx[4] | (uint32_t)hx[5] << 16,
+                      hx[6] | (uint32_t)hx[7] << 16);
+    o[1] = make_uint4(hy[0] | (uint32_t)hy[1] << 16, hy[2] | (uint32_t)hy[3] << 16, hy[4] | (uint32_t)hy[5] << 16,
+                      hy[6] | (uint32_t)hy[7] << 16);
+    o[2] = make_uint4(hz[0] | (uint32_t)hz[1] << 16, hz[2] | (uint32_t)hz[3] << 16, hz[4] | (uint32_t)hz[5] << 16,
+                      hz[6] | (uint32_t)hz[7] << 16);
+    o[3] = make_uint4((uint32_t)child[0], (uint32_t)child[1], (uint32_t)child[2], (uint32_t)child[3]);
+    atomicMax(depth4, preDepth[p] / 2 + 1);
+}
+
+// A tree whose root is a leaf (n <= maxLeaf): one BVH4 node with one slot.
+__global__ void k_emit_leaf_root(const rt::Node2* n2, uint4* out, int* depth4) {
+    const rt::Node2& r = n2[0];
+    uint16_t h[3][8];
+    for (int a = 0; a < 3; a++) {
+        for (int i = 0; i < 4; i++) { h[a][i] = 0x7c00u; h[a][4 + i] = 0xfc00u; }
+        h[a][0] = f2h_down(r.box.mn[a]);
+        h[a][4] = f2h_up(r.box.mx[a]);
+    }
+    for (int a = 0; a < 3; a++)
+        out[a] = make_uint4(h[a][0] | (uint32_t)h[a][1] << 16, h[a][2] | (uint32_t)h[a][3] << 16,
+                            h[a][4] | (uint32_t)h[a][5] << 16, h[a][6] | (uint32_t)h[a][7] << 16);
+    out[3] = make_uint4((uint32_t)leaf_code(r.triStart, r.triCount), 0xffffffffu, 0xffffffffu, 0xffffffffu);
+    *depth4 = 1;
+}
+
+// Leaf-order triangles {v0.xyz, bits(id)}, {v1.xyz, 0}, {v2.xyz, 0} and the
+// inverse permutation.
+__global__ void k_leaf_tris(const float* __restrict__ tris, const int* idx, int n, float4* out, int* leafOf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = idx[i];
+    const float* s = tris + 9 * (size_t)p;
+    out[3 * (size_t)i] = make_float4(s[0], s[1], s[2], __int_as_float(p));
+    out[3 * (size_t)i + 1] = make_float4(s[3], s[4], s[5], 0.f);
+    out[3 * (size_t)i + 2] = make_float4(s[6], s[7], s[8], 0.f);
+    leafOf[p] = i;
+}
+
+// ------------------------------- scan ---------------------------------------
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kThreads * kScanItems;
+
+__global__ __launch_bounds__(kThreads) void k_tile_sums(const int* in, int n, int* tiles) {
+    __shared__ int lds[kThreads / 64 + 1];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int s = 0;
+    for (int k = 0; k < kScanItems; k++) s += (base + k < n) ? in[base + k] : 0;
+    int total;
+    rts::block_exclusive_scan<kThreads>(s, lds, &total);
+    if (threadIdx.x == 0) tiles[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tile_sums(int* tiles, int nb, int* total) {
+    __shared__ int lds[1024 / 64 + 1];
+    int carry = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = i < nb ? tiles[i] : 0;
+        int tot;
+        const int ex = rts::block_exclusive_scan<1024>(v, lds, &tot);
+        if (i < nb) tiles[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kThreads) void k_tile_scan(const int* in, int n, const int* tiles, int* out) {
+    __shared__ int lds[kThreads / 64 + 1];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems], s = 0;
+    for (int k = 0; k < kScanItems; k++) { v[k] = (base + k < n) ? in[base + k] : 0; s += v[k]; }
+    int total;
+    int run = tiles[blockIdx.x] + rts::block_exclusive_scan<kThreads>(s, lds, &total);
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+}  // namespace rtb
+
+namespace rt {
+
+// Device-wide exclusive scan of n ints (out may not alias in); *total (device)
+// receives the sum. tiles needs ceil(n / 2048) ints.
+void scan_exclusive(const int* in, int* out, int n, int* total, int* tiles, hipStream_t s) {
+    const int nb = (n + rtb::kScanTile - 1) / rtb::kScanTile;
+    if (nb > 0) hipLaunchKernelGGL(rtb::k_tile_sums, dim3(nb), dim3(rtb::kThreads), 0, s, in, n, tiles);
+    hipLaunchKernelGGL(rtb::k_scan_tile_sums, dim3(1), dim3(1024), 0, s, tiles, nb, total);
+    if (nb > 0) hipLaunchKernelGGL(rtb::k_tile_scan, dim3(nb), dim3(rtb::kThreads), 0, s, in, n, tiles, out);
+}
+
+int scan_tiles_needed(int n) { return (n + rtb::kScanTile - 1) / rtb::kScanTile + 1; }
+
+struct DevBuf {
+    std::vector<void*> ptrs;
+    template <class T> T* get(size_t count) {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return (T*)p;
+    }
+    ~DevBuf() { for (void* p : ptrs) hipFree(p); }
+};
+
+#define BUILD_CHECK(cond, msg)              \
+    do {                                    \
+        if (!(cond)) { set_error("%s", msg); return -1; } \
+    } while (0)
+
+// Builds the BVH on the device from d_tris (n x 9 floats, original order).
+// Outputs (device, allocated here): nodes4 (numNodes4 x 64 B), leaf-order
+// triangles (n x 48 B), leafOf (n ints), ordered (n ints, leaf order ->
+// original index), nodes2 (pre-order rt::Node2). Returns 0 on success.
+int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& out) {
+    using namespace rtb;
+    DevBuf tmp;
+    Build b{};
+    b.n = n; b.maxLeaf = maxLeaf;
+    const size_t cap = 2 * (size_t)n;
+    float* pbox = tmp.get<float>(6 * (size_t)n);
+    float4* cen = tmp.get<float4>(n);
+    b.pbox = pbox; b.cen = cen;
+    b.idx = tmp.get<int>(n); b.pnode = tmp.get<int>(n);
+    b.idx2 = tmp.get<int>(n); b.pnode2 = tmp.get<int>(n);
+    b.flag = tmp.get<int>(n + 1); b.rank = tmp.get<int>(n + 1);
+    b.nbox = tmp.get<int>(6 * cap); b.ncen = tmp.get<int>(6 * cap);
+    b.ns = tmp.get<int>(cap); b.ncnt = tmp.get<int>(cap);
+    b.nleft = tmp.get<int>(cap); b.nright = tmp.get<int>(cap); b.ndepth = tmp.get<int>(cap);
+    b.nodeSlot = tmp.get<int>(cap);
+    b.act = tmp.get<int>(n); int* act2 = tmp.get<int>(n);
+    b.sAxis = tmp.get<int>(n); b.sBin = tmp.get<int>(n); b.sNl = tmp.get<int>(n);
+    int* tiles = tmp.get<int>(scan_tiles_needed((int)cap) + 8);
+    int* dcount = tmp.get<int>(4);
+    BUILD_CHECK(pbox && cen && b.idx && b.pnode && b.idx2 && b.pnode2 && b.flag && b.rank && b.nbox && b.ncen &&
+                b.ns && b.ncnt && b.nleft && b.nright && b.ndepth && b.nodeSlot && b.act && act2 && b.sAxis &&
+                b.sBin && b.sNl && tiles && dcount, "gpu build: out of device memory");
+
+    const int T = kThreads;
+    auto grid = [](long long items) { return dim3((unsigned)((items + kThreads - 1) / kThreads)); };
+    const int chunks = (n + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(k_prims, grid(n), dim3(T), 0, s, d_tris, n, pbox, cen, b.idx, b.pnode);
+    hipLaunchKernelGGL(k_init_root, dim3(1), dim3(1), 0, s, b);
+    hipLaunchKernelGGL(k_init_nodes, dim3(1), dim3(T), 0, s, b.nbox, b.ncen, 0, 1);
+    hipLaunchKernelGGL(k_node_bounds, dim3(chunks), dim3(T), 0, s, b, 0);
+
+    std::vector<int> levelStart = {0, 1};     // node ids of level L: [levelStart[L], levelStart[L+1])
+    int nodeCount = 1;
+    int m = n > maxLeaf ? 1 : 0;
+    int* bins = nullptr;
+    size_t binsCap = 0;
+    while (m > 0) {
+        if ((size_t)m > binsCap) {
+            if (bins) hipFree(bins);
+            binsCap = std::max<size_t>((size_t)m, 2 * binsCap);
+            if (hipMalloc((void**)&bins, binsCap * kSlotInts * sizeof(int)) != hipSuccess) {
+                set_error("gpu build: out of device memory for bins");
+                return -1;
+            }
+        }
+        b.bins = bins;
+        const int childBase = nodeCount;
+        hipLaunchKernelGGL(k_clear_bins, grid((long long)m * kSlotInts), dim3(T), 0, s, bins, m);
+        hipLaunchKernelGGL(k_bin, dim3(chunks), dim3(T), 0, s, b);
+        hipLaunchKernelGGL(k_sah, grid(m), dim3(T), 0, s, b, m, childBase);
+        hipLaunchKernelGGL(k_flags, grid(n), dim3(T), 0, s, b);
+        scan_exclusive(b.flag, b.rank, n, dcount, tiles, s);
+        hipLaunchKernelGGL(k_scatter, grid(n), dim3(T), 0, s, b);
+        std::swap(b.idx, b.idx2);
+        std::swap(b.pnode, b.pnode2);
+        hipLaunchKernelGGL(k_init_nodes, grid(2 * m), dim3(T), 0, s, b.nbox, b.ncen, childBase, 2 * m);
+        hipLaunchKernelGGL(k_node_bounds, dim3(chunks), dim3(T), 0, s, b, childBase);
+        // next level's active nodes: children with more than maxLeaf primitives, in id order
+        hipLaunchKernelGGL(k_active_flags, grid(2 * m), dim3(T), 0, s, b.ncnt, childBase, 2 * m, maxLeaf, b.flag);
+        scan_exclusive(b.flag, b.rank, 2 * m, dcount, tiles, s);
+        hipLaunchKernelGGL(k_next_active, grid(2 * m), dim3(T), 0, s, b, childBase, 2 * m, b.rank, act2);
+        std::swap(b.act, act2);
+        nodeCount += 2 * m;
+        levelStart.push_back(nodeCount);
+        int next = 0;
+        if (hipMemcpyAsync(&next, dcount, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            hipFree(bins);
+            set_error("gpu build: level %zu failed", levelStart.size() - 2);
+            return -1;
+        }
+        m = next;
+    }
+    if (bins) hipFree(bins);
+
+    // pre-order renumbering
+    int* size = tmp.get<int>(nodeCount);
+    int* pre = tmp.get<int>(nodeCount);
+    int* head = tmp.get<int>(nodeCount + 1);
+    int* b4id = tmp.get<int>(nodeCount + 1);
+    int* preDepth = tmp.get<int>(nodeCount);
+    BUILD_CHECK(size && pre && head && b4id && preDepth, "gpu build: out of device memory");
+    const int levels = (int)levelStart.size() - 1;
+    for (int L = levels - 1; L >= 0; L--) {
+        const int f = levelStart[L], c = levelStart[L + 1] - f;
+        hipLaunchKernelGGL(k_sizes, grid(c), dim3(T), 0, s, b.nleft, b.nright, size, f, c);
+    }
+    hipMemsetAsync(pre, 0, sizeof(int), s);
+    for (int L = 0; L < levels; L++) {
+        const int f = levelStart[L], c = levelStart[L + 1] - f;
+        hipLaunchKernelGGL(k_preorder, grid(c), dim3(T), 0, s, b.nleft, b.nright, size, pre, f, c);
+    }
+    out.numNodes2 = nodeCount;
+    if (hipMalloc((void**)&out.nodes2, (size_t)nodeCount * sizeof(Node2)) != hipSuccess) {
+        set_error("gpu build: out of device memory");
+        return -1;
+    }
+    hipLaunchKernelGGL(k_emit_bvh2, grid(nodeCount), dim3(T), 0, s, b, pre, nodeCount, out.nodes2, head, preDepth);
+    scan_exclusive(head, b4id, nodeCount, dcount, tiles, s);
+    int heads = 0;
+    if (hipMemcpyAsync(&heads, dcount, 4, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+        set_error("gpu build: renumbering failed");
+        return -1;
+    }
+    out.numNodes4 = std::max(heads, 1);
+    int* dDepth = dcount + 1;
+    hipMemsetAsync(dDepth, 0, sizeof(int), s);
+    if (hipMalloc(&out.nodes4, (size_t)out.numNodes4 * 64) != hipSuccess ||
+        hipMalloc(&out.tris, (size_t)n * 48) != hipSuccess || hipMalloc((void**)&out.leafOf, (size_t)n * 4) != hipSuccess ||
+        hipMalloc((void**)&out.ordered, (size_t)n * 4) != hipSuccess) {
+        set_error("gpu build: out of device memory");
+        return -1;
+    }
+    if (heads > 0)
+        hipLaunchKernelGGL(k_emit_bvh4, grid(nodeCount), dim3(T), 0, s, out.nodes2, head, b4id, preDepth, nodeCount,
+                           (uint4*)out.nodes4, dDepth);
+    else
+        hipLaunchKernelGGL(k_emit_leaf_root, dim3(1), dim3(1), 0, s, out.nodes2, (uint4*)out.nodes4, dDepth);
+    hipLaunchKernelGGL(k_leaf_tris, grid(n), dim3(T), 0, s, d_tris, b.idx, n, (float4*)out.tris, out.leafOf);
+    hipMemcpyAsync(out.ordered, b.idx, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
+    hipMemcpyAsync(&out.depth4, dDepth, 4, hipMemcpyDeviceToHost, s);
+    hipError_t e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) { set_error("gpu build failed: %s", hipGetErrorString(e)); return -1; }
+    out.levels = levels;
+    return 0;
+}
+
+}  // namespace rt

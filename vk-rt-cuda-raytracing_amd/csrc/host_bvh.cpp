@@ -1,6 +1,6 @@
 // host_bvh.cpp — host-side parts of the BVH pipeline: error reporting, FP16
-// conversion, BVH4 collapse and the wire-format exports the Vulkan layer
-// uploads (stackless BVH2, packed triangles, LBVH TLAS).
+// conversion and the wire-format exports the Vulkan layer uploads (stackless
+// BVH2, LBVH TLAS).
 #include "rt_internal.h"
 
 #include <cmath>
@@ -77,63 +77,6 @@ uint16_t f2h_up(float x) {
         else if (h != 0x7c00u) h++;
     }
     return h;
-}
-
-// ---------------------------------------------------------------------------
-// BVH2 -> BVH4 collapse. A BVH4 node stands for each internal BVH2 node at
-// even depth; its slots are the BVH2 grandchildren reached by opening every
-// internal child (a leaf child stays a slot). Pre-order numbering matches a
-// recursive collapse (backend :391-418).
-// ---------------------------------------------------------------------------
-void collapse_bvh4(HostBVH& b) {
-    const std::vector<Node2>& n2 = b.nodes2;
-    b.nodes4.assign(size_t(std::max<size_t>(n2.size(), 1)) * kNode4Bytes, 0);
-    struct Task { int n2, parent, slot, depth; };
-    std::vector<Task> st;
-    st.push_back({0, -1, 0, 1});
-    int cnt = 0, maxd = 0;
-    while (!st.empty()) {
-        Task t = st.back(); st.pop_back();
-        int me = cnt++;
-        maxd = std::max(maxd, t.depth);
-        if (t.parent >= 0) {
-            int32_t v = me;
-            memcpy(&b.nodes4[size_t(t.parent) * kNode4Bytes + 48 + 4 * t.slot], &v, 4);
-        }
-        int gather[4], ng = 0;
-        const Node2& nd = n2[t.n2];
-        if (nd.triCount > 0) {
-            gather[ng++] = t.n2;                 // single-leaf tree: one-slot root
-        } else {
-            for (int c : {nd.left, nd.right}) {
-                if (n2[c].triCount > 0) gather[ng++] = c;
-                else { gather[ng++] = n2[c].left; gather[ng++] = n2[c].right; }
-            }
-        }
-        uint16_t hx[8], hy[8], hz[8];
-        int32_t child[4];
-        for (int i = 0; i < 4; i++) {
-            if (i < ng) {
-                const Box& bx = n2[gather[i]].box;
-                hx[i] = f2h_down(bx.mn[0]); hx[4 + i] = f2h_up(bx.mx[0]);
-                hy[i] = f2h_down(bx.mn[1]); hy[4 + i] = f2h_up(bx.mx[1]);
-                hz[i] = f2h_down(bx.mn[2]); hz[4 + i] = f2h_up(bx.mx[2]);
-                const Node2& c = n2[gather[i]];
-                child[i] = c.triCount > 0 ? encode_leaf(c.triStart, c.triCount) : -1;
-            } else {
-                hx[i] = hy[i] = hz[i] = 0x7c00u;          // empty slot: (+inf, -inf)
-                hx[4 + i] = hy[4 + i] = hz[4 + i] = 0xfc00u;
-                child[i] = -1;
-            }
-        }
-        uint8_t* out = &b.nodes4[size_t(me) * kNode4Bytes];
-        memcpy(out, hx, 16); memcpy(out + 16, hy, 16); memcpy(out + 32, hz, 16); memcpy(out + 48, child, 16);
-        for (int i = ng - 1; i >= 0; i--)
-            if (n2[gather[i]].triCount == 0) st.push_back({gather[i], me, i, t.depth + 1});
-    }
-    b.nodes4.resize(size_t(cnt) * kNode4Bytes);
-    b.numNodes4 = cnt;
-    b.depth4 = maxd;
 }
 
 // Stackless export: the pre-order numbering is already the DFS order, and
@@ -232,108 +175,3 @@ int build_tlas_lbvh(const float* aabbs, int n, std::vector<uint32_t>& dfs, std::
 
 }  // namespace rt
 
-// ---------------------------------------------------------------------------
-// Binned-SAH BVH2 on the host (backend :201-251 with the partition taken on
-// the evaluated bin boundary and kept stable). Pre-order numbering.
-// ---------------------------------------------------------------------------
-namespace rt {
-
-static inline Box merge(const Box& a, const Box& b) {
-    Box r;
-    for (int k = 0; k < 3; k++) { r.mn[k] = fminf(a.mn[k], b.mn[k]); r.mx[k] = fmaxf(a.mx[k], b.mx[k]); }
-    return r;
-}
-static inline float area(const Box& b) {
-    float dx = b.mx[0] - b.mn[0], dy = b.mx[1] - b.mn[1], dz = b.mx[2] - b.mn[2];
-    return 2.f * (dx * dy + dy * dz + dz * dx);
-}
-static constexpr int kBins = 16;
-static inline int bin_index(float c, float cmin, float cmax) {
-    int b = int((c - cmin) / (cmax - cmin) * float(kBins - 1));
-    return b < 0 ? 0 : (b >= kBins ? kBins - 1 : b);
-}
-
-void build_sah_host(const float* tris, int n, int maxLeaf, HostBVH& out) {
-    std::vector<Box> pb(n);
-    std::vector<float> cen(size_t(n) * 3);
-    std::vector<int> idx(n), tmp(n);
-    for (int i = 0; i < n; i++) {
-        const float* t = tris + 9 * size_t(i);
-        for (int a = 0; a < 3; a++) {
-            pb[i].mn[a] = fminf(fminf(t[a], t[3 + a]), t[6 + a]);
-            pb[i].mx[a] = fmaxf(fmaxf(t[a], t[3 + a]), t[6 + a]);
-            cen[3 * size_t(i) + a] = (pb[i].mn[a] + pb[i].mx[a]) * .5f;
-        }
-        idx[i] = i;
-    }
-    out.numTris = n;
-    out.maxLeaf = maxLeaf;
-    out.nodes2.clear();
-    out.nodes2.reserve(size_t(2) * n);
-    out.ordered.clear();
-    out.ordered.reserve(n);
-    struct T { int s, e, parent, side; };
-    std::vector<T> st;
-    st.push_back({0, n, -1, 0});
-    while (!st.empty()) {
-        T t = st.back(); st.pop_back();
-        const int s = t.s, e = t.e, cnt = e - s;
-        const int id = int(out.nodes2.size());
-        out.nodes2.push_back(Node2{});
-        if (t.parent >= 0) (t.side == 0 ? out.nodes2[t.parent].left : out.nodes2[t.parent].right) = id;
-        Node2 nd{};
-        nd.box = pb[idx[s]];
-        for (int i = s + 1; i < e; i++) nd.box = merge(nd.box, pb[idx[i]]);
-        if (cnt <= maxLeaf) {
-            nd.triStart = int(out.ordered.size());
-            nd.triCount = cnt;
-            for (int i = s; i < e; i++) out.ordered.push_back(idx[i]);
-            out.nodes2[id] = nd;
-            continue;
-        }
-        float bestCost = 1e30f, pA = area(nd.box);
-        int bestAxis = -1, bestBin = -1;
-        float bestMin = 0.f, bestMax = 0.f;
-        for (int ax = 0; ax < 3; ax++) {
-            float cmin = 1e30f, cmax = -1e30f;
-            for (int i = s; i < e; i++) { float c = cen[3 * size_t(idx[i]) + ax]; cmin = fminf(cmin, c); cmax = fmaxf(cmax, c); }
-            if (cmax - cmin < 1e-8f) continue;
-            Box bb[kBins]; int bc[kBins] = {0};
-            for (int b = 0; b < kBins; b++)
-                for (int a = 0; a < 3; a++) { bb[b].mn[a] = 1e30f; bb[b].mx[a] = -1e30f; }
-            for (int i = s; i < e; i++) {
-                int b = bin_index(cen[3 * size_t(idx[i]) + ax], cmin, cmax);
-                bb[b] = merge(bb[b], pb[idx[i]]); bc[b]++;
-            }
-            Box R[kBins]; int rc[kBins];
-            R[kBins - 1] = bb[kBins - 1]; rc[kBins - 1] = bc[kBins - 1];
-            for (int b = kBins - 2; b >= 0; b--) { R[b] = merge(R[b + 1], bb[b]); rc[b] = rc[b + 1] + bc[b]; }
-            Box L = bb[0]; int lc = bc[0];
-            for (int b = 0; b < kBins - 1; b++) {
-                if (b > 0) { L = merge(L, bb[b]); lc += bc[b]; }
-                if (lc == 0 || rc[b + 1] == 0) continue;
-                float cost = float(lc) * area(L) / pA + float(rc[b + 1]) * area(R[b + 1]) / pA + 1.f;
-                if (cost < bestCost) { bestCost = cost; bestAxis = ax; bestBin = b; bestMin = cmin; bestMax = cmax; }
-            }
-        }
-        int mid;
-        if (bestAxis >= 0) {
-            int nl = 0, nr = 0;
-            for (int i = s; i < e; i++) {
-                int p = idx[i];
-                if (bin_index(cen[3 * size_t(p) + bestAxis], bestMin, bestMax) <= bestBin) idx[s + nl++] = p;
-                else tmp[nr++] = p;
-            }
-            std::copy(tmp.begin(), tmp.begin() + nr, idx.begin() + s + nl);
-            mid = s + nl;
-        } else {
-            mid = s + cnt / 2;
-        }
-        out.nodes2[id] = nd;
-        st.push_back({mid, e, id, 1});
-        st.push_back({s, mid, id, 0});
-    }
-    collapse_bvh4(out);
-}
-
-}  // namespace rt

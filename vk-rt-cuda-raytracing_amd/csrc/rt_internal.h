@@ -6,6 +6,9 @@
 #include <string>
 #include <vector>
 
+// hipStream_t without the HIP headers (host-only files include this header).
+typedef struct ihipStream_t* hipStream_t;
+
 #include "../../include/cudabvh.h"
 
 namespace rt {
@@ -17,20 +20,10 @@ void clear_error();
 bool verbose();
 
 // ---------------------------------------------------------------------------
-// Host-side BVH produced by the builder (all arrays in leaf / pre-order).
+// BVH2 node of the build (pre-order numbering; children follow parents).
 // ---------------------------------------------------------------------------
 struct Box { float mn[3], mx[3]; };
 struct Node2 { Box box; int left, right, triStart, triCount; };
-
-struct HostBVH {
-    int numTris = 0;
-    int maxLeaf = 4;
-    std::vector<Node2> nodes2;          // BVH2, pre-order (DFS) numbering
-    std::vector<int> ordered;           // leaf-order position -> original index
-    std::vector<uint8_t> nodes4;        // BVH4, 64 B per node, pre-order
-    int numNodes4 = 0;
-    int depth4 = 0;                     // node levels on the longest path
-};
 
 // BVH4 node format shared with the reference (backend :31-34).
 constexpr int kNode4Bytes = 64;
@@ -46,9 +39,20 @@ uint16_t f2h_down(float x);
 uint16_t f2h_up(float x);
 float h2f(uint16_t h);
 
-// Collapses a pre-order BVH2 into BVH4 nodes (backend :391-418 rule: every
-// internal child is opened into its two children), FP16 bounds rounded outward.
-void collapse_bvh4(HostBVH& b);
+// Result of the device build (build.hip); all pointers are device memory
+// owned by the caller afterwards.
+struct GpuBVH {
+    Node2* nodes2 = nullptr;  int numNodes2 = 0;   // pre-order BVH2
+    void* nodes4 = nullptr;   int numNodes4 = 0;   // BVH4, 64 B per node, pre-order
+    int depth4 = 0;
+    void* tris = nullptr;                          // leaf order, 48 B per triangle
+    int* leafOf = nullptr;                         // original index -> leaf position
+    int* ordered = nullptr;                        // leaf position -> original index
+    int levels = 0;
+};
+int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& out);
+void scan_exclusive(const int* in, int* out, int n, int* total, int* tiles, hipStream_t s);
+int scan_tiles_needed(int n);
 
 // Stackless DFS BVH2 export (backend :456-522): 8 uint32 per node.
 void export_dfs(const std::vector<Node2>& nodes, std::vector<uint32_t>& out);
@@ -117,4 +121,5 @@ struct CudaBVHHandle {
     rt::TraceScratch scratch;           // traversal scratch of calls on this handle
     rt::TraceConfig tune[rt::kNumClasses];  // per ray class launch configuration
     void* d_work = nullptr;   size_t workBytes = 0;   // staging for host-pointer calls
+    void* d_sortWs = nullptr; size_t sortWsBytes = 0; // ray-sort workspace
 };
