@@ -255,6 +255,47 @@ const void* cudaBVH_deviceTris(CudaBVH_t bvh);
 /* Depth of the BVH4 (node levels on the longest root-to-leaf path). */
 int cudaBVH_getDepth(CudaBVH_t bvh);
 
+/* ----- Two-level scenes (instances of BLAS handles under a TLAS) ----- */
+typedef struct CudaSceneHandle* CudaScene_t;
+
+/* objectToWorld: row-major 3x4 affine map (p_world = M p_object + t). */
+typedef struct {
+    int blas;
+    float objectToWorld[12];
+    float albedo[3];
+} CudaInstance;
+
+/* Closest hit in a scene: t, instance index, ORIGINAL primitive index of the
+ * instance's BLAS, barycentrics. Misses: t 1e30, instId = primId = -1. */
+typedef struct {
+    float* t;
+    int*   instId;
+    int*   primId;
+    float* u;
+    float* v;
+} CudaSceneHit;
+
+/* Builds the TLAS (binned-SAH BVH4 on the GPU over the instances' world
+ * boxes, one instance per leaf). BLAS handles are borrowed and must outlive
+ * the scene. Rays are transformed into each instance's object space
+ * (unnormalised, so t is the world parameter); the closest hit is the
+ * lexicographic minimum of (t, instance, primitive). */
+CudaScene_t cudaBVH_sceneCreate(const CudaBVH_t* blas, int numBlas, const CudaInstance* instances, int numInstances);
+void cudaBVH_sceneDestroy(CudaScene_t scene);
+int cudaBVH_sceneSetStream(CudaScene_t scene, void* hipStream);
+CudaBVH_t cudaBVH_sceneTLAS(CudaScene_t scene);                 /* borrowed */
+int cudaBVH_sceneWorldToObject(CudaScene_t scene, float* out);  /* 12 floats per instance */
+int cudaBVH_sceneTraceClosest(CudaScene_t scene, int numRays, const CudaRayPacket* rays,
+                              const CudaSceneHit* hits, unsigned flags);
+int cudaBVH_sceneTraceAny(CudaScene_t scene, int numRays, const CudaRayPacket* rays,
+                          uint8_t* occluded, unsigned flags);
+int cudaBVH_sceneTraceClosestCounted(CudaScene_t scene, int numRays, const CudaRayPacket* rays,
+                                     const CudaSceneHit* hits, int* nodeVisits, int* triTests, unsigned flags);
+int cudaBVH_sceneTraceClosestQueued(CudaScene_t scene, int capacity, const int* d_count,
+                                    const CudaRayPacket* rays, const CudaSceneHit* hits, unsigned flags);
+int cudaBVH_sceneTraceAnyQueued(CudaScene_t scene, int capacity, const int* d_count,
+                                const CudaRayPacket* rays, uint8_t* occluded, unsigned flags);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char* cudaBVH_lastError(void);
 

@@ -46,6 +46,8 @@ def lib() -> C.CDLL:
             "orc_isect_watertight": (I, [P, P, P, P, P]), "orc_isect_mt": (I, [P, P, P, P, P]),
             "orc_sort_keys": (None, [P, I, P, P, P]),
             "orc_num_threads": (I, [I]),
+            "orc_trace_closest_2l": (None, [P, I, P, P, P, P, P, I, P, P, P, P, P, P, P, I]),
+            "orc_trace_any_2l": (None, [P, I, P, P, P, P, P, I, P, I]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -216,3 +218,88 @@ def f2h(x: float, mode: str = "rne") -> int:
 
 def h2f(h: int) -> float:
     return lib().orc_h2f(h)
+
+
+class _Blas(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("num_nodes", C.c_int), ("tris", C.c_void_p), ("prim_id", C.c_void_p),
+                ("num_tris", C.c_int)]
+
+
+def instance_world_boxes(blas_bounds, o2w, blas_of) -> np.ndarray:
+    """World AABB of every instance: the 8 corners of its BLAS box mapped by
+    the object-to-world 3x4 map in float64, widened by
+    1e-5*extent + 1e-6*(|lo|+|hi|) + 1e-30 per axis, rounded to float32."""
+    n = len(blas_of)
+    out = np.zeros((n, 6), np.float32)
+    for i in range(n):
+        lo3, hi3 = blas_bounds[blas_of[i]]
+        m = np.asarray(o2w[i], np.float32).astype(np.float64).reshape(3, 4)
+        lo = np.full(3, 1e300)
+        hi = np.full(3, -1e300)
+        for c in range(8):
+            p = [float(hi3[0] if c & 1 else lo3[0]), float(hi3[1] if c & 2 else lo3[1]),
+                 float(hi3[2] if c & 4 else lo3[2])]
+            for r in range(3):
+                w = m[r, 0] * p[0] + m[r, 1] * p[1] + m[r, 2] * p[2] + m[r, 3]
+                lo[r] = min(lo[r], w)
+                hi[r] = max(hi[r], w)
+        for r in range(3):
+            pad = 1e-5 * (hi[r] - lo[r]) + 1e-6 * (abs(lo[r]) + abs(hi[r])) + 1e-30
+            out[i, r] = np.float32(lo[r] - pad)
+            out[i, 3 + r] = np.float32(hi[r] + pad)
+    return out
+
+
+def aabb_tris(aabbs: np.ndarray) -> np.ndarray:
+    """Degenerate triangles whose bounds are the given boxes: min, max, (max.x, min.y, min.z)."""
+    a = np.asarray(aabbs, np.float32).reshape(-1, 6)
+    return np.ascontiguousarray(np.stack([a[:, 0], a[:, 1], a[:, 2], a[:, 3], a[:, 4], a[:, 5], a[:, 3], a[:, 1],
+                                          a[:, 2]], 1))
+
+
+class OracleScene:
+    """Two-level scene for the oracle: BLAS OracleBVH objects, a TLAS built
+    over the instances' world boxes with one instance per leaf, and the
+    world-to-object maps (row-major 3x4 float32, one per instance)."""
+
+    def __init__(self, blas_tris: list, o2w: np.ndarray, blas_of, w2o: np.ndarray):
+        self.blas = [OracleBVH(t) for t in blas_tris]
+        self.blas_nodes = [b.nodes() for b in self.blas]
+        self.blas_leaf = [b.leaf_tris() for b in self.blas]
+        self.blas_pid = [b.order() for b in self.blas]
+        bounds = []
+        for t in blas_tris:
+            v = np.asarray(t, np.float32).reshape(-1, 3)
+            bounds.append((v.min(0), v.max(0)))
+        self.blas_of = np.ascontiguousarray(blas_of, np.int32)
+        self.world_boxes = instance_world_boxes(bounds, o2w, self.blas_of)
+        self.tlas = OracleBVH(aabb_tris(self.world_boxes), max_leaf=1)
+        self.tlas_nodes = self.tlas.nodes()
+        self.tlas_prim = self.tlas.order()
+        self.w2o = np.ascontiguousarray(w2o, np.float32).reshape(-1, 12)
+        self._arr = (_Blas * len(self.blas))()
+        for i in range(len(self.blas)):
+            self._arr[i] = _Blas(self.blas_nodes[i].ctypes.data, self.blas_nodes[i].shape[0],
+                                 self.blas_leaf[i].ctypes.data, self.blas_pid[i].ctypes.data,
+                                 self.blas_leaf[i].shape[0])
+
+    def trace_closest(self, rays: np.ndarray, threads: int = 0) -> dict:
+        n = rays.shape[1]
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = dict(t=np.zeros(n, np.float32), inst=np.zeros(n, np.int32), id=np.zeros(n, np.int32),
+                   u=np.zeros(n, np.float32), v=np.zeros(n, np.float32), nodes=np.zeros(n, np.int32),
+                   tris=np.zeros(n, np.int32))
+        lib().orc_trace_closest_2l(self.tlas_nodes.ctypes.data, self.tlas_nodes.shape[0], self.tlas_prim.ctypes.data,
+                                   self._arr, self.blas_of.ctypes.data, self.w2o.ctypes.data, rays.ctypes.data, n,
+                                   *[out[k].ctypes.data for k in ("t", "inst", "id", "u", "v", "nodes", "tris")],
+                                   threads)
+        return out
+
+    def trace_any(self, rays: np.ndarray, threads: int = 0) -> np.ndarray:
+        n = rays.shape[1]
+        rays = np.ascontiguousarray(rays, np.float32)
+        occ = np.zeros(n, np.uint8)
+        lib().orc_trace_any_2l(self.tlas_nodes.ctypes.data, self.tlas_nodes.shape[0], self.tlas_prim.ctypes.data,
+                               self._arr, self.blas_of.ctypes.data, self.w2o.ctypes.data, rays.ctypes.data, n,
+                               occ.ctypes.data, threads)
+        return occ

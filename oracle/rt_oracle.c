@@ -749,3 +749,142 @@ void orc_brute_closest_f64(const float* tris, const int* prim_id, int num_tris, 
         t[i] = bid < 0 ? (double)MISS_T : best; id[i] = bid;
     }
 }
+
+/* ===================================================================== */
+/* Two-level traversal (TLAS over instances, BLAS per mesh)               */
+/* ===================================================================== */
+/* The TLAS is traversed like a BVH4; reaching a TLAS leaf (one instance)
+ * pushes a return marker, transforms the ray into the instance's object
+ * space and continues into the BLAS; popping the marker restores the world
+ * ray. Node order and culling are those of closest_one. */
+#define RET_MARK INT_MIN
+
+static void xform_ray(const xray* w, const float* M, xray* o) {
+    float r[8];
+    for (int k = 0; k < 3; k++) {
+        const float* m = M + 4 * k;
+        r[k] = ((m[0] * w->o[0] + m[1] * w->o[1]) + m[2] * w->o[2]) + m[3];
+        r[3 + k] = (m[0] * w->d[0] + m[1] * w->d[1]) + m[2] * w->d[2];
+    }
+    r[6] = w->tmin; r[7] = w->tmax;
+    setup_ray(o, r, 1, 0);
+}
+
+static void two_level_one(const uint8_t* tlas, const int* tlas_prim, const orc_blas* blas, const int* blas_of,
+                          const float* w2o, const xray* rw, int any, sentry* stk,
+                          float* ot, int* oinst, int* oid, float* ou, float* ov, int* found_out, int* nv, int* nt) {
+    float best = rw->tmax, bu = 0.f, bv = 0.f;
+    int bid = -1, binst = -1, visits = 0, tests = 0, sp = 0, inst = -1, found = 0;
+    xray ro;
+    const xray* r = rw;
+    const uint8_t* nodes = tlas;
+    stk[sp++] = (sentry){0, -INFINITY};
+    while (sp > 0 && !found) {
+        sentry e = stk[--sp];
+        if (e.ref == RET_MARK) { inst = -1; r = rw; nodes = tlas; continue; }
+        if (!any && e.tn > best) continue;
+        if (e.ref < 0) {
+            int enc = -(e.ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
+            if (inst < 0) {                                   /* TLAS leaf: enter the instance */
+                int k = tlas_prim[ts];
+                stk[sp++] = (sentry){RET_MARK, -INFINITY};
+                inst = k;
+                xform_ray(rw, w2o + 12 * (size_t)k, &ro);
+                r = &ro;
+                nodes = (const uint8_t*)blas[blas_of[k]].nodes;
+                stk[sp++] = (sentry){0, -INFINITY};
+                (void)tc;
+                continue;
+            }
+            const orc_blas* B = &blas[blas_of[inst]];
+            for (int q = 0; q < tc && !found; q++) {
+                int ti = ts + q; float tt, uu, vv;
+                tests++;
+                if (!isect_wt(r, B->tris + 9 * (size_t)ti, &tt, &uu, &vv)) continue;
+                if (!(tt > r->tmin) || !(tt < r->tmax)) continue;
+                if (any) { found = 1; break; }
+                int id = B->prim_id[ti];
+                if (tt < best || (tt == best && (inst < binst || (inst == binst && id < bid)))) {
+                    best = tt; bid = id; binst = inst; bu = uu; bv = vv;
+                }
+            }
+            continue;
+        }
+        const uint8_t* nd = nodes + 64 * (size_t)e.ref;
+        visits++;
+        int32_t ch[4]; memcpy(ch, nd + 48, 16);
+        float lim = any ? r->tmax : best;
+        if (any) {
+            for (int c = 0; c < 4; c++) {
+                float tn;
+                if (ch[c] != -1 && slab4(nd, c, r, lim, &tn)) stk[sp++] = (sentry){ch[c], 0.f};
+            }
+            continue;
+        }
+        float dist[4]; int child[4];
+        for (int c = 0; c < 4; c++) {
+            float tn;
+            if (ch[c] != -1 && slab4(nd, c, r, lim, &tn)) { dist[c] = tn; child[c] = ch[c]; }
+            else { dist[c] = INFINITY; child[c] = -1; }
+        }
+#define CSWAP(a, b) do { if (dist[a] > dist[b]) { float td = dist[a]; dist[a] = dist[b]; dist[b] = td; \
+                         int tc2 = child[a]; child[a] = child[b]; child[b] = tc2; } } while (0)
+        CSWAP(0, 1); CSWAP(2, 3); CSWAP(0, 2); CSWAP(1, 3); CSWAP(1, 2);
+#undef CSWAP
+        for (int c = 3; c >= 0; c--)
+            if (child[c] != -1) stk[sp++] = (sentry){child[c], dist[c]};
+    }
+    if (found_out) *found_out = found;
+    if (ot) {
+        if (bid < 0) { *ot = MISS_T; *oinst = -1; *oid = -1; *ou = 0.f; *ov = 0.f; }
+        else { *ot = best; *oinst = binst; *oid = bid; *ou = bu; *ov = bv; }
+    }
+    if (nv) *nv = visits;
+    if (nt) *nt = tests;
+}
+
+static int max_nodes_2l(int tlas_num_nodes, const orc_blas* blas, const int* blas_of, int n_inst) {
+    int m = 0;
+    for (int i = 0; i < n_inst; i++) if (blas[blas_of[i]].num_nodes > m) m = blas[blas_of[i]].num_nodes;
+    return m + tlas_num_nodes;
+}
+
+void orc_trace_closest_2l(const void* tlas_nodes, int tlas_num_nodes, const int* tlas_prim, const orc_blas* blas,
+                          const int* blas_of, const float* w2o, const float* rays, int n_rays, float* t, int* inst,
+                          int* id, float* u, float* v, int* node_visits, int* tri_tests, int threads) {
+    int nth = orc_num_threads(threads);
+    int cap = 3 * (tlas_num_nodes + 4096) + 16;
+    (void)max_nodes_2l;
+#pragma omp parallel num_threads(nth)
+    {
+        sentry* stk = (sentry*)malloc(sizeof(sentry) * (size_t)cap * 4);
+#pragma omp for schedule(dynamic, 256)
+        for (int i = 0; i < n_rays; i++) {
+            xray r; setup_ray(&r, rays, (size_t)n_rays, (size_t)i);
+            two_level_one((const uint8_t*)tlas_nodes, tlas_prim, blas, blas_of, w2o, &r, 0, stk, &t[i], &inst[i],
+                          &id[i], &u[i], &v[i], NULL, node_visits ? &node_visits[i] : NULL,
+                          tri_tests ? &tri_tests[i] : NULL);
+        }
+        free(stk);
+    }
+}
+
+void orc_trace_any_2l(const void* tlas_nodes, int tlas_num_nodes, const int* tlas_prim, const orc_blas* blas,
+                      const int* blas_of, const float* w2o, const float* rays, int n_rays, uint8_t* occluded,
+                      int threads) {
+    int nth = orc_num_threads(threads);
+    int cap = 3 * (tlas_num_nodes + 4096) + 16;
+#pragma omp parallel num_threads(nth)
+    {
+        sentry* stk = (sentry*)malloc(sizeof(sentry) * (size_t)cap * 4);
+#pragma omp for schedule(dynamic, 256)
+        for (int i = 0; i < n_rays; i++) {
+            xray r; setup_ray(&r, rays, (size_t)n_rays, (size_t)i);
+            int f = 0;
+            two_level_one((const uint8_t*)tlas_nodes, tlas_prim, blas, blas_of, w2o, &r, 1, stk, NULL, NULL, NULL,
+                          NULL, NULL, &f, NULL, NULL);
+            occluded[i] = (uint8_t)f;
+        }
+        free(stk);
+    }
+}

@@ -114,6 +114,28 @@ void orc_sort_keys(const float* rays, int n, const float* lo3, const float* inv3
 
 int orc_num_threads(int requested);
 
+/* ---------------- Two-level scenes ----------------
+ * TLAS: a BVH4 whose leaves hold one instance each (leaf-order index ->
+ * instance id via tlas_prim). Instance i: BLAS blas_of[i] (nodes, leaf-order
+ * tris, prim ids given per BLAS), world-to-object affine w2o[12*i] (row-major
+ * 3x4: p' = M p + t). A ray is transformed with
+ *   o'[r] = ((M[r][0]*ox + M[r][1]*oy) + M[r][2]*oz) + t[r],
+ *   d'[r] = (M[r][0]*dx + M[r][1]*dy) + M[r][2]*dz
+ * and traced unnormalised, so t is the world-space distance parameter.
+ * Closest = lexicographic min of (t, instance, prim). */
+typedef struct {
+    const void* nodes; int num_nodes;
+    const float* tris; const int* prim_id; int num_tris;
+} orc_blas;
+
+void orc_trace_closest_2l(const void* tlas_nodes, int tlas_num_nodes, const int* tlas_prim,
+                          const orc_blas* blas, const int* blas_of, const float* w2o,
+                          const float* rays, int n_rays, float* t, int* inst, int* id, float* u, float* v,
+                          int* node_visits, int* tri_tests, int threads);
+void orc_trace_any_2l(const void* tlas_nodes, int tlas_num_nodes, const int* tlas_prim,
+                      const orc_blas* blas, const int* blas_of, const float* w2o,
+                      const float* rays, int n_rays, uint8_t* occluded, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,14 @@ class CudaCamera(C.Structure):
                 ("scaleX", C.c_float), ("scaleY", C.c_float)]
 
 
+class CudaInstance(C.Structure):
+    _fields_ = [("blas", C.c_int), ("objectToWorld", C.c_float * 12), ("albedo", C.c_float * 3)]
+
+
+class CudaSceneHit(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("t", "instId", "primId", "u", "v")]
+
+
 # name -> (restype, argtypes); the exported C ABI, in header order.
 _H = C.c_void_p
 _P = C.c_void_p
@@ -106,6 +114,18 @@ SIGNATURES = {
     "cudaBVH_deviceNodes": (_P, [_H]),
     "cudaBVH_deviceTris": (_P, [_H]),
     "cudaBVH_getDepth": (_I, [_H]),
+    "cudaBVH_sceneCreate": (_H, [_P, _I, _P, _I]),
+    "cudaBVH_sceneDestroy": (None, [_H]),
+    "cudaBVH_sceneSetStream": (_I, [_H, _P]),
+    "cudaBVH_sceneTLAS": (_H, [_H]),
+    "cudaBVH_sceneWorldToObject": (_I, [_H, _P]),
+    "cudaBVH_sceneTraceClosest": (_I, [_H, _I, C.POINTER(CudaRayPacket), C.POINTER(CudaSceneHit), C.c_uint]),
+    "cudaBVH_sceneTraceAny": (_I, [_H, _I, C.POINTER(CudaRayPacket), _P, C.c_uint]),
+    "cudaBVH_sceneTraceClosestCounted": (_I, [_H, _I, C.POINTER(CudaRayPacket), C.POINTER(CudaSceneHit), _P, _P,
+                                              C.c_uint]),
+    "cudaBVH_sceneTraceClosestQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), C.POINTER(CudaSceneHit),
+                                             C.c_uint]),
+    "cudaBVH_sceneTraceAnyQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), _P, C.c_uint]),
     "cudaBVH_lastError": (C.c_char_p, []),
 }
 
@@ -191,7 +211,7 @@ class BVH:
         return cls(load_library().cudaBVH_buildFromAABBs(a.ctypes.data, a.shape[0]), a.shape[0])
 
     def destroy(self) -> None:
-        if self.handle:
+        if self.handle and not getattr(self, "_borrowed", False):
             self.lib.cudaBVH_destroy(self.handle)
             self.handle = C.c_void_p(0)
 
@@ -363,6 +383,88 @@ class BVH:
         table = [(int(res[4 * i]), int(res[4 * i + 1]), int(res[4 * i + 2]), float(res[4 * i + 3]))
                  for i in range(nres.value)]
         return self.trace_config(ray_class), table
+
+
+class Scene:
+    """Owner of a two-level scene: a TLAS over instances of borrowed BLAS
+    handles (:class:`BVH`), each with a 3x4 object-to-world map and albedo."""
+
+    def __init__(self, blas: list, object_to_world: np.ndarray, blas_of, albedo=None):
+        self.lib = load_library()
+        self.blas = list(blas)  # keeps the borrowed handles alive
+        m = np.ascontiguousarray(object_to_world, dtype=np.float32).reshape(-1, 12)
+        n = m.shape[0]
+        alb = np.full((n, 3), 0.5, np.float32) if albedo is None else np.asarray(albedo, np.float32).reshape(n, 3)
+        insts = (CudaInstance * n)()
+        for i in range(n):
+            insts[i].blas = int(blas_of[i])
+            for k in range(12):
+                insts[i].objectToWorld[k] = float(m[i, k])
+            for k in range(3):
+                insts[i].albedo[k] = float(alb[i, k])
+        hs = (C.c_void_p * len(self.blas))(*[b.handle.value for b in self.blas])
+        h = self.lib.cudaBVH_sceneCreate(hs, len(self.blas), insts, n)
+        if not h:
+            raise CudaBVHError(f"sceneCreate: {last_error()}")
+        self.handle = C.c_void_p(h)
+        self.num_instances = n
+
+    def destroy(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.cudaBVH_sceneDestroy(self.handle)
+            self.handle = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def set_stream(self, stream: int | None) -> None:
+        _check(self.lib.cudaBVH_sceneSetStream(self.handle, stream), "sceneSetStream")
+
+    def tlas(self) -> BVH:
+        """The TLAS as a borrowed (non-owning) BVH view."""
+        v = BVH.__new__(BVH)
+        v.lib, v.handle, v.num_input = self.lib, C.c_void_p(self.lib.cudaBVH_sceneTLAS(self.handle)), \
+            self.num_instances
+        v._borrowed = True
+        return v
+
+    def world_to_object(self) -> np.ndarray:
+        out = np.zeros((self.num_instances, 12), np.float32)
+        _check(self.lib.cudaBVH_sceneWorldToObject(self.handle, out.ctypes.data), "sceneWorldToObject")
+        return out
+
+    def trace_closest(self, rays, t, inst_id=None, prim_id=None, u=None, v=None, flags: int = 0) -> None:
+        rp = ray_packet(rays)
+        hp = CudaSceneHit(_ptr(t), _ptr(inst_id), _ptr(prim_id), _ptr(u), _ptr(v))
+        _check(self.lib.cudaBVH_sceneTraceClosest(self.handle, int(t.shape[0]), C.byref(rp), C.byref(hp), flags),
+               "sceneTraceClosest")
+
+    def trace_closest_counted(self, rays, t, inst_id, prim_id, u, v, node_visits, tri_tests, flags: int = 0):
+        rp = ray_packet(rays)
+        hp = CudaSceneHit(_ptr(t), _ptr(inst_id), _ptr(prim_id), _ptr(u), _ptr(v))
+        _check(self.lib.cudaBVH_sceneTraceClosestCounted(self.handle, int(t.shape[0]), C.byref(rp), C.byref(hp),
+                                                         _ptr(node_visits), _ptr(tri_tests), flags),
+               "sceneTraceClosestCounted")
+
+    def trace_any(self, rays, occluded, flags: int = 0) -> None:
+        rp = ray_packet(rays)
+        _check(self.lib.cudaBVH_sceneTraceAny(self.handle, int(occluded.shape[0]), C.byref(rp), _ptr(occluded),
+                                              flags), "sceneTraceAny")
+
+    def trace_closest_queued(self, capacity: int, d_count, rays, t, inst_id=None, prim_id=None, u=None, v=None,
+                             flags: int = 0) -> None:
+        rp = ray_packet(rays)
+        hp = CudaSceneHit(_ptr(t), _ptr(inst_id), _ptr(prim_id), _ptr(u), _ptr(v))
+        _check(self.lib.cudaBVH_sceneTraceClosestQueued(self.handle, capacity, _ptr(d_count), C.byref(rp),
+                                                        C.byref(hp), flags), "sceneTraceClosestQueued")
+
+    def trace_any_queued(self, capacity: int, d_count, rays, occluded, flags: int = 0) -> None:
+        rp = ray_packet(rays)
+        _check(self.lib.cudaBVH_sceneTraceAnyQueued(self.handle, capacity, _ptr(d_count), C.byref(rp),
+                                                    _ptr(occluded), flags), "sceneTraceAnyQueued")
 
 
 def gen_pinhole_rays(width: int, height: int, cam: CudaCamera, rays, stream: int | None = None) -> None:

@@ -16,6 +16,7 @@
 
 #include "rt_internal.h"
 #include "trace_launch.h"
+#include "scene.h"
 
 namespace rt {
 const char* last_error();
@@ -821,6 +822,167 @@ int cudaBVH_sortRays(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays, uint
     void* ws = sort_workspace(bvh, numRays);
     if (!ws) return -1;
     return rtk::sort_rays(bvh, numRays, *rays, keysOut, permOut, ws, hstream(bvh));
+}
+
+// ------------------------------- two-level scenes ----------------------------
+
+static void invert_affine(const float* m, float* out) {
+    // double-precision inverse of the 3x4 affine map, rounded to float
+    double a[3][3], t[3];
+    for (int r = 0; r < 3; r++) { for (int c = 0; c < 3; c++) a[r][c] = m[4 * r + c]; t[r] = m[4 * r + 3]; }
+    const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+                       a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                       a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    double inv[3][3];
+    inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+    inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+    inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+    inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+    inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+    inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+    inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+    inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+    inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+    for (int r = 0; r < 3; r++) {
+        double tr = 0;
+        for (int c = 0; c < 3; c++) { out[4 * r + c] = (float)inv[r][c]; tr -= inv[r][c] * t[c]; }
+        out[4 * r + 3] = (float)tr;
+    }
+}
+
+static bool scene_ok(CudaScene_t s, const char* fn) {
+    if (!s) { set_error("%s: NULL scene", fn); return false; }
+    return true;
+}
+
+CudaScene_t cudaBVH_sceneCreate(const CudaBVH_t* blas, int numBlas, const CudaInstance* inst, int numInst) {
+    rt::clear_error();
+    if (!blas || numBlas <= 0 || !inst || numInst <= 0) { set_error("cudaBVH_sceneCreate: bad arguments"); return nullptr; }
+    for (int b = 0; b < numBlas; b++)
+        if (!blas[b]) { set_error("cudaBVH_sceneCreate: BLAS %d is NULL", b); return nullptr; }
+    auto* sc = new CudaSceneHandle();
+    sc->blas.assign(blas, blas + numBlas);
+    sc->inst.resize(numInst);
+    sc->o2w.resize(size_t(numInst) * 12);
+    std::vector<float> aabbs(size_t(numInst) * 6);
+    for (int i = 0; i < numInst; i++) {
+        const CudaInstance& I = inst[i];
+        if (I.blas < 0 || I.blas >= numBlas) { set_error("instance %d: bad BLAS index %d", i, I.blas); delete sc; return nullptr; }
+        memcpy(&sc->o2w[12 * size_t(i)], I.objectToWorld, 48);
+        invert_affine(I.objectToWorld, sc->inst[i].w2o);
+        sc->inst[i].blas = I.blas;
+        for (int a = 0; a < 3; a++) sc->inst[i].albedo[a] = I.albedo[a];
+        // world box: the 8 transformed corners of the BLAS box (double), widened
+        const CudaBVHHandle* B = blas[I.blas];
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int c = 0; c < 8; c++) {
+            const double p[3] = {(c & 1) ? B->bboxMax[0] : B->bboxMin[0], (c & 2) ? B->bboxMax[1] : B->bboxMin[1],
+                                 (c & 4) ? B->bboxMax[2] : B->bboxMin[2]};
+            for (int r = 0; r < 3; r++) {
+                const double w = I.objectToWorld[4 * r] * p[0] + I.objectToWorld[4 * r + 1] * p[1] +
+                                 I.objectToWorld[4 * r + 2] * p[2] + I.objectToWorld[4 * r + 3];
+                lo[r] = std::min(lo[r], w); hi[r] = std::max(hi[r], w);
+            }
+        }
+        for (int r = 0; r < 3; r++) {
+            const double pad = 1e-5 * (hi[r] - lo[r]) + 1e-6 * (std::fabs(lo[r]) + std::fabs(hi[r])) + 1e-30;
+            aabbs[6 * size_t(i) + r] = (float)(lo[r] - pad);
+            aabbs[6 * size_t(i) + 3 + r] = (float)(hi[r] + pad);
+        }
+        sc->maxBlasDepth = std::max(sc->maxBlasDepth, B->depth4);
+    }
+    std::vector<float> t(size_t(numInst) * 9);
+    for (int i = 0; i < numInst; i++) {
+        const float* b = &aabbs[6 * size_t(i)];
+        float* d = &t[9 * size_t(i)];
+        d[0] = b[0]; d[1] = b[1]; d[2] = b[2]; d[3] = b[3]; d[4] = b[4]; d[5] = b[5]; d[6] = b[3]; d[7] = b[1]; d[8] = b[2];
+    }
+    sc->tlas = build_impl(t.data(), numInst, 1);      // one instance per TLAS leaf
+    if (!sc->tlas) { delete sc; return nullptr; }
+    std::vector<rtk::BlasDev> bd(numBlas);
+    for (int b = 0; b < numBlas; b++)
+        bd[b] = rtk::BlasDev{(const uint4*)blas[b]->d_nodes4, (const float4*)blas[b]->d_tris, blas[b]->d_leafOfPrim,
+                             blas[b]->depth4, 0};
+    if (hipMalloc((void**)&sc->d_inst, sizeof(rtk::InstanceDev) * numInst) != hipSuccess ||
+        hipMalloc((void**)&sc->d_blas, sizeof(rtk::BlasDev) * numBlas) != hipSuccess ||
+        hipMemcpy(sc->d_inst, sc->inst.data(), sizeof(rtk::InstanceDev) * numInst, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(sc->d_blas, bd.data(), sizeof(rtk::BlasDev) * numBlas, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("cudaBVH_sceneCreate: device allocation failed");
+        hipFree(sc->d_inst); hipFree(sc->d_blas);
+        cudaBVH_destroy(sc->tlas);
+        delete sc;
+        return nullptr;
+    }
+    return sc;
+}
+
+void cudaBVH_sceneDestroy(CudaScene_t sc) {
+    if (!sc) return;
+    if (sc->stream) hipStreamSynchronize((hipStream_t)sc->stream); else hipDeviceSynchronize();
+    hipFree(sc->d_inst); hipFree(sc->d_blas); hipFree(sc->d_pt);
+    rtk::free_scratch(sc->scratch);
+    cudaBVH_destroy(sc->tlas);
+    delete sc;
+}
+
+int cudaBVH_sceneSetStream(CudaScene_t sc, void* stream) {
+    if (!scene_ok(sc, "cudaBVH_sceneSetStream")) return -1;
+    sc->stream = stream;
+    sc->tlas->stream = stream;
+    return 0;
+}
+
+CudaBVH_t cudaBVH_sceneTLAS(CudaScene_t sc) { return sc ? sc->tlas : nullptr; }
+
+int cudaBVH_sceneWorldToObject(CudaScene_t sc, float* out) {
+    if (!scene_ok(sc, "cudaBVH_sceneWorldToObject") || !out) return -1;
+    for (size_t i = 0; i < sc->inst.size(); i++) memcpy(out + 12 * i, sc->inst[i].w2o, 48);
+    return 0;
+}
+
+static int scene_trace(CudaScene_t sc, int n, const int* d_count, const CudaRayPacket* rays, const CudaSceneHit* hits,
+                       uint8_t* occ, int* cn, int* ct, unsigned flags, bool any, const char* fn) {
+    rt::clear_error();
+    if (!scene_ok(sc, fn)) return -1;
+    if (n == 0) return 0;
+    if (n < 0 || !packet_ok(rays) || (!any && (!hits || !hits->t)) || (any && !occ)) { set_error("%s: bad arguments", fn); return -1; }
+    if (flags & ~(unsigned)CUDABVH_FLAG_SYNC) { set_error("%s: unsupported flags 0x%x", fn, flags); return -1; }
+    rtk::Trace2LArgs a;
+    a.n = n; a.nDev = d_count;
+    a.ox = rays->ox; a.oy = rays->oy; a.oz = rays->oz; a.dx = rays->dx; a.dy = rays->dy; a.dz = rays->dz;
+    a.tmin = rays->tmin; a.tmax = rays->tmax;
+    if (hits) { a.t = hits->t; a.inst_out = hits->instId; a.id = hits->primId; a.u = hits->u; a.v = hits->v; }
+    a.occ = occ; a.cntNodes = cn; a.cntTris = ct;
+    hipStream_t s = (hipStream_t)sc->stream;
+    int rc = rtk::launch_trace2l(sc, sc->scratch, a, any, s);
+    if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(s), return -1);
+    return rc;
+}
+
+int cudaBVH_sceneTraceClosest(CudaScene_t sc, int numRays, const CudaRayPacket* rays, const CudaSceneHit* hits,
+                              unsigned flags) {
+    return scene_trace(sc, numRays, nullptr, rays, hits, nullptr, nullptr, nullptr, flags, false, "cudaBVH_sceneTraceClosest");
+}
+int cudaBVH_sceneTraceAny(CudaScene_t sc, int numRays, const CudaRayPacket* rays, uint8_t* occluded, unsigned flags) {
+    return scene_trace(sc, numRays, nullptr, rays, nullptr, occluded, nullptr, nullptr, flags, true, "cudaBVH_sceneTraceAny");
+}
+int cudaBVH_sceneTraceClosestCounted(CudaScene_t sc, int numRays, const CudaRayPacket* rays, const CudaSceneHit* hits,
+                                     int* nodeVisits, int* triTests, unsigned flags) {
+    if (!nodeVisits || !triTests) { set_error("cudaBVH_sceneTraceClosestCounted: NULL counters"); return -1; }
+    return scene_trace(sc, numRays, nullptr, rays, hits, nullptr, nodeVisits, triTests, flags, false,
+                       "cudaBVH_sceneTraceClosestCounted");
+}
+int cudaBVH_sceneTraceClosestQueued(CudaScene_t sc, int capacity, const int* d_count, const CudaRayPacket* rays,
+                                    const CudaSceneHit* hits, unsigned flags) {
+    if (!d_count) { set_error("cudaBVH_sceneTraceClosestQueued: NULL count"); return -1; }
+    return scene_trace(sc, capacity, d_count, rays, hits, nullptr, nullptr, nullptr, flags, false,
+                       "cudaBVH_sceneTraceClosestQueued");
+}
+int cudaBVH_sceneTraceAnyQueued(CudaScene_t sc, int capacity, const int* d_count, const CudaRayPacket* rays,
+                                uint8_t* occluded, unsigned flags) {
+    if (!d_count) { set_error("cudaBVH_sceneTraceAnyQueued: NULL count"); return -1; }
+    return scene_trace(sc, capacity, d_count, rays, nullptr, occluded, nullptr, nullptr, flags, true,
+                       "cudaBVH_sceneTraceAnyQueued");
 }
 
 }  // extern "C"

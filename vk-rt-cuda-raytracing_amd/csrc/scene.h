@@ -1,0 +1,67 @@
+// scene.h — two-level scene structures shared by the traversal kernels
+// (scene.hip), the path tracer (pt.hip) and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "rt_internal.h"
+#include "trace_launch.h"
+
+namespace rtk {
+
+// Instance as the kernels read it: world-to-object affine (row-major 3x4),
+// BLAS index, Lambertian albedo.
+struct InstanceDev {
+    float w2o[12];
+    int blas;
+    float albedo[3];
+};
+
+struct BlasDev {
+    const uint4* nodes;
+    const float4* tris;
+    const int* leafOf;
+    int depth;
+    int pad;
+};
+
+struct Trace2LArgs {
+    const uint4* tlasNodes = nullptr;
+    const float4* tlasTris = nullptr;
+    const InstanceDev* inst = nullptr;
+    const BlasDev* blas = nullptr;
+    int n = 0;
+    const int* nDev = nullptr;
+    const float *ox = nullptr, *oy = nullptr, *oz = nullptr, *dx = nullptr, *dy = nullptr, *dz = nullptr;
+    const float *tmin = nullptr, *tmax = nullptr;
+    float* t = nullptr; int* inst_out = nullptr; int* id = nullptr; float* u = nullptr; float* v = nullptr;
+    uint8_t* occ = nullptr;
+    int* cntNodes = nullptr; int* cntTris = nullptr;
+    uint32_t* counter = nullptr;
+    int2* spill = nullptr; int spillStride = 0; int maxStack = 0;
+    int* status = nullptr;
+};
+
+}  // namespace rtk
+
+// The opaque scene handle: a TLAS over instance boxes (one instance per
+// leaf) plus borrowed BLAS handles.
+struct CudaSceneHandle {
+    CudaBVHHandle* tlas = nullptr;
+    std::vector<CudaBVHHandle*> blas;          // borrowed
+    std::vector<rtk::InstanceDev> inst;
+    std::vector<float> o2w;                    // 12 floats per instance
+    rtk::InstanceDev* d_inst = nullptr;
+    rtk::BlasDev* d_blas = nullptr;
+    int maxBlasDepth = 0;
+    rt::TraceScratch scratch;
+    void* stream = nullptr;
+    void* d_pt = nullptr; size_t ptBytes = 0;  // path-tracer state
+};
+
+namespace rtk {
+int launch_trace2l(const CudaSceneHandle* sc, rt::TraceScratch& scratch, Trace2LArgs& a, bool any, hipStream_t s);
+}

@@ -1,0 +1,281 @@
+// scene.hip — two-level scenes: a TLAS (BVH4 over instance boxes, one
+// instance per leaf) above per-mesh BLAS handles, and its closest-hit /
+// any-hit traversal.
+//
+// One depth-first traversal per ray: reaching a TLAS leaf pushes a return
+// marker, transforms the ray into the instance's object space with the
+// world-to-object affine map (unnormalised, so t stays the world distance
+// parameter) and continues into the BLAS; popping the marker restores the
+// world ray. The closest hit is the lexicographic minimum of (t, instance,
+// primitive). The schedule is the while-while one of trace.hip (node and leaf
+// phases, idle lanes refilled from a wave-local pool of 64-ray batches).
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "rt_device.h"
+#include "rt_internal.h"
+#include "scene.h"
+
+namespace rtk {
+
+using rtd::Ray;
+
+constexpr int kRetMark = INT_MIN;
+
+__device__ __forceinline__ void xform(const float* M, const Ray& w, Ray& o) {
+    o.ox = ((M[0] * w.ox + M[1] * w.oy) + M[2] * w.oz) + M[3];
+    o.oy = ((M[4] * w.ox + M[5] * w.oy) + M[6] * w.oz) + M[7];
+    o.oz = ((M[8] * w.ox + M[9] * w.oy) + M[10] * w.oz) + M[11];
+    o.dx = (M[0] * w.dx + M[1] * w.dy) + M[2] * w.dz;
+    o.dy = (M[4] * w.dx + M[5] * w.dy) + M[6] * w.dz;
+    o.dz = (M[8] * w.dx + M[9] * w.dy) + M[10] * w.dz;
+    o.tmin = w.tmin;
+    o.tmax = w.tmax;
+    rtd::ray_setup(o);
+}
+
+template <bool ANY, class Push>
+__device__ __forceinline__ void visit2(const uint4* __restrict__ nodes, const Ray& r, float limit, int& ref, float& tn,
+                                       Push&& push) {
+    const uint4* np = nodes + 4 * (size_t)ref;
+    const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
+    if constexpr (ANY) {
+        float d0, d1, d2, d3;
+        const bool h0 = (int)qc.x != -1 && rtd::slab_child<0, true>(r, qx, qy, qz, limit, d0);
+        const bool h1 = (int)qc.y != -1 && rtd::slab_child<1, true>(r, qx, qy, qz, limit, d1);
+        const bool h2 = (int)qc.z != -1 && rtd::slab_child<2, true>(r, qx, qy, qz, limit, d2);
+        const bool h3 = (int)qc.w != -1 && rtd::slab_child<3, true>(r, qx, qy, qz, limit, d3);
+        int nxt = -1;
+        if (h0) { nxt = (int)qc.x; }
+        if (h1) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.y; }
+        if (h2) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.z; }
+        if (h3) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.w; }
+        ref = nxt;
+        tn = 0.f;
+    } else {
+        float d[4];
+        int c[4];
+        float tt;
+        if ((int)qc.x != -1 && rtd::slab_child<0, true>(r, qx, qy, qz, limit, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
+        if ((int)qc.y != -1 && rtd::slab_child<1, true>(r, qx, qy, qz, limit, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
+        if ((int)qc.z != -1 && rtd::slab_child<2, true>(r, qx, qy, qz, limit, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
+        if ((int)qc.w != -1 && rtd::slab_child<3, true>(r, qx, qy, qz, limit, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
+#define RT_CSWAP(i, j)                                                                  \
+    {                                                                                   \
+        const bool s = d[i] > d[j];                                                     \
+        const float di = d[i], dj = d[j];                                               \
+        const int ci = c[i], cj = c[j];                                                 \
+        d[i] = s ? dj : di; d[j] = s ? di : dj; c[i] = s ? cj : ci; c[j] = s ? ci : cj; \
+    }
+        RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
+#undef RT_CSWAP
+        int nxt = -1;
+        float nt = 0.f;
+        if (c[3] != -1) { nxt = c[3]; nt = d[3]; }
+        if (c[2] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[2]; nt = d[2]; }
+        if (c[1] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[1]; nt = d[1]; }
+        if (c[0] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[0]; nt = d[0]; }
+        ref = nxt;
+        tn = nt;
+    }
+}
+
+template <bool ANY, int LDSD, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
+    __shared__ int s_ref[LDSD * BLOCK];
+    __shared__ float s_tn[LDSD * BLOCK];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int gtid = blockIdx.x * BLOCK + tid;
+    const int n = a.nDev ? min(*a.nDev, a.n) : a.n;
+
+    Ray rw, ro;
+    int ri = 0, inst = -1;
+    const uint4* nodes = a.tlasNodes;
+    const float4* tris = nullptr;
+    bool has = false, exhausted = false, found = false;
+    int poolNext = 0, poolEnd = 0;
+    int sp = 0, ref = -1, bid = -1, binst = -1, visits = 0, tests = 0;
+    float tn = 0.f, best = 0.f, bu = 0.f, bv = 0.f;
+
+    auto push = [&](int rf, float t) {
+        if (sp < LDSD) {
+            s_ref[sp * BLOCK + tid] = rf;
+            s_tn[sp * BLOCK + tid] = t;
+        } else if (sp < a.maxStack) {
+            a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid] = make_int2(rf, __float_as_int(t));
+        } else {
+            atomicOr(a.status, 1);
+            return;
+        }
+        sp++;
+    };
+
+    for (;;) {
+        if (!exhausted || poolNext < poolEnd) {
+            const uint64_t idle = __ballot(!has);
+            if (idle) {
+                const int cnt = __popcll(idle);
+                const int avail = poolEnd - poolNext;
+                int fresh = n;
+                if (cnt > avail && !exhausted) {
+                    uint32_t b = 0;
+                    if (lane == 0) b = atomicAdd(a.counter, 64u);
+                    fresh = min(__builtin_amdgcn_readfirstlane((int)b), n);
+                    if (fresh + 64 >= n) exhausted = true;
+                }
+                const int k = __popcll(idle & ((1ull << lane) - 1ull));
+                int w = n;
+                if (k < avail) w = poolNext + k;
+                else if (fresh < n) w = fresh + (k - avail);
+                if (cnt > avail) {
+                    if (fresh < n) { poolNext = fresh + (cnt - avail); poolEnd = fresh + 64; }
+                    else { poolNext = 0; poolEnd = 0; }
+                } else {
+                    poolNext += cnt;
+                }
+                if (!has && w < n) {
+                    ri = w;
+                    rw.ox = a.ox[ri]; rw.oy = a.oy[ri]; rw.oz = a.oz[ri];
+                    rw.dx = a.dx[ri]; rw.dy = a.dy[ri]; rw.dz = a.dz[ri];
+                    rw.tmin = a.tmin[ri]; rw.tmax = a.tmax[ri];
+                    rtd::ray_setup(rw);
+                    has = true;
+                    sp = 0; ref = 0; tn = -INFINITY; inst = -1; nodes = a.tlasNodes;
+                    best = rw.tmax; bu = 0.f; bv = 0.f; bid = -1; binst = -1;
+                    visits = 0; tests = 0; found = false;
+                }
+            }
+        }
+        if (!__any(has)) {
+            if (exhausted && poolNext >= poolEnd) break;
+            continue;
+        }
+        bool fin = false;
+        for (;;) {
+            if (has && !fin) {
+                while (ref == -1 || ref == kRetMark || (!ANY && tn > best)) {
+                    if (ref == kRetMark) { inst = -1; nodes = a.tlasNodes; ref = -1; continue; }
+                    if (sp == 0) { fin = true; break; }
+                    sp--;
+                    if (sp < LDSD) {
+                        ref = s_ref[sp * BLOCK + tid];
+                        tn = s_tn[sp * BLOCK + tid];
+                    } else {
+                        const int2 e = a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid];
+                        ref = e.x; tn = __int_as_float(e.y);
+                    }
+                }
+            }
+            // TLAS nodes, TLAS leaves (instance entry) and BLAS nodes are node work
+            const bool atNode = has && !fin && (ref >= 0 || inst < 0);
+            if (!__any(atNode)) break;
+            if (atNode) {
+                if (ref >= 0) {
+                    visits++;
+                    const Ray& r = inst < 0 ? rw : ro;
+                    visit2<ANY>(nodes, r, ANY ? r.tmax : best, ref, tn, push);
+                } else {
+                    // TLAS leaf: enter its instance
+                    const int enc = -(ref + 2);
+                    const int k = __float_as_int(a.tlasTris[3 * (enc >> 3)].w);
+                    push(kRetMark, -INFINITY);
+                    inst = k;
+                    const InstanceDev& I = a.inst[k];
+                    xform(I.w2o, rw, ro);
+                    const BlasDev& B = a.blas[I.blas];
+                    nodes = B.nodes;
+                    tris = B.tris;
+                    ref = 0;
+                    tn = -INFINITY;
+                }
+            }
+        }
+        if (has && !fin) {
+            const int enc = -(ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
+            for (int q = 0; q < tc; q++) {
+                const int ti = ts + q;
+                tests++;
+                const float4 p0 = tris[3 * ti];
+                float t, u, v;
+                if (!rtd::isect_watertight(ro, p0, tris[3 * ti + 1], tris[3 * ti + 2], t, u, v)) continue;
+                if (!(t > ro.tmin) || !(t < ro.tmax)) continue;
+                if constexpr (ANY) {
+                    found = true;
+                    fin = true;
+                    break;
+                } else {
+                    const int id = __float_as_int(p0.w);
+                    if (t < best || (t == best && (inst < binst || (inst == binst && id < bid)))) {
+                        best = t; bid = id; binst = inst; bu = u; bv = v;
+                    }
+                }
+            }
+            ref = -1;
+        }
+        if (has && fin) {
+            if constexpr (ANY) {
+                a.occ[ri] = found ? 1 : 0;
+            } else {
+                const bool hit = bid >= 0;
+                a.t[ri] = hit ? best : RT_MISS_T;
+                if (a.inst_out) a.inst_out[ri] = binst;
+                if (a.id) a.id[ri] = bid;
+                if (a.u) a.u[ri] = hit ? bu : 0.f;
+                if (a.v) a.v[ri] = hit ? bv : 0.f;
+            }
+            if (a.cntNodes) { a.cntNodes[ri] = visits; a.cntTris[ri] = tests; }
+            has = false;
+        }
+    }
+}
+
+constexpr int kBlock2 = 256;
+
+__global__ void k_reset_counter_2l(uint32_t* c) { c[0] = 0u; }
+
+template <bool ANY, int LDSD>
+static int launch2(Trace2LArgs& a, rt::TraceScratch& sc, int maxStack, hipStream_t s) {
+    static int maxBlocks = 0;
+    auto kern = k_trace2l<ANY, LDSD, kBlock2>;
+    if (!maxBlocks) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock2, 0) != hipSuccess || nb <= 0) nb = 1;
+        maxBlocks = nb;
+    }
+    int dev = 0, cus = 1;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int grid = maxBlocks * cus, threads = grid * kBlock2;
+    a.maxStack = maxStack;
+    const int spillDepth = maxStack > LDSD ? maxStack - LDSD : 0;
+    const size_t need = (size_t)spillDepth * threads * sizeof(int2);
+    if (alloc_scratch(sc) != 0) return -1;
+    if (need > sc.spillBytes) {
+        hipFree(sc.spill);
+        sc.spill = nullptr; sc.spillBytes = 0;
+        if (hipMalloc(&sc.spill, need) != hipSuccess) { rt::set_error("cannot allocate %zu spill bytes", need); return -1; }
+        sc.spillBytes = need;
+    }
+    a.spill = (int2*)sc.spill;
+    a.spillStride = threads;
+    a.counter = sc.counter;
+    a.status = sc.status;
+    hipLaunchKernelGGL(k_reset_counter_2l, dim3(1), dim3(1), 0, s, sc.counter);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock2), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { rt::set_error("two-level trace launch failed: %s", hipGetErrorString(e)); return -1; }
+    return 0;
+}
+
+int launch_trace2l(const CudaSceneHandle* sc, rt::TraceScratch& scratch, Trace2LArgs& a, bool any, hipStream_t s) {
+    if (a.n <= 0) return 0;
+    a.tlasNodes = (const uint4*)sc->tlas->d_nodes4;
+    a.tlasTris = (const float4*)sc->tlas->d_tris;
+    a.inst = sc->d_inst;
+    a.blas = sc->d_blas;
+    const int maxStack = 3 * sc->tlas->depth4 + 4 + 1 + 3 * sc->maxBlasDepth + 4;
+    return any ? launch2<true, 8>(a, scratch, maxStack, s) : launch2<false, 8>(a, scratch, maxStack, s);
+}
+
+}  // namespace rtk

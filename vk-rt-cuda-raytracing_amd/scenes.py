@@ -141,3 +141,67 @@ def cam_array(cam) -> np.ndarray:
     """14 floats: o f r u scale_x scale_y (the oracle's camera layout)."""
     return np.concatenate([cam["o"], cam["f"], cam["r"], cam["u"],
                            np.array([cam["scale_x"], cam["scale_y"]], np.float32)]).astype(np.float32)
+
+
+# ----------------------------------------------------------------------------
+# Instanced two-level scenes (TLAS over instances of a few meshes)
+# ----------------------------------------------------------------------------
+# Four heightfield patches over [-1,1]^2 of 10,082 / 49,928 / 199,712 /
+# 500,000 triangles (grid resolutions 71 / 158 / 316 / 500).
+INSTANCED_MESH_RES = (71, 158, 316, 500)
+
+
+def random_rotation(rng) -> np.ndarray:
+    """Uniform random rotation (from a normalised Gaussian quaternion), float64 3x3."""
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def instanced_scene(mesh_res=INSTANCED_MESH_RES, n_inst: int = 64, seed: int = 2, extent: float = 1.0,
+                    scale=(0.5, 2.0), span: float = 10.0):
+    """Meshes, per-instance object-to-world maps and albedos.
+
+    Instance i uses mesh i % len(mesh_res), a seeded uniform rotation, a
+    uniform scale in `scale` and a translation uniform in [-span, span]^3;
+    albedo is uniform in [0.2, 0.8] per channel. Returns
+    dict(meshes=[(n_k, 9) float32], o2w=(n_inst, 12) float32 row-major 3x4,
+    blas_of=(n_inst,) int32, albedo=(n_inst, 3) float32)."""
+    rng = np.random.default_rng(seed)
+    meshes = [heightfield(res=r, extent=extent, seed=seed + 1 + k) for k, r in enumerate(mesh_res)]
+    o2w = np.zeros((n_inst, 12), np.float32)
+    for i in range(n_inst):
+        R = random_rotation(rng) * rng.uniform(scale[0], scale[1])
+        T = rng.uniform(-span, span, 3)
+        o2w[i] = np.concatenate([R, T[:, None]], axis=1).reshape(12).astype(np.float32)
+    blas_of = (np.arange(n_inst) % len(mesh_res)).astype(np.int32)
+    albedo = rng.uniform(0.2, 0.8, (n_inst, 3)).astype(np.float32)
+    return dict(meshes=meshes, o2w=o2w, blas_of=blas_of, albedo=albedo)
+
+
+def instance_world_tris(sc) -> tuple:
+    """All instanced triangles mapped to world space in float64 (for brute
+    force checks): (tris (m, 9) float64, inst (m,), prim (m,))."""
+    out, inst, prim = [], [], []
+    for i, b in enumerate(sc["blas_of"]):
+        M = sc["o2w"][i].astype(np.float64).reshape(3, 4)
+        v = sc["meshes"][b].astype(np.float64).reshape(-1, 3)
+        w = v @ M[:, :3].T + M[:, 3]
+        out.append(w.reshape(-1, 9))
+        inst.append(np.full(len(w) // 3, i, np.int32))
+        prim.append(np.arange(len(w) // 3, dtype=np.int32))
+    return np.concatenate(out), np.concatenate(inst), np.concatenate(prim)
+
+
+# Camera of the instanced scene: from (0,6,-26) toward the origin, 60 degree
+# vertical FOV, which frames the [-10,10]^3 instance volume.
+INSTANCED_EYE = (0.0, 6.0, -26.0)
+INSTANCED_TARGET = (0.0, 0.0, 0.0)
+SUN_DIR = (0.35, 0.85, -0.40)   # direction towards the sun (normalised on use)
+
+
+def instanced_camera(width: int = 1920, height: int = 1080):
+    return look_at(INSTANCED_EYE, INSTANCED_TARGET, width, height, 60.0)
