@@ -166,7 +166,8 @@ enum {
     CUDABVH_ISECT_MASK         = 0xF,
     CUDABVH_FLAG_HOST_POINTERS = 1u << 4, /* packets hold host pointers: copy in/out */
     CUDABVH_FLAG_SORT_RAYS     = 1u << 5, /* octant+Morton sort before tracing */
-    CUDABVH_FLAG_SYNC          = 1u << 6  /* wait for completion before returning */
+    CUDABVH_FLAG_SYNC          = 1u << 6, /* wait for completion before returning */
+    CUDABVH_FLAG_GRAPH         = 1u << 7  /* path tracer: capture one sample in a hipGraph, replay it */
 };
 
 /* Closest hit for n rays. Device pointers unless CUDABVH_FLAG_HOST_POINTERS.
@@ -295,6 +296,42 @@ int cudaBVH_sceneTraceClosestQueued(CudaScene_t scene, int capacity, const int* 
                                     const CudaRayPacket* rays, const CudaSceneHit* hits, unsigned flags);
 int cudaBVH_sceneTraceAnyQueued(CudaScene_t scene, int capacity, const int* d_count,
                                 const CudaRayPacket* rays, uint8_t* occluded, unsigned flags);
+
+/* ----- Wavefront path tracer over a scene ----- */
+#define CUDABVH_PT_MAX_DEPTH 16
+
+typedef struct {
+    int width, height;
+    int spp;              /* samples per pixel; the image is their mean */
+    int maxDepth;         /* extension rays per path, camera ray included (<= 16) */
+    int rrStart;          /* first extension ray (0 = camera ray) subject to Russian roulette */
+    uint32_t seed;
+    float sunDir[3];      /* towards the sun (normalised by the call) */
+    float sunRadiance[3]; /* irradiance on a surface facing the sun */
+    float skyZenith[3];   /* escaped-ray radiance: horizon + (zenith - horizon) * max(d.y, 0) */
+    float skyHorizon[3];
+    float clampMax;       /* per-contribution max-channel clamp; <= 0 disables */
+} CudaPathParams;
+
+/* Device buffers of width*height floats per plane. rgb: 3 planes (R, G, B).
+ * albedo/normal (3 planes) and depth (1 plane) are the first hit of sample 0
+ * (zeros where the camera ray escapes). */
+typedef struct {
+    float* rgb;
+    float* albedo;
+    float* normal;
+    float* depth;
+} CudaPathImage;
+
+/* Renders spp samples per pixel with next-event estimation toward the sun,
+ * cosine-weighted Lambertian bounces and Russian roulette. Deterministic for
+ * a given seed. rayCounts (host, nullable, 2*maxDepth entries): extension rays
+ * traced per bounce, then shadow rays per bounce, summed over samples (the
+ * call then waits for completion). With CUDABVH_FLAG_GRAPH one sample's launch
+ * sequence is captured once and replayed (re-captured when sizes, buffers or
+ * parameters other than seed and spp change). */
+int cudaBVH_scenePathTrace(CudaScene_t scene, const CudaCamera* camera, const CudaPathParams* params,
+                           const CudaPathImage* image, long long* rayCounts, unsigned flags);
 
 /* Message of the last failure on the calling thread ("" if none). */
 const char* cudaBVH_lastError(void);

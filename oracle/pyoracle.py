@@ -48,6 +48,9 @@ def lib() -> C.CDLL:
             "orc_num_threads": (I, [I]),
             "orc_trace_closest_2l": (None, [P, I, P, P, P, P, P, I, P, P, P, P, P, P, P, I]),
             "orc_trace_any_2l": (None, [P, I, P, P, P, P, P, I, P, I]),
+            "orc_path_trace_2l": (None, [P, I, P, P, P, P, P, P, P, P, P, P, I]),
+            "orc_path_trace_f64": (None, [P, P, I, P, P, P, P, I]),
+            "orc_pcg32": (None, [C.c_uint64, C.c_uint64, I, P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -303,3 +306,56 @@ class OracleScene:
                                self._arr, self.blas_of.ctypes.data, self.w2o.ctypes.data, rays.ctypes.data, n,
                                occ.ctypes.data, threads)
         return occ
+
+
+class PathParams(C.Structure):
+    """Layout of orc_pt_params (and of the library's CudaPathParams)."""
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("maxDepth", C.c_int),
+                ("rrStart", C.c_int), ("seed", C.c_uint32), ("sunDir", C.c_float * 3), ("sunRadiance", C.c_float * 3),
+                ("skyZenith", C.c_float * 3), ("skyHorizon", C.c_float * 3), ("clampMax", C.c_float)]
+
+
+def path_params(width, height, spp=1, max_depth=8, rr_start=3, seed=1, sun_dir=(0.35, 0.85, -0.40),
+                sun_radiance=(3.0, 2.85, 2.6), sky_zenith=(0.35, 0.55, 1.0), sky_horizon=(1.0, 1.0, 1.0),
+                clamp_max=20.0) -> PathParams:
+    p = PathParams()
+    p.width, p.height, p.spp, p.maxDepth, p.rrStart, p.seed = width, height, spp, max_depth, rr_start, seed
+    for dst, src in ((p.sunDir, sun_dir), (p.sunRadiance, sun_radiance), (p.skyZenith, sky_zenith),
+                     (p.skyHorizon, sky_horizon)):
+        for k in range(3):
+            dst[k] = float(src[k])
+    p.clampMax = float(clamp_max)
+    return p
+
+
+def path_trace_2l(sc: "OracleScene", albedo, cam14, params: PathParams, threads: int = 0):
+    """FP32 path tracer: returns (rgb (3, H, W), aov (7, H, W), counts (2*depth,))."""
+    W, H = params.width, params.height
+    alb = np.ascontiguousarray(albedo, np.float32).reshape(-1, 3)
+    cam = np.ascontiguousarray(cam14, np.float32)
+    rgb = np.zeros((3, H, W), np.float32)
+    aov = np.zeros((7, H, W), np.float32)
+    counts = np.zeros(2 * params.maxDepth, np.int64)
+    lib().orc_path_trace_2l(sc.tlas_nodes.ctypes.data, sc.tlas_nodes.shape[0], sc.tlas_prim.ctypes.data, sc._arr,
+                            sc.blas_of.ctypes.data, sc.w2o.ctypes.data, alb.ctypes.data, cam.ctypes.data,
+                            C.byref(params), rgb.ctypes.data, aov.ctypes.data, counts.ctypes.data, threads)
+    return rgb, aov, counts
+
+
+def path_trace_f64(world_tris, tri_inst, albedo, cam14, params: PathParams, threads: int = 0) -> np.ndarray:
+    """FP64 brute-force path tracer: rgb (3, H, W) float32."""
+    W, H = params.width, params.height
+    wt = np.ascontiguousarray(world_tris, np.float64).reshape(-1, 9)
+    ti = np.ascontiguousarray(tri_inst, np.int32)
+    alb = np.ascontiguousarray(albedo, np.float32).reshape(-1, 3)
+    cam = np.ascontiguousarray(cam14, np.float32)
+    rgb = np.zeros((3, H, W), np.float32)
+    lib().orc_path_trace_f64(wt.ctypes.data, ti.ctypes.data, wt.shape[0], alb.ctypes.data, cam.ctypes.data,
+                             C.byref(params), rgb.ctypes.data, threads)
+    return rgb
+
+
+def pcg32(initstate: int, initseq: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    lib().orc_pcg32(initstate, initseq, n, out.ctypes.data)
+    return out

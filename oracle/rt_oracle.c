@@ -772,9 +772,10 @@ static void xform_ray(const xray* w, const float* M, xray* o) {
 
 static void two_level_one(const uint8_t* tlas, const int* tlas_prim, const orc_blas* blas, const int* blas_of,
                           const float* w2o, const xray* rw, int any, sentry* stk,
-                          float* ot, int* oinst, int* oid, float* ou, float* ov, int* found_out, int* nv, int* nt) {
+                          float* ot, int* oinst, int* oid, float* ou, float* ov, int* found_out, int* nv, int* nt,
+                          int* oleaf) {
     float best = rw->tmax, bu = 0.f, bv = 0.f;
-    int bid = -1, binst = -1, visits = 0, tests = 0, sp = 0, inst = -1, found = 0;
+    int bid = -1, binst = -1, bleaf = -1, visits = 0, tests = 0, sp = 0, inst = -1, found = 0;
     xray ro;
     const xray* r = rw;
     const uint8_t* nodes = tlas;
@@ -805,7 +806,7 @@ static void two_level_one(const uint8_t* tlas, const int* tlas_prim, const orc_b
                 if (any) { found = 1; break; }
                 int id = B->prim_id[ti];
                 if (tt < best || (tt == best && (inst < binst || (inst == binst && id < bid)))) {
-                    best = tt; bid = id; binst = inst; bu = uu; bv = vv;
+                    best = tt; bid = id; binst = inst; bu = uu; bv = vv; bleaf = ti;
                 }
             }
             continue;
@@ -841,6 +842,7 @@ static void two_level_one(const uint8_t* tlas, const int* tlas_prim, const orc_b
     }
     if (nv) *nv = visits;
     if (nt) *nt = tests;
+    if (oleaf) *oleaf = bleaf;
 }
 
 static int max_nodes_2l(int tlas_num_nodes, const orc_blas* blas, const int* blas_of, int n_inst) {
@@ -863,7 +865,7 @@ void orc_trace_closest_2l(const void* tlas_nodes, int tlas_num_nodes, const int*
             xray r; setup_ray(&r, rays, (size_t)n_rays, (size_t)i);
             two_level_one((const uint8_t*)tlas_nodes, tlas_prim, blas, blas_of, w2o, &r, 0, stk, &t[i], &inst[i],
                           &id[i], &u[i], &v[i], NULL, node_visits ? &node_visits[i] : NULL,
-                          tri_tests ? &tri_tests[i] : NULL);
+                          tri_tests ? &tri_tests[i] : NULL, NULL);
         }
         free(stk);
     }
@@ -882,9 +884,320 @@ void orc_trace_any_2l(const void* tlas_nodes, int tlas_num_nodes, const int* tla
             xray r; setup_ray(&r, rays, (size_t)n_rays, (size_t)i);
             int f = 0;
             two_level_one((const uint8_t*)tlas_nodes, tlas_prim, blas, blas_of, w2o, &r, 1, stk, NULL, NULL, NULL,
-                          NULL, NULL, &f, NULL, NULL);
+                          NULL, NULL, &f, NULL, NULL, NULL);
             occluded[i] = (uint8_t)f;
         }
         free(stk);
     }
+}
+
+/* ===================================================================== */
+/* Path tracer                                                            */
+/* ===================================================================== */
+/* Structure of the reference's wavefront renderer (cuda_rt_v34.cu: state
+ * :62-97, generate :1077-1150, shade with next-event estimation and Russian
+ * roulette :1185-1455, shadow :1458-1480), with the lighting and sampling of
+ * this project (directional sun, sky gradient, Lambertian per-instance albedo,
+ * PCG32 + Malley cosine sampling). Per pixel and sample, one path: the
+ * contributions are added in the order the wavefront kernels add them (each
+ * bounce: an escaped ray's sky term, or the hit's unoccluded sun term). */
+
+typedef struct { uint64_t state, inc; } pcg32;
+
+static uint32_t pcg_next(pcg32* r) {
+    uint64_t old = r->state;
+    r->state = old * 6364136223846793005ULL + r->inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+static void pcg_seed(pcg32* r, uint64_t initstate, uint64_t initseq) {
+    r->state = 0u;
+    r->inc = (initseq << 1u) | 1u;
+    pcg_next(r);
+    r->state += initstate;
+    pcg_next(r);
+}
+static float pcg_uniform(pcg32* r) { return (float)(pcg_next(r) >> 8) * 0x1p-24f; }
+
+/* Malley's method in the basis of Duff et al. 2017 (float32). */
+static void cosine_dir_f(const float n[3], pcg32* rng, float d[3]) {
+    float x = 0.f, y = 0.f, s = 0.f;
+    int ok = 0;
+    for (int k = 0; k < 64 && !ok; k++) {
+        x = 2.0f * pcg_uniform(rng) - 1.0f;
+        y = 2.0f * pcg_uniform(rng) - 1.0f;
+        s = x * x + y * y;
+        ok = s < 1.0f;
+    }
+    if (!ok) { x = 0.f; y = 0.f; s = 0.f; }
+    float z = sqrtf(1.0f - s);
+    float sg = copysignf(1.0f, n[2]);
+    float ca = -1.0f / (sg + n[2]);
+    float cb = n[0] * n[1] * ca;
+    float t1x = 1.0f + sg * n[0] * n[0] * ca, t1y = sg * cb, t1z = -sg * n[0];
+    float t2x = cb, t2y = sg + n[1] * n[1] * ca, t2z = -n[1];
+    d[0] = (x * t1x + y * t2x) + z * n[0];
+    d[1] = (x * t1y + y * t2y) + z * n[1];
+    d[2] = (x * t1z + y * t2z) + z * n[2];
+    float dl = 1.0f / sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+    d[0] = d[0] * dl; d[1] = d[1] * dl; d[2] = d[2] * dl;
+}
+
+static void sun_unit(const orc_pt_params* p, float out[3]) {
+    double sl = sqrt((double)p->sun_dir[0] * p->sun_dir[0] + (double)p->sun_dir[1] * p->sun_dir[1] +
+                     (double)p->sun_dir[2] * p->sun_dir[2]);
+    for (int k = 0; k < 3; k++) out[k] = (float)(p->sun_dir[k] / sl);
+}
+
+static void clamp3f(float c[3], float m) {
+    float mx = fmaxf(c[0], fmaxf(c[1], c[2]));
+    if (m > 0.0f && mx > m) { float s = m / mx; c[0] = c[0] * s; c[1] = c[1] * s; c[2] = c[2] * s; }
+}
+
+static void make_xray(xray* r, const float o[3], const float d[3], float tmin, float tmax) {
+    float a[8] = {o[0], o[1], o[2], d[0], d[1], d[2], tmin, tmax};
+    setup_ray(r, a, 1, 0);
+}
+
+void orc_path_trace_2l(const void* tlas_nodes, int tlas_num_nodes, const int* tlas_prim, const orc_blas* blas,
+                       const int* blas_of, const float* w2o, const float* albedo, const float* cam,
+                       const orc_pt_params* p, float* rgb, float* aov, long long* counts, int threads) {
+    const int W = p->width, H = p->height, N = W * H;
+    const float INV_PI = 0.318309886183790671538f;
+    float sun[3];
+    sun_unit(p, sun);
+    int nth = orc_num_threads(threads);
+    int cap = 3 * (tlas_num_nodes + 4096) + 16;
+    long long tot[2 * ORC_PT_MAX_DEPTH] = {0};
+#pragma omp parallel num_threads(nth)
+    {
+        sentry* stk = (sentry*)malloc(sizeof(sentry) * (size_t)cap * 4);
+        long long loc[2 * ORC_PT_MAX_DEPTH] = {0};
+#pragma omp for schedule(dynamic, 64)
+        for (int i = 0; i < N; i++) {
+            const int py = i / W, px = i - py * W;
+            float acc[3] = {0.f, 0.f, 0.f};
+            for (int s = 0; s < p->spp; s++) {
+                pcg32 rng;
+                pcg_seed(&rng, ((uint64_t)p->seed << 32) ^ (uint64_t)(uint32_t)s, (uint64_t)i);
+                float jx = pcg_uniform(&rng), jy = pcg_uniform(&rng);
+                float sx = (((float)px + jx) / (float)W) * 2.0f - 1.0f;
+                float sy = (((float)py + jy) / (float)H) * 2.0f - 1.0f;
+                float ax = sx * cam[12], ay = sy * cam[13];
+                float d[3], o[3] = {cam[0], cam[1], cam[2]};
+                for (int k = 0; k < 3; k++) d[k] = (cam[3 + k] + ax * cam[6 + k]) + ay * cam[9 + k];
+                float il = 1.0f / sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+                for (int k = 0; k < 3; k++) d[k] = d[k] * il;
+                float thr[3] = {1.f, 1.f, 1.f};
+                for (int b = 0; b < p->max_depth; b++) {
+                    loc[b]++;
+                    xray r;
+                    make_xray(&r, o, d, 0.0f, MISS_T);
+                    float t, u, v; int inst, id, leaf;
+                    two_level_one((const uint8_t*)tlas_nodes, tlas_prim, blas, blas_of, w2o, &r, 0, stk, &t, &inst,
+                                  &id, &u, &v, NULL, NULL, NULL, &leaf);
+                    const int aovw = aov && b == 0 && s == 0;
+                    if (id < 0) {
+                        float e = fmaxf(d[1], 0.0f), c[3];
+                        for (int k = 0; k < 3; k++)
+                            c[k] = thr[k] * (p->sky_horizon[k] + (p->sky_zenith[k] - p->sky_horizon[k]) * e);
+                        clamp3f(c, p->clamp_max);
+                        for (int k = 0; k < 3; k++) acc[k] += c[k];
+                        if (aovw) for (int k = 0; k < 7; k++) aov[(size_t)k * N + i] = 0.0f;
+                        break;
+                    }
+                    const orc_blas* B = &blas[blas_of[inst]];
+                    const float* tri = B->tris + 9 * (size_t)leaf;
+                    float e1x = tri[3] - tri[0], e1y = tri[4] - tri[1], e1z = tri[5] - tri[2];
+                    float e2x = tri[6] - tri[0], e2y = tri[7] - tri[1], e2z = tri[8] - tri[2];
+                    float mx = e1y * e2z - e1z * e2y, my = e1z * e2x - e1x * e2z, mz = e1x * e2y - e1y * e2x;
+                    const float* M = w2o + 12 * (size_t)inst;
+                    float n[3];
+                    n[0] = (M[0] * mx + M[4] * my) + M[8] * mz;
+                    n[1] = (M[1] * mx + M[5] * my) + M[9] * mz;
+                    n[2] = (M[2] * mx + M[6] * my) + M[10] * mz;
+                    float nl = 1.0f / sqrtf((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
+                    for (int k = 0; k < 3; k++) n[k] = n[k] * nl;
+                    if ((n[0] * d[0] + n[1] * d[1]) + n[2] * d[2] > 0.0f) for (int k = 0; k < 3; k++) n[k] = -n[k];
+                    float h[3];
+                    for (int k = 0; k < 3; k++) h[k] = o[k] + t * d[k];
+                    float eps = 1e-4f * fmaxf(1.0f, fmaxf(fabsf(h[0]), fmaxf(fabsf(h[1]), fabsf(h[2]))));
+                    float q[3];
+                    for (int k = 0; k < 3; k++) q[k] = h[k] + n[k] * eps;
+                    const float* alb = albedo + 3 * (size_t)inst;
+                    if (aovw) {
+                        for (int k = 0; k < 3; k++) {
+                            aov[(size_t)k * N + i] = alb[k];
+                            aov[(size_t)(3 + k) * N + i] = n[k];
+                        }
+                        aov[(size_t)6 * N + i] = t;
+                    }
+                    float cosL = (n[0] * sun[0] + n[1] * sun[1]) + n[2] * sun[2];
+                    if (cosL > 0.0f) {
+                        float kf = cosL * INV_PI, c[3];
+                        for (int k = 0; k < 3; k++) c[k] = ((thr[k] * alb[k]) * p->sun_rad[k]) * kf;
+                        clamp3f(c, p->clamp_max);
+                        loc[ORC_PT_MAX_DEPTH + b]++;
+                        xray sr;
+                        make_xray(&sr, q, sun, 0.0f, MISS_T);
+                        int occl = 0;
+                        two_level_one((const uint8_t*)tlas_nodes, tlas_prim, blas, blas_of, w2o, &sr, 1, stk, NULL,
+                                      NULL, NULL, NULL, NULL, &occl, NULL, NULL, NULL);
+                        if (!occl) for (int k = 0; k < 3; k++) acc[k] += c[k];
+                    }
+                    if (b + 1 >= p->max_depth) break;
+                    float nd[3];
+                    cosine_dir_f(n, &rng, nd);
+                    for (int k = 0; k < 3; k++) thr[k] = thr[k] * alb[k];
+                    if (b + 1 >= p->rr_start) {
+                        float pm = fminf(fmaxf(thr[0], fmaxf(thr[1], thr[2])), 1.0f);
+                        float uu = pcg_uniform(&rng);
+                        if (uu >= pm) break;
+                        for (int k = 0; k < 3; k++) thr[k] = thr[k] / pm;
+                    }
+                    for (int k = 0; k < 3; k++) { o[k] = q[k]; d[k] = nd[k]; }
+                }
+            }
+            if (p->spp > 1) {
+                float inv = 1.0f / (float)p->spp;
+                for (int k = 0; k < 3; k++) acc[k] = acc[k] * inv;
+            }
+            for (int k = 0; k < 3; k++) rgb[(size_t)k * N + i] = acc[k];
+        }
+#pragma omp critical
+        for (int k = 0; k < 2 * ORC_PT_MAX_DEPTH; k++) tot[k] += loc[k];
+        free(stk);
+    }
+    if (counts) {
+        for (int k = 0; k < p->max_depth; k++) {
+            counts[k] = tot[k];
+            counts[p->max_depth + k] = tot[ORC_PT_MAX_DEPTH + k];
+        }
+    }
+}
+
+/* ---- FP64 path tracer: brute force over world-space triangles ---- */
+static int mt_f64(const double* o, const double* d, const double* tri, double tmin, double tmax, double* tout) {
+    double e1[3], e2[3], pv[3], tv[3], qv[3];
+    for (int k = 0; k < 3; k++) { e1[k] = tri[3 + k] - tri[k]; e2[k] = tri[6 + k] - tri[k]; }
+    pv[0] = d[1] * e2[2] - d[2] * e2[1]; pv[1] = d[2] * e2[0] - d[0] * e2[2]; pv[2] = d[0] * e2[1] - d[1] * e2[0];
+    double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+    if (det == 0.0) return 0;
+    double id = 1.0 / det;
+    for (int k = 0; k < 3; k++) tv[k] = o[k] - tri[k];
+    double u = (tv[0] * pv[0] + tv[1] * pv[1] + tv[2] * pv[2]) * id;
+    if (u < 0.0 || u > 1.0) return 0;
+    qv[0] = tv[1] * e1[2] - tv[2] * e1[1]; qv[1] = tv[2] * e1[0] - tv[0] * e1[2]; qv[2] = tv[0] * e1[1] - tv[1] * e1[0];
+    double v = (d[0] * qv[0] + d[1] * qv[1] + d[2] * qv[2]) * id;
+    if (v < 0.0 || u + v > 1.0) return 0;
+    double t = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * id;
+    if (!(t > tmin) || !(t < tmax)) return 0;
+    *tout = t;
+    return 1;
+}
+
+void orc_path_trace_f64(const double* wtris, const int* tri_inst, int m, const float* albedo, const float* cam,
+                        const orc_pt_params* p, float* rgb, int threads) {
+    const int W = p->width, H = p->height, N = W * H;
+    double sun[3];
+    {
+        double sl = sqrt((double)p->sun_dir[0] * p->sun_dir[0] + (double)p->sun_dir[1] * p->sun_dir[1] +
+                         (double)p->sun_dir[2] * p->sun_dir[2]);
+        for (int k = 0; k < 3; k++) sun[k] = p->sun_dir[k] / sl;
+    }
+    int nth = orc_num_threads(threads);
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 16)
+    for (int i = 0; i < N; i++) {
+        const int py = i / W, px = i - py * W;
+        double acc[3] = {0, 0, 0};
+        for (int s = 0; s < p->spp; s++) {
+            pcg32 rng;
+            pcg_seed(&rng, ((uint64_t)p->seed << 32) ^ (uint64_t)(uint32_t)s, (uint64_t)i);
+            double jx = pcg_uniform(&rng), jy = pcg_uniform(&rng);
+            double sx = ((px + jx) / W) * 2.0 - 1.0, sy = ((py + jy) / H) * 2.0 - 1.0;
+            double ax = sx * cam[12], ay = sy * cam[13];
+            double o[3] = {cam[0], cam[1], cam[2]}, d[3];
+            for (int k = 0; k < 3; k++) d[k] = cam[3 + k] + ax * cam[6 + k] + ay * cam[9 + k];
+            double dl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            for (int k = 0; k < 3; k++) d[k] /= dl;
+            double thr[3] = {1, 1, 1};
+            for (int b = 0; b < p->max_depth; b++) {
+                double best = 1e300; int bi = -1;
+                for (int j = 0; j < m; j++) {
+                    double t;
+                    if (mt_f64(o, d, wtris + 9 * (size_t)j, 0.0, best, &t)) { best = t; bi = j; }
+                }
+                if (bi < 0) {
+                    double e = d[1] > 0 ? d[1] : 0;
+                    double c[3];
+                    for (int k = 0; k < 3; k++)
+                        c[k] = thr[k] * (p->sky_horizon[k] + (p->sky_zenith[k] - p->sky_horizon[k]) * e);
+                    double mx = fmax(c[0], fmax(c[1], c[2]));
+                    if (p->clamp_max > 0 && mx > p->clamp_max) for (int k = 0; k < 3; k++) c[k] *= p->clamp_max / mx;
+                    for (int k = 0; k < 3; k++) acc[k] += c[k];
+                    break;
+                }
+                const double* tri = wtris + 9 * (size_t)bi;
+                double e1[3], e2[3], n[3];
+                for (int k = 0; k < 3; k++) { e1[k] = tri[3 + k] - tri[k]; e2[k] = tri[6 + k] - tri[k]; }
+                n[0] = e1[1] * e2[2] - e1[2] * e2[1]; n[1] = e1[2] * e2[0] - e1[0] * e2[2]; n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+                double nl = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                for (int k = 0; k < 3; k++) n[k] /= nl;
+                if (n[0] * d[0] + n[1] * d[1] + n[2] * d[2] > 0) for (int k = 0; k < 3; k++) n[k] = -n[k];
+                double h[3], q[3];
+                for (int k = 0; k < 3; k++) h[k] = o[k] + best * d[k];
+                double eps = 1e-4 * fmax(1.0, fmax(fabs(h[0]), fmax(fabs(h[1]), fabs(h[2]))));
+                for (int k = 0; k < 3; k++) q[k] = h[k] + n[k] * eps;
+                const float* alb = albedo + 3 * (size_t)tri_inst[bi];
+                double cosL = n[0] * sun[0] + n[1] * sun[1] + n[2] * sun[2];
+                if (cosL > 0) {
+                    double c[3];
+                    for (int k = 0; k < 3; k++) c[k] = thr[k] * alb[k] * p->sun_rad[k] * cosL / 3.14159265358979323846;
+                    double mx = fmax(c[0], fmax(c[1], c[2]));
+                    if (p->clamp_max > 0 && mx > p->clamp_max) for (int k = 0; k < 3; k++) c[k] *= p->clamp_max / mx;
+                    int occl = 0;
+                    for (int j = 0; j < m && !occl; j++) {
+                        double t;
+                        occl = mt_f64(q, sun, wtris + 9 * (size_t)j, 0.0, 1e300, &t);
+                    }
+                    if (!occl) for (int k = 0; k < 3; k++) acc[k] += c[k];
+                }
+                if (b + 1 >= p->max_depth) break;
+                /* Malley + Duff basis in double, same random numbers */
+                double x = 0, y = 0, ss = 0;
+                int ok = 0;
+                for (int k = 0; k < 64 && !ok; k++) {
+                    x = 2.0 * pcg_uniform(&rng) - 1.0;
+                    y = 2.0 * pcg_uniform(&rng) - 1.0;
+                    ss = x * x + y * y;
+                    ok = ss < 1.0;
+                }
+                if (!ok) { x = 0; y = 0; ss = 0; }
+                double z = sqrt(1.0 - ss), sg = n[2] >= 0 ? 1.0 : -1.0;
+                double ca = -1.0 / (sg + n[2]), cb = n[0] * n[1] * ca;
+                double t1[3] = {1.0 + sg * n[0] * n[0] * ca, sg * cb, -sg * n[0]};
+                double t2[3] = {cb, sg + n[1] * n[1] * ca, -n[1]};
+                double nd[3];
+                for (int k = 0; k < 3; k++) nd[k] = x * t1[k] + y * t2[k] + z * n[k];
+                double ndl = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
+                for (int k = 0; k < 3; k++) { nd[k] /= ndl; thr[k] *= alb[k]; }
+                if (b + 1 >= p->rr_start) {
+                    double pm = fmin(fmax(thr[0], fmax(thr[1], thr[2])), 1.0);
+                    double uu = pcg_uniform(&rng);
+                    if (uu >= pm) break;
+                    for (int k = 0; k < 3; k++) thr[k] /= pm;
+                }
+                for (int k = 0; k < 3; k++) { o[k] = q[k]; d[k] = nd[k]; }
+            }
+        }
+        for (int k = 0; k < 3; k++) rgb[(size_t)k * N + i] = (float)(acc[k] / p->spp);
+    }
+}
+
+/* PCG32 outputs for a known-answer check of the generator. */
+void orc_pcg32(uint64_t initstate, uint64_t initseq, int n, uint32_t* out) {
+    pcg32 r;
+    pcg_seed(&r, initstate, initseq);
+    for (int i = 0; i < n; i++) out[i] = pcg_next(&r);
 }

@@ -136,6 +136,34 @@ void orc_trace_any_2l(const void* tlas_nodes, int tlas_num_nodes, const int* tla
                       const orc_blas* blas, const int* blas_of, const float* w2o,
                       const float* rays, int n_rays, uint8_t* occluded, int threads);
 
+/* ---------------- Path tracer ----------------
+ * Same layout as the library's CudaPathParams. */
+#define ORC_PT_MAX_DEPTH 16
+typedef struct {
+    int width, height, spp, max_depth, rr_start;
+    uint32_t seed;
+    float sun_dir[3], sun_rad[3], sky_zenith[3], sky_horizon[3];
+    float clamp_max;
+} orc_pt_params;
+
+/* FP32 restatement of the wavefront path tracer over a two-level scene
+ * (albedo: 3 floats per instance; cam: 14 floats as orc_gen_pinhole).
+ * rgb: 3 planes of W*H; aov (nullable): 7 planes (albedo 3, normal 3, depth)
+ * of sample 0's first hit; counts (nullable): 2*max_depth ray counts
+ * (extension rays per bounce, then shadow rays per bounce). */
+void orc_path_trace_2l(const void* tlas_nodes, int tlas_num_nodes, const int* tlas_prim, const orc_blas* blas,
+                       const int* blas_of, const float* w2o, const float* albedo, const float* cam,
+                       const orc_pt_params* p, float* rgb, float* aov, long long* counts, int threads);
+
+/* The same estimator in FP64 with brute-force Moller-Trumbore over
+ * world-space triangles (wtris: m x 9 doubles; tri_inst: instance of each),
+ * independent of the BVHs and of the FP32 operation order. */
+void orc_path_trace_f64(const double* wtris, const int* tri_inst, int m, const float* albedo, const float* cam,
+                        const orc_pt_params* p, float* rgb, int threads);
+
+/* PCG32 (pcg32_srandom_r / pcg32_random_r): n outputs. */
+void orc_pcg32(uint64_t initstate, uint64_t initseq, int n, uint32_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,8 @@ ISECT_REFERENCE_MT = 1
 FLAG_HOST_POINTERS = 1 << 4
 FLAG_SORT_RAYS = 1 << 5
 FLAG_SYNC = 1 << 6
+FLAG_GRAPH = 1 << 7
+PT_MAX_DEPTH = 16
 MISS_T = np.float32(1e30)
 
 
@@ -64,6 +66,30 @@ class CudaInstance(C.Structure):
 
 class CudaSceneHit(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("t", "instId", "primId", "u", "v")]
+
+
+class CudaPathParams(C.Structure):
+    _fields_ = [("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("maxDepth", C.c_int),
+                ("rrStart", C.c_int), ("seed", C.c_uint32), ("sunDir", C.c_float * 3), ("sunRadiance", C.c_float * 3),
+                ("skyZenith", C.c_float * 3), ("skyHorizon", C.c_float * 3), ("clampMax", C.c_float)]
+
+
+class CudaPathImage(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("rgb", "albedo", "normal", "depth")]
+
+
+def path_params(width: int, height: int, spp: int = 1, max_depth: int = 8, rr_start: int = 3, seed: int = 1,
+                sun_dir=(0.35, 0.85, -0.40), sun_radiance=(3.0, 2.85, 2.6), sky_zenith=(0.35, 0.55, 1.0),
+                sky_horizon=(1.0, 1.0, 1.0), clamp_max: float = 20.0) -> CudaPathParams:
+    """Path-tracer parameters; the defaults are the instanced-scene renders'."""
+    p = CudaPathParams()
+    p.width, p.height, p.spp, p.maxDepth, p.rrStart, p.seed = width, height, spp, max_depth, rr_start, seed
+    for dst, src in ((p.sunDir, sun_dir), (p.sunRadiance, sun_radiance), (p.skyZenith, sky_zenith),
+                     (p.skyHorizon, sky_horizon)):
+        for k in range(3):
+            dst[k] = float(src[k])
+    p.clampMax = float(clamp_max)
+    return p
 
 
 # name -> (restype, argtypes); the exported C ABI, in header order.
@@ -126,6 +152,8 @@ SIGNATURES = {
     "cudaBVH_sceneTraceClosestQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), C.POINTER(CudaSceneHit),
                                              C.c_uint]),
     "cudaBVH_sceneTraceAnyQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), _P, C.c_uint]),
+    "cudaBVH_scenePathTrace": (_I, [_H, C.POINTER(CudaCamera), C.POINTER(CudaPathParams), C.POINTER(CudaPathImage),
+                                    _P, C.c_uint]),
     "cudaBVH_lastError": (C.c_char_p, []),
 }
 
@@ -465,6 +493,20 @@ class Scene:
         rp = ray_packet(rays)
         _check(self.lib.cudaBVH_sceneTraceAnyQueued(self.handle, capacity, _ptr(d_count), C.byref(rp),
                                                     _ptr(occluded), flags), "sceneTraceAnyQueued")
+
+
+def _scene_path_trace(self, cam: CudaCamera, params: CudaPathParams, rgb, albedo, normal, depth,
+                      counts: bool = False, flags: int = 0):
+    """Renders into device buffers rgb/albedo/normal (3*W*H) and depth (W*H);
+    returns the per-bounce ray counts (extension, then shadow) when asked."""
+    img = CudaPathImage(_ptr(rgb), _ptr(albedo), _ptr(normal), _ptr(depth))
+    cnt = np.zeros(2 * params.maxDepth, np.int64) if counts else None
+    _check(self.lib.cudaBVH_scenePathTrace(self.handle, C.byref(cam), C.byref(params), C.byref(img), _ptr(cnt),
+                                           flags), "scenePathTrace")
+    return cnt
+
+
+Scene.path_trace = _scene_path_trace
 
 
 def gen_pinhole_rays(width: int, height: int, cam: CudaCamera, rays, stream: int | None = None) -> None:

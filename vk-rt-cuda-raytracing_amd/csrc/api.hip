@@ -24,15 +24,6 @@ const char* last_error();
 
 using rt::set_error;
 
-#define HIP_TRY(expr, ...)                                                        \
-    do {                                                                          \
-        hipError_t _e = (expr);                                                   \
-        if (_e != hipSuccess) {                                                   \
-            set_error("%s failed: %s", #expr, hipGetErrorString(_e));             \
-            __VA_ARGS__;                                                          \
-        }                                                                         \
-    } while (0)
-
 static inline hipStream_t hstream(const CudaBVHHandle* h) { return (hipStream_t)h->stream; }
 
 static void free_handle_device(CudaBVHHandle* h) {
@@ -919,7 +910,8 @@ CudaScene_t cudaBVH_sceneCreate(const CudaBVH_t* blas, int numBlas, const CudaIn
 void cudaBVH_sceneDestroy(CudaScene_t sc) {
     if (!sc) return;
     if (sc->stream) hipStreamSynchronize((hipStream_t)sc->stream); else hipDeviceSynchronize();
-    hipFree(sc->d_inst); hipFree(sc->d_blas); hipFree(sc->d_pt);
+    rtk::pt_free(sc);
+    hipFree(sc->d_inst); hipFree(sc->d_blas);
     rtk::free_scratch(sc->scratch);
     cudaBVH_destroy(sc->tlas);
     delete sc;
@@ -983,6 +975,18 @@ int cudaBVH_sceneTraceAnyQueued(CudaScene_t sc, int capacity, const int* d_count
     if (!d_count) { set_error("cudaBVH_sceneTraceAnyQueued: NULL count"); return -1; }
     return scene_trace(sc, capacity, d_count, rays, nullptr, occluded, nullptr, nullptr, flags, true,
                        "cudaBVH_sceneTraceAnyQueued");
+}
+
+int cudaBVH_scenePathTrace(CudaScene_t sc, const CudaCamera* cam, const CudaPathParams* params,
+                           const CudaPathImage* image, long long* rayCounts, unsigned flags) {
+    rt::clear_error();
+    if (!scene_ok(sc, "cudaBVH_scenePathTrace")) return -1;
+    if (!cam || !params || !image) { set_error("cudaBVH_scenePathTrace: NULL argument"); return -1; }
+    if (flags & ~(unsigned)(CUDABVH_FLAG_SYNC | CUDABVH_FLAG_GRAPH)) {
+        set_error("cudaBVH_scenePathTrace: unsupported flags 0x%x", flags);
+        return -1;
+    }
+    return rtk::path_trace(sc, *cam, *params, *image, rayCounts, flags);
 }
 
 }  // extern "C"

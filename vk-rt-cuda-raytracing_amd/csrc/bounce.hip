@@ -7,11 +7,11 @@
 // memory, where the queued traces read it), then emit rays at their compacted
 // positions. The diffuse direction follows the reference's diffuse-ray
 // generator (cuda_rt_v37.cu:514-540: per-ray seeded RNG, hemisphere around
-// the surface normal) with PCG32 and Malley's method: a uniform point of the
-// unit disk by rejection, lifted to the hemisphere, in the orthonormal basis
-// of Duff et al. 2017. Only IEEE +,-,*,/,sqrt are used, so a host
-// restatement reproduces every ray bit for bit.
+// the surface normal) with PCG32 and Malley's method (rt_sample.h). Only IEEE
+// +,-,*,/,sqrt are used, so a host restatement reproduces every ray bit for
+// bit.
 #include "rt_device.h"
+#include "rt_sample.h"
 #include "rt_scan.h"
 #include "trace_launch.h"
 
@@ -20,25 +20,6 @@ namespace rtk {
 constexpr int kTileThreads = 256;
 constexpr int kTileItems = 8;
 constexpr int kTile = kTileThreads * kTileItems;
-
-struct Pcg32 {
-    uint64_t state, inc;
-    __device__ __forceinline__ uint32_t next() {
-        const uint64_t old = state;
-        state = old * 6364136223846793005ULL + inc;
-        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
-        const uint32_t rot = (uint32_t)(old >> 59u);
-        return (xs >> rot) | (xs << ((0u - rot) & 31u));
-    }
-    __device__ __forceinline__ void seed(uint64_t initstate, uint64_t initseq) {
-        state = 0u;
-        inc = (initseq << 1u) | 1u;
-        next();
-        state += initstate;
-        next();
-    }
-    __device__ __forceinline__ float uniform() { return (float)(next() >> 8) * 0x1p-24f; }
-};
 
 struct BounceArgs {
     rtd::Cam cam;
@@ -100,28 +81,10 @@ __device__ __forceinline__ void emit(const BounceArgs& a, int i, int pos) {
     nx = nx * nl; ny = ny * nl; nz = nz * nl;
     if ((nx * r.dx + ny * r.dy) + nz * r.dz > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
 
-    Pcg32 rng;
+    rtd::Pcg32 rng;
     rng.seed((uint64_t)a.seed, (uint64_t)i);
-    float x = 0.0f, y = 0.0f, s = 0.0f;
-    bool ok = false;
-    for (int k = 0; k < 64 && !ok; k++) {                // bounded rejection
-        x = 2.0f * rng.uniform() - 1.0f;
-        y = 2.0f * rng.uniform() - 1.0f;
-        s = x * x + y * y;
-        ok = s < 1.0f;
-    }
-    if (!ok) { x = 0.0f; y = 0.0f; s = 0.0f; }
-    const float z = sqrtf(1.0f - s);
-    const float sg = copysignf(1.0f, nz);
-    const float ca = -1.0f / (sg + nz);
-    const float cb = nx * ny * ca;
-    const float t1x = 1.0f + sg * nx * nx * ca, t1y = sg * cb, t1z = -sg * nx;
-    const float t2x = cb, t2y = sg + ny * ny * ca, t2z = -ny;
-    float dx = (x * t1x + y * t2x) + z * nx;
-    float dy = (x * t1y + y * t2y) + z * ny;
-    float dz = (x * t1z + y * t2z) + z * nz;
-    const float dl = 1.0f / sqrtf((dx * dx + dy * dy) + dz * dz);
-    dx = dx * dl; dy = dy * dl; dz = dz * dl;
+    float dx, dy, dz;
+    rtd::cosine_dir(nx, ny, nz, rng, dx, dy, dz);
     a.dox[pos] = hx; a.doy[pos] = hy; a.doz[pos] = hz;
     a.ddx[pos] = dx; a.ddy[pos] = dy; a.ddz[pos] = dz;
     a.dtmin[pos] = 1e-3f; a.dtmax[pos] = RT_MISS_T;

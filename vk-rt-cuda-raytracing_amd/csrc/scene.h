@@ -10,6 +10,16 @@
 #include "rt_internal.h"
 #include "trace_launch.h"
 
+// Runs expr (a HIP call); on failure records the error and runs the rest.
+#define HIP_TRY(expr, ...)                                                        \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) {                                                   \
+            rt::set_error("%s failed: %s", #expr, hipGetErrorString(_e));         \
+            __VA_ARGS__;                                                          \
+        }                                                                         \
+    } while (0)
+
 namespace rtk {
 
 // Instance as the kernels read it: world-to-object affine (row-major 3x4),
@@ -59,9 +69,12 @@ struct CudaSceneHandle {
     int maxBlasDepth = 0;
     rt::TraceScratch scratch;
     void* stream = nullptr;
-    void* d_pt = nullptr; size_t ptBytes = 0;  // path-tracer state
+    void* pt = nullptr;                        // path-tracer queues and graph (pt.hip)
 };
 
 namespace rtk {
 int launch_trace2l(const CudaSceneHandle* sc, rt::TraceScratch& scratch, Trace2LArgs& a, bool any, hipStream_t s);
+int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams& p, const CudaPathImage& img,
+               long long* rayCounts, unsigned flags);
+void pt_free(CudaSceneHandle* sc);
 }
