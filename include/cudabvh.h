@@ -333,6 +333,40 @@ typedef struct {
 int cudaBVH_scenePathTrace(CudaScene_t scene, const CudaCamera* camera, const CudaPathParams* params,
                            const CudaPathImage* image, long long* rayCounts, unsigned flags);
 
+/* ----- Learned denoiser ----- */
+/* A 4-level convolutional encoder-decoder (32/64/128/256 channels, two 3x3
+ * convolutions + ReLU per level, 2x2 max pooling, 2x nearest upsampling, skip
+ * concatenation, 1x1 head added to the log-radiance input), BF16 weights and
+ * activations, FP32 accumulation on MFMA, FP32 master weights trained with
+ * Adam by the library's own forward and backward kernels.
+ * Pixels are NHWC with 16 BF16 channels: log(1+rgb), albedo, normal,
+ * log(1+depth)/4, zeros (cudaBVH_denoiserPack builds them from planes).
+ * Images: height and width multiples of 8; batch*height*width <= maxPixels.
+ * Parameters: per layer, weights [cout][taps][cin] then biases [cout]. */
+typedef struct CudaDenoiserHandle* CudaDenoiser_t;
+
+CudaDenoiser_t cudaBVH_denoiserCreate(int maxPixels, uint64_t seed);
+void cudaBVH_denoiserDestroy(CudaDenoiser_t dn);
+int cudaBVH_denoiserSetStream(CudaDenoiser_t dn, void* hipStream);
+long long cudaBVH_denoiserNumParams(CudaDenoiser_t dn);
+/* which: 0 weights, 1 gradients of the last training step, 2/3 Adam moments */
+int cudaBVH_denoiserGetParams(CudaDenoiser_t dn, float* hostOut, int which);
+int cudaBVH_denoiserSetParams(CudaDenoiser_t dn, const float* hostIn);   /* resets Adam */
+/* planes of numPixels floats (rgb/albedo/normal: 3 planes) -> 16-channel pixels */
+int cudaBVH_denoiserPack(CudaDenoiser_t dn, int numPixels, const float* rgb, const float* albedo,
+                         const float* normal, const float* depth, uint16_t* out);
+/* head output (before the residual), FP32 [pixels][3]; input stays on the device */
+int cudaBVH_denoiserForward(CudaDenoiser_t dn, int batch, int height, int width, const uint16_t* input,
+                            float* headOut);
+/* One Adam step (lr <= 0: gradients only) on the L1 loss between
+ * head + log(1+rgb) and target (FP32 [pixels][3], log(1+radiance)).
+ * lossOut (host, nullable) makes the call wait for the loss. */
+int cudaBVH_denoiserTrainStep(CudaDenoiser_t dn, int batch, int height, int width, const uint16_t* input,
+                              const float* target, float learningRate, float* lossOut);
+/* Denoise one image of planes: out (3 planes) = exp(head + log(1+rgb)) - 1. */
+int cudaBVH_denoise(CudaDenoiser_t dn, int width, int height, const float* rgb, const float* albedo,
+                    const float* normal, const float* depth, float* out);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char* cudaBVH_lastError(void);
 

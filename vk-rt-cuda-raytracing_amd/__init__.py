@@ -154,6 +154,16 @@ SIGNATURES = {
     "cudaBVH_sceneTraceAnyQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), _P, C.c_uint]),
     "cudaBVH_scenePathTrace": (_I, [_H, C.POINTER(CudaCamera), C.POINTER(CudaPathParams), C.POINTER(CudaPathImage),
                                     _P, C.c_uint]),
+    "cudaBVH_denoiserCreate": (_H, [_I, C.c_uint64]),
+    "cudaBVH_denoiserDestroy": (None, [_H]),
+    "cudaBVH_denoiserSetStream": (_I, [_H, _P]),
+    "cudaBVH_denoiserNumParams": (C.c_longlong, [_H]),
+    "cudaBVH_denoiserGetParams": (_I, [_H, _P, _I]),
+    "cudaBVH_denoiserSetParams": (_I, [_H, _P]),
+    "cudaBVH_denoiserPack": (_I, [_H, _I, _P, _P, _P, _P, _P]),
+    "cudaBVH_denoiserForward": (_I, [_H, _I, _I, _I, _P, _P]),
+    "cudaBVH_denoiserTrainStep": (_I, [_H, _I, _I, _I, _P, _P, _F, _P]),
+    "cudaBVH_denoise": (_I, [_H, _I, _I, _P, _P, _P, _P, _P]),
     "cudaBVH_lastError": (C.c_char_p, []),
 }
 
@@ -507,6 +517,62 @@ def _scene_path_trace(self, cam: CudaCamera, params: CudaPathParams, rgb, albedo
 
 
 Scene.path_trace = _scene_path_trace
+
+
+class Denoiser:
+    """Owner of a CudaDenoiser_t (the BF16 MFMA U-Net and its Adam state)."""
+
+    def __init__(self, max_pixels: int, seed: int = 0):
+        self.lib = load_library()
+        h = self.lib.cudaBVH_denoiserCreate(max_pixels, seed)
+        if not h:
+            raise CudaBVHError(f"denoiserCreate: {last_error()}")
+        self.handle = C.c_void_p(h)
+        self.num_params = int(self.lib.cudaBVH_denoiserNumParams(self.handle))
+
+    def destroy(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.cudaBVH_denoiserDestroy(self.handle)
+            self.handle = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def set_stream(self, stream: int | None) -> None:
+        _check(self.lib.cudaBVH_denoiserSetStream(self.handle, stream), "denoiserSetStream")
+
+    def params(self, which: int = 0) -> np.ndarray:
+        """0 weights, 1 gradients of the last step, 2/3 Adam moments (FP32, host)."""
+        out = np.zeros(self.num_params, np.float32)
+        _check(self.lib.cudaBVH_denoiserGetParams(self.handle, out.ctypes.data, which), "denoiserGetParams")
+        return out
+
+    def set_params(self, w: np.ndarray) -> None:
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.num_params
+        _check(self.lib.cudaBVH_denoiserSetParams(self.handle, w.ctypes.data), "denoiserSetParams")
+
+    def pack(self, n: int, rgb, albedo, normal, depth, out) -> None:
+        _check(self.lib.cudaBVH_denoiserPack(self.handle, n, _ptr(rgb), _ptr(albedo), _ptr(normal), _ptr(depth),
+                                             _ptr(out)), "denoiserPack")
+
+    def forward(self, batch: int, height: int, width: int, x, head_out) -> None:
+        _check(self.lib.cudaBVH_denoiserForward(self.handle, batch, height, width, _ptr(x), _ptr(head_out)),
+               "denoiserForward")
+
+    def train_step(self, batch: int, height: int, width: int, x, target, lr: float, want_loss: bool = True):
+        loss = C.c_float(0.0)
+        _check(self.lib.cudaBVH_denoiserTrainStep(self.handle, batch, height, width, _ptr(x), _ptr(target),
+                                                  float(lr), C.byref(loss) if want_loss else None),
+               "denoiserTrainStep")
+        return loss.value if want_loss else None
+
+    def denoise(self, width: int, height: int, rgb, albedo, normal, depth, out) -> None:
+        _check(self.lib.cudaBVH_denoise(self.handle, width, height, _ptr(rgb), _ptr(albedo), _ptr(normal),
+                                        _ptr(depth), _ptr(out)), "denoise")
 
 
 def gen_pinhole_rays(width: int, height: int, cam: CudaCamera, rays, stream: int | None = None) -> None:

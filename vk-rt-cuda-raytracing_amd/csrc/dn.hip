@@ -1,0 +1,885 @@
+// dn.hip — learned denoiser: a 4-level convolutional encoder-decoder (U-Net)
+// on NHWC BF16 activations, trained on the GPU by its own kernels.
+//
+// Network (the reference's U-Net shape, train_denoiser.py:13-54 and
+// train_denoiser_v2.py:60-110, at 32/64/128/256 channels): per level two 3x3
+// convolutions with ReLU, 2x2 max pooling down, parameter-free 2x nearest
+// upsampling up, skip concatenation, and a 1x1 head whose 3 outputs are added
+// to the log-radiance input channels (a residual in log space, as the
+// reference's residual head). Input: 16 channels per pixel — log(1+rgb),
+// albedo, normal, log(1+depth)/4, zero padding.
+//
+// Kernels:
+//   * k_conv: implicit GEMM (M = pixels, N = output channels, K = taps x
+//     input channels) on MFMA 16x16x32 (or 16x16x16 for 16-channel inputs)
+//     BF16 with FP32 accumulation; bias + ReLU fused; writes into a channel
+//     slice of a wider buffer, so skip concatenation costs nothing. The same
+//     kernel computes input gradients with the flipped, transposed weights.
+//   * k_wgrad: weight gradients, M = output channels, N = input channels,
+//     K = pixels; 32-pixel tiles staged through LDS transposed, split over
+//     pixel ranges into partials reduced in a fixed order (deterministic).
+//   * pooling / upsampling / ReLU-mask / loss / Adam / layout kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/cudabvh.h"
+#include "rt_internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace dn {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);   // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                                              // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ---------------------------------------------------------------- conv ----
+struct ConvArgs {
+    const uint16_t* x; int ldx; int cin;     // input pixels (channel stride ldx), channels per tap
+    const uint16_t* w;                       // [rows >= gridDim.y*16*WN][taps*cin]
+    const float* bias;                       // nullable
+    uint16_t* y; int ldy;                    // BF16 output slice (nullable)
+    float* yf; int ldyf;                     // FP32 output (nullable)
+    int n;                                   // output channels written
+    int B, H, W, relu;
+};
+
+// Block: 4 waves x 32 pixels = 128 pixels; each wave 32 pixels x (16*WN)
+// channels (2 x WN MFMA tiles). A operand: the 8 (or 4) consecutive channels
+// of one neighbour pixel per lane; B operand: 8 (or 4) consecutive K of one
+// weight row per lane.
+template <int TAPS, int KS, int WN>
+__global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int P = a.B * a.H * a.W;
+    const int m0 = blockIdx.x * 128 + wave * 32;
+    const int n0 = blockIdx.y * (16 * WN);
+    const int row = lane & 15, kg = lane >> 4;
+    int pb[2], py[2], px[2];
+    bool pv[2];
+    for (int r = 0; r < 2; r++) {
+        const int p = m0 + r * 16 + row;
+        pv[r] = p < P;
+        const int pp = pv[r] ? p : 0;
+        px[r] = pp % a.W;
+        const int q = pp / a.W;
+        py[r] = q % a.H;
+        pb[r] = q / a.H;
+    }
+    f32x4 acc[2][WN];
+    for (int r = 0; r < 2; r++)
+        for (int j = 0; j < WN; j++) acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int K = TAPS * a.cin;
+    for (int t = 0; t < TAPS; t++) {
+        const int dy = TAPS == 9 ? t / 3 - 1 : 0, dx = TAPS == 9 ? t % 3 - 1 : 0;
+        const uint16_t* src[2];
+        for (int r = 0; r < 2; r++) {
+            const int yy = py[r] + dy, xx = px[r] + dx;
+            const bool ok = pv[r] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+            src[r] = ok ? a.x + ((size_t)(pb[r] * a.H + yy) * a.W + xx) * a.ldx : nullptr;
+        }
+        for (int c0 = 0; c0 < a.cin; c0 += KS) {
+            if constexpr (KS == 32) {
+                bf16x8 av[2], bv[WN];
+                for (int r = 0; r < 2; r++) {
+                    uint4 v = make_uint4(0, 0, 0, 0);
+                    if (src[r]) v = *(const uint4*)(src[r] + c0 + 8 * kg);
+                    av[r] = __builtin_bit_cast(bf16x8, v);
+                }
+                for (int j = 0; j < WN; j++) {
+                    const uint4 v = *(const uint4*)(a.w + (size_t)(n0 + j * 16 + row) * K + t * a.cin + c0 + 8 * kg);
+                    bv[j] = __builtin_bit_cast(bf16x8, v);
+                }
+                for (int r = 0; r < 2; r++)
+                    for (int j = 0; j < WN; j++)
+                        acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], bv[j], acc[r][j], 0, 0, 0);
+            } else {
+                s16x4 av[2], bv[WN];
+                for (int r = 0; r < 2; r++) {
+                    uint2 v = make_uint2(0, 0);
+                    if (src[r]) v = *(const uint2*)(src[r] + c0 + 4 * kg);
+                    av[r] = __builtin_bit_cast(s16x4, v);
+                }
+                for (int j = 0; j < WN; j++) {
+                    const uint2 v = *(const uint2*)(a.w + (size_t)(n0 + j * 16 + row) * K + t * a.cin + c0 + 4 * kg);
+                    bv[j] = __builtin_bit_cast(s16x4, v);
+                }
+                for (int r = 0; r < 2; r++)
+                    for (int j = 0; j < WN; j++)
+                        acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[r], bv[j], acc[r][j], 0, 0, 0);
+            }
+        }
+    }
+    // D[i][j]: lane j + 16*(i/4), item i%4
+    for (int r = 0; r < 2; r++)
+        for (int j = 0; j < WN; j++) {
+            const int col = n0 + j * 16 + row;
+            if (col >= a.n) continue;
+            const float b = a.bias ? a.bias[col] : 0.0f;
+            for (int i = 0; i < 4; i++) {
+                const int p = m0 + r * 16 + 4 * kg + i;
+                if (p >= P) continue;
+                float v = acc[r][j][i] + b;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                if (a.y) a.y[(size_t)p * a.ldy + col] = f2bf(v);
+                if (a.yf) a.yf[(size_t)p * a.ldyf + col] = v;
+            }
+        }
+}
+
+// --------------------------------------------------------------- wgrad ----
+struct WgradArgs {
+    const uint16_t* g; int ldg; int cout;    // output gradient (ReLU mask applied)
+    const uint16_t* x; int ldx; int cin;     // layer input
+    float* part;                             // [splits][cout][taps*cin]
+    int B, H, W, splits, chunk;              // pixels per split (multiple of 32)
+};
+
+constexpr int kLdsRow = 40;   // 32 pixels + 8 padding (bf16 elements)
+
+// grid: (ceil(cout/64), ceil(cin/64), taps*splits). Tile 64 (co) x 64 (ci);
+// wave w: co [32*(w>>1), +32) x ci [32*(w&1), +32).
+template <int TAPS>
+__global__ __launch_bounds__(kBlock) void k_wgrad(WgradArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t Gt[64 * kLdsRow];
+    __shared__ __attribute__((aligned(16))) uint16_t Xt[64 * kLdsRow];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int co0 = blockIdx.x * 64, ci0 = blockIdx.y * 64;
+    const int t = blockIdx.z % TAPS, split = blockIdx.z / TAPS;
+    const int dy = TAPS == 9 ? t / 3 - 1 : 0, dx = TAPS == 9 ? t % 3 - 1 : 0;
+    const int P = a.B * a.H * a.W;
+    const int pBeg = split * a.chunk, pEnd = min(P, pBeg + a.chunk);
+    const int row = lane & 15, kg = lane >> 4;
+    const int wco = 32 * (wave >> 1), wci = 32 * (wave & 1);
+    // loader: thread -> (pixel lp = tid/8, channel chunk 8*(tid%8))
+    const int lp = threadIdx.x >> 3, lc = 8 * (threadIdx.x & 7);
+    f32x4 acc[2][2];
+    for (int r = 0; r < 2; r++)
+        for (int j = 0; j < 2; j++) acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int p0 = pBeg; p0 < pEnd; p0 += 32) {
+        const int p = p0 + lp;
+        uint4 gv = make_uint4(0, 0, 0, 0), xv = make_uint4(0, 0, 0, 0);
+        if (p < pEnd) {
+            if (co0 + lc < a.cout) gv = *(const uint4*)(a.g + (size_t)p * a.ldg + co0 + lc);
+            const int xx = p % a.W, q = p / a.W, yy = q % a.H, bb = q / a.H;
+            const int sy = yy + dy, sx = xx + dx;
+            if (sy >= 0 && sy < a.H && sx >= 0 && sx < a.W && ci0 + lc < a.cin)
+                xv = *(const uint4*)(a.x + ((size_t)(bb * a.H + sy) * a.W + sx) * a.ldx + ci0 + lc);
+        }
+        const uint16_t* ge = (const uint16_t*)&gv;
+        const uint16_t* xe = (const uint16_t*)&xv;
+        for (int e = 0; e < 8; e++) {
+            Gt[(lc + e) * kLdsRow + lp] = ge[e];
+            Xt[(lc + e) * kLdsRow + lp] = xe[e];
+        }
+        __syncthreads();
+        bf16x8 av[2], bv[2];
+        for (int r = 0; r < 2; r++)
+            av[r] = __builtin_bit_cast(bf16x8, *(const uint4*)&Gt[(wco + r * 16 + row) * kLdsRow + 8 * kg]);
+        for (int j = 0; j < 2; j++)
+            bv[j] = __builtin_bit_cast(bf16x8, *(const uint4*)&Xt[(wci + j * 16 + row) * kLdsRow + 8 * kg]);
+        for (int r = 0; r < 2; r++)
+            for (int j = 0; j < 2; j++)
+                acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], bv[j], acc[r][j], 0, 0, 0);
+        __syncthreads();
+    }
+    const int K = TAPS * a.cin;
+    float* out = a.part + (size_t)split * a.cout * K;
+    for (int r = 0; r < 2; r++)
+        for (int j = 0; j < 2; j++) {
+            const int ci = ci0 + wci + j * 16 + row;
+            if (ci >= a.cin) continue;
+            for (int i = 0; i < 4; i++) {
+                const int co = co0 + wco + r * 16 + 4 * kg + i;
+                if (co < a.cout) out[(size_t)co * K + t * a.cin + ci] = acc[r][j][i];
+            }
+        }
+}
+
+// grad[k] = sum over splits of part[s][k], in split order
+__global__ __launch_bounds__(kBlock) void k_reduce_splits(const float* part, int splits, size_t n, float* grad) {
+    for (size_t k = blockIdx.x * (size_t)kBlock + threadIdx.x; k < n; k += (size_t)gridDim.x * kBlock) {
+        float s = 0.0f;
+        for (int i = 0; i < splits; i++) s += part[(size_t)i * n + k];
+        grad[k] = s;
+    }
+}
+
+// bias gradient partials: block b sums pixels [b*chunk, ...) of 64 channels
+__global__ __launch_bounds__(kBlock) void k_bias_part(const uint16_t* g, int ldg, int cout, int P, int chunk,
+                                                      float* part) {
+    __shared__ float red[kBlock];
+    const int c = blockIdx.y * 64 + (threadIdx.x & 63), sub = threadIdx.x >> 6;
+    const int pBeg = blockIdx.x * chunk, pEnd = min(P, pBeg + chunk);
+    float s = 0.0f;
+    if (c < cout)
+        for (int p = pBeg + sub; p < pEnd; p += 4) s += bf2f(g[(size_t)p * ldg + c]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (sub == 0 && c < cout)
+        part[(size_t)blockIdx.x * cout + c] = ((red[threadIdx.x] + red[threadIdx.x + 64]) + red[threadIdx.x + 128]) +
+                                              red[threadIdx.x + 192];
+}
+
+// ---------------------------------------------------------- elementwise ----
+// g *= (y > 0), 8 channels per thread
+__global__ __launch_bounds__(kBlock) void k_relu_bwd(uint16_t* g, int ldg, const uint16_t* y, int ldy, int c, int P) {
+    const int cg = c / 8;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < (size_t)P * cg; i += (size_t)gridDim.x * kBlock) {
+        const size_t p = i / cg;
+        const int k = 8 * (int)(i % cg);
+        uint4 gv = *(uint4*)(g + p * ldg + k);
+        const uint4 yv = *(const uint4*)(y + p * ldy + k);
+        uint16_t* ge = (uint16_t*)&gv;
+        const uint16_t* ye = (const uint16_t*)&yv;
+        for (int e = 0; e < 8; e++)
+            if (!(bf2f(ye[e]) > 0.0f)) ge[e] = 0;
+        *(uint4*)(g + p * ldg + k) = gv;
+    }
+}
+
+// 2x2 max pool: out [B][H/2][W/2][c] (stride ldo) from in (stride ldi)
+__global__ __launch_bounds__(kBlock) void k_pool(const uint16_t* in, int ldi, uint16_t* out, int ldo, int c, int B,
+                                                 int H, int W) {
+    const int Ho = H / 2, Wo = W / 2, cg = c / 8;
+    const size_t n = (size_t)B * Ho * Wo * cg;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const int k = 8 * (int)(i % cg);
+        const size_t po = i / cg;
+        const int xo = (int)(po % Wo), q = (int)(po / Wo), yo = q % Ho, b = q / Ho;
+        float m[8];
+        for (int e = 0; e < 8; e++) m[e] = -INFINITY;
+        for (int s = 0; s < 4; s++) {
+            const size_t pi = ((size_t)(b * H + 2 * yo + (s >> 1)) * W + 2 * xo + (s & 1));
+            const uint4 v = *(const uint4*)(in + pi * ldi + k);
+            const uint16_t* ve = (const uint16_t*)&v;
+            for (int e = 0; e < 8; e++) m[e] = fmaxf(m[e], bf2f(ve[e]));
+        }
+        uint4 o;
+        uint16_t* oe = (uint16_t*)&o;
+        for (int e = 0; e < 8; e++) oe[e] = (uint16_t)(__float_as_uint(m[e]) >> 16);   // exact: inputs are bf16
+        *(uint4*)(out + po * ldo + k) = o;
+    }
+}
+
+// pool backward: gin[argmax] += gout (first maximum in window order)
+__global__ __launch_bounds__(kBlock) void k_pool_bwd(const uint16_t* in, int ldi, const uint16_t* gout, int ldgo,
+                                                     uint16_t* gin, int ldgi, int c, int B, int H, int W) {
+    const int Ho = H / 2, Wo = W / 2, cg = c / 8;
+    const size_t n = (size_t)B * Ho * Wo * cg;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const int k = 8 * (int)(i % cg);
+        const size_t po = i / cg;
+        const int xo = (int)(po % Wo), q = (int)(po / Wo), yo = q % Ho, b = q / Ho;
+        float m[8];
+        int arg[8];
+        size_t pis[4];
+        for (int e = 0; e < 8; e++) { m[e] = -INFINITY; arg[e] = 0; }
+        for (int s = 0; s < 4; s++) {
+            pis[s] = ((size_t)(b * H + 2 * yo + (s >> 1)) * W + 2 * xo + (s & 1));
+            const uint4 v = *(const uint4*)(in + pis[s] * ldi + k);
+            const uint16_t* ve = (const uint16_t*)&v;
+            for (int e = 0; e < 8; e++) {
+                const float f = bf2f(ve[e]);
+                if (f > m[e]) { m[e] = f; arg[e] = s; }
+            }
+        }
+        const uint4 gv = *(const uint4*)(gout + po * ldgo + k);
+        const uint16_t* gve = (const uint16_t*)&gv;
+        for (int s = 0; s < 4; s++) {
+            uint4 d = *(uint4*)(gin + pis[s] * ldgi + k);
+            uint16_t* de = (uint16_t*)&d;
+            for (int e = 0; e < 8; e++)
+                if (arg[e] == s) de[e] = f2bf(bf2f(de[e]) + bf2f(gve[e]));
+            *(uint4*)(gin + pis[s] * ldgi + k) = d;
+        }
+    }
+}
+
+// 2x nearest upsampling: out [B][2H][2W] (slice, stride ldo) from in [B][H][W]
+__global__ __launch_bounds__(kBlock) void k_up(const uint16_t* in, int ldi, uint16_t* out, int ldo, int c, int B, int H,
+                                               int W) {
+    const int Ho = 2 * H, Wo = 2 * W, cg = c / 8;
+    const size_t n = (size_t)B * Ho * Wo * cg;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const int k = 8 * (int)(i % cg);
+        const size_t po = i / cg;
+        const int xo = (int)(po % Wo), q = (int)(po / Wo), yo = q % Ho, b = q / Ho;
+        const size_t pi = (size_t)(b * H + yo / 2) * W + xo / 2;
+        *(uint4*)(out + po * ldo + k) = *(const uint4*)(in + pi * ldi + k);
+    }
+}
+
+// upsampling backward: gin[p] = sum of the 4 gout children (window order)
+__global__ __launch_bounds__(kBlock) void k_up_bwd(const uint16_t* gout, int ldgo, uint16_t* gin, int ldgi, int c,
+                                                   int B, int H, int W) {
+    const int cg = c / 8;
+    const size_t n = (size_t)B * H * W * cg;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const int k = 8 * (int)(i % cg);
+        const size_t pi = i / cg;
+        const int x = (int)(pi % W), q = (int)(pi / W), y = q % H, b = q / H;
+        float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int t = 0; t < 4; t++) {
+            const size_t po = (size_t)(b * 2 * H + 2 * y + (t >> 1)) * (2 * W) + 2 * x + (t & 1);
+            const uint4 v = *(const uint4*)(gout + po * ldgo + k);
+            const uint16_t* ve = (const uint16_t*)&v;
+            for (int e = 0; e < 8; e++) s[e] += bf2f(ve[e]);
+        }
+        uint4 o;
+        uint16_t* oe = (uint16_t*)&o;
+        for (int e = 0; e < 8; e++) oe[e] = f2bf(s[e]);
+        *(uint4*)(gin + pi * ldgi + k) = o;
+    }
+}
+
+// L1 loss on log radiance: pred = head + input log-rgb; g (16 channels,
+// zero beyond 3) = sign(pred - target) / (3P); per-block |d| partial sums.
+__global__ __launch_bounds__(kBlock) void k_loss(const float* head, const uint16_t* xin, const float* target, int P,
+                                                 uint16_t* g, float* lossPart) {
+    __shared__ float red[kBlock];
+    const float sc = 1.0f / (3.0f * (float)P);
+    float s = 0.0f;
+    for (int p = blockIdx.x * kBlock + threadIdx.x; p < P; p += gridDim.x * kBlock) {
+        uint4 o0 = make_uint4(0, 0, 0, 0), o1 = make_uint4(0, 0, 0, 0);
+        uint16_t* oe = (uint16_t*)&o0;
+        for (int c = 0; c < 3; c++) {
+            const float pred = head[(size_t)p * 3 + c] + bf2f(xin[(size_t)p * 16 + c]);
+            const float d = pred - target[(size_t)p * 3 + c];
+            s += fabsf(d);
+            oe[c] = f2bf(d > 0.0f ? sc : (d < 0.0f ? -sc : 0.0f));
+        }
+        *(uint4*)(g + (size_t)p * 16) = o0;
+        *(uint4*)(g + (size_t)p * 16 + 8) = o1;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lossPart[blockIdx.x] = red[0];
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters.
+__global__ __launch_bounds__(kBlock) void k_adam(float* w, float* m, float* v, const float* g, size_t n, float lr,
+                                                 float b1, float b2, float eps, float c1, float c2) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// master [cout][taps][cin] -> forward BF16 [rowsF][taps*cin] and the flipped
+// transpose for input gradients [rowsB][taps*coutK]: wb[ci][t][co] = w[co][taps-1-t][ci]
+__global__ __launch_bounds__(kBlock) void k_wlayout(const float* w, int cout, int taps, int cin, uint16_t* wf,
+                                                    uint16_t* wb, int coutK) {
+    const size_t n = (size_t)cout * taps * cin;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const int ci = (int)(i % cin);
+        const int t = (int)((i / cin) % taps);
+        const int co = (int)(i / ((size_t)cin * taps));
+        const uint16_t h = f2bf(w[i]);
+        wf[i] = h;
+        wb[((size_t)ci * taps + (taps - 1 - t)) * coutK + co] = h;
+    }
+}
+
+// planes (rgb, albedo, normal: 3 x P; depth: P) -> 16-channel BF16 pixels
+__global__ __launch_bounds__(kBlock) void k_pack(const float* rgb, const float* alb, const float* nrm, const float* dep,
+                                                 int P, uint16_t* x) {
+    for (int p = blockIdx.x * kBlock + threadIdx.x; p < P; p += gridDim.x * kBlock) {
+        uint4 o0, o1 = make_uint4(0, 0, 0, 0);
+        uint16_t* e = (uint16_t*)&o0;
+        for (int c = 0; c < 3; c++) {
+            e[c] = f2bf(log1pf(fmaxf(rgb[(size_t)c * P + p], 0.0f)));
+            e[3 + c] = f2bf(alb[(size_t)c * P + p]);
+        }
+        e[6] = f2bf(nrm[p]);
+        e[7] = f2bf(nrm[(size_t)P + p]);
+        uint16_t* f = (uint16_t*)&o1;
+        f[0] = f2bf(nrm[(size_t)2 * P + p]);
+        f[1] = f2bf(0.25f * log1pf(fmaxf(dep[p], 0.0f)));
+        *(uint4*)(x + (size_t)p * 16) = o0;
+        *(uint4*)(x + (size_t)p * 16 + 8) = o1;
+    }
+}
+
+// head + log input -> radiance planes: exp(pred) - 1, clamped at 0
+__global__ __launch_bounds__(kBlock) void k_unpack(const float* head, const uint16_t* xin, int P, float* rgb) {
+    for (int p = blockIdx.x * kBlock + threadIdx.x; p < P; p += gridDim.x * kBlock)
+        for (int c = 0; c < 3; c++) {
+            const float pred = head[(size_t)p * 3 + c] + bf2f(xin[(size_t)p * 16 + c]);
+            rgb[(size_t)c * P + p] = fmaxf(expm1f(pred), 0.0f);
+        }
+}
+
+__global__ void k_fill_f32(float* p, size_t n, float v) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) p[i] = v;
+}
+
+// ---------------------------------------------------------------- host ----
+struct LayerDef { int cin, cout, taps; };
+// c1a c1b | c2a c2b | c3a c3b | c4a c4b | c5a c5b | c6a c6b | c7a c7b | head
+static const LayerDef kLayers[15] = {
+    {16, 32, 9},  {32, 32, 9},   {32, 64, 9},   {64, 64, 9},   {64, 128, 9}, {128, 128, 9}, {128, 256, 9},
+    {256, 256, 9}, {384, 128, 9}, {128, 128, 9}, {192, 64, 9}, {64, 64, 9},  {96, 32, 9},   {32, 32, 9},
+    {32, 3, 1}};
+constexpr int kNumLayers = 15;
+
+struct Layer {
+    LayerDef d;
+    size_t wOff, bOff;       // into the master parameter array
+    int coutK, rowsF, rowsB;
+    uint16_t *wf, *wb;       // BF16 forward / input-gradient layouts
+};
+
+static inline int rup(int x, int m) { return (x + m - 1) / m * m; }
+
+struct Net {
+    int maxB, maxH, maxW;
+    Layer L[kNumLayers];
+    size_t nParams = 0;
+    float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;   // master params, Adam state, grads
+    uint16_t* wbuf = nullptr;                                        // all BF16 weight layouts
+    // activations (level l has (H>>l) x (W>>l) pixels)
+    uint16_t *X, *A0, *CAT1, *A12, *D1, *P1, *A2, *CAT2, *A10, *D2, *P2, *A4, *CAT3, *A8, *D3, *P3, *A6, *E4;
+    uint16_t *gHead, *gD1, *gA12, *gCAT1, *gD2, *gA10, *gCAT2, *gD3, *gA8, *gCAT3, *gE4, *gA6, *gP3, *gA4, *gP2, *gA2,
+        *gP1, *gA0;
+    float *head, *target, *part, *lossPart;
+    size_t partFloats = 0;
+    void* act = nullptr;
+    int step = 0;
+    hipStream_t stream = nullptr;
+};
+
+static int grid_for(size_t n) { return (int)std::min<size_t>((n + kBlock - 1) / kBlock, 8192); }
+
+static void conv_fwd(Net& N, int li, const uint16_t* x, int ldx, uint16_t* y, int ldy, int B, int H, int W, bool relu,
+                     float* yf = nullptr, int ldyf = 0) {
+    const Layer& L = N.L[li];
+    ConvArgs a{x, ldx, L.d.cin, L.wf, N.w + L.bOff, y, ldy, yf, ldyf, L.d.cout, B, H, W, relu ? 1 : 0};
+    const int P = B * H * W;
+    const int WN = L.d.cout >= 64 ? 4 : (L.d.cout >= 32 ? 2 : 1);
+    dim3 grid((P + 127) / 128, (L.d.cout + 16 * WN - 1) / (16 * WN));
+    hipStream_t s = N.stream;
+    if (L.d.taps == 1) {
+        hipLaunchKernelGGL((k_conv<1, 32, 1>), grid, dim3(kBlock), 0, s, a);
+    } else if (L.d.cin % 32) {
+        if (WN == 2) hipLaunchKernelGGL((k_conv<9, 16, 2>), grid, dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_conv<9, 16, 4>), grid, dim3(kBlock), 0, s, a);
+    } else if (WN == 4) {
+        hipLaunchKernelGGL((k_conv<9, 32, 4>), grid, dim3(kBlock), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_conv<9, 32, 2>), grid, dim3(kBlock), 0, s, a);
+    }
+}
+
+// input gradient: dx = conv(g, flipped transposed weights)
+static void conv_dx(Net& N, int li, const uint16_t* g, int ldg, uint16_t* dx, int lddx, int B, int H, int W) {
+    const Layer& L = N.L[li];
+    ConvArgs a{g, ldg, L.coutK, L.wb, nullptr, dx, lddx, nullptr, 0, L.d.cin, B, H, W, 0};
+    const int P = B * H * W;
+    const int WN = L.d.cin >= 64 ? 4 : 2;
+    dim3 grid((P + 127) / 128, (L.d.cin + 16 * WN - 1) / (16 * WN));
+    hipStream_t s = N.stream;
+    if (L.d.taps == 1) {
+        hipLaunchKernelGGL((k_conv<1, 16, 2>), grid, dim3(kBlock), 0, s, a);
+    } else if (WN == 4) {
+        hipLaunchKernelGGL((k_conv<9, 32, 4>), grid, dim3(kBlock), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k_conv<9, 32, 2>), grid, dim3(kBlock), 0, s, a);
+    }
+}
+
+static void conv_dw(Net& N, int li, const uint16_t* g, int ldg, const uint16_t* x, int ldx, int B, int H, int W) {
+    const Layer& L = N.L[li];
+    const int P = B * H * W;
+    const size_t K = (size_t)L.d.taps * L.d.cin, n = (size_t)L.d.cout * K;
+    int splits = std::max(1, std::min(64, P / 2048));
+    while (splits > 1 && (size_t)splits * n > N.partFloats) splits--;
+    const int chunk = rup((P + splits - 1) / splits, 32);
+    splits = (P + chunk - 1) / chunk;
+    WgradArgs a{g, ldg, L.d.cout, x, ldx, L.d.cin, N.part, B, H, W, splits, chunk};
+    dim3 grid((L.d.cout + 63) / 64, (L.d.cin + 63) / 64, L.d.taps * splits);
+    if (L.d.taps == 9) hipLaunchKernelGGL(k_wgrad<9>, grid, dim3(kBlock), 0, N.stream, a);
+    else hipLaunchKernelGGL(k_wgrad<1>, grid, dim3(kBlock), 0, N.stream, a);
+    hipLaunchKernelGGL(k_reduce_splits, dim3(grid_for(n)), dim3(kBlock), 0, N.stream, N.part, splits, n, N.g + L.wOff);
+    // bias: partials over pixel chunks, then the same fixed-order reduction
+    const int bchunk = 4096, nb = (P + bchunk - 1) / bchunk;
+    hipLaunchKernelGGL(k_bias_part, dim3(nb, (L.d.cout + 63) / 64), dim3(kBlock), 0, N.stream, g, ldg, L.d.cout, P,
+                       bchunk, N.part);
+    hipLaunchKernelGGL(k_reduce_splits, dim3(grid_for(L.d.cout)), dim3(kBlock), 0, N.stream, N.part, nb,
+                       (size_t)L.d.cout, N.g + L.bOff);
+}
+
+static void relu_bwd(Net& N, uint16_t* g, int ldg, const uint16_t* y, int ldy, int c, int P) {
+    hipLaunchKernelGGL(k_relu_bwd, dim3(grid_for((size_t)P * c / 8)), dim3(kBlock), 0, N.stream, g, ldg, y, ldy, c, P);
+}
+static void pool(Net& N, const uint16_t* in, int ldi, uint16_t* out, int ldo, int c, int B, int H, int W) {
+    hipLaunchKernelGGL(k_pool, dim3(grid_for((size_t)B * H * W / 4 * c / 8)), dim3(kBlock), 0, N.stream, in, ldi, out,
+                       ldo, c, B, H, W);
+}
+static void pool_bwd(Net& N, const uint16_t* in, int ldi, const uint16_t* go, int ldgo, uint16_t* gi, int ldgi, int c,
+                     int B, int H, int W) {
+    hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for((size_t)B * H * W / 4 * c / 8)), dim3(kBlock), 0, N.stream, in, ldi,
+                       go, ldgo, gi, ldgi, c, B, H, W);
+}
+static void up(Net& N, const uint16_t* in, int ldi, uint16_t* out, int ldo, int c, int B, int H, int W) {
+    hipLaunchKernelGGL(k_up, dim3(grid_for((size_t)B * H * W * 4 * c / 8)), dim3(kBlock), 0, N.stream, in, ldi, out,
+                       ldo, c, B, H, W);
+}
+static void up_bwd(Net& N, const uint16_t* go, int ldgo, uint16_t* gi, int ldgi, int c, int B, int H, int W) {
+    hipLaunchKernelGGL(k_up_bwd, dim3(grid_for((size_t)B * H * W * c / 8)), dim3(kBlock), 0, N.stream, go, ldgo, gi,
+                       ldgi, c, B, H, W);
+}
+
+static void update_layouts(Net& N) {
+    for (int i = 0; i < kNumLayers; i++) {
+        const Layer& L = N.L[i];
+        const size_t n = (size_t)L.d.cout * L.d.taps * L.d.cin;
+        hipLaunchKernelGGL(k_wlayout, dim3(grid_for(n)), dim3(kBlock), 0, N.stream, N.w + L.wOff, L.d.cout, L.d.taps,
+                           L.d.cin, L.wf, L.wb, L.coutK);
+    }
+}
+
+// forward from N.X (B x H x W x 16) to N.head (P x 3 FP32, before the residual)
+static void forward(Net& N, int B, int H, int W) {
+    const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
+    conv_fwd(N, 0, N.X, 16, N.A0, 32, B, H, W, true);
+    conv_fwd(N, 1, N.A0, 32, N.CAT1 + 64, 96, B, H, W, true);             // E1
+    pool(N, N.CAT1 + 64, 96, N.P1, 32, 32, B, H, W);
+    conv_fwd(N, 2, N.P1, 32, N.A2, 64, B, H1, W1, true);
+    conv_fwd(N, 3, N.A2, 64, N.CAT2 + 128, 192, B, H1, W1, true);         // E2
+    pool(N, N.CAT2 + 128, 192, N.P2, 64, 64, B, H1, W1);
+    conv_fwd(N, 4, N.P2, 64, N.A4, 128, B, H2, W2, true);
+    conv_fwd(N, 5, N.A4, 128, N.CAT3 + 256, 384, B, H2, W2, true);        // E3
+    pool(N, N.CAT3 + 256, 384, N.P3, 128, 128, B, H2, W2);
+    conv_fwd(N, 6, N.P3, 128, N.A6, 256, B, H3, W3, true);
+    conv_fwd(N, 7, N.A6, 256, N.E4, 256, B, H3, W3, true);
+    up(N, N.E4, 256, N.CAT3, 384, 256, B, H3, W3);
+    conv_fwd(N, 8, N.CAT3, 384, N.A8, 128, B, H2, W2, true);
+    conv_fwd(N, 9, N.A8, 128, N.D3, 128, B, H2, W2, true);
+    up(N, N.D3, 128, N.CAT2, 192, 128, B, H2, W2);
+    conv_fwd(N, 10, N.CAT2, 192, N.A10, 64, B, H1, W1, true);
+    conv_fwd(N, 11, N.A10, 64, N.D2, 64, B, H1, W1, true);
+    up(N, N.D2, 64, N.CAT1, 96, 64, B, H1, W1);
+    conv_fwd(N, 12, N.CAT1, 96, N.A12, 32, B, H, W, true);
+    conv_fwd(N, 13, N.A12, 32, N.D1, 32, B, H, W, true);
+    conv_fwd(N, 14, N.D1, 32, nullptr, 0, B, H, W, false, N.head, 3);
+}
+
+// backward from N.gHead (P x 16, loss gradient in channels 0-2) to N.g
+static void backward(Net& N, int B, int H, int W) {
+    const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
+    const int P0 = B * H * W, P1 = B * H1 * W1, P2 = B * H2 * W2, P3 = B * H3 * W3;
+    conv_dw(N, 14, N.gHead, 16, N.D1, 32, B, H, W);
+    conv_dx(N, 14, N.gHead, 16, N.gD1, 32, B, H, W);
+    relu_bwd(N, N.gD1, 32, N.D1, 32, 32, P0);
+    conv_dw(N, 13, N.gD1, 32, N.A12, 32, B, H, W);
+    conv_dx(N, 13, N.gD1, 32, N.gA12, 32, B, H, W);
+    relu_bwd(N, N.gA12, 32, N.A12, 32, 32, P0);
+    conv_dw(N, 12, N.gA12, 32, N.CAT1, 96, B, H, W);
+    conv_dx(N, 12, N.gA12, 32, N.gCAT1, 96, B, H, W);
+    up_bwd(N, N.gCAT1, 96, N.gD2, 64, 64, B, H1, W1);
+    relu_bwd(N, N.gD2, 64, N.D2, 64, 64, P1);
+    conv_dw(N, 11, N.gD2, 64, N.A10, 64, B, H1, W1);
+    conv_dx(N, 11, N.gD2, 64, N.gA10, 64, B, H1, W1);
+    relu_bwd(N, N.gA10, 64, N.A10, 64, 64, P1);
+    conv_dw(N, 10, N.gA10, 64, N.CAT2, 192, B, H1, W1);
+    conv_dx(N, 10, N.gA10, 64, N.gCAT2, 192, B, H1, W1);
+    up_bwd(N, N.gCAT2, 192, N.gD3, 128, 128, B, H2, W2);
+    relu_bwd(N, N.gD3, 128, N.D3, 128, 128, P2);
+    conv_dw(N, 9, N.gD3, 128, N.A8, 128, B, H2, W2);
+    conv_dx(N, 9, N.gD3, 128, N.gA8, 128, B, H2, W2);
+    relu_bwd(N, N.gA8, 128, N.A8, 128, 128, P2);
+    conv_dw(N, 8, N.gA8, 128, N.CAT3, 384, B, H2, W2);
+    conv_dx(N, 8, N.gA8, 128, N.gCAT3, 384, B, H2, W2);
+    up_bwd(N, N.gCAT3, 384, N.gE4, 256, 256, B, H3, W3);
+    relu_bwd(N, N.gE4, 256, N.E4, 256, 256, P3);
+    conv_dw(N, 7, N.gE4, 256, N.A6, 256, B, H3, W3);
+    conv_dx(N, 7, N.gE4, 256, N.gA6, 256, B, H3, W3);
+    relu_bwd(N, N.gA6, 256, N.A6, 256, 256, P3);
+    conv_dw(N, 6, N.gA6, 256, N.P3, 128, B, H3, W3);
+    conv_dx(N, 6, N.gA6, 256, N.gP3, 128, B, H3, W3);
+    pool_bwd(N, N.CAT3 + 256, 384, N.gP3, 128, N.gCAT3 + 256, 384, 128, B, H2, W2);
+    relu_bwd(N, N.gCAT3 + 256, 384, N.CAT3 + 256, 384, 128, P2);
+    conv_dw(N, 5, N.gCAT3 + 256, 384, N.A4, 128, B, H2, W2);
+    conv_dx(N, 5, N.gCAT3 + 256, 384, N.gA4, 128, B, H2, W2);
+    relu_bwd(N, N.gA4, 128, N.A4, 128, 128, P2);
+    conv_dw(N, 4, N.gA4, 128, N.P2, 64, B, H2, W2);
+    conv_dx(N, 4, N.gA4, 128, N.gP2, 64, B, H2, W2);
+    pool_bwd(N, N.CAT2 + 128, 192, N.gP2, 64, N.gCAT2 + 128, 192, 64, B, H1, W1);
+    relu_bwd(N, N.gCAT2 + 128, 192, N.CAT2 + 128, 192, 64, P1);
+    conv_dw(N, 3, N.gCAT2 + 128, 192, N.A2, 64, B, H1, W1);
+    conv_dx(N, 3, N.gCAT2 + 128, 192, N.gA2, 64, B, H1, W1);
+    relu_bwd(N, N.gA2, 64, N.A2, 64, 64, P1);
+    conv_dw(N, 2, N.gA2, 64, N.P1, 32, B, H1, W1);
+    conv_dx(N, 2, N.gA2, 64, N.gP1, 32, B, H1, W1);
+    pool_bwd(N, N.CAT1 + 64, 96, N.gP1, 32, N.gCAT1 + 64, 96, 32, B, H, W);
+    relu_bwd(N, N.gCAT1 + 64, 96, N.CAT1 + 64, 96, 32, P0);
+    conv_dw(N, 1, N.gCAT1 + 64, 96, N.A0, 32, B, H, W);
+    conv_dx(N, 1, N.gCAT1 + 64, 96, N.gA0, 32, B, H, W);
+    relu_bwd(N, N.gA0, 32, N.A0, 32, 32, P0);
+    conv_dw(N, 0, N.gA0, 32, N.X, 16, B, H, W);
+}
+
+// Seeded He-uniform initialisation (weights of the zero input-padding
+// channels of the first layer stay 0), biases 0.
+static void init_params(std::vector<float>& w, const Net& N, uint64_t seed) {
+    uint64_t s = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto next = [&]() {
+        s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+        return (double)(s >> 11) * 0x1p-53;
+    };
+    for (int i = 0; i < kNumLayers; i++) {
+        const Layer& L = N.L[i];
+        const int fan = L.d.taps * (i == 0 ? 10 : L.d.cin);
+        const double lim = std::sqrt(6.0 / fan) * (i == kNumLayers - 1 ? 0.1 : 1.0);
+        for (int co = 0; co < L.d.cout; co++)
+            for (int t = 0; t < L.d.taps; t++)
+                for (int ci = 0; ci < L.d.cin; ci++) {
+                    const double r = (2.0 * next() - 1.0) * lim;
+                    w[L.wOff + ((size_t)co * L.d.taps + t) * L.d.cin + ci] = (i == 0 && ci >= 10) ? 0.0f : (float)r;
+                }
+    }
+}
+
+}  // namespace dn
+
+struct CudaDenoiserHandle {
+    dn::Net net;
+};
+
+using dn::Net;
+
+static bool dn_ok(CudaDenoiser_t d, const char* fn) {
+    if (!d) { rt::set_error("%s: NULL denoiser", fn); return false; }
+    return true;
+}
+
+static bool dn_dims(const Net& N, int B, int H, int W, const char* fn) {
+    if (B <= 0 || H <= 0 || W <= 0 || H % 8 || W % 8 || B * H * W > N.maxB * N.maxH * N.maxW) {
+        rt::set_error("%s: batch %d x %dx%d: need H, W multiples of 8 and B*H*W <= %d", fn, B, H, W,
+                      N.maxB * N.maxH * N.maxW);
+        return false;
+    }
+    return true;
+}
+
+extern "C" {
+
+CudaDenoiser_t cudaBVH_denoiserCreate(int maxPixels, uint64_t seed) {
+    rt::clear_error();
+    if (maxPixels <= 0 || maxPixels % 64) {
+        rt::set_error("cudaBVH_denoiserCreate: maxPixels must be a positive multiple of 64 (got %d)", maxPixels);
+        return nullptr;
+    }
+    auto* h = new CudaDenoiserHandle();
+    Net& N = h->net;
+    N.maxB = 1; N.maxH = 1; N.maxW = maxPixels;
+    size_t off = 0, wbytes = 0;
+    for (int i = 0; i < dn::kNumLayers; i++) {
+        dn::Layer& L = N.L[i];
+        L.d = dn::kLayers[i];
+        L.wOff = off; off += (size_t)L.d.cout * L.d.taps * L.d.cin;
+        L.bOff = off; off += (size_t)L.d.cout;
+        L.coutK = dn::rup(L.d.cout, 16);
+        L.rowsF = dn::rup(L.d.cout, 64);
+        L.rowsB = dn::rup(L.d.cin, 64);
+        wbytes += ((size_t)L.rowsF * L.d.taps * L.d.cin + (size_t)L.rowsB * L.d.taps * L.coutK) * 2;
+    }
+    N.nParams = off;
+    const size_t P = (size_t)maxPixels;
+    // activation channel counts at each level (level l: P >> 2l pixels)
+    const size_t lv0 = P, lv1 = P / 4, lv2 = P / 16, lv3 = P / 64;
+    struct Buf { uint16_t** p; size_t elems; };
+    std::vector<Buf> bufs = {
+        {&N.X, lv0 * 16}, {&N.A0, lv0 * 32}, {&N.CAT1, lv0 * 96}, {&N.A12, lv0 * 32}, {&N.D1, lv0 * 32},
+        {&N.P1, lv1 * 32}, {&N.A2, lv1 * 64}, {&N.CAT2, lv1 * 192}, {&N.A10, lv1 * 64}, {&N.D2, lv1 * 64},
+        {&N.P2, lv2 * 64}, {&N.A4, lv2 * 128}, {&N.CAT3, lv2 * 384}, {&N.A8, lv2 * 128}, {&N.D3, lv2 * 128},
+        {&N.P3, lv3 * 128}, {&N.A6, lv3 * 256}, {&N.E4, lv3 * 256},
+        {&N.gHead, lv0 * 16}, {&N.gD1, lv0 * 32}, {&N.gA12, lv0 * 32}, {&N.gCAT1, lv0 * 96}, {&N.gD2, lv1 * 64},
+        {&N.gA10, lv1 * 64}, {&N.gCAT2, lv1 * 192}, {&N.gD3, lv2 * 128}, {&N.gA8, lv2 * 128}, {&N.gCAT3, lv2 * 384},
+        {&N.gE4, lv3 * 256}, {&N.gA6, lv3 * 256}, {&N.gP3, lv3 * 128}, {&N.gA4, lv2 * 128}, {&N.gP2, lv2 * 64},
+        {&N.gA2, lv1 * 64}, {&N.gP1, lv1 * 32}, {&N.gA0, lv0 * 32}};
+    size_t actBytes = 0;
+    for (auto& b : bufs) actBytes += (b.elems * 2 + 255) & ~(size_t)255;
+    N.partFloats = std::max<size_t>(16u << 20, (size_t)(P / 4096 + 1) * 256);
+    const size_t fBytes = (lv0 * 3 * 2 + N.partFloats + 8192) * 4;
+    if (hipMalloc(&N.act, actBytes + fBytes) != hipSuccess ||
+        hipMalloc((void**)&N.w, N.nParams * 4 * 4) != hipSuccess || hipMalloc((void**)&N.wbuf, wbytes) != hipSuccess) {
+        rt::set_error("cudaBVH_denoiserCreate: cannot allocate %zu bytes", actBytes + fBytes + N.nParams * 16 + wbytes);
+        hipFree(N.act); hipFree(N.w); hipFree(N.wbuf);
+        delete h;
+        return nullptr;
+    }
+    char* p = (char*)N.act;
+    for (auto& b : bufs) { *b.p = (uint16_t*)p; p += (b.elems * 2 + 255) & ~(size_t)255; }
+    N.head = (float*)p; p += lv0 * 3 * 4;
+    N.target = (float*)p; p += lv0 * 3 * 4;
+    N.lossPart = (float*)p; p += 8192 * 4;
+    N.part = (float*)p;
+    N.m = N.w + N.nParams; N.v = N.m + N.nParams; N.g = N.v + N.nParams;
+    uint16_t* wp = N.wbuf;
+    for (int i = 0; i < dn::kNumLayers; i++) {
+        dn::Layer& L = N.L[i];
+        L.wf = wp; wp += (size_t)L.rowsF * L.d.taps * L.d.cin;
+        L.wb = wp; wp += (size_t)L.rowsB * L.d.taps * L.coutK;
+    }
+    std::vector<float> w(N.nParams, 0.0f);
+    dn::init_params(w, N, seed);
+    if (hipMemset(N.wbuf, 0, wbytes) != hipSuccess || hipMemset(N.m, 0, N.nParams * 12) != hipSuccess ||
+        hipMemcpy(N.w, w.data(), N.nParams * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        rt::set_error("cudaBVH_denoiserCreate: initialisation failed");
+        hipFree(N.act); hipFree(N.w); hipFree(N.wbuf);
+        delete h;
+        return nullptr;
+    }
+    dn::update_layouts(N);
+    if (hipStreamSynchronize(N.stream) != hipSuccess) { rt::set_error("cudaBVH_denoiserCreate: layout failed"); }
+    return h;
+}
+
+void cudaBVH_denoiserDestroy(CudaDenoiser_t d) {
+    if (!d) return;
+    hipStreamSynchronize(d->net.stream);
+    hipFree(d->net.act); hipFree(d->net.w); hipFree(d->net.wbuf);
+    delete d;
+}
+
+int cudaBVH_denoiserSetStream(CudaDenoiser_t d, void* stream) {
+    if (!dn_ok(d, "cudaBVH_denoiserSetStream")) return -1;
+    d->net.stream = (hipStream_t)stream;
+    return 0;
+}
+
+long long cudaBVH_denoiserNumParams(CudaDenoiser_t d) { return d ? (long long)d->net.nParams : 0; }
+
+int cudaBVH_denoiserGetParams(CudaDenoiser_t d, float* out, int which) {
+    if (!dn_ok(d, "cudaBVH_denoiserGetParams") || !out || which < 0 || which > 3) {
+        rt::set_error("cudaBVH_denoiserGetParams: bad arguments");
+        return -1;
+    }
+    const float* src = which == 0 ? d->net.w : which == 1 ? d->net.g : which == 2 ? d->net.m : d->net.v;
+    if (hipStreamSynchronize(d->net.stream) != hipSuccess ||
+        hipMemcpy(out, src, d->net.nParams * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+        rt::set_error("cudaBVH_denoiserGetParams: copy failed");
+        return -1;
+    }
+    return 0;
+}
+
+int cudaBVH_denoiserSetParams(CudaDenoiser_t d, const float* in) {
+    if (!dn_ok(d, "cudaBVH_denoiserSetParams") || !in) return -1;
+    Net& N = d->net;
+    if (hipStreamSynchronize(N.stream) != hipSuccess ||
+        hipMemcpy(N.w, in, N.nParams * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(N.m, 0, N.nParams * 8) != hipSuccess) {
+        rt::set_error("cudaBVH_denoiserSetParams: copy failed");
+        return -1;
+    }
+    N.step = 0;
+    dn::update_layouts(N);
+    return 0;
+}
+
+int cudaBVH_denoiserPack(CudaDenoiser_t d, int numPixels, const float* rgb, const float* albedo, const float* normal,
+                         const float* depth, uint16_t* out) {
+    if (!dn_ok(d, "cudaBVH_denoiserPack")) return -1;
+    if (numPixels <= 0 || !rgb || !albedo || !normal || !depth || !out) {
+        rt::set_error("cudaBVH_denoiserPack: bad arguments");
+        return -1;
+    }
+    hipLaunchKernelGGL(dn::k_pack, dim3(dn::grid_for(numPixels)), dim3(dn::kBlock), 0, d->net.stream, rgb, albedo,
+                       normal, depth, numPixels, out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int cudaBVH_denoiserForward(CudaDenoiser_t d, int batch, int height, int width, const uint16_t* input,
+                            float* headOut) {
+    if (!dn_ok(d, "cudaBVH_denoiserForward") || !dn_dims(d->net, batch, height, width, "cudaBVH_denoiserForward"))
+        return -1;
+    if (!input) { rt::set_error("cudaBVH_denoiserForward: NULL input"); return -1; }
+    Net& N = d->net;
+    const size_t P = (size_t)batch * height * width;
+    if (hipMemcpyAsync(N.X, input, P * 32, hipMemcpyDeviceToDevice, N.stream) != hipSuccess) {
+        rt::set_error("cudaBVH_denoiserForward: input copy failed");
+        return -1;
+    }
+    dn::forward(N, batch, height, width);
+    if (headOut && hipMemcpyAsync(headOut, N.head, P * 12, hipMemcpyDeviceToDevice, N.stream) != hipSuccess) {
+        rt::set_error("cudaBVH_denoiserForward: output copy failed");
+        return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int cudaBVH_denoise(CudaDenoiser_t d, int width, int height, const float* rgb, const float* albedo,
+                    const float* normal, const float* depth, float* out) {
+    if (!dn_ok(d, "cudaBVH_denoise") || !dn_dims(d->net, 1, height, width, "cudaBVH_denoise")) return -1;
+    if (!rgb || !albedo || !normal || !depth || !out) { rt::set_error("cudaBVH_denoise: NULL buffer"); return -1; }
+    Net& N = d->net;
+    const int P = width * height;
+    hipLaunchKernelGGL(dn::k_pack, dim3(dn::grid_for(P)), dim3(dn::kBlock), 0, N.stream, rgb, albedo, normal, depth, P,
+                       N.X);
+    dn::forward(N, 1, height, width);
+    hipLaunchKernelGGL(dn::k_unpack, dim3(dn::grid_for(P)), dim3(dn::kBlock), 0, N.stream, N.head, N.X, P, out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int cudaBVH_denoiserTrainStep(CudaDenoiser_t d, int batch, int height, int width, const uint16_t* input,
+                              const float* target, float learningRate, float* lossOut) {
+    if (!dn_ok(d, "cudaBVH_denoiserTrainStep") || !dn_dims(d->net, batch, height, width, "cudaBVH_denoiserTrainStep"))
+        return -1;
+    if (!input || !target) { rt::set_error("cudaBVH_denoiserTrainStep: NULL buffer"); return -1; }
+    Net& N = d->net;
+    const int P = batch * height * width;
+    if (hipMemcpyAsync(N.X, input, (size_t)P * 32, hipMemcpyDeviceToDevice, N.stream) != hipSuccess ||
+        hipMemcpyAsync(N.target, target, (size_t)P * 12, hipMemcpyDeviceToDevice, N.stream) != hipSuccess) {
+        rt::set_error("cudaBVH_denoiserTrainStep: input copy failed");
+        return -1;
+    }
+    dn::forward(N, batch, height, width);
+    const int lb = std::min(dn::grid_for(P), 8192);
+    hipLaunchKernelGGL(dn::k_loss, dim3(lb), dim3(dn::kBlock), 0, N.stream, N.head, N.X, N.target, P, N.gHead,
+                       N.lossPart);
+    dn::backward(N, batch, height, width);
+    if (learningRate > 0.0f) {
+        N.step++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.0f - (float)std::pow(0.9, N.step), c2 = 1.0f - (float)std::pow(0.999, N.step);
+        hipLaunchKernelGGL(dn::k_adam, dim3(dn::grid_for(N.nParams)), dim3(dn::kBlock), 0, N.stream, N.w, N.m, N.v,
+                           N.g, N.nParams, learningRate, b1, b2, 1e-8f, c1, c2);
+        dn::update_layouts(N);
+    }
+    if (lossOut) {
+        std::vector<float> lp(lb);
+        if (hipMemcpyAsync(lp.data(), N.lossPart, lb * 4, hipMemcpyDeviceToHost, N.stream) != hipSuccess ||
+            hipStreamSynchronize(N.stream) != hipSuccess) {
+            rt::set_error("cudaBVH_denoiserTrainStep: loss copy failed");
+            return -1;
+        }
+        double s = 0;
+        for (float x : lp) s += x;
+        *lossOut = (float)(s / (3.0 * P));
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // extern "C"
