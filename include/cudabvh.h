@@ -333,6 +333,29 @@ typedef struct {
 int cudaBVH_scenePathTrace(CudaScene_t scene, const CudaCamera* camera, const CudaPathParams* params,
                            const CudaPathImage* image, long long* rayCounts, unsigned flags);
 
+/* ----- Many independent scenes in one build and one launch ----- */
+typedef struct CudaBVHBatchHandle* CudaBVHBatch_t;
+
+/* Builds the BVHs of numScenes scenes (triangles concatenated; scene k is
+ * [triOffsets[k], triOffsets[k+1]), each with more than 4 triangles) in one
+ * level-synchronous GPU pass: one forest, each tree identical to the scene's
+ * own cudaBVH_build tree, leaf triangles carrying their index within the
+ * scene. Host inputs. */
+CudaBVHBatch_t cudaBVH_buildBatch(const CudaTri* tris, const int* triOffsets, int numScenes);
+void cudaBVH_batchDestroy(CudaBVHBatch_t batch);
+int cudaBVH_batchSetStream(CudaBVHBatch_t batch, void* hipStream);
+CudaBVH_t cudaBVH_batchForest(CudaBVHBatch_t batch);         /* borrowed: the forest's nodes/triangles */
+int cudaBVH_batchRoots(CudaBVHBatch_t batch, int* roots4);   /* BVH4 root of each scene (host) */
+/* Rays of all scenes concatenated; ray i belongs to the last scene k with
+ * d_rayOffsets[k] <= i (device array of numScenes+1 ints). All rays go
+ * through one persistent launch fed by a device work counter; primId is the
+ * index within the ray's scene. CUDABVH_FLAG_GRAPH captures the launch in a
+ * hipGraph once (per buffers and size) and replays it. */
+int cudaBVH_batchTraceClosest(CudaBVHBatch_t batch, int totalRays, const int* d_rayOffsets,
+                              const CudaRayPacket* rays, const CudaHitPacket* hits, unsigned flags);
+int cudaBVH_batchTraceAny(CudaBVHBatch_t batch, int totalRays, const int* d_rayOffsets,
+                          const CudaRayPacket* rays, uint8_t* occluded, unsigned flags);
+
 /* ----- Learned denoiser ----- */
 /* A 4-level convolutional encoder-decoder (32/64/128/256 channels, two 3x3
  * convolutions + ReLU per level, 2x2 max pooling, 2x nearest upsampling, skip

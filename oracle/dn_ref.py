@@ -118,11 +118,16 @@ def up_bwd(g):
     return g[:, 0::2, 0::2] + g[:, 0::2, 1::2] + g[:, 1::2, 0::2] + g[:, 1::2, 1::2]
 
 
-def forward(params, x, round_bf16: bool = True):
-    """x: [B,H,W,16] float64 (the BF16 input values). Returns (head [B,H,W,3], cache)."""
+def forward(params, x, round_bf16: bool = True, emulate: bool = False):
+    """x: [B,H,W,16] float64 (the BF16 input values). Returns (head [B,H,W,3], cache).
+
+    emulate=True also rounds every stored activation to BF16 (through FP32),
+    where the HIP kernels store BF16, so that the remaining differences are
+    only the FP32 accumulation order."""
     W, Bs = unpack(params, round_bf16)
     c = {}
-    relu = lambda v: np.maximum(v, 0.0)
+    R = (lambda v: bf16(v.astype(np.float32))) if emulate else (lambda v: v)
+    relu = lambda v: R(np.maximum(v, 0.0))
     c["A0"] = relu(conv(x, W[0], Bs[0]))
     c["E1"] = relu(conv(c["A0"], W[1], Bs[1]))
     c["P1"], c["arg1"] = pool(c["E1"])
@@ -148,15 +153,21 @@ def forward(params, x, round_bf16: bool = True):
     return head, c
 
 
-def loss_and_grads(params, x, target, round_bf16: bool = True):
+def loss_and_grads(params, x, target, round_bf16: bool = True, emulate: bool = False):
     """L1 between head + x[..., :3] and target (mean over pixels and channels).
-    Returns (loss, grads flat in the parameter layout, head)."""
+    Returns (loss, grads flat in the parameter layout, head).
+
+    emulate=True rounds stored activations and stored gradients to BF16 where
+    the HIP kernels do (see forward)."""
     W, Bs = unpack(params, round_bf16)
-    head, c = forward(params, x, round_bf16)
+    head, c = forward(params, x, round_bf16, emulate)
+    R = (lambda v: bf16(v.astype(np.float32))) if emulate else (lambda v: v)
+    if emulate:
+        head = head.astype(np.float32).astype(np.float64)
     d = head + x[..., :3] - target
     n = d.size
     loss = np.abs(d).sum() / n
-    g = np.sign(d) / n
+    g = R(np.sign(d) / n)
     offs, total = offsets()
     grads = np.zeros(total)
 
@@ -170,30 +181,37 @@ def loss_and_grads(params, x, target, round_bf16: bool = True):
             gy = gy * (out > 0)
         dx, dw, db = conv_bwd(inp, W[i], gy)
         put(i, dw, db)
-        return dx
+        return R(dx)
+
+    def pool_add(gE, arg, gP, shape):
+        pb = pool_bwd(arg, gP, shape)
+        hit = np.zeros(shape, bool)
+        for s_, (oy, ox) in enumerate(((0, 0), (0, 1), (1, 0), (1, 1))):
+            hit[:, oy::2, ox::2] = arg == s_
+        return np.where(hit, R(gE + pb), gE)
 
     gD1 = back(14, c["D1"], None, g, relu=False)
     gA12 = back(13, c["A12"], c["D1"], gD1)
     gCAT1 = back(12, c["CAT1"], c["A12"], gA12)
-    gD2 = up_bwd(gCAT1[..., :64])
+    gD2 = R(up_bwd(gCAT1[..., :64]))
     gE1 = gCAT1[..., 64:]
     gA10 = back(11, c["A10"], c["D2"], gD2)
     gCAT2 = back(10, c["CAT2"], c["A10"], gA10)
-    gD3 = up_bwd(gCAT2[..., :128])
+    gD3 = R(up_bwd(gCAT2[..., :128]))
     gE2 = gCAT2[..., 128:]
     gA8 = back(9, c["A8"], c["D3"], gD3)
     gCAT3 = back(8, c["CAT3"], c["A8"], gA8)
-    gE4 = up_bwd(gCAT3[..., :256])
+    gE4 = R(up_bwd(gCAT3[..., :256]))
     gE3 = gCAT3[..., 256:]
     gA6 = back(7, c["A6"], c["E4"], gE4)
     gP3 = back(6, c["P3"], c["A6"], gA6)
-    gE3 = gE3 + pool_bwd(c["arg3"], gP3, c["E3"].shape)
+    gE3 = pool_add(gE3, c["arg3"], gP3, c["E3"].shape)
     gA4 = back(5, c["A4"], c["E3"], gE3)
     gP2 = back(4, c["P2"], c["A4"], gA4)
-    gE2 = gE2 + pool_bwd(c["arg2"], gP2, c["E2"].shape)
+    gE2 = pool_add(gE2, c["arg2"], gP2, c["E2"].shape)
     gA2 = back(3, c["A2"], c["E2"], gE2)
     gP1 = back(2, c["P1"], c["A2"], gA2)
-    gE1 = gE1 + pool_bwd(c["arg1"], gP1, c["E1"].shape)
+    gE1 = pool_add(gE1, c["arg1"], gP1, c["E1"].shape)
     gA0 = back(1, c["A0"], c["E1"], gE1)
     back(0, x, c["A0"], gA0)
     return loss, grads, head

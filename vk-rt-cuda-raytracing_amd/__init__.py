@@ -154,6 +154,13 @@ SIGNATURES = {
     "cudaBVH_sceneTraceAnyQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), _P, C.c_uint]),
     "cudaBVH_scenePathTrace": (_I, [_H, C.POINTER(CudaCamera), C.POINTER(CudaPathParams), C.POINTER(CudaPathImage),
                                     _P, C.c_uint]),
+    "cudaBVH_buildBatch": (_H, [_P, _P, _I]),
+    "cudaBVH_batchDestroy": (None, [_H]),
+    "cudaBVH_batchSetStream": (_I, [_H, _P]),
+    "cudaBVH_batchForest": (_H, [_H]),
+    "cudaBVH_batchRoots": (_I, [_H, _P]),
+    "cudaBVH_batchTraceClosest": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), C.POINTER(CudaHitPacket), C.c_uint]),
+    "cudaBVH_batchTraceAny": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), _P, C.c_uint]),
     "cudaBVH_denoiserCreate": (_H, [_I, C.c_uint64]),
     "cudaBVH_denoiserDestroy": (None, [_H]),
     "cudaBVH_denoiserSetStream": (_I, [_H, _P]),
@@ -517,6 +524,57 @@ def _scene_path_trace(self, cam: CudaCamera, params: CudaPathParams, rgb, albedo
 
 
 Scene.path_trace = _scene_path_trace
+
+
+class Batch:
+    """Owner of a CudaBVHBatch_t: the BVHs of many scenes built in one pass."""
+
+    def __init__(self, scenes_tris: list):
+        self.lib = load_library()
+        counts = [len(np.asarray(t).reshape(-1, 9)) for t in scenes_tris]
+        self.tri_offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+        allt = np.ascontiguousarray(np.concatenate([np.asarray(t, np.float32).reshape(-1, 9) for t in scenes_tris]))
+        h = self.lib.cudaBVH_buildBatch(allt.ctypes.data, self.tri_offsets.ctypes.data, len(counts))
+        if not h:
+            raise CudaBVHError(f"buildBatch: {last_error()}")
+        self.handle = C.c_void_p(h)
+        self.num_scenes = len(counts)
+
+    def destroy(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.cudaBVH_batchDestroy(self.handle)
+            self.handle = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def set_stream(self, stream: int | None) -> None:
+        _check(self.lib.cudaBVH_batchSetStream(self.handle, stream), "batchSetStream")
+
+    def forest(self) -> BVH:
+        v = BVH.__new__(BVH)
+        v.lib, v.handle = self.lib, C.c_void_p(self.lib.cudaBVH_batchForest(self.handle))
+        v.num_input = int(self.tri_offsets[-1])
+        v._borrowed = True
+        return v
+
+    def roots(self) -> np.ndarray:
+        out = np.zeros(self.num_scenes, np.int32)
+        _check(self.lib.cudaBVH_batchRoots(self.handle, out.ctypes.data), "batchRoots")
+        return out
+
+    def trace_closest(self, total: int, d_ray_offsets, rays, t, prim_id=None, u=None, v=None, flags: int = 0) -> None:
+        rp, hp = ray_packet(rays), hit_packet(t, prim_id, u, v)
+        _check(self.lib.cudaBVH_batchTraceClosest(self.handle, total, _ptr(d_ray_offsets), C.byref(rp), C.byref(hp),
+                                                  flags), "batchTraceClosest")
+
+    def trace_any(self, total: int, d_ray_offsets, rays, occluded, flags: int = 0) -> None:
+        rp = ray_packet(rays)
+        _check(self.lib.cudaBVH_batchTraceAny(self.handle, total, _ptr(d_ray_offsets), C.byref(rp), _ptr(occluded),
+                                              flags), "batchTraceAny")
 
 
 class Denoiser:

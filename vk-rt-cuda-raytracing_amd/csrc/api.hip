@@ -35,7 +35,8 @@ static void free_handle_device(CudaBVHHandle* h) {
 }
 
 // Builds a handle from n x 9 floats (original order).
-static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf) {
+static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf, const int* sceneOff = nullptr,
+                                 int numScenes = 1, std::vector<int>* roots4 = nullptr) {
     rt::clear_error();
     if (!tri9 || n <= 0) { set_error("cudaBVH_build: need tris != NULL and numTris > 0 (got %d)", n); return nullptr; }
     if (n > (1 << 27)) { set_error("cudaBVH_build: %d triangles exceed the leaf encoding (2^27)", n); return nullptr; }
@@ -60,7 +61,7 @@ static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf) {
         rt::GpuBVH g;
         HIP_TRY(hipMalloc((void**)&d_in, size_t(n) * 36), goto fail);
         HIP_TRY(hipMemcpyAsync(d_in, tri9, size_t(n) * 36, hipMemcpyHostToDevice, s), hipFree(d_in); goto fail);
-        if (rt::gpu_build(d_in, n, maxLeaf, s, g) != 0) {
+        if (rt::gpu_build(d_in, n, maxLeaf, s, g, sceneOff, numScenes, roots4) != 0) {
             hipFree(d_in); hipFree(g.nodes2); hipFree(g.nodes4); hipFree(g.tris); hipFree(g.leafOf); hipFree(g.ordered);
             goto fail;
         }
@@ -90,8 +91,10 @@ static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf) {
             for (int a = 0; a < 9; a++) { h->h_soa[a][i] = src[a]; pk[a] = src[a]; }
             pk[9] = (float)p;   // the value, not the bits (backend :730-733)
         }
-        rt::export_dfs(nodes2, h->h_dfs);
-        h->numDFS = int(nodes2.size());
+        if (numScenes == 1) {
+            rt::export_dfs(nodes2, h->h_dfs);
+            h->numDFS = int(nodes2.size());
+        }
     }
     if (rtk::alloc_scratch(h->scratch) != 0) goto fail;
     h->tune[rt::kClassIncoherent].schedule = rt::kScheduleWhileWhile;
@@ -975,6 +978,142 @@ int cudaBVH_sceneTraceAnyQueued(CudaScene_t sc, int capacity, const int* d_count
     if (!d_count) { set_error("cudaBVH_sceneTraceAnyQueued: NULL count"); return -1; }
     return scene_trace(sc, capacity, d_count, rays, nullptr, occluded, nullptr, nullptr, flags, true,
                        "cudaBVH_sceneTraceAnyQueued");
+}
+
+// ------------------------------- batched scenes ------------------------------
+
+struct CudaBVHBatchHandle {
+    CudaBVHHandle* forest = nullptr;     // every scene's nodes and triangles
+    int numScenes = 0;
+    std::vector<int> triOff, roots4;
+    int* d_roots = nullptr;
+    hipStream_t own = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    rtk::TraceArgs key{};
+    int keyMode = -1;
+};
+
+static bool batch_ok(CudaBVHBatch_t b, const char* fn) {
+    if (!b) { set_error("%s: NULL batch", fn); return false; }
+    return true;
+}
+
+CudaBVHBatch_t cudaBVH_buildBatch(const CudaTri* tris, const int* triOffsets, int numScenes) {
+    rt::clear_error();
+    if (!tris || !triOffsets || numScenes <= 0) { set_error("cudaBVH_buildBatch: bad arguments"); return nullptr; }
+    if (triOffsets[0] != 0) { set_error("cudaBVH_buildBatch: triOffsets[0] must be 0"); return nullptr; }
+    for (int k = 0; k < numScenes; k++)
+        if (triOffsets[k + 1] <= triOffsets[k]) { set_error("cudaBVH_buildBatch: scene %d is empty", k); return nullptr; }
+    auto* b = new CudaBVHBatchHandle();
+    b->numScenes = numScenes;
+    b->triOff.assign(triOffsets, triOffsets + numScenes + 1);
+    b->forest = build_impl(reinterpret_cast<const float*>(tris), triOffsets[numScenes], 4, triOffsets, numScenes,
+                           &b->roots4);
+    if (!b->forest) { delete b; return nullptr; }
+    if (hipMalloc((void**)&b->d_roots, sizeof(int) * numScenes) != hipSuccess ||
+        hipMemcpy(b->d_roots, b->roots4.data(), sizeof(int) * numScenes, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("cudaBVH_buildBatch: device allocation failed");
+        cudaBVH_destroy(b->forest);
+        hipFree(b->d_roots);
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+
+void cudaBVH_batchDestroy(CudaBVHBatch_t b) {
+    if (!b) return;
+    if (b->forest && b->forest->stream) hipStreamSynchronize((hipStream_t)b->forest->stream);
+    if (b->own) hipStreamSynchronize(b->own);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->own) hipStreamDestroy(b->own);
+    hipFree(b->d_roots);
+    cudaBVH_destroy(b->forest);
+    delete b;
+}
+
+int cudaBVH_batchSetStream(CudaBVHBatch_t b, void* stream) {
+    if (!batch_ok(b, "cudaBVH_batchSetStream")) return -1;
+    b->forest->stream = stream;
+    return 0;
+}
+
+CudaBVH_t cudaBVH_batchForest(CudaBVHBatch_t b) { return b ? b->forest : nullptr; }
+
+int cudaBVH_batchRoots(CudaBVHBatch_t b, int* roots4) {
+    if (!batch_ok(b, "cudaBVH_batchRoots") || !roots4) return -1;
+    memcpy(roots4, b->roots4.data(), sizeof(int) * b->numScenes);
+    return 0;
+}
+
+static int batch_trace(CudaBVHBatch_t b, int totalRays, const int* d_rayOffsets, const CudaRayPacket* rays,
+                       const CudaHitPacket* hits, uint8_t* occ, unsigned flags, bool any, const char* fn) {
+    rt::clear_error();
+    if (!batch_ok(b, fn)) return -1;
+    if (totalRays == 0) return 0;
+    if (totalRays < 0 || !d_rayOffsets || !packet_ok(rays) || (!any && (!hits || !hits->t)) || (any && !occ)) {
+        set_error("%s: bad arguments", fn);
+        return -1;
+    }
+    if (flags & ~(unsigned)(CUDABVH_FLAG_SYNC | CUDABVH_FLAG_GRAPH)) { set_error("%s: unsupported flags 0x%x", fn, flags); return -1; }
+    rtk::TraceArgs a;
+    a.n = totalRays;
+    a.ox = rays->ox; a.oy = rays->oy; a.oz = rays->oz; a.dx = rays->dx; a.dy = rays->dy; a.dz = rays->dz;
+    a.tmin = rays->tmin; a.tmax = rays->tmax;
+    if (hits) { a.t = hits->t; a.id = hits->primId; a.u = hits->u; a.v = hits->v; }
+    a.occ = occ;
+    a.rayOff = d_rayOffsets; a.roots = b->d_roots; a.numScenes = b->numScenes;
+    const int mode = any ? rtk::kModeAny : rtk::kModeClosest;
+    hipStream_t s = hstream(b->forest);
+    if (!(flags & CUDABVH_FLAG_GRAPH)) {
+        const int rc = rtk::launch_trace(b->forest, b->forest->scratch, a, mode, false, false, false, s);
+        if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(s), return -1);
+        return rc;
+    }
+    // graph mode: one captured launch (counter reset + persistent trace), replayed
+    if (!s) {
+        if (!b->own && hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking) != hipSuccess) {
+            set_error("%s: cannot create a stream", fn);
+            return -1;
+        }
+        HIP_TRY(hipStreamSynchronize(nullptr), return -1);
+        s = b->own;
+    }
+    if (!b->exec || b->keyMode != mode || memcmp(&b->key, &a, sizeof(a)) != 0) {
+        if (b->exec) { hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+        if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+        rtk::TraceArgs e = a;      // eager launch allocates the traversal scratch
+        if (rtk::launch_trace(b->forest, b->forest->scratch, e, mode, false, false, false, s) != 0) return -1;
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), return -1);
+        rtk::TraceArgs c = a;
+        const int rc = rtk::launch_trace(b->forest, b->forest->scratch, c, mode, false, false, false, s);
+        hipGraph_t gr = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s, &gr);
+        if (rc != 0 || ec != hipSuccess) {
+            if (gr) hipGraphDestroy(gr);
+            if (rc == 0) set_error("%s: capture failed: %s", fn, hipGetErrorString(ec));
+            return -1;
+        }
+        b->graph = gr;
+        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), return -1);
+        b->key = a;
+        b->keyMode = mode;
+    }
+    HIP_TRY(hipGraphLaunch(b->exec, s), return -1);
+    if ((flags & CUDABVH_FLAG_SYNC) || s == b->own) HIP_TRY(hipStreamSynchronize(s), return -1);
+    return 0;
+}
+
+int cudaBVH_batchTraceClosest(CudaBVHBatch_t b, int totalRays, const int* d_rayOffsets, const CudaRayPacket* rays,
+                              const CudaHitPacket* hits, unsigned flags) {
+    return batch_trace(b, totalRays, d_rayOffsets, rays, hits, nullptr, flags, false, "cudaBVH_batchTraceClosest");
+}
+
+int cudaBVH_batchTraceAny(CudaBVHBatch_t b, int totalRays, const int* d_rayOffsets, const CudaRayPacket* rays,
+                          uint8_t* occluded, unsigned flags) {
+    return batch_trace(b, totalRays, d_rayOffsets, rays, nullptr, occluded, flags, true, "cudaBVH_batchTraceAny");
 }
 
 int cudaBVH_scenePathTrace(CudaScene_t sc, const CudaCamera* cam, const CudaPathParams* params,

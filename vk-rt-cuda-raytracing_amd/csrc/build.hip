@@ -83,7 +83,18 @@ struct Build {
 };
 
 // ---------------------------------------------------------------------------
-__global__ void k_prims(const float* __restrict__ tris, int n, float* pbox, float4* cen, int* idx, int* pnode) {
+// Scene of primitive i: the last s with off[s] <= i (off: numScenes+1 ascending offsets).
+__device__ __forceinline__ int scene_of(const int* off, int numScenes, int i) {
+    int lo = 0, hi = numScenes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_prims(const float* __restrict__ tris, int n, float* pbox, float4* cen, int* idx, int* pnode,
+                        const int* off, int numScenes) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* t = tris + 9 * (size_t)i;
@@ -97,12 +108,17 @@ __global__ void k_prims(const float* __restrict__ tris, int n, float* pbox, floa
     }
     cen[i] = make_float4((mn[0] + mx[0]) * .5f, (mn[1] + mx[1]) * .5f, (mn[2] + mx[2]) * .5f, 0.f);
     idx[i] = i;
-    pnode[i] = 0;
+    pnode[i] = numScenes == 1 ? 0 : scene_of(off, numScenes, i);
 }
 
-__global__ void k_init_root(Build b) {
-    b.ns[0] = 0; b.ncnt[0] = b.n; b.nleft[0] = -1; b.nright[0] = -1; b.ndepth[0] = 0;
-    b.nodeSlot[0] = 0; b.act[0] = 0;
+// Roots: node s spans scene s's primitives; slots[s] is its active slot (or -1).
+__global__ void k_init_roots(Build b, const int* off, const int* slots, int numScenes) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= numScenes) return;
+    b.ns[s] = off[s]; b.ncnt[s] = off[s + 1] - off[s];
+    b.nleft[s] = -1; b.nright[s] = -1; b.ndepth[s] = 0;
+    b.nodeSlot[s] = slots[s];
+    if (slots[s] >= 0) b.act[slots[s]] = s;
 }
 
 __global__ void k_init_nodes(int* nbox, int* ncen, int first, int count) {
@@ -338,6 +354,18 @@ __global__ void k_sizes(const int* nleft, const int* nright, int* size, int firs
     size[v] = nleft[v] < 0 ? 1 : 1 + size[nleft[v]] + size[nright[v]];
 }
 
+// Pre-order positions of the roots: trees one after the other.
+__global__ void k_root_pre(const int* size, int* pre, int numScenes) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int acc = 0;
+    for (int s = 0; s < numScenes; s++) { pre[s] = acc; acc += size[s]; }
+}
+
+__global__ void k_root4(const int* pre, const int* b4id, int numScenes, int* root4) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < numScenes) root4[s] = b4id[pre[s]];
+}
+
 __global__ void k_preorder(const int* nleft, const int* nright, const int* size, int* pre, int first, int count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
@@ -452,12 +480,15 @@ __global__ void k_emit_leaf_root(const rt::Node2* n2, uint4* out, int* depth4) {
 
 // Leaf-order triangles {v0.xyz, bits(id)}, {v1.xyz, 0}, {v2.xyz, 0} and the
 // inverse permutation.
-__global__ void k_leaf_tris(const float* __restrict__ tris, const int* idx, int n, float4* out, int* leafOf) {
+// In a forest the id is the index within the triangle's scene.
+__global__ void k_leaf_tris(const float* __restrict__ tris, const int* idx, int n, float4* out, int* leafOf,
+                            const int* off, int numScenes) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int p = idx[i];
     const float* s = tris + 9 * (size_t)p;
-    out[3 * (size_t)i] = make_float4(s[0], s[1], s[2], __int_as_float(p));
+    const int id = numScenes == 1 ? p : p - off[scene_of(off, numScenes, p)];
+    out[3 * (size_t)i] = make_float4(s[0], s[1], s[2], __int_as_float(id));
     out[3 * (size_t)i + 1] = make_float4(s[3], s[4], s[5], 0.f);
     out[3 * (size_t)i + 2] = make_float4(s[6], s[7], s[8], 0.f);
     leafOf[p] = i;
@@ -539,9 +570,30 @@ struct DevBuf {
 // Outputs (device, allocated here): nodes4 (numNodes4 x 64 B), leaf-order
 // triangles (n x 48 B), leafOf (n ints), ordered (n ints, leaf order ->
 // original index), nodes2 (pre-order rt::Node2). Returns 0 on success.
-int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& out) {
+int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& out, const int* sceneOff,
+              int numScenes, std::vector<int>* roots4) {
     using namespace rtb;
     DevBuf tmp;
+    const int R = numScenes;
+    std::vector<int> hOff = sceneOff ? std::vector<int>(sceneOff, sceneOff + R + 1) : std::vector<int>{0, n};
+    std::vector<int> hSlots(R, -1);
+    int m = 0;
+    for (int k = 0; k < R; k++) {
+        const int c = hOff[k + 1] - hOff[k];
+        if (c <= 0 || (R > 1 && c <= maxLeaf)) {
+            set_error("gpu build: scene %d has %d primitives (a batch needs more than %d per scene)", k, c, maxLeaf);
+            return -1;
+        }
+        if (c > maxLeaf) hSlots[k] = m++;
+    }
+    int* dOff = tmp.get<int>(R + 1);
+    int* dSlots = tmp.get<int>(R);
+    BUILD_CHECK(dOff && dSlots, "gpu build: out of device memory");
+    if (hipMemcpyAsync(dOff, hOff.data(), (R + 1) * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dSlots, hSlots.data(), R * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess) {
+        set_error("gpu build: upload failed");
+        return -1;
+    }
     Build b{};
     b.n = n; b.maxLeaf = maxLeaf;
     const size_t cap = 2 * (size_t)n;
@@ -566,14 +618,13 @@ int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& ou
     const int T = kThreads;
     auto grid = [](long long items) { return dim3((unsigned)((items + kThreads - 1) / kThreads)); };
     const int chunks = (n + kChunk - 1) / kChunk;
-    hipLaunchKernelGGL(k_prims, grid(n), dim3(T), 0, s, d_tris, n, pbox, cen, b.idx, b.pnode);
-    hipLaunchKernelGGL(k_init_root, dim3(1), dim3(1), 0, s, b);
-    hipLaunchKernelGGL(k_init_nodes, dim3(1), dim3(T), 0, s, b.nbox, b.ncen, 0, 1);
+    hipLaunchKernelGGL(k_prims, grid(n), dim3(T), 0, s, d_tris, n, pbox, cen, b.idx, b.pnode, dOff, R);
+    hipLaunchKernelGGL(k_init_roots, grid(R), dim3(T), 0, s, b, dOff, dSlots, R);
+    hipLaunchKernelGGL(k_init_nodes, grid(R), dim3(T), 0, s, b.nbox, b.ncen, 0, R);
     hipLaunchKernelGGL(k_node_bounds, dim3(chunks), dim3(T), 0, s, b, 0);
 
-    std::vector<int> levelStart = {0, 1};     // node ids of level L: [levelStart[L], levelStart[L+1])
-    int nodeCount = 1;
-    int m = n > maxLeaf ? 1 : 0;
+    std::vector<int> levelStart = {0, R};     // node ids of level L: [levelStart[L], levelStart[L+1])
+    int nodeCount = R;
     int* bins = nullptr;
     size_t binsCap = 0;
     while (m > 0) {
@@ -627,7 +678,7 @@ int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& ou
         const int f = levelStart[L], c = levelStart[L + 1] - f;
         hipLaunchKernelGGL(k_sizes, grid(c), dim3(T), 0, s, b.nleft, b.nright, size, f, c);
     }
-    hipMemsetAsync(pre, 0, sizeof(int), s);
+    hipLaunchKernelGGL(k_root_pre, dim3(1), dim3(1), 0, s, size, pre, R);
     for (int L = 0; L < levels; L++) {
         const int f = levelStart[L], c = levelStart[L + 1] - f;
         hipLaunchKernelGGL(k_preorder, grid(c), dim3(T), 0, s, b.nleft, b.nright, size, pre, f, c);
@@ -658,7 +709,17 @@ int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& ou
                            (uint4*)out.nodes4, dDepth);
     else
         hipLaunchKernelGGL(k_emit_leaf_root, dim3(1), dim3(1), 0, s, out.nodes2, (uint4*)out.nodes4, dDepth);
-    hipLaunchKernelGGL(k_leaf_tris, grid(n), dim3(T), 0, s, d_tris, b.idx, n, (float4*)out.tris, out.leafOf);
+    hipLaunchKernelGGL(k_leaf_tris, grid(n), dim3(T), 0, s, d_tris, b.idx, n, (float4*)out.tris, out.leafOf, dOff, R);
+    if (roots4) {
+        int* dRoot4 = tmp.get<int>(R);
+        BUILD_CHECK(dRoot4, "gpu build: out of device memory");
+        hipLaunchKernelGGL(k_root4, grid(R), dim3(T), 0, s, pre, b4id, R, dRoot4);
+        roots4->assign(R, 0);
+        if (hipMemcpyAsync(roots4->data(), dRoot4, R * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) {
+            set_error("gpu build: root download failed");
+            return -1;
+        }
+    }
     hipMemcpyAsync(out.ordered, b.idx, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
     hipMemcpyAsync(&out.depth4, dDepth, 4, hipMemcpyDeviceToHost, s);
     hipError_t e = hipStreamSynchronize(s);

@@ -50,7 +50,12 @@ struct GpuBVH {
     int* ordered = nullptr;                        // leaf position -> original index
     int levels = 0;
 };
-int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& out);
+// Builds one BVH, or with sceneOff (numScenes+1 host offsets into d_tris) a
+// forest of numScenes independent BVHs in one level-synchronous pass: node
+// arrays shared, each tree contiguous in pre-order, leaf triangles carrying
+// their index within their scene; roots4 receives each tree's BVH4 root.
+int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& out, const int* sceneOff = nullptr,
+              int numScenes = 1, std::vector<int>* roots4 = nullptr);
 void scan_exclusive(const int* in, int* out, int n, int* total, int* tiles, hipStream_t s);
 int scan_tiles_needed(int n);
 

@@ -128,16 +128,27 @@ struct Result {
     int visits, tests;
 };
 
+// First BVH4 node of ray ri: 0, or in a forest its scene's root.
+__device__ __forceinline__ int ray_root(const TraceArgs& a, int ri) {
+    if (!a.roots) return 0;
+    int lo = 0, hi = a.numScenes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.rayOff[mid] <= ri) lo = mid; else hi = mid - 1;
+    }
+    return a.roots[lo];
+}
+
 // Batch schedule: traverses one ray in one loop (node and leaf steps share
 // iterations). Stack entries (node ref, entry distance) live in LDS
 // (slots [0, LDSD)) and spill to HBM beyond.
 template <int MODE, bool MT, bool FLAT, int LDSD, int BLOCK>
 __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int* s_ref, float* s_tn, int tid,
-                                          int gtid, Result& res) {
+                                          int gtid, Result& res, int root) {
     const uint4* __restrict__ nodes = a.nodes;
     const float4* __restrict__ tris = a.tris;
     int sp = 0, visits = 0, tests = 0;
-    int ref = 0;
+    int ref = root;
     float tn = -INFINITY;
     float best = r.tmax, bu = 0.f, bv = 0.f;
     int bid = -1, bti = -1;
@@ -294,8 +305,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace_batch(TraceArgs a) {
         const bool anyFlat = __any(live && rtd::ray_flat(r));
         if (!live) continue;
         Result res;
-        if (anyFlat) trace_ray<MODE, MT, true, LDSD, BLOCK>(a, r, s_ref, s_tn, tid, gtid, res);
-        else trace_ray<MODE, MT, false, LDSD, BLOCK>(a, r, s_ref, s_tn, tid, gtid, res);
+        const int root = ray_root(a, ri);
+        if (anyFlat) trace_ray<MODE, MT, true, LDSD, BLOCK>(a, r, s_ref, s_tn, tid, gtid, res, root);
+        else trace_ray<MODE, MT, false, LDSD, BLOCK>(a, r, s_ref, s_tn, tid, gtid, res, root);
 
         if constexpr (MODE == kModeAny) {
             a.occ[ri] = res.occluded ? 1 : 0;
@@ -411,7 +423,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_ww(TraceArgs a) {
                         rtd::ray_setup(r);
                         flat = rtd::ray_flat(r);
                         has = true;
-                        sp = 0; ref = 0; tn = -INFINITY;
+                        sp = 0; ref = ray_root(a, ri); tn = -INFINITY;
                         best = r.tmax; bu = 0.f; bv = 0.f; bid = -1; bti = -1;
                         visits = 0; tests = 0; found = false;
                     }

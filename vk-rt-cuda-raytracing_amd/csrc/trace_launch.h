@@ -32,6 +32,9 @@ struct TraceArgs {
     uint32_t* counter = nullptr;
     int2* spill = nullptr; int spillStride = 0; int maxStack = 0;
     int* status = nullptr;
+    // batched scenes (a forest of BVHs): ray i belongs to the last scene s
+    // with rayOff[s] <= i and starts at BVH4 node roots[s] (root 0 if unset)
+    const int* rayOff = nullptr; const int* roots = nullptr; int numScenes = 0;
 };
 
 int launch_trace(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int mode, bool mt, bool pinhole,

@@ -1,0 +1,148 @@
+"""Training and evaluation of the learned denoiser on path-traced pairs.
+
+Everything runs through the library's C ABI: the path tracer renders the
+pairs (1 spp input with first-hit albedo / normal / depth, many-spp target),
+the denoiser's own forward/backward kernels and Adam train on them, and the
+evaluation compares, at equal GPU time, the 1 spp + denoiser image with plain
+sample accumulation against a high-spp reference (PSNR of the tonemapped
+image x / (1 + x), peak 1).
+
+Training pairs are 256x256 windows of 1920x1080 views: the window camera is
+the full-frame camera restricted to the window (same pixel footprint), so the
+network sees the feature scale and noise of the 1080p renders it denoises.
+"""
+from __future__ import annotations
+
+import importlib
+import math
+import time
+
+import numpy as np
+
+pkg = importlib.import_module("vk-rt-cuda-raytracing_amd")
+scenes = importlib.import_module("vk-rt-cuda-raytracing_amd.scenes")
+
+
+def window_camera(cam: dict, W: int, H: int, x0: int, y0: int, w: int, h: int) -> dict:
+    """Camera of the w x h window at (x0, y0) of a W x H image of `cam`.
+
+    Pixel (px, py) of the window looks where pixel (x0+px, y0+py) of the full
+    image looks: d = f' + sx'*scale_x'*r + sy'*scale_y'*u with
+    f' = f + cx*scale_x*r + cy*scale_y*u (cx, cy: window centre in the full
+    image's [-1, 1] coordinates), scale' = scale * (w/W, h/H)."""
+    cx = (x0 + 0.5 * w) / W * 2.0 - 1.0
+    cy = (y0 + 0.5 * h) / H * 2.0 - 1.0
+    f = (cam["f"].astype(np.float64) + cx * float(cam["scale_x"]) * cam["r"].astype(np.float64) +
+         cy * float(cam["scale_y"]) * cam["u"].astype(np.float64))
+    return dict(o=cam["o"], f=f.astype(np.float32), r=cam["r"], u=cam["u"],
+                scale_x=np.float32(float(cam["scale_x"]) * w / W), scale_y=np.float32(float(cam["scale_y"]) * h / H))
+
+
+def cam_struct(cam: dict):
+    return pkg.camera(cam["o"], cam["f"], cam["r"], cam["u"], cam["scale_x"], cam["scale_y"])
+
+
+def random_view(rng, W: int = 1920, H: int = 1080, span: float = 10.0) -> dict:
+    """A seeded camera inside or near the instance volume looking at a point in it."""
+    for _ in range(100):
+        eye = rng.uniform(-1.3 * span, 1.3 * span, 3)
+        tgt = rng.uniform(-0.5 * span, 0.5 * span, 3)
+        if np.linalg.norm(tgt - eye) > 4.0:
+            return scenes.look_at(tuple(eye), tuple(tgt), W, H, 60.0)
+    return scenes.instanced_camera(W, H)
+
+
+class Renderer:
+    """A two-level scene on the GPU plus image buffers for w x h renders."""
+
+    def __init__(self, sc: dict, dev):
+        import torch
+        self.torch = torch
+        self.dev = dev
+        self.blas = [pkg.BVH.build(m) for m in sc["meshes"]]
+        self.scene = pkg.Scene(self.blas, sc["o2w"], sc["blas_of"], sc["albedo"])
+
+    def render(self, cam: dict, w: int, h: int, spp: int, seed: int, flags: int = 0):
+        t = self.torch
+        n = w * h
+        out = dict(rgb=t.empty(3 * n, device=self.dev), alb=t.empty(3 * n, device=self.dev),
+                   nrm=t.empty(3 * n, device=self.dev), dep=t.empty(n, device=self.dev))
+        p = pkg.path_params(w, h, spp=spp, seed=seed)
+        cnt = self.scene.path_trace(cam_struct(cam), p, out["rgb"], out["alb"], out["nrm"], out["dep"], counts=True,
+                                    flags=flags)
+        return out, cnt
+
+    def destroy(self):
+        self.scene.destroy()
+        for b in self.blas:
+            b.destroy()
+
+
+def tonemap(x):
+    return x / (1.0 + x)
+
+
+def psnr(a, ref) -> float:
+    """PSNR of tonemapped images (peak 1)."""
+    mse = float(((tonemap(np.maximum(a, 0)) - tonemap(np.maximum(ref, 0))) ** 2).mean())
+    return 10.0 * math.log10(1.0 / max(mse, 1e-20))
+
+
+def rel_mse(a, ref) -> float:
+    return float(((a - ref) ** 2 / (ref ** 2 + 1e-2)).mean())
+
+
+def make_pairs(dev, scene_seeds, views_per_scene: int, windows_per_view: int, target_spp: int, crop: int = 256,
+               seed: int = 0, mesh_res=scenes.INSTANCED_MESH_RES, log=print):
+    """Training pairs: returns (inputs int16 [N, crop, crop, 16] bf16 bits on
+    the device, targets float32 [N, crop, crop, 3] = log(1 + reference))."""
+    t = importlib.import_module("torch")
+    rng = np.random.default_rng(seed)
+    xs, ys = [], []
+    dn = pkg.Denoiser(crop * crop, seed=0)      # only for packing
+    W, H = 1920, 1080
+    for s in scene_seeds:
+        sc = scenes.instanced_scene(mesh_res=mesh_res, seed=int(s))
+        r = Renderer(sc, dev)
+        for v in range(views_per_scene):
+            cam = random_view(rng)
+            for k in range(windows_per_view):
+                x0 = int(rng.integers(0, (W - crop) // 8 + 1)) * 8
+                y0 = int(rng.integers(0, (H - crop) // 8 + 1)) * 8
+                wc = window_camera(cam, W, H, x0, y0, crop, crop)
+                sd = int(rng.integers(1, 2 ** 31))
+                noisy, _ = r.render(wc, crop, crop, 1, sd)
+                ref, _ = r.render(wc, crop, crop, target_spp, sd + 1)
+                x = t.empty((crop * crop, 16), dtype=t.int16, device=dev)
+                dn.pack(crop * crop, noisy["rgb"], noisy["alb"], noisy["nrm"], noisy["dep"], x)
+                xs.append(x.view(crop, crop, 16))
+                ys.append(t.log1p(ref["rgb"].view(3, crop, crop).permute(1, 2, 0).clamp_min(0)).contiguous())
+        t.cuda.synchronize()
+        r.destroy()
+        log(f"pairs: scene {s} done ({len(xs)} pairs)")
+    dn.destroy()
+    return t.stack(xs), t.stack(ys)
+
+
+def train(dn, X, Y, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 0, log=print, log_every: int = 100):
+    """Adam on random minibatches (with random flips) of the pairs; cosine
+    learning-rate decay. Returns the loss history."""
+    t = importlib.import_module("torch")
+    rng = np.random.default_rng(seed)
+    N, h, w = X.shape[0], X.shape[1], X.shape[2]
+    hist = []
+    t0 = time.time()
+    for it in range(steps):
+        idx = t.from_numpy(rng.integers(0, N, batch)).to(X.device)
+        xb, yb = X[idx], Y[idx]
+        if rng.random() < 0.5:          # horizontal flip (normals keep their world-space values)
+            xb, yb = xb.flip(2), yb.flip(2)
+        xb = xb.contiguous()
+        yb = yb.contiguous()
+        cur = lr * 0.5 * (1.0 + math.cos(math.pi * it / steps))
+        want = (it % log_every == 0) or it == steps - 1
+        loss = dn.train_step(batch, h, w, xb, yb, cur, want_loss=want)
+        if want:
+            hist.append((it, loss))
+            log(f"step {it}: loss {loss:.5f} lr {cur:.2e} ({time.time() - t0:.1f} s)")
+    return hist

@@ -207,3 +207,42 @@ SUN_DIR = (0.35, 0.85, -0.40)   # direction towards the sun (normalised on use)
 
 def instanced_camera(width: int = 1920, height: int = 1080):
     return look_at(INSTANCED_EYE, INSTANCED_TARGET, width, height, 60.0)
+
+
+# ----------------------------------------------------------------------------
+# Many independent scenes (batched build and traversal)
+# ----------------------------------------------------------------------------
+def batch_specs(n: int = 256, seed: int = 3, tris=(1000, 200000), rays=(65536, 1 << 20)):
+    """Per scene (triangle count, ray count), both log-uniform."""
+    rng = np.random.default_rng(seed)
+    t = np.exp(rng.uniform(np.log(tris[0]), np.log(tris[1]), n)).astype(np.int64)
+    r = np.exp(rng.uniform(np.log(rays[0]), np.log(rays[1]), n)).astype(np.int64)
+    return [(int(a), int(b)) for a, b in zip(t, r)]
+
+
+def batch_rays(n: int, seed: int) -> np.ndarray:
+    """n rays (8, n) for one scene of the random-triangle generator: the first
+    half pinhole primaries of the small camera ((0,0,-3) toward +z, 60 degree
+    vertical FOV) on a near-square grid, the rest from uniform points of
+    [-1,1]^3 in uniform sphere directions; tmin 0, tmax 1e30."""
+    rng = np.random.default_rng(seed)
+    npri = n // 2
+    w = max(1, int(np.ceil(np.sqrt(npri))))
+    h = max(1, (npri + w - 1) // w)
+    cam = parity_camera(w, h)
+    i = np.arange(npri)
+    px, py = i % w, i // w
+    sx = ((px + 0.5) / w * 2.0 - 1.0) * float(cam["scale_x"])
+    sy = ((py + 0.5) / h * 2.0 - 1.0) * float(cam["scale_y"])
+    d1 = cam["f"][None].astype(np.float64) + sx[:, None] * cam["r"][None] + sy[:, None] * cam["u"][None]
+    d1 /= np.linalg.norm(d1, axis=1, keepdims=True)
+    o1 = np.broadcast_to(cam["o"], (npri, 3))
+    nr = n - npri
+    d2 = rng.normal(size=(nr, 3))
+    d2 /= np.linalg.norm(d2, axis=1, keepdims=True)
+    o2 = rng.uniform(-1.0, 1.0, (nr, 3))
+    out = np.zeros((8, n), np.float32)
+    out[0:3] = np.concatenate([o1, o2]).T
+    out[3:6] = np.concatenate([d1, d2]).T
+    out[7] = 1e30
+    return out
