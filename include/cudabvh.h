@@ -297,6 +297,16 @@ int cudaBVH_sceneTraceClosestQueued(CudaScene_t scene, int capacity, const int* 
 int cudaBVH_sceneTraceAnyQueued(CudaScene_t scene, int capacity, const int* d_count,
                                 const CudaRayPacket* rays, uint8_t* occluded, unsigned flags);
 
+/* Launch configuration of the scene traversal per ray class (0 closest hit,
+ * 1 any hit): LDS stack depth (4, 8, 12, 16) and resident workgroups per CU
+ * (0 = as many as fit). sceneAutotune times every candidate on the caller's
+ * rays (results: depth, blocks/CU, ms per candidate; up to 12) and keeps the
+ * fastest. */
+int cudaBVH_sceneSetTraceConfig(CudaScene_t scene, int rayClass, int ldsDepth, int blocksPerCU);
+int cudaBVH_sceneGetTraceConfig(CudaScene_t scene, int rayClass, int* ldsDepth, int* blocksPerCU);
+int cudaBVH_sceneAutotune(CudaScene_t scene, int rayClass, int numRays, const CudaRayPacket* rays, int reps,
+                          float* results, int* numResults);
+
 /* ----- Wavefront path tracer over a scene ----- */
 #define CUDABVH_PT_MAX_DEPTH 16
 
@@ -311,6 +321,9 @@ typedef struct {
     float skyZenith[3];   /* escaped-ray radiance: horizon + (zenith - horizon) * max(d.y, 0) */
     float skyHorizon[3];
     float clampMax;       /* per-contribution max-channel clamp; <= 0 disables */
+    int tailPaths;        /* from the first bounce with at most this many live paths on, one
+                             thread per path runs all remaining bounces (0: wavefront only);
+                             the image does not depend on it */
 } CudaPathParams;
 
 /* Device buffers of width*height floats per plane. rgb: 3 planes (R, G, B).

@@ -47,7 +47,7 @@ def lib() -> C.CDLL:
             "orc_sort_keys": (None, [P, I, P, P, P]),
             "orc_num_threads": (I, [I]),
             "orc_trace_closest_2l": (None, [P, I, P, P, P, P, P, I, P, P, P, P, P, P, P, I]),
-            "orc_trace_any_2l": (None, [P, I, P, P, P, P, P, I, P, I]),
+            "orc_trace_any_2l": (None, [P, I, P, P, P, P, P, I, P, P, P, I]),
             "orc_path_trace_2l": (None, [P, I, P, P, P, P, P, P, P, P, P, P, I]),
             "orc_path_trace_f64": (None, [P, P, I, P, P, P, P, I]),
             "orc_pcg32": (None, [C.c_uint64, C.c_uint64, I, P]),
@@ -298,13 +298,18 @@ class OracleScene:
                                    threads)
         return out
 
-    def trace_any(self, rays: np.ndarray, threads: int = 0) -> np.ndarray:
+    def trace_any(self, rays: np.ndarray, threads: int = 0, counts: dict | None = None) -> np.ndarray:
+        """Occlusion flags; pass a dict as counts to receive per-ray node visits and triangle tests."""
         n = rays.shape[1]
         rays = np.ascontiguousarray(rays, np.float32)
         occ = np.zeros(n, np.uint8)
+        nv = np.zeros(n, np.int32) if counts is not None else None
+        nt = np.zeros(n, np.int32) if counts is not None else None
         lib().orc_trace_any_2l(self.tlas_nodes.ctypes.data, self.tlas_nodes.shape[0], self.tlas_prim.ctypes.data,
                                self._arr, self.blas_of.ctypes.data, self.w2o.ctypes.data, rays.ctypes.data, n,
-                               occ.ctypes.data, threads)
+                               occ.ctypes.data, _p(nv), _p(nt), threads)
+        if counts is not None:
+            counts["nodes"], counts["tris"] = nv, nt
         return occ
 
 
@@ -312,7 +317,8 @@ class PathParams(C.Structure):
     """Layout of orc_pt_params (and of the library's CudaPathParams)."""
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("maxDepth", C.c_int),
                 ("rrStart", C.c_int), ("seed", C.c_uint32), ("sunDir", C.c_float * 3), ("sunRadiance", C.c_float * 3),
-                ("skyZenith", C.c_float * 3), ("skyHorizon", C.c_float * 3), ("clampMax", C.c_float)]
+                ("skyZenith", C.c_float * 3), ("skyHorizon", C.c_float * 3), ("clampMax", C.c_float),
+                ("tailPaths", C.c_int)]
 
 
 def path_params(width, height, spp=1, max_depth=8, rr_start=3, seed=1, sun_dir=(0.35, 0.85, -0.40),

@@ -873,7 +873,7 @@ void orc_trace_closest_2l(const void* tlas_nodes, int tlas_num_nodes, const int*
 
 void orc_trace_any_2l(const void* tlas_nodes, int tlas_num_nodes, const int* tlas_prim, const orc_blas* blas,
                       const int* blas_of, const float* w2o, const float* rays, int n_rays, uint8_t* occluded,
-                      int threads) {
+                      int* node_visits, int* tri_tests, int threads) {
     int nth = orc_num_threads(threads);
     int cap = 3 * (tlas_num_nodes + 4096) + 16;
 #pragma omp parallel num_threads(nth)
@@ -884,7 +884,8 @@ void orc_trace_any_2l(const void* tlas_nodes, int tlas_num_nodes, const int* tla
             xray r; setup_ray(&r, rays, (size_t)n_rays, (size_t)i);
             int f = 0;
             two_level_one((const uint8_t*)tlas_nodes, tlas_prim, blas, blas_of, w2o, &r, 1, stk, NULL, NULL, NULL,
-                          NULL, NULL, &f, NULL, NULL, NULL);
+                          NULL, NULL, &f, node_visits ? &node_visits[i] : NULL, tri_tests ? &tri_tests[i] : NULL,
+                          NULL);
             occluded[i] = (uint8_t)f;
         }
         free(stk);

@@ -134,7 +134,8 @@ void orc_trace_closest_2l(const void* tlas_nodes, int tlas_num_nodes, const int*
                           int* node_visits, int* tri_tests, int threads);
 void orc_trace_any_2l(const void* tlas_nodes, int tlas_num_nodes, const int* tlas_prim,
                       const orc_blas* blas, const int* blas_of, const float* w2o,
-                      const float* rays, int n_rays, uint8_t* occluded, int threads);
+                      const float* rays, int n_rays, uint8_t* occluded, int* node_visits, int* tri_tests,
+                      int threads);
 
 /* ---------------- Path tracer ----------------
  * Same layout as the library's CudaPathParams. */
@@ -144,6 +145,7 @@ typedef struct {
     uint32_t seed;
     float sun_dir[3], sun_rad[3], sky_zenith[3], sky_horizon[3];
     float clamp_max;
+    int tail_paths;      /* a scheduling choice of the HIP path tracer; the image does not depend on it */
 } orc_pt_params;
 
 /* FP32 restatement of the wavefront path tracer over a two-level scene

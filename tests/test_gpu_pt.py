@@ -34,13 +34,17 @@ def _scene(pkg, sc):
     return pkg.Scene(blas, sc["o2w"], sc["blas_of"], sc["albedo"])
 
 
-@pytest.mark.parametrize("spp,depth,rr,flags", [(1, 8, 3, 0), (4, 8, 3, 0), (2, 3, 1, 0), (3, 8, 3, 1 << 7)])
-def test_path_tracer_parity(pkg, oracle, scenes, dev, spp, depth, rr, flags):
-    print(f"pt spp={spp} depth={depth} rr={rr} flags={flags}", flush=True)
+# tail: paths at which one thread per path takes over (0: wavefront only; 200:
+# from a later bounce on; 32768: this small image is a tail from the start)
+@pytest.mark.parametrize("spp,depth,rr,flags,tail", [(1, 8, 3, 0, 0), (4, 8, 3, 0, 0), (2, 3, 1, 0, 0),
+                                                     (3, 8, 3, 1 << 7, 0), (2, 8, 3, 0, 200), (2, 8, 3, 0, 32768),
+                                                     (3, 8, 3, 1 << 7, 200)])
+def test_path_tracer_parity(pkg, oracle, scenes, dev, spp, depth, rr, flags, tail):
+    print(f"pt spp={spp} depth={depth} rr={rr} flags={flags} tail={tail}", flush=True)
     sc, cam = tiny_pt_scene(scenes)
     gs = _scene(pkg, sc)
     osc = oracle.OracleScene(sc["meshes"], sc["o2w"], sc["blas_of"], gs.world_to_object())
-    p = pkg.path_params(48, 32, spp=spp, max_depth=depth, rr_start=rr, seed=7)
+    p = pkg.path_params(48, 32, spp=spp, max_depth=depth, rr_start=rr, seed=7, tail_paths=tail)
     rgb, aov, cnt = _render(pkg, gs, cam, p, dev, flags)
     op = oracle.path_params(48, 32, spp=spp, max_depth=depth, rr_start=rr, seed=7)
     ref, raov, rcnt = oracle.path_trace_2l(osc, sc["albedo"], scenes.cam_array(cam), op)
@@ -62,7 +66,7 @@ def test_path_tracer_graph_replays(pkg, oracle, scenes, dev):
     gs = _scene(pkg, sc)
     osc = oracle.OracleScene(sc["meshes"], sc["o2w"], sc["blas_of"], gs.world_to_object())
     for k, seed in enumerate((11, 12, 13)):
-        p = pkg.path_params(96, 64, spp=2, seed=seed)
+        p = pkg.path_params(96, 64, spp=2, seed=seed, tail_paths=(0, 1000, 32768)[k])
         rgb, aov, cnt = _render(pkg, gs, cam, p, dev, flags=pkg.FLAG_GRAPH)
         x = torch.arange(1000, device=dev, dtype=torch.float32).sum()     # eager work between replays
         torch.cuda.synchronize()
@@ -73,7 +77,7 @@ def test_path_tracer_graph_replays(pkg, oracle, scenes, dev):
         assert np.array_equal(_bits(rgb), _bits(ref)), k
         assert np.array_equal(_bits(aov), _bits(raov)), k
     # eager render of the last seed equals the graph render
-    e_rgb, _, e_cnt = _render(pkg, gs, cam, pkg.path_params(96, 64, spp=2, seed=13), dev)
+    e_rgb, _, e_cnt = _render(pkg, gs, cam, pkg.path_params(96, 64, spp=2, seed=13, tail_paths=0), dev)
     assert np.array_equal(_bits(e_rgb), _bits(rgb)) and np.array_equal(e_cnt, cnt)
     gs.destroy()
 

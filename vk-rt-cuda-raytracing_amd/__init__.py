@@ -71,7 +71,8 @@ class CudaSceneHit(C.Structure):
 class CudaPathParams(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int), ("spp", C.c_int), ("maxDepth", C.c_int),
                 ("rrStart", C.c_int), ("seed", C.c_uint32), ("sunDir", C.c_float * 3), ("sunRadiance", C.c_float * 3),
-                ("skyZenith", C.c_float * 3), ("skyHorizon", C.c_float * 3), ("clampMax", C.c_float)]
+                ("skyZenith", C.c_float * 3), ("skyHorizon", C.c_float * 3), ("clampMax", C.c_float),
+                ("tailPaths", C.c_int)]
 
 
 class CudaPathImage(C.Structure):
@@ -80,7 +81,7 @@ class CudaPathImage(C.Structure):
 
 def path_params(width: int, height: int, spp: int = 1, max_depth: int = 8, rr_start: int = 3, seed: int = 1,
                 sun_dir=(0.35, 0.85, -0.40), sun_radiance=(3.0, 2.85, 2.6), sky_zenith=(0.35, 0.55, 1.0),
-                sky_horizon=(1.0, 1.0, 1.0), clamp_max: float = 20.0) -> CudaPathParams:
+                sky_horizon=(1.0, 1.0, 1.0), clamp_max: float = 20.0, tail_paths: int = 32768) -> CudaPathParams:
     """Path-tracer parameters; the defaults are the instanced-scene renders'."""
     p = CudaPathParams()
     p.width, p.height, p.spp, p.maxDepth, p.rrStart, p.seed = width, height, spp, max_depth, rr_start, seed
@@ -89,6 +90,7 @@ def path_params(width: int, height: int, spp: int = 1, max_depth: int = 8, rr_st
         for k in range(3):
             dst[k] = float(src[k])
     p.clampMax = float(clamp_max)
+    p.tailPaths = int(tail_paths)
     return p
 
 
@@ -152,6 +154,9 @@ SIGNATURES = {
     "cudaBVH_sceneTraceClosestQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), C.POINTER(CudaSceneHit),
                                              C.c_uint]),
     "cudaBVH_sceneTraceAnyQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), _P, C.c_uint]),
+    "cudaBVH_sceneSetTraceConfig": (_I, [_H, _I, _I, _I]),
+    "cudaBVH_sceneGetTraceConfig": (_I, [_H, _I, _P, _P]),
+    "cudaBVH_sceneAutotune": (_I, [_H, _I, _I, C.POINTER(CudaRayPacket), _I, _P, _P]),
     "cudaBVH_scenePathTrace": (_I, [_H, C.POINTER(CudaCamera), C.POINTER(CudaPathParams), C.POINTER(CudaPathImage),
                                     _P, C.c_uint]),
     "cudaBVH_buildBatch": (_H, [_P, _P, _I]),
@@ -475,6 +480,26 @@ class Scene:
             self.num_instances
         v._borrowed = True
         return v
+
+    def set_trace_config(self, ray_class: int, lds_depth: int, blocks_per_cu: int = 0) -> None:
+        _check(self.lib.cudaBVH_sceneSetTraceConfig(self.handle, ray_class, lds_depth, blocks_per_cu),
+               "sceneSetTraceConfig")
+
+    def trace_config(self, ray_class: int):
+        d, b = C.c_int(), C.c_int()
+        _check(self.lib.cudaBVH_sceneGetTraceConfig(self.handle, ray_class, C.byref(d), C.byref(b)),
+               "sceneGetTraceConfig")
+        return d.value, b.value
+
+    def autotune(self, ray_class: int, n: int, rays, reps: int = 3):
+        """Picks the fastest (LDS depth, blocks/CU) for these rays; returns (config, table)."""
+        rp = ray_packet(rays)
+        res = np.zeros(36, np.float32)
+        k = C.c_int()
+        _check(self.lib.cudaBVH_sceneAutotune(self.handle, ray_class, n, C.byref(rp), reps, res.ctypes.data,
+                                              C.byref(k)), "sceneAutotune")
+        return self.trace_config(ray_class), [(int(res[3 * i]), int(res[3 * i + 1]), float(res[3 * i + 2]))
+                                              for i in range(k.value)]
 
     def world_to_object(self) -> np.ndarray:
         out = np.zeros((self.num_instances, 12), np.float32)

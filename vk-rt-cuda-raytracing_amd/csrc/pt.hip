@@ -23,12 +23,14 @@
 // renders the same image bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 
 #include "rt_device.h"
 #include "rt_internal.h"
 #include "rt_sample.h"
 #include "scene.h"
+#include "scene_dev.h"
 
 namespace rtk {
 
@@ -62,19 +64,26 @@ struct PtArgs {
     float* nrm[3];
     float* depth;
     rtd::Cam cam;
-    int W, H, N, maxDepth, rrStart;
+    int W, H, N, maxDepth, rrStart, tailPaths;
     float sun[3], sunRad[3], skyZ[3], skyH[3], clampMax;
+    // tail kernel: the scene and a global stack of tailDepth entries per thread
+    const uint4* tlasNodes;
+    const float4* tlasTris;
+    int2* tailStack;
+    int tailStride, tailDepth;
 };
 
 struct PtResources {
     int cap = 0;
     void* block = nullptr;            // one allocation for every per-ray buffer
     int* small = nullptr;             // counters, totals, frame
+    int2* tailStack = nullptr;        // tail kernel: [depth][threads] stack entries
+    size_t tailBytes = 0;
     hipStream_t own = nullptr;        // stream for graph capture when the scene has none
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     PtArgs key{};                     // arguments the graph was captured with
-    int keyDepth = 0;
+    rt::TraceConfig keyTune[2];       // and the scene's traversal configuration
 };
 
 __device__ __forceinline__ void clamp_contrib(float c[3], float m) {
@@ -136,7 +145,84 @@ __device__ __forceinline__ int wave_append(bool want, int* counter) {
     return want ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
 }
 
+// One path event: extension ray b (origin o, direction d) has hit (t, inst,
+// prim) or escaped (prim < 0). Adds the sky term of an escaped ray, writes the
+// first-hit buffers when aov, and for a hit prepares the shadow ray (origin so,
+// contribution c: wantShadow) and the next extension ray (origin so,
+// direction nd, throughput thr updated: wantNext). rng continues the path's
+// stream (state; the increment is fixed by the pixel).
+struct ShadeOut {
+    bool shadow, next;
+    float so[3], c[3], nd[3];
+};
+
+__device__ __forceinline__ void shade_event(const PtArgs& a, int b, bool aov, int pix, const float o[3],
+                                            const float d[3], float thr[3], rtd::Pcg32& rng, float t, int inst,
+                                            int prim, ShadeOut& out) {
+    out.shadow = false;
+    out.next = false;
+    if (prim < 0) {
+        // escaped: sky gradient by elevation
+        const float e = fmaxf(d[1], 0.0f);
+        float s[3];
+        for (int k = 0; k < 3; k++) s[k] = thr[k] * (a.skyH[k] + (a.skyZ[k] - a.skyH[k]) * e);
+        clamp_contrib(s, a.clampMax);
+        a.rgb[0][pix] += s[0]; a.rgb[1][pix] += s[1]; a.rgb[2][pix] += s[2];
+        if (aov) {
+            for (int k = 0; k < 3; k++) { a.alb[k][pix] = 0.0f; a.nrm[k][pix] = 0.0f; }
+            a.depth[pix] = 0.0f;
+        }
+        return;
+    }
+    const InstanceDev& I = a.inst[inst];
+    const BlasDev& B = a.blas[I.blas];
+    const int li = B.leafOf[prim];
+    const float4 p0 = B.tris[3 * li], p1 = B.tris[3 * li + 1], p2 = B.tris[3 * li + 2];
+    const float e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
+    const float e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
+    const float mx = e1y * e2z - e1z * e2y, my = e1z * e2x - e1x * e2z, mz = e1x * e2y - e1y * e2x;
+    const float* W = I.w2o;
+    // normals map with the transpose of the world-to-object matrix
+    float nx = (W[0] * mx + W[4] * my) + W[8] * mz;
+    float ny = (W[1] * mx + W[5] * my) + W[9] * mz;
+    float nz = (W[2] * mx + W[6] * my) + W[10] * mz;
+    const float nl = 1.0f / sqrtf((nx * nx + ny * ny) + nz * nz);
+    nx = nx * nl; ny = ny * nl; nz = nz * nl;
+    if ((nx * d[0] + ny * d[1]) + nz * d[2] > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
+    const float hx = o[0] + t * d[0], hy = o[1] + t * d[1], hz = o[2] + t * d[2];
+    const float eps = 1e-4f * fmaxf(1.0f, fmaxf(fabsf(hx), fmaxf(fabsf(hy), fabsf(hz))));
+    out.so[0] = hx + nx * eps; out.so[1] = hy + ny * eps; out.so[2] = hz + nz * eps;
+    const float alb[3] = {I.albedo[0], I.albedo[1], I.albedo[2]};
+    if (aov) {
+        for (int k = 0; k < 3; k++) a.alb[k][pix] = alb[k];
+        a.nrm[0][pix] = nx; a.nrm[1][pix] = ny; a.nrm[2][pix] = nz;
+        a.depth[pix] = t;
+    }
+    const float cosL = (nx * a.sun[0] + ny * a.sun[1]) + nz * a.sun[2];
+    if (cosL > 0.0f) {
+        const float k = cosL * kInvPi;
+        for (int j = 0; j < 3; j++) out.c[j] = ((thr[j] * alb[j]) * a.sunRad[j]) * k;
+        clamp_contrib(out.c, a.clampMax);
+        out.shadow = true;
+    }
+    if (b + 1 < a.maxDepth) {
+        rtd::cosine_dir(nx, ny, nz, rng, out.nd[0], out.nd[1], out.nd[2]);
+        for (int j = 0; j < 3; j++) thr[j] = thr[j] * alb[j];
+        out.next = true;
+        if (b + 1 >= a.rrStart) {
+            const float pm = fminf(fmaxf(thr[0], fmaxf(thr[1], thr[2])), 1.0f);
+            const float u = rng.uniform();
+            if (u >= pm) out.next = false;
+            else for (int j = 0; j < 3; j++) thr[j] = thr[j] / pm;
+        }
+    }
+}
+
+// Wavefront mode runs bounce b while more than tailPaths paths are alive.
+__device__ __forceinline__ bool wavefront_bounce(const PtArgs& a, int b) { return a.cnt[b] > a.tailPaths; }
+
 __global__ __launch_bounds__(kPtBlock) void k_pt_shade(PtArgs a, int b) {
+    if (!wavefront_bounce(a, b)) return;
     const int n = min(a.cnt[b], a.N);
     const bool aov = b == 0 && a.frame[0] == 0;
     const PtQueue& q = a.q[b & 1];
@@ -145,88 +231,31 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(PtArgs a, int b) {
     for (int base = blockIdx.x * kPtBlock; base < n; base += stride) {
         const int i = base + (int)threadIdx.x;
         const bool act = i < n;
-        bool wantShadow = false, wantNext = false;
-        float ox = 0, oy = 0, oz = 0, nx = 0, ny = 0, nz = 0, ndx = 0, ndy = 0, ndz = 0;
-        float c[3] = {0, 0, 0}, thr[3] = {0, 0, 0};
+        ShadeOut so;
+        so.shadow = so.next = false;
+        float thr[3] = {0, 0, 0};
         int pix = 0;
         rtd::Pcg32 rng{0, 1};
         if (act) {
             pix = q.pix[i];
-            const float dx = q.r[3][i], dy = q.r[4][i], dz = q.r[5][i];
+            const float o[3] = {q.r[0][i], q.r[1][i], q.r[2][i]}, d[3] = {q.r[3][i], q.r[4][i], q.r[5][i]};
             thr[0] = q.thr[0][i]; thr[1] = q.thr[1][i]; thr[2] = q.thr[2][i];
-            const int hp = a.hprim[i];
-            if (hp < 0) {
-                // escaped: sky gradient by elevation
-                const float e = fmaxf(dy, 0.0f);
-                float s[3];
-                for (int k = 0; k < 3; k++) s[k] = thr[k] * (a.skyH[k] + (a.skyZ[k] - a.skyH[k]) * e);
-                clamp_contrib(s, a.clampMax);
-                a.rgb[0][pix] += s[0]; a.rgb[1][pix] += s[1]; a.rgb[2][pix] += s[2];
-                if (aov) {
-                    for (int k = 0; k < 3; k++) { a.alb[k][pix] = 0.0f; a.nrm[k][pix] = 0.0f; }
-                    a.depth[pix] = 0.0f;
-                }
-            } else {
-                const float t = a.ht[i];
-                const int ii = a.hinst[i];
-                const InstanceDev& I = a.inst[ii];
-                const BlasDev& B = a.blas[I.blas];
-                const int li = B.leafOf[hp];
-                const float4 p0 = B.tris[3 * li], p1 = B.tris[3 * li + 1], p2 = B.tris[3 * li + 2];
-                const float e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
-                const float e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
-                const float mx = e1y * e2z - e1z * e2y, my = e1z * e2x - e1x * e2z, mz = e1x * e2y - e1y * e2x;
-                const float* W = I.w2o;
-                // normals map with the transpose of the world-to-object matrix
-                nx = (W[0] * mx + W[4] * my) + W[8] * mz;
-                ny = (W[1] * mx + W[5] * my) + W[9] * mz;
-                nz = (W[2] * mx + W[6] * my) + W[10] * mz;
-                const float nl = 1.0f / sqrtf((nx * nx + ny * ny) + nz * nz);
-                nx = nx * nl; ny = ny * nl; nz = nz * nl;
-                if ((nx * dx + ny * dy) + nz * dz > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
-                const float hx = q.r[0][i] + t * dx, hy = q.r[1][i] + t * dy, hz = q.r[2][i] + t * dz;
-                const float eps = 1e-4f * fmaxf(1.0f, fmaxf(fabsf(hx), fmaxf(fabsf(hy), fabsf(hz))));
-                ox = hx + nx * eps; oy = hy + ny * eps; oz = hz + nz * eps;
-                const float alb[3] = {I.albedo[0], I.albedo[1], I.albedo[2]};
-                if (aov) {
-                    for (int k = 0; k < 3; k++) a.alb[k][pix] = alb[k];
-                    a.nrm[0][pix] = nx; a.nrm[1][pix] = ny; a.nrm[2][pix] = nz;
-                    a.depth[pix] = t;
-                }
-                const float cosL = (nx * a.sun[0] + ny * a.sun[1]) + nz * a.sun[2];
-                if (cosL > 0.0f) {
-                    const float k = cosL * kInvPi;
-                    for (int j = 0; j < 3; j++) c[j] = ((thr[j] * alb[j]) * a.sunRad[j]) * k;
-                    clamp_contrib(c, a.clampMax);
-                    wantShadow = true;
-                }
-                if (b + 1 < a.maxDepth) {
-                    rng.state = q.rng[i];
-                    rng.inc = ((uint64_t)pix << 1u) | 1u;
-                    rtd::cosine_dir(nx, ny, nz, rng, ndx, ndy, ndz);
-                    for (int j = 0; j < 3; j++) thr[j] = thr[j] * alb[j];
-                    wantNext = true;
-                    if (b + 1 >= a.rrStart) {
-                        const float pm = fminf(fmaxf(thr[0], fmaxf(thr[1], thr[2])), 1.0f);
-                        const float u = rng.uniform();
-                        if (u >= pm) wantNext = false;
-                        else for (int j = 0; j < 3; j++) thr[j] = thr[j] / pm;
-                    }
-                }
-            }
+            rng.state = q.rng[i];
+            rng.inc = ((uint64_t)pix << 1u) | 1u;
+            shade_event(a, b, aov, pix, o, d, thr, rng, a.ht[i], a.hinst[i], a.hprim[i], so);
         }
-        const int sp = wave_append(wantShadow, &a.cnt[kPtMaxDepth + b]);
-        if (wantShadow) {
-            a.s[0][sp] = ox; a.s[1][sp] = oy; a.s[2][sp] = oz;
+        const int sp = wave_append(so.shadow, &a.cnt[kPtMaxDepth + b]);
+        if (so.shadow) {
+            a.s[0][sp] = so.so[0]; a.s[1][sp] = so.so[1]; a.s[2][sp] = so.so[2];
             a.s[3][sp] = a.sun[0]; a.s[4][sp] = a.sun[1]; a.s[5][sp] = a.sun[2];
             a.s[6][sp] = 0.0f; a.s[7][sp] = RT_MISS_T;
-            a.sc[0][sp] = c[0]; a.sc[1][sp] = c[1]; a.sc[2][sp] = c[2];
+            a.sc[0][sp] = so.c[0]; a.sc[1][sp] = so.c[1]; a.sc[2][sp] = so.c[2];
             a.spix[sp] = pix;
         }
-        const int np = wave_append(wantNext, &a.cnt[b + 1]);
-        if (wantNext) {
-            nq.r[0][np] = ox; nq.r[1][np] = oy; nq.r[2][np] = oz;
-            nq.r[3][np] = ndx; nq.r[4][np] = ndy; nq.r[5][np] = ndz;
+        const int np = wave_append(so.next, &a.cnt[b + 1]);
+        if (so.next) {
+            nq.r[0][np] = so.so[0]; nq.r[1][np] = so.so[1]; nq.r[2][np] = so.so[2];
+            nq.r[3][np] = so.nd[0]; nq.r[4][np] = so.nd[1]; nq.r[5][np] = so.nd[2];
             nq.r[6][np] = 0.0f; nq.r[7][np] = RT_MISS_T;
             nq.thr[0][np] = thr[0]; nq.thr[1][np] = thr[1]; nq.thr[2][np] = thr[2];
             nq.pix[np] = pix;
@@ -235,7 +264,62 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(PtArgs a, int b) {
     }
 }
 
+// Tail: at the first bounce b0 with at most tailPaths live paths, one thread
+// per path runs all of its remaining bounces (closest hit, shading, shadow
+// ray, next ray) with a one-ray traversal. Each pixel's contributions are
+// added in the wavefront order, so the image is the same; the launch's
+// duration is one longest path instead of one slowest ray per bounce and
+// stage.
+__global__ __launch_bounds__(64) void k_pt_tail(PtArgs a, int b0) {
+    const int n0 = a.cnt[b0];
+    if (n0 > a.tailPaths || (b0 > 0 && a.cnt[b0 - 1] <= a.tailPaths) || n0 == 0) return;
+    const bool sample0 = a.frame[0] == 0;
+    const PtQueue& q = a.q[b0 & 1];
+    const int stride = gridDim.x * 64;
+    const int g = blockIdx.x * 64 + threadIdx.x;
+    int2* stk = a.tailStack + g;
+    for (int i = g; i < n0; i += stride) {
+        const int pix = q.pix[i];
+        float o[3] = {q.r[0][i], q.r[1][i], q.r[2][i]}, d[3] = {q.r[3][i], q.r[4][i], q.r[5][i]};
+        float thr[3] = {q.thr[0][i], q.thr[1][i], q.thr[2][i]};
+        rtd::Pcg32 rng;
+        rng.state = q.rng[i];
+        rng.inc = ((uint64_t)pix << 1u) | 1u;
+        for (int b = b0; b < a.maxDepth; b++) {
+            if (b > b0) atomicAdd(&a.cnt[b], 1);
+            rtd::Ray r;
+            r.ox = o[0]; r.oy = o[1]; r.oz = o[2]; r.dx = d[0]; r.dy = d[1]; r.dz = d[2];
+            r.tmin = 0.0f; r.tmax = RT_MISS_T;
+            rtd::ray_setup(r);
+            float t;
+            int inst, prim;
+            const bool hit = trace2l_one<false>(a.tlasNodes, a.tlasTris, a.inst, a.blas, r, stk, a.tailStride,
+                                                a.tailDepth, t, inst, prim);
+            ShadeOut so;
+            shade_event(a, b, b == 0 && sample0, pix, o, d, thr, rng, hit ? t : RT_MISS_T, hit ? inst : -1,
+                        hit ? prim : -1, so);
+            if (so.shadow) {
+                atomicAdd(&a.cnt[kPtMaxDepth + b], 1);
+                rtd::Ray sr;
+                sr.ox = so.so[0]; sr.oy = so.so[1]; sr.oz = so.so[2];
+                sr.dx = a.sun[0]; sr.dy = a.sun[1]; sr.dz = a.sun[2];
+                sr.tmin = 0.0f; sr.tmax = RT_MISS_T;
+                rtd::ray_setup(sr);
+                float st;
+                int si, sprim;
+                if (!trace2l_one<true>(a.tlasNodes, a.tlasTris, a.inst, a.blas, sr, stk, a.tailStride, a.tailDepth, st,
+                                       si, sprim)) {
+                    a.rgb[0][pix] += so.c[0]; a.rgb[1][pix] += so.c[1]; a.rgb[2][pix] += so.c[2];
+                }
+            }
+            if (!so.next) break;
+            for (int k = 0; k < 3; k++) { o[k] = so.so[k]; d[k] = so.nd[k]; }
+        }
+    }
+}
+
 __global__ __launch_bounds__(kPtBlock) void k_pt_connect(PtArgs a, int b) {
+    if (!wavefront_bounce(a, b)) return;
     const int n = min(a.cnt[kPtMaxDepth + b], a.N);
     const int stride = gridDim.x * kPtBlock;
     for (int i = blockIdx.x * kPtBlock + threadIdx.x; i < n; i += stride) {
@@ -279,6 +363,7 @@ static int pt_sample(CudaSceneHandle* sc, PtArgs& a, hipStream_t s) {
         t.ox = q.r[0]; t.oy = q.r[1]; t.oz = q.r[2]; t.dx = q.r[3]; t.dy = q.r[4]; t.dz = q.r[5];
         t.tmin = q.r[6]; t.tmax = q.r[7];
         t.t = a.ht; t.inst_out = a.hinst; t.id = a.hprim;
+        t.gate = &a.cnt[b]; t.gateMax = a.tailPaths;
         if (launch_trace2l(sc, sc->scratch, t, false, s) != 0) return -1;
         hipLaunchKernelGGL(k_pt_shade, dim3(g), dim3(kPtBlock), 0, s, a, b);
         Trace2LArgs u;
@@ -286,8 +371,10 @@ static int pt_sample(CudaSceneHandle* sc, PtArgs& a, hipStream_t s) {
         u.ox = a.s[0]; u.oy = a.s[1]; u.oz = a.s[2]; u.dx = a.s[3]; u.dy = a.s[4]; u.dz = a.s[5];
         u.tmin = a.s[6]; u.tmax = a.s[7];
         u.occ = a.occ;
+        u.gate = &a.cnt[b]; u.gateMax = a.tailPaths;
         if (launch_trace2l(sc, sc->scratch, u, true, s) != 0) return -1;
         hipLaunchKernelGGL(k_pt_connect, dim3(g), dim3(kPtBlock), 0, s, a, b);
+        if (a.tailPaths > 0) hipLaunchKernelGGL(k_pt_tail, dim3(a.tailStride / 64), dim3(64), 0, s, a, b);
     }
     hipLaunchKernelGGL(k_pt_end, dim3(1), dim3(64), 0, s, a);
     const hipError_t e = hipGetLastError();
@@ -342,6 +429,7 @@ void pt_free(CudaSceneHandle* sc) {
     if (R->exec) hipGraphExecDestroy(R->exec);
     if (R->graph) hipGraphDestroy(R->graph);
     if (R->own) hipStreamDestroy(R->own);
+    hipFree(R->tailStack);
     hipFree(R->block);
     hipFree(R->small);
     delete R;
@@ -389,6 +477,24 @@ int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams&
         a.sunRad[k] = p.sunRadiance[k]; a.skyZ[k] = p.skyZenith[k]; a.skyH[k] = p.skyHorizon[k];
     }
     a.clampMax = p.clampMax;
+    a.tailPaths = std::max(0, std::min(p.tailPaths, N));
+    a.tlasNodes = (const uint4*)sc->tlas->d_nodes4;
+    a.tlasTris = (const float4*)sc->tlas->d_tris;
+    if (a.tailPaths > 0) {
+        a.tailStride = std::min(64 * 1024, (a.tailPaths + 63) / 64 * 64);
+        a.tailDepth = trace2l_max_stack(sc);
+        const size_t need = (size_t)a.tailStride * a.tailDepth * sizeof(int2);
+        if (need > R.tailBytes) {
+            hipFree(R.tailStack);
+            R.tailStack = nullptr; R.tailBytes = 0;
+            if (hipMalloc((void**)&R.tailStack, need) != hipSuccess) {
+                rt::set_error("scenePathTrace: cannot allocate %zu stack bytes", need);
+                return -1;
+            }
+            R.tailBytes = need;
+        }
+        a.tailStack = R.tailStack;
+    }
 
     const bool graph = (flags & CUDABVH_FLAG_GRAPH) != 0;
     hipStream_t s = (hipStream_t)sc->stream;
@@ -404,7 +510,7 @@ int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams&
     hipLaunchKernelGGL(k_pt_begin, dim3(1), dim3(64), 0, s, a, p.seed);
     int first = 0;
     if (graph) {
-        if (!R.exec || !same_key(R.key, a)) {
+        if (!R.exec || !same_key(R.key, a) || memcmp(R.keyTune, sc->tune2l, sizeof(R.keyTune)) != 0) {
             if (R.exec) { hipGraphExecDestroy(R.exec); R.exec = nullptr; }
             if (R.graph) { hipGraphDestroy(R.graph); R.graph = nullptr; }
             // one eager sample allocates every scratch buffer the traces need
@@ -422,6 +528,7 @@ int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams&
             R.graph = gr;
             HIP_TRY(hipGraphInstantiate(&R.exec, R.graph, nullptr, nullptr, 0), return -1);
             R.key = a;
+            memcpy(R.keyTune, sc->tune2l, sizeof(R.keyTune));
         }
         for (int k = first; k < p.spp; k++) HIP_TRY(hipGraphLaunch(R.exec, s), return -1);
     } else {

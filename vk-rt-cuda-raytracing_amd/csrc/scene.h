@@ -53,6 +53,7 @@ struct Trace2LArgs {
     uint32_t* counter = nullptr;
     int2* spill = nullptr; int spillStride = 0; int maxStack = 0;
     int* status = nullptr;
+    const int* gate = nullptr; int gateMax = -1;   // skip the launch when *gate <= gateMax
 };
 
 }  // namespace rtk
@@ -70,10 +71,12 @@ struct CudaSceneHandle {
     rt::TraceScratch scratch;
     void* stream = nullptr;
     void* pt = nullptr;                        // path-tracer queues and graph (pt.hip)
+    rt::TraceConfig tune2l[2];                 // launch configuration: closest hit, any hit
 };
 
 namespace rtk {
 int launch_trace2l(const CudaSceneHandle* sc, rt::TraceScratch& scratch, Trace2LArgs& a, bool any, hipStream_t s);
+int trace2l_max_stack(const CudaSceneHandle* sc);
 int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams& p, const CudaPathImage& img,
                long long* rayCounts, unsigned flags);
 void pt_free(CudaSceneHandle* sc);

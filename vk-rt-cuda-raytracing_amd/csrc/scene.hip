@@ -11,75 +11,17 @@
 // phases, idle lanes refilled from a wave-local pool of 64-ray batches).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 
 #include "rt_device.h"
 #include "rt_internal.h"
 #include "scene.h"
+#include "scene_dev.h"
 
 namespace rtk {
 
 using rtd::Ray;
-
-constexpr int kRetMark = INT_MIN;
-
-__device__ __forceinline__ void xform(const float* M, const Ray& w, Ray& o) {
-    o.ox = ((M[0] * w.ox + M[1] * w.oy) + M[2] * w.oz) + M[3];
-    o.oy = ((M[4] * w.ox + M[5] * w.oy) + M[6] * w.oz) + M[7];
-    o.oz = ((M[8] * w.ox + M[9] * w.oy) + M[10] * w.oz) + M[11];
-    o.dx = (M[0] * w.dx + M[1] * w.dy) + M[2] * w.dz;
-    o.dy = (M[4] * w.dx + M[5] * w.dy) + M[6] * w.dz;
-    o.dz = (M[8] * w.dx + M[9] * w.dy) + M[10] * w.dz;
-    o.tmin = w.tmin;
-    o.tmax = w.tmax;
-    rtd::ray_setup(o);
-}
-
-template <bool ANY, class Push>
-__device__ __forceinline__ void visit2(const uint4* __restrict__ nodes, const Ray& r, float limit, int& ref, float& tn,
-                                       Push&& push) {
-    const uint4* np = nodes + 4 * (size_t)ref;
-    const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
-    if constexpr (ANY) {
-        float d0, d1, d2, d3;
-        const bool h0 = (int)qc.x != -1 && rtd::slab_child<0, true>(r, qx, qy, qz, limit, d0);
-        const bool h1 = (int)qc.y != -1 && rtd::slab_child<1, true>(r, qx, qy, qz, limit, d1);
-        const bool h2 = (int)qc.z != -1 && rtd::slab_child<2, true>(r, qx, qy, qz, limit, d2);
-        const bool h3 = (int)qc.w != -1 && rtd::slab_child<3, true>(r, qx, qy, qz, limit, d3);
-        int nxt = -1;
-        if (h0) { nxt = (int)qc.x; }
-        if (h1) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.y; }
-        if (h2) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.z; }
-        if (h3) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.w; }
-        ref = nxt;
-        tn = 0.f;
-    } else {
-        float d[4];
-        int c[4];
-        float tt;
-        if ((int)qc.x != -1 && rtd::slab_child<0, true>(r, qx, qy, qz, limit, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
-        if ((int)qc.y != -1 && rtd::slab_child<1, true>(r, qx, qy, qz, limit, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
-        if ((int)qc.z != -1 && rtd::slab_child<2, true>(r, qx, qy, qz, limit, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
-        if ((int)qc.w != -1 && rtd::slab_child<3, true>(r, qx, qy, qz, limit, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
-#define RT_CSWAP(i, j)                                                                  \
-    {                                                                                   \
-        const bool s = d[i] > d[j];                                                     \
-        const float di = d[i], dj = d[j];                                               \
-        const int ci = c[i], cj = c[j];                                                 \
-        d[i] = s ? dj : di; d[j] = s ? di : dj; c[i] = s ? cj : ci; c[j] = s ? ci : cj; \
-    }
-        RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
-#undef RT_CSWAP
-        int nxt = -1;
-        float nt = 0.f;
-        if (c[3] != -1) { nxt = c[3]; nt = d[3]; }
-        if (c[2] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[2]; nt = d[2]; }
-        if (c[1] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[1]; nt = d[1]; }
-        if (c[0] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[0]; nt = d[0]; }
-        ref = nxt;
-        tn = nt;
-    }
-}
 
 template <bool ANY, int LDSD, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
@@ -87,6 +29,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
     __shared__ float s_tn[LDSD * BLOCK];
     const int tid = threadIdx.x, lane = tid & 63;
     const int gtid = blockIdx.x * BLOCK + tid;
+    if (a.gate && *a.gate <= a.gateMax) return;
     const int n = a.nDev ? min(*a.nDev, a.n) : a.n;
 
     Ray rw, ro;
@@ -154,8 +97,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
         bool fin = false;
         for (;;) {
             if (has && !fin) {
-                while (ref == -1 || ref == kRetMark || (!ANY && tn > best)) {
-                    if (ref == kRetMark) { inst = -1; nodes = a.tlasNodes; ref = -1; continue; }
+                while (ref == -1 || ref == kReturnMark || (!ANY && tn > best)) {
+                    if (ref == kReturnMark) { inst = -1; nodes = a.tlasNodes; ref = -1; continue; }
                     if (sp == 0) { fin = true; break; }
                     sp--;
                     if (sp < LDSD) {
@@ -179,7 +122,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
                     // TLAS leaf: enter its instance
                     const int enc = -(ref + 2);
                     const int k = __float_as_int(a.tlasTris[3 * (enc >> 3)].w);
-                    push(kRetMark, -INFINITY);
+                    push(kReturnMark, -INFINITY);
                     inst = k;
                     const InstanceDev& I = a.inst[k];
                     xform(I.w2o, rw, ro);
@@ -235,7 +178,7 @@ constexpr int kBlock2 = 256;
 __global__ void k_reset_counter_2l(uint32_t* c) { c[0] = 0u; }
 
 template <bool ANY, int LDSD>
-static int launch2(Trace2LArgs& a, rt::TraceScratch& sc, int maxStack, hipStream_t s) {
+static int launch2(Trace2LArgs& a, rt::TraceScratch& sc, int maxStack, int blocksPerCU, hipStream_t s) {
     static int maxBlocks = 0;
     auto kern = k_trace2l<ANY, LDSD, kBlock2>;
     if (!maxBlocks) {
@@ -246,7 +189,8 @@ static int launch2(Trace2LArgs& a, rt::TraceScratch& sc, int maxStack, hipStream
     int dev = 0, cus = 1;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int grid = maxBlocks * cus, threads = grid * kBlock2;
+    const int perCU = blocksPerCU > 0 ? std::min(blocksPerCU, maxBlocks) : maxBlocks;
+    const int grid = perCU * cus, threads = grid * kBlock2;
     a.maxStack = maxStack;
     const int spillDepth = maxStack > LDSD ? maxStack - LDSD : 0;
     const size_t need = (size_t)spillDepth * threads * sizeof(int2);
@@ -268,14 +212,28 @@ static int launch2(Trace2LArgs& a, rt::TraceScratch& sc, int maxStack, hipStream
     return 0;
 }
 
+// Stack entries a two-level traversal can need: each visited node pushes at
+// most 3 entries along a root-to-leaf path, plus the return marker.
+int trace2l_max_stack(const CudaSceneHandle* sc) { return 3 * sc->tlas->depth4 + 4 + 1 + 3 * sc->maxBlasDepth + 4; }
+
 int launch_trace2l(const CudaSceneHandle* sc, rt::TraceScratch& scratch, Trace2LArgs& a, bool any, hipStream_t s) {
     if (a.n <= 0) return 0;
     a.tlasNodes = (const uint4*)sc->tlas->d_nodes4;
     a.tlasTris = (const float4*)sc->tlas->d_tris;
     a.inst = sc->d_inst;
     a.blas = sc->d_blas;
-    const int maxStack = 3 * sc->tlas->depth4 + 4 + 1 + 3 * sc->maxBlasDepth + 4;
-    return any ? launch2<true, 8>(a, scratch, maxStack, s) : launch2<false, 8>(a, scratch, maxStack, s);
+    const int maxStack = trace2l_max_stack(sc);
+    const rt::TraceConfig& c = sc->tune2l[any ? 1 : 0];
+    switch (c.ldsDepth) {
+        case 4: return any ? launch2<true, 4>(a, scratch, maxStack, c.blocksPerCU, s)
+                           : launch2<false, 4>(a, scratch, maxStack, c.blocksPerCU, s);
+        case 12: return any ? launch2<true, 12>(a, scratch, maxStack, c.blocksPerCU, s)
+                            : launch2<false, 12>(a, scratch, maxStack, c.blocksPerCU, s);
+        case 16: return any ? launch2<true, 16>(a, scratch, maxStack, c.blocksPerCU, s)
+                            : launch2<false, 16>(a, scratch, maxStack, c.blocksPerCU, s);
+        default: return any ? launch2<true, 8>(a, scratch, maxStack, c.blocksPerCU, s)
+                            : launch2<false, 8>(a, scratch, maxStack, c.blocksPerCU, s);
+    }
 }
 
 }  // namespace rtk

@@ -1,0 +1,141 @@
+// scene_dev.h — device pieces of the two-level traversal shared by the
+// persistent traversal kernel (scene.hip) and the path tracer's tail kernel
+// (pt.hip): the instance transform, the node visit, and a one-ray traversal.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "rt_device.h"
+#include "scene.h"
+
+namespace rtk {
+
+using rtd::Ray;
+
+constexpr int kReturnMark = INT_MIN;   // return marker: leave the instance
+
+__device__ __forceinline__ void xform(const float* M, const Ray& w, Ray& o) {
+    o.ox = ((M[0] * w.ox + M[1] * w.oy) + M[2] * w.oz) + M[3];
+    o.oy = ((M[4] * w.ox + M[5] * w.oy) + M[6] * w.oz) + M[7];
+    o.oz = ((M[8] * w.ox + M[9] * w.oy) + M[10] * w.oz) + M[11];
+    o.dx = (M[0] * w.dx + M[1] * w.dy) + M[2] * w.dz;
+    o.dy = (M[4] * w.dx + M[5] * w.dy) + M[6] * w.dz;
+    o.dz = (M[8] * w.dx + M[9] * w.dy) + M[10] * w.dz;
+    o.tmin = w.tmin;
+    o.tmax = w.tmax;
+    rtd::ray_setup(o);
+}
+
+template <bool ANY, class Push>
+__device__ __forceinline__ void visit2(const uint4* __restrict__ nodes, const Ray& r, float limit, int& ref, float& tn,
+                                       Push&& push) {
+    const uint4* np = nodes + 4 * (size_t)ref;
+    const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
+    if constexpr (ANY) {
+        float d0, d1, d2, d3;
+        const bool h0 = (int)qc.x != -1 && rtd::slab_child<0, true>(r, qx, qy, qz, limit, d0);
+        const bool h1 = (int)qc.y != -1 && rtd::slab_child<1, true>(r, qx, qy, qz, limit, d1);
+        const bool h2 = (int)qc.z != -1 && rtd::slab_child<2, true>(r, qx, qy, qz, limit, d2);
+        const bool h3 = (int)qc.w != -1 && rtd::slab_child<3, true>(r, qx, qy, qz, limit, d3);
+        int nxt = -1;
+        if (h0) { nxt = (int)qc.x; }
+        if (h1) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.y; }
+        if (h2) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.z; }
+        if (h3) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.w; }
+        ref = nxt;
+        tn = 0.f;
+    } else {
+        float d[4];
+        int c[4];
+        float tt;
+        if ((int)qc.x != -1 && rtd::slab_child<0, true>(r, qx, qy, qz, limit, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
+        if ((int)qc.y != -1 && rtd::slab_child<1, true>(r, qx, qy, qz, limit, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
+        if ((int)qc.z != -1 && rtd::slab_child<2, true>(r, qx, qy, qz, limit, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
+        if ((int)qc.w != -1 && rtd::slab_child<3, true>(r, qx, qy, qz, limit, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
+#define RT_CSWAP(i, j)                                                                  \
+    {                                                                                   \
+        const bool s = d[i] > d[j];                                                     \
+        const float di = d[i], dj = d[j];                                               \
+        const int ci = c[i], cj = c[j];                                                 \
+        d[i] = s ? dj : di; d[j] = s ? di : dj; c[i] = s ? cj : ci; c[j] = s ? ci : cj; \
+    }
+        RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
+#undef RT_CSWAP
+        int nxt = -1;
+        float nt = 0.f;
+        if (c[3] != -1) { nxt = c[3]; nt = d[3]; }
+        if (c[2] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[2]; nt = d[2]; }
+        if (c[1] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[1]; nt = d[1]; }
+        if (c[0] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[0]; nt = d[0]; }
+        ref = nxt;
+        tn = nt;
+    }
+}
+
+
+// One ray through TLAS and BLAS with a stack in global memory (entries
+// stk[k * stride], k < maxStack): the traversal of k_trace2l for a single
+// ray (same node order, culling and tie rule), for kernels that give each
+// thread its own ray. Closest hit: returns whether something was hit and
+// (t, inst, id); any hit: whether the ray is occluded.
+template <bool ANY>
+__device__ __forceinline__ bool trace2l_one(const uint4* tlasNodes, const float4* tlasTris, const InstanceDev* insts,
+                                            const BlasDev* blas, const Ray& rw, int2* stk, int stride, int maxStack,
+                                            float& bt, int& binst, int& bid) {
+    Ray ro;
+    int sp = 0, ref = 0, inst = -1;
+    float tn = -INFINITY, best = rw.tmax;
+    const uint4* nodes = tlasNodes;
+    const float4* tris = nullptr;
+    bid = -1; binst = -1; bt = rw.tmax;
+    auto push = [&](int rf, float t) {
+        if (sp < maxStack) { stk[(size_t)sp * stride] = make_int2(rf, __float_as_int(t)); sp++; }
+    };
+    for (;;) {
+        while (ref == -1 || ref == kReturnMark || (!ANY && tn > best)) {
+            if (ref == kReturnMark) { inst = -1; nodes = tlasNodes; ref = -1; continue; }
+            if (sp == 0) return ANY ? false : bid >= 0;
+            sp--;
+            const int2 e = stk[(size_t)sp * stride];
+            ref = e.x; tn = __int_as_float(e.y);
+        }
+        if (ref >= 0) {
+            const Ray& r = inst < 0 ? rw : ro;
+            visit2<ANY>(nodes, r, ANY ? r.tmax : best, ref, tn, push);
+        } else if (inst < 0) {
+            const int enc = -(ref + 2);
+            const int k = __float_as_int(tlasTris[3 * (enc >> 3)].w);
+            push(kReturnMark, -INFINITY);
+            inst = k;
+            const InstanceDev& I = insts[k];
+            xform(I.w2o, rw, ro);
+            const BlasDev& B = blas[I.blas];
+            nodes = B.nodes;
+            tris = B.tris;
+            ref = 0;
+            tn = -INFINITY;
+        } else {
+            const int enc = -(ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
+            for (int q = 0; q < tc; q++) {
+                const int ti = ts + q;
+                const float4 p0 = tris[3 * ti];
+                float t, u, v;
+                if (!rtd::isect_watertight(ro, p0, tris[3 * ti + 1], tris[3 * ti + 2], t, u, v)) continue;
+                if (!(t > ro.tmin) || !(t < ro.tmax)) continue;
+                if constexpr (ANY) {
+                    return true;
+                } else {
+                    const int id = __float_as_int(p0.w);
+                    if (t < best || (t == best && (inst < binst || (inst == binst && id < bid)))) {
+                        best = t; bid = id; binst = inst; bt = t;
+                    }
+                }
+            }
+            ref = -1;
+        }
+    }
+}
+
+}  // namespace rtk
