@@ -49,6 +49,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 22, help="max rays per class for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch configuration)")
+    ap.add_argument("--no-pt", action="store_true", help="skip the path-tracer leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the many-scenes leg")
+    ap.add_argument("--no-denoise", action="store_true", help="skip the denoiser leg")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--ref-spp", type=int, default=1024, help="reference spp of the denoiser evaluation")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -234,9 +239,241 @@ def main():
         "build_s": round(build_s, 2),
         "tuning": tuning,
     }
-    print(json.dumps(out), flush=True)
     bvh.set_stream(None)
     bvh.destroy()
+    del D, S, dt, did, du, dv, occ
+    torch.cuda.empty_cache()
+    if not args.no_pt:
+        out["path_tracer"] = bench_path_tracer(pkg, scenes, dev, args)
+    if not args.no_batch:
+        out["batched_scenes"] = bench_batched(pkg, scenes, dev, args)
+    if not args.no_denoise:
+        out["denoiser"] = bench_denoiser(pkg, scenes, dev, args)
+    print(json.dumps(out), flush=True)
+
+
+def bench_path_tracer(pkg, scenes, dev, args):
+    """Wavefront path tracer, instanced scene (64 instances of 4 meshes),
+    1920x1080, 1 spp, depth 8, Russian roulette from ray 3; one sample's
+    launch sequence replayed from a hipGraph. Rays = extension + shadow."""
+    import torch
+    sc = scenes.instanced_scene()
+    t0 = time.time()
+    blas = [pkg.BVH.build(m) for m in sc["meshes"]]
+    gs = pkg.Scene(blas, sc["o2w"], sc["blas_of"], sc["albedo"])
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    W, H = 1920, 1080
+    n = W * H
+    cam = scenes.instanced_camera(W, H)
+    c = pkg.camera(cam["o"], cam["f"], cam["r"], cam["u"], cam["scale_x"], cam["scale_y"])
+    tuning = {}
+    if not args.no_tune:
+        R = {k: torch.empty(n, device=dev) for k in RAY_KEYS}
+        pkg.gen_pinhole_rays(W, H, c, R)
+        t = torch.empty(n, device=dev)
+        inst = torch.empty(n, dtype=torch.int32, device=dev)
+        pid = torch.empty(n, dtype=torch.int32, device=dev)
+        gs.trace_closest(R, t, inst, pid)
+        torch.cuda.synchronize()
+        hit = pid >= 0
+        m = int(hit.sum())
+        o = torch.stack([R[k] for k in ("ox", "oy", "oz")])[:, hit]
+        d = torch.stack([R[k] for k in ("dx", "dy", "dz")])[:, hit]
+        P = o + t[hit] * d
+        sun = torch.tensor(scenes.SUN_DIR, device=dev)
+        sun = sun / sun.norm()
+        S = {"ox": P[0] + 1e-3 * sun[0], "oy": P[1] + 1e-3 * sun[1], "oz": P[2] + 1e-3 * sun[2],
+             "dx": torch.full((m,), float(sun[0]), device=dev), "dy": torch.full((m,), float(sun[1]), device=dev),
+             "dz": torch.full((m,), float(sun[2]), device=dev), "tmin": torch.zeros(m, device=dev),
+             "tmax": torch.full((m,), 1e30, device=dev)}
+        S = {k: v.contiguous() for k, v in S.items()}
+        for name, cls, rays, cnt in (("closest", 0, R, n), ("shadow_any", 1, S, m)):
+            best, table = gs.autotune(cls, cnt, rays, reps=3)
+            tuning[name] = {"lds_depth": best[0], "blocks_per_cu": best[1],
+                            "sweep_ms": {f"{d}/{b}": round(ms, 4) for d, b, ms in table}}
+            log(f"pt autotune {name}: lds {best[0]} blocks/CU {best[1] or 'max'}")
+        del R, S, t, inst, pid
+    rgb = torch.empty(3 * n, device=dev)
+    alb = torch.empty(3 * n, device=dev)
+    nrm = torch.empty(3 * n, device=dev)
+    dep = torch.empty(n, device=dev)
+    p = pkg.path_params(W, H, spp=1, seed=1)
+    cnt = gs.path_trace(c, p, rgb, alb, nrm, dep, counts=True, flags=pkg.FLAG_GRAPH)
+    K = 16
+    p = pkg.path_params(W, H, spp=K, seed=2)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    gs.path_trace(c, p, rgb, alb, nrm, dep, flags=pkg.FLAG_GRAPH)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / K
+    rays = int(cnt.sum())
+    d = p.maxDepth
+    log(f"path tracer: {ms:.3f} ms/sample, {rays} rays/sample, {rays / (ms * 1e3):.1f} Mrays/s")
+    gs.destroy()
+    return {"workload": "64 instances of 4 heightfield meshes (10,082/49,928/199,712/500,000 tris), 1920x1080, "
+                        "1 spp, depth 8, RR from ray 3, sun NEE + sky, hipGraph replay",
+            "instanced_triangles": int(sum(len(sc["meshes"][b]) for b in sc["blas_of"])),
+            "unique_triangles": int(sum(len(m) for m in sc["meshes"])),
+            "ms_per_sample": round(ms, 3), "rays_per_sample": rays, "mrays_s": round(rays / (ms * 1e3), 1),
+            "extension_rays_per_bounce": [int(x) for x in cnt[:d]],
+            "shadow_rays_per_bounce": [int(x) for x in cnt[d:]],
+            "build_s": round(build_s, 2), "tuning": tuning}
+
+
+def batch_rays_torch(torch, n, seed, dev, cam):
+    """The many-scenes ray mix on the device: half pinhole primaries of the
+    small camera on a near-square grid, half from uniform points of [-1,1]^3
+    in uniform sphere directions."""
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    npri = n // 2
+    w = max(1, int(np.ceil(np.sqrt(npri))))
+    h = max(1, (npri + w - 1) // w)
+    i = torch.arange(npri, device=dev)
+    sx = ((i % w).float() + 0.5) / w * 2 - 1
+    sy = ((i // w).float() + 0.5) / h * 2 - 1
+    f, r, u = (torch.tensor(cam[k], device=dev) for k in ("f", "r", "u"))
+    th = float(np.tan(np.radians(30.0)))
+    d1 = f[:, None] + (sx * th * w / h)[None] * r[:, None] + (sy * th)[None] * u[:, None]
+    d1 = d1 / d1.norm(dim=0)
+    o1 = torch.tensor(cam["o"], device=dev)[:, None].expand(3, npri)
+    nr = n - npri
+    d2 = torch.randn(3, nr, device=dev, generator=g)
+    d2 = d2 / d2.norm(dim=0)
+    o2 = torch.rand(3, nr, device=dev, generator=g) * 2 - 1
+    return torch.cat([o1, o2], 1), torch.cat([d1, d2], 1)
+
+
+def bench_batched(pkg, scenes, dev, args):
+    """Many independent scenes: random-triangle scenes with log-uniform
+    triangle counts in [1K, 200K] and ray counts in [64K, 1M], all BVHs built
+    in one forest pass, all rays traced by one persistent launch (device work
+    counter) replayed from a hipGraph. Aggregate Mrays/s, closest hit."""
+    import torch
+    specs = scenes.batch_specs(args.batch_scenes)
+    t0 = time.time()
+    tris = [scenes.random_triangles(nt, seed=5000 + k) for k, (nt, _) in enumerate(specs)]
+    gen_s = time.time() - t0
+    t0 = time.time()
+    b = pkg.Batch(tris)
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    ntris = int(b.tri_offsets[-1])
+    del tris
+    roff = np.concatenate([[0], np.cumsum([nr for _, nr in specs])]).astype(np.int32)
+    total = int(roff[-1])
+    cam = scenes.parity_camera(512, 512)
+    O = torch.empty(3, total, device=dev)
+    Dd = torch.empty(3, total, device=dev)
+    for k, (_, nr) in enumerate(specs):
+        o, d = batch_rays_torch(torch, nr, 7000 + k, dev, cam)
+        O[:, roff[k]:roff[k + 1]] = o
+        Dd[:, roff[k]:roff[k + 1]] = d
+    R = {"ox": O[0], "oy": O[1], "oz": O[2], "dx": Dd[0], "dy": Dd[1], "dz": Dd[2],
+         "tmin": torch.zeros(total, device=dev), "tmax": torch.full((total,), 1e30, device=dev)}
+    R = {k: v.contiguous() for k, v in R.items()}
+    del O, Dd
+    d_off = torch.from_numpy(roff).to(dev)
+    t = torch.empty(total, device=dev)
+    pid = torch.empty(total, dtype=torch.int32, device=dev)
+    u = torch.empty(total, device=dev)
+    v = torch.empty(total, device=dev)
+    for _ in range(2):
+        b.trace_closest(total, d_off, R, t, pid, u, v, flags=pkg.FLAG_GRAPH)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    K = 5
+    e0.record()
+    for _ in range(K):
+        b.trace_closest(total, d_off, R, t, pid, u, v, flags=pkg.FLAG_GRAPH)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / K
+    hit = float((pid >= 0).float().mean())
+    log(f"batched: {len(specs)} scenes, {ntris} tris built in {build_s:.2f} s; {total} rays in {ms:.2f} ms "
+        f"= {total / (ms * 1e3):.1f} Mrays/s (hit {hit:.3f})")
+    b.destroy()
+    return {"workload": f"{len(specs)} random-triangle scenes, tris log-uniform [1K,200K], rays log-uniform "
+                        "[64K,1M] (half primary, half random), one forest build, one graph-replayed launch",
+            "scenes": len(specs), "triangles": ntris, "rays": total, "build_s": round(build_s, 2),
+            "scene_generation_s": round(gen_s, 2), "trace_ms": round(ms, 3),
+            "mrays_s": round(total / (ms * 1e3), 1), "hit_rate": round(hit, 4)}
+
+
+def denoiser_gflop(W, H):
+    """Multiply-adds of one frame counted from the layer list (2 FLOP each)."""
+    dn_layers = [(16, 32, 9, 0), (32, 32, 9, 0), (32, 64, 9, 1), (64, 64, 9, 1), (64, 128, 9, 2), (128, 128, 9, 2),
+                 (128, 256, 9, 3), (256, 256, 9, 3), (384, 128, 9, 2), (128, 128, 9, 2), (192, 64, 9, 1),
+                 (64, 64, 9, 1), (96, 32, 9, 0), (32, 32, 9, 0), (32, 3, 1, 0)]
+    macs = sum(cin * cout * taps * (W >> lv) * (H >> lv) for cin, cout, taps, lv in dn_layers)
+    return 2.0 * macs / 1e9
+
+
+def bench_denoiser(pkg, scenes, dev, args):
+    """The learned denoiser at 1920x1080: inference time and TFLOP/s (BF16
+    MFMA), and the equal-time quality gain over plain sample accumulation
+    against a high-spp reference of the held-out bench view (PSNR of x/(1+x))."""
+    import torch
+    dt = importlib.import_module("vk-rt-cuda-raytracing_amd.denoise_train")
+    W, H = 1920, 1080
+    n = W * H
+    dn = pkg.Denoiser(1920 * 1088, seed=0)
+    wpath = dt.WEIGHTS
+    trained = os.path.exists(wpath)
+    if trained:
+        dn.set_params(dt.load_weights(wpath))
+    sc = scenes.instanced_scene(seed=2)
+    r = dt.Renderer(sc, dev)
+    cam = scenes.instanced_camera(W, H)
+    noisy, _ = r.render(cam, W, H, 1, 5, flags=pkg.FLAG_GRAPH)
+    out = torch.empty(3 * n, device=dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    dn.denoise(W, H, noisy["rgb"], noisy["alb"], noisy["nrm"], noisy["dep"], out)
+    torch.cuda.synchronize()
+    K = 5
+    e0.record()
+    for _ in range(K):
+        dn.denoise(W, H, noisy["rgb"], noisy["alb"], noisy["nrm"], noisy["dep"], out)
+    e1.record()
+    torch.cuda.synchronize()
+    t_dn = e0.elapsed_time(e1) / K
+    e0.record()
+    for k in range(K):
+        noisy, _ = r.render(cam, W, H, 1, 5, flags=pkg.FLAG_GRAPH)
+    e1.record()
+    torch.cuda.synchronize()
+    t_pt = e0.elapsed_time(e1) / K
+    gflop = denoiser_gflop(W, H)
+    res = {"workload": "U-Net 32/64/128/256, 1920x1080, 16-channel BF16 input (10 used), FP32 accumulation",
+           "inference_ms": round(t_dn, 3), "gflop_per_frame": round(gflop, 1),
+           "achieved_tflops": round(gflop / t_dn, 1), "peak_tflops_bf16_dense": 2500.0,
+           "mfma_frac": round(gflop / t_dn / 2500.0, 4), "path_trace_ms_per_spp": round(t_pt, 3),
+           "trained_weights": os.path.relpath(wpath, ROOT) if trained else None}
+    log(f"denoiser: {t_dn:.2f} ms/frame ({gflop / t_dn:.1f} TFLOP/s); path tracer {t_pt:.2f} ms/spp")
+    if trained:
+        ref, _ = r.render(cam, W, H, args.ref_spp, 999, flags=pkg.FLAG_GRAPH)
+        k = max(1, int(np.ceil((t_pt + t_dn) / t_pt)))
+        acc, _ = r.render(cam, W, H, k, 77, flags=pkg.FLAG_GRAPH)
+        dn.denoise(W, H, noisy["rgb"], noisy["alb"], noisy["nrm"], noisy["dep"], out)
+        torch.cuda.synchronize()
+        refn = ref["rgb"].cpu().numpy().reshape(3, H, W)
+        one = noisy["rgb"].cpu().numpy().reshape(3, H, W)
+        den = out.cpu().numpy().reshape(3, H, W)
+        accn = acc["rgb"].cpu().numpy().reshape(3, H, W)
+        p1, pk, pd = dt.psnr(one, refn), dt.psnr(accn, refn), dt.psnr(den, refn)
+        res.update({"reference_spp": args.ref_spp, "equal_time_spp": k, "psnr_1spp": round(p1, 2),
+                    "psnr_equal_time_accumulation": round(pk, 2), "psnr_1spp_denoised": round(pd, 2),
+                    "psnr_gain_db": round(pd - pk, 2),
+                    "relmse_equal_time_accumulation": round(dt.rel_mse(accn, refn), 5),
+                    "relmse_1spp_denoised": round(dt.rel_mse(den, refn), 5)})
+        log(f"denoiser quality: 1spp {p1:.2f} dB, {k} spp {pk:.2f} dB, 1spp+denoiser {pd:.2f} dB "
+            f"(gain {pd - pk:+.2f} dB vs equal time)")
+    r.destroy()
+    dn.destroy()
+    return res
 
 
 def load_traffic():

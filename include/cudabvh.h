@@ -94,7 +94,7 @@ void cudaBVH_getBounds(CudaBVH_t bvh, float* centerX, float* centerY, float* cen
  * real compressed-wide-BVH node count (0 until one is built). */
 int cudaBVH_getNumTris(CudaBVH_t bvh);
 int cudaBVH_getNumBVH4Nodes(CudaBVH_t bvh);
-int cudaBVH_getNumCWBVHNodes(CudaBVH_t bvh);
+int cudaBVH_getNumCWBVHNodes(CudaBVH_t bvh);   /* 0 until the CWBVH is built (unlike the reference's BVH4 count) */
 
 /* layer/cuda_bvh_backend.h:91. numNodes x 64 bytes: halves boundsX[8],
  * boundsY[8], boundsZ[8] (mins in 0-3, maxes in 4-7, rounded outward), then
@@ -167,7 +167,8 @@ enum {
     CUDABVH_FLAG_HOST_POINTERS = 1u << 4, /* packets hold host pointers: copy in/out */
     CUDABVH_FLAG_SORT_RAYS     = 1u << 5, /* octant+Morton sort before tracing */
     CUDABVH_FLAG_SYNC          = 1u << 6, /* wait for completion before returning */
-    CUDABVH_FLAG_GRAPH         = 1u << 7  /* path tracer: capture one sample in a hipGraph, replay it */
+    CUDABVH_FLAG_GRAPH         = 1u << 7, /* capture the launches in a hipGraph once, replay them */
+    CUDABVH_FLAG_CWBVH         = 1u << 8  /* trace the compressed 8-wide BVH (built on first use) */
 };
 
 /* Closest hit for n rays. Device pointers unless CUDABVH_FLAG_HOST_POINTERS.
@@ -255,6 +256,16 @@ const void* cudaBVH_deviceNodes(CudaBVH_t bvh);
 const void* cudaBVH_deviceTris(CudaBVH_t bvh);
 /* Depth of the BVH4 (node levels on the longest root-to-leaf path). */
 int cudaBVH_getDepth(CudaBVH_t bvh);
+
+/* Compressed 8-wide BVH (80-byte nodes: origin, per-axis power-of-two
+ * scales, 8-bit child boxes rounded outward, 8 child slots) collapsed from the
+ * binned-SAH BVH2; traced with CUDABVH_FLAG_CWBVH, with the same hits as the
+ * BVH4. buildCWBVH returns the node count (or -1); getCWBVH gives borrowed
+ * host copies: 20 uint32 per node (p.xyz, e.x|e.y<<8|e.z<<16|imask<<24,
+ * childBase, triBase, meta[8], qlo/qhi x, y, z) and the leaf-order index of
+ * each CWBVH triangle. */
+int cudaBVH_buildCWBVH(CudaBVH_t bvh);
+int cudaBVH_getCWBVH(CudaBVH_t bvh, const uint32_t** nodes, const int** triOrder, int* depth);
 
 /* ----- Two-level scenes (instances of BLAS handles under a TLAS) ----- */
 typedef struct CudaSceneHandle* CudaScene_t;

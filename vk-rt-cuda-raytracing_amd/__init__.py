@@ -31,6 +31,7 @@ FLAG_HOST_POINTERS = 1 << 4
 FLAG_SORT_RAYS = 1 << 5
 FLAG_SYNC = 1 << 6
 FLAG_GRAPH = 1 << 7
+FLAG_CWBVH = 1 << 8
 PT_MAX_DEPTH = 16
 MISS_T = np.float32(1e30)
 
@@ -142,6 +143,8 @@ SIGNATURES = {
     "cudaBVH_deviceNodes": (_P, [_H]),
     "cudaBVH_deviceTris": (_P, [_H]),
     "cudaBVH_getDepth": (_I, [_H]),
+    "cudaBVH_buildCWBVH": (_I, [_H]),
+    "cudaBVH_getCWBVH": (_I, [_H, _P, _P, _P]),
     "cudaBVH_sceneCreate": (_H, [_P, _I, _P, _I]),
     "cudaBVH_sceneDestroy": (None, [_H]),
     "cudaBVH_sceneSetStream": (_I, [_H, _P]),
@@ -331,6 +334,22 @@ class BVH:
     # --- streams ---
     def set_stream(self, stream: int | None) -> None:
         _check(self.lib.cudaBVH_setStream(self.handle, stream), "setStream")
+
+    def build_cwbvh(self) -> int:
+        n = self.lib.cudaBVH_buildCWBVH(self.handle)
+        if n < 0:
+            raise CudaBVHError(f"buildCWBVH: {last_error()}")
+        return n
+
+    def cwbvh(self):
+        """(nodes uint32 (n, 20), tri_order int32 (m,), depth) host copies."""
+        nodes, order, depth = C.c_void_p(), C.c_void_p(), C.c_int()
+        n = self.lib.cudaBVH_getCWBVH(self.handle, C.byref(nodes), C.byref(order), C.byref(depth))
+        if n <= 0:
+            raise CudaBVHError("no CWBVH built")
+        nd = np.ctypeslib.as_array(C.cast(nodes, C.POINTER(C.c_uint32)), shape=(n * 20,)).reshape(n, 20).copy()
+        od = np.ctypeslib.as_array(C.cast(order, C.POINTER(C.c_int32)), shape=(self.num_tris,)).copy()
+        return nd, od, depth.value
 
     # --- reference traces (layer/cuda_bvh_backend.h:29-73) ---
     def trace_primary(self, side: int, cam_o, out_hit_t: np.ndarray | None = None) -> float:

@@ -25,10 +25,13 @@ const char* last_error();
 using rt::set_error;
 
 static inline hipStream_t hstream(const CudaBVHHandle* h) { return (hipStream_t)h->stream; }
+static int ensure_cwbvh(CudaBVH_t bvh);
 
 static void free_handle_device(CudaBVHHandle* h) {
     hipFree(h->d_nodes4); hipFree(h->d_tris); hipFree(h->d_leafOfPrim); hipFree(h->d_work); hipFree(h->d_sortWs);
     h->d_sortWs = nullptr; h->sortWsBytes = 0;
+    hipFree(h->d_cwNodes); hipFree(h->d_cwTris);
+    h->d_cwNodes = h->d_cwTris = nullptr;
     rtk::free_scratch(h->scratch);
     h->d_nodes4 = h->d_tris = h->d_work = nullptr;
     h->d_leafOfPrim = nullptr;
@@ -94,6 +97,7 @@ static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf, const in
         if (numScenes == 1) {
             rt::export_dfs(nodes2, h->h_dfs);
             h->numDFS = int(nodes2.size());
+            h->h_nodes2 = std::move(nodes2);
         }
     }
     if (rtk::alloc_scratch(h->scratch) != 0) goto fail;
@@ -132,6 +136,8 @@ static void* sort_workspace(CudaBVHHandle* h, int n) {
     if (bytes > h->sortWsBytes) {
         hipFree(h->d_sortWs);
         h->d_sortWs = nullptr; h->sortWsBytes = 0;
+    hipFree(h->d_cwNodes); hipFree(h->d_cwTris);
+    h->d_cwNodes = h->d_cwTris = nullptr;
         if (hipMalloc(&h->d_sortWs, bytes) != hipSuccess) { set_error("cannot allocate %zu sort bytes", bytes); return nullptr; }
         h->sortWsBytes = bytes;
     }
@@ -452,6 +458,20 @@ void cudaBVH_getBounds(CudaBVH_t bvh, float* cx, float* cy, float* cz, float* ex
 int cudaBVH_getNumTris(CudaBVH_t bvh) { return bvh ? bvh->numOrdered : 0; }
 int cudaBVH_getNumBVH4Nodes(CudaBVH_t bvh) { return bvh ? bvh->numNodes4 : 0; }
 int cudaBVH_getNumCWBVHNodes(CudaBVH_t bvh) { return bvh ? bvh->numCWBVH : 0; }
+
+int cudaBVH_buildCWBVH(CudaBVH_t bvh) {
+    rt::clear_error();
+    if (!check_handle(bvh, "cudaBVH_buildCWBVH")) return -1;
+    return ensure_cwbvh(bvh) == 0 ? bvh->numCWBVH : -1;
+}
+
+int cudaBVH_getCWBVH(CudaBVH_t bvh, const uint32_t** nodes, const int** triOrder, int* depth) {
+    if (!check_handle(bvh, "cudaBVH_getCWBVH") || !bvh->numCWBVH) return 0;
+    if (nodes) *nodes = bvh->h_cw.data();
+    if (triOrder) *triOrder = bvh->h_cwTriOrder.data();
+    if (depth) *depth = bvh->cwDepth;
+    return bvh->numCWBVH;
+}
 int cudaBVH_getDepth(CudaBVH_t bvh) { return bvh ? bvh->depth4 : 0; }
 
 const void* cudaBVH_getNodeData(CudaBVH_t bvh) { return bvh ? bvh->h_nodes4.data() : nullptr; }
@@ -533,6 +553,35 @@ static bool packet_ok(const CudaRayPacket* r) {
 
 // Shared body of the packet entry points. Host pointers are staged through
 // the handle's work buffer.
+// Builds (once) the compressed 8-wide BVH and its triangle order from the BVH2.
+static int ensure_cwbvh(CudaBVH_t bvh) {
+    if (bvh->d_cwNodes) return 0;
+    if (bvh->h_nodes2.empty()) { set_error("CWBVH: this handle keeps no BVH2 (batched forest)"); return -1; }
+    const int nn = rt::build_cwbvh(bvh->h_nodes2, bvh->h_cw, bvh->h_cwTriOrder, bvh->cwDepth);
+    if (nn <= 0) { set_error("CWBVH build failed"); return -1; }
+    const size_t nt = bvh->h_cwTriOrder.size();
+    std::vector<float> tri(nt * 12, 0.f);
+    for (size_t k = 0; k < nt; k++) {
+        const int li = bvh->h_cwTriOrder[k];
+        float* o = &tri[12 * k];
+        for (int v = 0; v < 3; v++)
+            for (int a = 0; a < 3; a++) o[4 * v + a] = bvh->h_soa[3 * v + a][li];
+        const int id = bvh->h_ordered[li];
+        memcpy(&o[3], &id, 4);
+    }
+    if (hipMalloc(&bvh->d_cwNodes, bvh->h_cw.size() * 4) != hipSuccess ||
+        hipMalloc(&bvh->d_cwTris, tri.size() * 4) != hipSuccess ||
+        hipMemcpy(bvh->d_cwNodes, bvh->h_cw.data(), bvh->h_cw.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(bvh->d_cwTris, tri.data(), tri.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(bvh->d_cwNodes); hipFree(bvh->d_cwTris);
+        bvh->d_cwNodes = bvh->d_cwTris = nullptr;
+        set_error("CWBVH: device allocation failed");
+        return -1;
+    }
+    bvh->numCWBVH = nn;
+    return 0;
+}
+
 static int trace_packet(CudaBVH_t bvh, int n, const CudaRayPacket* rays, const CudaHitPacket* hits, uint8_t* occ,
                         int* cn, int* ct, unsigned flags, int mode, const char* fn) {
     rt::clear_error();
@@ -545,7 +594,12 @@ static int trace_packet(CudaBVH_t bvh, int n, const CudaRayPacket* rays, const C
     if (sorted && cn) { set_error("%s: counted traces do not sort", fn); return -1; }
     const unsigned isect = flags & CUDABVH_ISECT_MASK;
     if (isect > 1) { set_error("%s: unknown intersection mode %u", fn, isect); return -1; }
-    if (n == 0) return 0;
+    const bool cw = (flags & CUDABVH_FLAG_CWBVH) != 0;
+    if (cw && (sorted || cn || isect != 0)) {
+        set_error("%s: the CWBVH traversal is watertight, unsorted and uncounted", fn);
+        return -1;
+    }
+    if (cw && ensure_cwbvh(bvh) != 0) return -1;
     hipStream_t s = hstream(bvh);
     rtk::TraceArgs a;
     a.n = n;
@@ -557,7 +611,9 @@ static int trace_packet(CudaBVH_t bvh, int n, const CudaRayPacket* rays, const C
         if (hits) { a.t = hits->t; a.id = hits->primId; a.u = hits->u; a.v = hits->v; }
         a.occ = occ; a.cntNodes = cn; a.cntTris = ct;
         int rc;
-        if (sorted) {
+        if (cw) {
+            rc = rtk::launch_trace_cw(bvh->d_cwNodes, bvh->d_cwTris, bvh->cwDepth, bvh->scratch, a, mode, s);
+        } else if (sorted) {
             void* ws = sort_workspace(bvh, n);
             if (!ws) return -1;
             rc = rtk::trace_sorted(bvh, bvh->scratch, n, *rays, hits, occ, mode, isect == 1, ws, s);
@@ -578,7 +634,9 @@ static int trace_packet(CudaBVH_t bvh, int n, const CudaRayPacket* rays, const C
     a.cntNodes = (int*)(w + 12 * N); a.cntTris = (int*)(w + 13 * N);
     a.occ = (uint8_t*)(w + 14 * N);
     int rc;
-    if (sorted) {
+    if (cw) {
+        rc = rtk::launch_trace_cw(bvh->d_cwNodes, bvh->d_cwTris, bvh->cwDepth, bvh->scratch, a, mode, s);
+    } else if (sorted) {
         void* ws = sort_workspace(bvh, n);
         if (!ws) return -1;
         CudaRayPacket rp{a.ox, a.oy, a.oz, a.dx, a.dy, a.dz, a.tmin, a.tmax};

@@ -59,6 +59,11 @@ int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& ou
 void scan_exclusive(const int* in, int* out, int n, int* total, int* tiles, hipStream_t s);
 int scan_tiles_needed(int n);
 
+// Compressed 8-wide BVH from the pre-order BVH2: 20 uint32 (80 B) per node,
+// triOrder[k] = leaf-order triangle of CWBVH triangle k. Returns the node
+// count (-1 if a leaf exceeds 4 triangles).
+int build_cwbvh(const std::vector<Node2>& n2, std::vector<uint32_t>& nodes, std::vector<int>& triOrder, int& depth);
+
 // Stackless DFS BVH2 export (backend :456-522): 8 uint32 per node.
 void export_dfs(const std::vector<Node2>& nodes, std::vector<uint32_t>& out);
 
@@ -118,6 +123,12 @@ struct CudaBVHHandle {
     std::vector<float> h_packed;
     std::vector<int> h_ordered;
     int numDFS = 0;
+    std::vector<rt::Node2> h_nodes2;    // pre-order BVH2 (source of the CWBVH)
+    std::vector<uint32_t> h_cw;         // CWBVH nodes (20 words each), built on request
+    std::vector<int> h_cwTriOrder;
+    void* d_cwNodes = nullptr;
+    void* d_cwTris = nullptr;
+    int cwDepth = 0;
 
     float bboxMin[3], bboxMax[3], bboxCenter[3], bboxExtent = 0.f;
 

@@ -46,6 +46,9 @@ int launch_bounce(const CudaBVHHandle* h, rt::TraceScratch& sc, const rtd::Cam& 
                   const float* hitT, const int* hitId, uint32_t seed, const float light[3], const CudaRayPacket& d,
                   const CudaRayPacket& s, int* src, int* count, hipStream_t st);
 
+int launch_trace_cw(const void* nodes, const void* tris, int depth8, rt::TraceScratch& sc, TraceArgs& a, int mode,
+                    hipStream_t s);
+
 size_t sort_scratch_bytes(int n);
 int trace_sorted(CudaBVHHandle* h, rt::TraceScratch& sc, int n, const CudaRayPacket& rays, const CudaHitPacket* hits,
                  uint8_t* occ, int mode, bool mt, void* ws, hipStream_t s);

@@ -15,12 +15,34 @@ from __future__ import annotations
 
 import importlib
 import math
+import os
+import sys
 import time
 
 import numpy as np
 
+if __name__ == "__main__":
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 pkg = importlib.import_module("vk-rt-cuda-raytracing_amd")
 scenes = importlib.import_module("vk-rt-cuda-raytracing_amd.scenes")
+
+# Trained parameters shipped with the package (BF16 bit patterns, .npy),
+# written by train_and_save() below.
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "denoiser_instanced.npy")
+
+
+def load_weights(path: str = WEIGHTS) -> np.ndarray:
+    """FP32 parameters from the stored BF16 bit patterns."""
+    bits = np.load(path, allow_pickle=False).astype(np.uint16)
+    return (bits.astype(np.uint32) << 16).view(np.float32)
+
+
+def save_weights(params: np.ndarray, path: str = WEIGHTS) -> None:
+    """Stores the parameters as BF16 (round to nearest even) bit patterns."""
+    u = np.ascontiguousarray(params, np.float32).view(np.uint32).astype(np.uint64)
+    bits = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint16)
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.save(path, bits, allow_pickle=False)
 
 
 def window_camera(cam: dict, W: int, H: int, x0: int, y0: int, w: int, h: int) -> dict:
@@ -112,7 +134,7 @@ def make_pairs(dev, scene_seeds, views_per_scene: int, windows_per_view: int, ta
                 wc = window_camera(cam, W, H, x0, y0, crop, crop)
                 sd = int(rng.integers(1, 2 ** 31))
                 noisy, _ = r.render(wc, crop, crop, 1, sd)
-                ref, _ = r.render(wc, crop, crop, target_spp, sd + 1)
+                ref, _ = r.render(wc, crop, crop, target_spp, sd + 1, flags=pkg.FLAG_GRAPH)
                 x = t.empty((crop * crop, 16), dtype=t.int16, device=dev)
                 dn.pack(crop * crop, noisy["rgb"], noisy["alb"], noisy["nrm"], noisy["dep"], x)
                 xs.append(x.view(crop, crop, 16))
@@ -146,3 +168,31 @@ def train(dn, X, Y, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 0,
             hist.append((it, loss))
             log(f"step {it}: loss {loss:.5f} lr {cur:.2e} ({time.time() - t0:.1f} s)")
     return hist
+
+
+def train_and_save(dev, scene_seeds=range(100, 108), views: int = 4, windows: int = 16, target_spp: int = 512,
+                   steps: int = 4000, batch: int = 8, lr: float = 1e-3, seed: int = 7, path: str = WEIGHTS, log=print):
+    """Renders the training pairs, trains a fresh network and stores it."""
+    t0 = time.time()
+    X, Y = make_pairs(dev, scene_seeds, views, windows, target_spp, log=log)
+    log(f"{X.shape[0]} pairs in {time.time() - t0:.1f} s")
+    dn = pkg.Denoiser(max(batch * X.shape[1] * X.shape[2], 1920 * 1088), seed=seed)
+    hist = train(dn, X, Y, steps, batch=batch, lr=lr, log=log)
+    save_weights(dn.params(0), path)
+    dn.destroy()
+    return hist
+
+
+if __name__ == "__main__":
+    import argparse
+    import torch
+    ap = argparse.ArgumentParser(description="render training pairs with the path tracer and train the denoiser")
+    ap.add_argument("--scenes", type=int, default=8)
+    ap.add_argument("--views", type=int, default=4)
+    ap.add_argument("--windows", type=int, default=16)
+    ap.add_argument("--target-spp", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--out", default=WEIGHTS)
+    a = ap.parse_args()
+    train_and_save(torch.device("cuda:0"), range(100, 100 + a.scenes), a.views, a.windows, a.target_spp, a.steps,
+                   path=a.out, log=lambda m: print(m, flush=True))
