@@ -290,9 +290,11 @@ def bench_path_tracer(pkg, scenes, dev, args):
         S = {k: v.contiguous() for k, v in S.items()}
         for name, cls, rays, cnt in (("closest", 0, R, n), ("shadow_any", 1, S, m)):
             best, table = gs.autotune(cls, cnt, rays, reps=3)
-            tuning[name] = {"lds_depth": best[0], "blocks_per_cu": best[1],
-                            "sweep_ms": {f"{d}/{b}": round(ms, 4) for d, b, ms in table}}
-            log(f"pt autotune {name}: lds {best[0]} blocks/CU {best[1] or 'max'}")
+            sched = ("while-while", "batch", "batch-nearest-first")
+            tuning[name] = {"schedule": sched[best[0]], "lds_depth": best[1], "blocks_per_cu": best[2],
+                            "sweep_ms": {f"{sched[sc]}/{d}/{b}": round(ms, 4) for sc, d, b, ms in table}}
+            log(f"pt autotune {name}: {sched[best[0]]} lds {best[1]} blocks/CU {best[2] or 'max'} "
+                f"({min(x[3] for x in table):.3f} ms)")
         del R, S, t, inst, pid
     rgb = torch.empty(3 * n, device=dev)
     alb = torch.empty(3 * n, device=dev)

@@ -309,12 +309,13 @@ int cudaBVH_sceneTraceAnyQueued(CudaScene_t scene, int capacity, const int* d_co
                                 const CudaRayPacket* rays, uint8_t* occluded, unsigned flags);
 
 /* Launch configuration of the scene traversal per ray class (0 closest hit,
- * 1 any hit): LDS stack depth (4, 8, 12, 16) and resident workgroups per CU
- * (0 = as many as fit). sceneAutotune times every candidate on the caller's
- * rays (results: depth, blocks/CU, ms per candidate; up to 12) and keeps the
- * fastest. */
-int cudaBVH_sceneSetTraceConfig(CudaScene_t scene, int rayClass, int ldsDepth, int blocksPerCU);
-int cudaBVH_sceneGetTraceConfig(CudaScene_t scene, int rayClass, int* ldsDepth, int* blocksPerCU);
+ * 1 any hit): schedule (0 while-while with a wave pool, 1 one ray per lane in
+ * 64-ray batches, 2 = 1 with nearest-first children for any hit), LDS stack
+ * depth (4, 8, 12, 16) and resident workgroups per CU (0 = as many as fit).
+ * sceneAutotune times every candidate on the caller's rays (results: schedule,
+ * depth, blocks/CU, ms per candidate; up to 36) and keeps the fastest. */
+int cudaBVH_sceneSetTraceConfig(CudaScene_t scene, int rayClass, int schedule, int ldsDepth, int blocksPerCU);
+int cudaBVH_sceneGetTraceConfig(CudaScene_t scene, int rayClass, int* schedule, int* ldsDepth, int* blocksPerCU);
 int cudaBVH_sceneAutotune(CudaScene_t scene, int rayClass, int numRays, const CudaRayPacket* rays, int reps,
                           float* results, int* numResults);
 

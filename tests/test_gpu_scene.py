@@ -115,3 +115,46 @@ def test_scene_edge_cases(pkg, oracle, scenes, dev):
     # bad arguments are reported, not crashed on
     with pytest.raises(pkg.CudaBVHError):
         pkg.Scene([pkg.BVH.build(sc["meshes"][0])], sc["o2w"][:1], [3])
+
+
+def test_scene_every_launch_configuration(pkg, oracle, scenes, dev):
+    """Every schedule (while-while, batch, batch nearest-first for any hit)
+    and a range of LDS depths gives the oracle's hits; the autotuner keeps a
+    configuration from its own table."""
+    import torch
+    sc = scenes.instanced_scene(mesh_res=(6, 11, 17, 23), n_inst=24, seed=5, span=6.0)
+    gs = _make(pkg, sc)
+    osc = oracle.OracleScene(sc["meshes"], sc["o2w"], sc["blas_of"], gs.world_to_object())
+    n = 20000
+    rays = scene_rays(n, 9, span=6.0)
+    rng = np.random.default_rng(1)
+    srays = rays.copy()
+    srays[7] = rng.uniform(0.5, 40.0, n).astype(np.float32)
+    ref = osc.trace_closest(rays)
+    occ_ref = osc.trace_any(srays)
+    R, S = to_dev(rays, dev), to_dev(srays, dev)
+    for sched in (0, 1, 2):
+        for depth in (4, 16):
+            print(f"scene config schedule={sched} lds={depth}", flush=True)
+            if sched < 2:
+                gs.set_trace_config(0, sched, depth, 0)
+                t = torch.empty(n, device=dev)
+                inst = torch.empty(n, dtype=torch.int32, device=dev)
+                pid = torch.empty(n, dtype=torch.int32, device=dev)
+                u = torch.empty(n, device=dev)
+                v = torch.empty(n, device=dev)
+                gs.trace_closest(R, t, inst, pid, u, v)
+                torch.cuda.synchronize()
+                assert np.array_equal(pid.cpu().numpy(), ref["id"]) and np.array_equal(inst.cpu().numpy(), ref["inst"])
+                for k, g in (("t", t), ("u", u), ("v", v)):
+                    assert np.array_equal(_bits(g.cpu().numpy()), _bits(ref[k])), (sched, depth, k)
+            gs.set_trace_config(1, sched, depth, 4)
+            occ = torch.full((n,), 9, dtype=torch.uint8, device=dev)
+            gs.trace_any(S, occ)
+            torch.cuda.synchronize()
+            assert np.array_equal(occ.cpu().numpy(), occ_ref), (sched, depth)
+    with pytest.raises(pkg.CudaBVHError):
+        gs.set_trace_config(0, 2, 8, 0)          # nearest-first ordering is an any-hit schedule
+    best, table = gs.autotune(1, n, S, reps=1)
+    assert len(table) == 36 and best in [(s_, d_, b_) for s_, d_, b_, _ in table]
+    gs.destroy()

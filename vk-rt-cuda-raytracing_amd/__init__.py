@@ -157,8 +157,8 @@ SIGNATURES = {
     "cudaBVH_sceneTraceClosestQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), C.POINTER(CudaSceneHit),
                                              C.c_uint]),
     "cudaBVH_sceneTraceAnyQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), _P, C.c_uint]),
-    "cudaBVH_sceneSetTraceConfig": (_I, [_H, _I, _I, _I]),
-    "cudaBVH_sceneGetTraceConfig": (_I, [_H, _I, _P, _P]),
+    "cudaBVH_sceneSetTraceConfig": (_I, [_H, _I, _I, _I, _I]),
+    "cudaBVH_sceneGetTraceConfig": (_I, [_H, _I, _P, _P, _P]),
     "cudaBVH_sceneAutotune": (_I, [_H, _I, _I, C.POINTER(CudaRayPacket), _I, _P, _P]),
     "cudaBVH_scenePathTrace": (_I, [_H, C.POINTER(CudaCamera), C.POINTER(CudaPathParams), C.POINTER(CudaPathImage),
                                     _P, C.c_uint]),
@@ -500,25 +500,25 @@ class Scene:
         v._borrowed = True
         return v
 
-    def set_trace_config(self, ray_class: int, lds_depth: int, blocks_per_cu: int = 0) -> None:
-        _check(self.lib.cudaBVH_sceneSetTraceConfig(self.handle, ray_class, lds_depth, blocks_per_cu),
+    def set_trace_config(self, ray_class: int, schedule: int, lds_depth: int, blocks_per_cu: int = 0) -> None:
+        _check(self.lib.cudaBVH_sceneSetTraceConfig(self.handle, ray_class, schedule, lds_depth, blocks_per_cu),
                "sceneSetTraceConfig")
 
     def trace_config(self, ray_class: int):
-        d, b = C.c_int(), C.c_int()
-        _check(self.lib.cudaBVH_sceneGetTraceConfig(self.handle, ray_class, C.byref(d), C.byref(b)),
+        sc, d, b = C.c_int(), C.c_int(), C.c_int()
+        _check(self.lib.cudaBVH_sceneGetTraceConfig(self.handle, ray_class, C.byref(sc), C.byref(d), C.byref(b)),
                "sceneGetTraceConfig")
-        return d.value, b.value
+        return sc.value, d.value, b.value
 
     def autotune(self, ray_class: int, n: int, rays, reps: int = 3):
-        """Picks the fastest (LDS depth, blocks/CU) for these rays; returns (config, table)."""
+        """Picks the fastest (schedule, LDS depth, blocks/CU) for these rays; returns (config, table)."""
         rp = ray_packet(rays)
-        res = np.zeros(36, np.float32)
+        res = np.zeros(4 * 36, np.float32)
         k = C.c_int()
         _check(self.lib.cudaBVH_sceneAutotune(self.handle, ray_class, n, C.byref(rp), reps, res.ctypes.data,
                                               C.byref(k)), "sceneAutotune")
-        return self.trace_config(ray_class), [(int(res[3 * i]), int(res[3 * i + 1]), float(res[3 * i + 2]))
-                                              for i in range(k.value)]
+        return self.trace_config(ray_class), [(int(res[4 * i]), int(res[4 * i + 1]), int(res[4 * i + 2]),
+                                               float(res[4 * i + 3])) for i in range(k.value)]
 
     def world_to_object(self) -> np.ndarray:
         out = np.zeros((self.num_instances, 12), np.float32)

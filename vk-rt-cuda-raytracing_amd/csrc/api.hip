@@ -913,7 +913,9 @@ CudaScene_t cudaBVH_sceneCreate(const CudaBVH_t* blas, int numBlas, const CudaIn
     for (int b = 0; b < numBlas; b++)
         if (!blas[b]) { set_error("cudaBVH_sceneCreate: BLAS %d is NULL", b); return nullptr; }
     auto* sc = new CudaSceneHandle();
+    sc->tune2l[0].schedule = 0;      // while-while
     sc->tune2l[0].ldsDepth = 8;
+    sc->tune2l[1].schedule = 0;
     sc->tune2l[1].ldsDepth = 8;
     sc->blas.assign(blas, blas + numBlas);
     sc->inst.resize(numInst);
@@ -1176,29 +1178,31 @@ int cudaBVH_batchTraceAny(CudaBVHBatch_t b, int totalRays, const int* d_rayOffse
     return batch_trace(b, totalRays, d_rayOffsets, rays, nullptr, occluded, flags, true, "cudaBVH_batchTraceAny");
 }
 
-int cudaBVH_sceneSetTraceConfig(CudaScene_t sc, int rayClass, int ldsDepth, int blocksPerCU) {
+int cudaBVH_sceneSetTraceConfig(CudaScene_t sc, int rayClass, int schedule, int ldsDepth, int blocksPerCU) {
     rt::clear_error();
     if (!scene_ok(sc, "cudaBVH_sceneSetTraceConfig")) return -1;
-    if (rayClass < 0 || rayClass > 1 || (ldsDepth != 4 && ldsDepth != 8 && ldsDepth != 12 && ldsDepth != 16) ||
-        blocksPerCU < 0) {
-        set_error("cudaBVH_sceneSetTraceConfig: class %d, LDS depth %d, blocks/CU %d not supported", rayClass,
-                  ldsDepth, blocksPerCU);
+    if (rayClass < 0 || rayClass > 1 || schedule < 0 || schedule > (rayClass ? 2 : 1) ||
+        (ldsDepth != 4 && ldsDepth != 8 && ldsDepth != 12 && ldsDepth != 16) || blocksPerCU < 0) {
+        set_error("cudaBVH_sceneSetTraceConfig: class %d, schedule %d, LDS depth %d, blocks/CU %d not supported",
+                  rayClass, schedule, ldsDepth, blocksPerCU);
         return -1;
     }
+    sc->tune2l[rayClass].schedule = schedule;
     sc->tune2l[rayClass].ldsDepth = ldsDepth;
     sc->tune2l[rayClass].blocksPerCU = blocksPerCU;
     return 0;
 }
 
-int cudaBVH_sceneGetTraceConfig(CudaScene_t sc, int rayClass, int* ldsDepth, int* blocksPerCU) {
+int cudaBVH_sceneGetTraceConfig(CudaScene_t sc, int rayClass, int* schedule, int* ldsDepth, int* blocksPerCU) {
     if (!scene_ok(sc, "cudaBVH_sceneGetTraceConfig") || rayClass < 0 || rayClass > 1) return -1;
+    if (schedule) *schedule = sc->tune2l[rayClass].schedule;
     if (ldsDepth) *ldsDepth = sc->tune2l[rayClass].ldsDepth;
     if (blocksPerCU) *blocksPerCU = sc->tune2l[rayClass].blocksPerCU;
     return 0;
 }
 
-// Times every (LDS depth, blocks per CU) candidate on the caller's rays
-// (best of reps launches each) and keeps the fastest.
+// Times every (schedule, LDS depth, blocks per CU) candidate on the caller's
+// rays (best of reps launches each) and keeps the fastest.
 int cudaBVH_sceneAutotune(CudaScene_t sc, int rayClass, int numRays, const CudaRayPacket* rays, int reps,
                           float* results, int* numResults) {
     rt::clear_error();
@@ -1209,40 +1213,45 @@ int cudaBVH_sceneAutotune(CudaScene_t sc, int rayClass, int numRays, const CudaR
     }
     hipStream_t s = (hipStream_t)sc->stream;
     void* buf = nullptr;
-    if (hipMalloc(&buf, (size_t)numRays * 12) != hipSuccess) { set_error("cudaBVH_sceneAutotune: no memory"); return -1; }
-    CudaSceneHit hit{(float*)buf, (int*)buf + numRays, (int*)buf + 2 * (size_t)numRays, nullptr, nullptr};
+    if (hipMalloc(&buf, (size_t)numRays * 20) != hipSuccess) { set_error("cudaBVH_sceneAutotune: no memory"); return -1; }
+    CudaSceneHit hit{(float*)buf, (int*)buf + numRays, (int*)buf + 2 * (size_t)numRays,
+                     (float*)buf + 3 * (size_t)numRays, (float*)buf + 4 * (size_t)numRays};
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     const int depths[4] = {4, 8, 12, 16}, bpcs[3] = {0, 4, 2};
+    const int nsched = rayClass ? 3 : 2;
     rt::TraceConfig keep = sc->tune2l[rayClass], best = keep;
     float bestMs = 1e30f;
     int k = 0, rc = 0;
-    for (int di = 0; di < 4 && rc == 0; di++)
-        for (int bi = 0; bi < 3 && rc == 0; bi++) {
-            sc->tune2l[rayClass].ldsDepth = depths[di];
-            sc->tune2l[rayClass].blocksPerCU = bpcs[bi];
-            float ms = 1e30f;
-            for (int r = 0; r <= reps && rc == 0; r++) {      // the first launch warms up
-                hipEventRecord(e0, s);
-                rc = rayClass ? scene_trace(sc, numRays, nullptr, rays, nullptr, (uint8_t*)buf, nullptr, nullptr, 0, true,
-                                            "cudaBVH_sceneAutotune")
-                              : scene_trace(sc, numRays, nullptr, rays, &hit, nullptr, nullptr, nullptr, 0, false,
-                                            "cudaBVH_sceneAutotune");
-                hipEventRecord(e1, s);
-                hipEventSynchronize(e1);
-                float t = 0.f;
-                hipEventElapsedTime(&t, e0, e1);
-                if (r > 0) ms = std::min(ms, t);
+    for (int si = 0; si < nsched && rc == 0; si++)
+        for (int di = 0; di < 4 && rc == 0; di++)
+            for (int bi = 0; bi < 3 && rc == 0; bi++) {
+                sc->tune2l[rayClass].schedule = si;
+                sc->tune2l[rayClass].ldsDepth = depths[di];
+                sc->tune2l[rayClass].blocksPerCU = bpcs[bi];
+                float ms = 1e30f;
+                for (int r = 0; r <= reps && rc == 0; r++) {      // the first launch warms up
+                    hipEventRecord(e0, s);
+                    rc = rayClass ? scene_trace(sc, numRays, nullptr, rays, nullptr, (uint8_t*)buf, nullptr, nullptr, 0,
+                                                true, "cudaBVH_sceneAutotune")
+                                  : scene_trace(sc, numRays, nullptr, rays, &hit, nullptr, nullptr, nullptr, 0, false,
+                                                "cudaBVH_sceneAutotune");
+                    hipEventRecord(e1, s);
+                    hipEventSynchronize(e1);
+                    float t = 0.f;
+                    hipEventElapsedTime(&t, e0, e1);
+                    if (r > 0) ms = std::min(ms, t);
+                }
+                if (results) {
+                    results[4 * k] = (float)si;
+                    results[4 * k + 1] = (float)depths[di];
+                    results[4 * k + 2] = (float)bpcs[bi];
+                    results[4 * k + 3] = ms;
+                }
+                k++;
+                if (ms < bestMs) { bestMs = ms; best = sc->tune2l[rayClass]; }
             }
-            if (results) {
-                results[3 * k] = (float)depths[di];
-                results[3 * k + 1] = (float)bpcs[bi];
-                results[3 * k + 2] = ms;
-            }
-            k++;
-            if (ms < bestMs) { bestMs = ms; best = sc->tune2l[rayClass]; }
-        }
     sc->tune2l[rayClass] = rc == 0 ? best : keep;
     if (numResults) *numResults = k;
     hipEventDestroy(e0);

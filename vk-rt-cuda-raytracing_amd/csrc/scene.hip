@@ -173,14 +173,55 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
     }
 }
 
+// Batch schedule: each wave takes 64 rays from the device counter and each
+// lane runs its ray's whole traversal (trace2l_ray) with an LDS stack.
+template <bool ANY, bool ORD, int LDSD, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace2l_batch(Trace2LArgs a) {
+    __shared__ int s_ref[LDSD * BLOCK];
+    __shared__ float s_tn[LDSD * BLOCK];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int gtid = blockIdx.x * BLOCK + tid;
+    if (a.gate && *a.gate <= a.gateMax) return;
+    const int n = a.nDev ? min(*a.nDev, a.n) : a.n;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(a.counter, 64u);
+        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+        if (base >= (uint32_t)n) break;
+        const int ri = (int)base + lane;
+        if (ri >= n) continue;
+        Ray r;
+        r.ox = a.ox[ri]; r.oy = a.oy[ri]; r.oz = a.oz[ri];
+        r.dx = a.dx[ri]; r.dy = a.dy[ri]; r.dz = a.dz[ri];
+        r.tmin = a.tmin[ri]; r.tmax = a.tmax[ri];
+        rtd::ray_setup(r);
+        LdsStack<LDSD, BLOCK> st{s_ref, s_tn, a.spill, (size_t)a.spillStride, tid, gtid, a.maxStack, 0, a.status};
+        float t, u, v;
+        int inst, id;
+        const bool hit = trace2l_ray<ANY, ORD>(a.tlasNodes, a.tlasTris, a.inst, a.blas, r, st, t, inst, id, u, v);
+        if constexpr (ANY) {
+            a.occ[ri] = hit ? 1 : 0;
+        } else {
+            a.t[ri] = hit ? t : RT_MISS_T;
+            if (a.inst_out) a.inst_out[ri] = hit ? inst : -1;
+            if (a.id) a.id[ri] = hit ? id : -1;
+            if (a.u) a.u[ri] = hit ? u : 0.f;
+            if (a.v) a.v[ri] = hit ? v : 0.f;
+        }
+    }
+}
+
 constexpr int kBlock2 = 256;
 
 __global__ void k_reset_counter_2l(uint32_t* c) { c[0] = 0u; }
 
-template <bool ANY, int LDSD>
+template <bool ANY, int LDSD, int SCHED>
 static int launch2(Trace2LArgs& a, rt::TraceScratch& sc, int maxStack, int blocksPerCU, hipStream_t s) {
     static int maxBlocks = 0;
-    auto kern = k_trace2l<ANY, LDSD, kBlock2>;
+    // SCHED: 0 while-while, 1 batch, 2 batch with nearest-first any hit
+    auto kern = SCHED == 0 ? k_trace2l<ANY, LDSD, kBlock2>
+                           : (SCHED == 1 ? k_trace2l_batch<ANY, false, LDSD, kBlock2>
+                                         : k_trace2l_batch<ANY, true, LDSD, kBlock2>);
     if (!maxBlocks) {
         int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock2, 0) != hipSuccess || nb <= 0) nb = 1;
@@ -224,16 +265,21 @@ int launch_trace2l(const CudaSceneHandle* sc, rt::TraceScratch& scratch, Trace2L
     a.blas = sc->d_blas;
     const int maxStack = trace2l_max_stack(sc);
     const rt::TraceConfig& c = sc->tune2l[any ? 1 : 0];
+    // counted traces always use the while-while kernel (it keeps the counters)
+    const int sched = (a.cntNodes || c.schedule < 0 || c.schedule > 2 || (!any && c.schedule == 2)) ? 0 : c.schedule;
+#define RT_L2(D)                                                                                                  \
+    if (sched == 0) return any ? launch2<true, D, 0>(a, scratch, maxStack, c.blocksPerCU, s)                     \
+                               : launch2<false, D, 0>(a, scratch, maxStack, c.blocksPerCU, s);                   \
+    if (sched == 1) return any ? launch2<true, D, 1>(a, scratch, maxStack, c.blocksPerCU, s)                     \
+                               : launch2<false, D, 1>(a, scratch, maxStack, c.blocksPerCU, s);                   \
+    return launch2<true, D, 2>(a, scratch, maxStack, c.blocksPerCU, s);
     switch (c.ldsDepth) {
-        case 4: return any ? launch2<true, 4>(a, scratch, maxStack, c.blocksPerCU, s)
-                           : launch2<false, 4>(a, scratch, maxStack, c.blocksPerCU, s);
-        case 12: return any ? launch2<true, 12>(a, scratch, maxStack, c.blocksPerCU, s)
-                            : launch2<false, 12>(a, scratch, maxStack, c.blocksPerCU, s);
-        case 16: return any ? launch2<true, 16>(a, scratch, maxStack, c.blocksPerCU, s)
-                            : launch2<false, 16>(a, scratch, maxStack, c.blocksPerCU, s);
-        default: return any ? launch2<true, 8>(a, scratch, maxStack, c.blocksPerCU, s)
-                            : launch2<false, 8>(a, scratch, maxStack, c.blocksPerCU, s);
+        case 4: { RT_L2(4) }
+        case 12: { RT_L2(12) }
+        case 16: { RT_L2(16) }
+        default: { RT_L2(8) }
     }
+#undef RT_L2
 }
 
 }  // namespace rtk

@@ -75,35 +75,81 @@ __device__ __forceinline__ void visit2(const uint4* __restrict__ nodes, const Ra
 }
 
 
-// One ray through TLAS and BLAS with a stack in global memory (entries
-// stk[k * stride], k < maxStack): the traversal of k_trace2l for a single
-// ray (same node order, culling and tie rule), for kernels that give each
-// thread its own ray. Closest hit: returns whether something was hit and
-// (t, inst, id); any hit: whether the ray is occluded.
-template <bool ANY>
-__device__ __forceinline__ bool trace2l_one(const uint4* tlasNodes, const float4* tlasTris, const InstanceDev* insts,
-                                            const BlasDev* blas, const Ray& rw, int2* stk, int stride, int maxStack,
-                                            float& bt, int& binst, int& bid) {
+// Stack in global memory: entries base[k * stride], k < max.
+struct GlobalStack {
+    int2* base;
+    int stride, max, sp;
+    __device__ __forceinline__ void push(int rf, float t) {
+        if (sp < max) { base[(size_t)sp * stride] = make_int2(rf, __float_as_int(t)); sp++; }
+    }
+    __device__ __forceinline__ bool pop(int& rf, float& t) {
+        if (sp == 0) return false;
+        sp--;
+        const int2 e = base[(size_t)sp * stride];
+        rf = e.x; t = __int_as_float(e.y);
+        return true;
+    }
+};
+
+// Stack in LDS ([LDSD][BLOCK]) spilling to global memory beyond LDSD entries.
+template <int LDSD, int BLOCK>
+struct LdsStack {
+    int* sref;
+    float* stn;
+    int2* spill;
+    size_t stride;
+    int tid, gtid, max, sp;
+    int* status;
+    __device__ __forceinline__ void push(int rf, float t) {
+        if (sp < LDSD) {
+            sref[sp * BLOCK + tid] = rf;
+            stn[sp * BLOCK + tid] = t;
+        } else if (sp < max) {
+            spill[(size_t)(sp - LDSD) * stride + gtid] = make_int2(rf, __float_as_int(t));
+        } else {
+            atomicOr(status, 1);   // unreachable: max bounds a DFS of the two levels
+            return;
+        }
+        sp++;
+    }
+    __device__ __forceinline__ bool pop(int& rf, float& t) {
+        if (sp == 0) return false;
+        sp--;
+        if (sp < LDSD) {
+            rf = sref[sp * BLOCK + tid];
+            t = stn[sp * BLOCK + tid];
+        } else {
+            const int2 e = spill[(size_t)(sp - LDSD) * stride + gtid];
+            rf = e.x; t = __int_as_float(e.y);
+        }
+        return true;
+    }
+};
+
+// One ray through TLAS and BLAS: the traversal of k_trace2l for a single ray
+// (same culling and tie rule; any hit may visit children nearest first,
+// ORD). Closest hit: returns whether something was hit and (t, inst, id, u,
+// v); any hit: whether the ray is occluded.
+template <bool ANY, bool ORD, class Stack>
+__device__ __forceinline__ bool trace2l_ray(const uint4* tlasNodes, const float4* tlasTris, const InstanceDev* insts,
+                                            const BlasDev* blas, const Ray& rw, Stack& st, float& bt, int& binst,
+                                            int& bid, float& bu, float& bv) {
     Ray ro;
-    int sp = 0, ref = 0, inst = -1;
+    int ref = 0, inst = -1;
     float tn = -INFINITY, best = rw.tmax;
     const uint4* nodes = tlasNodes;
     const float4* tris = nullptr;
-    bid = -1; binst = -1; bt = rw.tmax;
-    auto push = [&](int rf, float t) {
-        if (sp < maxStack) { stk[(size_t)sp * stride] = make_int2(rf, __float_as_int(t)); sp++; }
-    };
+    bid = -1; binst = -1; bt = rw.tmax; bu = 0.f; bv = 0.f;
+    auto push = [&](int rf, float t) { st.push(rf, t); };
     for (;;) {
         while (ref == -1 || ref == kReturnMark || (!ANY && tn > best)) {
             if (ref == kReturnMark) { inst = -1; nodes = tlasNodes; ref = -1; continue; }
-            if (sp == 0) return ANY ? false : bid >= 0;
-            sp--;
-            const int2 e = stk[(size_t)sp * stride];
-            ref = e.x; tn = __int_as_float(e.y);
+            if (!st.pop(ref, tn)) return ANY ? false : bid >= 0;
         }
         if (ref >= 0) {
             const Ray& r = inst < 0 ? rw : ro;
-            visit2<ANY>(nodes, r, ANY ? r.tmax : best, ref, tn, push);
+            if constexpr (ANY && ORD) visit2<false>(nodes, r, r.tmax, ref, tn, push);
+            else visit2<ANY>(nodes, r, ANY ? r.tmax : best, ref, tn, push);
         } else if (inst < 0) {
             const int enc = -(ref + 2);
             const int k = __float_as_int(tlasTris[3 * (enc >> 3)].w);
@@ -129,13 +175,23 @@ __device__ __forceinline__ bool trace2l_one(const uint4* tlasNodes, const float4
                 } else {
                     const int id = __float_as_int(p0.w);
                     if (t < best || (t == best && (inst < binst || (inst == binst && id < bid)))) {
-                        best = t; bid = id; binst = inst; bt = t;
+                        best = t; bid = id; binst = inst; bt = t; bu = u; bv = v;
                     }
                 }
             }
             ref = -1;
         }
     }
+}
+
+// Global-stack form used by the path tracer's tail kernel.
+template <bool ANY>
+__device__ __forceinline__ bool trace2l_one(const uint4* tlasNodes, const float4* tlasTris, const InstanceDev* insts,
+                                            const BlasDev* blas, const Ray& rw, int2* stk, int stride, int maxStack,
+                                            float& bt, int& binst, int& bid) {
+    GlobalStack st{stk, stride, maxStack, 0};
+    float bu, bv;
+    return trace2l_ray<ANY, false>(tlasNodes, tlasTris, insts, blas, rw, st, bt, binst, bid, bu, bv);
 }
 
 }  // namespace rtk
