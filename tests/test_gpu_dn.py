@@ -135,3 +135,23 @@ def test_adam_update_and_training(pkg, dn_ref, dev):
     assert np.allclose(got, ref, rtol=2 ** -7, atol=1e-6)
     assert np.all(pk[:, 10:] == 0)
     d.destroy()
+
+
+def test_forward_large_tiles(pkg, dn_ref, dev):
+    """Images of 65536 pixels and more use 256-pixel convolution tiles: the
+    output stays within BF16 drift of the oracle (a tiling error would not)."""
+    import torch
+    d = pkg.Denoiser(1 << 17, seed=1)
+    params = dn_ref.init_like_gpu_shapes(seed=4)
+    d.set_params(params)
+    B, H, W = 1, 256, 256
+    bits, xv, _ = _inputs(6, B, H, W)
+    X = torch.from_numpy(bits.view(np.int16)).to(dev)
+    head = torch.empty((B, H, W, 3), device=dev)
+    d.forward(B, H, W, X, head)
+    torch.cuda.synchronize()
+    h_emu, _ = dn_ref.forward(params, xv, emulate=True)
+    err = _rel(head.cpu().numpy(), h_emu)
+    print("256x256 head vs emulated", err, flush=True)
+    assert err < 3e-2
+    d.destroy()

@@ -55,20 +55,20 @@ struct ConvArgs {
     int B, H, W, relu;
 };
 
-// Block: 4 waves x 32 pixels = 128 pixels; each wave 32 pixels x (16*WN)
-// channels (2 x WN MFMA tiles). A operand: the 8 (or 4) consecutive channels
+// Block: 4 waves x (16*WM) pixels; each wave (16*WM) pixels x (16*WN)
+// channels (WM x WN MFMA tiles). A operand: the 8 (or 4) consecutive channels
 // of one neighbour pixel per lane; B operand: 8 (or 4) consecutive K of one
 // weight row per lane.
-template <int TAPS, int KS, int WN>
+template <int TAPS, int KS, int WN, int WM>
 __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int P = a.B * a.H * a.W;
-    const int m0 = blockIdx.x * 128 + wave * 32;
+    const int m0 = blockIdx.x * (64 * WM) + wave * (16 * WM);
     const int n0 = blockIdx.y * (16 * WN);
     const int row = lane & 15, kg = lane >> 4;
-    int pb[2], py[2], px[2];
-    bool pv[2];
-    for (int r = 0; r < 2; r++) {
+    int pb[WM], py[WM], px[WM];
+    bool pv[WM];
+    for (int r = 0; r < WM; r++) {
         const int p = m0 + r * 16 + row;
         pv[r] = p < P;
         const int pp = pv[r] ? p : 0;
@@ -77,22 +77,22 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
         py[r] = q % a.H;
         pb[r] = q / a.H;
     }
-    f32x4 acc[2][WN];
-    for (int r = 0; r < 2; r++)
+    f32x4 acc[WM][WN];
+    for (int r = 0; r < WM; r++)
         for (int j = 0; j < WN; j++) acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int K = TAPS * a.cin;
     for (int t = 0; t < TAPS; t++) {
         const int dy = TAPS == 9 ? t / 3 - 1 : 0, dx = TAPS == 9 ? t % 3 - 1 : 0;
-        const uint16_t* src[2];
-        for (int r = 0; r < 2; r++) {
+        const uint16_t* src[WM];
+        for (int r = 0; r < WM; r++) {
             const int yy = py[r] + dy, xx = px[r] + dx;
             const bool ok = pv[r] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
             src[r] = ok ? a.x + ((size_t)(pb[r] * a.H + yy) * a.W + xx) * a.ldx : nullptr;
         }
         for (int c0 = 0; c0 < a.cin; c0 += KS) {
             if constexpr (KS == 32) {
-                bf16x8 av[2], bv[WN];
-                for (int r = 0; r < 2; r++) {
+                bf16x8 av[WM], bv[WN];
+                for (int r = 0; r < WM; r++) {
                     uint4 v = make_uint4(0, 0, 0, 0);
                     if (src[r]) v = *(const uint4*)(src[r] + c0 + 8 * kg);
                     av[r] = __builtin_bit_cast(bf16x8, v);
@@ -101,12 +101,12 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
                     const uint4 v = *(const uint4*)(a.w + (size_t)(n0 + j * 16 + row) * K + t * a.cin + c0 + 8 * kg);
                     bv[j] = __builtin_bit_cast(bf16x8, v);
                 }
-                for (int r = 0; r < 2; r++)
+                for (int r = 0; r < WM; r++)
                     for (int j = 0; j < WN; j++)
                         acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], bv[j], acc[r][j], 0, 0, 0);
             } else {
-                s16x4 av[2], bv[WN];
-                for (int r = 0; r < 2; r++) {
+                s16x4 av[WM], bv[WN];
+                for (int r = 0; r < WM; r++) {
                     uint2 v = make_uint2(0, 0);
                     if (src[r]) v = *(const uint2*)(src[r] + c0 + 4 * kg);
                     av[r] = __builtin_bit_cast(s16x4, v);
@@ -115,14 +115,14 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
                     const uint2 v = *(const uint2*)(a.w + (size_t)(n0 + j * 16 + row) * K + t * a.cin + c0 + 4 * kg);
                     bv[j] = __builtin_bit_cast(s16x4, v);
                 }
-                for (int r = 0; r < 2; r++)
+                for (int r = 0; r < WM; r++)
                     for (int j = 0; j < WN; j++)
                         acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[r], bv[j], acc[r][j], 0, 0, 0);
             }
         }
     }
     // D[i][j]: lane j + 16*(i/4), item i%4
-    for (int r = 0; r < 2; r++)
+    for (int r = 0; r < WM; r++)
         for (int j = 0; j < WN; j++) {
             const int col = n0 + j * 16 + row;
             if (col >= a.n) continue;
@@ -470,24 +470,31 @@ struct Net {
 
 static int grid_for(size_t n) { return (int)std::min<size_t>((n + kBlock - 1) / kBlock, 8192); }
 
+// Pixels per block: 256 (4 row groups per wave) on large images, 128 below.
+template <int TAPS, int KS, int WN>
+static void launch_conv(const ConvArgs& a, int P, int n, hipStream_t s) {
+    if (P >= (1 << 16)) {
+        dim3 grid((P + 255) / 256, (n + 16 * WN - 1) / (16 * WN));
+        hipLaunchKernelGGL((k_conv<TAPS, KS, WN, 4>), grid, dim3(kBlock), 0, s, a);
+    } else {
+        dim3 grid((P + 127) / 128, (n + 16 * WN - 1) / (16 * WN));
+        hipLaunchKernelGGL((k_conv<TAPS, KS, WN, 2>), grid, dim3(kBlock), 0, s, a);
+    }
+}
+
 static void conv_fwd(Net& N, int li, const uint16_t* x, int ldx, uint16_t* y, int ldy, int B, int H, int W, bool relu,
                      float* yf = nullptr, int ldyf = 0) {
     const Layer& L = N.L[li];
     ConvArgs a{x, ldx, L.d.cin, L.wf, N.w + L.bOff, y, ldy, yf, ldyf, L.d.cout, B, H, W, relu ? 1 : 0};
     const int P = B * H * W;
     const int WN = L.d.cout >= 64 ? 4 : (L.d.cout >= 32 ? 2 : 1);
-    dim3 grid((P + 127) / 128, (L.d.cout + 16 * WN - 1) / (16 * WN));
     hipStream_t s = N.stream;
-    if (L.d.taps == 1) {
-        hipLaunchKernelGGL((k_conv<1, 32, 1>), grid, dim3(kBlock), 0, s, a);
-    } else if (L.d.cin % 32) {
-        if (WN == 2) hipLaunchKernelGGL((k_conv<9, 16, 2>), grid, dim3(kBlock), 0, s, a);
-        else hipLaunchKernelGGL((k_conv<9, 16, 4>), grid, dim3(kBlock), 0, s, a);
-    } else if (WN == 4) {
-        hipLaunchKernelGGL((k_conv<9, 32, 4>), grid, dim3(kBlock), 0, s, a);
-    } else {
-        hipLaunchKernelGGL((k_conv<9, 32, 2>), grid, dim3(kBlock), 0, s, a);
-    }
+    if (L.d.taps == 1) launch_conv<1, 32, 1>(a, P, L.d.cout, s);
+    else if (L.d.cin % 32) {
+        if (WN == 2) launch_conv<9, 16, 2>(a, P, L.d.cout, s);
+        else launch_conv<9, 16, 4>(a, P, L.d.cout, s);
+    } else if (WN == 4) launch_conv<9, 32, 4>(a, P, L.d.cout, s);
+    else launch_conv<9, 32, 2>(a, P, L.d.cout, s);
 }
 
 // input gradient: dx = conv(g, flipped transposed weights)
@@ -496,15 +503,10 @@ static void conv_dx(Net& N, int li, const uint16_t* g, int ldg, uint16_t* dx, in
     ConvArgs a{g, ldg, L.coutK, L.wb, nullptr, dx, lddx, nullptr, 0, L.d.cin, B, H, W, 0};
     const int P = B * H * W;
     const int WN = L.d.cin >= 64 ? 4 : 2;
-    dim3 grid((P + 127) / 128, (L.d.cin + 16 * WN - 1) / (16 * WN));
     hipStream_t s = N.stream;
-    if (L.d.taps == 1) {
-        hipLaunchKernelGGL((k_conv<1, 16, 2>), grid, dim3(kBlock), 0, s, a);
-    } else if (WN == 4) {
-        hipLaunchKernelGGL((k_conv<9, 32, 4>), grid, dim3(kBlock), 0, s, a);
-    } else {
-        hipLaunchKernelGGL((k_conv<9, 32, 2>), grid, dim3(kBlock), 0, s, a);
-    }
+    if (L.d.taps == 1) launch_conv<1, 16, 2>(a, P, L.d.cin, s);
+    else if (WN == 4) launch_conv<9, 32, 4>(a, P, L.d.cin, s);
+    else launch_conv<9, 32, 2>(a, P, L.d.cin, s);
 }
 
 static void conv_dw(Net& N, int li, const uint16_t* g, int ldg, const uint16_t* x, int ldx, int B, int H, int W) {
