@@ -53,7 +53,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the many-scenes leg")
     ap.add_argument("--no-denoise", action="store_true", help="skip the denoiser leg")
     ap.add_argument("--batch-scenes", type=int, default=256)
-    ap.add_argument("--ref-spp", type=int, default=1024, help="reference spp of the denoiser evaluation")
+    ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the denoiser evaluation")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -170,6 +170,34 @@ def main():
                   "mrays_s_incl_sort": round(m / (srt_ms * 1e3), 1)}
     log(f"incoherent with ray sort: {srt_ms:.3f} ms (sort+trace) vs {uns_ms:.3f} ms unsorted")
 
+    # The compressed 8-wide BVH on the same primary and diffuse rays (reported).
+    cw_cls = {}
+    with torch.cuda.stream(stream):
+        t0 = time.time()
+        ncw = bvh.build_cwbvh()
+        cw_build_s = time.time() - t0
+        Pp = {k: torch.empty(n, device=dev) for k in RAY_KEYS}
+        pkg.gen_pinhole_rays(W, H, c, Pp, stream.cuda_stream)
+        Dm = {k: v[:m] for k, v in D.items()}
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        for _ in range(2):
+            bvh.trace_closest(Pp, pt, pid, pu, pv, flags=pkg.FLAG_CWBVH)
+            bvh.trace_closest(Dm, dt[:m], did[:m], du[:m], dv[:m], flags=pkg.FLAG_CWBVH)
+        e[0].record(stream)
+        for _ in range(5):
+            bvh.trace_closest(Pp, pt, pid, pu, pv, flags=pkg.FLAG_CWBVH)
+        e[1].record(stream)
+        for _ in range(5):
+            bvh.trace_closest(Dm, dt[:m], did[:m], du[:m], dv[:m], flags=pkg.FLAG_CWBVH)
+        e[2].record(stream)
+    torch.cuda.synchronize()
+    cwp, cwd = e[0].elapsed_time(e[1]) / 5, e[1].elapsed_time(e[2]) / 5
+    cw_cls = {"nodes": ncw, "build_s": round(cw_build_s, 2),
+              "coherent_closest_ms": round(cwp, 4), "coherent_mrays_s": round(n / (cwp * 1e3), 1),
+              "incoherent_closest_ms": round(cwd, 4), "incoherent_mrays_s": round(m / (cwd * 1e3), 1)}
+    log(f"CWBVH ({ncw} nodes, {cw_build_s:.2f} s): primary {cwp:.3f} ms, diffuse {cwd:.3f} ms")
+    del Pp
+
     # Algorithmic bytes of the dominant kernel (incoherent closest hit) from
     # exact visit counts; the counted kernel's counts equal the oracle's.
     with torch.cuda.stream(stream):
@@ -225,6 +253,7 @@ def main():
                                    "tris_per_ray": round(d_tris / max(m, 1), 2)},
             "bounce_generation": {"ms": round(bounce_ms, 4)},
             "incoherent_closest_sorted": sorted_cls,
+            "cwbvh": cw_cls,
             "shadow_any": {"rays": m, "ms": round(shadow_ms, 4), "mrays_s": round(m / (shadow_ms * 1e3), 1),
                            "occluded": round(occ_rate, 4)},
         },
