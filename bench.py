@@ -277,7 +277,7 @@ def main():
     if not args.no_batch:
         out["batched_scenes"] = bench_batched(pkg, scenes, dev, args)
     if not args.no_denoise:
-        out["denoiser"] = bench_denoiser(pkg, scenes, dev, args)
+        out["denoiser"] = bench_denoiser(pkg, scenes, dev, args, out.get("path_tracer", {}).get("tuning", {}))
     print(json.dumps(out), flush=True)
 
 
@@ -443,7 +443,7 @@ def denoiser_gflop(W, H):
     return 2.0 * macs / 1e9
 
 
-def bench_denoiser(pkg, scenes, dev, args):
+def bench_denoiser(pkg, scenes, dev, args, pt_tuning=None):
     """The learned denoiser at 1920x1080: inference time and TFLOP/s (BF16
     MFMA), and the equal-time quality gain over plain sample accumulation
     against a high-spp reference of the held-out bench view (PSNR of x/(1+x))."""
@@ -458,6 +458,11 @@ def bench_denoiser(pkg, scenes, dev, args):
         dn.set_params(dt.load_weights(wpath))
     sc = scenes.instanced_scene(seed=2)
     r = dt.Renderer(sc, dev)
+    sched = {"while-while": 0, "batch": 1, "batch-nearest-first": 2}
+    for cls, key in ((0, "closest"), (1, "shadow_any")):      # the path-tracer leg's tuned launches
+        tcfg = (pt_tuning or {}).get(key)
+        if tcfg:
+            r.scene.set_trace_config(cls, sched[tcfg["schedule"]], tcfg["lds_depth"], tcfg["blocks_per_cu"])
     cam = scenes.instanced_camera(W, H)
     noisy, _ = r.render(cam, W, H, 1, 5, flags=pkg.FLAG_GRAPH)
     out = torch.empty(3 * n, device=dev)
