@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--no-denoise", action="store_true", help="skip the denoiser leg")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--ref-spp", type=int, default=4096, help="reference spp of the denoiser evaluation")
+    ap.add_argument("--pmc-frame", action="store_true",
+                    help="run two frames with the tuned launch configurations between two plain 512 MiB device "
+                         "copies and exit (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -106,6 +109,10 @@ def main():
         if ev: ev[3].record(stream)
         bvh.trace_any_queued(n, cnt, S, occ)
         if ev: ev[4].record(stream)
+
+    if args.pmc_frame:
+        pmc_frame(pkg, bvh, step, dev, cnt)
+        return
 
     for _ in range(args.warmup):
         step()
@@ -510,6 +517,33 @@ def bench_denoiser(pkg, scenes, dev, args, pt_tuning=None):
     r.destroy()
     dn.destroy()
     return res
+
+
+# Launch configurations the autotuner picks on the 1080p terrain frame on
+# MI355X (schedule, LDS stack depth, workgroups per CU; 0 = as many as fit).
+PMC_FRAME_CONFIG = (("CLASS_COHERENT", "SCHEDULE_BATCH", 12, 0),
+                    ("CLASS_INCOHERENT", "SCHEDULE_WHILE_WHILE", 12, 0),
+                    ("CLASS_ANY", "SCHEDULE_BATCH", 12, 4))
+
+
+def pmc_frame(pkg, bvh, step, dev, cnt):
+    """Two frames of the benchmark between two plain device copies of
+    512 MiB. The copies calibrate FETCH_SIZE / WRITE_SIZE (a copy of B bytes
+    reads B and writes B); the frames give the per-dispatch counters of the
+    traversal kernels with the launch configurations the benchmark times."""
+    import torch
+    for cls, sched, depth, bpc in PMC_FRAME_CONFIG:
+        bvh.set_trace_config(getattr(pkg, cls), getattr(pkg, sched), depth, bpc)
+    x = torch.ones(128 << 20, device=dev)
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"pmc frame: {bvh.num_bvh4_nodes} nodes, {int(cnt.item())} secondary rays, copy {x.numel() * 4} bytes")
 
 
 def load_traffic():

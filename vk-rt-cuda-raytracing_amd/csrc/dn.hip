@@ -68,6 +68,7 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
     const int row = lane & 15, kg = lane >> 4;
     int pb[WM], py[WM], px[WM];
     bool pv[WM];
+    #pragma unroll
     for (int r = 0; r < WM; r++) {
         const int p = m0 + r * 16 + row;
         pv[r] = p < P;
@@ -78,12 +79,15 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
         pb[r] = q / a.H;
     }
     f32x4 acc[WM][WN];
+    #pragma unroll
     for (int r = 0; r < WM; r++)
+        #pragma unroll
         for (int j = 0; j < WN; j++) acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int K = TAPS * a.cin;
     for (int t = 0; t < TAPS; t++) {
         const int dy = TAPS == 9 ? t / 3 - 1 : 0, dx = TAPS == 9 ? t % 3 - 1 : 0;
         const uint16_t* src[WM];
+        #pragma unroll
         for (int r = 0; r < WM; r++) {
             const int yy = py[r] + dy, xx = px[r] + dx;
             const bool ok = pv[r] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
@@ -92,41 +96,52 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
         for (int c0 = 0; c0 < a.cin; c0 += KS) {
             if constexpr (KS == 32) {
                 bf16x8 av[WM], bv[WN];
+                #pragma unroll
                 for (int r = 0; r < WM; r++) {
                     uint4 v = make_uint4(0, 0, 0, 0);
                     if (src[r]) v = *(const uint4*)(src[r] + c0 + 8 * kg);
                     av[r] = __builtin_bit_cast(bf16x8, v);
                 }
+                #pragma unroll
                 for (int j = 0; j < WN; j++) {
                     const uint4 v = *(const uint4*)(a.w + (size_t)(n0 + j * 16 + row) * K + t * a.cin + c0 + 8 * kg);
                     bv[j] = __builtin_bit_cast(bf16x8, v);
                 }
+                #pragma unroll
                 for (int r = 0; r < WM; r++)
+                    #pragma unroll
                     for (int j = 0; j < WN; j++)
                         acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], bv[j], acc[r][j], 0, 0, 0);
             } else {
                 s16x4 av[WM], bv[WN];
+                #pragma unroll
                 for (int r = 0; r < WM; r++) {
                     uint2 v = make_uint2(0, 0);
                     if (src[r]) v = *(const uint2*)(src[r] + c0 + 4 * kg);
                     av[r] = __builtin_bit_cast(s16x4, v);
                 }
+                #pragma unroll
                 for (int j = 0; j < WN; j++) {
                     const uint2 v = *(const uint2*)(a.w + (size_t)(n0 + j * 16 + row) * K + t * a.cin + c0 + 4 * kg);
                     bv[j] = __builtin_bit_cast(s16x4, v);
                 }
+                #pragma unroll
                 for (int r = 0; r < WM; r++)
+                    #pragma unroll
                     for (int j = 0; j < WN; j++)
                         acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[r], bv[j], acc[r][j], 0, 0, 0);
             }
         }
     }
     // D[i][j]: lane j + 16*(i/4), item i%4
+    #pragma unroll
     for (int r = 0; r < WM; r++)
+        #pragma unroll
         for (int j = 0; j < WN; j++) {
             const int col = n0 + j * 16 + row;
             if (col >= a.n) continue;
             const float b = a.bias ? a.bias[col] : 0.0f;
+            #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int p = m0 + r * 16 + 4 * kg + i;
                 if (p >= P) continue;

@@ -117,6 +117,17 @@ __device__ __forceinline__ float h_hi(uint32_t w) {
     return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
 }
 
+// RN(h - o) for the FP16 value h in the low (HI = false) or high half of w,
+// in one v_fma_mix_f32: h * 1.0 - o with h widened exactly and one rounding,
+// which is the value of (float)h - o. Saves the separate conversion.
+template <bool HI>
+__device__ __forceinline__ float h_sub(uint32_t w, float o) {
+    float r;
+    if constexpr (HI) asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(w), "v"(o));
+    else asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(w), "v"(o));
+    return r;
+}
+
 // Slab test of child slot C of a 64-byte BVH4 node (qx, qy, qz hold the
 // x/y/z halves: mins in slots 0-3, maxes in 4-7). Robust far distance. FLAT
 // (chosen per wave when any lane's ray is flat) resolves the 0*inf NaN so
@@ -129,12 +140,10 @@ __device__ __forceinline__ bool slab_child(const Ray& r, const uint4& qx, const 
     const uint32_t wxl = (C < 2) ? qx.x : qx.y, wxh = (C < 2) ? qx.z : qx.w;
     const uint32_t wyl = (C < 2) ? qy.x : qy.y, wyh = (C < 2) ? qy.z : qy.w;
     const uint32_t wzl = (C < 2) ? qz.x : qz.y, wzh = (C < 2) ? qz.z : qz.w;
-    float x0 = (C & 1) ? h_hi(wxl) : h_lo(wxl), x1 = (C & 1) ? h_hi(wxh) : h_lo(wxh);
-    float y0 = (C & 1) ? h_hi(wyl) : h_lo(wyl), y1 = (C & 1) ? h_hi(wyh) : h_lo(wyh);
-    float z0 = (C & 1) ? h_hi(wzl) : h_lo(wzl), z1 = (C & 1) ? h_hi(wzh) : h_lo(wzh);
-    float t1x = (x0 - r.ox) * r.ix, t2x = (x1 - r.ox) * r.ix;
-    float t1y = (y0 - r.oy) * r.iy, t2y = (y1 - r.oy) * r.iy;
-    float t1z = (z0 - r.oz) * r.iz, t2z = (z1 - r.oz) * r.iz;
+    constexpr bool HI = (C & 1) != 0;
+    float t1x = h_sub<HI>(wxl, r.ox) * r.ix, t2x = h_sub<HI>(wxh, r.ox) * r.ix;
+    float t1y = h_sub<HI>(wyl, r.oy) * r.iy, t2y = h_sub<HI>(wyh, r.oy) * r.iy;
+    float t1z = h_sub<HI>(wzl, r.oz) * r.iz, t2z = h_sub<HI>(wzh, r.oz) * r.iz;
     if constexpr (FLAT) {
         t1x = t1x != t1x ? -r.ix : t1x; t2x = t2x != t2x ? r.ix : t2x;
         t1y = t1y != t1y ? -r.iy : t1y; t2y = t2y != t2y ? r.iy : t2y;
@@ -144,6 +153,17 @@ __device__ __forceinline__ bool slab_child(const Ray& r, const uint4& qx, const 
     float tF = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z)) * RT_ROBUST_TFAR;
     tnear = tN;
     return (tN <= tF) && (tF >= r.tmin) && (tN <= limit);
+}
+
+// Slot C of a node: true, with the entry distance, when the slot holds a
+// child (ref != -1) whose box passes the slab test. The slab test runs for
+// empty slots too and its result is discarded, so the four slots of a node
+// are tested without branches.
+template <int C, bool FLAT>
+__device__ __forceinline__ bool child_hit(const Ray& r, const uint4& qx, const uint4& qy, const uint4& qz, int ref,
+                                          float limit, float& tnear) {
+    const bool h = slab_child<C, FLAT>(r, qx, qy, qz, limit, tnear);
+    return h & (ref != -1);
 }
 
 // Pinhole ray for pixel (px, py) of a W x H image; the same sequence as the

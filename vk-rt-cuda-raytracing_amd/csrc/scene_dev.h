@@ -35,10 +35,10 @@ __device__ __forceinline__ void visit2(const uint4* __restrict__ nodes, const Ra
     const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
     if constexpr (ANY) {
         float d0, d1, d2, d3;
-        const bool h0 = (int)qc.x != -1 && rtd::slab_child<0, true>(r, qx, qy, qz, limit, d0);
-        const bool h1 = (int)qc.y != -1 && rtd::slab_child<1, true>(r, qx, qy, qz, limit, d1);
-        const bool h2 = (int)qc.z != -1 && rtd::slab_child<2, true>(r, qx, qy, qz, limit, d2);
-        const bool h3 = (int)qc.w != -1 && rtd::slab_child<3, true>(r, qx, qy, qz, limit, d3);
+        const bool h0 = rtd::child_hit<0, true>(r, qx, qy, qz, (int)qc.x, limit, d0);
+        const bool h1 = rtd::child_hit<1, true>(r, qx, qy, qz, (int)qc.y, limit, d1);
+        const bool h2 = rtd::child_hit<2, true>(r, qx, qy, qz, (int)qc.z, limit, d2);
+        const bool h3 = rtd::child_hit<3, true>(r, qx, qy, qz, (int)qc.w, limit, d3);
         int nxt = -1;
         if (h0) { nxt = (int)qc.x; }
         if (h1) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.y; }
@@ -50,10 +50,10 @@ __device__ __forceinline__ void visit2(const uint4* __restrict__ nodes, const Ra
         float d[4];
         int c[4];
         float tt;
-        if ((int)qc.x != -1 && rtd::slab_child<0, true>(r, qx, qy, qz, limit, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
-        if ((int)qc.y != -1 && rtd::slab_child<1, true>(r, qx, qy, qz, limit, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
-        if ((int)qc.z != -1 && rtd::slab_child<2, true>(r, qx, qy, qz, limit, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
-        if ((int)qc.w != -1 && rtd::slab_child<3, true>(r, qx, qy, qz, limit, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
+        if (rtd::child_hit<0, true>(r, qx, qy, qz, (int)qc.x, limit, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
+        if (rtd::child_hit<1, true>(r, qx, qy, qz, (int)qc.y, limit, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
+        if (rtd::child_hit<2, true>(r, qx, qy, qz, (int)qc.z, limit, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
+        if (rtd::child_hit<3, true>(r, qx, qy, qz, (int)qc.w, limit, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
 #define RT_CSWAP(i, j)                                                                  \
     {                                                                                   \
         const bool s = d[i] > d[j];                                                     \

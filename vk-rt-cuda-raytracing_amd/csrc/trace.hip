@@ -78,10 +78,10 @@ __device__ __forceinline__ void visit_closest(const uint4* __restrict__ nodes, c
     float d[4];
     int c[4];
     float tt;
-    if ((int)qc.x != -1 && rtd::slab_child<0, FLAT>(r, qx, qy, qz, best, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
-    if ((int)qc.y != -1 && rtd::slab_child<1, FLAT>(r, qx, qy, qz, best, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
-    if ((int)qc.z != -1 && rtd::slab_child<2, FLAT>(r, qx, qy, qz, best, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
-    if ((int)qc.w != -1 && rtd::slab_child<3, FLAT>(r, qx, qy, qz, best, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
+    if (rtd::child_hit<0, FLAT>(r, qx, qy, qz, (int)qc.x, best, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
+    if (rtd::child_hit<1, FLAT>(r, qx, qy, qz, (int)qc.y, best, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
+    if (rtd::child_hit<2, FLAT>(r, qx, qy, qz, (int)qc.z, best, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
+    if (rtd::child_hit<3, FLAT>(r, qx, qy, qz, (int)qc.w, best, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
 #define RT_CSWAP(i, j)                                                                  \
     {                                                                                   \
         const bool s = d[i] > d[j];                                                     \
@@ -108,10 +108,10 @@ __device__ __forceinline__ void visit_any(const uint4* __restrict__ nodes, const
     const uint4* np = nodes + 4 * (size_t)ref;
     const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
     float d0, d1, d2, d3;
-    const bool h0 = (int)qc.x != -1 && rtd::slab_child<0, FLAT>(r, qx, qy, qz, r.tmax, d0);
-    const bool h1 = (int)qc.y != -1 && rtd::slab_child<1, FLAT>(r, qx, qy, qz, r.tmax, d1);
-    const bool h2 = (int)qc.z != -1 && rtd::slab_child<2, FLAT>(r, qx, qy, qz, r.tmax, d2);
-    const bool h3 = (int)qc.w != -1 && rtd::slab_child<3, FLAT>(r, qx, qy, qz, r.tmax, d3);
+    const bool h0 = rtd::child_hit<0, FLAT>(r, qx, qy, qz, (int)qc.x, r.tmax, d0);
+    const bool h1 = rtd::child_hit<1, FLAT>(r, qx, qy, qz, (int)qc.y, r.tmax, d1);
+    const bool h2 = rtd::child_hit<2, FLAT>(r, qx, qy, qz, (int)qc.z, r.tmax, d2);
+    const bool h3 = rtd::child_hit<3, FLAT>(r, qx, qy, qz, (int)qc.w, r.tmax, d3);
     int nxt = -1;
     if (h0) { nxt = (int)qc.x; }
     if (h1) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.y; }
@@ -185,10 +185,10 @@ __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int*
                 const uint4* np = nodes + 4 * (size_t)ref;
                 const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
                 float d0, d1, d2, d3;
-                const bool h0 = (int)qc.x != -1 && rtd::slab_child<0, FLAT>(r, qx, qy, qz, r.tmax, d0);
-                const bool h1 = (int)qc.y != -1 && rtd::slab_child<1, FLAT>(r, qx, qy, qz, r.tmax, d1);
-                const bool h2 = (int)qc.z != -1 && rtd::slab_child<2, FLAT>(r, qx, qy, qz, r.tmax, d2);
-                const bool h3 = (int)qc.w != -1 && rtd::slab_child<3, FLAT>(r, qx, qy, qz, r.tmax, d3);
+                const bool h0 = rtd::child_hit<0, FLAT>(r, qx, qy, qz, (int)qc.x, r.tmax, d0);
+                const bool h1 = rtd::child_hit<1, FLAT>(r, qx, qy, qz, (int)qc.y, r.tmax, d1);
+                const bool h2 = rtd::child_hit<2, FLAT>(r, qx, qy, qz, (int)qc.z, r.tmax, d2);
+                const bool h3 = rtd::child_hit<3, FLAT>(r, qx, qy, qz, (int)qc.w, r.tmax, d3);
                 // pushed in slot order; the last valid one is taken next instead of popped
                 int nxt = -1;
                 if (h0) { nxt = (int)qc.x; }
@@ -218,10 +218,10 @@ __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int*
                     float d[4];
                     int c[4];
                     float tt;
-                    if ((int)qc.x != -1 && rtd::slab_child<0, FLAT>(r, qx, qy, qz, best, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
-                    if ((int)qc.y != -1 && rtd::slab_child<1, FLAT>(r, qx, qy, qz, best, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
-                    if ((int)qc.z != -1 && rtd::slab_child<2, FLAT>(r, qx, qy, qz, best, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
-                    if ((int)qc.w != -1 && rtd::slab_child<3, FLAT>(r, qx, qy, qz, best, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
+                    if (rtd::child_hit<0, FLAT>(r, qx, qy, qz, (int)qc.x, best, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
+                    if (rtd::child_hit<1, FLAT>(r, qx, qy, qz, (int)qc.y, best, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
+                    if (rtd::child_hit<2, FLAT>(r, qx, qy, qz, (int)qc.z, best, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
+                    if (rtd::child_hit<3, FLAT>(r, qx, qy, qz, (int)qc.w, best, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
 #define RT_CSWAP(i, j)                                                                  \
     {                                                                                   \
         const bool s = d[i] > d[j];                                                     \

@@ -183,10 +183,36 @@ def train_and_save(dev, scene_seeds=range(100, 108), views: int = 4, windows: in
     return hist
 
 
+def time_inference(dev, frames: int = 3, W: int = 1920, H: int = 1080, path: str = WEIGHTS, log=print) -> float:
+    """Runs `frames` denoiser passes (pack, U-Net, unpack) on W x H planes of
+    uniform noise with the stored weights; returns the mean ms per frame."""
+    t = importlib.import_module("torch")
+    n = W * H
+    dn = pkg.Denoiser(((W + 7) // 8 * 8) * ((H + 7) // 8 * 8), seed=0)     # levels pad to multiples of 8
+    dn.set_params(load_weights(path))
+    g = t.Generator(device=dev)
+    g.manual_seed(0)
+    planes = [t.rand(c * n, device=dev, generator=g) for c in (3, 3, 3, 1)]
+    out = t.empty(3 * n, device=dev)
+    dn.denoise(W, H, *planes, out)
+    e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(frames):
+        dn.denoise(W, H, *planes, out)
+    e1.record()
+    t.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / frames
+    log(f"denoiser {W}x{H}: {ms:.3f} ms/frame over {frames} frames (mean output {float(out.mean()):.4f})")
+    dn.destroy()
+    return ms
+
+
 if __name__ == "__main__":
     import argparse
     import torch
     ap = argparse.ArgumentParser(description="render training pairs with the path tracer and train the denoiser")
+    ap.add_argument("--time-inference", type=int, default=0, metavar="FRAMES",
+                    help="only time FRAMES 1920x1080 denoiser passes with the stored weights")
     ap.add_argument("--scenes", type=int, default=8)
     ap.add_argument("--views", type=int, default=4)
     ap.add_argument("--windows", type=int, default=16)
@@ -194,5 +220,8 @@ if __name__ == "__main__":
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--out", default=WEIGHTS)
     a = ap.parse_args()
+    if a.time_inference:
+        time_inference(torch.device("cuda:0"), a.time_inference, log=lambda m: print(m, flush=True))
+        sys.exit(0)
     train_and_save(torch.device("cuda:0"), range(100, 100 + a.scenes), a.views, a.windows, a.target_spp, a.steps,
                    path=a.out, log=lambda m: print(m, flush=True))
