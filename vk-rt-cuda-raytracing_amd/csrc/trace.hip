@@ -139,12 +139,17 @@ __device__ __forceinline__ int ray_root(const TraceArgs& a, int ri) {
     return a.roots[lo];
 }
 
-// Batch schedule: traverses one ray in one loop (node and leaf steps share
-// iterations). Stack entries (node ref, entry distance) live in LDS
-// (slots [0, LDSD)) and spill to HBM beyond.
+// Batch schedule: traverses one ray in one loop. Each iteration is one short
+// step: one node visit, or one triangle of the current leaf (a leaf cursor
+// holds the next triangle and the number left), then a pop when the lane has
+// nothing to descend into. A lane at a leaf and a lane at a node thus run
+// steps of similar length side by side, instead of the node lanes waiting for
+// a whole leaf loop. Stack entries (node ref, entry distance) live in LDS
+// (slots [0, LDSD)) and spill to HBM beyond. Visit order and node / triangle
+// counters are those of a one-ray-at-a-time depth-first traversal.
 template <int MODE, bool MT, bool FLAT, int LDSD, int BLOCK>
 __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int* s_ref, float* s_tn, int tid,
-                                          int gtid, Result& res, int root) {
+                                             int gtid, Result& res, int root) {
     const uint4* __restrict__ nodes = a.nodes;
     const float4* __restrict__ tris = a.tris;
     int sp = 0, visits = 0, tests = 0;
@@ -152,6 +157,7 @@ __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int*
     float tn = -INFINITY;
     float best = r.tmax, bu = 0.f, bv = 0.f;
     int bid = -1, bti = -1;
+    int lts = 0, ltc = 0;          // leaf cursor: next triangle, triangles left
     bool found = false;
 
     auto push = [&](int rf, float t) {
@@ -161,87 +167,40 @@ __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int*
         } else if (sp < a.maxStack) {
             a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid] = make_int2(rf, __float_as_int(t));
         } else {
-            atomicOr(a.status, 1);   // unreachable: maxStack bounds a DFS of this BVH
+            atomicOr(a.status, 1);
             return;
         }
         sp++;
     };
 
     for (;;) {
-        bool descend = false;
-        if constexpr (MODE == kModeAny) {
-            if (ref < 0) {
-                const int enc = -(ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
-                for (int k = 0; k < tc; k++) {
-                    const int ti = ts + k;
-                    tests++;
-                    float t, u, v;
-                    if (rtd::isect<MT>(r, tris[3 * ti], tris[3 * ti + 1], tris[3 * ti + 2], t, u, v) &&
-                        t > r.tmin && t < r.tmax) { found = true; break; }
-                }
-                if (found) break;
-            } else {
-                visits++;
-                const uint4* np = nodes + 4 * (size_t)ref;
-                const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
-                float d0, d1, d2, d3;
-                const bool h0 = rtd::child_hit<0, FLAT>(r, qx, qy, qz, (int)qc.x, r.tmax, d0);
-                const bool h1 = rtd::child_hit<1, FLAT>(r, qx, qy, qz, (int)qc.y, r.tmax, d1);
-                const bool h2 = rtd::child_hit<2, FLAT>(r, qx, qy, qz, (int)qc.z, r.tmax, d2);
-                const bool h3 = rtd::child_hit<3, FLAT>(r, qx, qy, qz, (int)qc.w, r.tmax, d3);
-                // pushed in slot order; the last valid one is taken next instead of popped
-                int nxt = -1;
-                if (h0) { nxt = (int)qc.x; }
-                if (h1) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.y; }
-                if (h2) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.z; }
-                if (h3) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.w; }
-                if (nxt != -1) { ref = nxt; descend = true; }
-            }
-        } else {
-            if (tn <= best) {
-                if (ref < 0) {
-                    const int enc = -(ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
-                    for (int k = 0; k < tc; k++) {
-                        const int ti = ts + k;
-                        tests++;
-                        const float4 p0 = tris[3 * ti];
-                        float t, u, v;
-                        if (!rtd::isect<MT>(r, p0, tris[3 * ti + 1], tris[3 * ti + 2], t, u, v)) continue;
-                        if (!(t > r.tmin) || !(t < r.tmax)) continue;
-                        const int id = __float_as_int(p0.w);
-                        if (t < best || (t == best && id < bid)) { best = t; bid = id; bu = u; bv = v; bti = ti; }
-                    }
-                } else {
-                    visits++;
-                    const uint4* np = nodes + 4 * (size_t)ref;
-                    const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
-                    float d[4];
-                    int c[4];
-                    float tt;
-                    if (rtd::child_hit<0, FLAT>(r, qx, qy, qz, (int)qc.x, best, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
-                    if (rtd::child_hit<1, FLAT>(r, qx, qy, qz, (int)qc.y, best, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
-                    if (rtd::child_hit<2, FLAT>(r, qx, qy, qz, (int)qc.z, best, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
-                    if (rtd::child_hit<3, FLAT>(r, qx, qy, qz, (int)qc.w, best, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
-#define RT_CSWAP(i, j)                                                                  \
-    {                                                                                   \
-        const bool s = d[i] > d[j];                                                     \
-        const float di = d[i], dj = d[j];                                               \
-        const int ci = c[i], cj = c[j];                                                 \
-        d[i] = s ? dj : di; d[j] = s ? di : dj; c[i] = s ? cj : ci; c[j] = s ? ci : cj; \
-    }
-                    RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
-#undef RT_CSWAP
-                    // pushed far-to-near; the nearest valid one is taken next
-                    int nxt = -1; float nt = 0.f;
-                    if (c[3] != -1) { nxt = c[3]; nt = d[3]; }
-                    if (c[2] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[2]; nt = d[2]; }
-                    if (c[1] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[1]; nt = d[1]; }
-                    if (c[0] != -1) { if (nxt != -1) push(nxt, nt); nxt = c[0]; nt = d[0]; }
-                    if (nxt != -1) { ref = nxt; tn = nt; descend = true; }
-                }
-            }
+        if (ltc == 0 && ref < 0 && (MODE == kModeAny || tn <= best)) {
+            const int enc = -(ref + 2);
+            lts = enc >> 3; ltc = (enc & 7) + 1;
         }
-        if (descend) continue;
+        if (ltc > 0) {
+            const int ti = lts;
+            lts++; ltc--; tests++;
+            const float4 p0 = tris[3 * ti];
+            float t, u, v;
+            if (rtd::isect<MT>(r, p0, tris[3 * ti + 1], tris[3 * ti + 2], t, u, v) && t > r.tmin && t < r.tmax) {
+                if constexpr (MODE == kModeAny) {
+                    found = true;
+                    break;
+                } else {
+                    const int id = __float_as_int(p0.w);
+                    if (t < best || (t == best && id < bid)) { best = t; bid = id; bu = u; bv = v; bti = ti; }
+                }
+            }
+            if (ltc > 0) continue;
+        } else if (ref >= 0 && (MODE == kModeAny || tn <= best)) {
+            visits++;
+            int nxt = ref;
+            float nt = 0.f;
+            if constexpr (MODE == kModeAny) visit_any<FLAT>(nodes, r, nxt, push);
+            else visit_closest<FLAT>(nodes, r, best, nxt, nt, push);
+            if (nxt != -1) { ref = nxt; tn = nt; continue; }
+        }
         if (sp == 0) break;
         sp--;
         if (sp < LDSD) {
@@ -338,9 +297,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace_batch(TraceArgs a) {
 // has finished); (3) leaf phase: all lanes at a leaf test its triangles
 // together; (4) finished rays write their result. Node and leaf work never
 // share an iteration, so their instruction streams do not serialise against
-// each other. Every ray still follows the reference's depth-first order
-// (pop, re-check against the closest t, visit), so its hits and its node /
-// triangle counters are exactly those of a one-ray-at-a-time traversal.
+// each other. In counted launches every ray follows the reference's
+// depth-first order (pop, re-check against the closest t, visit), so its
+// node / triangle counters are exactly those of a one-ray-at-a-time
+// traversal; uncounted launches postpone leaves speculatively (see the node
+// phase) and return the same hits.
 template <int MODE, bool MT, bool PIN, bool COUNT, int LDSD, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_trace_ww(TraceArgs a) {
     __shared__ int s_ref[LDSD * BLOCK];
@@ -361,6 +322,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_ww(TraceArgs a) {
     int poolNext = 0, poolEnd = 0;   // wave-uniform: reserved, not yet taken work items
     bool flat = false, found = false;
     int sp = 0, ref = -1, visits = 0, tests = 0, bid = -1, bti = -1;
+    int pend = -1;             // postponed leaf (speculative node phase), -1: none
     float tn = 0.f, best = 0.f, bu = 0.f, bv = 0.f;
 
     auto push = [&](int rf, float t) {
@@ -439,24 +401,38 @@ __global__ __launch_bounds__(BLOCK) void k_trace_ww(TraceArgs a) {
         const bool waveFlat = __any(has && flat);
 
         // ---- (2) node phase ----
+        // Speculative (uncounted launches): a lane whose next entry is a leaf
+        // postpones it (one at a time) and keeps visiting nodes; the phase
+        // ends when every lane holds a postponed leaf or is done, and the
+        // leaf phase tests them together (Aila and Laine, "Understanding the
+        // Efficiency of Ray Traversal on GPUs", HPG 2009). Nodes visited past
+        // a postponed leaf use a closest t that the leaf may still lower, so
+        // such a launch can visit a few more nodes than the depth-first
+        // order; every box test stays conservative and the hits are the same.
+        // Counted launches keep the exact depth-first order and counters.
+        constexpr bool SPEC = !COUNT;
         bool fin = false;
         for (;;) {
             if (has && !fin) {
-                // pop until an entry survives the closest-t re-check
-                while (ref == -1 || (MODE != kModeAny && tn > best)) {
-                    if (sp == 0) { fin = true; break; }
-                    sp--;
-                    if (sp < LDSD) {
-                        ref = s_ref[sp * BLOCK + tid];
-                        tn = s_tn[sp * BLOCK + tid];
-                    } else {
-                        const int2 e = a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid];
-                        ref = e.x; tn = __int_as_float(e.y);
+                for (;;) {
+                    // pop until an entry survives the closest-t re-check
+                    while (ref == -1 || (MODE != kModeAny && tn > best)) {
+                        if (sp == 0) { fin = true; break; }
+                        sp--;
+                        if (sp < LDSD) {
+                            ref = s_ref[sp * BLOCK + tid];
+                            tn = s_tn[sp * BLOCK + tid];
+                        } else {
+                            const int2 e = a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid];
+                            ref = e.x; tn = __int_as_float(e.y);
+                        }
                     }
+                    if (SPEC && !fin && ref < -1 && pend == -1) { pend = ref; ref = -1; continue; }
+                    break;
                 }
             }
             const bool atNode = has && !fin && ref >= 0;
-            if (!__any(atNode)) break;
+            if (!__any(atNode && (!SPEC || pend == -1))) break;
             if (atNode) {
                 visits++;
                 if constexpr (MODE == kModeAny) {
@@ -470,8 +446,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace_ww(TraceArgs a) {
         }
 
         // ---- (3) leaf phase ----
-        if (has && !fin) {
-            const int enc = -(ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
+        const int leaf = SPEC ? pend : ((has && !fin) ? ref : -1);
+        if (has && leaf < -1) {
+            const int enc = -(leaf + 2), ts = enc >> 3, tc = (enc & 7) + 1;
             for (int k = 0; k < tc; k++) {
                 const int ti = ts + k;
                 tests++;
@@ -488,7 +465,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace_ww(TraceArgs a) {
                     if (t < best || (t == best && id < bid)) { best = t; bid = id; bu = u; bv = v; bti = ti; }
                 }
             }
-            ref = -1;
+            if (SPEC) pend = -1;
+            else ref = -1;
         }
 
         // ---- (4) retire finished rays ----
