@@ -36,10 +36,16 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
     int ri = 0, inst = -1;
     const uint4* nodes = a.tlasNodes;
     const float4* tris = nullptr;
-    bool has = false, exhausted = false, found = false;
+    bool has = false, exhausted = false, found = false, flatW = false, flatO = false;
     int poolNext = 0, poolEnd = 0;
     int sp = 0, ref = -1, bid = -1, binst = -1, visits = 0, tests = 0;
+    int pend = -1;             // postponed BLAS leaf (speculative node phase), -1: none
     float tn = 0.f, best = 0.f, bu = 0.f, bv = 0.f;
+    // Uncounted launches postpone BLAS leaves speculatively, as the
+    // single-level while-while kernel does; a lane holding a postponed leaf
+    // stays inside its instance (it waits at the return marker), so the leaf
+    // is tested with that instance's object-space ray.
+    const bool spec = a.cntNodes == nullptr;
 
     auto push = [&](int rf, float t) {
         if (sp < LDSD) {
@@ -83,6 +89,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
                     rw.dx = a.dx[ri]; rw.dy = a.dy[ri]; rw.dz = a.dz[ri];
                     rw.tmin = a.tmin[ri]; rw.tmax = a.tmax[ri];
                     rtd::ray_setup(rw);
+                    flatW = rtd::ray_flat(rw);
                     has = true;
                     sp = 0; ref = 0; tn = -INFINITY; inst = -1; nodes = a.tlasNodes;
                     best = rw.tmax; bu = 0.f; bv = 0.f; bid = -1; binst = -1;
@@ -97,27 +104,41 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
         bool fin = false;
         for (;;) {
             if (has && !fin) {
-                while (ref == -1 || ref == kReturnMark || (!ANY && tn > best)) {
-                    if (ref == kReturnMark) { inst = -1; nodes = a.tlasNodes; ref = -1; continue; }
-                    if (sp == 0) { fin = true; break; }
-                    sp--;
-                    if (sp < LDSD) {
-                        ref = s_ref[sp * BLOCK + tid];
-                        tn = s_tn[sp * BLOCK + tid];
-                    } else {
-                        const int2 e = a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid];
-                        ref = e.x; tn = __int_as_float(e.y);
+                for (;;) {
+                    while (ref == -1 || ref == kReturnMark || (!ANY && tn > best)) {
+                        if (ref == kReturnMark) {
+                            if (pend != -1) break;           // wait inside the instance
+                            inst = -1; nodes = a.tlasNodes; ref = -1;
+                            continue;
+                        }
+                        if (sp == 0) { fin = true; break; }
+                        sp--;
+                        if (sp < LDSD) {
+                            ref = s_ref[sp * BLOCK + tid];
+                            tn = s_tn[sp * BLOCK + tid];
+                        } else {
+                            const int2 e = a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid];
+                            ref = e.x; tn = __int_as_float(e.y);
+                        }
                     }
+                    if (spec && !fin && inst >= 0 && ref < -1 && ref != kReturnMark && pend == -1) {
+                        pend = ref; ref = -1;
+                        continue;
+                    }
+                    break;
                 }
             }
-            // TLAS nodes, TLAS leaves (instance entry) and BLAS nodes are node work
-            const bool atNode = has && !fin && (ref >= 0 || inst < 0);
-            if (!__any(atNode)) break;
+            // TLAS nodes, TLAS leaves (instance entry) and BLAS nodes are node
+            // work; a lane waiting at the return marker with a postponed leaf
+            // is not
+            const bool atNode = has && !fin && ref != kReturnMark && (ref >= 0 || inst < 0);
+            if (!__any(atNode && pend == -1)) break;
             if (atNode) {
                 if (ref >= 0) {
                     visits++;
-                    const Ray& r = inst < 0 ? rw : ro;
-                    visit2<ANY>(nodes, r, ANY ? r.tmax : best, ref, tn, push);
+                    const bool w = inst < 0;
+                    const Ray& r = w ? rw : ro;
+                    visit2w<ANY>(nodes, r, w ? flatW : flatO, ANY ? r.tmax : best, ref, tn, push);
                 } else {
                     // TLAS leaf: enter its instance
                     const int enc = -(ref + 2);
@@ -126,6 +147,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
                     inst = k;
                     const InstanceDev& I = a.inst[k];
                     xform(I.w2o, rw, ro);
+                    flatO = rtd::ray_flat(ro);
                     const BlasDev& B = a.blas[I.blas];
                     nodes = B.nodes;
                     tris = B.tris;
@@ -134,8 +156,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
                 }
             }
         }
-        if (has && !fin) {
-            const int enc = -(ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
+        const int leaf = spec ? pend : ((has && !fin) ? ref : -1);
+        if (has && leaf < -1) {
+            const int enc = -(leaf + 2), ts = enc >> 3, tc = (enc & 7) + 1;
             for (int q = 0; q < tc; q++) {
                 const int ti = ts + q;
                 tests++;
@@ -154,7 +177,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
                     }
                 }
             }
-            ref = -1;
+            if (spec) pend = -1;
+            else ref = -1;
         }
         if (has && fin) {
             if constexpr (ANY) {

@@ -28,17 +28,17 @@ __device__ __forceinline__ void xform(const float* M, const Ray& w, Ray& o) {
     rtd::ray_setup(o);
 }
 
-template <bool ANY, class Push>
+template <bool ANY, bool FLAT, class Push>
 __device__ __forceinline__ void visit2(const uint4* __restrict__ nodes, const Ray& r, float limit, int& ref, float& tn,
                                        Push&& push) {
     const uint4* np = nodes + 4 * (size_t)ref;
     const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
     if constexpr (ANY) {
         float d0, d1, d2, d3;
-        const bool h0 = rtd::child_hit<0, true>(r, qx, qy, qz, (int)qc.x, limit, d0);
-        const bool h1 = rtd::child_hit<1, true>(r, qx, qy, qz, (int)qc.y, limit, d1);
-        const bool h2 = rtd::child_hit<2, true>(r, qx, qy, qz, (int)qc.z, limit, d2);
-        const bool h3 = rtd::child_hit<3, true>(r, qx, qy, qz, (int)qc.w, limit, d3);
+        const bool h0 = rtd::child_hit<0, FLAT>(r, qx, qy, qz, (int)qc.x, limit, d0);
+        const bool h1 = rtd::child_hit<1, FLAT>(r, qx, qy, qz, (int)qc.y, limit, d1);
+        const bool h2 = rtd::child_hit<2, FLAT>(r, qx, qy, qz, (int)qc.z, limit, d2);
+        const bool h3 = rtd::child_hit<3, FLAT>(r, qx, qy, qz, (int)qc.w, limit, d3);
         int nxt = -1;
         if (h0) { nxt = (int)qc.x; }
         if (h1) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.y; }
@@ -50,10 +50,10 @@ __device__ __forceinline__ void visit2(const uint4* __restrict__ nodes, const Ra
         float d[4];
         int c[4];
         float tt;
-        if (rtd::child_hit<0, true>(r, qx, qy, qz, (int)qc.x, limit, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
-        if (rtd::child_hit<1, true>(r, qx, qy, qz, (int)qc.y, limit, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
-        if (rtd::child_hit<2, true>(r, qx, qy, qz, (int)qc.z, limit, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
-        if (rtd::child_hit<3, true>(r, qx, qy, qz, (int)qc.w, limit, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
+        if (rtd::child_hit<0, FLAT>(r, qx, qy, qz, (int)qc.x, limit, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
+        if (rtd::child_hit<1, FLAT>(r, qx, qy, qz, (int)qc.y, limit, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
+        if (rtd::child_hit<2, FLAT>(r, qx, qy, qz, (int)qc.z, limit, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
+        if (rtd::child_hit<3, FLAT>(r, qx, qy, qz, (int)qc.w, limit, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
 #define RT_CSWAP(i, j)                                                                  \
     {                                                                                   \
         const bool s = d[i] > d[j];                                                     \
@@ -72,6 +72,16 @@ __device__ __forceinline__ void visit2(const uint4* __restrict__ nodes, const Ra
         ref = nxt;
         tn = nt;
     }
+}
+
+// Node visit with the slab variant picked per wave: the one that resolves
+// 0*inf when any active lane's current ray (world or object space) is flat,
+// the plain one otherwise; both agree on every ray that is not flat.
+template <bool ANY, class Push>
+__device__ __forceinline__ void visit2w(const uint4* __restrict__ nodes, const Ray& r, bool flat, float limit, int& ref,
+                                        float& tn, Push&& push) {
+    if (__any(flat)) visit2<ANY, true>(nodes, r, limit, ref, tn, push);
+    else visit2<ANY, false>(nodes, r, limit, ref, tn, push);
 }
 
 
@@ -129,47 +139,43 @@ struct LdsStack {
 // One ray through TLAS and BLAS: the traversal of k_trace2l for a single ray
 // (same culling and tie rule; any hit may visit children nearest first,
 // ORD). Closest hit: returns whether something was hit and (t, inst, id, u,
-// v); any hit: whether the ray is occluded.
+// v); any hit: whether the ray is occluded. Each loop iteration is one short
+// step — a node visit, an instance entry, or one triangle of the current
+// BLAS leaf (leaf cursor) — so lanes of a wave at different kinds of step do
+// not wait for whole leaf loops.
 template <bool ANY, bool ORD, class Stack>
 __device__ __forceinline__ bool trace2l_ray(const uint4* tlasNodes, const float4* tlasTris, const InstanceDev* insts,
                                             const BlasDev* blas, const Ray& rw, Stack& st, float& bt, int& binst,
                                             int& bid, float& bu, float& bv) {
     Ray ro;
     int ref = 0, inst = -1;
+    int lts = 0, ltc = 0;                   // leaf cursor: next triangle, triangles left
     float tn = -INFINITY, best = rw.tmax;
+    const bool flatW = rtd::ray_flat(rw);
+    bool flatO = false;
     const uint4* nodes = tlasNodes;
     const float4* tris = nullptr;
     bid = -1; binst = -1; bt = rw.tmax; bu = 0.f; bv = 0.f;
     auto push = [&](int rf, float t) { st.push(rf, t); };
     for (;;) {
-        while (ref == -1 || ref == kReturnMark || (!ANY && tn > best)) {
-            if (ref == kReturnMark) { inst = -1; nodes = tlasNodes; ref = -1; continue; }
-            if (!st.pop(ref, tn)) return ANY ? false : bid >= 0;
+        if (ltc == 0) {
+            while (ref == -1 || ref == kReturnMark || (!ANY && tn > best)) {
+                if (ref == kReturnMark) { inst = -1; nodes = tlasNodes; ref = -1; continue; }
+                if (!st.pop(ref, tn)) return ANY ? false : bid >= 0;
+            }
+            if (ref < 0 && inst >= 0) {         // BLAS leaf: start its cursor
+                const int enc = -(ref + 2);
+                lts = enc >> 3; ltc = (enc & 7) + 1;
+                ref = -1;
+            }
         }
-        if (ref >= 0) {
-            const Ray& r = inst < 0 ? rw : ro;
-            if constexpr (ANY && ORD) visit2<false>(nodes, r, r.tmax, ref, tn, push);
-            else visit2<ANY>(nodes, r, ANY ? r.tmax : best, ref, tn, push);
-        } else if (inst < 0) {
-            const int enc = -(ref + 2);
-            const int k = __float_as_int(tlasTris[3 * (enc >> 3)].w);
-            push(kReturnMark, -INFINITY);
-            inst = k;
-            const InstanceDev& I = insts[k];
-            xform(I.w2o, rw, ro);
-            const BlasDev& B = blas[I.blas];
-            nodes = B.nodes;
-            tris = B.tris;
-            ref = 0;
-            tn = -INFINITY;
-        } else {
-            const int enc = -(ref + 2), ts = enc >> 3, tc = (enc & 7) + 1;
-            for (int q = 0; q < tc; q++) {
-                const int ti = ts + q;
-                const float4 p0 = tris[3 * ti];
-                float t, u, v;
-                if (!rtd::isect_watertight(ro, p0, tris[3 * ti + 1], tris[3 * ti + 2], t, u, v)) continue;
-                if (!(t > ro.tmin) || !(t < ro.tmax)) continue;
+        if (ltc > 0) {
+            const int ti = lts;
+            lts++; ltc--;
+            const float4 p0 = tris[3 * ti];
+            float t, u, v;
+            if (rtd::isect_watertight(ro, p0, tris[3 * ti + 1], tris[3 * ti + 2], t, u, v) && t > ro.tmin &&
+                t < ro.tmax) {
                 if constexpr (ANY) {
                     return true;
                 } else {
@@ -179,7 +185,25 @@ __device__ __forceinline__ bool trace2l_ray(const uint4* tlasNodes, const float4
                     }
                 }
             }
-            ref = -1;
+        } else if (ref >= 0) {
+            const bool w = inst < 0;
+            const Ray& r = w ? rw : ro;
+            if constexpr (ANY && ORD) visit2w<false>(nodes, r, w ? flatW : flatO, r.tmax, ref, tn, push);
+            else visit2w<ANY>(nodes, r, w ? flatW : flatO, ANY ? r.tmax : best, ref, tn, push);
+        } else {
+            // TLAS leaf: enter its instance
+            const int enc = -(ref + 2);
+            const int k = __float_as_int(tlasTris[3 * (enc >> 3)].w);
+            push(kReturnMark, -INFINITY);
+            inst = k;
+            const InstanceDev& I = insts[k];
+            xform(I.w2o, rw, ro);
+            flatO = rtd::ray_flat(ro);
+            const BlasDev& B = blas[I.blas];
+            nodes = B.nodes;
+            tris = B.tris;
+            ref = 0;
+            tn = -INFINITY;
         }
     }
 }
