@@ -7,9 +7,11 @@ evaluation compares, at equal GPU time, the 1 spp + denoiser image with plain
 sample accumulation against a high-spp reference (PSNR of the tonemapped
 image x / (1 + x), peak 1).
 
-Training pairs are 256x256 windows of 1920x1080 views: the window camera is
-the full-frame camera restricted to the window (same pixel footprint), so the
-network sees the feature scale and noise of the 1080p renders it denoises.
+Training pairs are 256x256 crops of 1920x1080 views, so the network sees the
+feature scale and noise of the 1080p renders it denoises. By default each view
+is rendered whole (1 spp input, many-spp target) and cropped; the window mode
+renders each crop alone through a camera restricted to the window (same pixel
+footprint), which is cheaper for low target spp.
 """
 from __future__ import annotations
 
@@ -115,19 +117,39 @@ def rel_mse(a, ref) -> float:
 
 
 def make_pairs(dev, scene_seeds, views_per_scene: int, windows_per_view: int, target_spp: int, crop: int = 256,
-               seed: int = 0, mesh_res=scenes.INSTANCED_MESH_RES, log=print):
+               seed: int = 0, mesh_res=scenes.INSTANCED_MESH_RES, log=print, full_frame: bool = True):
     """Training pairs: returns (inputs int16 [N, crop, crop, 16] bf16 bits on
-    the device, targets float32 [N, crop, crop, 3] = log(1 + reference))."""
+    the device, targets float32 [N, crop, crop, 3] = log(1 + reference)).
+    N = len(scene_seeds) * views_per_scene * windows_per_view; crops start at
+    multiples of 8 pixels."""
     t = importlib.import_module("torch")
     rng = np.random.default_rng(seed)
     xs, ys = [], []
-    dn = pkg.Denoiser(crop * crop, seed=0)      # only for packing
     W, H = 1920, 1080
+    npx = W * H if full_frame else crop * crop
+    dn = pkg.Denoiser((npx + 63) // 64 * 64, seed=0)      # only for packing
     for s in scene_seeds:
         sc = scenes.instanced_scene(mesh_res=mesh_res, seed=int(s))
         r = Renderer(sc, dev)
         for v in range(views_per_scene):
             cam = random_view(rng)
+            if full_frame:
+                sd = int(rng.integers(1, 2 ** 31))
+                noisy, _ = r.render(cam, W, H, 1, sd)
+                ref, _ = r.render(cam, W, H, target_spp, sd + 1, flags=pkg.FLAG_GRAPH)
+                xf = t.empty((W * H, 16), dtype=t.int16, device=dev)
+                dn.pack(W * H, noisy["rgb"], noisy["alb"], noisy["nrm"], noisy["dep"], xf)
+                xf = xf.view(H, W, 16)
+                yf = t.log1p(ref["rgb"].view(3, H, W).permute(1, 2, 0).clamp_min(0))
+                for k in range(windows_per_view):
+                    x0 = int(rng.integers(0, (W - crop) // 8 + 1)) * 8
+                    y0 = int(rng.integers(0, (H - crop) // 8 + 1)) * 8
+                    xs.append(xf[y0:y0 + crop, x0:x0 + crop].contiguous())
+                    ys.append(yf[y0:y0 + crop, x0:x0 + crop].contiguous())
+                del noisy, ref, xf, yf
+                t.cuda.synchronize()
+                log(f"pairs: scene {s} view {v}: {target_spp} spp frame done ({len(xs)} pairs)")
+                continue
             for k in range(windows_per_view):
                 x0 = int(rng.integers(0, (W - crop) // 8 + 1)) * 8
                 y0 = int(rng.integers(0, (H - crop) // 8 + 1)) * 8
@@ -170,11 +192,13 @@ def train(dn, X, Y, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 0,
     return hist
 
 
-def train_and_save(dev, scene_seeds=range(100, 108), views: int = 4, windows: int = 16, target_spp: int = 512,
-                   steps: int = 4000, batch: int = 8, lr: float = 1e-3, seed: int = 7, path: str = WEIGHTS, log=print):
-    """Renders the training pairs, trains a fresh network and stores it."""
+def train_and_save(dev, scene_seeds=range(100, 108), views: int = 4, windows: int = 16, target_spp: int = 4096,
+                   steps: int = 4000, batch: int = 8, lr: float = 1e-3, seed: int = 7, path: str = WEIGHTS, log=print,
+                   full_frame: bool = True):
+    """Renders the training pairs (default: 8 scenes x 4 views x 16 crops =
+    512 pairs of 1 spp / 4096 spp), trains a fresh network and stores it."""
     t0 = time.time()
-    X, Y = make_pairs(dev, scene_seeds, views, windows, target_spp, log=log)
+    X, Y = make_pairs(dev, scene_seeds, views, windows, target_spp, log=log, full_frame=full_frame)
     log(f"{X.shape[0]} pairs in {time.time() - t0:.1f} s")
     dn = pkg.Denoiser(max(batch * X.shape[1] * X.shape[2], 1920 * 1088), seed=seed)
     hist = train(dn, X, Y, steps, batch=batch, lr=lr, log=log)
@@ -216,12 +240,13 @@ if __name__ == "__main__":
     ap.add_argument("--scenes", type=int, default=8)
     ap.add_argument("--views", type=int, default=4)
     ap.add_argument("--windows", type=int, default=16)
-    ap.add_argument("--target-spp", type=int, default=512)
+    ap.add_argument("--target-spp", type=int, default=4096)
     ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--window-mode", action="store_true", help="render each crop alone instead of whole frames")
     ap.add_argument("--out", default=WEIGHTS)
     a = ap.parse_args()
     if a.time_inference:
         time_inference(torch.device("cuda:0"), a.time_inference, log=lambda m: print(m, flush=True))
         sys.exit(0)
     train_and_save(torch.device("cuda:0"), range(100, 100 + a.scenes), a.views, a.windows, a.target_spp, a.steps,
-                   path=a.out, log=lambda m: print(m, flush=True))
+                   path=a.out, log=lambda m: print(m, flush=True), full_frame=not a.window_mode)
