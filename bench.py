@@ -130,7 +130,7 @@ def main():
                                   ("incoherent_closest", pkg.CLASS_INCOHERENT, dict(rays=Dm)),
                                   ("shadow_any", pkg.CLASS_ANY, dict(rays=Sm))):
                 best, table = bvh.autotune(cls, reps=3, **kw)
-                sched = ("batch", "while-while", "packet")
+                sched = ("batch", "while-while", "packet", "refill")
                 tuning[name] = {"schedule": sched[best[0]], "lds_depth": best[1], "blocks_per_cu": best[2],
                                 "sweep_ms": {f"{sched[sc]}/{d}/{b}": round(ms, 4) for sc, d, b, ms in table}}
                 log(f"autotune {name}: {sched[best[0]]} lds {best[1]} blocks/CU {best[2] or 'max'}  "
@@ -324,14 +324,25 @@ def bench_path_tracer(pkg, scenes, dev, args):
              "dz": torch.full((m,), float(sun[2]), device=dev), "tmin": torch.zeros(m, device=dev),
              "tmax": torch.full((m,), 1e30, device=dev)}
         S = {k: v.contiguous() for k, v in S.items()}
-        for name, cls, rays, cnt in (("closest", 0, R, n), ("shadow_any", 1, S, m)):
+        # scattered rays for the later bounces: from the same hit points, random
+        # directions in the hemisphere facing back towards the camera
+        g = torch.Generator(device=dev)
+        g.manual_seed(5)
+        dirs = torch.randn(3, m, device=dev, generator=g)
+        dirs = dirs / dirs.norm(dim=0, keepdim=True)
+        dirs = torch.where((dirs * d).sum(dim=0, keepdim=True) > 0, -dirs, dirs)
+        B = {"ox": P[0] + 1e-3 * dirs[0], "oy": P[1] + 1e-3 * dirs[1], "oz": P[2] + 1e-3 * dirs[2],
+             "dx": dirs[0], "dy": dirs[1], "dz": dirs[2], "tmin": torch.zeros(m, device=dev),
+             "tmax": torch.full((m,), 1e30, device=dev)}
+        B = {k: v.contiguous() for k, v in B.items()}
+        for name, cls, rays, cnt in (("closest", 0, R, n), ("shadow_any", 1, S, m), ("bounce_closest", 2, B, m)):
             best, table = gs.autotune(cls, cnt, rays, reps=3)
-            sched = ("while-while", "batch", "batch-nearest-first")
+            sched = ("while-while", "batch", "batch-nearest-first", "refill")
             tuning[name] = {"schedule": sched[best[0]], "lds_depth": best[1], "blocks_per_cu": best[2],
                             "sweep_ms": {f"{sched[sc]}/{d}/{b}": round(ms, 4) for sc, d, b, ms in table}}
             log(f"pt autotune {name}: {sched[best[0]]} lds {best[1]} blocks/CU {best[2] or 'max'} "
                 f"({min(x[3] for x in table):.3f} ms)")
-        del R, S, t, inst, pid
+        del R, S, B, t, inst, pid
     rgb = torch.empty(3 * n, device=dev)
     alb = torch.empty(3 * n, device=dev)
     nrm = torch.empty(3 * n, device=dev)
@@ -465,8 +476,8 @@ def bench_denoiser(pkg, scenes, dev, args, pt_tuning=None):
         dn.set_params(dt.load_weights(wpath))
     sc = scenes.instanced_scene(seed=2)
     r = dt.Renderer(sc, dev)
-    sched = {"while-while": 0, "batch": 1, "batch-nearest-first": 2}
-    for cls, key in ((0, "closest"), (1, "shadow_any")):      # the path-tracer leg's tuned launches
+    sched = {"while-while": 0, "batch": 1, "batch-nearest-first": 2, "refill": 3}
+    for cls, key in ((0, "closest"), (1, "shadow_any"), (2, "bounce_closest")):   # the path-tracer leg's tuned launches
         tcfg = (pt_tuning or {}).get(key)
         if tcfg:
             r.scene.set_trace_config(cls, sched[tcfg["schedule"]], tcfg["lds_depth"], tcfg["blocks_per_cu"])

@@ -224,7 +224,8 @@ int cudaBVH_traceAnyQueued(CudaBVH_t bvh, int capacity, const int* d_count,
  * (0 batch: a wave keeps 64 rays until all finish; 1 while-while: separate
  * node and leaf phases, idle lanes refilled from a wave-local work pool;
  * 2 packet, coherent class only: one 8x8 pixel tile per wave sharing one
- * traversal stack),
+ * traversal stack; 3 refill: the batch schedule's loop with idle lanes
+ * refilled from the wave-local pool),
  * LDS stack depth per lane (4, 8, 12 or 16 entries; deeper entries spill to
  * HBM) and resident workgroups per CU (0 = occupancy limit). */
 int cudaBVH_setTraceConfig(CudaBVH_t bvh, int rayClass, int schedule, int ldsDepth, int blocksPerCU);
@@ -233,7 +234,7 @@ int cudaBVH_getTraceConfig(CudaBVH_t bvh, int rayClass, int* schedule, int* ldsD
 /* Autotuner: times every candidate configuration of one ray class on the
  * caller's workload (coherent: a W x H pinhole frame; otherwise numRays
  * device rays), reps timed launches each, and keeps the fastest in the
- * handle. results (nullable, room for 144 floats: schedule, ldsDepth,
+ * handle. results (nullable, room for 192 floats: schedule, ldsDepth,
  * blocksPerCU, ms per candidate) and *numResults report the sweep. */
 int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const CudaCamera* cam,
                      int numRays, const CudaRayPacket* rays, int reps, float* results, int* numResults);
@@ -309,11 +310,13 @@ int cudaBVH_sceneTraceAnyQueued(CudaScene_t scene, int capacity, const int* d_co
                                 const CudaRayPacket* rays, uint8_t* occluded, unsigned flags);
 
 /* Launch configuration of the scene traversal per ray class (0 closest hit,
- * 1 any hit): schedule (0 while-while with a wave pool, 1 one ray per lane in
- * 64-ray batches, 2 = 1 with nearest-first children for any hit), LDS stack
- * depth (4, 8, 12, 16) and resident workgroups per CU (0 = as many as fit).
- * sceneAutotune times every candidate on the caller's rays (results: schedule,
- * depth, blocks/CU, ms per candidate; up to 36) and keeps the fastest. */
+ * 1 any hit, 2 closest hit of the path tracer's bounces after the first — the
+ * first bounce and sceneTraceClosest* use class 0): schedule (0 while-while with a wave pool, 1 one ray per lane in
+ * 64-ray batches, 2 = 1 with nearest-first children for any hit, 3 = 1 with
+ * idle lanes refilled from a wave-local pool), LDS stack depth (4, 8, 12, 16)
+ * and resident workgroups per CU (0 = as many as fit). sceneAutotune times
+ * every candidate on the caller's rays (results: schedule, depth, blocks/CU,
+ * ms per candidate; up to 48) and keeps the fastest. */
 int cudaBVH_sceneSetTraceConfig(CudaScene_t scene, int rayClass, int schedule, int ldsDepth, int blocksPerCU);
 int cudaBVH_sceneGetTraceConfig(CudaScene_t scene, int rayClass, int* schedule, int* ldsDepth, int* blocksPerCU);
 int cudaBVH_sceneAutotune(CudaScene_t scene, int rayClass, int numRays, const CudaRayPacket* rays, int reps,

@@ -95,3 +95,29 @@ def test_path_tracer_errors(pkg, scenes, dev):
     with pytest.raises(pkg.CudaBVHError):
         gs.path_trace(c, pkg.path_params(48, 32, sun_dir=(0, 0, 0)), buf, buf, buf, buf)
     gs.destroy()
+
+
+@pytest.mark.parametrize("configs", [((1, 8, 0), (3, 12, 0), (3, 8, 4)), ((0, 4, 0), (2, 16, 2), (1, 12, 0))])
+def test_path_tracer_schedules(pkg, oracle, scenes, dev, configs):
+    """The image is the same for every traversal schedule of the three scene
+    ray classes (first-bounce closest hit, shadow any hit, later bounces),
+    eager and replayed from a hipGraph."""
+    c0, c1, c2 = configs
+    print(f"pt schedules {configs}", flush=True)
+    sc = scenes.instanced_scene(mesh_res=(12, 20, 28, 36), n_inst=64, seed=9, span=6.0)
+    cam = scenes.look_at((0.0, 4.0, -18.0), (0.0, 0.0, 0.0), 96, 64, 60.0)
+    gs = _scene(pkg, sc)
+    gs.set_trace_config(0, *c0)
+    gs.set_trace_config(1, *c1)
+    gs.set_trace_config(2, *c2)
+    assert gs.trace_config(2) == c2
+    osc = oracle.OracleScene(sc["meshes"], sc["o2w"], sc["blas_of"], gs.world_to_object())
+    op = oracle.path_params(96, 64, spp=2, seed=21)
+    ref, raov, rcnt = oracle.path_trace_2l(osc, sc["albedo"], scenes.cam_array(cam), op)
+    for flags in (0, pkg.FLAG_GRAPH):
+        p = pkg.path_params(96, 64, spp=2, seed=21, tail_paths=0)
+        rgb, aov, cnt = _render(pkg, gs, cam, p, dev, flags=flags)
+        assert np.array_equal(cnt, rcnt), flags
+        assert np.array_equal(_bits(rgb), _bits(ref)), flags
+        assert np.array_equal(_bits(aov), _bits(raov)), flags
+    gs.destroy()

@@ -118,9 +118,9 @@ def test_scene_edge_cases(pkg, oracle, scenes, dev):
 
 
 def test_scene_every_launch_configuration(pkg, oracle, scenes, dev):
-    """Every schedule (while-while, batch, batch nearest-first for any hit)
-    and a range of LDS depths gives the oracle's hits; the autotuner keeps a
-    configuration from its own table."""
+    """Every schedule (while-while, batch, batch nearest-first for any hit,
+    refill) and a range of LDS depths gives the oracle's hits; the autotuner
+    keeps a configuration from its own table."""
     import torch
     sc = scenes.instanced_scene(mesh_res=(6, 11, 17, 23), n_inst=24, seed=5, span=6.0)
     gs = _make(pkg, sc)
@@ -133,10 +133,10 @@ def test_scene_every_launch_configuration(pkg, oracle, scenes, dev):
     ref = osc.trace_closest(rays)
     occ_ref = osc.trace_any(srays)
     R, S = to_dev(rays, dev), to_dev(srays, dev)
-    for sched in (0, 1, 2):
+    for sched in (0, 1, 2, 3):
         for depth in (4, 16):
             print(f"scene config schedule={sched} lds={depth}", flush=True)
-            if sched < 2:
+            if sched != 2:
                 gs.set_trace_config(0, sched, depth, 0)
                 t = torch.empty(n, device=dev)
                 inst = torch.empty(n, dtype=torch.int32, device=dev)
@@ -156,5 +156,9 @@ def test_scene_every_launch_configuration(pkg, oracle, scenes, dev):
     with pytest.raises(pkg.CudaBVHError):
         gs.set_trace_config(0, 2, 8, 0)          # nearest-first ordering is an any-hit schedule
     best, table = gs.autotune(1, n, S, reps=1)
-    assert len(table) == 36 and best in [(s_, d_, b_) for s_, d_, b_, _ in table]
+    assert len(table) == 48 and best in [(s_, d_, b_) for s_, d_, b_, _ in table]
+    # class 2 (the path tracer's secondary bounces) tunes closest-hit traces
+    best2, table2 = gs.autotune(2, n, R, reps=1)
+    assert len(table2) == 36 and {s_ for s_, _, _, _ in table2} == {0, 1, 3}
+    assert gs.trace_config(2) == best2 and best2 in [(s_, d_, b_) for s_, d_, b_, _ in table2]
     gs.destroy()

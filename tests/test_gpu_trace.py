@@ -187,6 +187,14 @@ def test_axis_parallel_rays(pkg, oracle, dev):
         _assert_closest_equal(got, ref)
         assert np.array_equal(got["nodes"], ref["nodes"])
         assert (ref["id"] >= 0).sum() > 100
+        # uncounted launches of every explicit-ray schedule (batch, speculative
+        # while-while, refill) on the same mixed waves
+        occ_ref = oracle.trace_any(nodes, lt, pid, rays)["occ"]
+        for sch in (0, 1, 3):
+            b.set_trace_config(pkg.CLASS_INCOHERENT, sch, 8, 0)
+            b.set_trace_config(pkg.CLASS_ANY, sch, 8, 0)
+            _assert_closest_equal(_gpu_closest(b, rays, dev), ref)
+            assert np.array_equal(_gpu_any(b, rays, dev), occ_ref)
 
 
 def test_edge_cases(pkg, oracle, dev):
@@ -311,11 +319,12 @@ def test_determinism_and_stream(pkg, oracle, scenes, parity_bvh, dev):
     _assert_closest_equal(a, {"id": b["prim_id"], "t": b["t"], "u": b["u"], "v": b["v"]})
 
 
-@pytest.mark.parametrize("schedule", [0, 1, 2])
+@pytest.mark.parametrize("schedule", [0, 1, 2, 3])
 @pytest.mark.parametrize("depth", [4, 8, 16])
 def test_every_launch_configuration(pkg, oracle, scenes, parity_bvh, dev, schedule, depth):
-    """Each wave schedule and LDS stack depth (4 spills most entries to HBM)
-    gives the oracle's answers for all three ray classes."""
+    """Each wave schedule (batch, while-while, packet, refill) and LDS stack
+    depth (4 spills most entries to HBM) gives the oracle's answers for all
+    three ray classes."""
     import torch
     W = H = 192
     cam = scenes.parity_camera(W, H)
@@ -332,7 +341,7 @@ def test_every_launch_configuration(pkg, oracle, scenes, parity_bvh, dev, schedu
     try:
         for cls in (pkg.CLASS_COHERENT, pkg.CLASS_INCOHERENT, pkg.CLASS_ANY):
             # the packet schedule exists for the coherent class only
-            sch = schedule if (schedule < 2 or cls == pkg.CLASS_COHERENT) else 0
+            sch = schedule if (schedule != 2 or cls == pkg.CLASS_COHERENT) else 0
             parity_bvh.set_trace_config(cls, sch, depth, 0 if depth != 8 else 2)
         hb = hit_buffers(W * H, dev)
         parity_bvh.trace_pinhole_closest(W, H, c, hb["t"], hb["prim_id"], hb["u"], hb["v"])

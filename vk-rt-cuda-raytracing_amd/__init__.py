@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(HERE, "libcudabvh_hip.so")
 
 # Ray classes and wave schedules of the launch configuration / autotuner
 CLASS_COHERENT, CLASS_INCOHERENT, CLASS_ANY = 0, 1, 2
-SCHEDULE_BATCH, SCHEDULE_WHILE_WHILE, SCHEDULE_PACKET = 0, 1, 2
+SCHEDULE_BATCH, SCHEDULE_WHILE_WHILE, SCHEDULE_PACKET, SCHEDULE_REFILL = 0, 1, 2, 3
 
 # Flags (include/cudabvh.h, part 2)
 ISECT_WATERTIGHT = 0
@@ -442,7 +442,7 @@ class BVH:
                  rays=None):
         """Sweeps the launch configurations of one ray class; returns the
         chosen (schedule, lds_depth, blocks_per_cu) and the table of timings."""
-        res = (C.c_float * 144)()
+        res = (C.c_float * 192)()
         nres = C.c_int()
         n = 0 if rays is None else int(rays["ox"].shape[0])
         rp = ray_packet(rays) if rays is not None else None
@@ -513,7 +513,7 @@ class Scene:
     def autotune(self, ray_class: int, n: int, rays, reps: int = 3):
         """Picks the fastest (schedule, LDS depth, blocks/CU) for these rays; returns (config, table)."""
         rp = ray_packet(rays)
-        res = np.zeros(4 * 36, np.float32)
+        res = np.zeros(4 * 48, np.float32)
         k = C.c_int()
         _check(self.lib.cudaBVH_sceneAutotune(self.handle, ray_class, n, C.byref(rp), reps, res.ctypes.data,
                                               C.byref(k)), "sceneAutotune")

@@ -769,7 +769,7 @@ int cudaBVH_setTraceConfig(CudaBVH_t bvh, int rayClass, int schedule, int ldsDep
     rt::clear_error();
     if (!check_handle(bvh, "cudaBVH_setTraceConfig")) return -1;
     if (rayClass < 0 || rayClass >= rt::kNumClasses || schedule < rt::kScheduleBatch ||
-        schedule > rt::kSchedulePacket || (schedule == rt::kSchedulePacket && rayClass != rt::kClassCoherent) ||
+        schedule > rt::kScheduleRefill || (schedule == rt::kSchedulePacket && rayClass != rt::kClassCoherent) ||
         (ldsDepth != 4 && ldsDepth != 8 && ldsDepth != 12 &&
         ldsDepth != 16) || blocksPerCU < 0) {
         set_error("cudaBVH_setTraceConfig: bad class %d / schedule %d / ldsDepth %d / blocksPerCU %d", rayClass,
@@ -820,7 +820,7 @@ int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const C
     if (rayClass == rt::kClassAny) base.occ = (uint8_t*)out;
     else { base.t = o; base.id = (int*)(o + n); base.u = o + 2 * size_t(n); base.v = o + 3 * size_t(n); }
     const int mode = rayClass == rt::kClassAny ? rtk::kModeAny : rtk::kModeClosest;
-    const int schedules[3] = {rt::kScheduleBatch, rt::kScheduleWhileWhile, rt::kSchedulePacket};
+    const int schedules[4] = {rt::kScheduleBatch, rt::kScheduleWhileWhile, rt::kSchedulePacket, rt::kScheduleRefill};
     const int depths[4] = {4, 8, 12, 16};
     const int bpcs[3] = {0, 4, 2};
     const rt::TraceConfig saved = bvh->tune[rayClass];
@@ -917,6 +917,7 @@ CudaScene_t cudaBVH_sceneCreate(const CudaBVH_t* blas, int numBlas, const CudaIn
     sc->tune2l[0].ldsDepth = 8;
     sc->tune2l[1].schedule = 0;
     sc->tune2l[1].ldsDepth = 8;
+    sc->tune2l[2] = sc->tune2l[0];
     sc->blas.assign(blas, blas + numBlas);
     sc->inst.resize(numInst);
     sc->o2w.resize(size_t(numInst) * 12);
@@ -998,7 +999,7 @@ int cudaBVH_sceneWorldToObject(CudaScene_t sc, float* out) {
 }
 
 static int scene_trace(CudaScene_t sc, int n, const int* d_count, const CudaRayPacket* rays, const CudaSceneHit* hits,
-                       uint8_t* occ, int* cn, int* ct, unsigned flags, bool any, const char* fn) {
+                       uint8_t* occ, int* cn, int* ct, unsigned flags, bool any, const char* fn, int cls = -1) {
     rt::clear_error();
     if (!scene_ok(sc, fn)) return -1;
     if (n == 0) return 0;
@@ -1011,7 +1012,7 @@ static int scene_trace(CudaScene_t sc, int n, const int* d_count, const CudaRayP
     if (hits) { a.t = hits->t; a.inst_out = hits->instId; a.id = hits->primId; a.u = hits->u; a.v = hits->v; }
     a.occ = occ; a.cntNodes = cn; a.cntTris = ct;
     hipStream_t s = (hipStream_t)sc->stream;
-    int rc = rtk::launch_trace2l(sc, sc->scratch, a, any, s);
+    int rc = rtk::launch_trace2l(sc, sc->scratch, a, cls >= 0 ? cls : (any ? 1 : 0), s);
     if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(s), return -1);
     return rc;
 }
@@ -1181,7 +1182,7 @@ int cudaBVH_batchTraceAny(CudaBVHBatch_t b, int totalRays, const int* d_rayOffse
 int cudaBVH_sceneSetTraceConfig(CudaScene_t sc, int rayClass, int schedule, int ldsDepth, int blocksPerCU) {
     rt::clear_error();
     if (!scene_ok(sc, "cudaBVH_sceneSetTraceConfig")) return -1;
-    if (rayClass < 0 || rayClass > 1 || schedule < 0 || schedule > (rayClass ? 2 : 1) ||
+    if (rayClass < 0 || rayClass > 2 || schedule < 0 || schedule > 3 || (schedule == 2 && rayClass != 1) ||
         (ldsDepth != 4 && ldsDepth != 8 && ldsDepth != 12 && ldsDepth != 16) || blocksPerCU < 0) {
         set_error("cudaBVH_sceneSetTraceConfig: class %d, schedule %d, LDS depth %d, blocks/CU %d not supported",
                   rayClass, schedule, ldsDepth, blocksPerCU);
@@ -1194,7 +1195,7 @@ int cudaBVH_sceneSetTraceConfig(CudaScene_t sc, int rayClass, int schedule, int 
 }
 
 int cudaBVH_sceneGetTraceConfig(CudaScene_t sc, int rayClass, int* schedule, int* ldsDepth, int* blocksPerCU) {
-    if (!scene_ok(sc, "cudaBVH_sceneGetTraceConfig") || rayClass < 0 || rayClass > 1) return -1;
+    if (!scene_ok(sc, "cudaBVH_sceneGetTraceConfig") || rayClass < 0 || rayClass > 2) return -1;
     if (schedule) *schedule = sc->tune2l[rayClass].schedule;
     if (ldsDepth) *ldsDepth = sc->tune2l[rayClass].ldsDepth;
     if (blocksPerCU) *blocksPerCU = sc->tune2l[rayClass].blocksPerCU;
@@ -1207,7 +1208,7 @@ int cudaBVH_sceneAutotune(CudaScene_t sc, int rayClass, int numRays, const CudaR
                           float* results, int* numResults) {
     rt::clear_error();
     if (!scene_ok(sc, "cudaBVH_sceneAutotune")) return -1;
-    if (rayClass < 0 || rayClass > 1 || numRays <= 0 || !packet_ok(rays) || reps <= 0) {
+    if (rayClass < 0 || rayClass > 2 || numRays <= 0 || !packet_ok(rays) || reps <= 0) {
         set_error("cudaBVH_sceneAutotune: bad arguments");
         return -1;
     }
@@ -1220,23 +1221,26 @@ int cudaBVH_sceneAutotune(CudaScene_t sc, int rayClass, int numRays, const CudaR
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     const int depths[4] = {4, 8, 12, 16}, bpcs[3] = {0, 4, 2};
-    const int nsched = rayClass ? 3 : 2;
+    // schedules: 0 while-while, 1 batch, 2 batch nearest-first (any hit), 3 refill
+    const int closestScheds[3] = {0, 1, 3}, anyScheds[4] = {0, 1, 2, 3};
+    const int* scheds = rayClass == 1 ? anyScheds : closestScheds;
+    const int nsched = rayClass == 1 ? 4 : 3;
     rt::TraceConfig keep = sc->tune2l[rayClass], best = keep;
     float bestMs = 1e30f;
     int k = 0, rc = 0;
     for (int si = 0; si < nsched && rc == 0; si++)
         for (int di = 0; di < 4 && rc == 0; di++)
             for (int bi = 0; bi < 3 && rc == 0; bi++) {
-                sc->tune2l[rayClass].schedule = si;
+                sc->tune2l[rayClass].schedule = scheds[si];
                 sc->tune2l[rayClass].ldsDepth = depths[di];
                 sc->tune2l[rayClass].blocksPerCU = bpcs[bi];
                 float ms = 1e30f;
                 for (int r = 0; r <= reps && rc == 0; r++) {      // the first launch warms up
                     hipEventRecord(e0, s);
-                    rc = rayClass ? scene_trace(sc, numRays, nullptr, rays, nullptr, (uint8_t*)buf, nullptr, nullptr, 0,
-                                                true, "cudaBVH_sceneAutotune")
-                                  : scene_trace(sc, numRays, nullptr, rays, &hit, nullptr, nullptr, nullptr, 0, false,
-                                                "cudaBVH_sceneAutotune");
+                    rc = rayClass == 1 ? scene_trace(sc, numRays, nullptr, rays, nullptr, (uint8_t*)buf, nullptr,
+                                                     nullptr, 0, true, "cudaBVH_sceneAutotune")
+                                       : scene_trace(sc, numRays, nullptr, rays, &hit, nullptr, nullptr, nullptr, 0,
+                                                     false, "cudaBVH_sceneAutotune", rayClass);
                     hipEventRecord(e1, s);
                     hipEventSynchronize(e1);
                     float t = 0.f;
@@ -1244,7 +1248,7 @@ int cudaBVH_sceneAutotune(CudaScene_t sc, int rayClass, int numRays, const CudaR
                     if (r > 0) ms = std::min(ms, t);
                 }
                 if (results) {
-                    results[4 * k] = (float)si;
+                    results[4 * k] = (float)scheds[si];
                     results[4 * k + 1] = (float)depths[di];
                     results[4 * k + 2] = (float)bpcs[bi];
                     results[4 * k + 3] = ms;

@@ -80,10 +80,13 @@ struct PtResources {
     int2* tailStack = nullptr;        // tail kernel: [depth][threads] stack entries
     size_t tailBytes = 0;
     hipStream_t own = nullptr;        // stream for graph capture when the scene has none
+    hipStream_t side = nullptr;       // shadow traces, concurrent with the next bounce's extension trace
+    hipEvent_t fork = nullptr, join = nullptr;
+    rt::TraceScratch scratch2;        // traversal scratch of the side stream
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     PtArgs key{};                     // arguments the graph was captured with
-    rt::TraceConfig keyTune[2];       // and the scene's traversal configuration
+    rt::TraceConfig keyTune[3];       // and the scene's traversal configuration
 };
 
 __device__ __forceinline__ void clamp_contrib(float c[3], float m) {
@@ -351,30 +354,47 @@ static int pt_grid(int n) {
     return max(1, min(want, cus * 8));
 }
 
+// Extension (closest-hit) trace of bounce b; the first bounce traces camera
+// rays (coherent, scene class 0), later ones scattered rays (class 2).
+static int pt_extend(CudaSceneHandle* sc, PtArgs& a, int b, hipStream_t s) {
+    const PtQueue& q = a.q[b & 1];
+    Trace2LArgs t;
+    t.n = a.N; t.nDev = &a.cnt[b];
+    t.ox = q.r[0]; t.oy = q.r[1]; t.oz = q.r[2]; t.dx = q.r[3]; t.dy = q.r[4]; t.dz = q.r[5];
+    t.tmin = q.r[6]; t.tmax = q.r[7];
+    t.t = a.ht; t.inst_out = a.hinst; t.id = a.hprim;
+    t.gate = &a.cnt[b]; t.gateMax = a.tailPaths;
+    return launch_trace2l(sc, sc->scratch, t, b == 0 ? 0 : 2, s);
+}
+
 // One sample: generate, then per bounce extend -> shade -> shadow -> connect,
-// then fold the counts. Every size is read on the device.
-static int pt_sample(CudaSceneHandle* sc, PtArgs& a, hipStream_t s) {
+// then fold the counts. Every size is read on the device. The shadow trace of
+// bounce b needs only shade(b), and the extension trace of bounce b + 1 needs
+// only shade(b) too, so they run side by side (the shadow trace on a second
+// stream with its own traversal scratch); connect(b), the tail kernel and
+// shade(b + 1) follow both, which keeps every pixel's additions in the
+// one-stream order.
+static int pt_sample(CudaSceneHandle* sc, PtResources& R, PtArgs& a, hipStream_t s) {
     const int g = pt_grid(a.N);
     hipLaunchKernelGGL(k_pt_generate, dim3(g), dim3(kPtBlock), 0, s, a);
+    if (pt_extend(sc, a, 0, s) != 0) return -1;
+    hipLaunchKernelGGL(k_pt_shade, dim3(g), dim3(kPtBlock), 0, s, a, 0);
     for (int b = 0; b < a.maxDepth; b++) {
-        const PtQueue& q = a.q[b & 1];
-        Trace2LArgs t;
-        t.n = a.N; t.nDev = &a.cnt[b];
-        t.ox = q.r[0]; t.oy = q.r[1]; t.oz = q.r[2]; t.dx = q.r[3]; t.dy = q.r[4]; t.dz = q.r[5];
-        t.tmin = q.r[6]; t.tmax = q.r[7];
-        t.t = a.ht; t.inst_out = a.hinst; t.id = a.hprim;
-        t.gate = &a.cnt[b]; t.gateMax = a.tailPaths;
-        if (launch_trace2l(sc, sc->scratch, t, false, s) != 0) return -1;
-        hipLaunchKernelGGL(k_pt_shade, dim3(g), dim3(kPtBlock), 0, s, a, b);
         Trace2LArgs u;
         u.n = a.N; u.nDev = &a.cnt[kPtMaxDepth + b];
         u.ox = a.s[0]; u.oy = a.s[1]; u.oz = a.s[2]; u.dx = a.s[3]; u.dy = a.s[4]; u.dz = a.s[5];
         u.tmin = a.s[6]; u.tmax = a.s[7];
         u.occ = a.occ;
         u.gate = &a.cnt[b]; u.gateMax = a.tailPaths;
-        if (launch_trace2l(sc, sc->scratch, u, true, s) != 0) return -1;
+        HIP_TRY(hipEventRecord(R.fork, s), return -1);
+        HIP_TRY(hipStreamWaitEvent(R.side, R.fork, 0), return -1);
+        if (launch_trace2l(sc, R.scratch2, u, 1, R.side) != 0) return -1;
+        HIP_TRY(hipEventRecord(R.join, R.side), return -1);
+        if (b + 1 < a.maxDepth && pt_extend(sc, a, b + 1, s) != 0) return -1;
+        HIP_TRY(hipStreamWaitEvent(s, R.join, 0), return -1);
         hipLaunchKernelGGL(k_pt_connect, dim3(g), dim3(kPtBlock), 0, s, a, b);
         if (a.tailPaths > 0) hipLaunchKernelGGL(k_pt_tail, dim3(a.tailStride / 64), dim3(64), 0, s, a, b);
+        if (b + 1 < a.maxDepth) hipLaunchKernelGGL(k_pt_shade, dim3(g), dim3(kPtBlock), 0, s, a, b + 1);
     }
     hipLaunchKernelGGL(k_pt_end, dim3(1), dim3(64), 0, s, a);
     const hipError_t e = hipGetLastError();
@@ -429,6 +449,10 @@ void pt_free(CudaSceneHandle* sc) {
     if (R->exec) hipGraphExecDestroy(R->exec);
     if (R->graph) hipGraphDestroy(R->graph);
     if (R->own) hipStreamDestroy(R->own);
+    if (R->side) hipStreamDestroy(R->side);
+    if (R->fork) hipEventDestroy(R->fork);
+    if (R->join) hipEventDestroy(R->join);
+    free_scratch(R->scratch2);
     hipFree(R->tailStack);
     hipFree(R->block);
     hipFree(R->small);
@@ -496,6 +520,16 @@ int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams&
         a.tailStack = R.tailStack;
     }
 
+    if (!R.side) {
+        if (hipStreamCreateWithFlags(&R.side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&R.fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&R.join, hipEventDisableTiming) != hipSuccess) {
+            rt::set_error("scenePathTrace: cannot create the shadow stream");
+            return -1;
+        }
+    }
+    if (alloc_scratch(R.scratch2) != 0) return -1;
+
     const bool graph = (flags & CUDABVH_FLAG_GRAPH) != 0;
     hipStream_t s = (hipStream_t)sc->stream;
     if (graph && !s) {
@@ -514,10 +548,10 @@ int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams&
             if (R.exec) { hipGraphExecDestroy(R.exec); R.exec = nullptr; }
             if (R.graph) { hipGraphDestroy(R.graph); R.graph = nullptr; }
             // one eager sample allocates every scratch buffer the traces need
-            if (pt_sample(sc, a, s) != 0) return -1;
+            if (pt_sample(sc, R, a, s) != 0) return -1;
             first = 1;
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), return -1);
-            const int rc = pt_sample(sc, a, s);
+            const int rc = pt_sample(sc, R, a, s);
             hipGraph_t gr = nullptr;
             const hipError_t ec = hipStreamEndCapture(s, &gr);
             if (rc != 0 || ec != hipSuccess) {
@@ -533,7 +567,7 @@ int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams&
         for (int k = first; k < p.spp; k++) HIP_TRY(hipGraphLaunch(R.exec, s), return -1);
     } else {
         for (int k = 0; k < p.spp; k++)
-            if (pt_sample(sc, a, s) != 0) return -1;
+            if (pt_sample(sc, R, a, s) != 0) return -1;
     }
     if (p.spp > 1) hipLaunchKernelGGL(k_pt_scale, dim3(pt_grid(N)), dim3(kPtBlock), 0, s, a, 1.0f / (float)p.spp);
     if (rayCounts) {

@@ -82,7 +82,7 @@ uint32_t morton3d(float x, float y, float z);
 // LDS stack depth per lane (4, 8, 12 or 16 entries) and resident workgroups
 // per CU (0 = as many as occupancy allows).
 enum { kClassCoherent = 0, kClassIncoherent = 1, kClassAny = 2, kNumClasses = 3 };
-enum { kScheduleBatch = 0, kScheduleWhileWhile = 1, kSchedulePacket = 2 };
+enum { kScheduleBatch = 0, kScheduleWhileWhile = 1, kSchedulePacket = 2, kScheduleRefill = 3 };
 struct TraceConfig {
     int schedule = kScheduleBatch;
     int ldsDepth = 16;

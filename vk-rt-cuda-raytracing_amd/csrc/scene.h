@@ -71,11 +71,13 @@ struct CudaSceneHandle {
     rt::TraceScratch scratch;
     void* stream = nullptr;
     void* pt = nullptr;                        // path-tracer queues and graph (pt.hip)
-    rt::TraceConfig tune2l[2];                 // launch configuration: closest hit, any hit
+    rt::TraceConfig tune2l[3];                 // launch configuration: closest hit, any hit, closest hit of
+                                               // the path tracer's secondary bounces
 };
 
 namespace rtk {
-int launch_trace2l(const CudaSceneHandle* sc, rt::TraceScratch& scratch, Trace2LArgs& a, bool any, hipStream_t s);
+// cls: 0 closest hit, 1 any hit, 2 closest hit with the secondary-bounce configuration
+int launch_trace2l(const CudaSceneHandle* sc, rt::TraceScratch& scratch, Trace2LArgs& a, int cls, hipStream_t s);
 int trace2l_max_stack(const CudaSceneHandle* sc);
 int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams& p, const CudaPathImage& img,
                long long* rayCounts, unsigned flags);

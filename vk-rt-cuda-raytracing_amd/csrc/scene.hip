@@ -235,6 +235,158 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l_batch(Trace2LArgs a) {
     }
 }
 
+// Refill schedule: trace2l_ray's one-short-step loop (a node visit, an
+// instance entry or one triangle of the current BLAS leaf per iteration),
+// with lanes whose ray finished refilled from the wave's pool of 64 rays once
+// at least kRefill2Min lanes are idle. Uncounted.
+constexpr int kRefill2Min = 16;
+
+template <bool ANY, int LDSD, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace2l_refill(Trace2LArgs a) {
+    __shared__ int s_ref[LDSD * BLOCK];
+    __shared__ float s_tn[LDSD * BLOCK];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int gtid = blockIdx.x * BLOCK + tid;
+    if (a.gate && *a.gate <= a.gateMax) return;
+    const int n = a.nDev ? min(*a.nDev, a.n) : a.n;
+
+    Ray rw, ro;
+    int ri = 0, inst = -1;
+    const uint4* nodes = a.tlasNodes;
+    const float4* tris = nullptr;
+    bool has = false, exhausted = false, flatW = false, flatO = false;
+    int poolNext = 0, poolEnd = 0;
+    int sp = 0, ref = -1, bid = -1, binst = -1, lts = 0, ltc = 0;
+    float tn = 0.f, best = 0.f, bu = 0.f, bv = 0.f;
+
+    auto push = [&](int rf, float t) {
+        if (sp < LDSD) {
+            s_ref[sp * BLOCK + tid] = rf;
+            s_tn[sp * BLOCK + tid] = t;
+        } else if (sp < a.maxStack) {
+            a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid] = make_int2(rf, __float_as_int(t));
+        } else {
+            atomicOr(a.status, 1);
+            return;
+        }
+        sp++;
+    };
+
+    for (;;) {
+        if (!exhausted || poolNext < poolEnd) {
+            const uint64_t idle = __ballot(!has);
+            if (idle && (__popcll(idle) >= kRefill2Min || !__any(has))) {
+                const int cnt = __popcll(idle);
+                const int avail = poolEnd - poolNext;
+                int fresh = n;
+                if (cnt > avail && !exhausted) {
+                    uint32_t b = 0;
+                    if (lane == 0) b = atomicAdd(a.counter, 64u);
+                    fresh = min(__builtin_amdgcn_readfirstlane((int)b), n);
+                    if (fresh + 64 >= n) exhausted = true;
+                }
+                const int k = __popcll(idle & ((1ull << lane) - 1ull));
+                int w = n;
+                if (k < avail) w = poolNext + k;
+                else if (fresh < n) w = fresh + (k - avail);
+                if (cnt > avail) {
+                    if (fresh < n) { poolNext = fresh + (cnt - avail); poolEnd = fresh + 64; }
+                    else { poolNext = 0; poolEnd = 0; }
+                } else {
+                    poolNext += cnt;
+                }
+                if (!has && w < n) {
+                    ri = w;
+                    rw.ox = a.ox[ri]; rw.oy = a.oy[ri]; rw.oz = a.oz[ri];
+                    rw.dx = a.dx[ri]; rw.dy = a.dy[ri]; rw.dz = a.dz[ri];
+                    rw.tmin = a.tmin[ri]; rw.tmax = a.tmax[ri];
+                    rtd::ray_setup(rw);
+                    flatW = rtd::ray_flat(rw);
+                    has = true;
+                    sp = 0; ref = 0; tn = -INFINITY; inst = -1; nodes = a.tlasNodes; ltc = 0;
+                    best = rw.tmax; bu = 0.f; bv = 0.f; bid = -1; binst = -1;
+                }
+            }
+        }
+        if (!__any(has)) {
+            if (exhausted && poolNext >= poolEnd) break;
+            continue;
+        }
+        if (!has) continue;
+        bool fin = false, found = false;
+        if (ltc == 0) {
+            while (ref == -1 || ref == kReturnMark || (!ANY && tn > best)) {
+                if (ref == kReturnMark) { inst = -1; nodes = a.tlasNodes; ref = -1; continue; }
+                if (sp == 0) { fin = true; break; }
+                sp--;
+                if (sp < LDSD) {
+                    ref = s_ref[sp * BLOCK + tid];
+                    tn = s_tn[sp * BLOCK + tid];
+                } else {
+                    const int2 e = a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid];
+                    ref = e.x; tn = __int_as_float(e.y);
+                }
+            }
+            if (!fin && ref < 0 && inst >= 0) {      // BLAS leaf: start its cursor
+                const int enc = -(ref + 2);
+                lts = enc >> 3; ltc = (enc & 7) + 1;
+                ref = -1;
+            }
+        }
+        if (!fin) {
+            if (ltc > 0) {
+                const int ti = lts;
+                lts++; ltc--;
+                const float4 p0 = tris[3 * ti];
+                float t, u, v;
+                if (rtd::isect_watertight(ro, p0, tris[3 * ti + 1], tris[3 * ti + 2], t, u, v) && t > ro.tmin &&
+                    t < ro.tmax) {
+                    if constexpr (ANY) {
+                        found = true;
+                        fin = true;
+                    } else {
+                        const int id = __float_as_int(p0.w);
+                        if (t < best || (t == best && (inst < binst || (inst == binst && id < bid)))) {
+                            best = t; bid = id; binst = inst; bu = u; bv = v;
+                        }
+                    }
+                }
+            } else if (ref >= 0) {
+                const bool w = inst < 0;
+                const Ray& r = w ? rw : ro;
+                visit2w<ANY>(nodes, r, w ? flatW : flatO, ANY ? r.tmax : best, ref, tn, push);
+            } else {
+                // TLAS leaf: enter its instance
+                const int enc = -(ref + 2);
+                const int k = __float_as_int(a.tlasTris[3 * (enc >> 3)].w);
+                push(kReturnMark, -INFINITY);
+                inst = k;
+                const InstanceDev& I = a.inst[k];
+                xform(I.w2o, rw, ro);
+                flatO = rtd::ray_flat(ro);
+                const BlasDev& B = a.blas[I.blas];
+                nodes = B.nodes;
+                tris = B.tris;
+                ref = 0;
+                tn = -INFINITY;
+            }
+        }
+        if (fin) {
+            if constexpr (ANY) {
+                a.occ[ri] = found ? 1 : 0;
+            } else {
+                const bool hit = bid >= 0;
+                a.t[ri] = hit ? best : RT_MISS_T;
+                if (a.inst_out) a.inst_out[ri] = hit ? binst : -1;
+                if (a.id) a.id[ri] = hit ? bid : -1;
+                if (a.u) a.u[ri] = hit ? bu : 0.f;
+                if (a.v) a.v[ri] = hit ? bv : 0.f;
+            }
+            has = false;
+        }
+    }
+}
+
 constexpr int kBlock2 = 256;
 
 __global__ void k_reset_counter_2l(uint32_t* c) { c[0] = 0u; }
@@ -242,10 +394,11 @@ __global__ void k_reset_counter_2l(uint32_t* c) { c[0] = 0u; }
 template <bool ANY, int LDSD, int SCHED>
 static int launch2(Trace2LArgs& a, rt::TraceScratch& sc, int maxStack, int blocksPerCU, hipStream_t s) {
     static int maxBlocks = 0;
-    // SCHED: 0 while-while, 1 batch, 2 batch with nearest-first any hit
-    auto kern = SCHED == 0 ? k_trace2l<ANY, LDSD, kBlock2>
-                           : (SCHED == 1 ? k_trace2l_batch<ANY, false, LDSD, kBlock2>
-                                         : k_trace2l_batch<ANY, true, LDSD, kBlock2>);
+    // SCHED: 0 while-while, 1 batch, 2 batch with nearest-first any hit, 3 refill
+    auto kern = SCHED == 0   ? k_trace2l<ANY, LDSD, kBlock2>
+                : SCHED == 1 ? k_trace2l_batch<ANY, false, LDSD, kBlock2>
+                : SCHED == 2 ? k_trace2l_batch<ANY, true, LDSD, kBlock2>
+                             : k_trace2l_refill<ANY, LDSD, kBlock2>;
     if (!maxBlocks) {
         int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock2, 0) != hipSuccess || nb <= 0) nb = 1;
@@ -281,21 +434,24 @@ static int launch2(Trace2LArgs& a, rt::TraceScratch& sc, int maxStack, int block
 // most 3 entries along a root-to-leaf path, plus the return marker.
 int trace2l_max_stack(const CudaSceneHandle* sc) { return 3 * sc->tlas->depth4 + 4 + 1 + 3 * sc->maxBlasDepth + 4; }
 
-int launch_trace2l(const CudaSceneHandle* sc, rt::TraceScratch& scratch, Trace2LArgs& a, bool any, hipStream_t s) {
+int launch_trace2l(const CudaSceneHandle* sc, rt::TraceScratch& scratch, Trace2LArgs& a, int cls, hipStream_t s) {
+    const bool any = cls == 1;
     if (a.n <= 0) return 0;
     a.tlasNodes = (const uint4*)sc->tlas->d_nodes4;
     a.tlasTris = (const float4*)sc->tlas->d_tris;
     a.inst = sc->d_inst;
     a.blas = sc->d_blas;
     const int maxStack = trace2l_max_stack(sc);
-    const rt::TraceConfig& c = sc->tune2l[any ? 1 : 0];
+    const rt::TraceConfig& c = sc->tune2l[cls >= 0 && cls <= 2 ? cls : 0];
     // counted traces always use the while-while kernel (it keeps the counters)
-    const int sched = (a.cntNodes || c.schedule < 0 || c.schedule > 2 || (!any && c.schedule == 2)) ? 0 : c.schedule;
+    const int sched = (a.cntNodes || c.schedule < 0 || c.schedule > 3 || (!any && c.schedule == 2)) ? 0 : c.schedule;
 #define RT_L2(D)                                                                                                  \
     if (sched == 0) return any ? launch2<true, D, 0>(a, scratch, maxStack, c.blocksPerCU, s)                     \
                                : launch2<false, D, 0>(a, scratch, maxStack, c.blocksPerCU, s);                   \
     if (sched == 1) return any ? launch2<true, D, 1>(a, scratch, maxStack, c.blocksPerCU, s)                     \
                                : launch2<false, D, 1>(a, scratch, maxStack, c.blocksPerCU, s);                   \
+    if (sched == 3) return any ? launch2<true, D, 3>(a, scratch, maxStack, c.blocksPerCU, s)                     \
+                               : launch2<false, D, 3>(a, scratch, maxStack, c.blocksPerCU, s);                   \
     return launch2<true, D, 2>(a, scratch, maxStack, c.blocksPerCU, s);
     switch (c.ldsDepth) {
         case 4: { RT_L2(4) }

@@ -215,6 +215,165 @@ __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int*
     res.occluded = found; res.visits = visits; res.tests = tests;
 }
 
+// Refill schedule: the batch schedule's one-short-step loop (a node visit or
+// one triangle per iteration, as trace_ray), but a lane whose ray finished
+// takes the next work item from the wave's pool of 64 (one 8x8 tile for
+// pinhole rays; one atomicAdd per 64 items) once at least kRefillMin lanes
+// are idle, so a wave does not wait for the slowest ray of its batch and
+// the ray set-up of new rays is shared by many lanes. Same depth-first order
+// per ray as trace_ray; uncounted.
+constexpr int kRefillMin = 16;
+
+template <int MODE, bool MT, bool PIN, int LDSD, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace_refill(TraceArgs a) {
+    __shared__ int s_ref[LDSD * BLOCK];
+    __shared__ float s_tn[LDSD * BLOCK];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int gtid = blockIdx.x * BLOCK + tid;
+    const int n = PIN ? a.nTiles * 64 : (a.nDev ? min(*a.nDev, a.n) : a.n);
+    const uint4* __restrict__ nodes = a.nodes;
+    const float4* __restrict__ tris = a.tris;
+    Ray r;
+    int ri = 0, py = 0;
+    bool has = false, exhausted = false, flat = false;
+    int poolNext = 0, poolEnd = 0;
+    int sp = 0, ref = -1, bid = -1, bti = -1, lts = 0, ltc = 0;
+    float tn = 0.f, best = 0.f, bu = 0.f, bv = 0.f;
+
+    auto push = [&](int rf, float t) {
+        if (sp < LDSD) {
+            s_ref[sp * BLOCK + tid] = rf;
+            s_tn[sp * BLOCK + tid] = t;
+        } else if (sp < a.maxStack) {
+            a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid] = make_int2(rf, __float_as_int(t));
+        } else {
+            atomicOr(a.status, 1);
+            return;
+        }
+        sp++;
+    };
+
+    for (;;) {
+        if (!exhausted || poolNext < poolEnd) {
+            const uint64_t idle = __ballot(!has);
+            if (idle && (__popcll(idle) >= kRefillMin || !__any(has))) {
+                const int cnt = __popcll(idle);
+                const int avail = poolEnd - poolNext;
+                int fresh = n;
+                if (cnt > avail && !exhausted) {
+                    uint32_t b = 0;
+                    if (lane == 0) b = atomicAdd(a.counter, 64u);
+                    fresh = min(__builtin_amdgcn_readfirstlane((int)b), n);
+                    if (fresh + 64 >= n) exhausted = true;
+                }
+                const int k = __popcll(idle & ((1ull << lane) - 1ull));
+                int w = n;
+                if (k < avail) w = poolNext + k;
+                else if (fresh < n) w = fresh + (k - avail);
+                if (cnt > avail) {
+                    if (fresh < n) { poolNext = fresh + (cnt - avail); poolEnd = fresh + 64; }
+                    else { poolNext = 0; poolEnd = 0; }
+                } else {
+                    poolNext += cnt;
+                }
+                if (!has) {
+                    bool live = w < n;
+                    if constexpr (PIN) {
+                        const int tile = w >> 6, kk = w & 63;
+                        const int ty = tile / a.tilesX, tx = tile - ty * a.tilesX;
+                        const int px = tx * 8 + (kk & 7);
+                        py = ty * 8 + (kk >> 3);
+                        live = live && px < a.W && py < a.H;
+                        ri = py * a.W + px;
+                        if (live) rtd::pinhole_ray(a.cam, px, py, a.W, a.H, r);
+                    } else {
+                        ri = w;
+                        if (live) {
+                            r.ox = a.ox[ri]; r.oy = a.oy[ri]; r.oz = a.oz[ri];
+                            r.dx = a.dx[ri]; r.dy = a.dy[ri]; r.dz = a.dz[ri];
+                            r.tmin = a.tmin[ri]; r.tmax = a.tmax[ri];
+                        }
+                    }
+                    if (live) {
+                        rtd::ray_setup(r);
+                        flat = rtd::ray_flat(r);
+                        has = true;
+                        sp = 0; ref = ray_root(a, ri); tn = -INFINITY; ltc = 0;
+                        best = r.tmax; bu = 0.f; bv = 0.f; bid = -1; bti = -1;
+                    }
+                }
+            }
+        }
+        if (!__any(has)) {
+            if (exhausted && poolNext >= poolEnd) break;
+            continue;
+        }
+        const bool waveFlat = __any(has && flat);
+        if (!has) continue;
+        bool cont = false, fin = false, found = false;
+        if (ltc == 0 && ref < 0 && (MODE == kModeAny || tn <= best)) {
+            const int enc = -(ref + 2);
+            lts = enc >> 3; ltc = (enc & 7) + 1;
+        }
+        if (ltc > 0) {
+            const int ti = lts;
+            lts++; ltc--;
+            const float4 p0 = tris[3 * ti];
+            float t, u, v;
+            if (rtd::isect<MT>(r, p0, tris[3 * ti + 1], tris[3 * ti + 2], t, u, v) && t > r.tmin && t < r.tmax) {
+                if constexpr (MODE == kModeAny) {
+                    found = true;
+                    fin = true;
+                } else {
+                    const int id = __float_as_int(p0.w);
+                    if (t < best || (t == best && id < bid)) { best = t; bid = id; bu = u; bv = v; bti = ti; }
+                }
+            }
+            if (!fin && ltc > 0) cont = true;
+        } else if (ref >= 0 && (MODE == kModeAny || tn <= best)) {
+            int nxt = ref;
+            float nt = 0.f;
+            if constexpr (MODE == kModeAny) {
+                if (waveFlat) visit_any<true>(nodes, r, nxt, push);
+                else visit_any<false>(nodes, r, nxt, push);
+            } else {
+                if (waveFlat) visit_closest<true>(nodes, r, best, nxt, nt, push);
+                else visit_closest<false>(nodes, r, best, nxt, nt, push);
+            }
+            if (nxt != -1) { ref = nxt; tn = nt; cont = true; }
+        }
+        if (!cont && !fin) {
+            if (sp == 0) {
+                fin = true;
+            } else {
+                sp--;
+                if (sp < LDSD) {
+                    ref = s_ref[sp * BLOCK + tid];
+                    tn = s_tn[sp * BLOCK + tid];
+                } else {
+                    const int2 e = a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid];
+                    ref = e.x; tn = __int_as_float(e.y);
+                }
+            }
+        }
+        if (fin) {
+            if constexpr (MODE == kModeAny) {
+                a.occ[ri] = found ? 1 : 0;
+            } else if constexpr (MODE == kModeClosest) {
+                const bool hit = bid >= 0;
+                a.t[ri] = hit ? best : RT_MISS_T;
+                if (a.id) a.id[ri] = bid;
+                if (a.u) a.u[ri] = hit ? bu : 0.f;
+                if (a.v) a.v[ri] = hit ? bv : 0.f;
+            } else {
+                shade_pixel(a, r, ri, py, bid >= 0, bti);
+            }
+            has = false;
+        }
+    }
+}
+
 // Batch schedule: each wave takes 64 rays (one 8x8 pixel tile for pinhole
 // rays) with one atomicAdd and runs until all 64 are done. Best for coherent
 // rays, whose lanes stay in step.
@@ -704,10 +863,13 @@ static int launch_kernel(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs
     return 0;
 }
 
-template <int MODE, bool MT, bool PIN, bool COUNT, int LDSD, bool WW>
+// SCHED: rt::kScheduleBatch, kScheduleWhileWhile or kScheduleRefill (uncounted)
+template <int MODE, bool MT, bool PIN, bool COUNT, int LDSD, int SCHED>
 static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int blocksPerCU, hipStream_t s) {
     static int maxBlocks = 0;
-    auto kern = WW ? k_trace_ww<MODE, MT, PIN, COUNT, LDSD, kBlock> : k_trace_batch<MODE, MT, PIN, COUNT, LDSD, kBlock>;
+    auto kern = SCHED == rt::kScheduleWhileWhile ? k_trace_ww<MODE, MT, PIN, COUNT, LDSD, kBlock>
+              : (SCHED == rt::kScheduleRefill && !COUNT) ? k_trace_refill<MODE, MT, PIN, LDSD, kBlock>
+                                                         : k_trace_batch<MODE, MT, PIN, COUNT, LDSD, kBlock>;
     return launch_kernel(h, sc, a, kern, LDSD, false, blocksPerCU, maxBlocks, s);
 }
 
@@ -737,14 +899,14 @@ void free_scratch(rt::TraceScratch& sc) {
 // Tunable variants (LDS stack depth, resident blocks per CU) exist for the
 // watertight closest/any kernels; the counted, Moller-Trumbore and shading
 // kernels use the default configuration.
-template <int MODE, bool PIN, bool WW>
+template <int MODE, bool PIN, int SCHED>
 static int launch_depth(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, const rt::TraceConfig& cfg,
                         hipStream_t s) {
     switch (cfg.ldsDepth) {
-        case 4: return launch_one<MODE, false, PIN, false, 4, WW>(h, sc, a, cfg.blocksPerCU, s);
-        case 8: return launch_one<MODE, false, PIN, false, 8, WW>(h, sc, a, cfg.blocksPerCU, s);
-        case 12: return launch_one<MODE, false, PIN, false, 12, WW>(h, sc, a, cfg.blocksPerCU, s);
-        default: return launch_one<MODE, false, PIN, false, kDefaultLds, WW>(h, sc, a, cfg.blocksPerCU, s);
+        case 4: return launch_one<MODE, false, PIN, false, 4, SCHED>(h, sc, a, cfg.blocksPerCU, s);
+        case 8: return launch_one<MODE, false, PIN, false, 8, SCHED>(h, sc, a, cfg.blocksPerCU, s);
+        case 12: return launch_one<MODE, false, PIN, false, 12, SCHED>(h, sc, a, cfg.blocksPerCU, s);
+        default: return launch_one<MODE, false, PIN, false, kDefaultLds, SCHED>(h, sc, a, cfg.blocksPerCU, s);
     }
 }
 
@@ -761,8 +923,9 @@ static int launch_tuned(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs&
             }
         }
     }
-    return cfg.schedule == rt::kScheduleWhileWhile ? launch_depth<MODE, PIN, true>(h, sc, a, cfg, s)
-                                                   : launch_depth<MODE, PIN, false>(h, sc, a, cfg, s);
+    if (cfg.schedule == rt::kScheduleWhileWhile) return launch_depth<MODE, PIN, rt::kScheduleWhileWhile>(h, sc, a, cfg, s);
+    if (cfg.schedule == rt::kScheduleRefill) return launch_depth<MODE, PIN, rt::kScheduleRefill>(h, sc, a, cfg, s);
+    return launch_depth<MODE, PIN, rt::kScheduleBatch>(h, sc, a, cfg, s);
 }
 
 int launch_trace(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int mode, bool mt, bool pinhole,
@@ -776,7 +939,8 @@ int launch_trace(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int
         if (mode == kModeAny) return pinhole ? launch_tuned<kModeAny, true>(h, sc, a, cfg, s)
                                              : launch_tuned<kModeAny, false>(h, sc, a, cfg, s);
     }
-#define RT_L(M, MTv, P, C) launch_one<M, MTv, P, C, kDefaultLds, !P>(h, sc, a, 0, s)
+#define RT_L(M, MTv, P, C) \
+    launch_one<M, MTv, P, C, kDefaultLds, P ? rt::kScheduleBatch : rt::kScheduleWhileWhile>(h, sc, a, 0, s)
     if (mode == kModeClosest) {
         if (mt) return pinhole ? (counted ? RT_L(kModeClosest, true, true, true) : RT_L(kModeClosest, true, true, false))
                                : (counted ? RT_L(kModeClosest, true, false, true) : RT_L(kModeClosest, true, false, false));
