@@ -130,7 +130,7 @@ def main():
                                   ("incoherent_closest", pkg.CLASS_INCOHERENT, dict(rays=Dm)),
                                   ("shadow_any", pkg.CLASS_ANY, dict(rays=Sm))):
                 best, table = bvh.autotune(cls, reps=3, **kw)
-                sched = ("batch", "while-while", "packet", "refill")
+                sched = SCHED_NAMES
                 tuning[name] = {"schedule": sched[best[0]], "lds_depth": best[1], "blocks_per_cu": best[2],
                                 "sweep_ms": {f"{sched[sc]}/{d}/{b}": round(ms, 4) for sc, d, b, ms in table}}
                 log(f"autotune {name}: {sched[best[0]]} lds {best[1]} blocks/CU {best[2] or 'max'}  "
@@ -228,7 +228,8 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(bvh, Pr, D, m, args.cpu_sample)
 
-    prof_traffic = load_traffic()
+    ic = tuning.get("incoherent_closest")
+    prof_traffic = load_traffic(f"{ic['schedule']}/{ic['lds_depth']}/{ic['blocks_per_cu']}") if ic else None
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -269,6 +270,7 @@ def main():
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": prof_traffic,
+            "launch_config": (f"{ic['schedule']}/{ic['lds_depth']}/{ic['blocks_per_cu']}" if ic else "default"),
             "alg_bytes_per_launch": diff_bytes, "kernel_ms": round(diff_ms, 4),
         },
         "cpu_baseline": cpu,
@@ -347,26 +349,34 @@ def bench_path_tracer(pkg, scenes, dev, args):
     alb = torch.empty(3 * n, device=dev)
     nrm = torch.empty(3 * n, device=dev)
     dep = torch.empty(n, device=dev)
-    p = pkg.path_params(W, H, spp=1, seed=1)
-    cnt = gs.path_trace(c, p, rgb, alb, nrm, dep, counts=True, flags=pkg.FLAG_GRAPH)
     K = 16
-    p = pkg.path_params(W, H, spp=K, seed=2)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record()
-    gs.path_trace(c, p, rgb, alb, nrm, dep, flags=pkg.FLAG_GRAPH)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / K
+    timed = {}
+    for name, extra in (("unsorted", 0), ("octant_sorted", pkg.FLAG_SORT_RAYS)):
+        # one graph capture per variant (sorting adds kernels), then K replayed samples
+        p = pkg.path_params(W, H, spp=1, seed=1)
+        cnt = gs.path_trace(c, p, rgb, alb, nrm, dep, counts=True, flags=pkg.FLAG_GRAPH | extra)
+        p = pkg.path_params(W, H, spp=K, seed=2)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        gs.path_trace(c, p, rgb, alb, nrm, dep, flags=pkg.FLAG_GRAPH | extra)
+        e1.record()
+        torch.cuda.synchronize()
+        timed[name] = e0.elapsed_time(e1) / K
+    best = min(timed, key=timed.get)
+    ms = timed[best]
     rays = int(cnt.sum())
     d = p.maxDepth
-    log(f"path tracer: {ms:.3f} ms/sample, {rays} rays/sample, {rays / (ms * 1e3):.1f} Mrays/s")
+    log(f"path tracer: {ms:.3f} ms/sample ({best}; " + ", ".join(f"{k} {v:.3f}" for k, v in timed.items())
+        + f"), {rays} rays/sample, {rays / (ms * 1e3):.1f} Mrays/s")
     gs.destroy()
     return {"workload": "64 instances of 4 heightfield meshes (10,082/49,928/199,712/500,000 tris), 1920x1080, "
                         "1 spp, depth 8, RR from ray 3, sun NEE + sky, hipGraph replay",
             "instanced_triangles": int(sum(len(sc["meshes"][b]) for b in sc["blas_of"])),
             "unique_triangles": int(sum(len(m) for m in sc["meshes"])),
             "ms_per_sample": round(ms, 3), "rays_per_sample": rays, "mrays_s": round(rays / (ms * 1e3), 1),
+            "extension_ray_order": best,
+            "ms_per_sample_by_order": {k: round(v, 3) for k, v in timed.items()},
             "extension_rays_per_bounce": [int(x) for x in cnt[:d]],
             "shadow_rays_per_bounce": [int(x) for x in cnt[d:]],
             "build_s": round(build_s, 2), "tuning": tuning}
@@ -531,40 +541,46 @@ def bench_denoiser(pkg, scenes, dev, args, pt_tuning=None):
 
 
 # Launch configurations the autotuner picks on the 1080p terrain frame on
-# MI355X (schedule, LDS stack depth, workgroups per CU; 0 = as many as fit).
-PMC_FRAME_CONFIG = (("CLASS_COHERENT", "SCHEDULE_BATCH", 12, 0),
-                    ("CLASS_INCOHERENT", "SCHEDULE_WHILE_WHILE", 12, 0),
-                    ("CLASS_ANY", "SCHEDULE_BATCH", 12, 4))
+# MI355X (schedule, LDS stack depth, workgroups per CU; 0 = as many as fit);
+# for the incoherent class it alternates between close candidates from run to
+# run, so each of them gets its own PMC frames.
+SCHED_NAMES = ("batch", "while-while", "packet", "refill")
+PMC_FRAME_CONFIG = (("CLASS_COHERENT", 0, 12, 0), ("CLASS_ANY", 0, 8, 4))
+PMC_INCOHERENT = ((3, 8, 0), (3, 12, 0), (1, 12, 0))
 
 
 def pmc_frame(pkg, bvh, step, dev, cnt):
-    """Two frames of the benchmark between two plain device copies of
-    512 MiB. The copies calibrate FETCH_SIZE / WRITE_SIZE (a copy of B bytes
-    reads B and writes B); the frames give the per-dispatch counters of the
-    traversal kernels with the launch configurations the benchmark times."""
+    """Frames of the benchmark between two plain device copies of 512 MiB.
+    The copies calibrate FETCH_SIZE / WRITE_SIZE (a copy of B bytes reads B
+    and writes B); the frames give the per-dispatch counters of the traversal
+    kernels with the launch configurations the benchmark times (two frames
+    per incoherent candidate; each candidate is a different kernel)."""
     import torch
     for cls, sched, depth, bpc in PMC_FRAME_CONFIG:
-        bvh.set_trace_config(getattr(pkg, cls), getattr(pkg, sched), depth, bpc)
+        bvh.set_trace_config(getattr(pkg, cls), sched, depth, bpc)
     x = torch.ones(128 << 20, device=dev)
     y = torch.empty_like(x)
     y.copy_(x)
     torch.cuda.synchronize()
-    for _ in range(2):
-        step()
+    for sched, depth, bpc in PMC_INCOHERENT:
+        bvh.set_trace_config(pkg.CLASS_INCOHERENT, sched, depth, bpc)
+        for _ in range(2):
+            step()
     torch.cuda.synchronize()
     y.copy_(x)
     torch.cuda.synchronize()
     log(f"pmc frame: {bvh.num_bvh4_nodes} nodes, {int(cnt.item())} secondary rays, copy {x.numel() * 4} bytes")
 
 
-def load_traffic():
-    """HBM bytes per launch of the dominant kernel, measured with rocprofv3
-    PMC counters and committed under profiles/ (null until measured)."""
+def load_traffic(config):
+    """HBM bytes per launch of the dominant kernel in launch configuration
+    `config` ("schedule/lds_depth/blocks_per_cu"), measured with rocprofv3 PMC
+    counters and committed under profiles/ (null when not measured)."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("incoherent_closest_bytes_per_launch")
+        return json.load(f).get("incoherent_closest_bytes_per_launch", {}).get(config)
 
 
 def cpu_baseline(bvh, Pr, D, m, sample, budget_s=10.0):

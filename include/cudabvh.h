@@ -357,7 +357,9 @@ typedef struct {
  * traced per bounce, then shadow rays per bounce, summed over samples (the
  * call then waits for completion). With CUDABVH_FLAG_GRAPH one sample's launch
  * sequence is captured once and replayed (re-captured when sizes, buffers or
- * parameters other than seed and spp change). */
+ * parameters other than seed and spp change). CUDABVH_FLAG_SORT_RAYS groups
+ * each bounce's next extension rays by direction octant before they are
+ * traced (same image). */
 int cudaBVH_scenePathTrace(CudaScene_t scene, const CudaCamera* camera, const CudaPathParams* params,
                            const CudaPathImage* image, long long* rayCounts, unsigned flags);
 

@@ -36,9 +36,11 @@ def _scene(pkg, sc):
 
 # tail: paths at which one thread per path takes over (0: wavefront only; 200:
 # from a later bounce on; 32768: this small image is a tail from the start)
+# flags: 1 << 7 hipGraph replay, 1 << 5 octant sort of the extension rays
 @pytest.mark.parametrize("spp,depth,rr,flags,tail", [(1, 8, 3, 0, 0), (4, 8, 3, 0, 0), (2, 3, 1, 0, 0),
                                                      (3, 8, 3, 1 << 7, 0), (2, 8, 3, 0, 200), (2, 8, 3, 0, 32768),
-                                                     (3, 8, 3, 1 << 7, 200)])
+                                                     (3, 8, 3, 1 << 7, 200), (2, 8, 3, 1 << 5, 0),
+                                                     (3, 8, 3, (1 << 7) | (1 << 5), 200), (1, 2, 1, 1 << 5, 0)])
 def test_path_tracer_parity(pkg, oracle, scenes, dev, spp, depth, rr, flags, tail):
     print(f"pt spp={spp} depth={depth} rr={rr} flags={flags} tail={tail}", flush=True)
     sc, cam = tiny_pt_scene(scenes)

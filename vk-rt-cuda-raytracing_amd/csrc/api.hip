@@ -1269,7 +1269,7 @@ int cudaBVH_scenePathTrace(CudaScene_t sc, const CudaCamera* cam, const CudaPath
     rt::clear_error();
     if (!scene_ok(sc, "cudaBVH_scenePathTrace")) return -1;
     if (!cam || !params || !image) { set_error("cudaBVH_scenePathTrace: NULL argument"); return -1; }
-    if (flags & ~(unsigned)(CUDABVH_FLAG_SYNC | CUDABVH_FLAG_GRAPH)) {
+    if (flags & ~(unsigned)(CUDABVH_FLAG_SYNC | CUDABVH_FLAG_GRAPH | CUDABVH_FLAG_SORT_RAYS)) {
         set_error("cudaBVH_scenePathTrace: unsupported flags 0x%x", flags);
         return -1;
     }

@@ -47,6 +47,9 @@ struct PtQueue {
 
 struct PtArgs {
     PtQueue q[2];         // extension queues (ping-pong by bounce parity)
+    PtQueue t;            // staging queue of the octant sort (shade appends here, the sort scatters to q)
+    int* oct;             // [b * 8 + o] rays of bounce b + 1 in direction octant o; [128 + b * 8 + o] sort cursors
+    int sortOct;          // sort each bounce's next extension rays by direction octant
     float* s[8];          // shadow rays
     float* sc[3];         // shadow contribution
     int* spix;
@@ -112,6 +115,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_generate(PtArgs a) {
     const int stride = gridDim.x * kPtBlock;
     const int g = blockIdx.x * kPtBlock + threadIdx.x;
     if (g < 2 * kPtMaxDepth) a.cnt[g] = g == 0 ? a.N : 0;
+    if (g < 16 * kPtMaxDepth) a.oct[g] = 0;
     for (int i = g; i < a.N; i += stride) {
         const int py = i / a.W, px = i - py * a.W;
         rtd::Pcg32 rng;
@@ -229,8 +233,11 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(PtArgs a, int b) {
     const int n = min(a.cnt[b], a.N);
     const bool aov = b == 0 && a.frame[0] == 0;
     const PtQueue& q = a.q[b & 1];
-    const PtQueue& nq = a.q[(b + 1) & 1];
+    const PtQueue& nq = a.sortOct ? a.t : a.q[(b + 1) & 1];
     const int stride = gridDim.x * kPtBlock;
+    __shared__ int octCount[8];     // this block's next rays per direction octant
+    if (threadIdx.x < 8) octCount[threadIdx.x] = 0;
+    __syncthreads();
     for (int base = blockIdx.x * kPtBlock; base < n; base += stride) {
         const int i = base + (int)threadIdx.x;
         const bool act = i < n;
@@ -263,6 +270,84 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_shade(PtArgs a, int b) {
             nq.thr[0][np] = thr[0]; nq.thr[1][np] = thr[1]; nq.thr[2][np] = thr[2];
             nq.pix[np] = pix;
             nq.rng[np] = rng.state;
+        }
+        if (a.sortOct) {
+            // rays per direction octant: LDS counts per block
+            const int o = (so.nd[0] < 0.0f ? 1 : 0) | (so.nd[1] < 0.0f ? 2 : 0) | (so.nd[2] < 0.0f ? 4 : 0);
+            const int lane = threadIdx.x & 63;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const uint64_t m = __ballot(so.next && o == k);
+                if (m && lane == 0) atomicAdd(&octCount[k], __popcll(m));
+            }
+        }
+    }
+    __syncthreads();
+    if (a.sortOct && threadIdx.x < 8 && octCount[threadIdx.x])
+        atomicAdd(&a.oct[b * 8 + threadIdx.x], octCount[threadIdx.x]);
+}
+
+// Octant sort of the next bounce's extension rays: scatters the staging
+// queue into q[(b + 1) & 1] grouped by direction octant (octants in order).
+// Each workgroup takes a contiguous range of the staging queue, counts it per
+// octant in LDS, reserves its slots with one atomic per octant, and scatters
+// with LDS cursors, so a wave traces rays of one octant whose origins keep
+// roughly their queue (pixel) order. Paths are independent and each pixel's
+// additions follow its own path, so the image does not depend on the queue
+// order.
+__global__ __launch_bounds__(kPtBlock) void k_pt_octsort(PtArgs a, int b) {
+    if (!wavefront_bounce(a, b)) return;      // bounce b was shaded by the tail kernel
+    const int n = min(a.cnt[b + 1], a.N);
+    __shared__ int cnt8[8], base8[8];
+    int start[8];
+    int acc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) { start[k] = acc; acc += a.oct[b * 8 + k]; }
+    const PtQueue& src = a.t;
+    const PtQueue& dst = a.q[(b + 1) & 1];
+    const int lane = threadIdx.x & 63;
+    const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int lo = min(n, (int)blockIdx.x * per), hi = min(n, lo + per);
+    auto octant = [&](int i) {
+        return (src.r[3][i] < 0.0f ? 1 : 0) | (src.r[4][i] < 0.0f ? 2 : 0) | (src.r[5][i] < 0.0f ? 4 : 0);
+    };
+    if (threadIdx.x < 8) cnt8[threadIdx.x] = 0;
+    __syncthreads();
+    for (int j = lo; j < hi; j += kPtBlock) {
+        const int i = j + (int)threadIdx.x;
+        const bool act = i < hi;
+        const int o = act ? octant(i) : 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint64_t m = __ballot(act && o == k);
+            if (m && lane == 0) atomicAdd(&cnt8[k], __popcll(m));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        base8[threadIdx.x] = cnt8[threadIdx.x] ? atomicAdd(&a.oct[128 + b * 8 + threadIdx.x], cnt8[threadIdx.x]) : 0;
+        cnt8[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (int j = lo; j < hi; j += kPtBlock) {
+        const int i = j + (int)threadIdx.x;
+        const bool act = i < hi;
+        const int o = act ? octant(i) : 0;
+        int pos = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint64_t m = __ballot(act && o == k);
+            if (!m) continue;
+            int w = 0;
+            if (lane == 0) w = atomicAdd(&cnt8[k], __popcll(m));
+            w = __builtin_amdgcn_readfirstlane(w);
+            if (act && o == k) pos = start[k] + base8[k] + w + __popcll(m & ((1ull << lane) - 1ull));
+        }
+        if (act) {
+            for (int q = 0; q < 8; q++) dst.r[q][pos] = src.r[q][i];
+            for (int q = 0; q < 3; q++) dst.thr[q][pos] = src.thr[q][i];
+            dst.pix[pos] = src.pix[i];
+            dst.rng[pos] = src.rng[i];
         }
     }
 }
@@ -379,6 +464,7 @@ static int pt_sample(CudaSceneHandle* sc, PtResources& R, PtArgs& a, hipStream_t
     hipLaunchKernelGGL(k_pt_generate, dim3(g), dim3(kPtBlock), 0, s, a);
     if (pt_extend(sc, a, 0, s) != 0) return -1;
     hipLaunchKernelGGL(k_pt_shade, dim3(g), dim3(kPtBlock), 0, s, a, 0);
+    if (a.sortOct && a.maxDepth > 1) hipLaunchKernelGGL(k_pt_octsort, dim3(g), dim3(kPtBlock), 0, s, a, 0);
     for (int b = 0; b < a.maxDepth; b++) {
         Trace2LArgs u;
         u.n = a.N; u.nDev = &a.cnt[kPtMaxDepth + b];
@@ -395,6 +481,8 @@ static int pt_sample(CudaSceneHandle* sc, PtResources& R, PtArgs& a, hipStream_t
         hipLaunchKernelGGL(k_pt_connect, dim3(g), dim3(kPtBlock), 0, s, a, b);
         if (a.tailPaths > 0) hipLaunchKernelGGL(k_pt_tail, dim3(a.tailStride / 64), dim3(64), 0, s, a, b);
         if (b + 1 < a.maxDepth) hipLaunchKernelGGL(k_pt_shade, dim3(g), dim3(kPtBlock), 0, s, a, b + 1);
+        if (a.sortOct && b + 2 < a.maxDepth)
+            hipLaunchKernelGGL(k_pt_octsort, dim3(g), dim3(kPtBlock), 0, s, a, b + 1);
     }
     hipLaunchKernelGGL(k_pt_end, dim3(1), dim3(64), 0, s, a);
     const hipError_t e = hipGetLastError();
@@ -406,8 +494,9 @@ static int pt_alloc(PtResources& R, int N) {
     if (R.cap >= N && R.block) return 0;
     hipFree(R.block);
     R.block = nullptr; R.cap = 0;
-    // per ray: 2 queues x (8 + 3 floats, pix, rng 8 B) + shadow (8 + 3 floats, pix, occ) + hit (t, inst, prim)
-    const size_t per = 2 * (11 * 4 + 4 + 8) + (11 * 4 + 4 + 1) + 12;
+    // per ray: 3 queues (two ping-pong, one octant-sort staging) x (8 + 3 floats, pix, rng 8 B)
+    // + shadow (8 + 3 floats, pix, occ) + hit (t, inst, prim)
+    const size_t per = 3 * (11 * 4 + 4 + 8) + (11 * 4 + 4 + 1) + 12;
     const size_t bytes = per * (size_t)N + 4096;
     if (hipMalloc(&R.block, bytes) != hipSuccess) {
         rt::set_error("path tracer: cannot allocate %zu bytes of queues", bytes);
@@ -425,11 +514,12 @@ static void pt_layout(PtResources& R, PtArgs& a) {
     const size_t N = (size_t)R.cap;
     char* p = (char*)R.block;
     auto take = [&](size_t bytes) { char* r = p; p += (bytes + 255) & ~(size_t)255; return r; };
-    for (int k = 0; k < 2; k++) {
-        a.q[k].rng = (uint64_t*)take(8 * N);
-        for (int j = 0; j < 8; j++) a.q[k].r[j] = (float*)take(4 * N);
-        for (int j = 0; j < 3; j++) a.q[k].thr[j] = (float*)take(4 * N);
-        a.q[k].pix = (int*)take(4 * N);
+    for (int k = 0; k < 3; k++) {
+        PtQueue& q = k < 2 ? a.q[k] : a.t;
+        q.rng = (uint64_t*)take(8 * N);
+        for (int j = 0; j < 8; j++) q.r[j] = (float*)take(4 * N);
+        for (int j = 0; j < 3; j++) q.thr[j] = (float*)take(4 * N);
+        q.pix = (int*)take(4 * N);
     }
     for (int j = 0; j < 8; j++) a.s[j] = (float*)take(4 * N);
     for (int j = 0; j < 3; j++) a.sc[j] = (float*)take(4 * N);
@@ -441,6 +531,7 @@ static void pt_layout(PtResources& R, PtArgs& a) {
     a.cnt = R.small;                                           // 2 * kPtMaxDepth ints
     a.frame = R.small + 2 * kPtMaxDepth;                       // 2 ints
     a.totals = (unsigned long long*)(R.small + 4 * kPtMaxDepth);   // 8-byte aligned
+    a.oct = R.small + 256;                                     // 16 * kPtMaxDepth ints
 }
 
 void pt_free(CudaSceneHandle* sc) {
@@ -501,6 +592,7 @@ int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams&
         a.sunRad[k] = p.sunRadiance[k]; a.skyZ[k] = p.skyZenith[k]; a.skyH[k] = p.skyHorizon[k];
     }
     a.clampMax = p.clampMax;
+    a.sortOct = (flags & CUDABVH_FLAG_SORT_RAYS) ? 1 : 0;
     a.tailPaths = std::max(0, std::min(p.tailPaths, N));
     a.tlasNodes = (const uint4*)sc->tlas->d_nodes4;
     a.tlasTris = (const float4*)sc->tlas->d_tris;
