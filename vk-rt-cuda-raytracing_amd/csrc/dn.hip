@@ -153,6 +153,96 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
         }
 }
 
+// 3x3 convolution with LDS-staged operands (cin a multiple of 32). A
+// workgroup computes BM consecutive pixels of one image row x BN output
+// channels; waves form a WGM x WGN grid of (16*WM) x (16*WN) tiles. Per
+// 32-channel chunk it stages the input halo (3 rows x (BM + 2) pixels) and the
+// chunk's weights for all 9 taps in LDS (rows padded to 80 bytes, so the
+// 16-byte operand reads of a wave's 16 rows hit distinct banks), then runs
+// the 9 taps' MFMAs from LDS: every input pixel is fetched from memory once
+// per chunk instead of once per tap and wave. Same arithmetic order per
+// output as k_conv (FP32 accumulation over taps, then channel chunks).
+constexpr int kPxStride = 40;   // bf16 elements per LDS row: 32 channels + 8 padding
+
+template <int WM, int WN, int WGM, int WGN>
+__global__ __launch_bounds__(kBlock) void k_conv3(ConvArgs a) {
+    static_assert(WGM * WGN == kBlock / 64, "four waves");
+    constexpr int BM = WGM * 16 * WM, BN = WGN * 16 * WN, HALO = BM + 2;
+    __shared__ __attribute__((aligned(16))) uint16_t As[3 * HALO * kPxStride];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[9 * BN * kPxStride];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tilesX = (a.W + BM - 1) / BM;
+    const int tile = blockIdx.x;
+    const int tx = tile % tilesX, ty = tile / tilesX;       // ty = image * H + row
+    const int yy = ty % a.H, bb = ty / a.H;
+    const int x0 = tx * BM, n0 = blockIdx.y * BN;
+    const int wm = wave % WGM, wn = wave / WGM;
+    const int row = lane & 15, kg = lane >> 4;
+    const int K = 9 * a.cin;
+    f32x4 acc[WM][WN];
+#pragma unroll
+    for (int r = 0; r < WM; r++)
+#pragma unroll
+        for (int j = 0; j < WN; j++) acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int c0 = 0; c0 < a.cin; c0 += 32) {
+        __syncthreads();       // the previous chunk's operands are no longer read
+        for (int e = tid; e < 3 * HALO * 4; e += kBlock) {
+            const int r = e / (HALO * 4), rem = e - r * (HALO * 4), h = rem >> 2, q = rem & 3;
+            const int y = yy + r - 1, x = x0 - 1 + h;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (y >= 0 && y < a.H && x >= 0 && x < a.W)
+                v = *(const uint4*)(a.x + ((size_t)(bb * a.H + y) * a.W + x) * a.ldx + c0 + 8 * q);
+            *(uint4*)&As[(r * HALO + h) * kPxStride + 8 * q] = v;
+        }
+        for (int e = tid; e < 9 * BN * 4; e += kBlock) {
+            const int t = e / (BN * 4), rem = e - t * (BN * 4), nr = rem >> 2, q = rem & 3;
+            *(uint4*)&Bs[(t * BN + nr) * kPxStride + 8 * q] =
+                *(const uint4*)(a.w + (size_t)(n0 + nr) * K + t * a.cin + c0 + 8 * q);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+            const int dy = t / 3, dx = t % 3;
+            bf16x8 av[WM], bv[WN];
+#pragma unroll
+            for (int r = 0; r < WM; r++) {
+                const int px = wm * 16 * WM + r * 16 + row + dx;
+                av[r] = __builtin_bit_cast(bf16x8, *(const uint4*)&As[(dy * HALO + px) * kPxStride + 8 * kg]);
+            }
+#pragma unroll
+            for (int j = 0; j < WN; j++) {
+                const int nr = wn * 16 * WN + j * 16 + row;
+                bv[j] = __builtin_bit_cast(bf16x8, *(const uint4*)&Bs[(t * BN + nr) * kPxStride + 8 * kg]);
+            }
+#pragma unroll
+            for (int r = 0; r < WM; r++)
+#pragma unroll
+                for (int j = 0; j < WN; j++)
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], bv[j], acc[r][j], 0, 0, 0);
+        }
+    }
+    // D[i][j]: lane j + 16*(i/4), item i%4
+#pragma unroll
+    for (int r = 0; r < WM; r++)
+#pragma unroll
+        for (int j = 0; j < WN; j++) {
+            const int col = n0 + wn * 16 * WN + j * 16 + row;
+            if (col >= a.n) continue;
+            const float b = a.bias ? a.bias[col] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int x = x0 + wm * 16 * WM + r * 16 + 4 * kg + i;
+                if (x >= a.W) continue;
+                const size_t p = (size_t)ty * a.W + x;
+                float v = acc[r][j][i] + b;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                if (a.y) a.y[p * a.ldy + col] = f2bf(v);
+                if (a.yf) a.yf[p * a.ldyf + col] = v;
+            }
+        }
+}
+
 // --------------------------------------------------------------- wgrad ----
 struct WgradArgs {
     const uint16_t* g; int ldg; int cout;    // output gradient (ReLU mask applied)
@@ -497,6 +587,21 @@ static void launch_conv(const ConvArgs& a, int P, int n, hipStream_t s) {
     }
 }
 
+// 3x3 convolution, cin a multiple of 32. Outputs of 64 channels or more go
+// through the LDS-staged kernel (128-pixel row tiles x 64 output channels;
+// weight rows are allocated up to a multiple of 64, so the staged weight rows
+// of the last channel tile are in bounds). 32-channel outputs keep k_conv:
+// with a single 32-channel chunk per tile the staged kernel's load phase is
+// not hidden behind its few MFMAs and it measured slower on MI355X.
+static void launch_conv3(const ConvArgs& a, int n, hipStream_t s) {
+    if (n <= 32) {
+        launch_conv<9, 32, 2>(a, a.B * a.H * a.W, n, s);
+        return;
+    }
+    dim3 grid(((a.W + 127) / 128) * (a.B * a.H), (n + 63) / 64);
+    hipLaunchKernelGGL((k_conv3<4, 2, 2, 2>), grid, dim3(kBlock), 0, s, a);
+}
+
 static void conv_fwd(Net& N, int li, const uint16_t* x, int ldx, uint16_t* y, int ldy, int B, int H, int W, bool relu,
                      float* yf = nullptr, int ldyf = 0) {
     const Layer& L = N.L[li];
@@ -508,8 +613,7 @@ static void conv_fwd(Net& N, int li, const uint16_t* x, int ldx, uint16_t* y, in
     else if (L.d.cin % 32) {
         if (WN == 2) launch_conv<9, 16, 2>(a, P, L.d.cout, s);
         else launch_conv<9, 16, 4>(a, P, L.d.cout, s);
-    } else if (WN == 4) launch_conv<9, 32, 4>(a, P, L.d.cout, s);
-    else launch_conv<9, 32, 2>(a, P, L.d.cout, s);
+    } else launch_conv3(a, L.d.cout, s);
 }
 
 // input gradient: dx = conv(g, flipped transposed weights)
@@ -520,8 +624,7 @@ static void conv_dx(Net& N, int li, const uint16_t* g, int ldg, uint16_t* dx, in
     const int WN = L.d.cin >= 64 ? 4 : 2;
     hipStream_t s = N.stream;
     if (L.d.taps == 1) launch_conv<1, 16, 2>(a, P, L.d.cin, s);
-    else if (WN == 4) launch_conv<9, 32, 4>(a, P, L.d.cin, s);
-    else launch_conv<9, 32, 2>(a, P, L.d.cin, s);
+    else launch_conv3(a, L.d.cin, s);
 }
 
 static void conv_dw(Net& N, int li, const uint16_t* g, int ldg, const uint16_t* x, int ldx, int B, int H, int W) {
