@@ -185,15 +185,35 @@ __global__ __launch_bounds__(kBlock) void k_conv3(ConvArgs a) {
 #pragma unroll
         for (int j = 0; j < WN; j++) acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+    // The next chunk's input halo is fetched into registers while the MFMAs
+    // of the current chunk run from LDS (software pipelining); the weights,
+    // shared by every workgroup of a channel tile and mostly L2-resident, are
+    // copied to LDS in the staging step.
+    constexpr int NA = (3 * HALO * 4 + kBlock - 1) / kBlock;
+    uint4 ra[NA];
+#define DN_FETCH(C0)                                                                                        \
+    {                                                                                                        \
+        _Pragma("unroll") for (int k = 0; k < NA; k++) {                                                     \
+            const int e = tid + k * kBlock;                                                                  \
+            ra[k] = make_uint4(0, 0, 0, 0);                                                                  \
+            if (e < 3 * HALO * 4) {                                                                          \
+                const int r = e / (HALO * 4), rem = e - r * (HALO * 4), h = rem >> 2, q = rem & 3;           \
+                const int y = yy + r - 1, x = x0 - 1 + h;                                                    \
+                if (y >= 0 && y < a.H && x >= 0 && x < a.W)                                                  \
+                    ra[k] = *(const uint4*)(a.x + ((size_t)(bb * a.H + y) * a.W + x) * a.ldx + (C0) + 8 * q); \
+            }                                                                                                \
+        }                                                                                                    \
+    }
+    DN_FETCH(0)
     for (int c0 = 0; c0 < a.cin; c0 += 32) {
         __syncthreads();       // the previous chunk's operands are no longer read
-        for (int e = tid; e < 3 * HALO * 4; e += kBlock) {
-            const int r = e / (HALO * 4), rem = e - r * (HALO * 4), h = rem >> 2, q = rem & 3;
-            const int y = yy + r - 1, x = x0 - 1 + h;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (y >= 0 && y < a.H && x >= 0 && x < a.W)
-                v = *(const uint4*)(a.x + ((size_t)(bb * a.H + y) * a.W + x) * a.ldx + c0 + 8 * q);
-            *(uint4*)&As[(r * HALO + h) * kPxStride + 8 * q] = v;
+#pragma unroll
+        for (int k = 0; k < NA; k++) {
+            const int e = tid + k * kBlock;
+            if (e < 3 * HALO * 4) {
+                const int r = e / (HALO * 4), rem = e - r * (HALO * 4), h = rem >> 2, q = rem & 3;
+                *(uint4*)&As[(r * HALO + h) * kPxStride + 8 * q] = ra[k];
+            }
         }
         for (int e = tid; e < 9 * BN * 4; e += kBlock) {
             const int t = e / (BN * 4), rem = e - t * (BN * 4), nr = rem >> 2, q = rem & 3;
@@ -201,6 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_conv3(ConvArgs a) {
                 *(const uint4*)(a.w + (size_t)(n0 + nr) * K + t * a.cin + c0 + 8 * q);
         }
         __syncthreads();
+        if (c0 + 32 < a.cin) DN_FETCH(c0 + 32)
 #pragma unroll
         for (int t = 0; t < 9; t++) {
             const int dy = t / 3, dx = t % 3;
@@ -242,6 +263,8 @@ __global__ __launch_bounds__(kBlock) void k_conv3(ConvArgs a) {
             }
         }
 }
+
+#undef DN_FETCH
 
 // --------------------------------------------------------------- wgrad ----
 struct WgradArgs {
