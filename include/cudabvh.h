@@ -171,6 +171,19 @@ enum {
     CUDABVH_FLAG_CWBVH         = 1u << 8  /* trace the compressed 8-wide BVH (built on first use) */
 };
 
+/* Build from geometry already in device memory (no counterpart in the
+ * reference; it replaces the host read-back of the layer's BLAS capture,
+ * VkLayer_CudaRT.cpp:3039-3234, which gathers vertex and index buffers on the
+ * CPU before cudaBVH_build). Triangle t has corners idx[3t], idx[3t+1],
+ * idx[3t+2] (indexBytes 2 or 4: 16- or 32-bit indices; 0: no index buffer,
+ * corners 3t..3t+2), each corner three floats at byte offset
+ * index * vertexStride of d_vertices. The triangles are gathered on the GPU
+ * and built as cudaBVH_build builds them (same tree). Indices >= numVertices
+ * make the call fail (NULL, cudaBVH_lastError()) without reading out of
+ * bounds. vertexStride >= 12 and a multiple of 4. */
+CudaBVH_t cudaBVH_buildDevice(const void* d_vertices, int numVertices, int vertexStride, const void* d_indices,
+                              int indexBytes, int numTris);
+
 /* Closest hit for n rays. Device pointers unless CUDABVH_FLAG_HOST_POINTERS.
  * Asynchronous on the handle's stream unless CUDABVH_FLAG_SYNC or host
  * pointers. 0 on success. */

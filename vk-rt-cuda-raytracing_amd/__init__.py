@@ -102,6 +102,7 @@ _F = C.c_float
 _I = C.c_int
 SIGNATURES = {
     "cudaBVH_build": (_H, [_P, _I]),
+    "cudaBVH_buildDevice": (_H, [_P, _I, _I, _P, _I, _I]),
     "cudaBVH_destroy": (None, [_H]),
     "cudaBVH_tracePrimary": (_F, [_H, _I, _F, _F, _F, _P]),
     "cudaBVH_traceDiffuse": (_F, [_H, _I, _P, _P, _P, _P, _P, _P, _P]),
@@ -257,6 +258,21 @@ class BVH:
         """tris: float32 array of shape (n, 9) or (n, 3, 3) in original order."""
         t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
         return cls(load_library().cudaBVH_build(t.ctypes.data, t.shape[0]), t.shape[0])
+
+    @classmethod
+    def build_device(cls, vertices, num_tris: int, indices=None, vertex_stride: int = 12,
+                     num_vertices: int | None = None) -> "BVH":
+        """Build from device tensors: vertices (float32, vertex_stride bytes per
+        vertex, position first) and optional indices (int32 or int16, three per
+        triangle); without indices the vertices are the triangle corners in order."""
+        nbytes = vertices.numel() * vertices.element_size()
+        nv = num_vertices if num_vertices is not None else nbytes // vertex_stride
+        ib = 0 if indices is None else indices.element_size()
+        h = load_library().cudaBVH_buildDevice(_ptr(vertices), nv, vertex_stride,
+                                               None if indices is None else _ptr(indices), ib, num_tris)
+        if not h:
+            raise CudaBVHError(f"buildDevice: {last_error()}")
+        return cls(h, num_tris)
 
     @classmethod
     def build_from_aabbs(cls, aabbs: np.ndarray) -> "BVH":

@@ -37,9 +37,11 @@ static void free_handle_device(CudaBVHHandle* h) {
     h->d_leafOfPrim = nullptr;
 }
 
-// Builds a handle from n x 9 floats (original order).
+// Builds a handle from n x 9 floats (original order). d_tri9 (nullable): the
+// same triangles already on the device (not freed here).
 static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf, const int* sceneOff = nullptr,
-                                 int numScenes = 1, std::vector<int>* roots4 = nullptr) {
+                                 int numScenes = 1, std::vector<int>* roots4 = nullptr,
+                                 const float* d_tri9 = nullptr) {
     rt::clear_error();
     if (!tri9 || n <= 0) { set_error("cudaBVH_build: need tris != NULL and numTris > 0 (got %d)", n); return nullptr; }
     if (n > (1 << 27)) { set_error("cudaBVH_build: %d triangles exceed the leaf encoding (2^27)", n); return nullptr; }
@@ -60,15 +62,19 @@ static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf, const in
         // Device build: upload the triangles once, build, keep the device
         // structure; host copies serve the reference's getters.
         hipStream_t s = hstream(h);
-        float* d_in = nullptr;
+        float* d_in = const_cast<float*>(d_tri9);
+        float* d_own = nullptr;
         rt::GpuBVH g;
-        HIP_TRY(hipMalloc((void**)&d_in, size_t(n) * 36), goto fail);
-        HIP_TRY(hipMemcpyAsync(d_in, tri9, size_t(n) * 36, hipMemcpyHostToDevice, s), hipFree(d_in); goto fail);
+        if (!d_in) {
+            HIP_TRY(hipMalloc((void**)&d_own, size_t(n) * 36), goto fail);
+            HIP_TRY(hipMemcpyAsync(d_own, tri9, size_t(n) * 36, hipMemcpyHostToDevice, s), hipFree(d_own); goto fail);
+            d_in = d_own;
+        }
         if (rt::gpu_build(d_in, n, maxLeaf, s, g, sceneOff, numScenes, roots4) != 0) {
-            hipFree(d_in); hipFree(g.nodes2); hipFree(g.nodes4); hipFree(g.tris); hipFree(g.leafOf); hipFree(g.ordered);
+            hipFree(d_own); hipFree(g.nodes2); hipFree(g.nodes4); hipFree(g.tris); hipFree(g.leafOf); hipFree(g.ordered);
             goto fail;
         }
-        hipFree(d_in);
+        hipFree(d_own);
         h->d_nodes4 = g.nodes4;
         h->d_tris = g.tris;
         h->d_leafOfPrim = g.leafOf;
@@ -188,6 +194,47 @@ const char* cudaBVH_lastError(void) { return rt::last_error(); }
 
 CudaBVH_t cudaBVH_build(const CudaTri* tris, int numTris) {
     return build_impl(reinterpret_cast<const float*>(tris), numTris, 4);
+}
+
+CudaBVH_t cudaBVH_buildDevice(const void* d_vertices, int numVertices, int vertexStride, const void* d_indices,
+                              int indexBytes, int numTris) {
+    rt::clear_error();
+    if (!d_vertices || numVertices <= 0 || numTris <= 0 || vertexStride < 12 || vertexStride % 4 ||
+        (indexBytes != 0 && indexBytes != 2 && indexBytes != 4) || (indexBytes && !d_indices)) {
+        set_error("cudaBVH_buildDevice: bad arguments (vertices %p x %d stride %d, indices %p x %d bytes, %d tris)",
+                  d_vertices, numVertices, vertexStride, d_indices, indexBytes, numTris);
+        return nullptr;
+    }
+    if (!indexBytes && (long long)numTris * 3 > numVertices) {
+        set_error("cudaBVH_buildDevice: %d triangles need %lld vertices, got %d", numTris, 3LL * numTris, numVertices);
+        return nullptr;
+    }
+    float* d_tri = nullptr;
+    int* d_bad = nullptr;
+    if (hipMalloc((void**)&d_tri, size_t(numTris) * 36) != hipSuccess || hipMalloc((void**)&d_bad, sizeof(int)) != hipSuccess) {
+        hipFree(d_tri);
+        set_error("cudaBVH_buildDevice: cannot allocate %zu bytes", size_t(numTris) * 36);
+        return nullptr;
+    }
+    std::vector<float> host(size_t(numTris) * 9);
+    int bad = 0;
+    hipError_t e = hipMemset(d_bad, 0, sizeof(int));
+    if (e == hipSuccess)
+        e = rtk::launch_gather_tris(d_vertices, numVertices, vertexStride, d_indices, indexBytes, numTris, d_tri, d_bad,
+                                    nullptr);
+    // the host copy feeds the reference's host-side exports (SoA, packed, DFS)
+    if (e == hipSuccess) e = hipMemcpy(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && !bad) e = hipMemcpy(host.data(), d_tri, host.size() * 4, hipMemcpyDeviceToHost);
+    hipFree(d_bad);
+    if (e != hipSuccess || bad) {
+        hipFree(d_tri);
+        if (bad) set_error("cudaBVH_buildDevice: an index is >= numVertices (%d)", numVertices);
+        else set_error("cudaBVH_buildDevice: gather failed: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    CudaBVHHandle* h = build_impl(host.data(), numTris, 4, nullptr, 1, nullptr, d_tri);
+    hipFree(d_tri);
+    return h;
 }
 
 void cudaBVH_destroy(CudaBVH_t bvh) {

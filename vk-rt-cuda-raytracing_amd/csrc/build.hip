@@ -535,7 +535,43 @@ __global__ __launch_bounds__(kThreads) void k_tile_scan(const int* in, int n, co
     }
 }
 
+// Triangle gather from device vertex / index buffers (one thread per
+// triangle): corner k of triangle t is vertex idx[3t + k] (or 3t + k without
+// an index buffer), three floats at byte offset index * stride. An index at or
+// beyond numVertices sets *bad and writes zeros instead of reading.
+__global__ void k_gather_tris(const uint8_t* verts, int numVertices, int stride, const void* idx, int indexBytes,
+                              int numTris, float* out, int* bad) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= numTris) return;
+    for (int k = 0; k < 3; k++) {
+        const size_t c = 3 * (size_t)t + k;
+        unsigned int v;
+        if (indexBytes == 4) v = ((const uint32_t*)idx)[c];
+        else if (indexBytes == 2) v = ((const uint16_t*)idx)[c];
+        else v = (unsigned int)c;
+        float x = 0.f, y = 0.f, z = 0.f;
+        if (v < (unsigned int)numVertices) {
+            const float* p = (const float*)(verts + (size_t)v * stride);
+            x = p[0]; y = p[1]; z = p[2];
+        } else {
+            atomicOr(bad, 1);
+        }
+        out[9 * (size_t)t + 3 * k] = x;
+        out[9 * (size_t)t + 3 * k + 1] = y;
+        out[9 * (size_t)t + 3 * k + 2] = z;
+    }
+}
+
 }  // namespace rtb
+
+namespace rtk {
+hipError_t launch_gather_tris(const void* verts, int numVertices, int stride, const void* idx, int indexBytes,
+                              int numTris, float* out, int* bad, hipStream_t s) {
+    hipLaunchKernelGGL(rtb::k_gather_tris, dim3((numTris + 255) / 256), dim3(256), 0, s, (const uint8_t*)verts,
+                       numVertices, stride, idx, indexBytes, numTris, out, bad);
+    return hipGetLastError();
+}
+}  // namespace rtk
 
 namespace rt {
 

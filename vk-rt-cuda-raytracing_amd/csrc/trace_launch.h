@@ -42,6 +42,9 @@ int launch_trace(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int
 int alloc_scratch(rt::TraceScratch& sc);
 void free_scratch(rt::TraceScratch& sc);
 int launch_gen_pinhole(const rtd::Cam& cam, int W, int H, const CudaRayPacket& p, hipStream_t s);
+// n x 9 floats gathered from device vertex / index buffers (build.hip)
+hipError_t launch_gather_tris(const void* verts, int numVertices, int stride, const void* idx, int indexBytes,
+                              int numTris, float* out, int* bad, hipStream_t s);
 int launch_bounce(const CudaBVHHandle* h, rt::TraceScratch& sc, const rtd::Cam& cam, int W, int H,
                   const float* hitT, const int* hitId, uint32_t seed, const float light[3], const CudaRayPacket& d,
                   const CudaRayPacket& s, int* src, int* count, hipStream_t st);
