@@ -184,4 +184,41 @@ __device__ __forceinline__ void pinhole_ray(const Cam& c, int px, int py, int W,
     r.tmin = 0.0f; r.tmax = RT_MISS_T;
 }
 
+// Work distribution of the persistent traversal kernels. Work items [0, n)
+// (rays, or 64 per 8x8 pinhole tile) are taken in 64-item chunks, and the
+// chunks are split into kBands contiguous bands, one per XCD: the dispatcher
+// places workgroups on the eight XCDs round robin, so blockIdx.x % 8 names
+// the XCD a wave runs on, and each wave starts on its own XCD's band. Nearby
+// pixels (or rays queued together) then share one XCD's L2 instead of being
+// spread over all eight. A wave whose band is used up moves on to the next
+// band, so no XCD idles while another still has work. ctr holds kBands
+// counters of chunks taken, kBandStride words apart, zeroed before the
+// launch; band is the wave's current band (wave-uniform). Returns the first
+// item of the chunk, or n when every band is used up.
+//
+// Each counter has a 128-byte cache line of its own: eight counters packed
+// into one line made every trace about twice as slow as a single counter,
+// while one line per counter makes the banded order 12-20 % faster than the
+// single counter's row-major order on the 1080p terrain frame.
+constexpr int kBands = 8;
+constexpr int kBandStride = 32;
+
+__device__ __forceinline__ int take_chunk(uint32_t* ctr, int n, int& band) {
+    const int nc = (n + 63) >> 6;
+    for (int k = 0; k < kBands; ++k) {
+        const int b = (band + k) & (kBands - 1);
+        const int lo = (int)(((long long)nc * b) / kBands);
+        const int hi = (int)(((long long)nc * (b + 1)) / kBands);
+        if (lo >= hi) continue;
+        uint32_t t = 0;
+        if ((threadIdx.x & 63) == 0) t = atomicAdd(&ctr[b * kBandStride], 1u);
+        t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+        if ((long long)lo + t < hi) {
+            band = b;
+            return (lo + (int)t) << 6;
+        }
+    }
+    return n;
+}
+
 }  // namespace rtd

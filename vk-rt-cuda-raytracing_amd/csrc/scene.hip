@@ -38,6 +38,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
     const float4* tris = nullptr;
     bool has = false, exhausted = false, found = false, flatW = false, flatO = false;
     int poolNext = 0, poolEnd = 0;
+    int band = blockIdx.x % rtd::kBands;   // wave-uniform: band of the work distribution
     int sp = 0, ref = -1, bid = -1, binst = -1, visits = 0, tests = 0;
     int pend = -1;             // postponed BLAS leaf (speculative node phase), -1: none
     float tn = 0.f, best = 0.f, bu = 0.f, bv = 0.f;
@@ -68,10 +69,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
                 const int avail = poolEnd - poolNext;
                 int fresh = n;
                 if (cnt > avail && !exhausted) {
-                    uint32_t b = 0;
-                    if (lane == 0) b = atomicAdd(a.counter, 64u);
-                    fresh = min(__builtin_amdgcn_readfirstlane((int)b), n);
-                    if (fresh + 64 >= n) exhausted = true;
+                    fresh = rtd::take_chunk(a.counter, n, band);
+                    if (fresh >= n) exhausted = true;
                 }
                 const int k = __popcll(idle & ((1ull << lane) - 1ull));
                 int w = n;
@@ -207,12 +206,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l_batch(Trace2LArgs a) {
     const int gtid = blockIdx.x * BLOCK + tid;
     if (a.gate && *a.gate <= a.gateMax) return;
     const int n = a.nDev ? min(*a.nDev, a.n) : a.n;
+    int band = blockIdx.x % rtd::kBands;
     for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(a.counter, 64u);
-        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
-        if (base >= (uint32_t)n) break;
-        const int ri = (int)base + lane;
+        const int base = rtd::take_chunk(a.counter, n, band);
+        if (base >= n) break;
+        const int ri = base + lane;
         if (ri >= n) continue;
         Ray r;
         r.ox = a.ox[ri]; r.oy = a.oy[ri]; r.oz = a.oz[ri];
@@ -256,6 +254,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l_refill(Trace2LArgs a) {
     const float4* tris = nullptr;
     bool has = false, exhausted = false, flatW = false, flatO = false;
     int poolNext = 0, poolEnd = 0;
+    int band = blockIdx.x % rtd::kBands;   // wave-uniform: band of the work distribution
     int sp = 0, ref = -1, bid = -1, binst = -1, lts = 0, ltc = 0;
     float tn = 0.f, best = 0.f, bu = 0.f, bv = 0.f;
 
@@ -280,10 +279,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l_refill(Trace2LArgs a) {
                 const int avail = poolEnd - poolNext;
                 int fresh = n;
                 if (cnt > avail && !exhausted) {
-                    uint32_t b = 0;
-                    if (lane == 0) b = atomicAdd(a.counter, 64u);
-                    fresh = min(__builtin_amdgcn_readfirstlane((int)b), n);
-                    if (fresh + 64 >= n) exhausted = true;
+                    fresh = rtd::take_chunk(a.counter, n, band);
+                    if (fresh >= n) exhausted = true;
                 }
                 const int k = __popcll(idle & ((1ull << lane) - 1ull));
                 int w = n;
@@ -389,7 +386,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l_refill(Trace2LArgs a) {
 
 constexpr int kBlock2 = 256;
 
-__global__ void k_reset_counter_2l(uint32_t* c) { c[0] = 0u; }
+// Zeroes the kBands work-distribution counters (launched with kBands threads).
+__global__ void k_reset_counter_2l(uint32_t* c) { c[threadIdx.x * rtd::kBandStride] = 0u; }
 
 template <bool ANY, int LDSD, int SCHED>
 static int launch2(Trace2LArgs& a, rt::TraceScratch& sc, int maxStack, int blocksPerCU, hipStream_t s) {
@@ -423,7 +421,7 @@ static int launch2(Trace2LArgs& a, rt::TraceScratch& sc, int maxStack, int block
     a.spillStride = threads;
     a.counter = sc.counter;
     a.status = sc.status;
-    hipLaunchKernelGGL(k_reset_counter_2l, dim3(1), dim3(1), 0, s, sc.counter);
+    hipLaunchKernelGGL(k_reset_counter_2l, dim3(1), dim3(rtd::kBands), 0, s, sc.counter);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock2), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { rt::set_error("two-level trace launch failed: %s", hipGetErrorString(e)); return -1; }

@@ -2,8 +2,9 @@
 // fused pinhole generation, shading) and their launchers.
 //
 // Execution model: persistent wave64 workers. Each wave takes 64 consecutive
-// rays with one atomicAdd, traces them one ray per lane, and repeats until the
-// ray counter passes the end. Each lane owns a traversal stack whose first
+// rays (or one 8x8 pinhole tile) with one atomicAdd on its XCD's band of the
+// work (rtd::take_chunk), traces them one ray per lane, and repeats until
+// every band is used up. Each lane owns a traversal stack whose first
 // LDS_DEPTH entries live in LDS (laid out [depth][thread] so a wave's lanes hit
 // consecutive banks) and whose deeper entries spill to a per-lane HBM region
 // sized from the BVH depth, so no entry is ever dropped (the reference's
@@ -22,7 +23,8 @@ namespace rtk {
 
 using rtd::Ray;
 
-__global__ void k_reset_counter(uint32_t* c) { c[0] = 0u; }
+// Zeroes the kBands work-distribution counters (launched with kBands threads).
+__global__ void k_reset_counter(uint32_t* c) { c[threadIdx.x * rtd::kBandStride] = 0u; }
 
 // Face-normal Lambert + rim shading of a primary hit, sky gradient on a miss
 // (backend :923-984; IEEE 1/sqrt and an explicit cube in place of rsqrtf and
@@ -238,6 +240,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_refill(TraceArgs a) {
     int ri = 0, py = 0;
     bool has = false, exhausted = false, flat = false;
     int poolNext = 0, poolEnd = 0;
+    int band = blockIdx.x % rtd::kBands;
     int sp = 0, ref = -1, bid = -1, bti = -1, lts = 0, ltc = 0;
     float tn = 0.f, best = 0.f, bu = 0.f, bv = 0.f;
 
@@ -262,10 +265,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace_refill(TraceArgs a) {
                 const int avail = poolEnd - poolNext;
                 int fresh = n;
                 if (cnt > avail && !exhausted) {
-                    uint32_t b = 0;
-                    if (lane == 0) b = atomicAdd(a.counter, 64u);
-                    fresh = min(__builtin_amdgcn_readfirstlane((int)b), n);
-                    if (fresh + 64 >= n) exhausted = true;
+                    fresh = rtd::take_chunk(a.counter, n, band);
+                    if (fresh >= n) exhausted = true;
                 }
                 const int k = __popcll(idle & ((1ull << lane) - 1ull));
                 int w = n;
@@ -387,13 +388,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace_batch(TraceArgs a) {
     // queued traces read their ray count from device memory (no host round trip)
     // pinhole work items cover 8x8 pixel tiles, one tile per wave
     const int n = PIN ? a.nTiles * 64 : (a.nDev ? min(*a.nDev, a.n) : a.n);
+    int band = blockIdx.x % rtd::kBands;
 
     for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(a.counter, 64u);
-        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
-        if (base >= (uint32_t)n) break;
-        int ri = (int)base + lane;
+        const int base = rtd::take_chunk(a.counter, n, band);
+        if (base >= n) break;
+        int ri = base + lane;
         int py = 0;
         bool live;
         if constexpr (PIN) {
@@ -479,6 +479,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_ww(TraceArgs a) {
     bool has = false;          // lane holds a ray
     bool exhausted = false;    // wave-uniform: the work counter has passed n
     int poolNext = 0, poolEnd = 0;   // wave-uniform: reserved, not yet taken work items
+    int band = blockIdx.x % rtd::kBands;   // wave-uniform: band of the work distribution
     bool flat = false, found = false;
     int sp = 0, ref = -1, visits = 0, tests = 0, bid = -1, bti = -1;
     int pend = -1;             // postponed leaf (speculative node phase), -1: none
@@ -507,10 +508,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace_ww(TraceArgs a) {
                 const int avail = poolEnd - poolNext;
                 int fresh = n;                               // n: no fresh batch
                 if (cnt > avail && !exhausted) {
-                    uint32_t b = 0;
-                    if (lane == 0) b = atomicAdd(a.counter, 64u);
-                    fresh = min(__builtin_amdgcn_readfirstlane((int)b), n);
-                    if (fresh + 64 >= n) exhausted = true;
+                    fresh = rtd::take_chunk(a.counter, n, band);
+                    if (fresh >= n) exhausted = true;
                 }
                 const int k = __popcll(idle & ((1ull << lane) - 1ull));
                 int w = n;                                   // n: no item
@@ -674,12 +673,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace_packet(TraceArgs a) {
     const cf32* tris = (const cf32*)a.tris;
     const int n = a.nTiles * 64;
 
+    int band = blockIdx.x % rtd::kBands;
+
     for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(a.counter, 64u);
-        base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
-        if (base >= (uint32_t)n) break;
-        const int tile = (int)base >> 6;
+        const int base = rtd::take_chunk(a.counter, n, band);
+        if (base >= n) break;
+        const int tile = base >> 6;
         const int ty = tile / a.tilesX, tx = tile - ty * a.tilesX;
         const int px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
         const bool live = px < a.W && py < a.H;
@@ -856,7 +855,7 @@ static int launch_kernel(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs
     a.status = sc.status;
     a.nodes = (const uint4*)h->d_nodes4;
     a.tris = (const float4*)h->d_tris;
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, sc.counter);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(rtd::kBands), 0, s, sc.counter);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { rt::set_error("trace launch failed: %s", hipGetErrorString(e)); return -1; }
@@ -881,7 +880,7 @@ static int launch_packet(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs
 
 int alloc_scratch(rt::TraceScratch& sc) {
     if (sc.counter) return 0;
-    if (hipMalloc((void**)&sc.counter, 64) != hipSuccess || hipMalloc((void**)&sc.status, 64) != hipSuccess ||
+    if (hipMalloc((void**)&sc.counter, rtd::kBands * rtd::kBandStride * sizeof(uint32_t)) != hipSuccess || hipMalloc((void**)&sc.status, 64) != hipSuccess ||
         hipMemset(sc.status, 0, 64) != hipSuccess) {
         free_scratch(sc);
         rt::set_error("cannot allocate traversal scratch");
