@@ -184,7 +184,7 @@ __device__ __forceinline__ void shade_event(const PtArgs& a, int b, bool aov, in
     const InstanceDev& I = a.inst[inst];
     const BlasDev& B = a.blas[I.blas];
     const int li = B.leafOf[prim];
-    const float4 p0 = B.tris[3 * li], p1 = B.tris[3 * li + 1], p2 = B.tris[3 * li + 2];
+    const float4 p0 = rtd::ldg_f4(B.tris + 3 * li), p1 = rtd::ldg_f4(B.tris + 3 * li + 1), p2 = rtd::ldg_f4(B.tris + 3 * li + 2);
     const float e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
     const float e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
     const float mx = e1y * e2z - e1z * e2y, my = e1z * e2x - e1x * e2z, mz = e1x * e2y - e1y * e2x;

@@ -184,6 +184,21 @@ __device__ __forceinline__ void pinhole_ray(const Cam& c, int px, int py, int W,
     r.tmin = 0.0f; r.tmax = RT_MISS_T;
 }
 
+// 16-byte loads through the global address space. A pointer chosen at run
+// time between arrays the compiler cannot trace to global memory (the TLAS
+// or an instance's BLAS, read from a descriptor) otherwise turns its loads
+// into generic flat loads, whose waits also cover every pending LDS access.
+typedef unsigned rt_v4u __attribute__((ext_vector_type(4)));
+typedef float rt_v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 ldg_u4(const uint4* p) {
+    const rt_v4u v = *(const __attribute__((address_space(1))) rt_v4u*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 ldg_f4(const float4* p) {
+    const rt_v4f v = *(const __attribute__((address_space(1))) rt_v4f*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // Work distribution of the persistent traversal kernels. Work items [0, n)
 // (rays, or 64 per 8x8 pinhole tile) are taken in 64-item chunks, and the
 // chunks are split into kBands contiguous bands, one per XCD: the dispatcher

@@ -161,9 +161,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
             for (int q = 0; q < tc; q++) {
                 const int ti = ts + q;
                 tests++;
-                const float4 p0 = tris[3 * ti];
+                const float4 p0 = rtd::ldg_f4(tris + 3 * ti);
                 float t, u, v;
-                if (!rtd::isect_watertight(ro, p0, tris[3 * ti + 1], tris[3 * ti + 2], t, u, v)) continue;
+                if (!rtd::isect_watertight(ro, p0, rtd::ldg_f4(tris + 3 * ti + 1), rtd::ldg_f4(tris + 3 * ti + 2), t, u, v)) continue;
                 if (!(t > ro.tmin) || !(t < ro.tmax)) continue;
                 if constexpr (ANY) {
                     found = true;
@@ -334,9 +334,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l_refill(Trace2LArgs a) {
             if (ltc > 0) {
                 const int ti = lts;
                 lts++; ltc--;
-                const float4 p0 = tris[3 * ti];
+                const float4 p0 = rtd::ldg_f4(tris + 3 * ti);
                 float t, u, v;
-                if (rtd::isect_watertight(ro, p0, tris[3 * ti + 1], tris[3 * ti + 2], t, u, v) && t > ro.tmin &&
+                if (rtd::isect_watertight(ro, p0, rtd::ldg_f4(tris + 3 * ti + 1), rtd::ldg_f4(tris + 3 * ti + 2), t, u, v) && t > ro.tmin &&
                     t < ro.tmax) {
                     if constexpr (ANY) {
                         found = true;

@@ -32,7 +32,7 @@ template <bool ANY, bool FLAT, class Push>
 __device__ __forceinline__ void visit2(const uint4* __restrict__ nodes, const Ray& r, float limit, int& ref, float& tn,
                                        Push&& push) {
     const uint4* np = nodes + 4 * (size_t)ref;
-    const uint4 qx = np[0], qy = np[1], qz = np[2], qc = np[3];
+    const uint4 qx = rtd::ldg_u4(np), qy = rtd::ldg_u4(np + 1), qz = rtd::ldg_u4(np + 2), qc = rtd::ldg_u4(np + 3);
     if constexpr (ANY) {
         float d0, d1, d2, d3;
         const bool h0 = rtd::child_hit<0, FLAT>(r, qx, qy, qz, (int)qc.x, limit, d0);
@@ -172,9 +172,9 @@ __device__ __forceinline__ bool trace2l_ray(const uint4* tlasNodes, const float4
         if (ltc > 0) {
             const int ti = lts;
             lts++; ltc--;
-            const float4 p0 = tris[3 * ti];
+            const float4 p0 = rtd::ldg_f4(tris + 3 * ti);
             float t, u, v;
-            if (rtd::isect_watertight(ro, p0, tris[3 * ti + 1], tris[3 * ti + 2], t, u, v) && t > ro.tmin &&
+            if (rtd::isect_watertight(ro, p0, rtd::ldg_f4(tris + 3 * ti + 1), rtd::ldg_f4(tris + 3 * ti + 2), t, u, v) && t > ro.tmin &&
                 t < ro.tmax) {
                 if constexpr (ANY) {
                     return true;

@@ -209,7 +209,11 @@ __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int*
             ref = s_ref[sp * BLOCK + tid];
             tn = s_tn[sp * BLOCK + tid];
         } else {
-            const int2 e = a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid];
+            // a non-temporal load: the spill read cannot be merged with the
+            // LDS read above into one generic (flat) load, which would send
+            // every pop through the vector memory path
+            typedef int v2i __attribute__((ext_vector_type(2)));
+            const v2i e = __builtin_nontemporal_load((const v2i*)&a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid]);
             ref = e.x; tn = __int_as_float(e.y);
         }
     }
