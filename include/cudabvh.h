@@ -109,7 +109,9 @@ void cudaBVH_getTriData(CudaBVH_t bvh, const float* outPtrs[9]);
 int cudaBVH_getStacklessBVH2(CudaBVH_t bvh, uint32_t** outData);
 
 /* layer/cuda_bvh_backend.h:111. 3 vec4 per leaf-order triangle:
- * {v0.xyz, v1.x} {v1.yz, v2.xy} {v2.z, float(originalIndex), 0, 0}.
+ * {v0.xyz, v1.x} {v1.yz, v2.xy} {v2.z, originalIndex, 0, 0}, the index
+ * stored as the int32 bit pattern (how the layer and its RT-IR executor read
+ * that word; the reference stored float(originalIndex), which they misread).
  * Returns the vec4 count. */
 int cudaBVH_getPackedTris(CudaBVH_t bvh, float** outData);
 

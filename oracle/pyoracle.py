@@ -51,6 +51,9 @@ def lib() -> C.CDLL:
             "orc_path_trace_2l": (None, [P, I, P, P, P, P, P, P, P, P, P, P, I]),
             "orc_path_trace_f64": (None, [P, P, I, P, P, P, P, I]),
             "orc_pcg32": (None, [C.c_uint64, C.c_uint64, I, P]),
+            "orc_glsl_rays": (None, [P, I, I, P]),
+            "orc_glsl_bvh4_trace": (None, [P, P, P, I, I, P, P]),
+            "orc_glsl_stackless_trace": (None, [P, P, P, I, I, P, P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -365,3 +368,25 @@ def pcg32(initstate: int, initseq: int, n: int) -> np.ndarray:
     out = np.zeros(n, np.uint32)
     lib().orc_pcg32(initstate, initseq, n, out.ctypes.data)
     return out
+
+
+def glsl_rays(pc13, W: int, H: int) -> np.ndarray:
+    """Camera rays of the layer's GLSL tracers (8 x W*H SoA)."""
+    out = np.zeros((8, W * H), np.float32)
+    pc = np.ascontiguousarray(pc13, np.float32)
+    lib().orc_glsl_rays(pc.ctypes.data, W, H, out.ctypes.data)
+    return out
+
+
+def glsl_trace(kind: str, nodes: np.ndarray, packed: np.ndarray, pc13, W: int, H: int):
+    """The layer's GLSL compute tracer over exported buffers: kind "bvh4"
+    (getNodeData bytes) or "stackless" (getStacklessBVH2 words); packed is the
+    getPackedTris vec4 array. Returns (t, leaf-order triangle index)."""
+    t = np.zeros(W * H, np.float32)
+    tri = np.zeros(W * H, np.int32)
+    n = np.ascontiguousarray(nodes).view(np.uint32)
+    pk = np.ascontiguousarray(packed, np.float32)
+    pc = np.ascontiguousarray(pc13, np.float32)
+    f = lib().orc_glsl_bvh4_trace if kind == "bvh4" else lib().orc_glsl_stackless_trace
+    f(n.ctypes.data, pk.ctypes.data, pc.ctypes.data, W, H, t.ctypes.data, tri.ctypes.data)
+    return t, tri

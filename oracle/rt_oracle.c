@@ -1202,3 +1202,135 @@ void orc_pcg32(uint64_t initstate, uint64_t initseq, int n, uint32_t* out) {
     pcg_seed(&r, initstate, initseq);
     for (int i = 0; i < n; i++) out[i] = pcg_next(&r);
 }
+
+/* ---------------- The layer's GLSL compute tracers ---------------- */
+
+/* Camera ray of pixel (px, py) as the shaders form it
+ * (bvh4_trace.comp:28-36, bvh2_stackless.comp:24-32). */
+static void glsl_ray(const float* pc, int W, int H, int px, int py, float ro[3], float rd[3], float invD[3],
+                     float ood[3]) {
+    float u = ((float)px + 0.5f) / (float)W * 2.0f - 1.0f;
+    float v = ((float)py + 0.5f) / (float)H * 2.0f - 1.0f;
+    u *= (float)W / (float)H * pc[12];
+    v *= pc[12];
+    float d[3];
+    for (int k = 0; k < 3; k++) d[k] = (pc[3 + k] + u * pc[6 + k]) + v * pc[9 + k];
+    const float il = 1.0f / sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+    for (int k = 0; k < 3; k++) {
+        ro[k] = pc[k];
+        rd[k] = d[k] * il;
+        /* mix(rd, sign(rd) * 1e-8, |rd| < 1e-8); sign(0) = 0 */
+        const float sg = rd[k] > 0.0f ? 1.0f : (rd[k] < 0.0f ? -1.0f : 0.0f);
+        const float c = fabsf(rd[k]) < 1e-8f ? sg * 1e-8f : rd[k];
+        invD[k] = 1.0f / c;
+        ood[k] = -ro[k] * invD[k];
+    }
+}
+
+/* The shaders' triangle loop body (bvh4_trace.comp:46-63). */
+static void glsl_tri(const float* packed, int ti, const float ro[3], const float rd[3], float* tHit, int* hitTri) {
+    const float* p = packed + 12 * (size_t)ti;
+    const float v0[3] = {p[0], p[1], p[2]};
+    const float e1[3] = {p[3] - v0[0], p[4] - v0[1], p[5] - v0[2]};
+    const float e2[3] = {p[6] - v0[0], p[7] - v0[1], p[8] - v0[2]};
+    const float pp[3] = {rd[1] * e2[2] - rd[2] * e2[1], rd[2] * e2[0] - rd[0] * e2[2], rd[0] * e2[1] - rd[1] * e2[0]};
+    const float det = (e1[0] * pp[0] + e1[1] * pp[1]) + e1[2] * pp[2];
+    const float inv = 1.0f / det;
+    const float tv[3] = {ro[0] - v0[0], ro[1] - v0[1], ro[2] - v0[2]};
+    const float uu = inv * ((tv[0] * pp[0] + tv[1] * pp[1]) + tv[2] * pp[2]);
+    if (uu < 0.0f || uu > 1.0f) return;
+    const float qq[3] = {tv[1] * e1[2] - tv[2] * e1[1], tv[2] * e1[0] - tv[0] * e1[2], tv[0] * e1[1] - tv[1] * e1[0]};
+    const float vv = inv * ((rd[0] * qq[0] + rd[1] * qq[1]) + rd[2] * qq[2]);
+    if (vv < 0.0f || uu + vv > 1.0f) return;
+    const float tt = inv * ((e2[0] * qq[0] + e2[1] * qq[1]) + e2[2] * qq[2]);
+    if (tt > 0.0f && tt < *tHit) { *tHit = tt; *hitTri = ti; }
+}
+
+void orc_glsl_rays(const float* pc, int W, int H, float* rays) {
+    const size_t n = (size_t)W * H;
+    for (int py = 0; py < H; py++)
+        for (int px = 0; px < W; px++) {
+            float ro[3], rd[3], invD[3], ood[3];
+            glsl_ray(pc, W, H, px, py, ro, rd, invD, ood);
+            const size_t i = (size_t)py * W + px;
+            for (int k = 0; k < 3; k++) { rays[k * n + i] = ro[k]; rays[(3 + k) * n + i] = rd[k]; }
+            rays[6 * n + i] = 0.0f; rays[7 * n + i] = MISS_T;
+        }
+}
+
+void orc_glsl_bvh4_trace(const uint32_t* nodes, const float* packed, const float* pc, int W, int H,
+                         float* tOut, int* triOut) {
+    enum { STK = 13 };
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < W * H; i++) {
+        const int px = i % W, py = i / W;
+        float ro[3], rd[3], invD[3], ood[3];
+        glsl_ray(pc, W, H, px, py, ro, rd, invD, ood);
+        const int sX = invD[0] >= 0.0f, sY = invD[1] >= 0.0f, sZ = invD[2] >= 0.0f;
+        float tHit = MISS_T;
+        int hitTri = -1;
+        int stk[STK], sp = 0;
+        stk[sp++] = 0;
+        while (sp > 0) {
+            const int ni = stk[--sp];
+            if (ni < 0) {
+                const int enc = -(ni + 2), ts = enc >> 3, tc = (enc & 7) + 1;
+                for (int t = 0; t < tc; t++) glsl_tri(packed, ts + t, ro, rd, &tHit, &hitTri);
+                continue;
+            }
+            /* uvec4 n0..n3: x, y, z halves (mins in slots 0-3, maxes 4-7), then children */
+            const uint32_t* w = nodes + 16 * (size_t)ni;
+            const uint16_t* hx = (const uint16_t*)w;
+            const uint16_t* hy = (const uint16_t*)(w + 4);
+            const uint16_t* hz = (const uint16_t*)(w + 8);
+            const int* ch = (const int*)(w + 12);
+            for (int c = 0; c < 4; c++) {   /* ch.x, ch.y, ch.z, ch.w in order */
+                if (ch[c] == -1) continue;
+                const float lx = orc_h2f(hx[c]), hxv = orc_h2f(hx[4 + c]);
+                const float ly = orc_h2f(hy[c]), hyv = orc_h2f(hy[4 + c]);
+                const float lz = orc_h2f(hz[c]), hzv = orc_h2f(hz[4 + c]);
+                const float nX = sX ? lx : hxv, fX = sX ? hxv : lx;
+                const float nY = sY ? ly : hyv, fY = sY ? hyv : ly;
+                const float nZ = sZ ? lz : hzv, fZ = sZ ? hzv : lz;
+                const float tN = fmaxf(fmaxf(fmaf(nX, invD[0], ood[0]), fmaf(nY, invD[1], ood[1])), fmaf(nZ, invD[2], ood[2]));
+                const float tF = fminf(fminf(fmaf(fX, invD[0], ood[0]), fmaf(fY, invD[1], ood[1])), fmaf(fZ, invD[2], ood[2]));
+                if (tN <= tF && tF > 0.0f && tN < tHit && sp < STK) stk[sp++] = ch[c];
+            }
+        }
+        tOut[i] = tHit;
+        triOut[i] = hitTri;
+    }
+}
+
+void orc_glsl_stackless_trace(const uint32_t* dfs, const float* packed, const float* pc, int W, int H,
+                              float* tOut, int* triOut) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < W * H; i++) {
+        const int px = i % W, py = i / W;
+        float ro[3], rd[3], invD[3], ood[3];
+        glsl_ray(pc, W, H, px, py, ro, rd, invD, ood);
+        float tHit = MISS_T;
+        int hitTri = -1;
+        int ni = 0;
+        while (ni >= 0) {
+            const uint32_t* w = dfs + 8 * (size_t)ni;
+            const int leaf = (int)w[6], skip = (int)w[7];
+            if (leaf != 0) {
+                const int enc = -(leaf + 2), ts = enc >> 3, tc = (enc & 7) + 1;
+                for (int t = 0; t < tc; t++) glsl_tri(packed, ts + t, ro, rd, &tHit, &hitTri);
+                ni = skip;
+            } else {
+                float b[6];
+                memcpy(b, w, sizeof b);   /* bmin.xyz, bmax.xyz */
+                const float t1x = fmaf(b[0], invD[0], ood[0]), t2x = fmaf(b[3], invD[0], ood[0]);
+                const float t1y = fmaf(b[1], invD[1], ood[1]), t2y = fmaf(b[4], invD[1], ood[1]);
+                const float t1z = fmaf(b[2], invD[2], ood[2]), t2z = fmaf(b[5], invD[2], ood[2]);
+                const float tN = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
+                const float tF = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
+                ni = (tN <= tF && tF > 0.0f && tN < tHit) ? ni + 1 : skip;
+            }
+        }
+        tOut[i] = tHit;
+        triOut[i] = hitTri;
+    }
+}

@@ -166,6 +166,25 @@ void orc_path_trace_f64(const double* wtris, const int* tri_inst, int m, const f
 /* PCG32 (pcg32_srandom_r / pcg32_random_r): n outputs. */
 void orc_pcg32(uint64_t initstate, uint64_t initseq, int n, uint32_t* out);
 
+/* ---------------- The layer's GLSL compute tracers ----------------
+ * Restatements of layer/shaders/bvh4_trace.comp:24-79 (BVH4 nodes as
+ * cudaBVH_getNodeData exports them, a 13-entry stack that drops overflow,
+ * children pushed unsorted in slot order) and bvh2_stackless.comp:20-78
+ * (32-B DFS nodes with skip links as cudaBVH_getStacklessBVH2 exports them),
+ * both over packed triangles as cudaBVH_getPackedTris exports them, with the
+ * shaders' Moller-Trumbore test (no det threshold, accept 0 < t < tHit, first
+ * found wins). They check that the exported formats are what the layer's
+ * Vulkan fallback consumes. pc: the push constants cam o[3], fwd[3],
+ * right[3], up[3], fov (13 floats); GLSL fma() is taken as fused and
+ * normalize(v) as v * (1 / sqrt(dot(v, v))). Per pixel: t (1e30 on a miss)
+ * and the leaf-order triangle index (-1 on a miss). rays (nullable): the
+ * pixel's ray as 8 SoA arrays (ox oy oz dx dy dz, tmin 0, tmax 1e30). */
+void orc_glsl_rays(const float* pc, int W, int H, float* rays);
+void orc_glsl_bvh4_trace(const uint32_t* nodes, const float* packed, const float* pc, int W, int H,
+                         float* t, int* tri);
+void orc_glsl_stackless_trace(const uint32_t* dfs, const float* packed, const float* pc, int W, int H,
+                              float* t, int* tri);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,10 +87,31 @@ def test_builder_matches_oracle(pkg, oracle, parity_scene, parity_bvh):
     assert parity_bvh.depth == ob.depth
     pk = parity_bvh.packed_tris().reshape(-1, 12)
     assert np.array_equal(pk[:, :9], tris[ob.order()])
-    assert np.array_equal(pk[:, 9], ob.order().astype(np.float32))
+    assert np.array_equal(pk[:, 9].view(np.int32), ob.order())   # ID bits, as the layer reads them
     cx, cy, cz, ext = parity_bvh.bounds()
     mn, mx = tris.reshape(-1, 3).min(0), tris.reshape(-1, 3).max(0)
     assert np.allclose([cx, cy, cz], (mn + mx) / 2) and np.isclose(ext, np.linalg.norm(mx - mn), rtol=1e-6)
+
+
+def test_exports_feed_glsl_tracers(oracle, parity_bvh, dev):
+    """The layer's GLSL compute tracers (bvh4_trace.comp, bvh2_stackless.comp;
+    restated in the oracle) over this library's exports — getNodeData,
+    getStacklessBVH2, getPackedTris — see the closest hits that the library's
+    own kernel reports for the same rays (their Moller-Trumbore test and
+    unsorted order aside, which move no hit on this scene)."""
+    pc = np.array([0, 0, -3, 0, 0, 1, 1, 0, 0, 0, 1, 0, np.tan(np.radians(30.0))], np.float32)
+    W, H = 256, 192
+    rays = oracle.glsl_rays(pc, W, H)
+    got = _gpu_closest(parity_bvh, rays, dev)
+    assert 0.3 < (got["prim_id"] >= 0).mean() < 0.9
+    order = parity_bvh.leaf_prim_ids()
+    for kind, nodes in (("bvh4", parity_bvh.node_data()), ("stackless", parity_bvh.stackless_bvh2())):
+        t, tri = oracle.glsl_trace(kind, nodes, parity_bvh.packed_tris(), pc, W, H)
+        gid = np.where(tri >= 0, order[np.maximum(tri, 0)], -1)
+        agree = gid == got["prim_id"]
+        assert agree.mean() >= 0.999, (kind, agree.mean())
+        both = agree & (gid >= 0)
+        assert np.all(np.abs(t[both] - got["t"][both]) <= 1e-4 * got["t"][both]), kind
 
 
 @pytest.mark.parametrize("isect", [0, 1])

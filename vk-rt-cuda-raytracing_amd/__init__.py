@@ -345,7 +345,7 @@ class BVH:
 
     def leaf_prim_ids(self) -> np.ndarray:
         """Original primitive index of each leaf-order triangle (from the packed export)."""
-        return self.packed_tris()[2::3, 1].astype(np.int64).astype(np.int32)
+        return self.packed_tris()[2::3, 1].view(np.int32).copy()
 
     # --- streams ---
     def set_stream(self, stream: int | None) -> None:

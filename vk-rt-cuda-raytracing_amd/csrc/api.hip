@@ -98,7 +98,12 @@ static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf, const in
             const float* src = tri9 + 9 * size_t(p);
             float* pk = &h->h_packed[12 * size_t(i)];
             for (int a = 0; a < 9; a++) { h->h_soa[a][i] = src[a]; pk[a] = src[a]; }
-            pk[9] = (float)p;   // the value, not the bits (backend :730-733)
+            // the original ID's int bits, which is how the layer and the
+            // RT-IR executor read this word (VkLayer_CudaRT.cpp:4410,
+            // layer/rt_ir_exec.cu:167); the reference writes float(ID)
+            // there (backend :730-733), which those readers misinterpret
+            const int32_t id = p;
+            memcpy(&pk[9], &id, sizeof id);
         }
         if (numScenes == 1) {
             rt::export_dfs(nodes2, h->h_dfs);
