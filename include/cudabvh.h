@@ -324,6 +324,19 @@ int cudaBVH_sceneTraceClosestQueued(CudaScene_t scene, int capacity, const int* 
 int cudaBVH_sceneTraceAnyQueued(CudaScene_t scene, int capacity, const int* d_count,
                                 const CudaRayPacket* rays, uint8_t* occluded, unsigned flags);
 
+/* World-space unit geometric normal of each closest hit of sceneTraceClosest*
+ * (reads hits->instId and hits->primId; device arrays nx, ny, nz of numRays).
+ * The normal is the object-space e1 x e2 of the hit triangle (the mesh's
+ * winding, not turned toward the ray) mapped with the inverse transpose of the
+ * instance's objectToWorld, normalised; a miss or an ID outside the scene
+ * gives (0, 0, 0). No backend counterpart: it serves callers of the scene
+ * trace that shade hits, as the layer's RT-IR executor does inline after its
+ * own TLAS/BLAS traversal (layer/rt_ir_exec.cu:167-179, which maps with the
+ * transpose of the forward matrix: the right direction only when that matrix
+ * is a uniform scale). */
+int cudaBVH_sceneHitNormals(CudaScene_t scene, int numRays, const CudaSceneHit* hits,
+                            float* nx, float* ny, float* nz, unsigned flags);
+
 /* Launch configuration of the scene traversal per ray class (0 closest hit,
  * 1 any hit, 2 closest hit of the path tracer's bounces after the first — the
  * first bounce and sceneTraceClosest* use class 0): schedule (0 while-while with a wave pool, 1 one ray per lane in

@@ -52,6 +52,7 @@ def lib() -> C.CDLL:
             "orc_path_trace_f64": (None, [P, P, I, P, P, P, P, I]),
             "orc_pcg32": (None, [C.c_uint64, C.c_uint64, I, P]),
             "orc_glsl_rays": (None, [P, I, I, P]),
+            "orc_scene_hit_normals": (None, [P, P, P, I, I, P, P, P]),
             "orc_glsl_bvh4_trace": (None, [P, P, P, I, I, P, P]),
             "orc_glsl_stackless_trace": (None, [P, P, P, I, I, P, P]),
         }
@@ -299,6 +300,16 @@ class OracleScene:
                                    self._arr, self.blas_of.ctypes.data, self.w2o.ctypes.data, rays.ctypes.data, n,
                                    *[out[k].ctypes.data for k in ("t", "inst", "id", "u", "v", "nodes", "tris")],
                                    threads)
+        return out
+
+    def hit_normals(self, inst: np.ndarray, prim: np.ndarray) -> np.ndarray:
+        """World-space unit geometric normals (3, n) of hits (instance, original ID)."""
+        inst = np.ascontiguousarray(inst, np.int32)
+        prim = np.ascontiguousarray(prim, np.int32)
+        n = inst.shape[0]
+        out = np.zeros((3, n), np.float32)
+        lib().orc_scene_hit_normals(self._arr, self.blas_of.ctypes.data, self.w2o.ctypes.data, self.blas_of.shape[0], n,
+                                    inst.ctypes.data, prim.ctypes.data, out.ctypes.data)
         return out
 
     def trace_any(self, rays: np.ndarray, threads: int = 0, counts: dict | None = None) -> np.ndarray:

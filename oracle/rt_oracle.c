@@ -961,6 +961,44 @@ static void make_xray(xray* r, const float o[3], const float d[3], float tmin, f
     setup_ray(r, a, 1, 0);
 }
 
+/* World-space unit geometric normal of a triangle (object space, 9 floats)
+ * under an instance with world-to-object matrix M (3x4): the object-space
+ * e1 x e2 (the mesh's winding) mapped with the transpose of M's 3x3, which is
+ * the inverse transpose of the object-to-world map, then normalised. */
+static void world_normal(const float* tri, const float* M, float n[3]) {
+    float e1x = tri[3] - tri[0], e1y = tri[4] - tri[1], e1z = tri[5] - tri[2];
+    float e2x = tri[6] - tri[0], e2y = tri[7] - tri[1], e2z = tri[8] - tri[2];
+    float mx = e1y * e2z - e1z * e2y, my = e1z * e2x - e1x * e2z, mz = e1x * e2y - e1y * e2x;
+    n[0] = (M[0] * mx + M[4] * my) + M[8] * mz;
+    n[1] = (M[1] * mx + M[5] * my) + M[9] * mz;
+    n[2] = (M[2] * mx + M[6] * my) + M[10] * mz;
+    float nl = 1.0f / sqrtf((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
+    for (int k = 0; k < 3; k++) n[k] = n[k] * nl;
+}
+
+void orc_scene_hit_normals(const orc_blas* blas, const int* blas_of, const float* w2o, int num_inst, int n,
+                           const int* inst, const int* prim, float* out) {
+    int nb = 0;
+    for (int k = 0; k < num_inst; k++) if (blas_of[k] + 1 > nb) nb = blas_of[k] + 1;
+    int** leaf_of = (int**)calloc((size_t)nb, sizeof(int*));   /* original ID -> leaf-order index */
+    for (int b = 0; b < nb; b++) {
+        leaf_of[b] = (int*)malloc(sizeof(int) * (size_t)(blas[b].num_tris > 0 ? blas[b].num_tris : 1));
+        for (int li = 0; li < blas[b].num_tris; li++) leaf_of[b][blas[b].prim_id[li]] = li;
+    }
+    for (int i = 0; i < n; i++) {
+        float nv[3] = {0.0f, 0.0f, 0.0f};
+        const int k = inst[i];
+        if (k >= 0 && k < num_inst) {
+            const orc_blas* B = &blas[blas_of[k]];
+            if (prim[i] >= 0 && prim[i] < B->num_tris)
+                world_normal(B->tris + 9 * (size_t)leaf_of[blas_of[k]][prim[i]], w2o + 12 * (size_t)k, nv);
+        }
+        for (int c = 0; c < 3; c++) out[(size_t)c * n + i] = nv[c];
+    }
+    for (int b = 0; b < nb; b++) free(leaf_of[b]);
+    free(leaf_of);
+}
+
 void orc_path_trace_2l(const void* tlas_nodes, int tlas_num_nodes, const int* tlas_prim, const orc_blas* blas,
                        const int* blas_of, const float* w2o, const float* albedo, const float* cam,
                        const orc_pt_params* p, float* rgb, float* aov, long long* counts, int threads) {
@@ -1009,17 +1047,8 @@ void orc_path_trace_2l(const void* tlas_nodes, int tlas_num_nodes, const int* tl
                         break;
                     }
                     const orc_blas* B = &blas[blas_of[inst]];
-                    const float* tri = B->tris + 9 * (size_t)leaf;
-                    float e1x = tri[3] - tri[0], e1y = tri[4] - tri[1], e1z = tri[5] - tri[2];
-                    float e2x = tri[6] - tri[0], e2y = tri[7] - tri[1], e2z = tri[8] - tri[2];
-                    float mx = e1y * e2z - e1z * e2y, my = e1z * e2x - e1x * e2z, mz = e1x * e2y - e1y * e2x;
-                    const float* M = w2o + 12 * (size_t)inst;
                     float n[3];
-                    n[0] = (M[0] * mx + M[4] * my) + M[8] * mz;
-                    n[1] = (M[1] * mx + M[5] * my) + M[9] * mz;
-                    n[2] = (M[2] * mx + M[6] * my) + M[10] * mz;
-                    float nl = 1.0f / sqrtf((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
-                    for (int k = 0; k < 3; k++) n[k] = n[k] * nl;
+                    world_normal(B->tris + 9 * (size_t)leaf, w2o + 12 * (size_t)inst, n);
                     if ((n[0] * d[0] + n[1] * d[1]) + n[2] * d[2] > 0.0f) for (int k = 0; k < 3; k++) n[k] = -n[k];
                     float h[3];
                     for (int k = 0; k < 3; k++) h[k] = o[k] + t * d[k];

@@ -166,6 +166,14 @@ void orc_path_trace_f64(const double* wtris, const int* tri_inst, int m, const f
 /* PCG32 (pcg32_srandom_r / pcg32_random_r): n outputs. */
 void orc_pcg32(uint64_t initstate, uint64_t initseq, int n, uint32_t* out);
 
+/* World-space unit geometric normals of scene hits (instance, ORIGINAL
+ * primitive ID of the instance's BLAS): normalize(M^T (e1 x e2)) with M the
+ * instance's world-to-object 3x3 and e1 x e2 in object space (the mesh's
+ * winding, not turned toward the ray); (0, 0, 0) for inst < 0 or an unknown
+ * ID. out: 3 SoA arrays of n. */
+void orc_scene_hit_normals(const orc_blas* blas, const int* blas_of, const float* w2o, int num_inst, int n,
+                           const int* inst, const int* prim, float* out);
+
 /* ---------------- The layer's GLSL compute tracers ----------------
  * Restatements of layer/shaders/bvh4_trace.comp:24-79 (BVH4 nodes as
  * cudaBVH_getNodeData exports them, a 13-entry stack that drops overflow,

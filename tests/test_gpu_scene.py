@@ -51,6 +51,13 @@ def test_scene_closest_any_parity(pkg, oracle, scenes, dev, seed, res, n_inst):
         assert np.array_equal(_bits(g.cpu().numpy()), _bits(ref[k])), k
     assert np.array_equal(cn.cpu().numpy(), ref["nodes"])
     assert np.array_equal(ct.cpu().numpy(), ref["tris"])
+    # world normals of the hits (misses give zero vectors), bit for bit
+    nrm = [torch.empty(n, device=dev) for _ in range(3)]
+    gs.hit_normals(inst, pid, *nrm)
+    torch.cuda.synchronize()
+    nref = osc.hit_normals(ref["inst"], ref["id"])
+    for k in range(3):
+        assert np.array_equal(_bits(nrm[k].cpu().numpy()), _bits(nref[k])), k
     # uncounted kernel: same hits
     t2 = torch.empty(n, device=dev)
     pid2 = torch.empty(n, dtype=torch.int32, device=dev)

@@ -109,3 +109,32 @@ def test_tlas_has_one_instance_per_leaf(oracle, small_inst):
         v = wt[winst == i].reshape(-1, 3)
         b = osc.world_boxes[i]
         assert np.all(v.min(0) >= b[:3]) and np.all(v.max(0) <= b[3:])
+
+
+def test_hit_normals_closed_form(oracle, scenes):
+    """World normals of hits: for an instance with object-to-world map M,
+    the image of the object-space normal under the inverse transpose of M is
+    parallel to (M e1) x (M e2), the cross product of the world-space edges,
+    with the sign of det(M). Checked in float64 for identity, rotated,
+    non-uniformly scaled and mirrored instances; misses and unknown IDs give
+    zero vectors."""
+    rng = np.random.default_rng(7)
+    meshes = [scenes.random_triangles(n, seed=40 + k, min_len=0.05, max_len=0.25) for k, n in enumerate((200, 500))]
+    R = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    R *= np.sign(np.linalg.det(R))
+    mats = [np.eye(3), R, R @ np.diag([0.5, 2.0, 1.3]), np.diag([-1.0, 1.0, 1.0]) @ R @ np.diag([1.0, 0.3, 2.5])]
+    o2w = np.stack([np.concatenate([m, rng.uniform(-3, 3, (3, 1))], 1).reshape(12) for m in mats]).astype(np.float32)
+    blas_of = [0, 1, 0, 1]
+    sc = oracle.OracleScene(meshes, o2w, blas_of, w2o_of(o2w))
+    for k in range(4):
+        tris = meshes[blas_of[k]].astype(np.float64).reshape(-1, 3, 3)
+        n = tris.shape[0]
+        got = sc.hit_normals(np.full(n, k, np.int32), np.arange(n, dtype=np.int32))
+        M = o2w[k].astype(np.float64).reshape(3, 4)[:, :3]
+        w = tris @ M.T
+        c = np.cross(w[:, 1] - w[:, 0], w[:, 2] - w[:, 0]) * np.sign(np.linalg.det(M))
+        c /= np.linalg.norm(c, axis=1, keepdims=True)
+        assert np.abs(got.T - c).max() < 2e-5, k
+        assert np.allclose(np.linalg.norm(got, axis=0), 1.0, atol=1e-6)
+    z = sc.hit_normals(np.array([-1, 0, 4], np.int32), np.array([-1, 10 ** 6, 0], np.int32))
+    assert np.array_equal(z, np.zeros((3, 3), np.float32))

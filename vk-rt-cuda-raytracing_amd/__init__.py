@@ -158,6 +158,7 @@ SIGNATURES = {
     "cudaBVH_sceneTraceClosestQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), C.POINTER(CudaSceneHit),
                                              C.c_uint]),
     "cudaBVH_sceneTraceAnyQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), _P, C.c_uint]),
+    "cudaBVH_sceneHitNormals": (_I, [_H, _I, C.POINTER(CudaSceneHit), _P, _P, _P, C.c_uint]),
     "cudaBVH_sceneSetTraceConfig": (_I, [_H, _I, _I, _I, _I]),
     "cudaBVH_sceneGetTraceConfig": (_I, [_H, _I, _P, _P, _P]),
     "cudaBVH_sceneAutotune": (_I, [_H, _I, _I, C.POINTER(CudaRayPacket), _I, _P, _P]),
@@ -565,6 +566,12 @@ class Scene:
         hp = CudaSceneHit(_ptr(t), _ptr(inst_id), _ptr(prim_id), _ptr(u), _ptr(v))
         _check(self.lib.cudaBVH_sceneTraceClosestQueued(self.handle, capacity, _ptr(d_count), C.byref(rp),
                                                         C.byref(hp), flags), "sceneTraceClosestQueued")
+
+    def hit_normals(self, inst_id, prim_id, nx, ny, nz, flags: int = 0) -> None:
+        """World-space unit geometric normals of closest hits (device arrays)."""
+        hp = CudaSceneHit(None, _ptr(inst_id), _ptr(prim_id), None, None)
+        _check(self.lib.cudaBVH_sceneHitNormals(self.handle, int(inst_id.shape[0]), C.byref(hp), _ptr(nx), _ptr(ny),
+                                                _ptr(nz), flags), "sceneHitNormals")
 
     def trace_any_queued(self, capacity: int, d_count, rays, occluded, flags: int = 0) -> None:
         rp = ray_packet(rays)

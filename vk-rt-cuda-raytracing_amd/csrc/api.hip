@@ -1011,7 +1011,7 @@ CudaScene_t cudaBVH_sceneCreate(const CudaBVH_t* blas, int numBlas, const CudaIn
     std::vector<rtk::BlasDev> bd(numBlas);
     for (int b = 0; b < numBlas; b++)
         bd[b] = rtk::BlasDev{(const uint4*)blas[b]->d_nodes4, (const float4*)blas[b]->d_tris, blas[b]->d_leafOfPrim,
-                             blas[b]->depth4, 0};
+                             blas[b]->depth4, blas[b]->numTris};
     if (hipMalloc((void**)&sc->d_inst, sizeof(rtk::InstanceDev) * numInst) != hipSuccess ||
         hipMalloc((void**)&sc->d_blas, sizeof(rtk::BlasDev) * numBlas) != hipSuccess ||
         hipMemcpy(sc->d_inst, sc->inst.data(), sizeof(rtk::InstanceDev) * numInst, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1088,6 +1088,22 @@ int cudaBVH_sceneTraceClosestQueued(CudaScene_t sc, int capacity, const int* d_c
     return scene_trace(sc, capacity, d_count, rays, hits, nullptr, nullptr, nullptr, flags, false,
                        "cudaBVH_sceneTraceClosestQueued");
 }
+int cudaBVH_sceneHitNormals(CudaScene_t sc, int numRays, const CudaSceneHit* hits, float* nx, float* ny, float* nz,
+                            unsigned flags) {
+    rt::clear_error();
+    if (!scene_ok(sc, "cudaBVH_sceneHitNormals")) return -1;
+    if (numRays == 0) return 0;
+    if (numRays < 0 || !hits || !hits->instId || !hits->primId || !nx || !ny || !nz) {
+        set_error("cudaBVH_sceneHitNormals: bad arguments");
+        return -1;
+    }
+    if (flags & ~(unsigned)CUDABVH_FLAG_SYNC) { set_error("cudaBVH_sceneHitNormals: unsupported flags 0x%x", flags); return -1; }
+    hipStream_t s = (hipStream_t)sc->stream;
+    int rc = rtk::launch_hit_normals(sc, numRays, hits->instId, hits->primId, nx, ny, nz, s);
+    if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(s), return -1);
+    return rc;
+}
+
 int cudaBVH_sceneTraceAnyQueued(CudaScene_t sc, int capacity, const int* d_count, const CudaRayPacket* rays,
                                 uint8_t* occluded, unsigned flags) {
     if (!d_count) { set_error("cudaBVH_sceneTraceAnyQueued: NULL count"); return -1; }

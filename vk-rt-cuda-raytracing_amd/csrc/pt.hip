@@ -184,17 +184,8 @@ __device__ __forceinline__ void shade_event(const PtArgs& a, int b, bool aov, in
     const InstanceDev& I = a.inst[inst];
     const BlasDev& B = a.blas[I.blas];
     const int li = B.leafOf[prim];
-    const float4 p0 = rtd::ldg_f4(B.tris + 3 * li), p1 = rtd::ldg_f4(B.tris + 3 * li + 1), p2 = rtd::ldg_f4(B.tris + 3 * li + 2);
-    const float e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
-    const float e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
-    const float mx = e1y * e2z - e1z * e2y, my = e1z * e2x - e1x * e2z, mz = e1x * e2y - e1y * e2x;
-    const float* W = I.w2o;
-    // normals map with the transpose of the world-to-object matrix
-    float nx = (W[0] * mx + W[4] * my) + W[8] * mz;
-    float ny = (W[1] * mx + W[5] * my) + W[9] * mz;
-    float nz = (W[2] * mx + W[6] * my) + W[10] * mz;
-    const float nl = 1.0f / sqrtf((nx * nx + ny * ny) + nz * nz);
-    nx = nx * nl; ny = ny * nl; nz = nz * nl;
+    float nx, ny, nz;
+    world_normal(I, B, li, nx, ny, nz);
     if ((nx * d[0] + ny * d[1]) + nz * d[2] > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
     const float hx = o[0] + t * d[0], hy = o[1] + t * d[1], hz = o[2] + t * d[2];
     const float eps = 1e-4f * fmaxf(1.0f, fmaxf(fabsf(hx), fmaxf(fabsf(hy), fabsf(hz))));

@@ -35,7 +35,7 @@ struct BlasDev {
     const float4* tris;
     const int* leafOf;
     int depth;
-    int pad;
+    int numTris;
 };
 
 struct Trace2LArgs {
@@ -78,6 +78,9 @@ struct CudaSceneHandle {
 namespace rtk {
 // cls: 0 closest hit, 1 any hit, 2 closest hit with the secondary-bounce configuration
 int launch_trace2l(const CudaSceneHandle* sc, rt::TraceScratch& scratch, Trace2LArgs& a, int cls, hipStream_t s);
+// World-space geometric normals of n hits (device arrays), on stream s.
+int launch_hit_normals(const CudaSceneHandle* sc, int n, const int* instId, const int* primId, float* nx, float* ny,
+                       float* nz, hipStream_t s);
 int trace2l_max_stack(const CudaSceneHandle* sc);
 int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams& p, const CudaPathImage& img,
                long long* rayCounts, unsigned flags);

@@ -460,4 +460,31 @@ int launch_trace2l(const CudaSceneHandle* sc, rt::TraceScratch& scratch, Trace2L
 #undef RT_L2
 }
 
+// World-space geometric normals of closest hits (instance, original
+// primitive ID); a miss or an ID outside the scene gives (0, 0, 0).
+__global__ void k_hit_normals(const InstanceDev* inst, int numInst, const BlasDev* blas, int n, const int* instId,
+                              const int* primId, float* nx, float* ny, float* nz) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    const int k = instId[i];
+    if (k >= 0 && k < numInst) {
+        const InstanceDev& I = inst[k];
+        const BlasDev& B = blas[I.blas];
+        const int p = primId[i];
+        if (p >= 0 && p < B.numTris) world_normal(I, B, B.leafOf[p], x, y, z);
+    }
+    nx[i] = x; ny[i] = y; nz[i] = z;
+}
+
+int launch_hit_normals(const CudaSceneHandle* sc, int n, const int* instId, const int* primId, float* nx, float* ny,
+                       float* nz, hipStream_t s) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_hit_normals, dim3((n + 255) / 256), dim3(256), 0, s, sc->d_inst, (int)sc->inst.size(),
+                       sc->d_blas, n, instId, primId, nx, ny, nz);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { rt::set_error("hit normals launch failed: %s", hipGetErrorString(e)); return -1; }
+    return 0;
+}
+
 }  // namespace rtk

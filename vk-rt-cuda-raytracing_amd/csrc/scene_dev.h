@@ -16,6 +16,25 @@ using rtd::Ray;
 
 constexpr int kReturnMark = INT_MIN;   // return marker: leave the instance
 
+// World-space unit geometric normal of leaf-order triangle li of instance I's
+// BLAS B: the object-space e1 x e2 (the mesh's winding, not turned toward any
+// ray) mapped with the transpose of the world-to-object 3x3 (the inverse
+// transpose of the object-to-world map), then normalised.
+__device__ __forceinline__ void world_normal(const InstanceDev& I, const BlasDev& B, int li, float& nx, float& ny,
+                                             float& nz) {
+    const float4 p0 = rtd::ldg_f4(B.tris + 3 * li), p1 = rtd::ldg_f4(B.tris + 3 * li + 1),
+                 p2 = rtd::ldg_f4(B.tris + 3 * li + 2);
+    const float e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
+    const float e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
+    const float mx = e1y * e2z - e1z * e2y, my = e1z * e2x - e1x * e2z, mz = e1x * e2y - e1y * e2x;
+    const float* W = I.w2o;
+    nx = (W[0] * mx + W[4] * my) + W[8] * mz;
+    ny = (W[1] * mx + W[5] * my) + W[9] * mz;
+    nz = (W[2] * mx + W[6] * my) + W[10] * mz;
+    const float nl = 1.0f / sqrtf((nx * nx + ny * ny) + nz * nz);
+    nx = nx * nl; ny = ny * nl; nz = nz * nl;
+}
+
 __device__ __forceinline__ void xform(const float* M, const Ray& w, Ray& o) {
     o.ox = ((M[0] * w.ox + M[1] * w.oy) + M[2] * w.oz) + M[3];
     o.oy = ((M[4] * w.ox + M[5] * w.oy) + M[6] * w.oz) + M[7];
