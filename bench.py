@@ -156,6 +156,19 @@ def main():
     log(f"primary {prim_ms:.3f} ms  bounce {bounce_ms:.3f} ms  diffuse {diff_ms:.3f} ms  shadow {shadow_ms:.3f} ms"
         f"  (wall {wall_s / args.steps * 1e3:.3f} ms/step incl. shadow)")
 
+    # Occupancy of each class's launch (VGPRs, LDS, workgroups and waves per
+    # SIMD), read back after one more untimed launch of each.
+    occupancy = {}
+    with torch.cuda.stream(stream):
+        for name, fn in (("coherent_closest", lambda: bvh.trace_pinhole_closest(W, H, c, pt, pid, pu, pv)),
+                         ("incoherent_closest", lambda: bvh.trace_closest_queued(n, cnt, D, dt, did, du, dv)),
+                         ("shadow_any", lambda: bvh.trace_any_queued(n, cnt, S, occ))):
+            fn()
+            occupancy[name] = bvh.launch_info()
+    torch.cuda.synchronize()
+    log("occupancy: " + "; ".join(f"{k}: {v['vgprs']} VGPRs, {v['lds_bytes']} B LDS, {v['waves_per_simd']} waves/SIMD"
+                                  for k, v in occupancy.items()))
+
     # Incoherent class with the ray sort (octant + Morton keys, GPU radix
     # sort, gather, trace, scatter back) for comparison; reported, not in value.
     sorted_cls = {}
@@ -274,6 +287,7 @@ def main():
             "alg_bytes_per_launch": diff_bytes, "kernel_ms": round(diff_ms, 4),
         },
         "cpu_baseline": cpu,
+        "occupancy": occupancy,
         "build_s": round(build_s, 2),
         "tuning": tuning,
     }

@@ -324,6 +324,16 @@ int cudaBVH_sceneTraceClosestQueued(CudaScene_t scene, int capacity, const int* 
 int cudaBVH_sceneTraceAnyQueued(CudaScene_t scene, int capacity, const int* d_count,
                                 const CudaRayPacket* rays, uint8_t* occluded, unsigned flags);
 
+/* Resources of the last traversal launch on the handle / scene (8 ints):
+ * VGPRs per lane, static LDS bytes per workgroup, scratch bytes per lane,
+ * workgroups per CU launched, workgroups per CU that fit, total workgroups,
+ * threads per workgroup, waves per SIMD. Zeros before the first launch.
+ * No backend counterpart (the reference fixes __launch_bounds__(256, 5),
+ * layer/cuda_bvh_backend.cu:104); it reports the occupancy that the launch
+ * configuration and the autotuner's choice give. */
+int cudaBVH_getLaunchInfo(CudaBVH_t bvh, int* info);
+int cudaBVH_sceneGetLaunchInfo(CudaScene_t scene, int* info);
+
 /* World-space unit geometric normal of each closest hit of sceneTraceClosest*
  * (reads hits->instId and hits->primId; device arrays nx, ny, nz of numRays).
  * The normal is the object-space e1 x e2 of the hit triangle (the mesh's

@@ -375,3 +375,33 @@ def test_every_launch_configuration(pkg, oracle, scenes, parity_bvh, dev, schedu
         parity_bvh.set_trace_config(pkg.CLASS_COHERENT, 0, 12, 0)
         parity_bvh.set_trace_config(pkg.CLASS_INCOHERENT, 1, 8, 0)
         parity_bvh.set_trace_config(pkg.CLASS_ANY, 0, 12, 0)
+
+
+def test_launch_info_reports_configuration(pkg, parity_scene, parity_bvh, dev):
+    """The last launch's resources follow the class's launch configuration:
+    workgroups per CU as configured (or as many as fit), 256-thread
+    workgroups on every CU, no scratch."""
+    import torch
+    _, cam = parity_scene
+    c = pkg.camera(*[np.asarray(cam[k], np.float32) for k in ("o", "f", "r", "u")], cam["scale_x"], cam["scale_y"])
+    W = H = 128
+    t = torch.empty(W * H, device=dev)
+    pid = torch.empty(W * H, dtype=torch.int32, device=dev)
+    u = torch.empty(W * H, device=dev)
+    v = torch.empty(W * H, device=dev)
+    props = torch.cuda.get_device_properties(0)
+    saved = parity_bvh.trace_config(pkg.CLASS_COHERENT)
+    for bpc in (2, 0):
+        parity_bvh.set_trace_config(pkg.CLASS_COHERENT, pkg.SCHEDULE_BATCH, 8, bpc)
+        parity_bvh.trace_pinhole_closest(W, H, c, t, pid, u, v)
+        torch.cuda.synchronize()
+        info = parity_bvh.launch_info()
+        assert info["vgprs"] > 0 and info["scratch_bytes"] == 0 and info["threads_per_workgroup"] == 256
+        assert 1 <= info["blocks_per_cu"] <= info["max_blocks_per_cu"]
+        if bpc:
+            assert info["blocks_per_cu"] == bpc
+        else:
+            assert info["blocks_per_cu"] == info["max_blocks_per_cu"]
+        assert info["workgroups"] == info["blocks_per_cu"] * props.multi_processor_count
+        assert info["waves_per_simd"] == info["blocks_per_cu"]
+    parity_bvh.set_trace_config(pkg.CLASS_COHERENT, *saved)

@@ -159,6 +159,8 @@ SIGNATURES = {
                                              C.c_uint]),
     "cudaBVH_sceneTraceAnyQueued": (_I, [_H, _I, _P, C.POINTER(CudaRayPacket), _P, C.c_uint]),
     "cudaBVH_sceneHitNormals": (_I, [_H, _I, C.POINTER(CudaSceneHit), _P, _P, _P, C.c_uint]),
+    "cudaBVH_getLaunchInfo": (_I, [_H, _P]),
+    "cudaBVH_sceneGetLaunchInfo": (_I, [_H, _P]),
     "cudaBVH_sceneSetTraceConfig": (_I, [_H, _I, _I, _I, _I]),
     "cudaBVH_sceneGetTraceConfig": (_I, [_H, _I, _P, _P, _P]),
     "cudaBVH_sceneAutotune": (_I, [_H, _I, _I, C.POINTER(CudaRayPacket), _I, _P, _P]),
@@ -344,6 +346,10 @@ class BVH:
         n = self.lib.cudaBVH_getPackedTris(self.handle, C.byref(out))
         return np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_float)), shape=(n * 4,)).reshape(n, 4).copy()
 
+    def launch_info(self) -> dict:
+        """Resources of the last traversal launch on this handle."""
+        return _launch_info(self.lib.cudaBVH_getLaunchInfo, self.handle)
+
     def leaf_prim_ids(self) -> np.ndarray:
         """Original primitive index of each leaf-order triangle (from the packed export)."""
         return self.packed_tris()[2::3, 1].view(np.int32).copy()
@@ -471,6 +477,16 @@ class BVH:
         return self.trace_config(ray_class), table
 
 
+_LAUNCH_INFO_KEYS = ("vgprs", "lds_bytes", "scratch_bytes", "blocks_per_cu", "max_blocks_per_cu", "workgroups",
+                     "threads_per_workgroup", "waves_per_simd")
+
+
+def _launch_info(fn, handle) -> dict:
+    out = (C.c_int * 8)()
+    _check(fn(handle, out), "getLaunchInfo")
+    return dict(zip(_LAUNCH_INFO_KEYS, [int(x) for x in out]))
+
+
 class Scene:
     """Owner of a two-level scene: a TLAS over instances of borrowed BLAS
     handles (:class:`BVH`), each with a 3x4 object-to-world map and albedo."""
@@ -566,6 +582,10 @@ class Scene:
         hp = CudaSceneHit(_ptr(t), _ptr(inst_id), _ptr(prim_id), _ptr(u), _ptr(v))
         _check(self.lib.cudaBVH_sceneTraceClosestQueued(self.handle, capacity, _ptr(d_count), C.byref(rp),
                                                         C.byref(hp), flags), "sceneTraceClosestQueued")
+
+    def launch_info(self) -> dict:
+        """Resources of the last traversal launch on this scene."""
+        return _launch_info(self.lib.cudaBVH_sceneGetLaunchInfo, self.handle)
 
     def hit_normals(self, inst_id, prim_id, nx, ny, nz, flags: int = 0) -> None:
         """World-space unit geometric normals of closest hits (device arrays)."""

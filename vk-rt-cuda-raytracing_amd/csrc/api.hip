@@ -1088,6 +1088,20 @@ int cudaBVH_sceneTraceClosestQueued(CudaScene_t sc, int capacity, const int* d_c
     return scene_trace(sc, capacity, d_count, rays, hits, nullptr, nullptr, nullptr, flags, false,
                        "cudaBVH_sceneTraceClosestQueued");
 }
+int cudaBVH_getLaunchInfo(CudaBVH_t bvh, int* info) {
+    rt::clear_error();
+    if (!bvh || !info) { set_error("cudaBVH_getLaunchInfo: NULL argument"); return -1; }
+    memcpy(info, bvh->scratch.launchInfo, sizeof(int) * rt::kLaunchInfoInts);
+    return 0;
+}
+
+int cudaBVH_sceneGetLaunchInfo(CudaScene_t sc, int* info) {
+    rt::clear_error();
+    if (!sc || !info) { set_error("cudaBVH_sceneGetLaunchInfo: NULL argument"); return -1; }
+    memcpy(info, sc->scratch.launchInfo, sizeof(int) * rt::kLaunchInfoInts);
+    return 0;
+}
+
 int cudaBVH_sceneHitNormals(CudaScene_t sc, int numRays, const CudaSceneHit* hits, float* nx, float* ny, float* nz,
                             unsigned flags) {
     rt::clear_error();

@@ -89,6 +89,19 @@ struct TraceConfig {
     int blocksPerCU = 0;
 };
 
+// Resources of one traversal kernel, read once per kernel (hipFuncGetAttributes
+// and the occupancy calculator).
+struct KernelInfo {
+    int maxBlocks = 0;                  // resident workgroups per CU
+    int vgprs = 0, ldsBytes = 0, scratchBytes = 0;
+};
+
+// What the last traversal launch of a handle (or scene) used: VGPRs per lane,
+// static LDS bytes per workgroup, scratch bytes per lane, workgroups per CU
+// launched, workgroups per CU that fit, total workgroups, threads per
+// workgroup, waves per SIMD.
+enum { kLaunchInfoInts = 8 };
+
 struct TraceScratch {
     void* spill = nullptr;
     size_t spillBytes = 0;
@@ -96,6 +109,7 @@ struct TraceScratch {
     int* status = nullptr;
     int* tiles = nullptr;               // per-tile counts of the compaction scans
     size_t tileCap = 0;
+    int launchInfo[kLaunchInfoInts] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 }  // namespace rt

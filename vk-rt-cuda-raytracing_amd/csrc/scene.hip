@@ -18,6 +18,7 @@
 #include "rt_internal.h"
 #include "scene.h"
 #include "scene_dev.h"
+#include "trace_launch.h"
 
 namespace rtk {
 
@@ -391,22 +392,19 @@ __global__ void k_reset_counter_2l(uint32_t* c) { c[threadIdx.x * rtd::kBandStri
 
 template <bool ANY, int LDSD, int SCHED>
 static int launch2(Trace2LArgs& a, rt::TraceScratch& sc, int maxStack, int blocksPerCU, hipStream_t s) {
-    static int maxBlocks = 0;
+    static rt::KernelInfo ki;
     // SCHED: 0 while-while, 1 batch, 2 batch with nearest-first any hit, 3 refill
     auto kern = SCHED == 0   ? k_trace2l<ANY, LDSD, kBlock2>
                 : SCHED == 1 ? k_trace2l_batch<ANY, false, LDSD, kBlock2>
                 : SCHED == 2 ? k_trace2l_batch<ANY, true, LDSD, kBlock2>
                              : k_trace2l_refill<ANY, LDSD, kBlock2>;
-    if (!maxBlocks) {
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock2, 0) != hipSuccess || nb <= 0) nb = 1;
-        maxBlocks = nb;
-    }
+    rt::note_launch(kern, kBlock2, ki, 0, 0, nullptr);
     int dev = 0, cus = 1;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int perCU = blocksPerCU > 0 ? std::min(blocksPerCU, maxBlocks) : maxBlocks;
+    const int perCU = blocksPerCU > 0 ? std::min(blocksPerCU, ki.maxBlocks) : ki.maxBlocks;
     const int grid = perCU * cus, threads = grid * kBlock2;
+    rt::note_launch(kern, kBlock2, ki, perCU, grid, sc.launchInfo);
     a.maxStack = maxStack;
     const int spillDepth = maxStack > LDSD ? maxStack - LDSD : 0;
     const size_t need = (size_t)spillDepth * threads * sizeof(int2);

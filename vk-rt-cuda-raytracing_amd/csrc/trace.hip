@@ -850,14 +850,11 @@ static int num_cus() {
 // Wave-packet kernels index the spill region per wave (spillStride = waves).
 template <class K>
 static int launch_kernel(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, K kern, int ldsd, bool perWave,
-                         int blocksPerCU, int& maxBlocks, hipStream_t s) {
-    if (!maxBlocks) {
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, 0) != hipSuccess || nb <= 0) nb = 1;
-        maxBlocks = nb;
-    }
-    const int bpc = (blocksPerCU > 0 && blocksPerCU < maxBlocks) ? blocksPerCU : maxBlocks;
+                         int blocksPerCU, rt::KernelInfo& ki, hipStream_t s) {
+    rt::note_launch(kern, kBlock, ki, 0, 0, nullptr);
+    const int bpc = (blocksPerCU > 0 && blocksPerCU < ki.maxBlocks) ? blocksPerCU : ki.maxBlocks;
     const int grid = bpc * num_cus();
+    rt::note_launch(kern, kBlock, ki, bpc, grid, sc.launchInfo);
     const int threads = grid * kBlock;
     a.maxStack = 3 * h->depth4 + 4;
     const int spillDepth = a.maxStack > ldsd ? a.maxStack - ldsd : 0;
@@ -891,17 +888,17 @@ static int launch_kernel(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs
 // SCHED: rt::kScheduleBatch, kScheduleWhileWhile or kScheduleRefill (uncounted)
 template <int MODE, bool MT, bool PIN, bool COUNT, int LDSD, int SCHED>
 static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int blocksPerCU, hipStream_t s) {
-    static int maxBlocks = 0;
+    static rt::KernelInfo ki;
     auto kern = SCHED == rt::kScheduleWhileWhile ? k_trace_ww<MODE, MT, PIN, COUNT, LDSD, kBlock>
               : (SCHED == rt::kScheduleRefill && !COUNT) ? k_trace_refill<MODE, MT, PIN, LDSD, kBlock>
                                                          : k_trace_batch<MODE, MT, PIN, COUNT, LDSD, kBlock>;
-    return launch_kernel(h, sc, a, kern, LDSD, false, blocksPerCU, maxBlocks, s);
+    return launch_kernel(h, sc, a, kern, LDSD, false, blocksPerCU, ki, s);
 }
 
 template <int MODE, bool MT, int LDSD>
 static int launch_packet(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int blocksPerCU, hipStream_t s) {
-    static int maxBlocks = 0;
-    return launch_kernel(h, sc, a, k_trace_packet<MODE, MT, LDSD, kBlock>, LDSD, true, blocksPerCU, maxBlocks, s);
+    static rt::KernelInfo ki;
+    return launch_kernel(h, sc, a, k_trace_packet<MODE, MT, LDSD, kBlock>, LDSD, true, blocksPerCU, ki, s);
 }
 
 int alloc_scratch(rt::TraceScratch& sc) {

@@ -8,6 +8,29 @@
 #include "rt_device.h"
 #include "rt_internal.h"
 
+namespace rt {
+
+// Fills ki on its first use for kernel kern (block threads per workgroup) and
+// records a launch of blocksPerCU workgroups per CU in info.
+template <class K>
+inline void note_launch(K kern, int block, KernelInfo& ki, int blocksPerCU, int grid, int* info) {
+    if (!ki.maxBlocks) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, block, 0) != hipSuccess || nb <= 0) nb = 1;
+        hipFuncAttributes fa{};
+        if (hipFuncGetAttributes(&fa, (const void*)kern) == hipSuccess) {
+            ki.vgprs = fa.numRegs; ki.ldsBytes = (int)fa.sharedSizeBytes; ki.scratchBytes = (int)fa.localSizeBytes;
+        }
+        ki.maxBlocks = nb;
+    }
+    if (!info) return;
+    info[0] = ki.vgprs; info[1] = ki.ldsBytes; info[2] = ki.scratchBytes;
+    info[3] = blocksPerCU; info[4] = ki.maxBlocks; info[5] = grid; info[6] = block;
+    info[7] = blocksPerCU * (block / 64) / 4;
+}
+
+}  // namespace rt
+
 namespace rtk {
 
 enum { kModeClosest = 0, kModeAny = 1, kModeShade = 2 };
