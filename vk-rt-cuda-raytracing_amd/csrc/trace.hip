@@ -130,6 +130,13 @@ struct Result {
     int visits, tests;
 };
 
+// Slot of stack entry sp (>= LDSD) of thread gtid in the HBM spill region.
+// Laid out [depth][thread] (a lane-major layout measured the same).
+template <int LDSD>
+__device__ __forceinline__ size_t spill_index(const TraceArgs& a, int sp, int gtid) {
+    return (size_t)(sp - LDSD) * a.spillStride + gtid;
+}
+
 // First BVH4 node of ray ri: 0, or in a forest its scene's root.
 __device__ __forceinline__ int ray_root(const TraceArgs& a, int ri) {
     if (!a.roots) return 0;
@@ -185,7 +192,7 @@ __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int*
             s_ref[sp * BLOCK + tid] = rf;
             s_tn[sp * BLOCK + tid] = t;
         } else if (sp < a.maxStack) {
-            a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid] = make_int2(rf, __float_as_int(t));
+            a.spill[spill_index<LDSD>(a, sp, gtid)] = make_int2(rf, __float_as_int(t));
         } else {
             atomicOr(a.status, 1);
             return;
@@ -231,7 +238,7 @@ __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int*
             // LDS read above into one generic (flat) load, which would send
             // every pop through the vector memory path
             typedef int v2i __attribute__((ext_vector_type(2)));
-            const v2i e = __builtin_nontemporal_load((const v2i*)&a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid]);
+            const v2i e = __builtin_nontemporal_load((const v2i*)&a.spill[spill_index<LDSD>(a, sp, gtid)]);
             ref = e.x; tn = __int_as_float(e.y);
         }
     }
@@ -271,7 +278,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_refill(TraceArgs a) {
             s_ref[sp * BLOCK + tid] = rf;
             s_tn[sp * BLOCK + tid] = t;
         } else if (sp < a.maxStack) {
-            a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid] = make_int2(rf, __float_as_int(t));
+            a.spill[spill_index<LDSD>(a, sp, gtid)] = make_int2(rf, __float_as_int(t));
         } else {
             atomicOr(a.status, 1);
             return;
@@ -376,7 +383,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_refill(TraceArgs a) {
                     ref = s_ref[sp * BLOCK + tid];
                     tn = s_tn[sp * BLOCK + tid];
                 } else {
-                    const int2 e = a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid];
+                    const int2 e = a.spill[spill_index<LDSD>(a, sp, gtid)];
                     ref = e.x; tn = __int_as_float(e.y);
                 }
             }
@@ -514,7 +521,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_ww(TraceArgs a) {
             s_ref[sp * BLOCK + tid] = rf;
             s_tn[sp * BLOCK + tid] = t;
         } else if (sp < a.maxStack) {
-            a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid] = make_int2(rf, __float_as_int(t));
+            a.spill[spill_index<LDSD>(a, sp, gtid)] = make_int2(rf, __float_as_int(t));
         } else {
             atomicOr(a.status, 1);   // unreachable: maxStack bounds a DFS of this BVH
             return;
@@ -606,7 +613,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_ww(TraceArgs a) {
                             ref = s_ref[sp * BLOCK + tid];
                             tn = s_tn[sp * BLOCK + tid];
                         } else {
-                            const int2 e = a.spill[(size_t)(sp - LDSD) * a.spillStride + gtid];
+                            const int2 e = a.spill[spill_index<LDSD>(a, sp, gtid)];
                             ref = e.x; tn = __int_as_float(e.y);
                         }
                     }
