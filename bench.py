@@ -103,7 +103,7 @@ def main():
         if ev: ev[0].record(stream)
         bvh.trace_pinhole_closest(W, H, c, pt, pid, pu, pv)
         if ev: ev[1].record(stream)
-        bvh.gen_bounce_rays(W, H, c, pt, pid, 1234, light, D, S, src, cnt)
+        bvh.gen_bounce_rays(W, H, c, pt, pid, 1234, light, D, S, src, cnt, flags=pkg.FLAG_TILE_ORDER)
         if ev: ev[2].record(stream)
         bvh.trace_closest_queued(n, cnt, D, dt, did, du, dv)
         if ev: ev[3].record(stream)

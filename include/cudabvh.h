@@ -170,7 +170,8 @@ enum {
     CUDABVH_FLAG_SORT_RAYS     = 1u << 5, /* octant+Morton sort before tracing */
     CUDABVH_FLAG_SYNC          = 1u << 6, /* wait for completion before returning */
     CUDABVH_FLAG_GRAPH         = 1u << 7, /* capture the launches in a hipGraph once, replay them */
-    CUDABVH_FLAG_CWBVH         = 1u << 8  /* trace the compressed 8-wide BVH (built on first use) */
+    CUDABVH_FLAG_CWBVH         = 1u << 8, /* trace the compressed 8-wide BVH (built on first use) */
+    CUDABVH_FLAG_TILE_ORDER    = 1u << 9  /* genBounceRays: compact in 8x8-tile order, not pixel order */
 };
 
 /* Build from geometry already in device memory (no counterpart in the
@@ -215,7 +216,10 @@ int cudaBVH_tracePinholeClosest(CudaBVH_t bvh, int width, int height, const Cuda
                                 const CudaHitPacket* hits, unsigned flags);
 
 /* Secondary rays of a pinhole frame (camera as in tracePinholeClosest): for
- * every pixel whose primary hit has primId >= 0, in pixel order, one
+ * every pixel whose primary hit has primId >= 0, in pixel order (with
+ * CUDABVH_FLAG_TILE_ORDER: 8x8 pixel tiles in 64-pixel-wide vertical strips,
+ * row-major inside a tile, then the margins row-major, so that rays taken
+ * together by the traversal come from a compact image block), one
  * cosine-weighted diffuse ray about the geometric normal facing the camera
  * (PCG32 stream seeded by (seed, pixel), t in (1e-3, 1e30)) and one shadow
  * ray toward light (t in (1e-3, distance)). Outputs are device packets of

@@ -67,3 +67,58 @@ def test_bounce_chain(pkg, oracle, scenes, dev, res, spheres, W, H):
         assert np.array_equal(occ.cpu().numpy()[:m], rs["occ"])
         assert (occ.cpu().numpy()[m:] == 9).all()
         assert 0.0 < rs["occ"].mean() < 1.0 and 0.0 < (rd["id"] >= 0).mean() < 1.0
+
+
+def _tile_strip_order(W, H):
+    """Pixels in the order of CUDABVH_FLAG_TILE_ORDER: complete 8x8 tiles in
+    vertical strips of 8 tiles (row-major inside a strip and inside a tile),
+    then the right margin and the bottom rows, row-major."""
+    tx8, ty8 = W // 8, H // 8
+    order = []
+    for s0 in range(0, tx8, 8):
+        for ty in range(ty8):
+            for tx in range(s0, min(s0 + 8, tx8)):
+                for k in range(64):
+                    order.append((ty * 8 + k // 8) * W + tx * 8 + k % 8)
+    for y in range(ty8 * 8):
+        order.extend(y * W + x for x in range(tx8 * 8, W))
+    for y in range(ty8 * 8, H):
+        order.extend(y * W + x for x in range(W))
+    assert sorted(order) == list(range(W * H))
+    return np.array(order)
+
+
+@pytest.mark.parametrize("W,H", [(320, 180), (257, 131)])
+def test_bounce_tile_order(pkg, scenes, dev, W, H):
+    """FLAG_TILE_ORDER emits the same rays as pixel order, permuted into the
+    tile walk: srcIndex lists the hit pixels in that order, and every ray
+    equals the pixel-order ray of its source pixel."""
+    import torch
+    tris = scenes.terrain_scene(res=128, spheres=16, level=2)
+    cam = scenes.terrain_camera(W, H)
+    c = pkg.camera(cam["o"], cam["f"], cam["r"], cam["u"], cam["scale_x"], cam["scale_y"])
+    n = W * H
+    with pkg.BVH.build(tris) as bvh:
+        t = torch.empty(n, device=dev)
+        ids = torch.empty(n, dtype=torch.int32, device=dev)
+        bvh.trace_pinhole_closest(W, H, c, t, ids)
+        out = []
+        for flags in (0, pkg.FLAG_TILE_ORDER):
+            D = {k: torch.full((n,), -7.0, device=dev) for k in RAY_KEYS}
+            S = {k: torch.full((n,), -7.0, device=dev) for k in RAY_KEYS}
+            src = torch.full((n,), -1, dtype=torch.int32, device=dev)
+            cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+            bvh.gen_bounce_rays(W, H, c, t, ids, 99, scenes.TERRAIN_LIGHT, D, S, src, cnt, flags=flags)
+            torch.cuda.synchronize()
+            m = int(cnt.item())
+            out.append((m, src.cpu().numpy()[:m], np.stack([D[k].cpu().numpy()[:m] for k in RAY_KEYS]),
+                        np.stack([S[k].cpu().numpy()[:m] for k in RAY_KEYS])))
+        hid = ids.cpu().numpy()
+    (m0, s0, d0, h0), (m1, s1, d1, h1) = out
+    assert m0 == m1 == (hid >= 0).sum()
+    order = _tile_strip_order(W, H)
+    assert np.array_equal(s1, order[hid[order] >= 0])
+    pos = np.full(n, -1)
+    pos[s0] = np.arange(m0)
+    assert np.array_equal(_bits(d1), _bits(d0[:, pos[s1]]))
+    assert np.array_equal(_bits(h1), _bits(h0[:, pos[s1]]))

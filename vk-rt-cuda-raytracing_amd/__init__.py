@@ -32,6 +32,7 @@ FLAG_SORT_RAYS = 1 << 5
 FLAG_SYNC = 1 << 6
 FLAG_GRAPH = 1 << 7
 FLAG_CWBVH = 1 << 8
+FLAG_TILE_ORDER = 1 << 9
 PT_MAX_DEPTH = 16
 MISS_T = np.float32(1e30)
 

@@ -773,8 +773,13 @@ int cudaBVH_genBounceRays(CudaBVH_t bvh, int width, int height, const CudaCamera
         set_error("cudaBVH_genBounceRays: bad arguments");
         return -1;
     }
+    if (flags & ~(unsigned)(CUDABVH_FLAG_SYNC | CUDABVH_FLAG_TILE_ORDER)) {
+        set_error("cudaBVH_genBounceRays: unsupported flags 0x%x", flags);
+        return -1;
+    }
     int rc = rtk::launch_bounce(bvh, bvh->scratch, to_cam(*cam), width, height, primary->t, primary->primId, seed,
-                                light, *diffuseOut, *shadowOut, srcIndex, d_count, hstream(bvh));
+                                light, *diffuseOut, *shadowOut, srcIndex, d_count, (flags & CUDABVH_FLAG_TILE_ORDER) != 0,
+                                hstream(bvh));
     if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(hstream(bvh)), return -1);
     return rc;
 }

@@ -35,17 +35,24 @@ struct BounceArgs {
     int* src;
     int* count;
     int* tileCounts;
+    int tileOrder;          // walk pixels in rtd::tile_strip_pixel order instead of row-major
 };
 
-// Pixel i of tile b, step k: b*kTile + k*kTileThreads + threadIdx.x, so the
-// lanes of a wave read and write consecutive pixels.
+// Pixel of compaction slot j.
+__device__ __forceinline__ int bounce_pixel(const BounceArgs& a, int j) {
+    return a.tileOrder ? rtd::tile_strip_pixel(j, a.W, a.H) : j;
+}
+
+// Slot j of tile b, step k: b*kTile + k*kTileThreads + threadIdx.x, so the
+// lanes of a wave handle consecutive slots (consecutive pixels in row-major
+// order, 8x8 pixel tiles in tile order).
 __global__ __launch_bounds__(kTileThreads) void k_count_hits(BounceArgs a) {
     __shared__ int lds[kTileThreads / 64 + 1];
     const int base = blockIdx.x * kTile + threadIdx.x;
     int c = 0;
     for (int k = 0; k < kTileItems; k++) {
-        const int i = base + k * kTileThreads;
-        if (i < a.n && a.hitId[i] >= 0) c++;
+        const int j = base + k * kTileThreads;
+        if (j < a.n && a.hitId[bounce_pixel(a, j)] >= 0) c++;
     }
     int total;
     rts::block_exclusive_scan<kTileThreads>(c, lds, &total);
@@ -106,8 +113,9 @@ __global__ __launch_bounds__(kTileThreads) void k_emit(BounceArgs a) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int running = a.tileCounts[blockIdx.x];
     for (int k = 0; k < kTileItems; k++) {
-        const int i = blockIdx.x * kTile + k * kTileThreads + threadIdx.x;
-        const bool hit = i < a.n && a.hitId[i] >= 0;
+        const int j = blockIdx.x * kTile + k * kTileThreads + threadIdx.x;
+        const int i = j < a.n ? bounce_pixel(a, j) : 0;
+        const bool hit = j < a.n && a.hitId[i] >= 0;
         const uint64_t m = __ballot(hit);
         if (lane == 0) waveCount[w] = __popcll(m);
         __syncthreads();
@@ -126,7 +134,7 @@ __global__ __launch_bounds__(kTileThreads) void k_emit(BounceArgs a) {
 
 int launch_bounce(const CudaBVHHandle* h, rt::TraceScratch& sc, const rtd::Cam& cam, int W, int H,
                   const float* hitT, const int* hitId, uint32_t seed, const float light[3], const CudaRayPacket& d,
-                  const CudaRayPacket& s, int* src, int* count, hipStream_t st) {
+                  const CudaRayPacket& s, int* src, int* count, bool tileOrder, hipStream_t st) {
     const int n = W * H;
     const int tiles = (n + kTile - 1) / kTile;
     if (alloc_scratch(sc) != 0) return -1;
@@ -152,6 +160,7 @@ int launch_bounce(const CudaBVHHandle* h, rt::TraceScratch& sc, const rtd::Cam& 
     a.sdx = (float*)s.dx; a.sdy = (float*)s.dy; a.sdz = (float*)s.dz;
     a.stmin = (float*)s.tmin; a.stmax = (float*)s.tmax;
     a.src = src; a.count = count; a.tileCounts = sc.tiles;
+    a.tileOrder = tileOrder ? 1 : 0;
     hipLaunchKernelGGL(k_count_hits, dim3(tiles), dim3(kTileThreads), 0, st, a);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, st, sc.tiles, tiles, count);
     hipLaunchKernelGGL(k_emit, dim3(tiles), dim3(kTileThreads), 0, st, a);

@@ -106,33 +106,7 @@ __global__ void k_pt_begin(PtArgs a, uint32_t seed) {
     for (int k = t; k < 2 * kPtMaxDepth; k += blockDim.x) a.totals[k] = 0ull;
 }
 
-// Pixel of camera-ray slot i: the complete 8x8 pixel tiles first, one per
-// 64 consecutive slots (so a wave's 64 rays are one tile), tiles in vertical
-// strips 8 tiles wide and row-major inside a strip; then the pixels of the
-// right and bottom margins that no complete tile covers, row-major. A
-// bijection on [0, W*H). The traversal's XCD bands then give each XCD a few
-// neighbouring strips of the image.
-__device__ __forceinline__ int pt_pixel(int i, int W, int H) {
-    const int tx8 = W >> 3, ty8 = H >> 3, inTiles = tx8 * ty8 * 64;
-    if (i < inTiles) {
-        const int tile = i >> 6, k = i & 63;
-        const int full = tx8 / 8, per = 8 * ty8;
-        int strip = tile / per, r = tile - strip * per, w = 8;
-        if (strip >= full) { strip = full; r = tile - full * per; w = tx8 - full * 8; }
-        const int ty = r / w, tx = strip * 8 + (r - ty * w);
-        return (ty * 8 + (k >> 3)) * W + tx * 8 + (k & 7);
-    }
-    int j = i - inTiles;
-    const int rw = W - tx8 * 8, H8 = ty8 * 8;
-    if (j < H8 * rw) {
-        const int y = j / rw;
-        return y * W + tx8 * 8 + (j - y * rw);
-    }
-    j -= H8 * rw;
-    return (H8 + j / W) * W + j % W;
-}
-
-// Camera rays, slot i holding pixel pt_pixel(i) (jittered pinhole: the pixel
+// Camera rays, slot i holding pixel rtd::tile_strip_pixel(i) (jittered pinhole: the pixel
 // offset is two uniforms instead of 0.5; the pixel's random stream is keyed
 // by its index, not the slot), throughput 1; clears the image on sample 0
 // and resets the queue counters.
@@ -144,7 +118,7 @@ __global__ __launch_bounds__(kPtBlock) void k_pt_generate(PtArgs a) {
     if (g < 2 * kPtMaxDepth) a.cnt[g] = g == 0 ? a.N : 0;
     if (g < 16 * kPtMaxDepth) a.oct[g] = 0;
     for (int i = g; i < a.N; i += stride) {
-        const int p = pt_pixel(i, a.W, a.H);
+        const int p = rtd::tile_strip_pixel(i, a.W, a.H);
         const int py = p / a.W, px = p - py * a.W;
         rtd::Pcg32 rng;
         rng.seed(((uint64_t)seed << 32) ^ (uint64_t)(uint32_t)sample, (uint64_t)p);

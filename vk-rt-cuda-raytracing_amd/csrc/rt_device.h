@@ -184,6 +184,33 @@ __device__ __forceinline__ void pinhole_ray(const Cam& c, int px, int py, int W,
     r.tmin = 0.0f; r.tmax = RT_MISS_T;
 }
 
+// Pixel of slot i when a W x H image is walked tile by tile: the complete
+// 8x8 pixel tiles first, 64 consecutive slots per tile (row-major inside),
+// tiles in vertical strips 8 tiles (64 pixels) wide and row-major inside a
+// strip, strips left to right; then the pixels of the right and bottom
+// margins that no complete tile covers, row-major. A bijection on [0, W*H).
+// Work taken in slot order then stays in compact image blocks, which keeps
+// neighbouring rays on one XCD's L2.
+__device__ __forceinline__ int tile_strip_pixel(int i, int W, int H) {
+    const int tx8 = W >> 3, ty8 = H >> 3, inTiles = tx8 * ty8 * 64;
+    if (i < inTiles) {
+        const int tile = i >> 6, k = i & 63;
+        const int full = tx8 / 8, per = 8 * ty8;
+        int strip = tile / per, r = tile - strip * per, w = 8;
+        if (strip >= full) { strip = full; r = tile - full * per; w = tx8 - full * 8; }
+        const int ty = r / w, tx = strip * 8 + (r - ty * w);
+        return (ty * 8 + (k >> 3)) * W + tx * 8 + (k & 7);
+    }
+    int j = i - inTiles;
+    const int rw = W - tx8 * 8, H8 = ty8 * 8;
+    if (j < H8 * rw) {
+        const int y = j / rw;
+        return y * W + tx8 * 8 + (j - y * rw);
+    }
+    j -= H8 * rw;
+    return (H8 + j / W) * W + j % W;
+}
+
 // 16-byte loads through the global address space. A pointer chosen at run
 // time between arrays the compiler cannot trace to global memory (the TLAS
 // or an instance's BLAS, read from a descriptor) otherwise turns its loads

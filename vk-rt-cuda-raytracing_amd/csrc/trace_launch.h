@@ -70,7 +70,7 @@ hipError_t launch_gather_tris(const void* verts, int numVertices, int stride, co
                               int numTris, float* out, int* bad, hipStream_t s);
 int launch_bounce(const CudaBVHHandle* h, rt::TraceScratch& sc, const rtd::Cam& cam, int W, int H,
                   const float* hitT, const int* hitId, uint32_t seed, const float light[3], const CudaRayPacket& d,
-                  const CudaRayPacket& s, int* src, int* count, hipStream_t st);
+                  const CudaRayPacket& s, int* src, int* count, bool tileOrder, hipStream_t st);
 
 int launch_trace_cw(const void* nodes, const void* tris, int depth8, rt::TraceScratch& sc, TraceArgs& a, int mode,
                     hipStream_t s);
