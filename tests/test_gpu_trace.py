@@ -405,3 +405,42 @@ def test_launch_info_reports_configuration(pkg, parity_scene, parity_bvh, dev):
         assert info["workgroups"] == info["blocks_per_cu"] * props.multi_processor_count
         assert info["waves_per_simd"] == info["blocks_per_cu"]
     parity_bvh.set_trace_config(pkg.CLASS_COHERENT, *saved)
+
+
+@pytest.mark.parametrize("W,H", [(1, 1), (9, 3), (17, 9), (67, 45), (130, 72)])
+def test_fused_pinhole_odd_sizes(pkg, oracle, scenes, parity_scene, parity_bvh, dev, W, H):
+    """Image sizes that leave partial 8x8 tiles, a partial last tile strip,
+    fewer tiles than XCD bands or a single pixel: every schedule writes
+    every pixel exactly as the oracle traces it, and nothing past W*H."""
+    import torch
+    _, cam = parity_scene
+    c = pkg.camera(cam["o"], cam["f"], cam["r"], cam["u"], cam["scale_x"], cam["scale_y"])
+    ref_rays = oracle.gen_pinhole(scenes.cam_array(cam), W, H)
+    nodes, lt, pid = _oracle_inputs(parity_bvh)
+    ref = oracle.trace_closest(nodes, lt, pid, ref_rays)
+    saved = parity_bvh.trace_config(pkg.CLASS_COHERENT)
+    for sched in (pkg.SCHEDULE_BATCH, pkg.SCHEDULE_WHILE_WHILE, pkg.SCHEDULE_REFILL, pkg.SCHEDULE_PACKET):
+        parity_bvh.set_trace_config(pkg.CLASS_COHERENT, sched, 8, 0)
+        hb = {k: torch.full((W * H + 64,), -3, dtype=torch.int32 if k == "prim_id" else torch.float32, device=dev)
+              for k in ("t", "prim_id", "u", "v")}
+        parity_bvh.trace_pinhole_closest(W, H, c, hb["t"], hb["prim_id"], hb["u"], hb["v"])
+        torch.cuda.synchronize()
+        got = {k: v.cpu().numpy() for k, v in hb.items()}
+        _assert_closest_equal({k: v[:W * H] for k, v in got.items()}, ref)
+        assert (got["prim_id"][W * H:] == -3).all(), sched
+    parity_bvh.set_trace_config(pkg.CLASS_COHERENT, *saved)
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 65, 511, 4097])
+def test_closest_small_ray_counts(oracle, parity_scene, parity_bvh, dev, n):
+    """Ray counts below, at and just past the 64-ray chunk and the eight
+    XCD bands: every ray is traced once, as the oracle traces it."""
+    _, cam = parity_scene
+    rng = np.random.default_rng(n)
+    rays = np.zeros((8, n), np.float32)
+    d = rng.normal(size=(3, n)); d /= np.linalg.norm(d, axis=0)
+    rays[0:3] = rng.uniform(-1.5, 1.5, (3, n)); rays[3:6] = d
+    rays[6] = 0.0; rays[7] = 1e30
+    nodes, lt, pid = _oracle_inputs(parity_bvh)
+    ref = oracle.trace_closest(nodes, lt, pid, rays)
+    _assert_closest_equal(_gpu_closest(parity_bvh, rays, dev), ref)
