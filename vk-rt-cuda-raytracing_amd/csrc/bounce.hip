@@ -1,8 +1,10 @@
 // bounce.hip — secondary rays from primary hits: one cosine-weighted diffuse
-// bounce and one shadow ray toward a point light per hit pixel, compacted in
-// pixel order (stable) with wave64 scans.
+// bounce and one shadow ray toward a point light per hit pixel, compacted
+// (stable) with wave64 scans in pixel order or, on request, in the tile walk
+// of rtd::tile_strip_pixel (8x8 pixel tiles in 64-pixel strips), which keeps
+// the rays that a trace takes together in a compact image block.
 //
-// Three launches, no host round trip: count hits per 2048-pixel tile,
+// Three launches, no host round trip: count hits per 2048-slot tile,
 // exclusive-scan the tile counts (which also writes the total to device
 // memory, where the queued traces read it), then emit rays at their compacted
 // positions. The diffuse direction follows the reference's diffuse-ray

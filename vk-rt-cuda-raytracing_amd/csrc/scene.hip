@@ -197,7 +197,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l(Trace2LArgs a) {
     }
 }
 
-// Batch schedule: each wave takes 64 rays from the device counter and each
+// Batch schedule: each wave takes 64 rays from its XCD band (rtd::take_chunk) and each
 // lane runs its ray's whole traversal (trace2l_ray) with an LDS stack.
 template <bool ANY, bool ORD, int LDSD, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_trace2l_batch(Trace2LArgs a) {
