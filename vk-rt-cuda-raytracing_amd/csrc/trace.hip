@@ -253,7 +253,7 @@ __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int*
 // are idle, so a wave does not wait for the slowest ray of its batch and
 // the ray set-up of new rays is shared by many lanes. Same depth-first order
 // per ray as trace_ray; uncounted.
-constexpr int kRefillMin = 16;
+constexpr int kRefillMin = 8;   // 8 measured ~1 % faster than 4, 16 or 24 on the 1080p diffuse rays
 
 template <int MODE, bool MT, bool PIN, int LDSD, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_trace_refill(TraceArgs a) {
