@@ -246,7 +246,8 @@ int cudaBVH_traceAnyQueued(CudaBVH_t bvh, int capacity, const int* d_count,
  * traversal stack; 3 refill: the batch schedule's loop with idle lanes
  * refilled from the wave-local pool),
  * LDS stack depth per lane (4, 8, 12 or 16 entries; deeper entries spill to
- * HBM) and resident workgroups per CU (0 = occupancy limit). */
+ * HBM) and resident workgroups per CU (0 = occupancy limit). Defaults after a
+ * build: batch / 12 / 0 for classes 0 and 2, refill / 12 / 0 for class 1. */
 int cudaBVH_setTraceConfig(CudaBVH_t bvh, int rayClass, int schedule, int ldsDepth, int blocksPerCU);
 int cudaBVH_getTraceConfig(CudaBVH_t bvh, int rayClass, int* schedule, int* ldsDepth, int* blocksPerCU);
 
