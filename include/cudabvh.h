@@ -359,7 +359,8 @@ int cudaBVH_sceneHitNormals(CudaScene_t scene, int numRays, const CudaSceneHit* 
  * idle lanes refilled from a wave-local pool), LDS stack depth (4, 8, 12, 16)
  * and resident workgroups per CU (0 = as many as fit). sceneAutotune times
  * every candidate on the caller's rays (results: schedule, depth, blocks/CU,
- * ms per candidate; up to 48) and keeps the fastest. */
+ * ms per candidate; up to 48) and keeps the fastest. Defaults after
+ * sceneCreate: 1 / 12 / 0 for every class. */
 int cudaBVH_sceneSetTraceConfig(CudaScene_t scene, int rayClass, int schedule, int ldsDepth, int blocksPerCU);
 int cudaBVH_sceneGetTraceConfig(CudaScene_t scene, int rayClass, int* schedule, int* ldsDepth, int* blocksPerCU);
 int cudaBVH_sceneAutotune(CudaScene_t scene, int rayClass, int numRays, const CudaRayPacket* rays, int reps,

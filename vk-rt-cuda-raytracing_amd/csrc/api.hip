@@ -973,10 +973,13 @@ CudaScene_t cudaBVH_sceneCreate(const CudaBVH_t* blas, int numBlas, const CudaIn
     for (int b = 0; b < numBlas; b++)
         if (!blas[b]) { set_error("cudaBVH_sceneCreate: BLAS %d is NULL", b); return nullptr; }
     auto* sc = new CudaSceneHandle();
-    sc->tune2l[0].schedule = 0;      // while-while
-    sc->tune2l[0].ldsDepth = 8;
-    sc->tune2l[1].schedule = 0;
-    sc->tune2l[1].ldsDepth = 8;
+    // defaults: batch / LDS 12, what the autotuner picks for all three
+    // classes on the instanced 1080p scene (while-while / 8 measured 15-70 %
+    // slower there)
+    sc->tune2l[0].schedule = 1;
+    sc->tune2l[0].ldsDepth = 12;
+    sc->tune2l[1].schedule = 1;
+    sc->tune2l[1].ldsDepth = 12;
     sc->tune2l[2] = sc->tune2l[0];
     sc->blas.assign(blas, blas + numBlas);
     sc->inst.resize(numInst);
