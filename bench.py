@@ -237,6 +237,42 @@ def main():
     achieved = diff_bytes / (diff_ms * 1e-3) / 1e9
     prim_bytes = 64 * p_nodes + 36 * p_tris + 16 * n
 
+    # The reference boundary's host-pointer calls: tracePrimary (side x side
+    # rays from a point looking +z, hit distances copied to a host array) and
+    # traceDiffuse (host SoA rays in, host hit distances out). Their rates
+    # include the PCIe copies and are reported beside the value, not in it.
+    host_calls = {}
+    with torch.cuda.stream(stream):
+        side = 1440
+        hit_host = np.zeros(side * side, np.float32)
+        cam_o = (0.0, 2.0, -12.0)
+        bvh.trace_primary(side, cam_o, hit_host)                 # warm-up
+        reps = 5
+        t0 = time.perf_counter()
+        kern = [bvh.trace_primary(side, cam_o, hit_host) for _ in range(reps)]
+        prim_wall = (time.perf_counter() - t0) / reps
+        o_h = torch.stack([D[k][:m] for k in ("ox", "oy", "oz")], 1).cpu().numpy()
+        d_h = torch.stack([D[k][:m] for k in ("dx", "dy", "dz")], 1).cpu().numpy()
+        dh = np.zeros(m, np.float32)
+        bvh.trace_diffuse(o_h, d_h, dh)
+        t0 = time.perf_counter()
+        kd = [bvh.trace_diffuse(o_h, d_h, dh) for _ in range(reps)]
+        diff_wall = (time.perf_counter() - t0) / reps
+    host_calls = {
+        "tracePrimary": {"rays": side * side, "wall_ms": round(prim_wall * 1e3, 3),
+                         "mrays_s_incl_pcie": round(side * side / prim_wall / 1e6, 1),
+                         "mrays_s_reported": round(float(np.median(kern)), 1),
+                         "hit_rate": round(float((hit_host < 1e29).mean()), 4)},
+        "traceDiffuse": {"rays": m, "wall_ms": round(diff_wall * 1e3, 3),
+                         "mrays_s_incl_pcie": round(m / diff_wall / 1e6, 1),
+                         "mrays_s_reported": round(float(np.median(kd)), 1),
+                         "note": "pageable host arrays, GPU octant+Morton sort included"},
+    }
+    log(f"host-pointer boundary: tracePrimary {host_calls['tracePrimary']['mrays_s_incl_pcie']} Mrays/s incl. PCIe "
+        f"({host_calls['tracePrimary']['mrays_s_reported']} reported), traceDiffuse "
+        f"{host_calls['traceDiffuse']['mrays_s_incl_pcie']} Mrays/s incl. PCIe "
+        f"({host_calls['traceDiffuse']['mrays_s_reported']} reported)")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(bvh, Pr, D, m, args.cpu_sample)
@@ -288,6 +324,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "occupancy": occupancy,
+        "host_pointer_calls": host_calls,
         "build_s": round(build_s, 2),
         "tuning": tuning,
     }
