@@ -151,5 +151,6 @@ struct CudaBVHHandle {
     rt::TraceScratch scratch;           // traversal scratch of calls on this handle
     rt::TraceConfig tune[rt::kNumClasses];  // per ray class launch configuration
     void* d_work = nullptr;   size_t workBytes = 0;   // staging for host-pointer calls
+    float* h_pinned = nullptr; size_t pinnedFloats = 0; // pinned host staging of host-pointer calls
     void* d_sortWs = nullptr; size_t sortWsBytes = 0; // ray-sort workspace
 };
