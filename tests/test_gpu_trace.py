@@ -272,17 +272,29 @@ def test_reference_trace_primary(pkg, oracle, parity_scene, parity_bvh):
 
 
 def test_reference_trace_diffuse(pkg, oracle, parity_bvh):
+    """cudaBVH_traceDiffuse: host SoA rays in, host distances out, t in
+    (0, 1e30) (layer/cuda_bvh_backend.cu:594). The calls run back to back on
+    one handle with growing and shrinking ray counts, so the host staging is
+    regrown and then reused with a stale tail; a tracePrimary in between
+    shares the staging."""
     rng = np.random.default_rng(12)
-    n = 50_000
-    o = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
-    d = rng.normal(size=(n, 3))
-    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
-    out = np.zeros(n, np.float32)
-    assert parity_bvh.trace_diffuse(o, d, out) > 0
-    rays = np.zeros((8, n), np.float32)
-    rays[0:3], rays[3:6], rays[6], rays[7] = o.T, d.T, 0.0, 1e30
     nodes, lt, pid = _oracle_inputs(parity_bvh)
-    assert _same_bits(out, oracle.trace_closest(nodes, lt, pid, rays)["t"])
+    for n in (3_000, 50_000, 1_000, 120_000, 7):
+        o = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+        d = rng.normal(size=(n, 3))
+        d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+        out = np.full(n, np.nan, np.float32)
+        assert parity_bvh.trace_diffuse(o, d, out) > 0
+        rays = np.zeros((8, n), np.float32)
+        rays[0:3], rays[3:6], rays[6], rays[7] = o.T, d.T, 0.0, 1e30
+        assert _same_bits(out, oracle.trace_closest(nodes, lt, pid, rays)["t"]), n
+        if n == 50_000:
+            side = 600   # 360,000 distances: more than the staging holds after this call
+            prim = np.zeros(side * side, np.float32)
+            assert parity_bvh.trace_primary(side, (0.0, 0.0, -3.0), prim) > 0
+            cam14 = np.array([0, 0, -3, 0, 0, 1, 1, 0, 0, 0, 1, 0, 1, 1], np.float32)
+            ref = oracle.trace_closest(nodes, lt, pid, oracle.gen_pinhole(cam14, side, side))
+            assert _same_bits(prim, ref["t"])
 
 
 def test_shading_formats(pkg, oracle, parity_bvh, dev):

@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_conv3(ConvArgs a) {
     static_assert(WGM * WGN == kBlock / 64, "four waves");
     constexpr int BM = WGM * 16 * WM, BN = WGN * 16 * WN, HALO = BM + 2;
     __shared__ __attribute__((aligned(16))) uint16_t As[3 * HALO * kPxStride];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[9 * BN * kPxStride];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[9 * BN * kPxStride + 8];   // + a 16-byte sink
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tilesX = (a.W + BM - 1) / BM;
     const int tile = blockIdx.x;
@@ -204,8 +204,38 @@ __global__ __launch_bounds__(kBlock) void k_conv3(ConvArgs a) {
             }                                                                                                \
         }                                                                                                    \
     }
+    // The chunk's weights are loaded in one batch of independent loads issued
+    // before the barrier, then written to LDS: one memory round trip per chunk
+    // instead of one per 256 16-byte pieces.
+    constexpr int NB = (9 * BN * 4 + kBlock - 1) / kBlock;
     DN_FETCH(0)
     for (int c0 = 0; c0 < a.cin; c0 += 32) {
+        // Named registers rather than an array: the compiler kept an array
+        // of this size in scratch memory.
+        uint4 w0, w1, w2, w3, w4, w5, w6, w7, w8;
+        static_assert(NB <= 9, "weight pieces per thread");
+        // unconditional loads (a piece past the end re-reads the last one and
+        // goes to the LDS sink below)
+        auto wload = [&](int k) {
+            const int e = min(tid + k * kBlock, 9 * BN * 4 - 1);
+            const int t = e / (BN * 4), rem = e - t * (BN * 4), nr = rem >> 2, q = rem & 3;
+            return *(const uint4*)(a.w + (size_t)(n0 + nr) * K + t * a.cin + c0 + 8 * q);
+        };
+        auto wstore = [&](int k, const uint4& v) {
+            const int e = tid + k * kBlock;
+            const int t = e / (BN * 4), rem = e - t * (BN * 4), nr = rem >> 2, q = rem & 3;
+            const int dst = e < 9 * BN * 4 ? (t * BN + nr) * kPxStride + 8 * q : 9 * BN * kPxStride;
+            *(uint4*)&Bs[dst] = v;
+        };
+        w0 = wload(0);
+        if constexpr (NB > 1) w1 = wload(1);
+        if constexpr (NB > 2) w2 = wload(2);
+        if constexpr (NB > 3) w3 = wload(3);
+        if constexpr (NB > 4) w4 = wload(4);
+        if constexpr (NB > 5) w5 = wload(5);
+        if constexpr (NB > 6) w6 = wload(6);
+        if constexpr (NB > 7) w7 = wload(7);
+        if constexpr (NB > 8) w8 = wload(8);
         __syncthreads();       // the previous chunk's operands are no longer read
 #pragma unroll
         for (int k = 0; k < NA; k++) {
@@ -215,11 +245,15 @@ __global__ __launch_bounds__(kBlock) void k_conv3(ConvArgs a) {
                 *(uint4*)&As[(r * HALO + h) * kPxStride + 8 * q] = ra[k];
             }
         }
-        for (int e = tid; e < 9 * BN * 4; e += kBlock) {
-            const int t = e / (BN * 4), rem = e - t * (BN * 4), nr = rem >> 2, q = rem & 3;
-            *(uint4*)&Bs[(t * BN + nr) * kPxStride + 8 * q] =
-                *(const uint4*)(a.w + (size_t)(n0 + nr) * K + t * a.cin + c0 + 8 * q);
-        }
+        wstore(0, w0);
+        if constexpr (NB > 1) wstore(1, w1);
+        if constexpr (NB > 2) wstore(2, w2);
+        if constexpr (NB > 3) wstore(3, w3);
+        if constexpr (NB > 4) wstore(4, w4);
+        if constexpr (NB > 5) wstore(5, w5);
+        if constexpr (NB > 6) wstore(6, w6);
+        if constexpr (NB > 7) wstore(7, w7);
+        if constexpr (NB > 8) wstore(8, w8);
         __syncthreads();
         if (c0 + 32 < a.cin) DN_FETCH(c0 + 32)
 #pragma unroll
