@@ -433,30 +433,6 @@ def bench_path_tracer(pkg, scenes, dev, args):
             "build_s": round(build_s, 2), "tuning": tuning}
 
 
-def batch_rays_torch(torch, n, seed, dev, cam):
-    """The many-scenes ray mix on the device: half pinhole primaries of the
-    small camera on a near-square grid, half from uniform points of [-1,1]^3
-    in uniform sphere directions."""
-    g = torch.Generator(device=dev)
-    g.manual_seed(seed)
-    npri = n // 2
-    w = max(1, int(np.ceil(np.sqrt(npri))))
-    h = max(1, (npri + w - 1) // w)
-    i = torch.arange(npri, device=dev)
-    sx = ((i % w).float() + 0.5) / w * 2 - 1
-    sy = ((i // w).float() + 0.5) / h * 2 - 1
-    f, r, u = (torch.tensor(cam[k], device=dev) for k in ("f", "r", "u"))
-    th = float(np.tan(np.radians(30.0)))
-    d1 = f[:, None] + (sx * th * w / h)[None] * r[:, None] + (sy * th)[None] * u[:, None]
-    d1 = d1 / d1.norm(dim=0)
-    o1 = torch.tensor(cam["o"], device=dev)[:, None].expand(3, npri)
-    nr = n - npri
-    d2 = torch.randn(3, nr, device=dev, generator=g)
-    d2 = d2 / d2.norm(dim=0)
-    o2 = torch.rand(3, nr, device=dev, generator=g) * 2 - 1
-    return torch.cat([o1, o2], 1), torch.cat([d1, d2], 1)
-
-
 def bench_batched(pkg, scenes, dev, args):
     """Many independent scenes: random-triangle scenes with log-uniform
     triangle counts in [1K, 200K] and ray counts in [64K, 1M], all BVHs built
@@ -473,19 +449,8 @@ def bench_batched(pkg, scenes, dev, args):
     build_s = time.time() - t0
     ntris = int(b.tri_offsets[-1])
     del tris
-    roff = np.concatenate([[0], np.cumsum([nr for _, nr in specs])]).astype(np.int32)
+    R, roff = scenes.batch_workload_device(specs, dev)
     total = int(roff[-1])
-    cam = scenes.parity_camera(512, 512)
-    O = torch.empty(3, total, device=dev)
-    Dd = torch.empty(3, total, device=dev)
-    for k, (_, nr) in enumerate(specs):
-        o, d = batch_rays_torch(torch, nr, 7000 + k, dev, cam)
-        O[:, roff[k]:roff[k + 1]] = o
-        Dd[:, roff[k]:roff[k + 1]] = d
-    R = {"ox": O[0], "oy": O[1], "oz": O[2], "dx": Dd[0], "dy": Dd[1], "dz": Dd[2],
-         "tmin": torch.zeros(total, device=dev), "tmax": torch.full((total,), 1e30, device=dev)}
-    R = {k: v.contiguous() for k, v in R.items()}
-    del O, Dd
     d_off = torch.from_numpy(roff).to(dev)
     t = torch.empty(total, device=dev)
     pid = torch.empty(total, dtype=torch.int32, device=dev)

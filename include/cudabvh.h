@@ -339,6 +339,19 @@ int cudaBVH_sceneTraceAnyQueued(CudaScene_t scene, int capacity, const int* d_co
 int cudaBVH_getLaunchInfo(CudaBVH_t bvh, int* info);
 int cudaBVH_sceneGetLaunchInfo(CudaScene_t scene, int* info);
 
+/* Traversal status of the handle / scene since the last check: waits for the
+ * handle's (scene's) stream, returns the status word and clears it. Bit 0 set
+ * means a traversal stack overflowed, so some hits may be missing (the stacks
+ * are sized 3 x BVH4 depth + 4 entries per level, which bounds a depth-first
+ * walk, so it reads 0); the message is also left for cudaBVH_lastError.
+ * Returns -1 for a NULL handle or a HIP error. Traces that synchronise
+ * (CUDABVH_FLAG_SYNC, the host-pointer calls) check it themselves and fail
+ * with -1 (0 Mrays/s) on an overflow. No backend counterpart: the reference's
+ * 16-entry stacks drop entries silently (layer/cuda_bvh_backend.cu:179). A
+ * batch's status is that of cudaBVH_batchForest(batch). */
+int cudaBVH_getTraceStatus(CudaBVH_t bvh);
+int cudaBVH_sceneGetTraceStatus(CudaScene_t scene);
+
 /* World-space unit geometric normal of each closest hit of sceneTraceClosest*
  * (reads hits->instId and hits->primId; device arrays nx, ny, nz of numRays).
  * The normal is the object-space e1 x e2 of the hit triangle (the mesh's

@@ -48,8 +48,14 @@ def test_bounce_chain(pkg, oracle, scenes, dev, res, spheres, W, H):
         assert np.array_equal(_bits(dg), _bits(dref))
         assert np.array_equal(_bits(sg), _bits(sref))
         assert (D["ox"].cpu().numpy()[m:] == -7.0).all()              # nothing written past the count
-        # diffuse directions lie in the hemisphere of the facing normal
+        # diffuse directions are unit and lie in the hemisphere of the facing
+        # normal (float64 normal of the hit triangle, turned against the ray)
         assert ((dref[3:6] * dref[3:6]).sum(0) > 0.9999).all()
+        T = tris[hid[srcref]].astype(np.float64).reshape(-1, 3, 3)
+        nrm = np.cross(T[:, 1] - T[:, 0], T[:, 2] - T[:, 0])
+        d0 = rays[3:6, srcref].T.astype(np.float64)
+        nrm[(nrm * d0).sum(1) > 0] *= -1
+        assert ((dg[3:6].T.astype(np.float64) * nrm).sum(1) >= 0).all()
         # queued traces read the count from device memory
         dt = torch.empty(n, device=dev)
         did = torch.empty(n, dtype=torch.int32, device=dev)

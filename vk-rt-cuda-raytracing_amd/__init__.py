@@ -162,6 +162,8 @@ SIGNATURES = {
     "cudaBVH_sceneHitNormals": (_I, [_H, _I, C.POINTER(CudaSceneHit), _P, _P, _P, C.c_uint]),
     "cudaBVH_getLaunchInfo": (_I, [_H, _P]),
     "cudaBVH_sceneGetLaunchInfo": (_I, [_H, _P]),
+    "cudaBVH_getTraceStatus": (_I, [_H]),
+    "cudaBVH_sceneGetTraceStatus": (_I, [_H]),
     "cudaBVH_sceneSetTraceConfig": (_I, [_H, _I, _I, _I, _I]),
     "cudaBVH_sceneGetTraceConfig": (_I, [_H, _I, _P, _P, _P]),
     "cudaBVH_sceneAutotune": (_I, [_H, _I, _I, C.POINTER(CudaRayPacket), _I, _P, _P]),
@@ -217,13 +219,34 @@ def _check(rc: int, what: str) -> None:
 
 
 def _ptr(x) -> int | None:
-    """Address of a numpy array or torch tensor (None passes NULL)."""
+    """Address of a contiguous numpy array or torch tensor (None passes NULL)."""
     if x is None:
         return None
     if isinstance(x, np.ndarray):
-        assert x.flags["C_CONTIGUOUS"]
+        if not x.flags["C_CONTIGUOUS"]:
+            raise ValueError("array passed to the C ABI must be C-contiguous")
         return x.ctypes.data
-    return x.data_ptr()  # torch tensor
+    if not x.is_contiguous():  # torch tensor
+        raise ValueError("tensor passed to the C ABI must be contiguous")
+    return x.data_ptr()
+
+
+def _out_ptr(x, n: int, dtype=np.float32) -> int | None:
+    """Address of an output array the library fills with n elements of dtype:
+    checks the element type and the size so the call cannot overrun it."""
+    if x is None:
+        return None
+    dt = np.dtype(dtype)
+    if isinstance(x, np.ndarray):
+        if x.dtype != dt or x.size < n:
+            raise ValueError(f"output must be {dt} with at least {n} elements (got {x.dtype}, {x.size})")
+    else:
+        import torch
+        want = {np.dtype(np.float32): torch.float32, np.dtype(np.int32): torch.int32,
+                np.dtype(np.uint8): torch.uint8, np.dtype(np.uint32): torch.uint32}[dt]
+        if x.dtype != want or x.numel() < n:
+            raise ValueError(f"output must be {want} with at least {n} elements (got {x.dtype}, {x.numel()})")
+    return _ptr(x)
 
 
 def ray_packet(rays) -> CudaRayPacket:
@@ -377,13 +400,23 @@ class BVH:
 
     # --- reference traces (layer/cuda_bvh_backend.h:29-73) ---
     def trace_primary(self, side: int, cam_o, out_hit_t: np.ndarray | None = None) -> float:
-        return self.lib.cudaBVH_tracePrimary(self.handle, side, *[float(c) for c in cam_o], _ptr(out_hit_t))
+        return self.lib.cudaBVH_tracePrimary(self.handle, side, *[float(c) for c in cam_o],
+                                             _out_ptr(out_hit_t, side * side))
 
     def trace_diffuse(self, o: np.ndarray, d: np.ndarray, out_hit_t: np.ndarray | None = None) -> float:
         o = [np.ascontiguousarray(o[:, k], np.float32) for k in range(3)]
         d = [np.ascontiguousarray(d[:, k], np.float32) for k in range(3)]
-        return self.lib.cudaBVH_traceDiffuse(self.handle, o[0].shape[0], *[x.ctypes.data for x in o + d],
-                                             _ptr(out_hit_t))
+        n = o[0].shape[0]
+        return self.lib.cudaBVH_traceDiffuse(self.handle, n, *[x.ctypes.data for x in o + d],
+                                             _out_ptr(out_hit_t, n))
+
+    def trace_status(self) -> int:
+        """Traversal status word since the last check (waits for the handle's
+        stream; bit 0: a stack overflowed); clears it."""
+        w = self.lib.cudaBVH_getTraceStatus(self.handle)
+        if w < 0:
+            raise CudaBVHError(f"getTraceStatus: {last_error()}")
+        return w
 
     def trace_to_rgba(self, width: int, height: int, cam_o) -> np.ndarray:
         out = np.zeros(width * height, np.uint32)
@@ -583,6 +616,14 @@ class Scene:
         hp = CudaSceneHit(_ptr(t), _ptr(inst_id), _ptr(prim_id), _ptr(u), _ptr(v))
         _check(self.lib.cudaBVH_sceneTraceClosestQueued(self.handle, capacity, _ptr(d_count), C.byref(rp),
                                                         C.byref(hp), flags), "sceneTraceClosestQueued")
+
+    def trace_status(self) -> int:
+        """Traversal status word of every trace on this scene since the last
+        check (bit 0: a stack overflowed); clears it."""
+        w = self.lib.cudaBVH_sceneGetTraceStatus(self.handle)
+        if w < 0:
+            raise CudaBVHError(f"sceneGetTraceStatus: {last_error()}")
+        return w
 
     def launch_info(self) -> dict:
         """Resources of the last traversal launch on this scene."""

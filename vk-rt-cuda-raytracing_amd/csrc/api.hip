@@ -25,6 +25,12 @@ const char* last_error();
 using rt::set_error;
 
 static inline hipStream_t hstream(const CudaBVHHandle* h) { return (hipStream_t)h->stream; }
+
+// Waits for stream s and checks the traversal status word of sc: 0 when every
+// trace on it kept its whole stack, -1 (error recorded) otherwise.
+static int finish_sync(rt::TraceScratch& sc, hipStream_t s, const char* fn) {
+    return rtk::take_status(sc, s, fn) == 0 ? 0 : -1;
+}
 static int ensure_cwbvh(CudaBVH_t bvh);
 
 static void free_handle_device(CudaBVHHandle* h) {
@@ -200,8 +206,6 @@ static void* sort_workspace(CudaBVHHandle* h, int n) {
     if (bytes > h->sortWsBytes) {
         hipFree(h->d_sortWs);
         h->d_sortWs = nullptr; h->sortWsBytes = 0;
-    hipFree(h->d_cwNodes); hipFree(h->d_cwTris);
-    h->d_cwNodes = h->d_cwTris = nullptr;
         if (hipMalloc(&h->d_sortWs, bytes) != hipSuccess) { set_error("cannot allocate %zu sort bytes", bytes); return nullptr; }
         h->sortWsBytes = bytes;
     }
@@ -325,7 +329,7 @@ float cudaBVH_tracePrimary(CudaBVH_t bvh, int side, float camOx, float camOy, fl
     float ms = 0.f;
     hipEventElapsedTime(&ms, e0, e1);
     hipEventDestroy(e0); hipEventDestroy(e1);
-    if (rc != 0) return 0.f;
+    if (rc != 0 || finish_sync(bvh->scratch, hstream(bvh), "cudaBVH_tracePrimary") != 0) return 0.f;
     if (outHitT) {
         float* pin = pinned_buffer(bvh, size_t(n));
         if (!pin) return 0.f;
@@ -377,7 +381,7 @@ float cudaBVH_traceDiffuse(CudaBVH_t bvh, int numRays, const float* rayOx, const
     float ms = 0.f;
     hipEventElapsedTime(&ms, e0, e1);
     hipEventDestroy(e0); hipEventDestroy(e1);
-    if (rc != 0) return 0.f;
+    if (rc != 0 || finish_sync(bvh->scratch, hstream(bvh), "cudaBVH_traceDiffuse") != 0) return 0.f;
     if (outHitT) {
         HIP_TRY(hipMemcpyAsync(pin, w + 8 * n, n * sizeof(float), hipMemcpyDeviceToHost, hstream(bvh)), return 0.f);
         HIP_TRY(hipStreamSynchronize(hstream(bvh)), return 0.f);
@@ -407,8 +411,7 @@ int cudaBVH_traceToRGBA(CudaBVH_t bvh, int width, int height, float camOx, float
     if (!d) return -1;
     if (shade_to_device(bvh, bvh->scratch, hstream(bvh), width, height, camOx, camOy, camOz, d, 0) != 0) return -1;
     HIP_TRY(hipMemcpyAsync(outRGBA_host, d, bytes, hipMemcpyDeviceToHost, hstream(bvh)), return -1);
-    HIP_TRY(hipStreamSynchronize(hstream(bvh)), return -1);
-    return 0;
+    return finish_sync(bvh->scratch, hstream(bvh), "cudaBVH_traceToRGBA");
 }
 
 // ---------------------------------------------------------------------------
@@ -743,7 +746,7 @@ static int trace_packet(CudaBVH_t bvh, int n, const CudaRayPacket* rays, const C
         } else {
             rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, isect == 1, false, cn != nullptr, s);
         }
-        if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(s), return -1);
+        if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) return finish_sync(bvh->scratch, s, fn);
         return rc;
     }
     // host pointers: 8 ray arrays + 4 hit arrays + 2 counters (+ occlusion bytes)
@@ -779,8 +782,7 @@ static int trace_packet(CudaBVH_t bvh, int n, const CudaRayPacket* rays, const C
     }
     if (cn) HIP_TRY(hipMemcpyAsync(cn, a.cntNodes, N * 4, hipMemcpyDeviceToHost, s), return -1);
     if (ct) HIP_TRY(hipMemcpyAsync(ct, a.cntTris, N * 4, hipMemcpyDeviceToHost, s), return -1);
-    HIP_TRY(hipStreamSynchronize(s), return -1);
-    return 0;
+    return finish_sync(bvh->scratch, s, fn);
 }
 
 int cudaBVH_traceClosest(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays, const CudaHitPacket* hits, unsigned flags) {
@@ -829,7 +831,7 @@ int cudaBVH_tracePinholeClosest(CudaBVH_t bvh, int width, int height, const Cuda
     a.cam = to_cam(*cam);
     a.t = hits->t; a.id = hits->primId; a.u = hits->u; a.v = hits->v;
     int rc = rtk::launch_trace(bvh, bvh->scratch, a, rtk::kModeClosest, isect == 1, true, false, hstream(bvh));
-    if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(hstream(bvh)), return -1);
+    if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) return finish_sync(bvh->scratch, hstream(bvh), "cudaBVH_tracePinholeClosest");
     return rc;
 }
 
@@ -877,7 +879,7 @@ static int trace_queued(CudaBVH_t bvh, int capacity, const int* d_count, const C
     if (hits) { a.t = hits->t; a.id = hits->primId; a.u = hits->u; a.v = hits->v; }
     a.occ = occ;
     int rc = rtk::launch_trace(bvh, bvh->scratch, a, mode, isect == 1, false, false, hstream(bvh));
-    if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(hstream(bvh)), return -1);
+    if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) return finish_sync(bvh->scratch, hstream(bvh), fn);
     return rc;
 }
 
@@ -1144,7 +1146,7 @@ static int scene_trace(CudaScene_t sc, int n, const int* d_count, const CudaRayP
     a.occ = occ; a.cntNodes = cn; a.cntTris = ct;
     hipStream_t s = (hipStream_t)sc->stream;
     int rc = rtk::launch_trace2l(sc, sc->scratch, a, cls >= 0 ? cls : (any ? 1 : 0), s);
-    if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(s), return -1);
+    if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) return finish_sync(sc->scratch, s, fn);
     return rc;
 }
 
@@ -1181,6 +1183,25 @@ int cudaBVH_sceneGetLaunchInfo(CudaScene_t sc, int* info) {
     return 0;
 }
 
+int cudaBVH_getTraceStatus(CudaBVH_t bvh) {
+    rt::clear_error();
+    if (!check_handle(bvh, "cudaBVH_getTraceStatus")) return -1;
+    return rtk::take_status(bvh->scratch, hstream(bvh), "cudaBVH_getTraceStatus");
+}
+
+int cudaBVH_sceneGetTraceStatus(CudaScene_t sc) {
+    rt::clear_error();
+    if (!scene_ok(sc, "cudaBVH_sceneGetTraceStatus")) return -1;
+    const int a = rtk::take_status(sc->scratch, (hipStream_t)sc->stream, "cudaBVH_sceneGetTraceStatus");
+    if (a < 0) return -1;
+    const int b = rtk::pt_take_status(sc);
+    if (b < 0) return -1;
+    const int c = rtk::take_status(sc->tlas->scratch, (hipStream_t)sc->stream, "cudaBVH_sceneGetTraceStatus");
+    if (c < 0) return -1;
+    if (a | b | c) set_error("cudaBVH_sceneGetTraceStatus: traversal stack overflow (status 0x%x)", (unsigned)(a | b | c));
+    return a | b | c;
+}
+
 int cudaBVH_sceneHitNormals(CudaScene_t sc, int numRays, const CudaSceneHit* hits, float* nx, float* ny, float* nz,
                             unsigned flags) {
     rt::clear_error();
@@ -1212,10 +1233,12 @@ struct CudaBVHBatchHandle {
     std::vector<int> triOff, roots4;
     int* d_roots = nullptr;
     hipStream_t own = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    rtk::TraceArgs key{};
-    int keyMode = -1;
+    // one captured launch per mode (closest hit, any hit), each valid for the
+    // arguments and traversal-scratch generation it was captured with
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    rtk::TraceArgs key[2]{};
+    uint64_t keyGen[2] = {0, 0};
 };
 
 static bool batch_ok(CudaBVHBatch_t b, const char* fn) {
@@ -1250,8 +1273,10 @@ void cudaBVH_batchDestroy(CudaBVHBatch_t b) {
     if (!b) return;
     if (b->forest && b->forest->stream) hipStreamSynchronize((hipStream_t)b->forest->stream);
     if (b->own) hipStreamSynchronize(b->own);
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    for (int m = 0; m < 2; m++) {
+        if (b->exec[m]) hipGraphExecDestroy(b->exec[m]);
+        if (b->graph[m]) hipGraphDestroy(b->graph[m]);
+    }
     if (b->own) hipStreamDestroy(b->own);
     hipFree(b->d_roots);
     cudaBVH_destroy(b->forest);
@@ -1283,6 +1308,7 @@ static int batch_trace(CudaBVHBatch_t b, int totalRays, const int* d_rayOffsets,
     }
     if (flags & ~(unsigned)(CUDABVH_FLAG_SYNC | CUDABVH_FLAG_GRAPH)) { set_error("%s: unsupported flags 0x%x", fn, flags); return -1; }
     rtk::TraceArgs a;
+    memset(&a, 0, sizeof a);      // the graph key compares the whole struct, padding included
     a.n = totalRays;
     a.ox = rays->ox; a.oy = rays->oy; a.oz = rays->oz; a.dx = rays->dx; a.dy = rays->dy; a.dz = rays->dz;
     a.tmin = rays->tmin; a.tmax = rays->tmax;
@@ -1293,7 +1319,7 @@ static int batch_trace(CudaBVHBatch_t b, int totalRays, const int* d_rayOffsets,
     hipStream_t s = hstream(b->forest);
     if (!(flags & CUDABVH_FLAG_GRAPH)) {
         const int rc = rtk::launch_trace(b->forest, b->forest->scratch, a, mode, false, false, false, s);
-        if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) HIP_TRY(hipStreamSynchronize(s), return -1);
+        if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) return finish_sync(b->forest->scratch, s, fn);
         return rc;
     }
     // graph mode: one captured launch (counter reset + persistent trace), replayed
@@ -1305,14 +1331,16 @@ static int batch_trace(CudaBVHBatch_t b, int totalRays, const int* d_rayOffsets,
         HIP_TRY(hipStreamSynchronize(nullptr), return -1);
         s = b->own;
     }
-    if (!b->exec || b->keyMode != mode || memcmp(&b->key, &a, sizeof(a)) != 0) {
-        if (b->exec) { hipGraphExecDestroy(b->exec); b->exec = nullptr; }
-        if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+    const int m = any ? 1 : 0;
+    rt::TraceScratch& scr = b->forest->scratch;
+    if (!b->exec[m] || b->keyGen[m] != scr.gen || memcmp(&b->key[m], &a, sizeof(a)) != 0) {
+        if (b->exec[m]) { hipGraphExecDestroy(b->exec[m]); b->exec[m] = nullptr; }
+        if (b->graph[m]) { hipGraphDestroy(b->graph[m]); b->graph[m] = nullptr; }
         rtk::TraceArgs e = a;      // eager launch allocates the traversal scratch
-        if (rtk::launch_trace(b->forest, b->forest->scratch, e, mode, false, false, false, s) != 0) return -1;
+        if (rtk::launch_trace(b->forest, scr, e, mode, false, false, false, s) != 0) return -1;
         HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), return -1);
         rtk::TraceArgs c = a;
-        const int rc = rtk::launch_trace(b->forest, b->forest->scratch, c, mode, false, false, false, s);
+        const int rc = rtk::launch_trace(b->forest, scr, c, mode, false, false, false, s);
         hipGraph_t gr = nullptr;
         const hipError_t ec = hipStreamEndCapture(s, &gr);
         if (rc != 0 || ec != hipSuccess) {
@@ -1320,13 +1348,13 @@ static int batch_trace(CudaBVHBatch_t b, int totalRays, const int* d_rayOffsets,
             if (rc == 0) set_error("%s: capture failed: %s", fn, hipGetErrorString(ec));
             return -1;
         }
-        b->graph = gr;
-        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), return -1);
-        b->key = a;
-        b->keyMode = mode;
+        b->graph[m] = gr;
+        HIP_TRY(hipGraphInstantiate(&b->exec[m], b->graph[m], nullptr, nullptr, 0), return -1);
+        b->key[m] = a;
+        b->keyGen[m] = scr.gen;
     }
-    HIP_TRY(hipGraphLaunch(b->exec, s), return -1);
-    if ((flags & CUDABVH_FLAG_SYNC) || s == b->own) HIP_TRY(hipStreamSynchronize(s), return -1);
+    HIP_TRY(hipGraphLaunch(b->exec[m], s), return -1);
+    if ((flags & CUDABVH_FLAG_SYNC) || s == b->own) return finish_sync(scr, s, fn);
     return 0;
 }
 

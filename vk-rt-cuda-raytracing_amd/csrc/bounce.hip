@@ -143,6 +143,7 @@ int launch_bounce(const CudaBVHHandle* h, rt::TraceScratch& sc, const rtd::Cam& 
     if ((size_t)tiles > sc.tileCap) {
         hipFree(sc.tiles);
         sc.tiles = nullptr; sc.tileCap = 0;
+        sc.gen++;
         if (hipMalloc((void**)&sc.tiles, (size_t)tiles * sizeof(int)) != hipSuccess) {
             rt::set_error("cannot allocate %d tile counters", tiles);
             return -1;

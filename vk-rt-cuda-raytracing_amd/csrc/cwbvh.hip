@@ -223,12 +223,7 @@ static int launch_cw(rt::TraceScratch& sc, TraceArgs& a, int depth8, hipStream_t
     a.maxStack = 7 * depth8 + 8;
     const size_t need = (size_t)std::max(0, a.maxStack - kCwLds) * threads * sizeof(int2);
     if (alloc_scratch(sc) != 0) return -1;
-    if (need > sc.spillBytes) {
-        hipFree(sc.spill);
-        sc.spill = nullptr; sc.spillBytes = 0;
-        if (hipMalloc(&sc.spill, need) != hipSuccess) { rt::set_error("cannot allocate %zu spill bytes", need); return -1; }
-        sc.spillBytes = need;
-    }
+    if (grow_spill(sc, need) != 0) return -1;
     a.spill = (int2*)sc.spill;
     a.spillStride = threads;
     a.counter = sc.counter;

@@ -74,6 +74,7 @@ struct PtArgs {
     const float4* tlasTris;
     int2* tailStack;
     int tailStride, tailDepth;
+    int* status;          // traversal status word (bit 0: stack overflow)
 };
 
 struct PtResources {
@@ -90,6 +91,7 @@ struct PtResources {
     hipGraph_t graph = nullptr;
     PtArgs key{};                     // arguments the graph was captured with
     rt::TraceConfig keyTune[3];       // and the scene's traversal configuration
+    uint64_t keyGen[2] = {0, 0};      // and the generations of the two traversal scratches
 };
 
 __device__ __forceinline__ void clamp_contrib(float c[3], float m) {
@@ -375,7 +377,7 @@ __global__ __launch_bounds__(64) void k_pt_tail(PtArgs a, int b0) {
             float t;
             int inst, prim;
             const bool hit = trace2l_one<false>(a.tlasNodes, a.tlasTris, a.inst, a.blas, r, stk, a.tailStride,
-                                                a.tailDepth, t, inst, prim);
+                                                a.tailDepth, a.status, t, inst, prim);
             ShadeOut so;
             shade_event(a, b, b == 0 && sample0, pix, o, d, thr, rng, hit ? t : RT_MISS_T, hit ? inst : -1,
                         hit ? prim : -1, so);
@@ -388,8 +390,8 @@ __global__ __launch_bounds__(64) void k_pt_tail(PtArgs a, int b0) {
                 rtd::ray_setup(sr);
                 float st;
                 int si, sprim;
-                if (!trace2l_one<true>(a.tlasNodes, a.tlasTris, a.inst, a.blas, sr, stk, a.tailStride, a.tailDepth, st,
-                                       si, sprim)) {
+                if (!trace2l_one<true>(a.tlasNodes, a.tlasTris, a.inst, a.blas, sr, stk, a.tailStride, a.tailDepth,
+                                       a.status, st, si, sprim)) {
                     a.rgb[0][pix] += so.c[0]; a.rgb[1][pix] += so.c[1]; a.rgb[2][pix] += so.c[2];
                 }
             }
@@ -544,6 +546,13 @@ void pt_free(CudaSceneHandle* sc) {
     sc->pt = nullptr;
 }
 
+int pt_take_status(CudaSceneHandle* sc) {
+    if (!sc->pt) return 0;
+    PtResources& R = *(PtResources*)sc->pt;
+    if (!R.side) return 0;
+    return take_status(R.scratch2, R.side, "scenePathTrace");
+}
+
 static bool same_key(const PtArgs& x, const PtArgs& y) { return memcmp(&x, &y, sizeof(PtArgs)) == 0; }
 
 int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams& p, const CudaPathImage& img,
@@ -562,6 +571,7 @@ int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams&
     if (pt_alloc(R, N) != 0) return -1;
 
     PtArgs a{};
+    memset(&a, 0, sizeof a);      // the graph key compares the whole struct, padding included
     pt_layout(R, a);
     a.inst = sc->d_inst; a.blas = sc->d_blas;
     for (int k = 0; k < 3; k++) {
@@ -613,7 +623,8 @@ int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams&
             return -1;
         }
     }
-    if (alloc_scratch(R.scratch2) != 0) return -1;
+    if (alloc_scratch(R.scratch2) != 0 || alloc_scratch(sc->scratch) != 0) return -1;
+    a.status = sc->scratch.status;
 
     const bool graph = (flags & CUDABVH_FLAG_GRAPH) != 0;
     hipStream_t s = (hipStream_t)sc->stream;
@@ -629,7 +640,10 @@ int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams&
     hipLaunchKernelGGL(k_pt_begin, dim3(1), dim3(64), 0, s, a, p.seed);
     int first = 0;
     if (graph) {
-        if (!R.exec || !same_key(R.key, a) || memcmp(R.keyTune, sc->tune2l, sizeof(R.keyTune)) != 0) {
+        // re-capture when the arguments, the launch configuration or any scratch
+        // buffer the captured kernels address has changed since the capture
+        if (!R.exec || !same_key(R.key, a) || memcmp(R.keyTune, sc->tune2l, sizeof(R.keyTune)) != 0 ||
+            R.keyGen[0] != sc->scratch.gen || R.keyGen[1] != R.scratch2.gen) {
             if (R.exec) { hipGraphExecDestroy(R.exec); R.exec = nullptr; }
             if (R.graph) { hipGraphDestroy(R.graph); R.graph = nullptr; }
             // one eager sample allocates every scratch buffer the traces need
@@ -648,6 +662,8 @@ int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams&
             HIP_TRY(hipGraphInstantiate(&R.exec, R.graph, nullptr, nullptr, 0), return -1);
             R.key = a;
             memcpy(R.keyTune, sc->tune2l, sizeof(R.keyTune));
+            R.keyGen[0] = sc->scratch.gen;
+            R.keyGen[1] = R.scratch2.gen;
         }
         for (int k = first; k < p.spp; k++) HIP_TRY(hipGraphLaunch(R.exec, s), return -1);
     } else {
@@ -664,7 +680,11 @@ int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams&
             rayCounts[p.maxDepth + k] = (long long)tot[kPtMaxDepth + k];
         }
     }
-    if ((flags & CUDABVH_FLAG_SYNC) || s == R.own) HIP_TRY(hipStreamSynchronize(s), return -1);
+    if ((flags & CUDABVH_FLAG_SYNC) || s == R.own) {
+        HIP_TRY(hipStreamSynchronize(R.side), return -1);
+        if (take_status(sc->scratch, s, "scenePathTrace") != 0 || take_status(R.scratch2, s, "scenePathTrace") != 0)
+            return -1;
+    }
     return 0;
 }
 

@@ -85,4 +85,7 @@ int trace2l_max_stack(const CudaSceneHandle* sc);
 int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams& p, const CudaPathImage& img,
                long long* rayCounts, unsigned flags);
 void pt_free(CudaSceneHandle* sc);
+// Status word of the path tracer's side-stream traversal scratch (read and
+// cleared after waiting for that stream); 0 when the scene has not traced paths.
+int pt_take_status(CudaSceneHandle* sc);
 }

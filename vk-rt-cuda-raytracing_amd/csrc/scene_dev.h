@@ -108,8 +108,10 @@ __device__ __forceinline__ void visit2w(const uint4* __restrict__ nodes, const R
 struct GlobalStack {
     int2* base;
     int stride, max, sp;
+    int* status;                        // bit 0 set on overflow (nullable)
     __device__ __forceinline__ void push(int rf, float t) {
         if (sp < max) { base[(size_t)sp * stride] = make_int2(rf, __float_as_int(t)); sp++; }
+        else if (status) atomicOr(status, 1);
     }
     __device__ __forceinline__ bool pop(int& rf, float& t) {
         if (sp == 0) return false;
@@ -231,8 +233,8 @@ __device__ __forceinline__ bool trace2l_ray(const uint4* tlasNodes, const float4
 template <bool ANY>
 __device__ __forceinline__ bool trace2l_one(const uint4* tlasNodes, const float4* tlasTris, const InstanceDev* insts,
                                             const BlasDev* blas, const Ray& rw, int2* stk, int stride, int maxStack,
-                                            float& bt, int& binst, int& bid) {
-    GlobalStack st{stk, stride, maxStack, 0};
+                                            int* status, float& bt, int& binst, int& bid) {
+    GlobalStack st{stk, stride, maxStack, 0, status};
     float bu, bv;
     return trace2l_ray<ANY, false>(tlasNodes, tlasTris, insts, blas, rw, st, bt, binst, bid, bu, bv);
 }

@@ -866,15 +866,7 @@ static int launch_kernel(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs
     a.maxStack = 3 * h->depth4 + 4;
     const int spillDepth = a.maxStack > ldsd ? a.maxStack - ldsd : 0;
     const size_t need = (size_t)spillDepth * threads * sizeof(int2);
-    if (need > sc.spillBytes) {
-        if (sc.spill) hipFree(sc.spill);
-        sc.spill = nullptr; sc.spillBytes = 0;
-        if (hipMalloc(&sc.spill, need) != hipSuccess) {
-            rt::set_error("cannot allocate %zu bytes of traversal stack spill", need);
-            return -1;
-        }
-        sc.spillBytes = need;
-    }
+    if (grow_spill(sc, need) != 0) return -1;
     if (a.W > 0) {
         a.tilesX = (a.W + 7) / 8;
         a.nTiles = a.tilesX * ((a.H + 7) / 8);
@@ -923,6 +915,40 @@ void free_scratch(rt::TraceScratch& sc) {
     hipFree(sc.spill); hipFree(sc.counter); hipFree(sc.status); hipFree(sc.tiles);
     sc.spill = nullptr; sc.spillBytes = 0; sc.counter = nullptr; sc.status = nullptr;
     sc.tiles = nullptr; sc.tileCap = 0;
+    sc.gen++;
+}
+
+int take_status(rt::TraceScratch& sc, hipStream_t s, const char* fn) {
+    if (!sc.status) return 0;
+    int w = 0;
+    if (hipMemcpyAsync(&w, sc.status, sizeof w, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        rt::set_error("%s: cannot read the traversal status word", fn);
+        return -1;
+    }
+    if (w) {
+        const int zero = 0;
+        if (hipMemcpyAsync(sc.status, &zero, sizeof zero, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            rt::set_error("%s: cannot clear the traversal status word", fn);
+            return -1;
+        }
+        rt::set_error("%s: traversal stack overflow (status 0x%x), hits may be missing", fn, (unsigned)w);
+    }
+    return w;
+}
+
+int grow_spill(rt::TraceScratch& sc, size_t need) {
+    if (need <= sc.spillBytes) return 0;
+    hipFree(sc.spill);
+    sc.spill = nullptr; sc.spillBytes = 0;
+    sc.gen++;
+    if (hipMalloc(&sc.spill, need) != hipSuccess) {
+        rt::set_error("cannot allocate %zu bytes of traversal stack spill", need);
+        return -1;
+    }
+    sc.spillBytes = need;
+    return 0;
 }
 
 // Tunable variants (LDS stack depth, resident blocks per CU) exist for the

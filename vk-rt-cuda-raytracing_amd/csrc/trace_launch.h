@@ -64,6 +64,13 @@ int launch_trace(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int
                  bool counted, hipStream_t s);
 int alloc_scratch(rt::TraceScratch& sc);
 void free_scratch(rt::TraceScratch& sc);
+// Grows the spill region of sc to at least need bytes (bumps sc.gen when it
+// reallocates); 0 on success.
+int grow_spill(rt::TraceScratch& sc, size_t need);
+// Synchronizes s, then reads and clears the status word of sc (bit 0: a
+// traversal stack overflowed, so hits may be missing). A set word is recorded
+// as the error of fn. Returns the word, or -1 if it could not be read.
+int take_status(rt::TraceScratch& sc, hipStream_t s, const char* fn);
 int launch_gen_pinhole(const rtd::Cam& cam, int W, int H, const CudaRayPacket& p, hipStream_t s);
 // n x 9 floats gathered from device vertex / index buffers (build.hip)
 hipError_t launch_gather_tris(const void* verts, int numVertices, int stride, const void* idx, int indexBytes,

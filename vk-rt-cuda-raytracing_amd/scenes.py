@@ -246,3 +246,49 @@ def batch_rays(n: int, seed: int) -> np.ndarray:
     out[3:6] = np.concatenate([d1, d2]).T
     out[7] = 1e30
     return out
+
+
+def batch_rays_device(n: int, seed: int, dev, cam=None):
+    """The many-scenes ray mix generated on a torch device (the generator of
+    the 256-scene workload): half pinhole primaries of the small camera on a
+    near-square grid, half from uniform points of [-1,1]^3 in uniform sphere
+    directions (torch's seeded device generator). Returns (origins (3, n),
+    directions (3, n)) float32 device tensors."""
+    import torch
+    cam = parity_camera(512, 512) if cam is None else cam
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    npri = n // 2
+    w = max(1, int(np.ceil(np.sqrt(npri))))
+    h = max(1, (npri + w - 1) // w)
+    i = torch.arange(npri, device=dev)
+    sx = ((i % w).float() + 0.5) / w * 2 - 1
+    sy = ((i // w).float() + 0.5) / h * 2 - 1
+    f, r, u = (torch.tensor(cam[k], device=dev) for k in ("f", "r", "u"))
+    d1 = f[:, None] + (sx * TAN30 * w / h)[None] * r[:, None] + (sy * TAN30)[None] * u[:, None]
+    d1 = d1 / d1.norm(dim=0)
+    o1 = torch.tensor(cam["o"], device=dev)[:, None].expand(3, npri)
+    nr = n - npri
+    d2 = torch.randn(3, nr, device=dev, generator=g)
+    d2 = d2 / d2.norm(dim=0)
+    o2 = torch.rand(3, nr, device=dev, generator=g) * 2 - 1
+    return torch.cat([o1, o2], 1), torch.cat([d1, d2], 1)
+
+
+def batch_workload_device(specs, dev, seed0: int = 7000):
+    """Rays of every scene of `specs` (list of (triangles, rays)) concatenated
+    on the device: (ray dict of contiguous (total,) tensors, host ray offsets
+    (numScenes + 1,) int32). Scene k's rays come from seed seed0 + k."""
+    import torch
+    roff = np.concatenate([[0], np.cumsum([nr for _, nr in specs])]).astype(np.int32)
+    total = int(roff[-1])
+    O = torch.empty(3, total, device=dev)
+    D = torch.empty(3, total, device=dev)
+    for k, (_, nr) in enumerate(specs):
+        o, d = batch_rays_device(nr, seed0 + k, dev)
+        O[:, roff[k]:roff[k + 1]] = o
+        D[:, roff[k]:roff[k + 1]] = d
+    R = {"ox": O[0], "oy": O[1], "oz": O[2], "dx": D[0], "dy": D[1], "dz": D[2],
+         "tmin": torch.zeros(total, device=dev), "tmax": torch.full((total,), 1e30, device=dev)}
+    return {k: v.contiguous() for k, v in R.items()}, roff
+
