@@ -275,7 +275,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(bvh, Pr, D, m, args.cpu_sample)
+        cpu = cpu_baseline(bvh, Pr, D, m, args.cpu_sample, gpu_hits=((pid, pt), (did, dt)))
 
     ic = tuning.get("incoherent_closest")
     prof_traffic = load_traffic(f"{ic['schedule']}/{ic['lds_depth']}/{ic['blocks_per_cu']}") if ic else None
@@ -610,12 +610,14 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(bvh, Pr, D, m, sample, budget_s=8.0):
+def cpu_baseline(bvh, Pr, D, m, sample, budget_s=8.0, gpu_hits=None):
     """The CPU oracle's traversal (C, OpenMP over all host cores) on an evenly
     strided sample of the same primary and diffuse rays and the same BVH:
     watertight triangle test (the parity arithmetic, `value`) and the
     reference's Moller-Trumbore test (`value_mt`), each one warm-up pass then
-    at least 5 timed passes (about budget_s seconds), median."""
+    at least 5 timed passes (about budget_s seconds), median. gpu_hits
+    ((primary ids, t), (diffuse ids, t)) device tensors: the watertight
+    answers are compared with them ray by ray (`parity`)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     nodes = bvh.node_data()
@@ -629,6 +631,18 @@ def cpu_baseline(bvh, Pr, D, m, sample, budget_s=8.0):
     threads = pyoracle.num_threads(0)
     rays = prim.shape[1] + diff.shape[1]
     res = {}
+    parity = None
+    if gpu_hits is not None:
+        (gi_p, gt_p), (gi_d, gt_d) = gpu_hits
+        rp = pyoracle.trace_closest(nodes, lt, pid, prim, counts=False)
+        rd = pyoracle.trace_closest(nodes, lt, pid, diff, counts=False)
+        ids = np.concatenate([gi_p.cpu().numpy()[ip], gi_d[:m].cpu().numpy()[idd]])
+        ts = np.concatenate([gt_p.cpu().numpy()[ip], gt_d[:m].cpu().numpy()[idd]]).view(np.uint32)
+        rid = np.concatenate([rp["id"], rd["id"]])
+        rt = np.concatenate([rp["t"], rd["t"]]).view(np.uint32)
+        parity = {"rays_compared": int(rid.shape[0]), "id_mismatches": int((ids != rid).sum()),
+                  "t_bit_mismatches": int((ts != rt).sum())}
+        log(f"parity vs oracle: {parity}")
     for name, isect in (("watertight", 0), ("mt", 1)):
         pyoracle.trace_closest(nodes, lt, pid, prim, isect=isect, counts=False)       # warm-up pass
         times = []
@@ -643,6 +657,7 @@ def cpu_baseline(bvh, Pr, D, m, sample, budget_s=8.0):
     tw, nw = res["watertight"]
     tm, nm = res["mt"]
     return {"value": round(rays / tw / 1e6, 2), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "parity": parity,
             "value_mt": round(rays / tm / 1e6, 2), "cpu_model": cpu_model(),
             "sample": f"{prim.shape[1]} primary + {diff.shape[1]} diffuse rays of the frame, closest hit, same BVH; "
                       f"value: watertight test (median of {nw} passes), value_mt: the reference's Moller-Trumbore "

@@ -32,6 +32,7 @@ static int finish_sync(rt::TraceScratch& sc, hipStream_t s, const char* fn) {
     return rtk::take_status(sc, s, fn) == 0 ? 0 : -1;
 }
 static int ensure_cwbvh(CudaBVH_t bvh);
+static void async_release(CudaBVH_t bvh);
 
 static void free_handle_device(CudaBVHHandle* h) {
     hipFree(h->d_nodes4); hipFree(h->d_tris); hipFree(h->d_leafOfPrim); hipFree(h->d_work); hipFree(h->d_sortWs);
@@ -301,6 +302,7 @@ CudaBVH_t cudaBVH_buildDevice(const void* d_vertices, int numVertices, int verte
 
 void cudaBVH_destroy(CudaBVH_t bvh) {
     if (!bvh) return;
+    async_release(bvh);          // the background tracer no longer renders from it
     if (bvh->stream) hipStreamSynchronize(hstream(bvh)); else hipDeviceSynchronize();
     free_handle_device(bvh);
     delete bvh;
@@ -425,7 +427,8 @@ struct AsyncTracer {
     std::thread th;
     bool running = false, hasRequest = false, hasResult = false;
     int device = 0;
-    CudaBVH_t bvh = nullptr;
+    CudaBVH_t bvh = nullptr;             // handle of the pending request
+    CudaBVH_t busy = nullptr;            // handle of the frame being rendered
     int w = 0, h = 0, fmt = 0;
     float cx = 0, cy = 0, cz = 0;
     void* gpuTarget = nullptr;
@@ -447,6 +450,7 @@ struct AsyncTracer {
             CudaBVH_t b = bvh; int W = w, H = h, F = fmt; float X = cx, Y = cy, Z = cz;
             void* tgt = gpuTarget; int wi = writeIdx; uint64_t sz = bufSize;
             hasRequest = false;
+            busy = b;
             lk.unlock();
             if (tgt) {
                 shade_to_device(b, scratch, stream, W, H, X, Y, Z, tgt, F);
@@ -460,12 +464,13 @@ struct AsyncTracer {
             }
             lk.lock();
             readIdx = wi; writeIdx = 1 - wi; hasResult = true;
+            busy = nullptr;
             ready.notify_all();
         }
+        // The worker only stops when the process exits (static destruction),
+        // when the HIP runtime may already be torn down: its stream, staging
+        // and traversal scratch are left to the process teardown.
         lk.unlock();
-        hipFree(dbuf); dbuf = nullptr; dbytes = 0;
-        rtk::free_scratch(scratch);
-        hipStreamDestroy(stream);
     }
     void start(uint64_t bytes) {
         std::lock_guard<std::mutex> sg(startMu);
@@ -488,6 +493,16 @@ struct AsyncTracer {
 AsyncTracer g_async;
 std::atomic<int> g_async_calls_host{0}, g_async_calls_gpu{0};
 }  // namespace
+
+// Waits until the background tracer neither renders from nor holds a pending
+// request for bvh (a pending one is dropped), so the handle can be freed.
+static void async_release(CudaBVH_t bvh) {
+    std::unique_lock<std::mutex> lk(g_async.mu);
+    if (!g_async.running) return;
+    if (g_async.hasRequest && g_async.bvh == bvh) g_async.hasRequest = false;
+    if (g_async.bvh == bvh) g_async.bvh = nullptr;
+    g_async.ready.wait(lk, [&] { return g_async.busy != bvh; });
+}
 
 int cudaBVH_traceToHostPtr(CudaBVH_t bvh, int width, int height, float camOx, float camOy, float camOz,
                            void* h_outRGBA, int outFmt, uint64_t bufSize) {
