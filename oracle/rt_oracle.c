@@ -149,6 +149,65 @@ static int bin_of(float c, float cmin, float cmax) {
 
 typedef struct { int s, e; int parent; int side; } btask;
 
+/* SAH collapse of the BVH2 into BVH4 nodes (dynamic programming over the
+ * BVH2, as in the wide-BVH collapse of Ylitie et al. 2017, here with four
+ * slots): cost[n][k-1] is the least total surface area of the BVH4 nodes
+ * needed to present BVH2 subtree n as k slots (k = 1: n's own slot, a leaf or
+ * a BVH4 node; k >= 2: n is dissolved into k slots of its parent). Leaves
+ * cost 0 as one slot; the choice is then the expected number of node visits
+ * of a ray. Ties take the fewer slots, then the smaller left count. */
+#define CINF 3.0e38f
+static void collapse_costs(const bnode2* nodes, int num, float* cost) {
+    for (int n = num - 1; n >= 0; n--) {           /* children follow parents in pre-order */
+        float* c = cost + 4 * (size_t)n;
+        const bnode2* nd = &nodes[n];
+        if (nd->triCount > 0) { c[0] = 0.0f; c[1] = c[2] = c[3] = CINF; continue; }
+        const float* l = cost + 4 * (size_t)nd->left;
+        const float* r = cost + 4 * (size_t)nd->right;
+        float g[5] = {CINF, CINF, CINF, CINF, CINF};
+        for (int k = 2; k <= 4; k++)
+            for (int a = 1; a < k; a++) {
+                const float x = l[a - 1] + r[k - a - 1];
+                if (x < g[k]) g[k] = x;
+            }
+        float best = g[2];
+        if (g[3] < best) best = g[3];
+        if (g[4] < best) best = g[4];
+        c[0] = sa_area(nd->box) + best;
+        c[1] = g[2]; c[2] = g[3]; c[3] = g[4];
+    }
+}
+/* Slot count m of subtree n inside its head -> (left count, right count). */
+static void collapse_split(const float* cost, const bnode2* nodes, int n, int m, int* al, int* ar) {
+    const float* l = cost + 4 * (size_t)nodes[n].left;
+    const float* r = cost + 4 * (size_t)nodes[n].right;
+    int ba = 1; float bx = CINF;
+    for (int a = 1; a < m; a++) {
+        const float x = l[a - 1] + r[m - a - 1];
+        if (x < bx) { bx = x; ba = a; }
+    }
+    *al = ba; *ar = m - ba;
+}
+/* Slots (BVH2 nodes, in pre-order) of the BVH4 node headed by internal n. */
+static int collapse_gather(const bnode2* nodes, const float* cost, int p, int gather[4]) {
+    const float* c = cost + 4 * (size_t)p;
+    int k = 2;
+    if (c[2] < c[k - 1]) k = 3;
+    if (c[3] < c[k - 1]) k = 4;
+    int stn[4], stm[4], sp = 0, ng = 0;
+    stn[sp] = p; stm[sp] = k; sp++;
+    while (sp > 0) {
+        sp--;
+        const int n = stn[sp], m = stm[sp];
+        if (m == 1) { gather[ng++] = n; continue; }
+        int al, ar;
+        collapse_split(cost, nodes, n, m, &al, &ar);
+        stn[sp] = nodes[n].right; stm[sp] = ar; sp++;
+        stn[sp] = nodes[n].left; stm[sp] = al; sp++;
+    }
+    return ng;
+}
+
 orc_bvh* orc_build(const float* tris, int n, int max_leaf) {
     if (!tris || n <= 0 || max_leaf < 1 || max_leaf > 8) return NULL;
     orc_bvh* B = (orc_bvh*)calloc(1, sizeof(orc_bvh));
@@ -229,8 +288,12 @@ orc_bvh* orc_build(const float* tris, int n, int max_leaf) {
     }
     free(st); free(pb); free(cen); free(idx); free(tmp);
 
-    /* ---- collapse to BVH4 (:391-418), pre-order numbering ---- */
+    /* ---- collapse to BVH4 (node format of :31-34 and :391-418; which BVH2
+     * nodes become BVH4 nodes is the SAH choice of collapse_costs, not the
+     * reference's open-both-children rule), pre-order numbering ---- */
     B->b4 = (uint8_t*)calloc((size_t)(B->num_nodes + 1), 64);
+    float* ccost = (float*)malloc(sizeof(float) * 4 * (size_t)B->num_nodes);
+    collapse_costs(B->nodes, B->num_nodes, ccost);
     typedef struct { int n2; int parent; int slot; int depth; } ctask;
     ctask* cs = (ctask*)malloc(sizeof(ctask) * (size_t)(B->num_nodes + 4));
     int csp = 0, maxd = 0;
@@ -247,12 +310,7 @@ orc_bvh* orc_build(const float* tris, int n, int max_leaf) {
         if (nn->triCount > 0) {
             gather[ng++] = ct.n2;                       /* leaf root: one-child node */
         } else {
-            int ch[2] = {nn->left, nn->right};
-            for (int c = 0; c < 2; c++) {
-                const bnode2* cn = &B->nodes[ch[c]];
-                if (cn->triCount > 0 || ng >= 3) { gather[ng++] = ch[c]; continue; }
-                gather[ng++] = cn->left; gather[ng++] = cn->right;
-            }
+            ng = collapse_gather(B->nodes, ccost, ct.n2, gather);
         }
         uint8_t* out = B->b4 + 64 * (size_t)me;
         uint16_t hx[8], hy[8], hz[8]; int32_t child[4];
@@ -277,6 +335,7 @@ orc_bvh* orc_build(const float* tris, int n, int max_leaf) {
             if (B->nodes[gather[i]].triCount == 0) cs[csp++] = (ctask){gather[i], me, i, ct.depth + 1};
     }
     free(cs);
+    free(ccost);
     B->b4_depth = maxd;
     return B;
 }

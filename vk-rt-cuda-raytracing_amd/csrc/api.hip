@@ -425,7 +425,9 @@ struct AsyncTracer {
     std::mutex mu;
     std::condition_variable req, ready;
     std::thread th;
-    bool running = false, hasRequest = false, hasResult = false;
+    bool running = false, hasRequest = false;
+    bool hasHostResult = false;          // host[readIdx] holds a finished host frame
+    uint64_t reqSeq = 0, doneSeq = 0;    // requests submitted / the latest one finished
     int device = 0;
     CudaBVH_t bvh = nullptr;             // handle of the pending request
     CudaBVH_t busy = nullptr;            // handle of the frame being rendered
@@ -449,6 +451,7 @@ struct AsyncTracer {
             if (!running) break;
             CudaBVH_t b = bvh; int W = w, H = h, F = fmt; float X = cx, Y = cy, Z = cz;
             void* tgt = gpuTarget; int wi = writeIdx; uint64_t sz = bufSize;
+            const uint64_t seq = reqSeq;     // requests not yet taken are merged into the latest
             hasRequest = false;
             busy = b;
             lk.unlock();
@@ -463,7 +466,8 @@ struct AsyncTracer {
                 }
             }
             lk.lock();
-            readIdx = wi; writeIdx = 1 - wi; hasResult = true;
+            if (!tgt) { readIdx = wi; writeIdx = 1 - wi; hasHostResult = true; }
+            doneSeq = seq;
             busy = nullptr;
             ready.notify_all();
         }
@@ -486,7 +490,7 @@ struct AsyncTracer {
         { std::lock_guard<std::mutex> g(mu); if (!running) return; running = false; }
         req.notify_all();
         if (th.joinable()) th.join();
-        hasResult = false; hasRequest = false;
+        hasHostResult = false; hasRequest = false;
     }
     ~AsyncTracer() { stop(); }
 };
@@ -499,7 +503,11 @@ std::atomic<int> g_async_calls_host{0}, g_async_calls_gpu{0};
 static void async_release(CudaBVH_t bvh) {
     std::unique_lock<std::mutex> lk(g_async.mu);
     if (!g_async.running) return;
-    if (g_async.hasRequest && g_async.bvh == bvh) g_async.hasRequest = false;
+    if (g_async.hasRequest && g_async.bvh == bvh) {
+        g_async.hasRequest = false;                 // dropped: nobody waits for it any longer
+        g_async.doneSeq = g_async.reqSeq;
+        g_async.ready.notify_all();
+    }
     if (g_async.bvh == bvh) g_async.bvh = nullptr;
     g_async.ready.wait(lk, [&] { return g_async.busy != bvh; });
 }
@@ -524,9 +532,10 @@ int cudaBVH_traceToHostPtr(CudaBVH_t bvh, int width, int height, float camOx, fl
     g_async.bvh = bvh; g_async.w = width; g_async.h = height; g_async.fmt = outFmt;
     g_async.cx = camOx; g_async.cy = camOy; g_async.cz = camOz; g_async.gpuTarget = nullptr;
     g_async.hasRequest = true;
+    const uint64_t my = ++g_async.reqSeq;
     g_async.req.notify_all();
-    if (call == 1) g_async.ready.wait(lk, [] { return g_async.hasResult; });   // first frame blocks
-    if (g_async.hasResult) memcpy(h_outRGBA, g_async.host[g_async.readIdx].data(), bufSize);
+    if (call == 1) g_async.ready.wait(lk, [my] { return g_async.doneSeq >= my; });   // first frame blocks
+    if (g_async.hasHostResult) memcpy(h_outRGBA, g_async.host[g_async.readIdx].data(), bufSize);
     return 0;
 }
 
@@ -540,10 +549,10 @@ void cudaBVH_traceToGPUAsync(CudaBVH_t bvh, int width, int height, float camOx, 
     std::unique_lock<std::mutex> lk(g_async.mu);
     g_async.bvh = bvh; g_async.w = width; g_async.h = height; g_async.fmt = outFmt;
     g_async.cx = camOx; g_async.cy = camOy; g_async.cz = camOz; g_async.gpuTarget = d_outRGBA;
-    g_async.hasResult = (call == 1) ? false : g_async.hasResult;
     g_async.hasRequest = true;
+    const uint64_t my = ++g_async.reqSeq;
     g_async.req.notify_all();
-    if (call == 1) g_async.ready.wait(lk, [] { return g_async.hasResult; });
+    if (call == 1) g_async.ready.wait(lk, [my] { return g_async.doneSeq >= my; });   // first frame blocks
 }
 
 void* cudaBVH_importBufferFd(int fd, uint64_t size) {
