@@ -187,6 +187,27 @@ enum {
 CudaBVH_t cudaBVH_buildDevice(const void* d_vertices, int numVertices, int vertexStride, const void* d_indices,
                               int indexBytes, int numTris);
 
+/* One geometry of a BLAS in device memory (the fields of cudaBVH_buildDevice). */
+typedef struct {
+    const void* vertices;   /* device; position (3 floats) at byte offset index * vertexStride */
+    int numVertices;
+    int vertexStride;       /* bytes, >= 12, multiple of 4 */
+    const void* indices;    /* device 16- or 32-bit indices, or NULL */
+    int indexBytes;         /* 2 or 4; 0 without indices (corners 3t..3t+2) */
+    int numTris;
+} CudaGeometry;
+
+/* Many BLASes from device geometry in one pass (no counterpart in the
+ * reference; it replaces the per-BLAS loop of the layer's pending-BLAS
+ * capture, VkLayer_CudaRT.cpp:3039-3234, which reads every geometry back to
+ * the host and calls cudaBVH_build per BLAS). BLAS k is the triangles of
+ * geometries [geomOffsets[k], geomOffsets[k+1]) in order. All are gathered on
+ * the GPU and built as one forest (BLASes of more than 4 triangles; smaller
+ * ones alone), then each tree becomes its own handle out[k], identical to
+ * cudaBVH_buildDevice on the same triangles. 0 on success; on failure every
+ * out[k] is NULL and cudaBVH_lastError() says why. */
+int cudaBVH_buildManyDevice(const CudaGeometry* geoms, const int* geomOffsets, int numBlas, CudaBVH_t* out);
+
 /* Closest hit for n rays. Device pointers unless CUDABVH_FLAG_HOST_POINTERS.
  * Asynchronous on the handle's stream unless CUDABVH_FLAG_SYNC or host
  * pointers. 0 on success. */

@@ -68,13 +68,3 @@ def test_null_handle_conventions(lib, pkg):
     assert lib.cudaBVH_build(None, 10) is None
     assert lib.cudaBVH_buildFromAABBs(None, 0) is None
     assert lib.cudaBVH_buildTLASFast(None, 0, None, None) == 0
-
-
-def test_tlas_fast_matches_oracle(pkg, oracle):
-    rng = np.random.default_rng(21)
-    for n in (1, 2, 3, 64, 777):
-        lo = rng.uniform(-10, 10, (n, 3)).astype(np.float32)
-        aabbs = np.concatenate([lo, lo + rng.uniform(0.1, 3, (n, 3)).astype(np.float32)], axis=1)
-        nodes, order = pkg.build_tlas_fast(aabbs)
-        onodes, oorder = oracle.lbvh_tlas(aabbs)
-        assert np.array_equal(nodes, onodes) and np.array_equal(order, oorder)

@@ -87,3 +87,71 @@ def test_build_device_rejects_bad_indices(pkg, dev):
         pkg.BVH.build_device(V, idx.shape[0], indices=torch.from_numpy(bad).to(dev), num_vertices=verts.shape[0])
     with pytest.raises(pkg.CudaBVHError):
         pkg.BVH.build_device(V, idx.shape[0], indices=None, num_vertices=verts.shape[0])   # too few corners
+
+
+def _geometry(verts, idx, index_bytes, stride, dev, soup=None):
+    """Device vertex buffer (position first, `stride` bytes per vertex) and
+    index buffer of one geometry."""
+    import torch
+    vb = soup.reshape(-1, 3) if index_bytes == 0 else verts
+    nv = vb.shape[0]
+    pad = np.zeros((nv, stride // 4), np.float32)
+    pad[:, :3] = vb
+    v = torch.from_numpy(pad.reshape(-1)).to(dev)
+    ind = None
+    if index_bytes == 2:
+        ind = torch.from_numpy(idx.astype(np.int16).reshape(-1)).to(dev)
+    elif index_bytes == 4:
+        ind = torch.from_numpy(idx.astype(np.int32).reshape(-1)).to(dev)
+    nt = idx.shape[0]
+    return (v, ind, stride, nv, nt)
+
+
+def test_build_many_device_equals_single_builds(pkg, oracle, dev):
+    """cudaBVH_buildManyDevice: BLASes of one or several geometries (16/32-bit
+    or no indices, strides 12-32 bytes, and a BLAS below the forest's
+    minimum of five triangles) built in one pass; every handle equals
+    cudaBVH_buildDevice / cudaBVH_build on the BLAS's triangles (nodes,
+    exports, depth, bounds, stackless export) and the oracle's tree."""
+    specs = [[(4, 12, 30)], [(2, 16, 20), (0, 12, 12)], [(0, 24, 1)], [(4, 32, 40), (2, 12, 8), (4, 16, 25)],
+             [(2, 12, 6)]]
+    blases, soups = [], []
+    for k, spec in enumerate(specs):
+        geoms, soup_k = [], []
+        for j, (ib, stride, grid) in enumerate(spec):
+            verts, idx, soup = _grid_mesh(grid, seed=100 * k + j)
+            if grid == 1:
+                idx, soup = idx[:1], soup[:1]                      # a one-triangle BLAS
+            geoms.append(_geometry(verts, idx, ib, stride, dev, soup))
+            soup_k.append(soup)
+        blases.append(geoms)
+        soups.append(np.concatenate(soup_k))
+    many = pkg.BVH.build_many_device(blases)
+    assert len(many) == len(specs)
+    for k, (b, soup) in enumerate(zip(many, soups)):
+        with pkg.BVH.build(soup) as ref:
+            _same_structure(pkg, b, ref)
+            assert np.array_equal(b.stackless_bvh2(), ref.stackless_bvh2())
+            assert np.array_equal(b.leaf_prim_ids(), ref.leaf_prim_ids())
+            assert b.bounds() == ref.bounds() and b.num_tris == soup.shape[0]
+            t, pid = _pinhole_hits(pkg, b, dev)
+            t2, pid2 = _pinhole_hits(pkg, ref, dev)
+            assert np.array_equal(pid, pid2) and np.array_equal(t.view(np.uint32), t2.view(np.uint32))
+        ob = oracle.OracleBVH(soup)
+        assert np.array_equal(b.node_data(), ob.nodes()), k
+        assert b.trace_status() == 0
+        b.destroy()
+
+
+def test_build_many_device_rejects_bad_indices(pkg, dev):
+    import torch
+    verts, idx, soup = _grid_mesh(8, seed=3)
+    good = _geometry(verts, idx, 4, 12, dev)
+    bad_idx = idx.copy()
+    bad_idx[5, 1] = verts.shape[0]                                # one past the last vertex
+    bad = _geometry(verts, bad_idx, 4, 12, dev)
+    with pytest.raises(pkg.CudaBVHError):
+        pkg.BVH.build_many_device([[good], [bad]])
+    with pytest.raises(pkg.CudaBVHError):
+        pkg.BVH.build_many_device([[(good[0], good[1], 10, good[3], good[4])]])   # stride below 12
+    torch.cuda.synchronize()

@@ -66,6 +66,11 @@ class CudaInstance(C.Structure):
     _fields_ = [("blas", C.c_int), ("objectToWorld", C.c_float * 12), ("albedo", C.c_float * 3)]
 
 
+class CudaGeometry(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("numVertices", C.c_int), ("vertexStride", C.c_int),
+                ("indices", C.c_void_p), ("indexBytes", C.c_int), ("numTris", C.c_int)]
+
+
 class CudaSceneHit(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("t", "instId", "primId", "u", "v")]
 
@@ -104,6 +109,7 @@ _I = C.c_int
 SIGNATURES = {
     "cudaBVH_build": (_H, [_P, _I]),
     "cudaBVH_buildDevice": (_H, [_P, _I, _I, _P, _I, _I]),
+    "cudaBVH_buildManyDevice": (_I, [_P, _P, _I, _P]),
     "cudaBVH_destroy": (None, [_H]),
     "cudaBVH_tracePrimary": (_F, [_H, _I, _F, _F, _F, _P]),
     "cudaBVH_traceDiffuse": (_F, [_H, _I, _P, _P, _P, _P, _P, _P, _P]),
@@ -300,6 +306,27 @@ class BVH:
         if not h:
             raise CudaBVHError(f"buildDevice: {last_error()}")
         return cls(h, num_tris)
+
+    @staticmethod
+    def build_many_device(blases: list) -> list:
+        """One BVH per BLAS from device geometry in one pass. blases: a list
+        of BLASes, each a list of geometries (vertices, indices, vertex_stride,
+        num_vertices, num_tris) of device tensors (indices may be None)."""
+        geoms, offs = [], [0]
+        for blas in blases:
+            for (vtx, idx, stride, nv, nt) in blas:
+                geoms.append(CudaGeometry(_ptr(vtx), int(nv), int(stride), None if idx is None else _ptr(idx),
+                                          0 if idx is None else idx.element_size(), int(nt)))
+            offs.append(len(geoms))
+        arr = (CudaGeometry * len(geoms))(*geoms)
+        off = np.asarray(offs, np.int32)
+        out = (C.c_void_p * len(blases))()
+        lib = load_library()
+        rc = lib.cudaBVH_buildManyDevice(arr, off.ctypes.data, len(blases), out)
+        if rc != 0:
+            raise CudaBVHError(f"buildManyDevice: {last_error()}")
+        counts = [sum(int(g[4]) for g in blas) for blas in blases]
+        return [BVH(out[k], counts[k]) for k in range(len(blases))]
 
     @classmethod
     def build_from_aabbs(cls, aabbs: np.ndarray) -> "BVH":

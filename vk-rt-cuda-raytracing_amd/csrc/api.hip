@@ -46,16 +46,16 @@ static void free_handle_device(CudaBVHHandle* h) {
     h->d_leafOfPrim = nullptr;
 }
 
-// Builds a handle from n x 9 floats (original order). d_tri9 (nullable): the
-// same triangles already on the device (not freed here).
-static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf, const int* sceneOff = nullptr,
-                                 int numScenes = 1, std::vector<int>* roots4 = nullptr,
-                                 const float* d_tri9 = nullptr) {
-    rt::clear_error();
-    if (!tri9 || n <= 0) { set_error("cudaBVH_build: need tris != NULL and numTris > 0 (got %d)", n); return nullptr; }
-    if (n > (1 << 27)) { set_error("cudaBVH_build: %d triangles exceed the leaf encoding (2^27)", n); return nullptr; }
-    auto* h = new CudaBVHHandle();
+// Completes a handle whose device structure (d_nodes4, d_tris, d_leafOfPrim,
+// numNodes4, depth4) is set: the bounds, the host copies the reference's
+// getters hand out (nodes, SoA and packed triangles, the stackless DFS export
+// when nodes2 is given) from the n x 9 host triangles (original order) and
+// the leaf order, the traversal scratch and the default launch
+// configurations. Frees the handle and returns nullptr on failure.
+static CudaBVHHandle* make_handle(CudaBVHHandle* h, const float* tri9, int n, std::vector<uint8_t>&& hNodes4,
+                                  std::vector<int>&& ordered, std::vector<rt::Node2>* nodes2) {
     h->numTris = n;
+    h->numOrdered = n;
     float bmin[3] = {1e30f, 1e30f, 1e30f}, bmax[3] = {-1e30f, -1e30f, -1e30f};
     for (int i = 0; i < n; i++)
         for (int k = 0; k < 9; k++) {
@@ -66,61 +66,33 @@ static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf, const in
     for (int a = 0; a < 3; a++) { h->bboxMin[a] = bmin[a]; h->bboxMax[a] = bmax[a]; h->bboxCenter[a] = (bmin[a] + bmax[a]) * 0.5f; }
     const float ex = bmax[0] - bmin[0], ey = bmax[1] - bmin[1], ez = bmax[2] - bmin[2];
     h->bboxExtent = sqrtf(ex * ex + ey * ey + ez * ez);
-
-    {
-        // Device build: upload the triangles once, build, keep the device
-        // structure; host copies serve the reference's getters.
-        hipStream_t s = hstream(h);
-        float* d_in = const_cast<float*>(d_tri9);
-        float* d_own = nullptr;
-        rt::GpuBVH g;
-        if (!d_in) {
-            HIP_TRY(hipMalloc((void**)&d_own, size_t(n) * 36), goto fail);
-            HIP_TRY(hipMemcpyAsync(d_own, tri9, size_t(n) * 36, hipMemcpyHostToDevice, s), hipFree(d_own); goto fail);
-            d_in = d_own;
-        }
-        if (rt::gpu_build(d_in, n, maxLeaf, s, g, sceneOff, numScenes, roots4) != 0) {
-            hipFree(d_own); hipFree(g.nodes2); hipFree(g.nodes4); hipFree(g.tris); hipFree(g.leafOf); hipFree(g.ordered);
-            goto fail;
-        }
-        hipFree(d_own);
-        h->d_nodes4 = g.nodes4;
-        h->d_tris = g.tris;
-        h->d_leafOfPrim = g.leafOf;
-        h->numOrdered = n;
-        h->numNodes4 = g.numNodes4;
-        h->depth4 = g.depth4;
-        h->h_nodes4.resize(size_t(g.numNodes4) * 64);
-        h->h_ordered.resize(n);
-        std::vector<rt::Node2> nodes2(g.numNodes2);
-        hipError_t e = hipMemcpy(h->h_nodes4.data(), g.nodes4, h->h_nodes4.size(), hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(h->h_ordered.data(), g.ordered, size_t(n) * 4, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(nodes2.data(), g.nodes2, nodes2.size() * sizeof(rt::Node2), hipMemcpyDeviceToHost);
-        hipFree(g.nodes2); hipFree(g.ordered);
-        if (e != hipSuccess) { set_error("copying the built BVH back failed: %s", hipGetErrorString(e)); goto fail; }
-        // Host exports for the Vulkan-side consumers (backend :656-735)
-        const int m = n;
-        for (int a = 0; a < 9; a++) h->h_soa[a].resize(m);
-        h->h_packed.assign(size_t(m) * 12, 0.f);
-        for (int i = 0; i < m; i++) {
-            const int p = h->h_ordered[i];
-            const float* src = tri9 + 9 * size_t(p);
-            float* pk = &h->h_packed[12 * size_t(i)];
-            for (int a = 0; a < 9; a++) { h->h_soa[a][i] = src[a]; pk[a] = src[a]; }
-            // the original ID's int bits, which is how the layer and the
-            // RT-IR executor read this word (VkLayer_CudaRT.cpp:4410,
-            // layer/rt_ir_exec.cu:167); the reference writes float(ID)
-            // there (backend :730-733), which those readers misinterpret
-            const int32_t id = p;
-            memcpy(&pk[9], &id, sizeof id);
-        }
-        if (numScenes == 1) {
-            rt::export_dfs(nodes2, h->h_dfs);
-            h->numDFS = int(nodes2.size());
-            h->h_nodes2 = std::move(nodes2);
-        }
+    h->h_nodes4 = std::move(hNodes4);
+    h->h_ordered = std::move(ordered);
+    // Host exports for the Vulkan-side consumers (backend :656-735)
+    for (int a = 0; a < 9; a++) h->h_soa[a].resize(n);
+    h->h_packed.assign(size_t(n) * 12, 0.f);
+    for (int i = 0; i < n; i++) {
+        const int p = h->h_ordered[i];
+        const float* src = tri9 + 9 * size_t(p);
+        float* pk = &h->h_packed[12 * size_t(i)];
+        for (int a = 0; a < 9; a++) { h->h_soa[a][i] = src[a]; pk[a] = src[a]; }
+        // the original ID's int bits, which is how the layer and the
+        // RT-IR executor read this word (VkLayer_CudaRT.cpp:4410,
+        // layer/rt_ir_exec.cu:167); the reference writes float(ID)
+        // there (backend :730-733), which those readers misinterpret
+        const int32_t id = p;
+        memcpy(&pk[9], &id, sizeof id);
     }
-    if (rtk::alloc_scratch(h->scratch) != 0) goto fail;
+    if (nodes2) {
+        rt::export_dfs(*nodes2, h->h_dfs);
+        h->numDFS = int(nodes2->size());
+        h->h_nodes2 = std::move(*nodes2);
+    }
+    if (rtk::alloc_scratch(h->scratch) != 0) {
+        free_handle_device(h);
+        delete h;
+        return nullptr;
+    }
     // defaults: the configurations the autotuner picks most often on the
     // 1080p terrain frame (refill / 12 also traces the 256-scene forest 10 %
     // faster than while-while / 8)
@@ -132,6 +104,50 @@ static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf, const in
         fprintf(stderr, "[CudaBVH] built %d tris: %d BVH4 nodes (depth %d), %d BVH2 nodes\n", n, h->numNodes4,
                 h->depth4, h->numDFS);
     return h;
+}
+
+// Builds a handle from n x 9 floats (original order). d_tri9 (nullable): the
+// same triangles already on the device (not freed here). With sceneOff the
+// handle is a forest of numScenes trees (roots4: each tree's BVH4 root).
+static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf, const int* sceneOff = nullptr,
+                                 int numScenes = 1, std::vector<int>* roots4 = nullptr,
+                                 const float* d_tri9 = nullptr, std::vector<rt::Node2>* forestNodes2 = nullptr) {
+    rt::clear_error();
+    if (!tri9 || n <= 0) { set_error("cudaBVH_build: need tris != NULL and numTris > 0 (got %d)", n); return nullptr; }
+    if (n > (1 << 27)) { set_error("cudaBVH_build: %d triangles exceed the leaf encoding (2^27)", n); return nullptr; }
+    auto* h = new CudaBVHHandle();
+    hipStream_t s = hstream(h);
+    float* d_in = const_cast<float*>(d_tri9);
+    float* d_own = nullptr;
+    rt::GpuBVH g;
+    std::vector<uint8_t> hNodes4;
+    std::vector<int> ordered(n);
+    std::vector<rt::Node2> nodes2;
+    hipError_t e;
+    if (!d_in) {
+        HIP_TRY(hipMalloc((void**)&d_own, size_t(n) * 36), goto fail);
+        HIP_TRY(hipMemcpyAsync(d_own, tri9, size_t(n) * 36, hipMemcpyHostToDevice, s), hipFree(d_own); goto fail);
+        d_in = d_own;
+    }
+    if (rt::gpu_build(d_in, n, maxLeaf, s, g, sceneOff, numScenes, roots4) != 0) {
+        hipFree(d_own); hipFree(g.nodes2); hipFree(g.nodes4); hipFree(g.tris); hipFree(g.leafOf); hipFree(g.ordered);
+        goto fail;
+    }
+    hipFree(d_own);
+    h->d_nodes4 = g.nodes4;
+    h->d_tris = g.tris;
+    h->d_leafOfPrim = g.leafOf;
+    h->numNodes4 = g.numNodes4;
+    h->depth4 = g.depth4;
+    hNodes4.resize(size_t(g.numNodes4) * 64);
+    nodes2.resize(g.numNodes2);
+    e = hipMemcpy(hNodes4.data(), g.nodes4, hNodes4.size(), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(ordered.data(), g.ordered, size_t(n) * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(nodes2.data(), g.nodes2, nodes2.size() * sizeof(rt::Node2), hipMemcpyDeviceToHost);
+    hipFree(g.nodes2); hipFree(g.ordered);
+    if (e != hipSuccess) { set_error("copying the built BVH back failed: %s", hipGetErrorString(e)); goto fail; }
+    if (numScenes > 1 && forestNodes2) *forestNodes2 = nodes2;
+    return make_handle(h, tri9, n, std::move(hNodes4), std::move(ordered), numScenes == 1 ? &nodes2 : nullptr);
 fail:
     free_handle_device(h);
     delete h;
@@ -298,6 +314,145 @@ CudaBVH_t cudaBVH_buildDevice(const void* d_vertices, int numVertices, int verte
     CudaBVHHandle* h = build_impl(host.data(), numTris, 4, nullptr, 1, nullptr, d_tri);
     hipFree(d_tri);
     return h;
+}
+
+// Levels of a host BVH4 (64-byte nodes, root 0): the length of its longest
+// root-to-leaf path of nodes.
+static int bvh4_depth(const std::vector<uint8_t>& nodes) {
+    const int nn = int(nodes.size() / 64);
+    if (nn == 0) return 0;
+    std::vector<int> lev(nn, 0);
+    lev[0] = 1;
+    int d = 1;
+    for (int i = 0; i < nn; i++) {                 // children follow their parent in pre-order
+        int32_t ch[4];
+        memcpy(ch, nodes.data() + 64 * size_t(i) + 48, 16);
+        for (int c = 0; c < 4; c++)
+            if (ch[c] >= 0 && ch[c] < nn) { lev[ch[c]] = lev[i] + 1; d = std::max(d, lev[ch[c]]); }
+    }
+    return d;
+}
+
+static bool geometry_ok(const CudaGeometry& g) {
+    return g.vertices && g.numVertices > 0 && g.numTris > 0 && g.vertexStride >= 12 && g.vertexStride % 4 == 0 &&
+           ((g.indices && (g.indexBytes == 2 || g.indexBytes == 4)) || (!g.indices && g.indexBytes == 0)) &&
+           (g.indices || (long long)g.numTris * 3 <= g.numVertices);
+}
+
+int cudaBVH_buildManyDevice(const CudaGeometry* geoms, const int* geomOffsets, int numBlas, CudaBVH_t* out) {
+    const char* fn = "cudaBVH_buildManyDevice";
+    rt::clear_error();
+    if (!geoms || !geomOffsets || numBlas <= 0 || !out || geomOffsets[0] != 0) {
+        set_error("%s: bad arguments", fn);
+        return -1;
+    }
+    for (int k = 0; k < numBlas; k++) out[k] = nullptr;
+    // triangles per BLAS; the forest takes the BLASes of more than four
+    // triangles (contiguous, first), the smaller ones are built alone
+    std::vector<long long> count(numBlas, 0);
+    for (int k = 0; k < numBlas; k++) {
+        if (geomOffsets[k + 1] <= geomOffsets[k]) { set_error("%s: BLAS %d has no geometry", fn, k); return -1; }
+        for (int g = geomOffsets[k]; g < geomOffsets[k + 1]; g++) {
+            if (!geometry_ok(geoms[g])) { set_error("%s: bad geometry %d of BLAS %d", fn, g, k); return -1; }
+            count[k] += geoms[g].numTris;
+        }
+        if (count[k] > (1 << 27)) { set_error("%s: BLAS %d exceeds the leaf encoding", fn, k); return -1; }
+    }
+    std::vector<int> big, small, start(numBlas);
+    for (int k = 0; k < numBlas; k++) (count[k] > 4 ? big : small).push_back(k);
+    long long total = 0;
+    for (int pass = 0; pass < 2; pass++)
+        for (int k : (pass == 0 ? big : small)) { start[k] = (int)total; total += count[k]; }
+    if (total > (1ll << 28)) { set_error("%s: %lld triangles in all", fn, total); return -1; }
+    float* dTri = nullptr;
+    int* dBad = nullptr;
+    if (hipMalloc((void**)&dTri, size_t(total) * 36) != hipSuccess || hipMalloc((void**)&dBad, sizeof(int)) != hipSuccess) {
+        hipFree(dTri);
+        set_error("%s: cannot allocate %lld triangles", fn, total);
+        return -1;
+    }
+    std::vector<float> host(size_t(total) * 9);
+    int bad = 0;
+    hipError_t e = hipMemset(dBad, 0, sizeof(int));
+    for (int k = 0; k < numBlas && e == hipSuccess; k++) {
+        long long off = start[k];
+        for (int g = geomOffsets[k]; g < geomOffsets[k + 1] && e == hipSuccess; g++) {
+            const CudaGeometry& G = geoms[g];
+            e = rtk::launch_gather_tris(G.vertices, G.numVertices, G.vertexStride, G.indices, G.indexBytes, G.numTris,
+                                        dTri + 9 * size_t(off), dBad, nullptr);
+            off += G.numTris;
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpy(&bad, dBad, sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && !bad) e = hipMemcpy(host.data(), dTri, host.size() * 4, hipMemcpyDeviceToHost);
+    hipFree(dBad);
+    if (e != hipSuccess || bad) {
+        hipFree(dTri);
+        if (bad) set_error("%s: an index is >= its geometry's numVertices", fn);
+        else set_error("%s: gather failed: %s", fn, hipGetErrorString(e));
+        return -1;
+    }
+    auto fail_all = [&]() {
+        for (int k = 0; k < numBlas; k++) { cudaBVH_destroy(out[k]); out[k] = nullptr; }
+        hipFree(dTri);
+        return -1;
+    };
+    for (int k : small) {
+        out[k] = build_impl(host.data() + 9 * size_t(start[k]), (int)count[k], 4, nullptr, 1, nullptr,
+                            dTri + 9 * size_t(start[k]));
+        if (!out[k]) return fail_all();
+    }
+    if (!big.empty()) {
+        const int R = (int)big.size();
+        std::vector<int> off(R + 1, 0);
+        for (int i = 0; i < R; i++) off[i + 1] = off[i] + (int)count[big[i]];
+        std::vector<int> roots4;
+        std::vector<rt::Node2> fn2;
+        CudaBVHHandle* forest = build_impl(host.data(), off[R], 4, off.data(), R, &roots4, dTri, &fn2);
+        if (!forest) return fail_all();
+        // BVH2 subtree sizes: the trees lie one after another in pre-order
+        std::vector<int> sz(fn2.size(), 1);
+        for (int v = (int)fn2.size() - 1; v >= 0; v--)
+            if (fn2[v].triCount == 0) sz[v] = 1 + sz[fn2[v].left] + sz[fn2[v].right];
+        int root2 = 0;
+        for (int i = 0; i < R; i++) {
+            const int k = big[i];
+            const int first4 = roots4[i], end4 = i + 1 < R ? roots4[i + 1] : forest->numNodes4;
+            const int t0 = off[i], nt = (int)count[k];
+            auto* h = new CudaBVHHandle();
+            h->numNodes4 = end4 - first4;
+            std::vector<uint8_t> hn4(size_t(h->numNodes4) * 64);
+            const bool ok = hipMalloc(&h->d_nodes4, hn4.size()) == hipSuccess &&
+                            hipMalloc(&h->d_tris, size_t(nt) * 48) == hipSuccess &&
+                            hipMalloc((void**)&h->d_leafOfPrim, size_t(nt) * 4) == hipSuccess &&
+                            rtk::launch_slice_forest(forest->d_nodes4, first4, h->numNodes4, forest->d_tris,
+                                                     forest->d_leafOfPrim, t0, nt, h->d_nodes4, h->d_tris,
+                                                     h->d_leafOfPrim, nullptr) == 0 &&
+                            hipMemcpy(hn4.data(), h->d_nodes4, hn4.size(), hipMemcpyDeviceToHost) == hipSuccess;
+            if (!ok) {
+                free_handle_device(h);
+                delete h;
+                cudaBVH_destroy(forest);
+                set_error("%s: cannot split tree %d out of the forest", fn, i);
+                return fail_all();
+            }
+            h->depth4 = bvh4_depth(hn4);
+            std::vector<int> ord(forest->h_ordered.begin() + t0, forest->h_ordered.begin() + t0 + nt);
+            for (int& x : ord) x -= t0;
+            // the tree's BVH2 nodes, rebased like its BVH4 nodes
+            std::vector<rt::Node2> sub(fn2.begin() + root2, fn2.begin() + root2 + sz[root2]);
+            for (rt::Node2& nd : sub) {
+                if (nd.triCount == 0) { nd.left -= root2; nd.right -= root2; }
+                else nd.triStart -= t0;
+            }
+            root2 += sz[root2];
+            out[k] = make_handle(h, host.data() + 9 * size_t(t0), nt, std::move(hn4), std::move(ord), &sub);
+            if (!out[k]) { cudaBVH_destroy(forest); return fail_all(); }
+        }
+        cudaBVH_destroy(forest);
+    }
+    hipFree(dTri);
+    return 0;
 }
 
 void cudaBVH_destroy(CudaBVH_t bvh) {
@@ -670,20 +825,41 @@ int cudaBVH_buildTLASFast(const float* aabbs, int numAABBs, uint32_t** outNodes,
     if (outNodes) *outNodes = nullptr;
     if (outOrdered) *outOrdered = nullptr;
     if (!aabbs || numAABBs <= 0 || !outNodes) { set_error("cudaBVH_buildTLASFast: bad arguments"); return 0; }
-    std::vector<uint32_t> dfs;
-    std::vector<int> ord;
-    const int nn = rt::build_tlas_lbvh(aabbs, numAABBs, dfs, ord);
-    uint32_t* p = (uint32_t*)malloc(dfs.size() * sizeof(uint32_t));
-    if (!p) { set_error("out of host memory"); return 0; }
-    memcpy(p, dfs.data(), dfs.size() * sizeof(uint32_t));
-    *outNodes = p;
-    if (outOrdered) {
-        int* o = (int*)malloc(ord.size() * sizeof(int));
-        if (!o) { free(p); *outNodes = nullptr; set_error("out of host memory"); return 0; }
-        memcpy(o, ord.data(), ord.size() * sizeof(int));
-        *outOrdered = o;
+    if (numAABBs > (1 << 27)) { set_error("cudaBVH_buildTLASFast: %d boxes exceed the leaf encoding", numAABBs); return 0; }
+    // The LBVH is built on the GPU (lbvh.hip); boxes in, nodes and leaf
+    // order out through host arrays the caller frees (backend :1620-1698).
+    const size_t n = size_t(numAABBs), total = 2 * n - 1;
+    const size_t inB = n * 24, nodeB = total * 32, ordB = n * 4, wsB = rtk::lbvh_scratch_bytes(numAABBs);
+    char* d = nullptr;
+    if (hipMalloc((void**)&d, inB + nodeB + ordB + wsB + 256) != hipSuccess) {
+        set_error("cudaBVH_buildTLASFast: cannot allocate %zu device bytes", inB + nodeB + ordB + wsB);
+        return 0;
     }
-    return nn;
+    float* dBoxes = (float*)d;
+    uint32_t* dNodes = (uint32_t*)(d + inB);
+    int* dOrd = (int*)(d + inB + nodeB);
+    void* ws = d + ((inB + nodeB + ordB + 255) & ~size_t(255));
+    uint32_t* p = (uint32_t*)malloc(nodeB);
+    int* o = (int*)malloc(ordB);
+    hipStream_t s = nullptr;
+    int rc = (p && o) ? 0 : -1;
+    if (rc != 0) set_error("out of host memory");
+    if (rc == 0 && hipMemcpyAsync(dBoxes, aabbs, inB, hipMemcpyHostToDevice, s) != hipSuccess) {
+        set_error("cudaBVH_buildTLASFast: upload failed");
+        rc = -1;
+    }
+    if (rc == 0) rc = rtk::lbvh_build(dBoxes, numAABBs, dNodes, dOrd, ws, s);
+    if (rc == 0 && (hipMemcpyAsync(p, dNodes, nodeB, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                    hipMemcpyAsync(o, dOrd, ordB, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                    hipStreamSynchronize(s) != hipSuccess)) {
+        set_error("cudaBVH_buildTLASFast: download failed");
+        rc = -1;
+    }
+    hipFree(d);
+    if (rc != 0) { free(p); free(o); return 0; }
+    *outNodes = p;
+    if (outOrdered) *outOrdered = o; else free(o);
+    return (int)total;
 }
 
 // ------------------------------- extensions -------------------------------

@@ -646,9 +646,50 @@ __global__ void k_gather_tris(const uint8_t* verts, int numVertices, int stride,
     }
 }
 
+// One tree of a forest as a BVH of its own: BVH4 nodes [first, first+count)
+// with inner children rebased by -first and leaf triangle starts by -triShift
+// (trees and their leaf triangles lie one after another in a forest).
+__global__ void k_slice_nodes4(const uint4* __restrict__ src, int first, int count, int triShift, uint4* dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint4* a = src + 4 * (size_t)(first + i);
+    uint4* o = dst + 4 * (size_t)i;
+    o[0] = a[0]; o[1] = a[1]; o[2] = a[2];
+    const uint4 c = a[3];
+    uint32_t w[4] = {c.x, c.y, c.z, c.w};
+    for (int k = 0; k < 4; k++) {
+        const int v = (int)w[k];
+        if (v >= 0) {
+            w[k] = (uint32_t)(v - first);
+        } else if (v < -1) {
+            const int enc = -(v + 2);
+            w[k] = (uint32_t)leaf_code((enc >> 3) - triShift, (enc & 7) + 1);
+        }
+    }
+    o[3] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// dst[i] = src[i] - sub, i < n
+__global__ void k_slice_sub(const int* __restrict__ src, int n, int sub, int* dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] - sub;
+}
+
 }  // namespace rtb
 
 namespace rtk {
+int launch_slice_forest(const void* nodes4, int first4, int count4, const void* tris, const int* leafOf, int t0,
+                        int ntris, void* dNodes4, void* dTris, int* dLeafOf, hipStream_t s) {
+    const int T = 256;
+    hipLaunchKernelGGL(rtb::k_slice_nodes4, dim3((count4 + T - 1) / T), dim3(T), 0, s, (const uint4*)nodes4, first4,
+                       count4, t0, (uint4*)dNodes4);
+    hipLaunchKernelGGL(rtb::k_slice_sub, dim3((ntris + T - 1) / T), dim3(T), 0, s, leafOf + t0, ntris, t0, dLeafOf);
+    hipMemcpyAsync(dTris, (const char*)tris + (size_t)t0 * 48, (size_t)ntris * 48, hipMemcpyDeviceToDevice, s);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { rt::set_error("forest slice launch failed: %s", hipGetErrorString(e)); return -1; }
+    return 0;
+}
+
 hipError_t launch_gather_tris(const void* verts, int numVertices, int stride, const void* idx, int indexBytes,
                               int numTris, float* out, int* bad, hipStream_t s) {
     hipLaunchKernelGGL(rtb::k_gather_tris, dim3((numTris + 255) / 256), dim3(256), 0, s, (const uint8_t*)verts,

@@ -98,80 +98,5 @@ void export_dfs(const std::vector<Node2>& nodes, std::vector<uint32_t>& out) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// LBVH (TLAS fast path)
-// ---------------------------------------------------------------------------
-static inline uint32_t spread10(uint32_t v) {     // bit i -> bit 3i
-    v &= 0x3ffu;
-    v = (v | (v << 16)) & 0x030000ffu;
-    v = (v | (v << 8)) & 0x0300f00fu;
-    v = (v | (v << 4)) & 0x030c30c3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-uint32_t morton3d(float x, float y, float z) {   // 10 bits per axis (backend :265-273)
-    x = fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
-    y = fminf(fmaxf(y * 1024.0f, 0.0f), 1023.0f);
-    z = fminf(fmaxf(z * 1024.0f, 0.0f), 1023.0f);
-    return spread10(uint32_t(x)) * 4u + spread10(uint32_t(y)) * 2u + spread10(uint32_t(z));
-}
-
-int build_tlas_lbvh(const float* aabbs, int n, std::vector<uint32_t>& dfs, std::vector<int>& ordered) {
-    float smin[3] = {1e30f, 1e30f, 1e30f}, smax[3] = {-1e30f, -1e30f, -1e30f};
-    std::vector<float> cen(size_t(n) * 3);
-    for (int i = 0; i < n; i++)
-        for (int a = 0; a < 3; a++) {
-            float c = (aabbs[6 * size_t(i) + a] + aabbs[6 * size_t(i) + 3 + a]) * 0.5f;
-            cen[3 * size_t(i) + a] = c;
-            smin[a] = fminf(smin[a], c); smax[a] = fmaxf(smax[a], c);
-        }
-    float inv[3];
-    for (int a = 0; a < 3; a++) inv[a] = (smax[a] - smin[a] > 1e-8f) ? 1.0f / (smax[a] - smin[a]) : 0.0f;
-    // Stable sort by 30-bit Morton code; ties keep input order.
-    std::vector<uint64_t> key(n);
-    for (int i = 0; i < n; i++) {
-        uint32_t c = morton3d((cen[3 * size_t(i)] - smin[0]) * inv[0], (cen[3 * size_t(i) + 1] - smin[1]) * inv[1],
-                              (cen[3 * size_t(i) + 2] - smin[2]) * inv[2]);
-        key[i] = (uint64_t(c) << 32) | uint32_t(i);
-    }
-    std::sort(key.begin(), key.end());
-    ordered.resize(n);
-    for (int i = 0; i < n; i++) ordered[i] = int(key[i] & 0xffffffffu);
-    // Median split, one item per leaf, pre-order numbering; boxes bottom-up.
-    std::vector<Node2> nodes;
-    nodes.reserve(size_t(2) * n);
-    struct T { int s, e, parent, side; bool post; };
-    std::vector<T> st;
-    st.push_back({0, n, -1, 0, false});
-    while (!st.empty()) {
-        T t = st.back(); st.pop_back();
-        if (t.post) {
-            Node2& p = nodes[t.parent];
-            const Box &L = nodes[p.left].box, &R = nodes[p.right].box;
-            for (int a = 0; a < 3; a++) { p.box.mn[a] = fminf(L.mn[a], R.mn[a]); p.box.mx[a] = fmaxf(L.mx[a], R.mx[a]); }
-            continue;
-        }
-        int me = int(nodes.size());
-        nodes.push_back(Node2{});
-        if (t.parent >= 0) (t.side == 0 ? nodes[t.parent].left : nodes[t.parent].right) = me;
-        Node2& nd = nodes[me];
-        nd.left = nd.right = nd.triStart = nd.triCount = 0;
-        for (int a = 0; a < 3; a++) { nd.box.mn[a] = 1e30f; nd.box.mx[a] = -1e30f; }
-        int cnt = t.e - t.s;
-        if (cnt <= 1) {
-            nd.triStart = t.s; nd.triCount = cnt;
-            const float* b = aabbs + 6 * size_t(ordered[t.s]);
-            for (int a = 0; a < 3; a++) { nd.box.mn[a] = fminf(nd.box.mn[a], b[a]); nd.box.mx[a] = fmaxf(nd.box.mx[a], b[3 + a]); }
-            continue;
-        }
-        int mid = t.s + cnt / 2;
-        st.push_back({0, 0, me, 0, true});
-        st.push_back({mid, t.e, me, 1, false});
-        st.push_back({t.s, mid, me, 0, false});
-    }
-    export_dfs(nodes, dfs);
-    return int(nodes.size());
-}
-
 }  // namespace rt
 

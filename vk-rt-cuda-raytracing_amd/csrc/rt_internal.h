@@ -67,10 +67,7 @@ int build_cwbvh(const std::vector<Node2>& n2, std::vector<uint32_t>& nodes, std:
 // Stackless DFS BVH2 export (backend :456-522): 8 uint32 per node.
 void export_dfs(const std::vector<Node2>& nodes, std::vector<uint32_t>& out);
 
-// LBVH over AABBs for the TLAS fast path (backend :257-389, :1620-1698).
-int build_tlas_lbvh(const float* aabbs, int n, std::vector<uint32_t>& dfs, std::vector<int>& ordered);
 
-uint32_t morton3d(float x, float y, float z);
 
 // Device scratch of one traversal context: the persistent-kernel work
 // counter, a status word (bit 0: stack overflow, unreachable by

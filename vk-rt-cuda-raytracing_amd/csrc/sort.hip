@@ -198,6 +198,45 @@ int trace_sorted(CudaBVHHandle* h, rt::TraceScratch& sc, int n, const CudaRayPac
 }
 
 // Sort only (for tests and measurement): keys and the permutation in sorted order.
+size_t radix_pairs_bytes(int n) {
+    const size_t N = (size_t)n;
+    const size_t tiles = (N + rts2::kTile - 1) / rts2::kTile;
+    return N * 8 + rts2::kRadix * tiles * 4 * 2 + ((rts2::kRadix * tiles + 2047) / 2048 + 16) * 4 + 256;
+}
+
+// Stable LSD radix sort of (key, value) pairs on the low `bits` bits of the
+// keys (8 bits per pass), in place in (keys, vals); ws holds
+// radix_pairs_bytes(n) bytes.
+int radix_sort_pairs(uint32_t* keys, int* vals, int n, int bits, void* ws, hipStream_t s) {
+    const size_t N = (size_t)n;
+    const int tiles = (int)((N + rts2::kTile - 1) / rts2::kTile);
+    char* p = (char*)ws;
+    uint32_t* k1 = (uint32_t*)p; p += N * 4;
+    int* v1 = (int*)p; p += N * 4;
+    int* hist = (int*)p; p += (size_t)rts2::kRadix * tiles * 4;
+    int* offs = (int*)p; p += (size_t)rts2::kRadix * tiles * 4;
+    int* stiles = (int*)p;
+    uint32_t* k0 = keys;
+    int* v0 = vals;
+    const int passes = (bits + 7) / 8;
+    for (int pass = 0; pass < passes; pass++) {
+        const int shift = 8 * pass;
+        hipLaunchKernelGGL(rts2::k_hist, dim3(tiles), dim3(rts2::kThreads), 0, s, k0, n, shift, tiles, hist);
+        rt::scan_exclusive(hist, offs, rts2::kRadix * tiles, stiles + 8, stiles + 16, s);
+        hipLaunchKernelGGL(rts2::k_scatter, dim3(tiles), dim3(rts2::kThreads), 0, s, k0, v0, n, shift, tiles, offs,
+                           k1, v1);
+        std::swap(k0, k1);
+        std::swap(v0, v1);
+    }
+    if (k0 != keys) {
+        hipMemcpyAsync(keys, k0, N * 4, hipMemcpyDeviceToDevice, s);
+        hipMemcpyAsync(vals, v0, N * 4, hipMemcpyDeviceToDevice, s);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { rt::set_error("radix sort launch failed: %s", hipGetErrorString(e)); return -1; }
+    return 0;
+}
+
 int sort_rays(CudaBVHHandle* h, int n, const CudaRayPacket& rays, uint32_t* keysOut, int* permOut, void* ws,
               hipStream_t s) {
     const size_t N = (size_t)n;

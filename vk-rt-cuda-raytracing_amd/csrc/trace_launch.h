@@ -75,6 +75,11 @@ int launch_gen_pinhole(const rtd::Cam& cam, int W, int H, const CudaRayPacket& p
 // n x 9 floats gathered from device vertex / index buffers (build.hip)
 hipError_t launch_gather_tris(const void* verts, int numVertices, int stride, const void* idx, int indexBytes,
                               int numTris, float* out, int* bad, hipStream_t s);
+// Tree of a forest as a BVH of its own (device copies): BVH4 nodes
+// [first4, first4 + count4) rebased, leaf-order triangles [t0, t0 + ntris)
+// and their leafOf entries rebased by -t0.
+int launch_slice_forest(const void* nodes4, int first4, int count4, const void* tris, const int* leafOf, int t0,
+                        int ntris, void* dNodes4, void* dTris, int* dLeafOf, hipStream_t s);
 int launch_bounce(const CudaBVHHandle* h, rt::TraceScratch& sc, const rtd::Cam& cam, int W, int H,
                   const float* hitT, const int* hitId, uint32_t seed, const float light[3], const CudaRayPacket& d,
                   const CudaRayPacket& s, int* src, int* count, bool tileOrder, hipStream_t st);
@@ -83,6 +88,16 @@ int launch_trace_cw(const void* nodes, const void* tris, int depth8, rt::TraceSc
                     hipStream_t s);
 
 size_t sort_scratch_bytes(int n);
+// Stable LSD radix sort of (key, value) pairs on the low `bits` key bits, in
+// place; ws holds radix_pairs_bytes(n) bytes of device scratch.
+size_t radix_pairs_bytes(int n);
+int radix_sort_pairs(uint32_t* keys, int* vals, int n, int bits, void* ws, hipStream_t s);
+// LBVH over n AABBs (6 floats each, device) on the GPU: the stackless DFS
+// nodes (8 words each, 2n - 1 of them) and the leaf order, both device
+// arrays of the caller; ws holds lbvh_scratch_bytes(n). Byte-identical to the
+// host restatement of the reference's LBVH (the oracle's orc_lbvh_tlas).
+size_t lbvh_scratch_bytes(int n);
+int lbvh_build(const float* d_aabbs, int n, uint32_t* d_nodes, int* d_ordered, void* ws, hipStream_t s);
 int trace_sorted(CudaBVHHandle* h, rt::TraceScratch& sc, int n, const CudaRayPacket& rays, const CudaHitPacket* hits,
                  uint8_t* occ, int mode, bool mt, void* ws, hipStream_t s);
 int sort_rays(CudaBVHHandle* h, int n, const CudaRayPacket& rays, uint32_t* keysOut, int* permOut, void* ws,
