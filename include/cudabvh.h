@@ -218,6 +218,20 @@ int cudaBVH_traceClosest(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays,
 int cudaBVH_traceAny(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays,
                      uint8_t* occluded, unsigned flags);
 
+/* cudaBVH_traceClosest without any allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). No counterpart in the reference. Device ray and hit
+ * packets only (flags: CUDABVH_FLAG_SYNC and the intersection mode; host
+ * pointers, ray sort and CWBVH are rejected); results are identical to
+ * cudaBVH_traceClosest. Its work counters, status word and stack spill live
+ * in one static device array of 64 MiB per GPU (held until the process
+ * ends), so a launch runs at most as many workgroups per CU as that spill
+ * region holds; a BVH whose stack would not fit one workgroup per CU is
+ * rejected with an error. Calls that overlap on one GPU (two streams or two
+ * threads) share that array and race: serialise them. */
+int cudaBVH_traceClosestNoAlloc(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays, const CudaHitPacket* hits,
+                                unsigned flags);
+
 /* Closest hit with per-ray counters (BVH4 nodes fetched, triangles tested);
  * the counters equal the CPU oracle's on the same BVH and rays. Device
  * pointers. Slower than cudaBVH_traceClosest; for measurement and tests. */

@@ -133,6 +133,7 @@ SIGNATURES = {
     # extensions
     "cudaBVH_traceClosest": (_I, [_H, _I, C.POINTER(CudaRayPacket), C.POINTER(CudaHitPacket), C.c_uint]),
     "cudaBVH_traceAny": (_I, [_H, _I, C.POINTER(CudaRayPacket), _P, C.c_uint]),
+    "cudaBVH_traceClosestNoAlloc": (_I, [_H, _I, C.POINTER(CudaRayPacket), C.POINTER(CudaHitPacket), C.c_uint]),
     "cudaBVH_traceClosestCounted": (_I, [_H, _I, C.POINTER(CudaRayPacket), C.POINTER(CudaHitPacket), _P, _P,
                                          C.c_uint]),
     "cudaBVH_traceAnyCounted": (_I, [_H, _I, C.POINTER(CudaRayPacket), _P, _P, _P, C.c_uint]),
@@ -460,6 +461,14 @@ class BVH:
         n = int(rays["ox"].shape[0]) if isinstance(rays, dict) else int(rays.ox.shape[0])
         rp, hp = ray_packet(rays), hit_packet(t, prim_id, u, v)
         _check(self.lib.cudaBVH_traceClosest(self.handle, n, C.byref(rp), C.byref(hp), flags), "traceClosest")
+
+    def trace_closest_no_alloc(self, rays, t, prim_id=None, u=None, v=None, flags: int = 0) -> None:
+        """trace_closest through the library's static device scratch: no
+        allocation (device packets only; overlapping calls on a GPU race)."""
+        n = int(rays["ox"].shape[0]) if isinstance(rays, dict) else int(rays.ox.shape[0])
+        rp, hp = ray_packet(rays), hit_packet(t, prim_id, u, v)
+        _check(self.lib.cudaBVH_traceClosestNoAlloc(self.handle, n, C.byref(rp), C.byref(hp), flags),
+               "traceClosestNoAlloc")
 
     def trace_any(self, rays, occluded, flags: int = 0) -> None:
         n = int(rays["ox"].shape[0]) if isinstance(rays, dict) else int(rays.ox.shape[0])

@@ -991,6 +991,35 @@ int cudaBVH_traceClosest(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays, 
 int cudaBVH_traceAny(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays, uint8_t* occluded, unsigned flags) {
     return trace_packet(bvh, numRays, rays, nullptr, occluded, nullptr, nullptr, flags, rtk::kModeAny, "cudaBVH_traceAny");
 }
+// Closest hit through the static scratch of the current device (no device or
+// host allocation): the ray class's launch configuration with at most as many
+// resident workgroups per CU as the static spill region holds.
+int cudaBVH_traceClosestNoAlloc(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays, const CudaHitPacket* hits,
+                                unsigned flags) {
+    const char* fn = "cudaBVH_traceClosestNoAlloc";
+    rt::clear_error();
+    if (!check_handle(bvh, fn)) return -1;
+    if (numRays == 0) return 0;
+    if (numRays < 0 || !packet_ok(rays) || !hits || !hits->t) { set_error("%s: bad arguments", fn); return -1; }
+    const unsigned isect = flags & CUDABVH_ISECT_MASK;
+    if (isect > 1 || (flags & ~(unsigned)(CUDABVH_FLAG_SYNC | CUDABVH_ISECT_MASK))) {
+        set_error("%s: device packets only (flags 0x%x: no host pointers, ray sort or CWBVH)", fn, flags);
+        return -1;
+    }
+    rt::TraceScratch sc;
+    if (rtk::noalloc_scratch(sc) != 0) return -1;
+    hipStream_t s = hstream(bvh);
+    rtk::TraceArgs a;
+    a.n = numRays;
+    a.ox = rays->ox; a.oy = rays->oy; a.oz = rays->oz; a.dx = rays->dx; a.dy = rays->dy; a.dz = rays->dz;
+    a.tmin = rays->tmin; a.tmax = rays->tmax;
+    a.t = hits->t; a.id = hits->primId; a.u = hits->u; a.v = hits->v;
+    if (rtk::clear_status(sc, s) != 0) return -1;
+    const int rc = rtk::launch_trace(bvh, sc, a, rtk::kModeClosest, isect == 1, false, false, s);
+    if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) return finish_sync(sc, s, fn);
+    return rc;
+}
+
 int cudaBVH_traceClosestCounted(CudaBVH_t bvh, int numRays, const CudaRayPacket* rays, const CudaHitPacket* hits,
                                 int* nodeVisits, int* triTests, unsigned flags) {
     if (!nodeVisits || !triTests) { set_error("cudaBVH_traceClosestCounted: NULL counters"); return -1; }

@@ -109,6 +109,9 @@ struct TraceScratch {
     // Bumped whenever a buffer above is freed or reallocated: a captured graph
     // whose kernels hold these addresses is valid only while gen is unchanged.
     uint64_t gen = 0;
+    // Borrowed fixed-size scratch (the no-alloc entry point): never grown or
+    // freed; a launch whose spill would not fit runs fewer workgroups per CU.
+    bool fixed = false;
     int launchInfo[kLaunchInfoInts] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 

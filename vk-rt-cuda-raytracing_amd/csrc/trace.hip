@@ -19,6 +19,9 @@
 #include "rt_internal.h"
 #include "trace_launch.h"
 
+#include <algorithm>
+#include <mutex>
+
 namespace rtk {
 
 using rtd::Ray;
@@ -859,14 +862,29 @@ template <class K>
 static int launch_kernel(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, K kern, int ldsd, bool perWave,
                          int blocksPerCU, rt::KernelInfo& ki, hipStream_t s) {
     rt::note_launch(kern, kBlock, ki, 0, 0, nullptr);
-    const int bpc = (blocksPerCU > 0 && blocksPerCU < ki.maxBlocks) ? blocksPerCU : ki.maxBlocks;
+    int bpc = (blocksPerCU > 0 && blocksPerCU < ki.maxBlocks) ? blocksPerCU : ki.maxBlocks;
+    a.maxStack = 3 * h->depth4 + 4;
+    const int spillDepth = a.maxStack > ldsd ? a.maxStack - ldsd : 0;
+    if (sc.fixed && spillDepth > 0) {
+        // a fixed scratch caps the resident workgroups so every lane's spill fits
+        const size_t perBlockRow = (size_t)spillDepth * kBlock * sizeof(int2) * (size_t)num_cus();
+        const size_t fit = sc.spillBytes / perBlockRow;
+        if (fit < 1) {
+            rt::set_error("BVH depth %d needs %zu bytes of stack spill for one workgroup per CU; the fixed scratch "
+                          "holds %zu", h->depth4, perBlockRow, sc.spillBytes);
+            return -1;
+        }
+        bpc = (int)std::min<size_t>((size_t)bpc, fit);
+    }
     const int grid = bpc * num_cus();
     rt::note_launch(kern, kBlock, ki, bpc, grid, sc.launchInfo);
     const int threads = grid * kBlock;
-    a.maxStack = 3 * h->depth4 + 4;
-    const int spillDepth = a.maxStack > ldsd ? a.maxStack - ldsd : 0;
     const size_t need = (size_t)spillDepth * threads * sizeof(int2);
-    if (grow_spill(sc, need) != 0) return -1;
+    if (sc.fixed) {
+        if (need > sc.spillBytes) { rt::set_error("fixed scratch too small (%zu < %zu)", sc.spillBytes, need); return -1; }
+    } else if (grow_spill(sc, need) != 0) {
+        return -1;
+    }
     if (a.W > 0) {
         a.tilesX = (a.W + 7) / 8;
         a.nTiles = a.tilesX * ((a.H + 7) / 8);
@@ -940,6 +958,10 @@ int take_status(rt::TraceScratch& sc, hipStream_t s, const char* fn) {
 
 int grow_spill(rt::TraceScratch& sc, size_t need) {
     if (need <= sc.spillBytes) return 0;
+    if (sc.fixed) {
+        rt::set_error("fixed traversal scratch too small: %zu bytes of spill needed, %zu held", need, sc.spillBytes);
+        return -1;
+    }
     hipFree(sc.spill);
     sc.spill = nullptr; sc.spillBytes = 0;
     sc.gen++;
@@ -1039,3 +1061,65 @@ int launch_gen_pinhole(const rtd::Cam& cam, int W, int H, const CudaRayPacket& p
 }
 
 }  // namespace rtk
+
+// ---------------------------------------------------------------------------
+// Static device scratch of the no-alloc entry point: one array per GPU,
+// zero-initialised at module load and held until the process ends. Layout:
+// [0, 1 KB) the kBands work-distribution counters (kBandStride words apart),
+// [1 KB, 1 KB + 64) the status word, [4 KB, end) the traversal stack spill.
+// Host code never uses the array's name as an address: its device address is
+// looked up once per device with hipGetSymbolAddress and kept per device.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[rtk::kNoAllocBytes];
+
+namespace rtk {
+namespace {
+constexpr size_t kNoAllocStatusOff = 1024;
+constexpr size_t kNoAllocSpillOff = 4096;
+static_assert(rtd::kBands * rtd::kBandStride * sizeof(uint32_t) <= kNoAllocStatusOff, "counters overlap the status");
+constexpr int kMaxDevices = 64;
+std::mutex g_noallocMu;
+void* g_noallocPtr[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_scratch(rt::TraceScratch& sc) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        rt::set_error("no-alloc scratch: no current device (or device %d beyond %d)", dev, kMaxDevices);
+        return -1;
+    }
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_noallocMu);
+        if (!g_noallocPtr[dev]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                rt::set_error("no-alloc scratch: hipGetSymbolAddress failed on device %d: %s", dev,
+                              hipGetErrorString(e));
+                return -1;
+            }
+            g_noallocPtr[dev] = p;
+        }
+        base = g_noallocPtr[dev];
+    }
+    uint8_t* b = static_cast<uint8_t*>(base);
+    sc.counter = reinterpret_cast<uint32_t*>(b);
+    sc.status = reinterpret_cast<int*>(b + kNoAllocStatusOff);
+    sc.spill = b + kNoAllocSpillOff;
+    sc.spillBytes = kNoAllocBytes - kNoAllocSpillOff;
+    sc.fixed = true;
+    return 0;
+}
+
+__global__ void k_zero_word(int* w) { *w = 0; }
+
+int clear_status(rt::TraceScratch& sc, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, s, sc.status);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { rt::set_error("status clear launch failed: %s", hipGetErrorString(e)); return -1; }
+    return 0;
+}
+
+}  // namespace rtk
+

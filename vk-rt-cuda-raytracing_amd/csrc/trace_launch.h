@@ -71,6 +71,16 @@ int grow_spill(rt::TraceScratch& sc, size_t need);
 // traversal stack overflowed, so hits may be missing). A set word is recorded
 // as the error of fn. Returns the word, or -1 if it could not be read.
 int take_status(rt::TraceScratch& sc, hipStream_t s, const char* fn);
+
+// Maximum bytes of the library's static device scratch of the no-alloc entry
+// point (per GPU, held until the process ends): band counters, status word
+// and traversal stack spill.
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+// Fills sc with views of the current device's static scratch (its device
+// address is looked up once per device); sc.fixed is set. 0 on success.
+int noalloc_scratch(rt::TraceScratch& sc);
+// Zeroes the status word of sc on stream s (a kernel, no memset).
+int clear_status(rt::TraceScratch& sc, hipStream_t s);
 int launch_gen_pinhole(const rtd::Cam& cam, int W, int H, const CudaRayPacket& p, hipStream_t s);
 // n x 9 floats gathered from device vertex / index buffers (build.hip)
 hipError_t launch_gather_tris(const void* verts, int numVertices, int stride, const void* idx, int indexBytes,
