@@ -73,6 +73,7 @@ def main():
     t0 = time.time()
     tris = scenes.terrain_scene(seed=args.seed)
     log(f"scene: {tris.shape[0]} triangles ({time.time() - t0:.1f} s)")
+    pkg.BVH.build(tris).destroy()          # warm-up: code objects and first allocations
     t0 = time.time()
     bvh = pkg.BVH.build(tris)
     build_s = time.time() - t0
@@ -274,8 +275,10 @@ def main():
         f"({host_calls['traceDiffuse']['mrays_s_reported']} reported)")
 
     cpu = None
+    build = build_report(bvh, tris, build_s)
     if not args.no_cpu:
         cpu = cpu_baseline(bvh, Pr, D, m, args.cpu_sample, gpu_hits=((pid, pt), (did, dt)))
+        build.update(cpu_build_baseline(bvh, tris))
 
     ic = tuning.get("incoherent_closest")
     prof_traffic = load_traffic(f"{ic['schedule']}/{ic['lds_depth']}/{ic['blocks_per_cu']}") if ic else None
@@ -325,7 +328,8 @@ def main():
         "cpu_baseline": cpu,
         "occupancy": occupancy,
         "host_pointer_calls": host_calls,
-        "build_s": round(build_s, 2),
+        "build_s": round(build_s, 3),
+        "build": build,
         "tuning": tuning,
     }
     bvh.set_stream(None)
@@ -597,6 +601,37 @@ def load_traffic(config):
         return None
     with open(p) as f:
         return json.load(f).get("incoherent_closest_bytes_per_launch", {}).get(config)
+
+
+def build_report(bvh, tris, build_s):
+    """The GPU build of the frame's BVH against its roofline: algorithmic bytes
+    per triangle 36 (read) + 36 (reordered write) + 36 per binning pass (one
+    per BVH2 level), plus 64 per BVH4 node, over the device build time."""
+    st = bvh.build_stats()
+    n = int(tris.shape[0])
+    alg = n * (36 + 36 + 36 * st["levels"]) + 64 * bvh.num_bvh4_nodes
+    dev_s = st["device_ms"] * 1e-3
+    gbs = alg / dev_s / 1e9
+    log(f"build: {build_s * 1e3:.1f} ms wall (upload {st['upload_ms']:.1f}, device {st['device_ms']:.1f}, "
+        f"handle {st['handle_ms']:.1f}), {st['levels']} levels, {gbs:.0f} GB/s algorithmic")
+    return {"wall_ms": round(build_s * 1e3, 2), "upload_ms": round(st["upload_ms"], 2),
+            "device_ms": round(st["device_ms"], 2), "handle_ms": round(st["handle_ms"], 2),
+            "levels": st["levels"], "bvh4_nodes": bvh.num_bvh4_nodes, "alg_bytes": int(alg),
+            "achieved_GBps": round(gbs, 1), "peak_GBps": HBM_PEAK_GBS, "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "bound": "launch/latency (one pass per BVH2 level)"}
+
+
+def cpu_build_baseline(bvh, tris):
+    """The oracle's host build of the same triangles (one pass, one thread):
+    its time, and whether its BVH4 nodes equal the GPU build's byte for byte."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    t0 = time.perf_counter()
+    ob = pyoracle.OracleBVH(tris)
+    cpu_s = time.perf_counter() - t0
+    same = bool(np.array_equal(ob.nodes(), bvh.node_data()))
+    log(f"cpu build (oracle, 1 thread): {cpu_s:.2f} s; GPU tree identical: {same}")
+    return {"cpu_s": round(cpu_s, 3), "cpu_kind": "port", "cpu_threads": 1, "tree_identical_to_cpu": same}
 
 
 def cpu_model() -> str:

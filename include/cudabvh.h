@@ -312,6 +312,12 @@ const void* cudaBVH_deviceNodes(CudaBVH_t bvh);
 const void* cudaBVH_deviceTris(CudaBVH_t bvh);
 /* Depth of the BVH4 (node levels on the longest root-to-leaf path). */
 int cudaBVH_getDepth(CudaBVH_t bvh);
+/* Where the build's time went (4 floats): host-to-device upload of the
+ * triangles (ms; 0 for device input), the device build including its level
+ * synchronisations (ms), completing the handle (bounds, scratch; ms), and the
+ * number of BVH2 levels (binning passes). Host exports are made on the first
+ * getter call and are not included. No backend counterpart. */
+int cudaBVH_getBuildStats(CudaBVH_t bvh, float* stats);
 
 /* Compressed 8-wide BVH (80-byte nodes: origin, per-axis power-of-two
  * scales, 8-bit child boxes rounded outward, 8 child slots) collapsed from the

@@ -152,6 +152,7 @@ SIGNATURES = {
     "cudaBVH_deviceNodes": (_P, [_H]),
     "cudaBVH_deviceTris": (_P, [_H]),
     "cudaBVH_getDepth": (_I, [_H]),
+    "cudaBVH_getBuildStats": (_I, [_H, _P]),
     "cudaBVH_buildCWBVH": (_I, [_H]),
     "cudaBVH_getCWBVH": (_I, [_H, _P, _P, _P]),
     "cudaBVH_sceneCreate": (_H, [_P, _I, _P, _I]),
@@ -367,6 +368,13 @@ class BVH:
     @property
     def depth(self) -> int:
         return self.lib.cudaBVH_getDepth(self.handle)
+
+    def build_stats(self) -> dict:
+        """upload_ms, device_ms, handle_ms of the build and its BVH2 levels."""
+        out = (C.c_float * 4)()
+        _check(self.lib.cudaBVH_getBuildStats(self.handle, out), "getBuildStats")
+        return {"upload_ms": float(out[0]), "device_ms": float(out[1]), "handle_ms": float(out[2]),
+                "levels": int(out[3])}
 
     def bounds(self):
         v = [C.c_float() for _ in range(4)]

@@ -39,6 +39,8 @@ static void free_handle_device(CudaBVHHandle* h) {
     h->d_sortWs = nullptr; h->sortWsBytes = 0;
     hipFree(h->d_cwNodes); hipFree(h->d_cwTris);
     h->d_cwNodes = h->d_cwTris = nullptr;
+    hipFree(h->d_nodes2); hipFree(h->d_ordered);
+    h->d_nodes2 = nullptr; h->d_ordered = nullptr;
     rtk::free_scratch(h->scratch);
     hipHostFree(h->h_pinned);
     h->h_pinned = nullptr; h->pinnedFloats = 0;
@@ -46,48 +48,107 @@ static void free_handle_device(CudaBVHHandle* h) {
     h->d_leafOfPrim = nullptr;
 }
 
-// Completes a handle whose device structure (d_nodes4, d_tris, d_leafOfPrim,
-// numNodes4, depth4) is set: the bounds, the host copies the reference's
-// getters hand out (nodes, SoA and packed triangles, the stackless DFS export
-// when nodes2 is given) from the n x 9 host triangles (original order) and
-// the leaf order, the traversal scratch and the default launch
-// configurations. Frees the handle and returns nullptr on failure.
-static CudaBVHHandle* make_handle(CudaBVHHandle* h, const float* tri9, int n, std::vector<uint8_t>&& hNodes4,
-                                  std::vector<int>&& ordered, std::vector<rt::Node2>* nodes2) {
-    h->numTris = n;
-    h->numOrdered = n;
+// Runs f(begin, end) over [0, n) split across up to 8 host threads.
+template <class F>
+static void host_parallel(size_t n, F&& f) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned nt = n < (size_t(1) << 16) ? 1u : std::max(1u, std::min(8u, hw));
+    const size_t chunk = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; t++)
+        th.emplace_back([&, t] { f(std::min(n, t * chunk), std::min(n, (t + 1) * chunk)); });
+    f(0, std::min(n, chunk));
+    for (auto& x : th) x.join();
+}
+
+// Bounds of n x 9 host floats as a sequential std::min / std::max pass
+// computes them (per-chunk passes combined in order give the same values,
+// signed zeros included), centre and diagonal extent.
+static void set_bounds(CudaBVHHandle* h, const float* tri9, int n) {
+    const unsigned nt = 8;
+    std::vector<float> cmn(3 * nt, 1e30f), cmx(3 * nt, -1e30f);
+    const size_t chunk = (size_t(n) + nt - 1) / nt;
+    std::vector<std::thread> th;
+    auto work = [&](unsigned t) {
+        const size_t b = std::min<size_t>(n, t * chunk), e = std::min<size_t>(n, (t + 1) * chunk);
+        float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};   // in registers
+        for (size_t i = b; i < e; i++)
+            for (int k = 0; k < 9; k++) {
+                const float x = tri9[9 * i + k];
+                mn[k % 3] = std::min(mn[k % 3], x);
+                mx[k % 3] = std::max(mx[k % 3], x);
+            }
+        for (int a = 0; a < 3; a++) { cmn[3 * t + a] = mn[a]; cmx[3 * t + a] = mx[a]; }
+    };
+    if (n < (1 << 16)) {
+        for (unsigned t = 0; t < nt; t++) work(t);
+    } else {
+        for (unsigned t = 1; t < nt; t++) th.emplace_back(work, t);
+        work(0);
+        for (auto& x : th) x.join();
+    }
     float bmin[3] = {1e30f, 1e30f, 1e30f}, bmax[3] = {-1e30f, -1e30f, -1e30f};
-    for (int i = 0; i < n; i++)
-        for (int k = 0; k < 9; k++) {
-            const float x = tri9[9 * size_t(i) + k];
-            bmin[k % 3] = std::min(bmin[k % 3], x);
-            bmax[k % 3] = std::max(bmax[k % 3], x);
-        }
+    for (unsigned t = 0; t < nt; t++)
+        for (int a = 0; a < 3; a++) { bmin[a] = std::min(bmin[a], cmn[3 * t + a]); bmax[a] = std::max(bmax[a], cmx[3 * t + a]); }
     for (int a = 0; a < 3; a++) { h->bboxMin[a] = bmin[a]; h->bboxMax[a] = bmax[a]; h->bboxCenter[a] = (bmin[a] + bmax[a]) * 0.5f; }
     const float ex = bmax[0] - bmin[0], ey = bmax[1] - bmin[1], ez = bmax[2] - bmin[2];
     h->bboxExtent = sqrtf(ex * ex + ey * ey + ez * ez);
-    h->h_nodes4 = std::move(hNodes4);
-    h->h_ordered = std::move(ordered);
-    // Host exports for the Vulkan-side consumers (backend :656-735)
+}
+
+// The host copies the reference's getters hand out (BVH4 nodes, SoA and
+// packed triangles, the stackless DFS export of a single tree) and the CWBVH
+// build reads, made once from the handle's device arrays. False on failure.
+static bool ensure_exports(CudaBVHHandle* h) {
+    if (h->exported) return true;
+    const int n = h->numOrdered;
+    std::vector<uint8_t> nodes4(size_t(h->numNodes4) * 64);
+    std::vector<int> ordered(n);
+    std::vector<float4> tris(size_t(n) * 3);
+    std::vector<rt::Node2> nodes2(h->d_nodes2 ? h->numNodes2 : 0);
+    hipStream_t s = hstream(h);
+    hipError_t e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipMemcpy(nodes4.data(), h->d_nodes4, nodes4.size(), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(ordered.data(), h->d_ordered, size_t(n) * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(tris.data(), h->d_tris, size_t(n) * 48, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && !nodes2.empty())
+        e = hipMemcpy(nodes2.data(), h->d_nodes2, nodes2.size() * sizeof(rt::Node2), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { set_error("copying the BVH to the host failed: %s", hipGetErrorString(e)); return false; }
     for (int a = 0; a < 9; a++) h->h_soa[a].resize(n);
     h->h_packed.assign(size_t(n) * 12, 0.f);
-    for (int i = 0; i < n; i++) {
-        const int p = h->h_ordered[i];
-        const float* src = tri9 + 9 * size_t(p);
-        float* pk = &h->h_packed[12 * size_t(i)];
-        for (int a = 0; a < 9; a++) { h->h_soa[a][i] = src[a]; pk[a] = src[a]; }
-        // the original ID's int bits, which is how the layer and the
-        // RT-IR executor read this word (VkLayer_CudaRT.cpp:4410,
-        // layer/rt_ir_exec.cu:167); the reference writes float(ID)
-        // there (backend :730-733), which those readers misinterpret
-        const int32_t id = p;
-        memcpy(&pk[9], &id, sizeof id);
+    host_parallel(size_t(n), [&](size_t b, size_t en) {
+        for (size_t i = b; i < en; i++) {
+            const float4* t = &tris[3 * i];
+            const float v[9] = {t[0].x, t[0].y, t[0].z, t[1].x, t[1].y, t[1].z, t[2].x, t[2].y, t[2].z};
+            float* pk = &h->h_packed[12 * i];
+            for (int a = 0; a < 9; a++) { h->h_soa[a][i] = v[a]; pk[a] = v[a]; }
+            // the original ID's int bits, which is how the layer and the
+            // RT-IR executor read this word (VkLayer_CudaRT.cpp:4410,
+            // layer/rt_ir_exec.cu:167); the reference writes float(ID)
+            // there (backend :730-733), which those readers misinterpret
+            const int32_t id = ordered[i];
+            memcpy(&pk[9], &id, sizeof id);
+        }
+    });
+    h->h_nodes4 = std::move(nodes4);
+    h->h_ordered = std::move(ordered);
+    if (!nodes2.empty()) {
+        rt::export_dfs(nodes2, h->h_dfs);
+        h->numDFS = int(nodes2.size());
+        h->h_nodes2 = std::move(nodes2);
     }
-    if (nodes2) {
-        rt::export_dfs(*nodes2, h->h_dfs);
-        h->numDFS = int(nodes2->size());
-        h->h_nodes2 = std::move(*nodes2);
-    }
+    h->exported = true;
+    return true;
+}
+
+// Completes a handle whose device structure (d_nodes4, d_tris, d_leafOfPrim,
+// d_ordered, d_nodes2 for a single tree, numNodes4, depth4) is set: the
+// bounds from the n x 9 host triangles (original order), the traversal
+// scratch and the default launch configurations; the host exports follow on
+// first use. Frees the handle and returns nullptr on failure.
+static CudaBVHHandle* make_handle(CudaBVHHandle* h, const float* tri9, int n) {
+    h->numTris = n;
+    h->numOrdered = n;
+    set_bounds(h, tri9, n);
     if (rtk::alloc_scratch(h->scratch) != 0) {
         free_handle_device(h);
         delete h;
@@ -102,7 +163,7 @@ static CudaBVHHandle* make_handle(CudaBVHHandle* h, const float* tri9, int n, st
     h->tune[rt::kClassAny].ldsDepth = 12;
     if (rt::verbose())
         fprintf(stderr, "[CudaBVH] built %d tris: %d BVH4 nodes (depth %d), %d BVH2 nodes\n", n, h->numNodes4,
-                h->depth4, h->numDFS);
+                h->depth4, h->numNodes2);
     return h;
 }
 
@@ -111,7 +172,8 @@ static CudaBVHHandle* make_handle(CudaBVHHandle* h, const float* tri9, int n, st
 // handle is a forest of numScenes trees (roots4: each tree's BVH4 root).
 static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf, const int* sceneOff = nullptr,
                                  int numScenes = 1, std::vector<int>* roots4 = nullptr,
-                                 const float* d_tri9 = nullptr, std::vector<rt::Node2>* forestNodes2 = nullptr) {
+                                 const float* d_tri9 = nullptr, std::vector<rt::Node2>* forestNodes2 = nullptr,
+                                 std::vector<int>* forestOrdered = nullptr) {
     rt::clear_error();
     if (!tri9 || n <= 0) { set_error("cudaBVH_build: need tris != NULL and numTris > 0 (got %d)", n); return nullptr; }
     if (n > (1 << 27)) { set_error("cudaBVH_build: %d triangles exceed the leaf encoding (2^27)", n); return nullptr; }
@@ -120,15 +182,17 @@ static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf, const in
     float* d_in = const_cast<float*>(d_tri9);
     float* d_own = nullptr;
     rt::GpuBVH g;
-    std::vector<uint8_t> hNodes4;
-    std::vector<int> ordered(n);
-    std::vector<rt::Node2> nodes2;
-    hipError_t e;
+    const auto clk = [] { return std::chrono::duration<double, std::milli>(
+                                     std::chrono::steady_clock::now().time_since_epoch()).count(); };
+    const double t0 = clk();
+    double t1 = t0;
     if (!d_in) {
         HIP_TRY(hipMalloc((void**)&d_own, size_t(n) * 36), goto fail);
         HIP_TRY(hipMemcpyAsync(d_own, tri9, size_t(n) * 36, hipMemcpyHostToDevice, s), hipFree(d_own); goto fail);
         d_in = d_own;
     }
+    hipStreamSynchronize(s);          // the build syncs anyway; this separates the upload time
+    t1 = clk();
     if (rt::gpu_build(d_in, n, maxLeaf, s, g, sceneOff, numScenes, roots4) != 0) {
         hipFree(d_own); hipFree(g.nodes2); hipFree(g.nodes4); hipFree(g.tris); hipFree(g.leafOf); hipFree(g.ordered);
         goto fail;
@@ -137,17 +201,39 @@ static CudaBVHHandle* build_impl(const float* tri9, int n, int maxLeaf, const in
     h->d_nodes4 = g.nodes4;
     h->d_tris = g.tris;
     h->d_leafOfPrim = g.leafOf;
+    h->d_ordered = g.ordered;
     h->numNodes4 = g.numNodes4;
     h->depth4 = g.depth4;
-    hNodes4.resize(size_t(g.numNodes4) * 64);
-    nodes2.resize(g.numNodes2);
-    e = hipMemcpy(hNodes4.data(), g.nodes4, hNodes4.size(), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(ordered.data(), g.ordered, size_t(n) * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(nodes2.data(), g.nodes2, nodes2.size() * sizeof(rt::Node2), hipMemcpyDeviceToHost);
-    hipFree(g.nodes2); hipFree(g.ordered);
-    if (e != hipSuccess) { set_error("copying the built BVH back failed: %s", hipGetErrorString(e)); goto fail; }
-    if (numScenes > 1 && forestNodes2) *forestNodes2 = nodes2;
-    return make_handle(h, tri9, n, std::move(hNodes4), std::move(ordered), numScenes == 1 ? &nodes2 : nullptr);
+    h->numNodes2 = g.numNodes2;
+    if (numScenes == 1) {
+        h->d_nodes2 = g.nodes2;
+    } else {
+        // a forest: its trees' BVH2 nodes and leaf order go to the caller
+        hipError_t e = hipSuccess;
+        if (forestNodes2) {
+            forestNodes2->resize(g.numNodes2);
+            e = hipMemcpy(forestNodes2->data(), g.nodes2, size_t(g.numNodes2) * sizeof(rt::Node2), hipMemcpyDeviceToHost);
+        }
+        if (e == hipSuccess && forestOrdered) {
+            forestOrdered->resize(n);
+            e = hipMemcpy(forestOrdered->data(), g.ordered, size_t(n) * 4, hipMemcpyDeviceToHost);
+        }
+        hipFree(g.nodes2);
+        if (e != hipSuccess) { set_error("copying the forest back failed: %s", hipGetErrorString(e)); goto fail; }
+    }
+    {
+        const double t2 = clk();
+        CudaBVHHandle* r = make_handle(h, tri9, n);
+        const double t3 = clk();
+        if (r) {
+            r->buildStats[0] = float(t1 - t0); r->buildStats[1] = float(t2 - t1);
+            r->buildStats[2] = float(t3 - t2); r->buildStats[3] = float(g.levels);
+        }
+        if (rt::verbose())
+            fprintf(stderr, "[CudaBVH] build of %d tris: upload %.2f ms, device build %.2f ms, handle %.2f ms\n", n,
+                    t1 - t0, t2 - t1, t3 - t2);
+        return r;
+    }
 fail:
     free_handle_device(h);
     delete h;
@@ -406,9 +492,9 @@ int cudaBVH_buildManyDevice(const CudaGeometry* geoms, const int* geomOffsets, i
         const int R = (int)big.size();
         std::vector<int> off(R + 1, 0);
         for (int i = 0; i < R; i++) off[i + 1] = off[i] + (int)count[big[i]];
-        std::vector<int> roots4;
+        std::vector<int> roots4, ford;
         std::vector<rt::Node2> fn2;
-        CudaBVHHandle* forest = build_impl(host.data(), off[R], 4, off.data(), R, &roots4, dTri, &fn2);
+        CudaBVHHandle* forest = build_impl(host.data(), off[R], 4, off.data(), R, &roots4, dTri, &fn2, &ford);
         if (!forest) return fail_all();
         // BVH2 subtree sizes: the trees lie one after another in pre-order
         std::vector<int> sz(fn2.size(), 1);
@@ -422,7 +508,20 @@ int cudaBVH_buildManyDevice(const CudaGeometry* geoms, const int* geomOffsets, i
             auto* h = new CudaBVHHandle();
             h->numNodes4 = end4 - first4;
             std::vector<uint8_t> hn4(size_t(h->numNodes4) * 64);
+            // the tree's BVH2 nodes and leaf order, rebased like its BVH4 nodes
+            std::vector<rt::Node2> sub(fn2.begin() + root2, fn2.begin() + root2 + sz[root2]);
+            for (rt::Node2& nd : sub) {
+                if (nd.triCount == 0) { nd.left -= root2; nd.right -= root2; }
+                else nd.triStart -= t0;
+            }
+            std::vector<int> ord(ford.begin() + t0, ford.begin() + t0 + nt);
+            for (int& x : ord) x -= t0;
+            h->numNodes2 = int(sub.size());
             const bool ok = hipMalloc(&h->d_nodes4, hn4.size()) == hipSuccess &&
+                            hipMalloc(&h->d_nodes2, sub.size() * sizeof(rt::Node2)) == hipSuccess &&
+                            hipMalloc((void**)&h->d_ordered, size_t(nt) * 4) == hipSuccess &&
+                            hipMemcpy(h->d_nodes2, sub.data(), sub.size() * sizeof(rt::Node2), hipMemcpyHostToDevice) == hipSuccess &&
+                            hipMemcpy(h->d_ordered, ord.data(), size_t(nt) * 4, hipMemcpyHostToDevice) == hipSuccess &&
                             hipMalloc(&h->d_tris, size_t(nt) * 48) == hipSuccess &&
                             hipMalloc((void**)&h->d_leafOfPrim, size_t(nt) * 4) == hipSuccess &&
                             rtk::launch_slice_forest(forest->d_nodes4, first4, h->numNodes4, forest->d_tris,
@@ -437,16 +536,8 @@ int cudaBVH_buildManyDevice(const CudaGeometry* geoms, const int* geomOffsets, i
                 return fail_all();
             }
             h->depth4 = bvh4_depth(hn4);
-            std::vector<int> ord(forest->h_ordered.begin() + t0, forest->h_ordered.begin() + t0 + nt);
-            for (int& x : ord) x -= t0;
-            // the tree's BVH2 nodes, rebased like its BVH4 nodes
-            std::vector<rt::Node2> sub(fn2.begin() + root2, fn2.begin() + root2 + sz[root2]);
-            for (rt::Node2& nd : sub) {
-                if (nd.triCount == 0) { nd.left -= root2; nd.right -= root2; }
-                else nd.triStart -= t0;
-            }
             root2 += sz[root2];
-            out[k] = make_handle(h, host.data() + 9 * size_t(t0), nt, std::move(hn4), std::move(ord), &sub);
+            out[k] = make_handle(h, host.data() + 9 * size_t(t0), nt);
             if (!out[k]) { cudaBVH_destroy(forest); return fail_all(); }
         }
         cudaBVH_destroy(forest);
@@ -779,23 +870,33 @@ int cudaBVH_getCWBVH(CudaBVH_t bvh, const uint32_t** nodes, const int** triOrder
 }
 int cudaBVH_getDepth(CudaBVH_t bvh) { return bvh ? bvh->depth4 : 0; }
 
-const void* cudaBVH_getNodeData(CudaBVH_t bvh) { return bvh ? bvh->h_nodes4.data() : nullptr; }
+int cudaBVH_getBuildStats(CudaBVH_t bvh, float* stats) {
+    rt::clear_error();
+    if (!bvh || !stats) { set_error("cudaBVH_getBuildStats: NULL argument"); return -1; }
+    memcpy(stats, bvh->buildStats, sizeof(bvh->buildStats));
+    return 0;
+}
+
+const void* cudaBVH_getNodeData(CudaBVH_t bvh) {
+    return (bvh && ensure_exports(bvh)) ? bvh->h_nodes4.data() : nullptr;
+}
 
 void cudaBVH_getTriData(CudaBVH_t bvh, const float* outPtrs[9]) {
     if (!outPtrs) return;
-    for (int i = 0; i < 9; i++) outPtrs[i] = bvh ? bvh->h_soa[i].data() : nullptr;
+    const bool ok = bvh && ensure_exports(bvh);
+    for (int i = 0; i < 9; i++) outPtrs[i] = ok ? bvh->h_soa[i].data() : nullptr;
 }
 
 int cudaBVH_getStacklessBVH2(CudaBVH_t bvh, uint32_t** outData) {
     if (!outData) return 0;
-    if (!bvh || bvh->h_dfs.empty()) { *outData = nullptr; return 0; }
+    if (!bvh || !ensure_exports(bvh) || bvh->h_dfs.empty()) { *outData = nullptr; return 0; }
     *outData = bvh->h_dfs.data();
     return bvh->numDFS;
 }
 
 int cudaBVH_getPackedTris(CudaBVH_t bvh, float** outData) {
     if (!outData) return 0;
-    if (!bvh || bvh->h_packed.empty()) { *outData = nullptr; return 0; }
+    if (!bvh || !ensure_exports(bvh) || bvh->h_packed.empty()) { *outData = nullptr; return 0; }
     *outData = bvh->h_packed.data();
     return int(bvh->h_packed.size() / 4);
 }
@@ -882,6 +983,7 @@ static bool packet_ok(const CudaRayPacket* r) {
 // Builds (once) the compressed 8-wide BVH and its triangle order from the BVH2.
 static int ensure_cwbvh(CudaBVH_t bvh) {
     if (bvh->d_cwNodes) return 0;
+    if (!ensure_exports(bvh)) return -1;
     if (bvh->h_nodes2.empty()) { set_error("CWBVH: this handle keeps no BVH2 (batched forest)"); return -1; }
     const int nn = rt::build_cwbvh(bvh->h_nodes2, bvh->h_cw, bvh->h_cwTriOrder, bvh->cwDepth);
     if (nn <= 0) { set_error("CWBVH build failed"); return -1; }

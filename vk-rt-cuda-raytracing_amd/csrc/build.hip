@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "rt_internal.h"
@@ -85,7 +86,15 @@ struct Build {
     int* sAxis;             // [m] split axis (-1: middle split)
     int* sBin;
     int* sNl;               // left count
+    int* small;             // [cap] node whose level was done by k_small_level (bounds written there)
+    int* bigSlot;           // [cap] bins slot of a node of more than kSmall primitives
+    int* bigCount;          // bins slots handed out this level
 };
+
+// Nodes of at most kSmall primitives take one level per launch inside one
+// wave (k_small_level): LDS bins, SAH, stable partition and child bounds
+// without global atomics. Larger nodes use the device-wide passes.
+constexpr int kSmall = 2048;
 
 // ---------------------------------------------------------------------------
 // Scene of primitive i: the last s with off[s] <= i (off: numScenes+1 ascending offsets).
@@ -126,9 +135,10 @@ __global__ void k_init_roots(Build b, const int* off, const int* slots, int numS
     if (slots[s] >= 0) b.act[slots[s]] = s;
 }
 
-__global__ void k_init_nodes(int* nbox, int* ncen, int first, int count) {
+__global__ void k_init_nodes(int* nbox, int* ncen, int* small, int first, int count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
+    small[first + i] = 0;
     const size_t o = 6 * (size_t)(first + i);
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -140,50 +150,65 @@ __global__ void k_init_nodes(int* nbox, int* ncen, int first, int count) {
 // Box and centroid bounds of every node at or above levelBase, reduced over
 // its primitives (LDS first when the chunk lies in one node).
 __global__ __launch_bounds__(kThreads) void k_node_bounds(Build b, int levelBase) {
-    __shared__ int lb[12];
-    __shared__ int uni;
+    // The nodes bounded here (children of nodes of more than kSmall
+    // primitives) are at most four per chunk: at most two such parents
+    // overlap kChunk consecutive positions. Each gets an LDS accumulator,
+    // claimed by node id; global atomics only if a fifth ever showed up.
+    __shared__ int key[4];
+    __shared__ int lb[4][12];
     const int c0 = blockIdx.x * kChunk, c1 = min(c0 + kChunk, b.n);
-    if (threadIdx.x == 0) {
-        const int a = b.pnode[c0], z = b.pnode[c1 - 1];
-        uni = (a == z && a >= levelBase) ? a : -1;
-    }
-    if (threadIdx.x < 12) lb[threadIdx.x] = (threadIdx.x % 6) < 3 ? kOrdPosInf : kOrdNegInf;
+    if (threadIdx.x < 4) key[threadIdx.x] = -1;
+    if (threadIdx.x < 48) lb[threadIdx.x / 12][threadIdx.x % 12] = (threadIdx.x % 6) < 3 ? kOrdPosInf : kOrdNegInf;
     __syncthreads();
-    const int u = uni;
     for (int i = c0 + threadIdx.x; i < c1; i += kThreads) {
         const int node = b.pnode[i];
-        if (node < levelBase) continue;
+        if (node < levelBase || b.small[node]) continue;
+        int j = 0;
+        for (; j < 4; j++) {
+            const int k = key[j];
+            if (k == node) break;
+            if (k == -1) {
+                const int prev = atomicCAS(&key[j], -1, node);
+                if (prev == -1 || prev == node) break;
+            }
+        }
         const int p = b.idx[i];
         const float* pb = b.pbox + 6 * (size_t)p;
         const float4 c = b.cen[p];
         const float cv[3] = {c.x, c.y, c.z};
-        if (u >= 0) {
+        int* nb = j < 4 ? &lb[j][0] : b.nbox + 6 * (size_t)node;
+        int* nc = j < 4 ? &lb[j][6] : b.ncen + 6 * (size_t)node;
 #pragma unroll
-            for (int a = 0; a < 3; a++) {
-                atomicMin(&lb[a], f2o(pb[a])); atomicMax(&lb[3 + a], f2o(pb[3 + a]));
-                atomicMin(&lb[6 + a], f2o(cv[a])); atomicMax(&lb[9 + a], f2o(cv[a]));
-            }
-        } else {
-            int* nb = b.nbox + 6 * (size_t)node;
-            int* nc = b.ncen + 6 * (size_t)node;
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                atomicMin(&nb[a], f2o(pb[a])); atomicMax(&nb[3 + a], f2o(pb[3 + a]));
-                atomicMin(&nc[a], f2o(cv[a])); atomicMax(&nc[3 + a], f2o(cv[a]));
-            }
+        for (int a = 0; a < 3; a++) {
+            atomicMin(&nb[a], f2o(pb[a])); atomicMax(&nb[3 + a], f2o(pb[3 + a]));
+            atomicMin(&nc[a], f2o(cv[a])); atomicMax(&nc[3 + a], f2o(cv[a]));
         }
     }
     __syncthreads();
-    if (u >= 0 && threadIdx.x < 12) {
-        const int k = threadIdx.x;
-        int* dst = (k < 6 ? b.nbox : b.ncen) + 6 * (size_t)u + (k % 6);
-        if ((k % 6) < 3) atomicMin(dst, lb[k]); else atomicMax(dst, lb[k]);
+    if (threadIdx.x < 48) {
+        const int j = threadIdx.x / 12, k = threadIdx.x % 12, u = key[j];
+        if (u >= 0) {
+            int* dst = (k < 6 ? b.nbox : b.ncen) + 6 * (size_t)u + (k % 6);
+            if ((k % 6) < 3) atomicMin(dst, lb[j][k]); else atomicMax(dst, lb[j][k]);
+        }
     }
 }
 
-__global__ void k_clear_bins(int* bins, int m) {
+// Bins slots of the level's nodes of more than kSmall primitives (the others
+// bin in LDS): at most n / (kSmall + 1) + numScenes of them; which node gets
+// which slot does not matter.
+__global__ void k_big_slots(Build b, int m) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= m) return;
+    const int node = b.act[slot];
+    if (b.ncnt[node] > kSmall) b.bigSlot[node] = atomicAdd(b.bigCount, 1);
+}
+
+__global__ void k_zero_int(int* w) { *w = 0; }
+
+__global__ void k_clear_bins(int* bins, int slots) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m * kSlotInts) return;
+    if (i >= slots * kSlotInts) return;
     const int f = i % kBinInts;
     bins[i] = f < 3 ? kOrdPosInf : (f < 6 ? kOrdNegInf : 0);
 }
@@ -208,50 +233,52 @@ __device__ __forceinline__ void bin_prim(const Build& b, int node, int p, int* d
 }
 
 __global__ __launch_bounds__(kThreads) void k_bin(Build b) {
-    __shared__ int lb[kSlotInts];
-    __shared__ int uni;
+    // A node of more than kSmall (>= kChunk) primitives that overlaps this
+    // chunk holds its first or its last position, so the chunk bins into at
+    // most two LDS copies (those two nodes) and flushes them once.
+    __shared__ int lb[2][kSlotInts];
+    __shared__ int nodes2[2];
     const int c0 = blockIdx.x * kChunk, c1 = min(c0 + kChunk, b.n);
     if (threadIdx.x == 0) {
         const int a = b.pnode[c0], z = b.pnode[c1 - 1];
-        uni = (a == z && b.ncnt[a] > b.maxLeaf) ? a : -1;
+        nodes2[0] = b.ncnt[a] > kSmall ? a : -1;
+        nodes2[1] = (z != a && b.ncnt[z] > kSmall) ? z : -1;
     }
-    for (int i = threadIdx.x; i < kSlotInts; i += kThreads) {
-        const int f = i % kBinInts;
-        lb[i] = f < 3 ? kOrdPosInf : (f < 6 ? kOrdNegInf : 0);
+    for (int i = threadIdx.x; i < 2 * kSlotInts; i += kThreads) {
+        const int f = (i % kSlotInts) % kBinInts;
+        lb[i / kSlotInts][i % kSlotInts] = f < 3 ? kOrdPosInf : (f < 6 ? kOrdNegInf : 0);
     }
     __syncthreads();
-    const int u = uni;
+    const int na = nodes2[0], nz = nodes2[1];
+    if (na < 0 && nz < 0) return;
     for (int i = c0 + threadIdx.x; i < c1; i += kThreads) {
         const int node = b.pnode[i];
-        if (b.ncnt[node] <= b.maxLeaf) continue;
-        if (u >= 0) bin_prim(b, node, b.idx[i], lb);
-        else bin_prim(b, node, b.idx[i], b.bins + (size_t)b.nodeSlot[node] * kSlotInts);
+        if (node == na) bin_prim(b, node, b.idx[i], lb[0]);
+        else if (node == nz) bin_prim(b, node, b.idx[i], lb[1]);
     }
     __syncthreads();
-    if (u >= 0) {
-        int* g = b.bins + (size_t)b.nodeSlot[u] * kSlotInts;
+    for (int q = 0; q < 2; q++) {
+        const int u = nodes2[q];
+        if (u < 0) continue;
+        int* g = b.bins + (size_t)b.bigSlot[u] * kSlotInts;
         for (int i = threadIdx.x; i < kSlotInts; i += kThreads) {
             const int f = i % kBinInts;
-            if (f == 6) { if (lb[i]) atomicAdd(&g[i], lb[i]); }
-            else if (f < 3) { if (lb[i] != kOrdPosInf) atomicMin(&g[i], lb[i]); }
-            else if (lb[i] != kOrdNegInf) atomicMax(&g[i], lb[i]);
+            const int v = lb[q][i];
+            if (f == 6) { if (v) atomicAdd(&g[i], v); }
+            else if (f < 3) { if (v != kOrdPosInf) atomicMin(&g[i], v); }
+            else if (v != kOrdNegInf) atomicMax(&g[i], v);
         }
     }
 }
 
-// One thread per active node: SAH over 3 axes x 15 boundaries in the host
-// order (axis-major, strict <), then the two children.
-__global__ void k_sah(Build b, int m, int childBase) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= m) return;
-    const int node = b.act[slot];
-    const int* nb = b.nbox + 6 * (size_t)node;
-    const int* nc = b.ncen + 6 * (size_t)node;
+// SAH over 3 axes x 15 boundaries in the host order (axis-major, strict <)
+// from the node's bins ([3][16][7], global or LDS): split axis (-1: middle
+// split), last left bin and left count.
+__device__ __forceinline__ void sah_choose(const int* bins, const int* nb, const int* nc, int cnt, int& bestAxis,
+                                           int& bestBin, int& bestNl) {
     const float pA = area(o2f(nb[0]), o2f(nb[1]), o2f(nb[2]), o2f(nb[3]), o2f(nb[4]), o2f(nb[5]));
-    const int cnt = b.ncnt[node];
     float bestCost = 1e30f;
-    int bestAxis = -1, bestBin = -1, bestNl = 0;
-    const int* bins = b.bins + (size_t)slot * kSlotInts;
+    bestAxis = -1; bestBin = -1; bestNl = 0;
     for (int ax = 0; ax < 3; ax++) {
         const float cmin = o2f(nc[ax]), cmax = o2f(nc[3 + ax]);
         if (cmax - cmin < 1e-8f) continue;
@@ -286,15 +313,113 @@ __global__ void k_sah(Build b, int m, int childBase) {
         }
     }
     if (bestAxis < 0) bestNl = cnt / 2;                      // coincident centroids: middle split
-    b.sAxis[slot] = bestAxis;
-    b.sBin[slot] = bestBin;
-    b.sNl[slot] = bestNl;
+}
+
+// Children of a split node: ids childBase + 2 slot (+1), ranges, depth.
+__device__ __forceinline__ void make_children(const Build& b, int node, int slot, int childBase, int nl) {
     const int L = childBase + 2 * slot, R = L + 1;
-    const int s = b.ns[node], d = b.ndepth[node] + 1;
+    const int s = b.ns[node], d = b.ndepth[node] + 1, cnt = b.ncnt[node];
     b.nleft[node] = L; b.nright[node] = R;
-    b.ns[L] = s; b.ncnt[L] = bestNl; b.ns[R] = s + bestNl; b.ncnt[R] = cnt - bestNl;
+    b.ns[L] = s; b.ncnt[L] = nl; b.ns[R] = s + nl; b.ncnt[R] = cnt - nl;
     b.ndepth[L] = d; b.ndepth[R] = d;
     b.nleft[L] = b.nright[L] = b.nleft[R] = b.nright[R] = -1;
+}
+
+// One thread per active node of more than kSmall primitives: SAH from the
+// device-wide bins, then the two children.
+__global__ void k_sah(Build b, int m, int childBase) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= m) return;
+    const int node = b.act[slot];
+    const int cnt = b.ncnt[node];
+    if (cnt <= kSmall) return;
+    int ax, bin, nl;
+    sah_choose(b.bins + (size_t)b.bigSlot[node] * kSlotInts, b.nbox + 6 * (size_t)node, b.ncen + 6 * (size_t)node, cnt, ax,
+               bin, nl);
+    b.sAxis[slot] = ax;
+    b.sBin[slot] = bin;
+    b.sNl[slot] = nl;
+    make_children(b, node, slot, childBase, nl);
+}
+
+// One level of every active node of at most kSmall primitives, one wave per
+// node: bins in LDS, the SAH choice, the two children, the stable partition
+// of its primitives into idx2 / pnode2 (ballot ranks, left ones first) and
+// the children's box and centroid bounds, written directly (small[child] = 1
+// tells k_node_bounds to leave them alone). Same results as the device-wide
+// passes, which skip these nodes.
+__global__ __launch_bounds__(64) void k_small_level(Build b, int m, int childBase) {
+    __shared__ int lb[kSlotInts];
+    __shared__ int cb[24];                   // left box, left centroids, right box, right centroids
+    __shared__ int choice[3];
+    const int slot = blockIdx.x;
+    if (slot >= m) return;
+    const int node = b.act[slot];
+    const int cnt = b.ncnt[node];
+    if (cnt > kSmall) return;                // uniform per workgroup
+    const int lane = threadIdx.x;
+    for (int i = lane; i < kSlotInts; i += 64) {
+        const int f = i % kBinInts;
+        lb[i] = f < 3 ? kOrdPosInf : (f < 6 ? kOrdNegInf : 0);
+    }
+    if (lane < 24) cb[lane] = (lane % 6) < 3 ? kOrdPosInf : kOrdNegInf;
+    __syncthreads();
+    const int s = b.ns[node];
+    for (int i = lane; i < cnt; i += 64) bin_prim(b, node, b.idx[s + i], lb);
+    __syncthreads();
+    if (lane == 0) {
+        int ax, bin, nl;
+        sah_choose(lb, b.nbox + 6 * (size_t)node, b.ncen + 6 * (size_t)node, cnt, ax, bin, nl);
+        choice[0] = ax; choice[1] = bin; choice[2] = nl;
+        b.sAxis[slot] = ax; b.sBin[slot] = bin; b.sNl[slot] = nl;
+        make_children(b, node, slot, childBase, nl);
+    }
+    __syncthreads();
+    const int ax = choice[0], bin = choice[1], nl = choice[2];
+    const int L = childBase + 2 * slot, R = L + 1;
+    const int* nc = b.ncen + 6 * (size_t)node;
+    const float cmin = ax >= 0 ? o2f(nc[ax]) : 0.f, cmax = ax >= 0 ? o2f(nc[3 + ax]) : 0.f;
+    int leftBefore = 0;
+    for (int base = 0; base < cnt; base += 64) {
+        const int i = base + lane;
+        const bool valid = i < cnt;
+        int p = 0;
+        bool left = false;
+        if (valid) {
+            p = b.idx[s + i];
+            if (ax < 0) {
+                left = i < nl;
+            } else {
+                const float4 c = b.cen[p];
+                const float cv = ax == 0 ? c.x : (ax == 1 ? c.y : c.z);
+                left = bin_of(cv, cmin, cmax) <= bin;
+            }
+        }
+        const uint64_t lm = __ballot(valid && left);
+        const int lr = __popcll(lm & ((1ull << lane) - 1ull));
+        if (valid) {
+            const int pos = left ? s + leftBefore + lr : s + nl + (i - leftBefore - lr);
+            b.idx2[pos] = p;
+            b.pnode2[pos] = left ? L : R;
+            const float* pb = b.pbox + 6 * (size_t)p;
+            const float4 c = b.cen[p];
+            const float cv[3] = {c.x, c.y, c.z};
+            int* d = cb + (left ? 0 : 12);
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                atomicMin(&d[a], f2o(pb[a])); atomicMax(&d[3 + a], f2o(pb[3 + a]));
+                atomicMin(&d[6 + a], f2o(cv[a])); atomicMax(&d[9 + a], f2o(cv[a]));
+            }
+        }
+        leftBefore += __popcll(lm);
+    }
+    __syncthreads();
+    if (lane < 24) {
+        const int child = lane < 12 ? L : R, k = lane % 12;
+        int* dst = (k < 6 ? b.nbox : b.ncen) + 6 * (size_t)child + (k % 6);
+        *dst = cb[lane];
+    }
+    if (lane == 0) { b.small[L] = 1; b.small[R] = 1; }
 }
 
 __global__ void k_flags(Build b) {
@@ -302,7 +427,7 @@ __global__ void k_flags(Build b) {
     if (i >= b.n) return;
     const int node = b.pnode[i];
     int f = 0;
-    if (b.ncnt[node] > b.maxLeaf) {
+    if (b.ncnt[node] > b.maxLeaf && b.ncnt[node] > kSmall) {
         const int slot = b.nodeSlot[node];
         const int ax = b.sAxis[slot];
         if (ax < 0) {
@@ -326,6 +451,7 @@ __global__ void k_scatter(Build b) {
         b.pnode2[i] = node;
         return;
     }
+    if (b.ncnt[node] <= kSmall) return;          // partitioned by k_small_level
     const int slot = b.nodeSlot[node];
     const int s = b.ns[node], nl = b.sNl[slot];
     const int lr = b.rank[i] - b.rank[s];
@@ -711,6 +837,10 @@ void scan_exclusive(const int* in, int* out, int n, int* total, int* tiles, hipS
 
 int scan_tiles_needed(int n) { return (n + rtb::kScanTile - 1) / rtb::kScanTile + 1; }
 
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 struct DevBuf {
     std::vector<void*> ptrs;
     template <class T> T* get(size_t count) {
@@ -734,6 +864,11 @@ struct DevBuf {
 int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& out, const int* sceneOff,
               int numScenes, std::vector<int>* roots4) {
     using namespace rtb;
+    const double tBegin = verbose() ? now_ms() : 0.0;
+    struct Report {
+        double t0;
+        ~Report() { if (verbose()) fprintf(stderr, "[CudaBVH] gpu_build: %.2f ms incl. workspace free\n", now_ms() - t0); }
+    } report{tBegin};
     DevBuf tmp;
     const int R = numScenes;
     std::vector<int> hOff = sceneOff ? std::vector<int>(sceneOff, sceneOff + R + 1) : std::vector<int>{0, n};
@@ -768,12 +903,18 @@ int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& ou
     b.ns = tmp.get<int>(cap); b.ncnt = tmp.get<int>(cap);
     b.nleft = tmp.get<int>(cap); b.nright = tmp.get<int>(cap); b.ndepth = tmp.get<int>(cap);
     b.nodeSlot = tmp.get<int>(cap);
+    b.small = tmp.get<int>(cap);
+    b.bigSlot = tmp.get<int>(cap);
+    b.bigCount = tmp.get<int>(1);
+    const int maxBig = n / (kSmall + 1) + R + 1;
+    b.bins = tmp.get<int>((size_t)maxBig * kSlotInts);
     b.act = tmp.get<int>(n); int* act2 = tmp.get<int>(n);
     b.sAxis = tmp.get<int>(n); b.sBin = tmp.get<int>(n); b.sNl = tmp.get<int>(n);
     int* tiles = tmp.get<int>(scan_tiles_needed((int)cap) + 8);
     int* dcount = tmp.get<int>(4);
     BUILD_CHECK(pbox && cen && b.idx && b.pnode && b.idx2 && b.pnode2 && b.flag && b.rank && b.nbox && b.ncen &&
-                b.ns && b.ncnt && b.nleft && b.nright && b.ndepth && b.nodeSlot && b.act && act2 && b.sAxis &&
+                b.ns && b.ncnt && b.nleft && b.nright && b.ndepth && b.nodeSlot && b.small && b.bigSlot && b.bigCount &&
+                b.bins && b.act && act2 && b.sAxis &&
                 b.sBin && b.sNl && tiles && dcount, "gpu build: out of device memory");
 
     const int T = kThreads;
@@ -781,33 +922,25 @@ int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& ou
     const int chunks = (n + kChunk - 1) / kChunk;
     hipLaunchKernelGGL(k_prims, grid(n), dim3(T), 0, s, d_tris, n, pbox, cen, b.idx, b.pnode, dOff, R);
     hipLaunchKernelGGL(k_init_roots, grid(R), dim3(T), 0, s, b, dOff, dSlots, R);
-    hipLaunchKernelGGL(k_init_nodes, grid(R), dim3(T), 0, s, b.nbox, b.ncen, 0, R);
+    hipLaunchKernelGGL(k_init_nodes, grid(R), dim3(T), 0, s, b.nbox, b.ncen, b.small, 0, R);
     hipLaunchKernelGGL(k_node_bounds, dim3(chunks), dim3(T), 0, s, b, 0);
 
     std::vector<int> levelStart = {0, R};     // node ids of level L: [levelStart[L], levelStart[L+1])
     int nodeCount = R;
-    int* bins = nullptr;
-    size_t binsCap = 0;
     while (m > 0) {
-        if ((size_t)m > binsCap) {
-            if (bins) hipFree(bins);
-            binsCap = std::max<size_t>((size_t)m, 2 * binsCap);
-            if (hipMalloc((void**)&bins, binsCap * kSlotInts * sizeof(int)) != hipSuccess) {
-                set_error("gpu build: out of device memory for bins");
-                return -1;
-            }
-        }
-        b.bins = bins;
         const int childBase = nodeCount;
-        hipLaunchKernelGGL(k_clear_bins, grid((long long)m * kSlotInts), dim3(T), 0, s, bins, m);
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, b.bigCount);
+        hipLaunchKernelGGL(k_big_slots, grid(m), dim3(T), 0, s, b, m);
+        hipLaunchKernelGGL(k_clear_bins, grid((long long)maxBig * kSlotInts), dim3(T), 0, s, b.bins, maxBig);
+        hipLaunchKernelGGL(k_init_nodes, grid(2 * m), dim3(T), 0, s, b.nbox, b.ncen, b.small, childBase, 2 * m);
         hipLaunchKernelGGL(k_bin, dim3(chunks), dim3(T), 0, s, b);
         hipLaunchKernelGGL(k_sah, grid(m), dim3(T), 0, s, b, m, childBase);
+        hipLaunchKernelGGL(k_small_level, dim3(m), dim3(64), 0, s, b, m, childBase);
         hipLaunchKernelGGL(k_flags, grid(n), dim3(T), 0, s, b);
         scan_exclusive(b.flag, b.rank, n, dcount, tiles, s);
         hipLaunchKernelGGL(k_scatter, grid(n), dim3(T), 0, s, b);
         std::swap(b.idx, b.idx2);
         std::swap(b.pnode, b.pnode2);
-        hipLaunchKernelGGL(k_init_nodes, grid(2 * m), dim3(T), 0, s, b.nbox, b.ncen, childBase, 2 * m);
         hipLaunchKernelGGL(k_node_bounds, dim3(chunks), dim3(T), 0, s, b, childBase);
         // next level's active nodes: children with more than maxLeaf primitives, in id order
         hipLaunchKernelGGL(k_active_flags, grid(2 * m), dim3(T), 0, s, b.ncnt, childBase, 2 * m, maxLeaf, b.flag);
@@ -819,14 +952,11 @@ int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& ou
         int next = 0;
         if (hipMemcpyAsync(&next, dcount, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess) {
-            hipFree(bins);
             set_error("gpu build: level %zu failed", levelStart.size() - 2);
             return -1;
         }
         m = next;
     }
-    if (bins) hipFree(bins);
-
     // pre-order renumbering
     int* size = tmp.get<int>(nodeCount);
     int* pre = tmp.get<int>(nodeCount);
@@ -896,6 +1026,7 @@ int gpu_build(const float* d_tris, int n, int maxLeaf, hipStream_t s, GpuBVH& ou
     hipError_t e = hipStreamSynchronize(s);
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) { set_error("gpu build failed: %s", hipGetErrorString(e)); return -1; }
+    if (verbose()) fprintf(stderr, "[CudaBVH] gpu_build: %d levels, %.2f ms to the last kernel\n", levels, now_ms() - tBegin);
     out.levels = levels;
     return 0;
 }

@@ -156,4 +156,13 @@ struct CudaBVHHandle {
     void* d_work = nullptr;   size_t workBytes = 0;   // staging for host-pointer calls
     float* h_pinned = nullptr; size_t pinnedFloats = 0; // pinned host staging of host-pointer calls
     void* d_sortWs = nullptr; size_t sortWsBytes = 0; // ray-sort workspace
+
+    // The host copies above are made on the first call that needs them
+    // (the reference's getters, the CWBVH build) from these device arrays,
+    // so a build that is only traced never pays for them.
+    bool exported = false;
+    float buildStats[4] = {0, 0, 0, 0};  // upload ms, device build ms, handle ms, BVH2 levels
+    void* d_nodes2 = nullptr;           // pre-order BVH2 (rt::Node2); NULL for a forest
+    int numNodes2 = 0;
+    int* d_ordered = nullptr;           // leaf position -> original index
 };
