@@ -404,13 +404,18 @@ def bench_path_tracer(pkg, scenes, dev, args):
     alb = torch.empty(3 * n, device=dev)
     nrm = torch.empty(3 * n, device=dev)
     dep = torch.empty(n, device=dev)
+    if not args.no_tune:
+        # queue size: below how many live paths one thread per path finishes the bounces
+        best_tail, table = gs.autotune_path_tracer(c, pkg.path_params(W, H, spp=1, seed=1), rgb, alb, nrm, dep, reps=4)
+        tuning["tail_paths"] = {"chosen": best_tail, "sweep_ms": {str(t): round(ms, 4) for t, ms in table}}
+        log(f"pt autotune tail paths: {best_tail}  " + " ".join(f"{t}:{ms:.3f}" for t, ms in table))
     K = 16
     timed = {}
     for name, extra in (("unsorted", 0), ("octant_sorted", pkg.FLAG_SORT_RAYS)):
         # one graph capture per variant (sorting adds kernels), then K replayed samples
-        p = pkg.path_params(W, H, spp=1, seed=1)
+        p = pkg.path_params(W, H, spp=1, seed=1, tail_paths=-1)
         cnt = gs.path_trace(c, p, rgb, alb, nrm, dep, counts=True, flags=pkg.FLAG_GRAPH | extra)
-        p = pkg.path_params(W, H, spp=K, seed=2)
+        p = pkg.path_params(W, H, spp=K, seed=2, tail_paths=-1)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         e0.record()

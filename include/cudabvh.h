@@ -435,8 +435,9 @@ typedef struct {
     float skyHorizon[3];
     float clampMax;       /* per-contribution max-channel clamp; <= 0 disables */
     int tailPaths;        /* from the first bounce with at most this many live paths on, one
-                             thread per path runs all remaining bounces (0: wavefront only);
-                             the image does not depend on it */
+                             thread per path runs all remaining bounces (0: wavefront only;
+                             < 0: the scene's tuned value, 32768 until
+                             cudaBVH_sceneAutotunePathTracer); the image does not depend on it */
 } CudaPathParams;
 
 /* Device buffers of width*height floats per plane. rgb: 3 planes (R, G, B).
@@ -460,6 +461,16 @@ typedef struct {
  * traced (same image). */
 int cudaBVH_scenePathTrace(CudaScene_t scene, const CudaCamera* camera, const CudaPathParams* params,
                            const CudaPathImage* image, long long* rayCounts, unsigned flags);
+
+/* The path tracer's queue-size knob, tuned on the caller's frame: renders
+ * `reps` graph-replayed samples (after one warm-up) with each tail threshold
+ * of {0, 4096, 16384, 32768, 65536, 131072} paths and keeps the fastest as
+ * the scene's value for params->tailPaths < 0 (the image is the same for
+ * all). results (nullable): (threshold, ms per sample) pairs, numResults
+ * receives their count. 0 on success. No backend counterpart: the
+ * reference's queues run every bounce as one launch (cuda_rt_v34.cu:1077-1545). */
+int cudaBVH_sceneAutotunePathTracer(CudaScene_t scene, const CudaCamera* camera, const CudaPathParams* params,
+                                    const CudaPathImage* image, int reps, float* results, int* numResults);
 
 /* ----- Many independent scenes in one build and one launch ----- */
 typedef struct CudaBVHBatchHandle* CudaBVHBatch_t;

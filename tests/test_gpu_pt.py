@@ -142,3 +142,27 @@ def test_path_tracer_schedules(pkg, oracle, scenes, dev, configs):
         assert np.array_equal(_bits(rgb), _bits(ref)), flags
         assert np.array_equal(_bits(aov), _bits(raov)), flags
     gs.destroy()
+
+
+def test_path_tracer_tail_autotune(pkg, oracle, scenes, dev):
+    """The tail threshold (queue size below which one thread per path takes
+    over) is tuned on the frame; renders with tail_paths = -1 use the tuned
+    value and still equal the oracle's image."""
+    import torch
+    sc = scenes.instanced_scene(mesh_res=(12, 20, 28, 36), n_inst=64, seed=9, span=6.0)
+    cam = scenes.look_at((0.0, 4.0, -18.0), (0.0, 0.0, 0.0), 96, 64, 60.0)
+    gs = _scene(pkg, sc)
+    n = 96 * 64
+    bufs = [torch.empty(3 * n, device=dev) for _ in range(3)] + [torch.empty(n, device=dev)]
+    c = pkg.camera(cam["o"], cam["f"], cam["r"], cam["u"], cam["scale_x"], cam["scale_y"])
+    best, table = gs.autotune_path_tracer(c, pkg.path_params(96, 64, spp=1, seed=3), *bufs, reps=2)
+    assert [t for t, _ in table] == [0, 4096, 16384, 32768, 65536, 131072]
+    assert best in [t for t, _ in table] and all(ms > 0 for _, ms in table)
+    osc = oracle.OracleScene(sc["meshes"], sc["o2w"], sc["blas_of"], gs.world_to_object())
+    op = oracle.path_params(96, 64, spp=2, seed=21)
+    ref, _, rcnt = oracle.path_trace_2l(osc, sc["albedo"], scenes.cam_array(cam), op)
+    for flags in (0, pkg.FLAG_GRAPH):
+        rgb, _, cnt = _render(pkg, gs, cam, pkg.path_params(96, 64, spp=2, seed=21, tail_paths=-1), dev, flags)
+        assert np.array_equal(cnt, rcnt) and np.array_equal(_bits(rgb), _bits(ref)), flags
+    assert gs.trace_status() == 0
+    gs.destroy()

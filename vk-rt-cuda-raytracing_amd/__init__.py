@@ -177,6 +177,8 @@ SIGNATURES = {
     "cudaBVH_sceneAutotune": (_I, [_H, _I, _I, C.POINTER(CudaRayPacket), _I, _P, _P]),
     "cudaBVH_scenePathTrace": (_I, [_H, C.POINTER(CudaCamera), C.POINTER(CudaPathParams), C.POINTER(CudaPathImage),
                                     _P, C.c_uint]),
+    "cudaBVH_sceneAutotunePathTracer": (_I, [_H, C.POINTER(CudaCamera), C.POINTER(CudaPathParams),
+                                             C.POINTER(CudaPathImage), _I, _P, _P]),
     "cudaBVH_buildBatch": (_H, [_P, _P, _I]),
     "cudaBVH_batchDestroy": (None, [_H]),
     "cudaBVH_batchSetStream": (_I, [_H, _P]),
@@ -697,6 +699,23 @@ def _scene_path_trace(self, cam: CudaCamera, params: CudaPathParams, rgb, albedo
 
 
 Scene.path_trace = _scene_path_trace
+
+
+def _scene_autotune_path_tracer(self, cam: CudaCamera, params: CudaPathParams, rgb, albedo, normal, depth,
+                                reps: int = 4):
+    """Tunes the tail threshold (the queue size below which one thread per
+    path finishes the bounces) on this frame; returns (chosen, [(threshold,
+    ms per sample)]). Later renders use it with tail_paths=-1."""
+    img = CudaPathImage(_ptr(rgb), _ptr(albedo), _ptr(normal), _ptr(depth))
+    res = np.zeros(12, np.float32)
+    k = C.c_int()
+    _check(self.lib.cudaBVH_sceneAutotunePathTracer(self.handle, C.byref(cam), C.byref(params), C.byref(img), reps,
+                                                    res.ctypes.data, C.byref(k)), "sceneAutotunePathTracer")
+    table = [(int(res[2 * i]), float(res[2 * i + 1])) for i in range(k.value)]
+    return min(table, key=lambda x: x[1])[0], table
+
+
+Scene.autotune_path_tracer = _scene_autotune_path_tracer
 
 
 class Batch:

@@ -1784,6 +1784,45 @@ int cudaBVH_sceneAutotune(CudaScene_t sc, int rayClass, int numRays, const CudaR
     return rc;
 }
 
+int cudaBVH_sceneAutotunePathTracer(CudaScene_t sc, const CudaCamera* cam, const CudaPathParams* params,
+                                    const CudaPathImage* image, int reps, float* results, int* numResults) {
+    rt::clear_error();
+    if (!scene_ok(sc, "cudaBVH_sceneAutotunePathTracer")) return -1;
+    if (!cam || !params || !image || reps <= 0) { set_error("cudaBVH_sceneAutotunePathTracer: bad arguments"); return -1; }
+    const int cands[6] = {0, 4096, 16384, 32768, 65536, 131072};
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        set_error("cudaBVH_sceneAutotunePathTracer: cannot create events");
+        return -1;
+    }
+    hipStream_t s = (hipStream_t)sc->stream;
+    float best = 1e30f;
+    int bestTail = sc->ptTail, rc = 0;
+    for (int k = 0; k < 6 && rc == 0; k++) {
+        CudaPathParams p = *params;
+        p.tailPaths = cands[k];
+        p.spp = 1;
+        rc = rtk::path_trace(sc, *cam, p, *image, nullptr, CUDABVH_FLAG_GRAPH | CUDABVH_FLAG_SYNC);   // capture
+        p.spp = reps;
+        if (rc == 0) {
+            hipEventRecord(e0, s);
+            rc = rtk::path_trace(sc, *cam, p, *image, nullptr, CUDABVH_FLAG_GRAPH);
+            hipEventRecord(e1, s);
+            hipEventSynchronize(e1);
+        }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)reps;
+        if (results) { results[2 * k] = (float)cands[k]; results[2 * k + 1] = ms; }
+        if (rc == 0 && ms < best) { best = ms; bestTail = cands[k]; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (numResults) *numResults = 6;
+    if (rc == 0) sc->ptTail = bestTail;
+    return rc;
+}
+
 int cudaBVH_scenePathTrace(CudaScene_t sc, const CudaCamera* cam, const CudaPathParams* params,
                            const CudaPathImage* image, long long* rayCounts, unsigned flags) {
     rt::clear_error();

@@ -596,7 +596,7 @@ int path_trace(CudaSceneHandle* sc, const CudaCamera& cam, const CudaPathParams&
     }
     a.clampMax = p.clampMax;
     a.sortOct = (flags & CUDABVH_FLAG_SORT_RAYS) ? 1 : 0;
-    a.tailPaths = std::max(0, std::min(p.tailPaths, N));
+    a.tailPaths = std::max(0, std::min(p.tailPaths < 0 ? sc->ptTail : p.tailPaths, N));
     a.tlasNodes = (const uint4*)sc->tlas->d_nodes4;
     a.tlasTris = (const float4*)sc->tlas->d_tris;
     if (a.tailPaths > 0) {

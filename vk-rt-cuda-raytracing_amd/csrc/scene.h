@@ -71,6 +71,7 @@ struct CudaSceneHandle {
     rt::TraceScratch scratch;
     void* stream = nullptr;
     void* pt = nullptr;                        // path-tracer queues and graph (pt.hip)
+    int ptTail = 32768;                        // path tracer's tail threshold for params.tailPaths < 0
     rt::TraceConfig tune2l[3];                 // launch configuration: closest hit, any hit, closest hit of
                                                // the path tracer's secondary bounces
 };
