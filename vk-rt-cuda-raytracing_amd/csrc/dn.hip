@@ -48,11 +48,17 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 struct ConvArgs {
     const uint16_t* x; int ldx; int cin;     // input pixels (channel stride ldx), channels per tap
     const uint16_t* w;                       // [rows >= gridDim.y*16*WN][taps*cin]
+    const uint16_t* wfrag;                   // the same weights in MFMA fragment order (k_conv3r)
     const float* bias;                       // nullable
     uint16_t* y; int ldy;                    // BF16 output slice (nullable)
     float* yf; int ldyf;                     // FP32 output (nullable)
     int n;                                   // output channels written
     int B, H, W, relu;
+    // k_conv3r only: input channels [0, cup) are read 2x nearest-upsampled
+    // from xu (B x H/2 x W/2, channel stride ldxu) instead of from x; and an
+    // optional 2x2 max-pooled copy of the BF16 output goes to yp (stride ldyp)
+    const uint16_t* xu; int ldxu; int cup;
+    uint16_t* yp; int ldyp;
 };
 
 // Block: 4 waves x (16*WM) pixels; each wave (16*WM) pixels x (16*WN)
@@ -153,152 +159,211 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
         }
 }
 
-// 3x3 convolution with LDS-staged operands (cin a multiple of 32). A
-// workgroup computes BM consecutive pixels of one image row x BN output
-// channels; waves form a WGM x WGN grid of (16*WM) x (16*WN) tiles. Per
-// 32-channel chunk it stages the input halo (3 rows x (BM + 2) pixels) and the
-// chunk's weights for all 9 taps in LDS (rows padded to 80 bytes, so the
-// 16-byte operand reads of a wave's 16 rows hit distinct banks), then runs
-// the 9 taps' MFMAs from LDS: every input pixel is fetched from memory once
-// per chunk instead of once per tap and wave. Same arithmetic order per
-// output as k_conv (FP32 accumulation over taps, then channel chunks).
-constexpr int kPxStride = 40;   // bf16 elements per LDS row: 32 channels + 8 padding
 
-template <int WM, int WN, int WGM, int WGN>
-__global__ __launch_bounds__(kBlock) void k_conv3(ConvArgs a) {
-    static_assert(WGM * WGN == kBlock / 64, "four waves");
-    constexpr int BM = WGM * 16 * WM, BN = WGN * 16 * WN, HALO = BM + 2;
-    __shared__ __attribute__((aligned(16))) uint16_t As[3 * HALO * kPxStride];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[9 * BN * kPxStride + 8];   // + a 16-byte sink
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int tilesX = (a.W + BM - 1) / BM;
-    const int tile = blockIdx.x;
-    const int tx = tile % tilesX, ty = tile / tilesX;       // ty = image * H + row
-    const int yy = ty % a.H, bb = ty / a.H;
-    const int x0 = tx * BM, n0 = blockIdx.y * BN;
-    const int wm = wave % WGM, wn = wave / WGM;
-    const int row = lane & 15, kg = lane >> 4;
-    const int K = 9 * a.cin;
-    f32x4 acc[WM][WN];
-#pragma unroll
-    for (int r = 0; r < WM; r++)
-#pragma unroll
-        for (int j = 0; j < WN; j++) acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-    // The next chunk's input halo is fetched into registers while the MFMAs
-    // of the current chunk run from LDS (software pipelining); the weights,
-    // shared by every workgroup of a channel tile and mostly L2-resident, are
-    // copied to LDS in the staging step.
-    constexpr int NA = (3 * HALO * 4 + kBlock - 1) / kBlock;
-    uint4 ra[NA];
-#define DN_FETCH(C0)                                                                                        \
-    {                                                                                                        \
-        _Pragma("unroll") for (int k = 0; k < NA; k++) {                                                     \
-            const int e = tid + k * kBlock;                                                                  \
-            ra[k] = make_uint4(0, 0, 0, 0);                                                                  \
-            if (e < 3 * HALO * 4) {                                                                          \
-                const int r = e / (HALO * 4), rem = e - r * (HALO * 4), h = rem >> 2, q = rem & 3;           \
-                const int y = yy + r - 1, x = x0 - 1 + h;                                                    \
-                if (y >= 0 && y < a.H && x >= 0 && x < a.W)                                                  \
-                    ra[k] = *(const uint4*)(a.x + ((size_t)(bb * a.H + y) * a.W + x) * a.ldx + (C0) + 8 * q); \
-            }                                                                                                \
-        }                                                                                                    \
+// 3x3 convolution with row reuse (cin a multiple of 32). A workgroup computes
+// an R-row x 16*WGX-pixel block of one image for 32*WGC output channels; wave
+// (wx, wc) owns the R x 16 pixels of column block wx for the 32 channels of
+// block wc (R x 2 MFMA 16x16x32 tiles). Per 32-channel input chunk the
+// workgroup stages the (R + 2) x (16*WGX + 2)-pixel input halo in LDS. A wave
+// then walks the halo rows once per horizontal tap dx: each 16-pixel row
+// fragment it reads feeds the MFMAs of every output row that uses it (up to
+// three: vertical taps dy = 0, 1, 2), so a wave reads 3 (R + 2) A fragments
+// per chunk for 9 R x 2 MFMAs instead of 9 R (two per MFMA pair without row
+// reuse). The B fragments (the three vertical taps of one dx, two 16-channel
+// tiles) come straight from memory into registers (weights are small and
+// L2-resident), the next dx's fragments loading while the current dx's
+// MFMAs run; the next chunk's halo loads into registers likewise.
+//
+// LDS layout: one 64-byte row (32 channels) per halo pixel hp, its four
+// 16-byte quarters q stored at slots q ^ (hp & 2). A ds_read_b128 serves its
+// 64 lanes in eight phases of eight lanes (lanes 0-3 with 20-23, 4-7 with
+// 16-19, 8-11 with 28-31, 12-15 with 24-27, the same in the upper half); for
+// the A fragment (lane = pixel row + 16 * quarter) this swizzle puts the eight
+// lanes of every phase on eight different 16-byte bank groups for any start
+// pixel, where the previous 80-byte padded rows gave two-way conflicts.
+// Scheduling groups of one halo row walk (R output rows): the MFMAs of row
+// hy (2 per output row that uses it: 2, 4, 6, ..., 6, 4, 2), then the LDS
+// read of row hy + 2.
+template <int R, int PD, int HYI>
+__device__ __forceinline__ void halo_row_order() {
+    if constexpr (HYI < R + 2) {
+        constexpr int first = HYI < 2 ? HYI : 2, last = HYI - (R - 1) > 0 ? HYI - (R - 1) : 0;
+        __builtin_amdgcn_sched_group_barrier(0x008, 2 * (first - last + 1), 0);
+        if constexpr (HYI + PD < R + 2) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        halo_row_order<R, PD, HYI + 1>();
     }
-    // The chunk's weights are loaded in one batch of independent loads issued
-    // before the barrier, then written to LDS: one memory round trip per chunk
-    // instead of one per 256 16-byte pieces.
-    constexpr int NB = (9 * BN * 4 + kBlock - 1) / kBlock;
-    DN_FETCH(0)
-    for (int c0 = 0; c0 < a.cin; c0 += 32) {
-        // Named registers rather than an array: the compiler kept an array
-        // of this size in scratch memory.
-        uint4 w0, w1, w2, w3, w4, w5, w6, w7, w8;
-        static_assert(NB <= 9, "weight pieces per thread");
-        // unconditional loads (a piece past the end re-reads the last one and
-        // goes to the LDS sink below)
-        auto wload = [&](int k) {
-            const int e = min(tid + k * kBlock, 9 * BN * 4 - 1);
-            const int t = e / (BN * 4), rem = e - t * (BN * 4), nr = rem >> 2, q = rem & 3;
-            return *(const uint4*)(a.w + (size_t)(n0 + nr) * K + t * a.cin + c0 + 8 * q);
-        };
-        auto wstore = [&](int k, const uint4& v) {
-            const int e = tid + k * kBlock;
-            const int t = e / (BN * 4), rem = e - t * (BN * 4), nr = rem >> 2, q = rem & 3;
-            const int dst = e < 9 * BN * 4 ? (t * BN + nr) * kPxStride + 8 * q : 9 * BN * kPxStride;
-            *(uint4*)&Bs[dst] = v;
-        };
-        w0 = wload(0);
-        if constexpr (NB > 1) w1 = wload(1);
-        if constexpr (NB > 2) w2 = wload(2);
-        if constexpr (NB > 3) w3 = wload(3);
-        if constexpr (NB > 4) w4 = wload(4);
-        if constexpr (NB > 5) w5 = wload(5);
-        if constexpr (NB > 6) w6 = wload(6);
-        if constexpr (NB > 7) w7 = wload(7);
-        if constexpr (NB > 8) w8 = wload(8);
-        __syncthreads();       // the previous chunk's operands are no longer read
-#pragma unroll
-        for (int k = 0; k < NA; k++) {
-            const int e = tid + k * kBlock;
-            if (e < 3 * HALO * 4) {
-                const int r = e / (HALO * 4), rem = e - r * (HALO * 4), h = rem >> 2, q = rem & 3;
-                *(uint4*)&As[(r * HALO + h) * kPxStride + 8 * q] = ra[k];
-            }
-        }
-        wstore(0, w0);
-        if constexpr (NB > 1) wstore(1, w1);
-        if constexpr (NB > 2) wstore(2, w2);
-        if constexpr (NB > 3) wstore(3, w3);
-        if constexpr (NB > 4) wstore(4, w4);
-        if constexpr (NB > 5) wstore(5, w5);
-        if constexpr (NB > 6) wstore(6, w6);
-        if constexpr (NB > 7) wstore(7, w7);
-        if constexpr (NB > 8) wstore(8, w8);
-        __syncthreads();
-        if (c0 + 32 < a.cin) DN_FETCH(c0 + 32)
-#pragma unroll
-        for (int t = 0; t < 9; t++) {
-            const int dy = t / 3, dx = t % 3;
-            bf16x8 av[WM], bv[WN];
-#pragma unroll
-            for (int r = 0; r < WM; r++) {
-                const int px = wm * 16 * WM + r * 16 + row + dx;
-                av[r] = __builtin_bit_cast(bf16x8, *(const uint4*)&As[(dy * HALO + px) * kPxStride + 8 * kg]);
-            }
-#pragma unroll
-            for (int j = 0; j < WN; j++) {
-                const int nr = wn * 16 * WN + j * 16 + row;
-                bv[j] = __builtin_bit_cast(bf16x8, *(const uint4*)&Bs[(t * BN + nr) * kPxStride + 8 * kg]);
-            }
-#pragma unroll
-            for (int r = 0; r < WM; r++)
-#pragma unroll
-                for (int j = 0; j < WN; j++)
-                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], bv[j], acc[r][j], 0, 0, 0);
-        }
-    }
-    // D[i][j]: lane j + 16*(i/4), item i%4
-#pragma unroll
-    for (int r = 0; r < WM; r++)
-#pragma unroll
-        for (int j = 0; j < WN; j++) {
-            const int col = n0 + wn * 16 * WN + j * 16 + row;
-            if (col >= a.n) continue;
-            const float b = a.bias ? a.bias[col] : 0.0f;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int x = x0 + wm * 16 * WM + r * 16 + 4 * kg + i;
-                if (x >= a.W) continue;
-                const size_t p = (size_t)ty * a.W + x;
-                float v = acc[r][j][i] + b;
-                if (a.relu) v = fmaxf(v, 0.0f);
-                if (a.y) a.y[p * a.ldy + col] = f2bf(v);
-                if (a.yf) a.yf[p * a.ldyf + col] = v;
-            }
-        }
 }
 
-#undef DN_FETCH
+template <int R, int WGX, int WGC, int OCC, int PD, bool BL>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) void k_conv3r(ConvArgs a) {
+    static_assert(WGX * WGC == kBlock / 64, "four waves");
+    static_assert(!BL || WGC == 1, "weights are shared through LDS only when every wave has the same channels");
+    constexpr int TX = 16 * WGX, HX = TX + 2, HY = R + 2, HP = HX * HY;
+    constexpr int NPC = HP * 4;                                // 16-byte pieces per chunk
+    constexpr int NA = (NPC + kBlock - 1) / kBlock;
+    __shared__ uint4 hs[HP * 4];
+    __shared__ uint4 hb[BL ? 18 * 64 : 1];          // BL: the chunk's 18 B fragments (2 tiles x 9 taps)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wx = wave % WGX, wc = wave / WGX;
+    const int row = lane & 15, kg = lane >> 4;
+    const int tilesX = (a.W + TX - 1) / TX, tilesY = (a.H + R - 1) / R;
+    const int tx = blockIdx.x % tilesX, tq = blockIdx.x / tilesX;
+    const int ty = tq % tilesY, bb = tq / tilesY;
+    const int x0 = tx * TX, y0 = ty * R;
+    const int n0 = blockIdx.y * (32 * WGC) + wc * 32;
+    // B fragments: a.wfrag is [16-channel block][chunk][tap][lane][8], so a
+    // wave's fragment is 1 KB of consecutive memory
+    const int nch = a.cin / 32;
+    const uint16_t* wf0 = a.wfrag + (size_t)(n0 / 16) * nch * 9 * 512 + lane * 8;
+    const uint16_t* wf1 = wf0 + (size_t)nch * 9 * 512;
+
+    // halo piece k of this thread: pixel (y0 - 1 + hy, x0 - 1 + hx), quarter
+    // e & 3 of the chunk; zero outside the image. Channels below cup come
+    // from the half-resolution xu at (y / 2, x / 2).
+    uint4 ra[NA];
+    auto fetch = [&](int c0) {
+        const bool up = c0 < a.cup;
+#pragma unroll
+        for (int k = 0; k < NA; k++) {
+            const int e = tid + k * kBlock, hp = e >> 2;
+            const int hy = hp / HX, hx = hp - hy * HX;
+            const int y = y0 - 1 + hy, x = x0 - 1 + hx;
+            ra[k] = make_uint4(0, 0, 0, 0);
+            if (e < NPC && y >= 0 && y < a.H && x >= 0 && x < a.W) {
+                const uint16_t* src =
+                    up ? a.xu + (size_t)((bb * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1)) * a.ldxu
+                       : a.x + (size_t)((bb * a.H + y) * a.W + x) * a.ldx;
+                ra[k] = *(const uint4*)(src + c0 + 8 * (e & 3));
+            }
+        }
+    };
+    auto bload = [&](bf16x8 (&b)[3][2], int c0, int dx) {
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++) {
+            if constexpr (BL) {
+                b[dy][0] = __builtin_bit_cast(bf16x8, hb[(dy * 3 + dx) * 64 + lane]);
+                b[dy][1] = __builtin_bit_cast(bf16x8, hb[(9 + dy * 3 + dx) * 64 + lane]);
+            } else {
+                const int o = ((c0 >> 5) * 9 + dy * 3 + dx) * 512;
+                b[dy][0] = __builtin_bit_cast(bf16x8, *(const uint4*)(wf0 + o));
+                b[dy][1] = __builtin_bit_cast(bf16x8, *(const uint4*)(wf1 + o));
+            }
+        }
+    };
+    // BL: the chunk's 18 fragments go straight from memory to LDS (1 KB per
+    // wave instruction, fragments wave, wave + 4, ...)
+    auto bstage = [&](int c0) {
+#pragma unroll
+        for (int f = wave; f < 18; f += 4) {
+            const uint16_t* src = (f < 9 ? wf0 : wf1) + ((c0 >> 5) * 9 + (f % 9)) * 512;
+            __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)&hb[f * 64], 16,
+                                             0, 0);
+        }
+    };
+    f32x4 acc[R][2];
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r][0] = acc[r][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // One horizontal tap: the MFMAs of dx with the fragments in bc, after
+    // issuing the loads of the next step's fragments into bn (kept ahead of
+    // the MFMAs by the scheduling barrier, so their latency is hidden).
+    auto dxstep = [&](const bf16x8 (&bc)[3][2], bf16x8 (&bn)[3][2], int dx, bool next, int nc0, int ndx) {
+        if (next) bload(bn, nc0, ndx);
+        __builtin_amdgcn_sched_barrier(0);
+        const int hp0 = wx * 16 + row + dx;
+        const int s0 = hp0 * 4 + (kg ^ (hp0 & 2)), s1 = hp0 * 4 + (kg ^ (hp0 & 2) ^ 2);
+#pragma unroll
+        for (int hy = 0; hy < HY; hy++) {
+            // (hp0 + hy * HX) & 2 flips with bit 1 of hy * HX (HX is even)
+            const int slot = hy * HX * 4 + (((hy * HX) & 2) ? s1 : s0);
+            const bf16x8 av = __builtin_bit_cast(bf16x8, hs[slot]);
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++) {
+                const int ro = hy - dy;
+                if (ro < 0 || ro >= R) continue;
+                acc[ro][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bc[dy][0], av, acc[ro][0], 0, 0, 0);
+                acc[ro][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bc[dy][1], av, acc[ro][1], 0, 0, 0);
+            }
+        }
+        // issue order: the A fragment of row hy + 2 is read while row hy's
+        // MFMAs run (two reads in flight keep the register count down)
+        __builtin_amdgcn_sched_group_barrier(0x100, PD, 0);
+        halo_row_order<R, PD, 0>();
+    };
+    // One channel chunk: stage its halo, start the next chunk's halo loads,
+    // run the three taps. The fragments alternate between two register sets
+    // (x: dx 0 and 2, y: dx 1 and the next chunk's dx 0), so no copy waits
+    // for a load in flight; chunks alternate the roles of the two sets.
+    auto chunk = [&](int c0, bf16x8 (&x)[3][2], bf16x8 (&y)[3][2]) {
+        __syncthreads();               // the previous chunk's halo (and weights) are no longer read
+        if constexpr (BL) bstage(c0);
+#pragma unroll
+        for (int k = 0; k < NA; k++) {
+            const int e = tid + k * kBlock, hp = e >> 2;
+            if (e < NPC) hs[hp * 4 + ((e & 3) ^ (hp & 2))] = ra[k];
+        }
+        if constexpr (BL) __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wave's weight copies landed
+        __syncthreads();
+        const bool more = c0 + 32 < a.cin;
+        if (more) fetch(c0 + 32);
+        if constexpr (BL) bload(x, c0, 0);
+        dxstep(x, y, 0, true, c0, 1);
+        dxstep(y, x, 1, true, c0, 2);
+        dxstep(x, y, 2, more && !BL, c0 + 32, 0);
+    };
+    bf16x8 b0[3][2], b1[3][2];
+    fetch(0);
+    if constexpr (!BL) bload(b0, 0, 0);
+    for (int c0 = 0; c0 < a.cin; c0 += 64) {
+        chunk(c0, b0, b1);
+        if (c0 + 32 < a.cin) chunk(c0 + 32, b1, b0);
+    }
+    // The weights are the MFMA's A operand and the pixels its B operand, so
+    // D is channels x pixels: lane (pixel l % 16, k = l / 16) holds channels
+    // 4k .. 4k + 3 of its pixel, stored as one 8-byte write.
+    // With yp, the 2x2 maximum of each output row pair (r, r + 1; y0 and R
+    // are even) and lane pair (l, l ^ 1: pixels 2m, 2m + 1) is written too:
+    // rounding to BF16 is monotonic, so this equals pooling the BF16 output.
+    const int x = x0 + wx * 16 + row;
+    float pool[2][4];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + r;
+        const size_t p = (size_t)(bb * a.H + y) * a.W + x;
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int col = n0 + j * 16 + 4 * kg;
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                v[i] = acc[r][j][i] + (a.bias && col < a.n ? a.bias[col + i] : 0.0f);
+                if (a.relu) v[i] = fmaxf(v[i], 0.0f);
+            }
+            const bool ok = y < a.H && x < a.W && col < a.n;
+            if (ok && a.y)
+                *(uint2*)(a.y + p * a.ldy + col) =
+                    make_uint2(f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16), f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
+            if (ok && a.yf) *(float4*)(a.yf + p * a.ldyf + col) = make_float4(v[0], v[1], v[2], v[3]);
+            if constexpr (R % 2 == 0) {
+                if (a.yp) {
+                    // rows r - 1 and r are a pair when r is odd: keep the even row's values
+                    float m[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const float h = fmaxf(v[i], __shfl_xor(v[i], 1));
+                        m[i] = (r & 1) ? fmaxf(h, pool[j][i]) : h;
+                        pool[j][i] = m[i];
+                    }
+                    if ((r & 1) && ok && !(row & 1)) {
+                        const size_t q = (size_t)(bb * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
+                        *(uint2*)(a.yp + q * a.ldyp + col) = make_uint2(f2bf(m[0]) | ((uint32_t)f2bf(m[1]) << 16),
+                                                                        f2bf(m[2]) | ((uint32_t)f2bf(m[3]) << 16));
+                    }
+                }
+            }
+        }
+    }
+}
 
 // --------------------------------------------------------------- wgrad ----
 struct WgradArgs {
@@ -408,30 +473,6 @@ __global__ __launch_bounds__(kBlock) void k_relu_bwd(uint16_t* g, int ldg, const
         for (int e = 0; e < 8; e++)
             if (!(bf2f(ye[e]) > 0.0f)) ge[e] = 0;
         *(uint4*)(g + p * ldg + k) = gv;
-    }
-}
-
-// 2x2 max pool: out [B][H/2][W/2][c] (stride ldo) from in (stride ldi)
-__global__ __launch_bounds__(kBlock) void k_pool(const uint16_t* in, int ldi, uint16_t* out, int ldo, int c, int B,
-                                                 int H, int W) {
-    const int Ho = H / 2, Wo = W / 2, cg = c / 8;
-    const size_t n = (size_t)B * Ho * Wo * cg;
-    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
-        const int k = 8 * (int)(i % cg);
-        const size_t po = i / cg;
-        const int xo = (int)(po % Wo), q = (int)(po / Wo), yo = q % Ho, b = q / Ho;
-        float m[8];
-        for (int e = 0; e < 8; e++) m[e] = -INFINITY;
-        for (int s = 0; s < 4; s++) {
-            const size_t pi = ((size_t)(b * H + 2 * yo + (s >> 1)) * W + 2 * xo + (s & 1));
-            const uint4 v = *(const uint4*)(in + pi * ldi + k);
-            const uint16_t* ve = (const uint16_t*)&v;
-            for (int e = 0; e < 8; e++) m[e] = fmaxf(m[e], bf2f(ve[e]));
-        }
-        uint4 o;
-        uint16_t* oe = (uint16_t*)&o;
-        for (int e = 0; e < 8; e++) oe[e] = (uint16_t)(__float_as_uint(m[e]) >> 16);   // exact: inputs are bf16
-        *(uint4*)(out + po * ldo + k) = o;
     }
 }
 
@@ -562,6 +603,22 @@ __global__ __launch_bounds__(kBlock) void k_wlayout(const float* w, int cout, in
     }
 }
 
+// [rows][9*cin] (row-major 3x3 weights, cin a multiple of 32) -> MFMA
+// fragment order [rows/16][cin/32][9][lane][8]: lane = row%16 + 16*kg holds
+// the 8 consecutive input channels 32*chunk + 8*kg .. +7 of tap t of that row
+// (the B operand of mfma_f32_16x16x32_bf16 for one tap and channel chunk).
+__global__ __launch_bounds__(kBlock) void k_wfrag(const uint16_t* w, int rows, int cin, uint16_t* out) {
+    const size_t n = (size_t)rows * 9 * cin;
+    const int nch = cin / 32;
+    for (size_t o = blockIdx.x * (size_t)kBlock + threadIdx.x; o < n; o += (size_t)gridDim.x * kBlock) {
+        const int e = (int)(o & 7), lane = (int)((o >> 3) & 63);
+        const size_t q = o >> 9;
+        const int t = (int)(q % 9), c = (int)((q / 9) % nch), blk = (int)(q / 9 / nch);
+        const int row = blk * 16 + (lane & 15), kg = lane >> 4;
+        out[o] = w[(size_t)row * 9 * cin + t * cin + c * 32 + kg * 8 + e];
+    }
+}
+
 // planes (rgb, albedo, normal: 3 x P; depth: P) -> 16-channel BF16 pixels
 __global__ __launch_bounds__(kBlock) void k_pack(const float* rgb, const float* alb, const float* nrm, const float* dep,
                                                  int P, uint16_t* x) {
@@ -609,6 +666,7 @@ struct Layer {
     size_t wOff, bOff;       // into the master parameter array
     int coutK, rowsF, rowsB;
     uint16_t *wf, *wb;       // BF16 forward / input-gradient layouts
+    uint16_t *wfF, *wfB;     // the same in MFMA fragment order (3x3 layers with 32-channel chunks)
 };
 
 static inline int rup(int x, int m) { return (x + m - 1) / m * m; }
@@ -644,37 +702,45 @@ static void launch_conv(const ConvArgs& a, int P, int n, hipStream_t s) {
     }
 }
 
-// 3x3 convolution, cin a multiple of 32. Outputs of 64 channels or more go
-// through the LDS-staged kernel (128-pixel row tiles x 64 output channels;
-// weight rows are allocated up to a multiple of 64, so the staged weight rows
-// of the last channel tile are in bounds). 32-channel outputs of a 32-channel
-// input keep k_conv: with a single channel chunk per tile the staged kernel's
-// load phase is not hidden behind its few MFMAs and it measured slower on
-// MI355X. 32-channel outputs of wider inputs (the decoder's full-resolution
-// concatenation, 64 channels, 265 MB at 1080p: more than the MALL holds, so
-// k_conv's per-tap re-reads go to HBM) use the staged kernel with a 128 x 32
-// tile: 2 % faster per 1080p frame.
+// 3x3 convolution, cin a multiple of 32, n output channels (a multiple of
+// 32; weight rows are allocated up to a multiple of 64, and up to 128 where n
+// is). Tile per workgroup, as measured fastest per layer at 1920x1080 on
+// MI355X: 128-channel multiples take 8 rows x 16 pixels x 128 channels (each
+// wave 32 channels; no two waves share weights, so nothing is gained by
+// staging them); 64-channel multiples with inputs of 128 channels or more
+// 8 rows x 32 pixels x 64 channels; the rest 4 rows x 64 pixels x 32
+// channels at three waves per SIMD (the full-resolution layers, where the
+// smaller accumulator set buys occupancy), with the weights staged once per
+// workgroup in LDS when there is more than one channel chunk (all four waves
+// use the same 32 channels; 12 % faster on the 96-channel decoder layer).
 static void launch_conv3(const ConvArgs& a, int n, hipStream_t s) {
-    if (n <= 32) {
-        if (a.cin > 32) {
-            dim3 g32(((a.W + 127) / 128) * (a.B * a.H), 1);
-            hipLaunchKernelGGL((k_conv3<2, 2, 4, 1>), g32, dim3(kBlock), 0, s, a);
-            return;
-        }
-        launch_conv<9, 32, 2>(a, a.B * a.H * a.W, n, s);
-        return;
-    }
-    dim3 grid(((a.W + 127) / 128) * (a.B * a.H), (n + 63) / 64);
-    hipLaunchKernelGGL((k_conv3<4, 2, 2, 2>), grid, dim3(kBlock), 0, s, a);
+    auto tiles = [&](int R, int TX) { return ((a.W + TX - 1) / TX) * ((a.H + R - 1) / R) * a.B; };
+    if (n % 128 == 0)
+        hipLaunchKernelGGL((k_conv3r<8, 1, 4, 2, 3, false>), dim3(tiles(8, 16), n / 128), dim3(kBlock), 0, s, a);
+    else if (n % 64 == 0 && a.cin >= 128)
+        hipLaunchKernelGGL((k_conv3r<8, 2, 2, 2, 3, false>), dim3(tiles(8, 32), n / 64), dim3(kBlock), 0, s, a);
+    else if (a.cin > 32)
+        hipLaunchKernelGGL((k_conv3r<4, 4, 1, 3, 2, true>), dim3(tiles(4, 64), (n + 31) / 32), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_conv3r<4, 4, 1, 3, 2, false>), dim3(tiles(4, 64), (n + 31) / 32), dim3(kBlock), 0, s, a);
 }
 
+// Optional fusions (3x3 layers with 32-channel chunks): `up` supplies input
+// channels [0, up.cup) 2x upsampled from a half-resolution tensor; `yp`
+// receives the 2x2 max-pooled output.
+struct UpSrc { const uint16_t* x = nullptr; int ld = 0, cup = 0; };
 static void conv_fwd(Net& N, int li, const uint16_t* x, int ldx, uint16_t* y, int ldy, int B, int H, int W, bool relu,
-                     float* yf = nullptr, int ldyf = 0) {
+                     float* yf = nullptr, int ldyf = 0, UpSrc up = UpSrc(), uint16_t* yp = nullptr, int ldyp = 0) {
     const Layer& L = N.L[li];
-    ConvArgs a{x, ldx, L.d.cin, L.wf, N.w + L.bOff, y, ldy, yf, ldyf, L.d.cout, B, H, W, relu ? 1 : 0};
+    ConvArgs a{x, ldx, L.d.cin, L.wf, L.wfF, N.w + L.bOff, y, ldy, yf, ldyf, L.d.cout, B, H, W, relu ? 1 : 0,
+               up.x, up.ld, up.cup, yp, ldyp};
     const int P = B * H * W;
     const int WN = L.d.cout >= 64 ? 4 : (L.d.cout >= 32 ? 2 : 1);
     hipStream_t s = N.stream;
+    if ((L.d.taps == 1 || L.d.cin % 32) && (up.x || yp)) {
+        rt::set_error("denoiser: fused up/pool on a layer without 32-channel chunks");
+        return;
+    }
     if (L.d.taps == 1) launch_conv<1, 32, 1>(a, P, L.d.cout, s);
     else if (L.d.cin % 32) {
         if (WN == 2) launch_conv<9, 16, 2>(a, P, L.d.cout, s);
@@ -685,7 +751,7 @@ static void conv_fwd(Net& N, int li, const uint16_t* x, int ldx, uint16_t* y, in
 // input gradient: dx = conv(g, flipped transposed weights)
 static void conv_dx(Net& N, int li, const uint16_t* g, int ldg, uint16_t* dx, int lddx, int B, int H, int W) {
     const Layer& L = N.L[li];
-    ConvArgs a{g, ldg, L.coutK, L.wb, nullptr, dx, lddx, nullptr, 0, L.d.cin, B, H, W, 0};
+    ConvArgs a{g, ldg, L.coutK, L.wb, L.wfB, nullptr, dx, lddx, nullptr, 0, L.d.cin, B, H, W, 0};
     const int P = B * H * W;
     const int WN = L.d.cin >= 64 ? 4 : 2;
     hipStream_t s = N.stream;
@@ -717,10 +783,6 @@ static void conv_dw(Net& N, int li, const uint16_t* g, int ldg, const uint16_t* 
 static void relu_bwd(Net& N, uint16_t* g, int ldg, const uint16_t* y, int ldy, int c, int P) {
     hipLaunchKernelGGL(k_relu_bwd, dim3(grid_for((size_t)P * c / 8)), dim3(kBlock), 0, N.stream, g, ldg, y, ldy, c, P);
 }
-static void pool(Net& N, const uint16_t* in, int ldi, uint16_t* out, int ldo, int c, int B, int H, int W) {
-    hipLaunchKernelGGL(k_pool, dim3(grid_for((size_t)B * H * W / 4 * c / 8)), dim3(kBlock), 0, N.stream, in, ldi, out,
-                       ldo, c, B, H, W);
-}
 static void pool_bwd(Net& N, const uint16_t* in, int ldi, const uint16_t* go, int ldgo, uint16_t* gi, int ldgi, int c,
                      int B, int H, int W) {
     hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for((size_t)B * H * W / 4 * c / 8)), dim3(kBlock), 0, N.stream, in, ldi,
@@ -741,31 +803,40 @@ static void update_layouts(Net& N) {
         const size_t n = (size_t)L.d.cout * L.d.taps * L.d.cin;
         hipLaunchKernelGGL(k_wlayout, dim3(grid_for(n)), dim3(kBlock), 0, N.stream, N.w + L.wOff, L.d.cout, L.d.taps,
                            L.d.cin, L.wf, L.wb, L.coutK);
+        if (L.wfF)
+            hipLaunchKernelGGL(k_wfrag, dim3(grid_for((size_t)L.rowsF * 9 * L.d.cin)), dim3(kBlock), 0, N.stream, L.wf,
+                               L.rowsF, L.d.cin, L.wfF);
+        if (L.wfB)
+            hipLaunchKernelGGL(k_wfrag, dim3(grid_for((size_t)L.rowsB * 9 * L.coutK)), dim3(kBlock), 0, N.stream,
+                               L.wb, L.rowsB, L.coutK, L.wfB);
     }
 }
 
 // forward from N.X (B x H x W x 16) to N.head (P x 3 FP32, before the residual)
-static void forward(Net& N, int B, int H, int W) {
+// forward from N.X (B x H x W x 16) to N.head (P x 3 FP32, before the
+// residual). The pooled encoder outputs come from the producing
+// convolutions' epilogues; the decoder convolutions read their upsampled
+// inputs straight from the half-resolution tensors. Training also stores the
+// upsampled channels in the concatenation buffers (the weight gradients read
+// them).
+static void forward(Net& N, int B, int H, int W, bool train) {
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
     conv_fwd(N, 0, N.X, 16, N.A0, 32, B, H, W, true);
-    conv_fwd(N, 1, N.A0, 32, N.CAT1 + 64, 96, B, H, W, true);             // E1
-    pool(N, N.CAT1 + 64, 96, N.P1, 32, 32, B, H, W);
+    conv_fwd(N, 1, N.A0, 32, N.CAT1 + 64, 96, B, H, W, true, nullptr, 0, UpSrc(), N.P1, 32);        // E1, P1
     conv_fwd(N, 2, N.P1, 32, N.A2, 64, B, H1, W1, true);
-    conv_fwd(N, 3, N.A2, 64, N.CAT2 + 128, 192, B, H1, W1, true);         // E2
-    pool(N, N.CAT2 + 128, 192, N.P2, 64, 64, B, H1, W1);
+    conv_fwd(N, 3, N.A2, 64, N.CAT2 + 128, 192, B, H1, W1, true, nullptr, 0, UpSrc(), N.P2, 64);    // E2, P2
     conv_fwd(N, 4, N.P2, 64, N.A4, 128, B, H2, W2, true);
-    conv_fwd(N, 5, N.A4, 128, N.CAT3 + 256, 384, B, H2, W2, true);        // E3
-    pool(N, N.CAT3 + 256, 384, N.P3, 128, 128, B, H2, W2);
+    conv_fwd(N, 5, N.A4, 128, N.CAT3 + 256, 384, B, H2, W2, true, nullptr, 0, UpSrc(), N.P3, 128);  // E3, P3
     conv_fwd(N, 6, N.P3, 128, N.A6, 256, B, H3, W3, true);
     conv_fwd(N, 7, N.A6, 256, N.E4, 256, B, H3, W3, true);
-    up(N, N.E4, 256, N.CAT3, 384, 256, B, H3, W3);
-    conv_fwd(N, 8, N.CAT3, 384, N.A8, 128, B, H2, W2, true);
+    if (train) up(N, N.E4, 256, N.CAT3, 384, 256, B, H3, W3);
+    conv_fwd(N, 8, N.CAT3, 384, N.A8, 128, B, H2, W2, true, nullptr, 0, UpSrc{N.E4, 256, 256});
     conv_fwd(N, 9, N.A8, 128, N.D3, 128, B, H2, W2, true);
-    up(N, N.D3, 128, N.CAT2, 192, 128, B, H2, W2);
-    conv_fwd(N, 10, N.CAT2, 192, N.A10, 64, B, H1, W1, true);
+    if (train) up(N, N.D3, 128, N.CAT2, 192, 128, B, H2, W2);
+    conv_fwd(N, 10, N.CAT2, 192, N.A10, 64, B, H1, W1, true, nullptr, 0, UpSrc{N.D3, 128, 128});
     conv_fwd(N, 11, N.A10, 64, N.D2, 64, B, H1, W1, true);
-    up(N, N.D2, 64, N.CAT1, 96, 64, B, H1, W1);
-    conv_fwd(N, 12, N.CAT1, 96, N.A12, 32, B, H, W, true);
+    if (train) up(N, N.D2, 64, N.CAT1, 96, 64, B, H1, W1);
+    conv_fwd(N, 12, N.CAT1, 96, N.A12, 32, B, H, W, true, nullptr, 0, UpSrc{N.D2, 64, 64});
     conv_fwd(N, 13, N.A12, 32, N.D1, 32, B, H, W, true);
     conv_fwd(N, 14, N.D1, 32, nullptr, 0, B, H, W, false, N.head, 3);
 }
@@ -889,6 +960,8 @@ CudaDenoiser_t cudaBVH_denoiserCreate(int maxPixels, uint64_t seed) {
         L.rowsF = dn::rup(L.d.cout, 64);
         L.rowsB = dn::rup(L.d.cin, 64);
         wbytes += ((size_t)L.rowsF * L.d.taps * L.d.cin + (size_t)L.rowsB * L.d.taps * L.coutK) * 2;
+        if (L.d.taps == 9 && L.d.cin % 32 == 0) wbytes += (size_t)L.rowsF * 9 * L.d.cin * 2;
+        if (L.d.taps == 9 && L.coutK % 32 == 0) wbytes += (size_t)L.rowsB * 9 * L.coutK * 2;
     }
     N.nParams = off;
     const size_t P = (size_t)maxPixels;
@@ -927,6 +1000,9 @@ CudaDenoiser_t cudaBVH_denoiserCreate(int maxPixels, uint64_t seed) {
         dn::Layer& L = N.L[i];
         L.wf = wp; wp += (size_t)L.rowsF * L.d.taps * L.d.cin;
         L.wb = wp; wp += (size_t)L.rowsB * L.d.taps * L.coutK;
+        L.wfF = L.wfB = nullptr;
+        if (L.d.taps == 9 && L.d.cin % 32 == 0) { L.wfF = wp; wp += (size_t)L.rowsF * 9 * L.d.cin; }
+        if (L.d.taps == 9 && L.coutK % 32 == 0) { L.wfB = wp; wp += (size_t)L.rowsB * 9 * L.coutK; }
     }
     std::vector<float> w(N.nParams, 0.0f);
     dn::init_params(w, N, seed);
@@ -1008,7 +1084,7 @@ int cudaBVH_denoiserForward(CudaDenoiser_t d, int batch, int height, int width, 
         rt::set_error("cudaBVH_denoiserForward: input copy failed");
         return -1;
     }
-    dn::forward(N, batch, height, width);
+    dn::forward(N, batch, height, width, false);
     if (headOut && hipMemcpyAsync(headOut, N.head, P * 12, hipMemcpyDeviceToDevice, N.stream) != hipSuccess) {
         rt::set_error("cudaBVH_denoiserForward: output copy failed");
         return -1;
@@ -1024,7 +1100,7 @@ int cudaBVH_denoise(CudaDenoiser_t d, int width, int height, const float* rgb, c
     const int P = width * height;
     hipLaunchKernelGGL(dn::k_pack, dim3(dn::grid_for(P)), dim3(dn::kBlock), 0, N.stream, rgb, albedo, normal, depth, P,
                        N.X);
-    dn::forward(N, 1, height, width);
+    dn::forward(N, 1, height, width, false);
     hipLaunchKernelGGL(dn::k_unpack, dim3(dn::grid_for(P)), dim3(dn::kBlock), 0, N.stream, N.head, N.X, P, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -1041,7 +1117,7 @@ int cudaBVH_denoiserTrainStep(CudaDenoiser_t d, int batch, int height, int width
         rt::set_error("cudaBVH_denoiserTrainStep: input copy failed");
         return -1;
     }
-    dn::forward(N, batch, height, width);
+    dn::forward(N, batch, height, width, true);
     const int lb = std::min(dn::grid_for(P), 8192);
     hipLaunchKernelGGL(dn::k_loss, dim3(lb), dim3(dn::kBlock), 0, N.stream, N.head, N.X, N.target, P, N.gHead,
                        N.lossPart);
