@@ -59,6 +59,12 @@ struct ConvArgs {
     // optional 2x2 max-pooled copy of the BF16 output goes to yp (stride ldyp)
     const uint16_t* xu; int ldxu; int cup;
     uint16_t* yp; int ldyp;
+    // k_conv3r with 32 output channels per workgroup only: the 1x1 head
+    // (3 x 32 BF16 weights hw, bias hbias) applied to the BF16-rounded output,
+    // written as FP32 head values (head, P x 3; nullable) and/or as denoised
+    // planes rgbOut[c][p] = max(expm1(head + residual), 0) (nullable), the
+    // residual being channel c of the 16-channel BF16 input pixels xres
+    const uint16_t* hw; const float* hbias; float* head; const uint16_t* xres; float* rgbOut;
 };
 
 // Block: 4 waves x (16*WM) pixels; each wave (16*WM) pixels x (16*WN)
@@ -194,14 +200,18 @@ __device__ __forceinline__ void halo_row_order() {
     }
 }
 
-template <int R, int WGX, int WGC, int OCC, int PD, bool BL>
+template <int R, int WGX, int WGC, int OCC, int PD, bool BL, int CS = 32, bool HD = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) void k_conv3r(ConvArgs a) {
     static_assert(WGX * WGC == kBlock / 64, "four waves");
     static_assert(!BL || WGC == 1, "weights are shared through LDS only when every wave has the same channels");
+    static_assert(CS == 32 || (CS == 16 && !BL), "32-channel chunks, or one 16-channel chunk");
+    static_assert(!HD || WGC == 1, "the head needs all 32 channels of a pixel in one wave");
     constexpr int TX = 16 * WGX, HX = TX + 2, HY = R + 2, HP = HX * HY;
-    constexpr int NPC = HP * 4;                                // 16-byte pieces per chunk
+    constexpr int QP = CS / 8;                                 // 16-byte pieces per halo pixel
+    constexpr int NPC = HP * QP;                               // 16-byte pieces per chunk
     constexpr int NA = (NPC + kBlock - 1) / kBlock;
-    __shared__ uint4 hs[HP * 4];
+    constexpr int FPB = CS == 32 ? 9 : 6;                      // B fragments per 16-channel block and chunk
+    __shared__ uint4 hs[HP * QP];
     __shared__ uint4 hb[BL ? 18 * 64 : 1];          // BL: the chunk's 18 B fragments (2 tiles x 9 taps)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wx = wave % WGX, wc = wave / WGX;
@@ -211,21 +221,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) v
     const int ty = tq % tilesY, bb = tq / tilesY;
     const int x0 = tx * TX, y0 = ty * R;
     const int n0 = blockIdx.y * (32 * WGC) + wc * 32;
-    // B fragments: a.wfrag is [16-channel block][chunk][tap][lane][8], so a
-    // wave's fragment is 1 KB of consecutive memory
-    const int nch = a.cin / 32;
-    const uint16_t* wf0 = a.wfrag + (size_t)(n0 / 16) * nch * 9 * 512 + lane * 8;
-    const uint16_t* wf1 = wf0 + (size_t)nch * 9 * 512;
+    // B fragments: a.wfrag is [16-channel block][chunk][fragment][lane][8]
+    // (fragment = tap for 32-channel chunks, (dy, tap pair) for the 16-channel
+    // input), so a wave's fragment is 1 KB of consecutive memory
+    const int nch = a.cin / CS;
+    const uint16_t* wf0 = a.wfrag + (size_t)(n0 / 16) * nch * FPB * 512 + lane * 8;
+    const uint16_t* wf1 = wf0 + (size_t)nch * FPB * 512;
 
     // halo piece k of this thread: pixel (y0 - 1 + hy, x0 - 1 + hx), quarter
-    // e & 3 of the chunk; zero outside the image. Channels below cup come
+    // e % QP of the chunk; zero outside the image. Channels below cup come
     // from the half-resolution xu at (y / 2, x / 2).
     uint4 ra[NA];
     auto fetch = [&](int c0) {
         const bool up = c0 < a.cup;
 #pragma unroll
         for (int k = 0; k < NA; k++) {
-            const int e = tid + k * kBlock, hp = e >> 2;
+            const int e = tid + k * kBlock, hp = e / QP;
             const int hy = hp / HX, hx = hp - hy * HX;
             const int y = y0 - 1 + hy, x = x0 - 1 + hx;
             ra[k] = make_uint4(0, 0, 0, 0);
@@ -233,7 +244,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) v
                 const uint16_t* src =
                     up ? a.xu + (size_t)((bb * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1)) * a.ldxu
                        : a.x + (size_t)((bb * a.H + y) * a.W + x) * a.ldx;
-                ra[k] = *(const uint4*)(src + c0 + 8 * (e & 3));
+                ra[k] = *(const uint4*)(src + c0 + 8 * (e % QP));
             }
         }
     };
@@ -244,7 +255,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) v
                 b[dy][0] = __builtin_bit_cast(bf16x8, hb[(dy * 3 + dx) * 64 + lane]);
                 b[dy][1] = __builtin_bit_cast(bf16x8, hb[(9 + dy * 3 + dx) * 64 + lane]);
             } else {
-                const int o = ((c0 >> 5) * 9 + dy * 3 + dx) * 512;
+                const int o = ((c0 / CS) * FPB + dy * (FPB / 3) + dx) * 512;
                 b[dy][0] = __builtin_bit_cast(bf16x8, *(const uint4*)(wf0 + o));
                 b[dy][1] = __builtin_bit_cast(bf16x8, *(const uint4*)(wf1 + o));
             }
@@ -261,8 +272,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) v
         }
     };
     f32x4 acc[R][2];
-#pragma unroll
-    for (int r = 0; r < R; r++) acc[r][0] = acc[r][1] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     // One horizontal tap: the MFMAs of dx with the fragments in bc, after
     // issuing the loads of the next step's fragments into bn (kept ahead of
@@ -270,13 +279,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) v
     auto dxstep = [&](const bf16x8 (&bc)[3][2], bf16x8 (&bn)[3][2], int dx, bool next, int nc0, int ndx) {
         if (next) bload(bn, nc0, ndx);
         __builtin_amdgcn_sched_barrier(0);
-        const int hp0 = wx * 16 + row + dx;
-        const int s0 = hp0 * 4 + (kg ^ (hp0 & 2)), s1 = hp0 * 4 + (kg ^ (hp0 & 2) ^ 2);
+        // 32-channel chunks: lane (row, kg) reads quarter kg of pixel row + dx.
+        // The 16-channel input pairs horizontal taps in one K = 32 step:
+        // lanes kg = 0, 1 read pixel row + 2 dx (tap 2 dx), kg = 2, 3 pixel
+        // row + 2 dx + 1 (tap 2 dx + 1). For dx = 1 that is tap 3, outside
+        // the filter: those lanes read tap 2's pixel (in bounds) and use zero
+        // (zero weights alone would turn an infinite input into NaN). Rows of
+        // 32 bytes need no swizzle.
+        const bool pad = CS == 16 && dx == 1 && kg >= 2;
+        const int hp0 = CS == 32 ? wx * 16 + row + dx : wx * 16 + row + 2 * dx + (pad ? 0 : kg >> 1);
+        const int s0 = CS == 32 ? hp0 * 4 + (kg ^ (hp0 & 2)) : hp0 * 2 + (kg & 1);
+        const int s1 = CS == 32 ? hp0 * 4 + (kg ^ (hp0 & 2) ^ 2) : s0;
 #pragma unroll
         for (int hy = 0; hy < HY; hy++) {
             // (hp0 + hy * HX) & 2 flips with bit 1 of hy * HX (HX is even)
-            const int slot = hy * HX * 4 + (((hy * HX) & 2) ? s1 : s0);
-            const bf16x8 av = __builtin_bit_cast(bf16x8, hs[slot]);
+            const int slot = hy * HX * QP + (((hy * HX) & 2) ? s1 : s0);
+            const bf16x8 av = __builtin_bit_cast(bf16x8, pad ? make_uint4(0, 0, 0, 0) : hs[slot]);
 #pragma unroll
             for (int dy = 0; dy < 3; dy++) {
                 const int ro = hy - dy;
@@ -291,78 +309,116 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) v
         halo_row_order<R, PD, 0>();
     };
     // One channel chunk: stage its halo, start the next chunk's halo loads,
-    // run the three taps. The fragments alternate between two register sets
-    // (x: dx 0 and 2, y: dx 1 and the next chunk's dx 0), so no copy waits
-    // for a load in flight; chunks alternate the roles of the two sets.
+    // run the taps. The fragments alternate between two register sets (x: dx
+    // 0 and 2, y: dx 1 and the next chunk's dx 0), so no copy waits for a
+    // load in flight; chunks alternate the roles of the two sets.
     auto chunk = [&](int c0, bf16x8 (&x)[3][2], bf16x8 (&y)[3][2]) {
         __syncthreads();               // the previous chunk's halo (and weights) are no longer read
         if constexpr (BL) bstage(c0);
 #pragma unroll
         for (int k = 0; k < NA; k++) {
             const int e = tid + k * kBlock, hp = e >> 2;
-            if (e < NPC) hs[hp * 4 + ((e & 3) ^ (hp & 2))] = ra[k];
+            if (e < NPC) hs[CS == 32 ? hp * 4 + ((e & 3) ^ (hp & 2)) : e] = ra[k];
         }
         if constexpr (BL) __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wave's weight copies landed
         __syncthreads();
-        const bool more = c0 + 32 < a.cin;
-        if (more) fetch(c0 + 32);
+        const bool more = c0 + CS < a.cin;
+        if (more) fetch(c0 + CS);
         if constexpr (BL) bload(x, c0, 0);
-        dxstep(x, y, 0, true, c0, 1);
-        dxstep(y, x, 1, true, c0, 2);
-        dxstep(x, y, 2, more && !BL, c0 + 32, 0);
+        if constexpr (CS == 32) {
+            dxstep(x, y, 0, true, c0, 1);
+            dxstep(y, x, 1, true, c0, 2);
+            dxstep(x, y, 2, more && !BL, c0 + 32, 0);
+        } else {
+            dxstep(x, y, 0, true, c0, 1);     // a single 16-channel chunk
+            dxstep(y, x, 1, false, 0, 0);
+        }
+    };
+    // Epilogue of one R-row tile.
+    auto store_tile = [&]() {
+        // The weights are the MFMA's A operand and the pixels its B operand, so
+        // D is channels x pixels: lane (pixel l % 16, k = l / 16) holds channels
+        // 4k .. 4k + 3 of its pixel, stored as one 8-byte write.
+        // With yp, the 2x2 maximum of each output row pair (r, r + 1; y0 and R
+        // are even) and lane pair (l, l ^ 1: pixels 2m, 2m + 1) is written too:
+        // rounding to BF16 is monotonic, so this equals pooling the BF16 output.
+        const int x = x0 + wx * 16 + row;
+        float pool[2][4];
+        // HD: the head as one MFMA per output row. K slots 8k .. 8k + 7 of lane
+        // group k hold channels 4k .. 4k + 3 and 16 + 4k .. 16 + 4k + 3, which is
+        // what a lane of D already holds (tiles j = 0, 1); the A operand row o
+        // holds head output o's weights for the same channels (rows 3-15 zero).
+        bf16x8 hwa;
+        if constexpr (HD) {
+            uint2 w0 = make_uint2(0, 0), w1 = make_uint2(0, 0);
+            if (row < 3) {
+                w0 = *(const uint2*)(a.hw + row * 32 + 4 * kg);
+                w1 = *(const uint2*)(a.hw + row * 32 + 16 + 4 * kg);
+            }
+            hwa = __builtin_bit_cast(bf16x8, make_uint4(w0.x, w0.y, w1.x, w1.y));
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int y = y0 + r;
+            const size_t p = (size_t)(bb * a.H + y) * a.W + x;
+            float v[2][4];
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int col = n0 + j * 16 + 4 * kg;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    v[j][i] = acc[r][j][i] + (a.bias && col < a.n ? a.bias[col + i] : 0.0f);
+                    if (a.relu) v[j][i] = fmaxf(v[j][i], 0.0f);
+                }
+                const bool ok = y < a.H && x < a.W && col < a.n;
+                if (ok && a.y)
+                    *(uint2*)(a.y + p * a.ldy + col) = make_uint2(f2bf(v[j][0]) | ((uint32_t)f2bf(v[j][1]) << 16),
+                                                                  f2bf(v[j][2]) | ((uint32_t)f2bf(v[j][3]) << 16));
+                if (ok && a.yf) *(float4*)(a.yf + p * a.ldyf + col) = make_float4(v[j][0], v[j][1], v[j][2], v[j][3]);
+                if constexpr (R % 2 == 0) {
+                    if (a.yp) {
+                        // rows r - 1 and r are a pair when r is odd: keep the even row's values
+                        float m[4];
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const float h = fmaxf(v[j][i], __shfl_xor(v[j][i], 1));
+                            m[i] = (r & 1) ? fmaxf(h, pool[j][i]) : h;
+                            pool[j][i] = m[i];
+                        }
+                        if ((r & 1) && ok && !(row & 1)) {
+                            const size_t q = (size_t)(bb * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
+                            *(uint2*)(a.yp + q * a.ldyp + col) = make_uint2(f2bf(m[0]) | ((uint32_t)f2bf(m[1]) << 16),
+                                                                            f2bf(m[2]) | ((uint32_t)f2bf(m[3]) << 16));
+                        }
+                    }
+                }
+            }
+            if constexpr (HD) {
+                const bf16x8 act = __builtin_bit_cast(
+                    bf16x8, make_uint4(f2bf(v[0][0]) | ((uint32_t)f2bf(v[0][1]) << 16), f2bf(v[0][2]) | ((uint32_t)f2bf(v[0][3]) << 16),
+                                       f2bf(v[1][0]) | ((uint32_t)f2bf(v[1][1]) << 16), f2bf(v[1][2]) | ((uint32_t)f2bf(v[1][3]) << 16)));
+                const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hwa, act, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                // outputs 0-2 of pixel `row` sit in lane row (items 0-2): lane group k < 3 takes output k
+                const float h0 = __shfl(d[0], row), h1 = __shfl(d[1], row), h2 = __shfl(d[2], row);
+                if (kg < 3 && y < a.H && x < a.W) {
+                    const float hv = (kg == 0 ? h0 : kg == 1 ? h1 : h2) + a.hbias[kg];
+                    if (a.head) a.head[p * 3 + kg] = hv;
+                    if (a.rgbOut)
+                        a.rgbOut[(size_t)kg * a.B * a.H * a.W + p] = fmaxf(expm1f(hv + bf2f(a.xres[p * 16 + kg])), 0.0f);
+                }
+            }
+        }
     };
     bf16x8 b0[3][2], b1[3][2];
     fetch(0);
     if constexpr (!BL) bload(b0, 0, 0);
-    for (int c0 = 0; c0 < a.cin; c0 += 64) {
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r][0] = acc[r][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < a.cin; c0 += 2 * CS) {
         chunk(c0, b0, b1);
-        if (c0 + 32 < a.cin) chunk(c0 + 32, b1, b0);
+        if (c0 + CS < a.cin) chunk(c0 + CS, b1, b0);
     }
-    // The weights are the MFMA's A operand and the pixels its B operand, so
-    // D is channels x pixels: lane (pixel l % 16, k = l / 16) holds channels
-    // 4k .. 4k + 3 of its pixel, stored as one 8-byte write.
-    // With yp, the 2x2 maximum of each output row pair (r, r + 1; y0 and R
-    // are even) and lane pair (l, l ^ 1: pixels 2m, 2m + 1) is written too:
-    // rounding to BF16 is monotonic, so this equals pooling the BF16 output.
-    const int x = x0 + wx * 16 + row;
-    float pool[2][4];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int y = y0 + r;
-        const size_t p = (size_t)(bb * a.H + y) * a.W + x;
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int col = n0 + j * 16 + 4 * kg;
-            float v[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                v[i] = acc[r][j][i] + (a.bias && col < a.n ? a.bias[col + i] : 0.0f);
-                if (a.relu) v[i] = fmaxf(v[i], 0.0f);
-            }
-            const bool ok = y < a.H && x < a.W && col < a.n;
-            if (ok && a.y)
-                *(uint2*)(a.y + p * a.ldy + col) =
-                    make_uint2(f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16), f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
-            if (ok && a.yf) *(float4*)(a.yf + p * a.ldyf + col) = make_float4(v[0], v[1], v[2], v[3]);
-            if constexpr (R % 2 == 0) {
-                if (a.yp) {
-                    // rows r - 1 and r are a pair when r is odd: keep the even row's values
-                    float m[4];
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const float h = fmaxf(v[i], __shfl_xor(v[i], 1));
-                        m[i] = (r & 1) ? fmaxf(h, pool[j][i]) : h;
-                        pool[j][i] = m[i];
-                    }
-                    if ((r & 1) && ok && !(row & 1)) {
-                        const size_t q = (size_t)(bb * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
-                        *(uint2*)(a.yp + q * a.ldyp + col) = make_uint2(f2bf(m[0]) | ((uint32_t)f2bf(m[1]) << 16),
-                                                                        f2bf(m[2]) | ((uint32_t)f2bf(m[3]) << 16));
-                    }
-                }
-            }
-        }
-    }
+    store_tile();
 }
 
 // --------------------------------------------------------------- wgrad ----
@@ -619,6 +675,21 @@ __global__ __launch_bounds__(kBlock) void k_wfrag(const uint16_t* w, int rows, i
     }
 }
 
+// [rows][9*16] (3x3 weights of a 16-channel input) -> fragment order
+// [rows/16][3 dy][2 tap pairs][lane][8]: lane = row%16 + 16*kg holds input
+// channels 8 (kg & 1) .. +7 of tap (dy, 2 pair + (kg >> 1)); tap column 3
+// (the second half of the last pair) is zero.
+__global__ __launch_bounds__(kBlock) void k_wfrag16(const uint16_t* w, int rows, uint16_t* out) {
+    const size_t n = (size_t)rows * 6 * 64;
+    for (size_t o = blockIdx.x * (size_t)kBlock + threadIdx.x; o < n; o += (size_t)gridDim.x * kBlock) {
+        const int e = (int)(o & 7), lane = (int)((o >> 3) & 63);
+        const size_t q = o >> 9;
+        const int pr = (int)(q % 2), dy = (int)((q / 2) % 3), blk = (int)(q / 6);
+        const int row = blk * 16 + (lane & 15), kg = lane >> 4, dx = 2 * pr + (kg >> 1);
+        out[o] = dx < 3 ? w[(size_t)row * 144 + (dy * 3 + dx) * 16 + 8 * (kg & 1) + e] : (uint16_t)0;
+    }
+}
+
 // planes (rgb, albedo, normal: 3 x P; depth: P) -> 16-channel BF16 pixels
 __global__ __launch_bounds__(kBlock) void k_pack(const float* rgb, const float* alb, const float* nrm, const float* dep,
                                                  int P, uint16_t* x) {
@@ -637,15 +708,6 @@ __global__ __launch_bounds__(kBlock) void k_pack(const float* rgb, const float* 
         *(uint4*)(x + (size_t)p * 16) = o0;
         *(uint4*)(x + (size_t)p * 16 + 8) = o1;
     }
-}
-
-// head + log input -> radiance planes: exp(pred) - 1, clamped at 0
-__global__ __launch_bounds__(kBlock) void k_unpack(const float* head, const uint16_t* xin, int P, float* rgb) {
-    for (int p = blockIdx.x * kBlock + threadIdx.x; p < P; p += gridDim.x * kBlock)
-        for (int c = 0; c < 3; c++) {
-            const float pred = head[(size_t)p * 3 + c] + bf2f(xin[(size_t)p * 16 + c]);
-            rgb[(size_t)c * P + p] = fmaxf(expm1f(pred), 0.0f);
-        }
 }
 
 __global__ void k_fill_f32(float* p, size_t n, float v) {
@@ -713,36 +775,51 @@ static void launch_conv(const ConvArgs& a, int P, int n, hipStream_t s) {
 // smaller accumulator set buys occupancy), with the weights staged once per
 // workgroup in LDS when there is more than one channel chunk (all four waves
 // use the same 32 channels; 12 % faster on the 96-channel decoder layer).
+template <int R, int WGX, int WGC, int OCC, int PD, bool BL, int CS = 32, bool HD = false>
+static void launch_r(const ConvArgs& a, int chanTiles, hipStream_t s) {
+    constexpr int TX = 16 * WGX;
+    const dim3 grid(((a.W + TX - 1) / TX) * ((a.H + R - 1) / R) * a.B, chanTiles);
+    hipLaunchKernelGGL((k_conv3r<R, WGX, WGC, OCC, PD, BL, CS, HD>), grid, dim3(kBlock), 0, s, a);
+}
+
 static void launch_conv3(const ConvArgs& a, int n, hipStream_t s) {
-    auto tiles = [&](int R, int TX) { return ((a.W + TX - 1) / TX) * ((a.H + R - 1) / R) * a.B; };
-    if (n % 128 == 0)
-        hipLaunchKernelGGL((k_conv3r<8, 1, 4, 2, 3, false>), dim3(tiles(8, 16), n / 128), dim3(kBlock), 0, s, a);
-    else if (n % 64 == 0 && a.cin >= 128)
-        hipLaunchKernelGGL((k_conv3r<8, 2, 2, 2, 3, false>), dim3(tiles(8, 32), n / 64), dim3(kBlock), 0, s, a);
-    else if (a.cin > 32)
-        hipLaunchKernelGGL((k_conv3r<4, 4, 1, 3, 2, true>), dim3(tiles(4, 64), (n + 31) / 32), dim3(kBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_conv3r<4, 4, 1, 3, 2, false>), dim3(tiles(4, 64), (n + 31) / 32), dim3(kBlock), 0, s, a);
+    if (n % 128 == 0) launch_r<8, 1, 4, 2, 3, false>(a, n / 128, s);
+    else if (n % 64 == 0 && a.cin >= 128) launch_r<8, 2, 2, 2, 3, false>(a, n / 64, s);
+    else if (a.cin > 32) launch_r<4, 4, 1, 3, 2, true>(a, (n + 31) / 32, s);
+    else if (a.hw) launch_r<4, 4, 1, 2, 2, false, 32, true>(a, 1, s);
+    else launch_r<4, 4, 1, 3, 2, false>(a, (n + 31) / 32, s);
 }
 
 // Optional fusions (3x3 layers with 32-channel chunks): `up` supplies input
 // channels [0, up.cup) 2x upsampled from a half-resolution tensor; `yp`
 // receives the 2x2 max-pooled output.
+// `head` applies layer `head.li` (the 1x1 head) in the epilogue.
 struct UpSrc { const uint16_t* x = nullptr; int ld = 0, cup = 0; };
+struct HeadFuse { int li = -1; float* head = nullptr; const uint16_t* xres = nullptr; float* rgbOut = nullptr; };
 static void conv_fwd(Net& N, int li, const uint16_t* x, int ldx, uint16_t* y, int ldy, int B, int H, int W, bool relu,
-                     float* yf = nullptr, int ldyf = 0, UpSrc up = UpSrc(), uint16_t* yp = nullptr, int ldyp = 0) {
+                     float* yf = nullptr, int ldyf = 0, UpSrc up = UpSrc(), uint16_t* yp = nullptr, int ldyp = 0,
+                     HeadFuse hf = HeadFuse()) {
     const Layer& L = N.L[li];
+    const Layer* Lh = hf.li >= 0 ? &N.L[hf.li] : nullptr;
     ConvArgs a{x, ldx, L.d.cin, L.wf, L.wfF, N.w + L.bOff, y, ldy, yf, ldyf, L.d.cout, B, H, W, relu ? 1 : 0,
-               up.x, up.ld, up.cup, yp, ldyp};
+               up.x, up.ld, up.cup, yp, ldyp,
+               Lh ? Lh->wf : nullptr, Lh ? N.w + Lh->bOff : nullptr, hf.head, hf.xres, hf.rgbOut};
+    if (Lh && !(L.d.taps == 9 && L.d.cin == 32 && L.d.cout == 32 && Lh->d.cin == 32 && Lh->d.cout == 3)) {
+        rt::set_error("denoiser: head fusion needs a 32 -> 32 3x3 layer and a 32 -> 3 head");
+        return;
+    }
     const int P = B * H * W;
     const int WN = L.d.cout >= 64 ? 4 : (L.d.cout >= 32 ? 2 : 1);
     hipStream_t s = N.stream;
-    if ((L.d.taps == 1 || L.d.cin % 32) && (up.x || yp)) {
+    if ((L.d.taps == 1 || L.d.cin % 32) && (up.x || yp || Lh)) {
         rt::set_error("denoiser: fused up/pool on a layer without 32-channel chunks");
         return;
     }
     if (L.d.taps == 1) launch_conv<1, 32, 1>(a, P, L.d.cout, s);
-    else if (L.d.cin % 32) {
+    else if (L.d.cin == 16 && L.d.cout == 32) {
+        // the 16-channel input layer: one chunk, tap pairs per MFMA, 4 x 64 x 32 tiles
+        launch_r<4, 4, 1, 3, 2, false, 16>(a, 1, s);
+    } else if (L.d.cin % 32) {
         if (WN == 2) launch_conv<9, 16, 2>(a, P, L.d.cout, s);
         else launch_conv<9, 16, 4>(a, P, L.d.cout, s);
     } else launch_conv3(a, L.d.cout, s);
@@ -803,7 +880,10 @@ static void update_layouts(Net& N) {
         const size_t n = (size_t)L.d.cout * L.d.taps * L.d.cin;
         hipLaunchKernelGGL(k_wlayout, dim3(grid_for(n)), dim3(kBlock), 0, N.stream, N.w + L.wOff, L.d.cout, L.d.taps,
                            L.d.cin, L.wf, L.wb, L.coutK);
-        if (L.wfF)
+        if (L.wfF && L.d.cin == 16)
+            hipLaunchKernelGGL(k_wfrag16, dim3(grid_for((size_t)L.rowsF * 6 * 64)), dim3(kBlock), 0, N.stream, L.wf,
+                               L.rowsF, L.wfF);
+        else if (L.wfF)
             hipLaunchKernelGGL(k_wfrag, dim3(grid_for((size_t)L.rowsF * 9 * L.d.cin)), dim3(kBlock), 0, N.stream, L.wf,
                                L.rowsF, L.d.cin, L.wfF);
         if (L.wfB)
@@ -814,12 +894,12 @@ static void update_layouts(Net& N) {
 
 // forward from N.X (B x H x W x 16) to N.head (P x 3 FP32, before the residual)
 // forward from N.X (B x H x W x 16) to N.head (P x 3 FP32, before the
-// residual). The pooled encoder outputs come from the producing
+// residual), or in inference with rgbOut straight to the denoised planes. The pooled encoder outputs come from the producing
 // convolutions' epilogues; the decoder convolutions read their upsampled
 // inputs straight from the half-resolution tensors. Training also stores the
 // upsampled channels in the concatenation buffers (the weight gradients read
 // them).
-static void forward(Net& N, int B, int H, int W, bool train) {
+static void forward(Net& N, int B, int H, int W, bool train, float* rgbOut = nullptr) {
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
     conv_fwd(N, 0, N.X, 16, N.A0, 32, B, H, W, true);
     conv_fwd(N, 1, N.A0, 32, N.CAT1 + 64, 96, B, H, W, true, nullptr, 0, UpSrc(), N.P1, 32);        // E1, P1
@@ -837,8 +917,19 @@ static void forward(Net& N, int B, int H, int W, bool train) {
     conv_fwd(N, 11, N.A10, 64, N.D2, 64, B, H1, W1, true);
     if (train) up(N, N.D2, 64, N.CAT1, 96, 64, B, H1, W1);
     conv_fwd(N, 12, N.CAT1, 96, N.A12, 32, B, H, W, true, nullptr, 0, UpSrc{N.D2, 64, 64});
-    conv_fwd(N, 13, N.A12, 32, N.D1, 32, B, H, W, true);
-    conv_fwd(N, 14, N.D1, 32, nullptr, 0, B, H, W, false, N.head, 3);
+    if (train) {
+        conv_fwd(N, 13, N.A12, 32, N.D1, 32, B, H, W, true);
+        conv_fwd(N, 14, N.D1, 32, nullptr, 0, B, H, W, false, N.head, 3);
+    } else {
+        // inference: the head (and with rgbOut the residual and expm1) in the
+        // last convolution's epilogue; its 32-channel output is not stored
+        HeadFuse hf;
+        hf.li = 14;
+        hf.head = rgbOut ? nullptr : N.head;
+        hf.xres = N.X;
+        hf.rgbOut = rgbOut;
+        conv_fwd(N, 13, N.A12, 32, nullptr, 0, B, H, W, true, nullptr, 0, UpSrc(), nullptr, 0, hf);
+    }
 }
 
 // backward from N.gHead (P x 16, loss gradient in channels 0-2) to N.g
@@ -961,6 +1052,7 @@ CudaDenoiser_t cudaBVH_denoiserCreate(int maxPixels, uint64_t seed) {
         L.rowsB = dn::rup(L.d.cin, 64);
         wbytes += ((size_t)L.rowsF * L.d.taps * L.d.cin + (size_t)L.rowsB * L.d.taps * L.coutK) * 2;
         if (L.d.taps == 9 && L.d.cin % 32 == 0) wbytes += (size_t)L.rowsF * 9 * L.d.cin * 2;
+        if (L.d.taps == 9 && L.d.cin == 16) wbytes += (size_t)L.rowsF * 6 * 64 * 2;
         if (L.d.taps == 9 && L.coutK % 32 == 0) wbytes += (size_t)L.rowsB * 9 * L.coutK * 2;
     }
     N.nParams = off;
@@ -1002,6 +1094,7 @@ CudaDenoiser_t cudaBVH_denoiserCreate(int maxPixels, uint64_t seed) {
         L.wb = wp; wp += (size_t)L.rowsB * L.d.taps * L.coutK;
         L.wfF = L.wfB = nullptr;
         if (L.d.taps == 9 && L.d.cin % 32 == 0) { L.wfF = wp; wp += (size_t)L.rowsF * 9 * L.d.cin; }
+        if (L.d.taps == 9 && L.d.cin == 16) { L.wfF = wp; wp += (size_t)L.rowsF * 6 * 64; }
         if (L.d.taps == 9 && L.coutK % 32 == 0) { L.wfB = wp; wp += (size_t)L.rowsB * 9 * L.coutK; }
     }
     std::vector<float> w(N.nParams, 0.0f);
@@ -1100,8 +1193,7 @@ int cudaBVH_denoise(CudaDenoiser_t d, int width, int height, const float* rgb, c
     const int P = width * height;
     hipLaunchKernelGGL(dn::k_pack, dim3(dn::grid_for(P)), dim3(dn::kBlock), 0, N.stream, rgb, albedo, normal, depth, P,
                        N.X);
-    dn::forward(N, 1, height, width, false);
-    hipLaunchKernelGGL(dn::k_unpack, dim3(dn::grid_for(P)), dim3(dn::kBlock), 0, N.stream, N.head, N.X, P, out);
+    dn::forward(N, 1, height, width, false, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
