@@ -187,6 +187,12 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
 // the A fragment (lane = pixel row + 16 * quarter) this swizzle puts the eight
 // lanes of every phase on eight different 16-byte bank groups for any start
 // pixel, where the previous 80-byte padded rows gave two-way conflicts.
+// The value of lane l ^ 1 (a DPP quad permutation [1, 0, 3, 2]: no LDS
+// traffic, unlike a general lane shuffle).
+__device__ __forceinline__ float swap_pair(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
 // Scheduling groups of one halo row walk (R output rows): the MFMAs of row
 // hy (2 per output row that uses it: 2, 4, 6, ..., 6, 4, 2), then the LDS
 // read of row hy + 2.
@@ -346,14 +352,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) v
         float pool[2][4];
         // HD: the head as one MFMA per output row. K slots 8k .. 8k + 7 of lane
         // group k hold channels 4k .. 4k + 3 and 16 + 4k .. 16 + 4k + 3, which is
-        // what a lane of D already holds (tiles j = 0, 1); the A operand row o
-        // holds head output o's weights for the same channels (rows 3-15 zero).
+        // what a lane of D already holds (tiles j = 0, 1); the A operand row 4c
+        // holds head output c's weights for the same channels (other rows
+        // zero), so output c lands in item 0 of lane group c.
         bf16x8 hwa;
         if constexpr (HD) {
             uint2 w0 = make_uint2(0, 0), w1 = make_uint2(0, 0);
-            if (row < 3) {
-                w0 = *(const uint2*)(a.hw + row * 32 + 4 * kg);
-                w1 = *(const uint2*)(a.hw + row * 32 + 16 + 4 * kg);
+            if ((row & 3) == 0 && row < 12) {
+                const int c = row >> 2;
+                w0 = *(const uint2*)(a.hw + c * 32 + 4 * kg);
+                w1 = *(const uint2*)(a.hw + c * 32 + 16 + 4 * kg);
             }
             hwa = __builtin_bit_cast(bf16x8, make_uint4(w0.x, w0.y, w1.x, w1.y));
         }
@@ -381,7 +389,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) v
                         float m[4];
 #pragma unroll
                         for (int i = 0; i < 4; i++) {
-                            const float h = fmaxf(v[j][i], __shfl_xor(v[j][i], 1));
+                            const float h = fmaxf(v[j][i], swap_pair(v[j][i]));
                             m[i] = (r & 1) ? fmaxf(h, pool[j][i]) : h;
                             pool[j][i] = m[i];
                         }
@@ -398,10 +406,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) v
                     bf16x8, make_uint4(f2bf(v[0][0]) | ((uint32_t)f2bf(v[0][1]) << 16), f2bf(v[0][2]) | ((uint32_t)f2bf(v[0][3]) << 16),
                                        f2bf(v[1][0]) | ((uint32_t)f2bf(v[1][1]) << 16), f2bf(v[1][2]) | ((uint32_t)f2bf(v[1][3]) << 16)));
                 const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hwa, act, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-                // outputs 0-2 of pixel `row` sit in lane row (items 0-2): lane group k < 3 takes output k
-                const float h0 = __shfl(d[0], row), h1 = __shfl(d[1], row), h2 = __shfl(d[2], row);
                 if (kg < 3 && y < a.H && x < a.W) {
-                    const float hv = (kg == 0 ? h0 : kg == 1 ? h1 : h2) + a.hbias[kg];
+                    const float hv = d[0] + a.hbias[kg];
                     if (a.head) a.head[p * 3 + kg] = hv;
                     if (a.rgbOut)
                         a.rgbOut[(size_t)kg * a.B * a.H * a.W + p] = fmaxf(expm1f(hv + bf2f(a.xres[p * 16 + kg])), 0.0f);
