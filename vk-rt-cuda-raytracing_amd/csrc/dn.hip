@@ -10,15 +10,19 @@
 // albedo, normal, log(1+depth)/4, zero padding.
 //
 // Kernels:
-//   * k_conv: implicit GEMM (M = pixels, N = output channels, K = taps x
-//     input channels) on MFMA 16x16x32 (or 16x16x16 for 16-channel inputs)
-//     BF16 with FP32 accumulation; bias + ReLU fused; writes into a channel
-//     slice of a wider buffer, so skip concatenation costs nothing. The same
-//     kernel computes input gradients with the flipped, transposed weights.
+//   * k_conv3r: every 3x3 convolution (forward and input gradients), an
+//     implicit GEMM on MFMA 16x16x32 BF16 with FP32 accumulation over an LDS
+//     input halo with row reuse; bias + ReLU fused; writes into a channel
+//     slice of a wider buffer, so skip concatenation costs nothing; optional
+//     fusions: 2x2 max pooling of the output, 2x nearest upsampling of part
+//     of the input, and the 1x1 head with the residual and expm1.
+//   * k_conv: the generic implicit GEMM, kept for the 1x1 head in training
+//     and its input gradient.
 //   * k_wgrad: weight gradients, M = output channels, N = input channels,
 //     K = pixels; 32-pixel tiles staged through LDS transposed, split over
 //     pixel ranges into partials reduced in a fixed order (deterministic).
-//   * pooling / upsampling / ReLU-mask / loss / Adam / layout kernels.
+//   * upsampling (training), pooling / upsampling backward, ReLU-mask, loss,
+//     Adam and weight layout kernels.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -176,17 +180,21 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
 // three: vertical taps dy = 0, 1, 2), so a wave reads 3 (R + 2) A fragments
 // per chunk for 9 R x 2 MFMAs instead of 9 R (two per MFMA pair without row
 // reuse). The B fragments (the three vertical taps of one dx, two 16-channel
-// tiles) come straight from memory into registers (weights are small and
-// L2-resident), the next dx's fragments loading while the current dx's
-// MFMAs run; the next chunk's halo loads into registers likewise.
+// tiles) come straight from memory into registers, from a copy of the
+// weights in fragment order (one 1 KB run per wave load: twice as fast as
+// 16 rows of 64 bytes), the next dx's fragments loading while the current
+// dx's MFMAs run; the next chunk's halo loads into registers likewise. With
+// BL (every wave of the workgroup on the same 32 channels) the chunk's
+// fragments are copied to LDS once per workgroup instead.
 //
 // LDS layout: one 64-byte row (32 channels) per halo pixel hp, its four
-// 16-byte quarters q stored at slots q ^ (hp & 2). A ds_read_b128 serves its
-// 64 lanes in eight phases of eight lanes (lanes 0-3 with 20-23, 4-7 with
-// 16-19, 8-11 with 28-31, 12-15 with 24-27, the same in the upper half); for
-// the A fragment (lane = pixel row + 16 * quarter) this swizzle puts the eight
-// lanes of every phase on eight different 16-byte bank groups for any start
-// pixel, where the previous 80-byte padded rows gave two-way conflicts.
+// 16-byte quarters q stored at slots q ^ (hp & 2). Against 80-byte padded
+// rows this cut the LDS bank-conflict cycles from 50 % to 27-45 % of the
+// LDS-active cycles (MI355X counters) and the LDS footprint by a fifth.
+//
+// The weights are the MFMA's A operand and the pixels its B operand, so the
+// accumulators are channels x pixels: a lane holds 4 consecutive channels of
+// one pixel, which it stores as one 8-byte write.
 // The value of lane l ^ 1 (a DPP quad permutation [1, 0, 3, 2]: no LDS
 // traffic, unlike a general lane shuffle).
 __device__ __forceinline__ float swap_pair(float v) {
