@@ -106,6 +106,8 @@ struct TraceScratch {
     int* status = nullptr;
     int* tiles = nullptr;               // per-tile counts of the compaction scans
     size_t tileCap = 0;
+    void* hits = nullptr;               // staged closest-hit records (16 bytes per ray)
+    size_t hitsBytes = 0;
     // Bumped whenever a buffer above is freed or reallocated: a captured graph
     // whose kernels hold these addresses is valid only while gen is unchanged.
     uint64_t gen = 0;

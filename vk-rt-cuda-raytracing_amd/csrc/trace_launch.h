@@ -48,6 +48,7 @@ struct TraceArgs {
     int tilesX = 0, nTiles = 0;         // pinhole: 8x8 pixel tiles, one per wave
     // outputs
     float* t = nullptr; int* id = nullptr; float* u = nullptr; float* v = nullptr;
+    uint4* hits = nullptr;              // closest hits staged as one (t, id, u, v) record per ray
     uint8_t* occ = nullptr;
     int* cntNodes = nullptr; int* cntTris = nullptr;
     void* img = nullptr; int fmt = 0;   // shading output (RGBA8, BGRA8, RGBA16F, RGBA32F)
@@ -64,6 +65,9 @@ int launch_trace(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int
                  bool counted, hipStream_t s);
 int alloc_scratch(rt::TraceScratch& sc);
 void free_scratch(rt::TraceScratch& sc);
+// Grows the staged hit records of sc to at least need bytes (bumps sc.gen
+// when it reallocates); 0 on success.
+int grow_hits(rt::TraceScratch& sc, size_t need);
 // Grows the spill region of sc to at least need bytes (bumps sc.gen when it
 // reallocates); 0 on success.
 int grow_spill(rt::TraceScratch& sc, size_t need);
