@@ -54,18 +54,30 @@ __device__ __forceinline__ void ray_setup(Ray& r) {
     r.Sz = 1.0f / dkz;
 }
 
+// Component k of (a, b, c); K >= 0 fixes k at compile time.
+template <int K>
+__device__ __forceinline__ float pick(int k, float a, float b, float c) {
+    if constexpr (K == 0) return a;
+    else if constexpr (K == 1) return b;
+    else if constexpr (K == 2) return c;
+    else return sel3(k, a, b, c);
+}
+
 // Watertight ray-triangle test. Returns true with t, u (weight of v1) and v
 // (weight of v2) when the ray's line crosses the triangle with a non-zero
 // determinant; the caller applies (tmin, tmax) and the closest-hit rule.
+// KX, KY, KZ >= 0 name the ray's axis permutation at compile time (the same
+// arithmetic without the per-lane component selects); -1 reads it from r.
+template <int KX = -1, int KY = -1, int KZ = -1>
 __device__ __forceinline__ bool isect_watertight(const Ray& r, float4 p0, float4 p1, float4 p2,
                                                  float& t, float& u, float& v) {
     float A0 = p0.x - r.ox, A1 = p0.y - r.oy, A2 = p0.z - r.oz;
     float B0 = p1.x - r.ox, B1 = p1.y - r.oy, B2 = p1.z - r.oz;
     float C0 = p2.x - r.ox, C1 = p2.y - r.oy, C2 = p2.z - r.oz;
-    float Akz = sel3(r.kz, A0, A1, A2), Bkz = sel3(r.kz, B0, B1, B2), Ckz = sel3(r.kz, C0, C1, C2);
-    float Ax = sel3(r.kx, A0, A1, A2) - r.Sx * Akz, Ay = sel3(r.ky, A0, A1, A2) - r.Sy * Akz;
-    float Bx = sel3(r.kx, B0, B1, B2) - r.Sx * Bkz, By = sel3(r.ky, B0, B1, B2) - r.Sy * Bkz;
-    float Cx = sel3(r.kx, C0, C1, C2) - r.Sx * Ckz, Cy = sel3(r.ky, C0, C1, C2) - r.Sy * Ckz;
+    float Akz = pick<KZ>(r.kz, A0, A1, A2), Bkz = pick<KZ>(r.kz, B0, B1, B2), Ckz = pick<KZ>(r.kz, C0, C1, C2);
+    float Ax = pick<KX>(r.kx, A0, A1, A2) - r.Sx * Akz, Ay = pick<KY>(r.ky, A0, A1, A2) - r.Sy * Akz;
+    float Bx = pick<KX>(r.kx, B0, B1, B2) - r.Sx * Bkz, By = pick<KY>(r.ky, B0, B1, B2) - r.Sy * Bkz;
+    float Cx = pick<KX>(r.kx, C0, C1, C2) - r.Sx * Ckz, Cy = pick<KY>(r.ky, C0, C1, C2) - r.Sy * Ckz;
     float U = Cx * By - Cy * Bx;
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
@@ -82,6 +94,37 @@ __device__ __forceinline__ bool isect_watertight(const Ray& r, float4 p0, float4
     float rcp = 1.0f / det;
     t = T * rcp; u = V * rcp; v = W * rcp;
     return true;
+}
+
+// Axis permutation of the watertight test as one code 0-5: 2 * kz, plus 1
+// when kx and ky are exchanged (negative direction along kz).
+__device__ __forceinline__ int perm_code(const Ray& r) {
+    return 2 * r.kz + (r.kx == (r.kz == 2 ? 0 : r.kz + 1) ? 0 : 1);
+}
+
+// Wave-uniform permutation code of a wave's live lanes (-1 when they
+// differ). Every lane of the wave must call it; dead lanes do not vote.
+__device__ __forceinline__ int wave_perm(const Ray& r, bool live) {
+    const int mine = live ? perm_code(r) : -1;
+    const uint64_t lm = __ballot(live);
+    const int lead = lm ? (int)__builtin_ctzll(lm) : 0;
+    const int p = __builtin_amdgcn_readlane(mine, lead);
+    return __all(!live || mine == p) ? p : -1;
+}
+
+// Watertight test with the permutation fixed when the wave shares one
+// (pc >= 0, wave-uniform), so that the branch is taken by the whole wave.
+__device__ __forceinline__ bool isect_watertight_wave(const Ray& r, int pc, float4 p0, float4 p1, float4 p2,
+                                                      float& t, float& u, float& v) {
+    switch (pc) {
+    case 0: return isect_watertight<1, 2, 0>(r, p0, p1, p2, t, u, v);
+    case 1: return isect_watertight<2, 1, 0>(r, p0, p1, p2, t, u, v);
+    case 2: return isect_watertight<2, 0, 1>(r, p0, p1, p2, t, u, v);
+    case 3: return isect_watertight<0, 2, 1>(r, p0, p1, p2, t, u, v);
+    case 4: return isect_watertight<0, 1, 2>(r, p0, p1, p2, t, u, v);
+    case 5: return isect_watertight<1, 0, 2>(r, p0, p1, p2, t, u, v);
+    default: return isect_watertight(r, p0, p1, p2, t, u, v);
+    }
 }
 
 // Moller-Trumbore with the reference's thresholds (layer/cuda_bvh_backend.cu
@@ -128,42 +171,72 @@ __device__ __forceinline__ float h_sub(uint32_t w, float o) {
     return r;
 }
 
-// Slab test of child slot C of a 64-byte BVH4 node (qx, qy, qz hold the
-// x/y/z halves: mins in slots 0-3, maxes in 4-7). Robust far distance. FLAT
-// (chosen per wave when any lane's ray is flat) resolves the 0*inf NaN so
-// that a parallel axis whose plane holds the origin does not constrain the
-// interval; for other rays both variants give identical results. True when
-// the box may hold a hit in [tmin, limit].
+// Bounds words of a 64-byte BVH4 node (x, y, z halves; per axis the mins of
+// slots 0-3, then the maxes) as one traversal variant reads them. FLAT
+// (chosen per wave when any lane's ray is flat) keeps the stored order and
+// takes per-axis min / max of the two plane distances. The other variant
+// exchanges the min and max words of every axis whose direction is negative,
+// so that the first half holds the entry planes and the second the exit
+// planes: with lo <= hi (outward rounding) and a finite non-zero reciprocal,
+// (lo - o) * i <= (hi - o) * i when i > 0 and >= when i < 0, so the entry and
+// exit distances are the per-axis min and max without computing them.
+template <bool FLAT>
+struct NodeBox { uint4 x, y, z; };
+
+// The exchange is a bitwise select (one v_bfi_b32 per word) under an
+// all-ones / all-zeros mask taken from the reciprocal's sign bit: written as
+// a conditional, the compiler turns it into loads from selected addresses or
+// into branches.
+__device__ __forceinline__ uint4 orient(const uint4& q, float inv) {
+    const uint32_t m = (uint32_t)(__float_as_int(inv) >> 31);
+    return make_uint4((q.z & m) | (q.x & ~m), (q.w & m) | (q.y & ~m), (q.x & m) | (q.z & ~m), (q.y & m) | (q.w & ~m));
+}
+
+template <bool FLAT>
+__device__ __forceinline__ NodeBox<FLAT> node_box(const Ray& r, const uint4& qx, const uint4& qy, const uint4& qz) {
+    if constexpr (FLAT) return NodeBox<FLAT>{qx, qy, qz};
+    else return NodeBox<FLAT>{orient(qx, r.ix), orient(qy, r.iy), orient(qz, r.iz)};
+}
+
+// Slab test of child slot C. Robust far distance. FLAT resolves the 0*inf
+// NaN so that a parallel axis whose plane holds the origin does not constrain
+// the interval; for other rays both variants give the same entry distance and
+// the same answer. True when the box may hold a hit in [tmin, limit].
 template <int C, bool FLAT>
-__device__ __forceinline__ bool slab_child(const Ray& r, const uint4& qx, const uint4& qy, const uint4& qz,
-                                           float limit, float& tnear) {
-    const uint32_t wxl = (C < 2) ? qx.x : qx.y, wxh = (C < 2) ? qx.z : qx.w;
-    const uint32_t wyl = (C < 2) ? qy.x : qy.y, wyh = (C < 2) ? qy.z : qy.w;
-    const uint32_t wzl = (C < 2) ? qz.x : qz.y, wzh = (C < 2) ? qz.z : qz.w;
+__device__ __forceinline__ bool slab_child(const Ray& r, const NodeBox<FLAT>& b, float limit, float& tnear) {
+    const uint32_t wxl = (C < 2) ? b.x.x : b.x.y, wxh = (C < 2) ? b.x.z : b.x.w;
+    const uint32_t wyl = (C < 2) ? b.y.x : b.y.y, wyh = (C < 2) ? b.y.z : b.y.w;
+    const uint32_t wzl = (C < 2) ? b.z.x : b.z.y, wzh = (C < 2) ? b.z.z : b.z.w;
     constexpr bool HI = (C & 1) != 0;
     float t1x = h_sub<HI>(wxl, r.ox) * r.ix, t2x = h_sub<HI>(wxh, r.ox) * r.ix;
     float t1y = h_sub<HI>(wyl, r.oy) * r.iy, t2y = h_sub<HI>(wyh, r.oy) * r.iy;
     float t1z = h_sub<HI>(wzl, r.oz) * r.iz, t2z = h_sub<HI>(wzh, r.oz) * r.iz;
+    float tN, tF;
     if constexpr (FLAT) {
         t1x = t1x != t1x ? -r.ix : t1x; t2x = t2x != t2x ? r.ix : t2x;
         t1y = t1y != t1y ? -r.iy : t1y; t2y = t2y != t2y ? r.iy : t2y;
         t1z = t1z != t1z ? -r.iz : t1z; t2z = t2z != t2z ? r.iz : t2z;
+        tN = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
+        tF = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z)) * RT_ROBUST_TFAR;
+    } else {
+        tN = fmaxf(fmaxf(t1x, t1y), t1z);
+        tF = fminf(fminf(t2x, t2y), t2z) * RT_ROBUST_TFAR;
     }
-    float tN = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
-    float tF = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z)) * RT_ROBUST_TFAR;
     tnear = tN;
     return (tN <= tF) && (tF >= r.tmin) && (tN <= limit);
 }
 
 // Slot C of a node: true, with the entry distance, when the slot holds a
 // child (ref != -1) whose box passes the slab test. The slab test runs for
-// empty slots too and its result is discarded, so the four slots of a node
-// are tested without branches.
+// empty slots too, so the four slots of a node are tested without branches.
+// An empty slot stores the box (+inf, -inf): the FLAT variant's min / max
+// turn it into the whole line, so its result is discarded by ref; in the
+// other variant it reads entry +inf and exit -inf and misses by itself.
 template <int C, bool FLAT>
-__device__ __forceinline__ bool child_hit(const Ray& r, const uint4& qx, const uint4& qy, const uint4& qz, int ref,
-                                          float limit, float& tnear) {
-    const bool h = slab_child<C, FLAT>(r, qx, qy, qz, limit, tnear);
-    return h & (ref != -1);
+__device__ __forceinline__ bool child_hit(const Ray& r, const NodeBox<FLAT>& b, int ref, float limit, float& tnear) {
+    const bool h = slab_child<C, FLAT>(r, b, limit, tnear);
+    if constexpr (FLAT) return h & (ref != -1);
+    else return h;
 }
 
 // Pinhole ray for pixel (px, py) of a W x H image; the same sequence as the

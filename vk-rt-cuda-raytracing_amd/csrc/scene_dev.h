@@ -51,13 +51,14 @@ template <bool ANY, bool FLAT, class Push>
 __device__ __forceinline__ void visit2(const uint4* __restrict__ nodes, const Ray& r, float limit, int& ref, float& tn,
                                        Push&& push) {
     const uint4* np = nodes + 4 * (size_t)ref;
-    const uint4 qx = rtd::ldg_u4(np), qy = rtd::ldg_u4(np + 1), qz = rtd::ldg_u4(np + 2), qc = rtd::ldg_u4(np + 3);
+    const uint4 qc = rtd::ldg_u4(np + 3);
+    const rtd::NodeBox<FLAT> b = rtd::node_box<FLAT>(r, rtd::ldg_u4(np), rtd::ldg_u4(np + 1), rtd::ldg_u4(np + 2));
     if constexpr (ANY) {
         float d0, d1, d2, d3;
-        const bool h0 = rtd::child_hit<0, FLAT>(r, qx, qy, qz, (int)qc.x, limit, d0);
-        const bool h1 = rtd::child_hit<1, FLAT>(r, qx, qy, qz, (int)qc.y, limit, d1);
-        const bool h2 = rtd::child_hit<2, FLAT>(r, qx, qy, qz, (int)qc.z, limit, d2);
-        const bool h3 = rtd::child_hit<3, FLAT>(r, qx, qy, qz, (int)qc.w, limit, d3);
+        const bool h0 = rtd::child_hit<0, FLAT>(r, b, (int)qc.x, limit, d0);
+        const bool h1 = rtd::child_hit<1, FLAT>(r, b, (int)qc.y, limit, d1);
+        const bool h2 = rtd::child_hit<2, FLAT>(r, b, (int)qc.z, limit, d2);
+        const bool h3 = rtd::child_hit<3, FLAT>(r, b, (int)qc.w, limit, d3);
         int nxt = -1;
         if (h0) { nxt = (int)qc.x; }
         if (h1) { if (nxt != -1) push(nxt, 0.f); nxt = (int)qc.y; }
@@ -69,10 +70,10 @@ __device__ __forceinline__ void visit2(const uint4* __restrict__ nodes, const Ra
         float d[4];
         int c[4];
         float tt;
-        if (rtd::child_hit<0, FLAT>(r, qx, qy, qz, (int)qc.x, limit, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
-        if (rtd::child_hit<1, FLAT>(r, qx, qy, qz, (int)qc.y, limit, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
-        if (rtd::child_hit<2, FLAT>(r, qx, qy, qz, (int)qc.z, limit, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
-        if (rtd::child_hit<3, FLAT>(r, qx, qy, qz, (int)qc.w, limit, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
+        if (rtd::child_hit<0, FLAT>(r, b, (int)qc.x, limit, tt)) { d[0] = tt; c[0] = (int)qc.x; } else { d[0] = INFINITY; c[0] = -1; }
+        if (rtd::child_hit<1, FLAT>(r, b, (int)qc.y, limit, tt)) { d[1] = tt; c[1] = (int)qc.y; } else { d[1] = INFINITY; c[1] = -1; }
+        if (rtd::child_hit<2, FLAT>(r, b, (int)qc.z, limit, tt)) { d[2] = tt; c[2] = (int)qc.z; } else { d[2] = INFINITY; c[2] = -1; }
+        if (rtd::child_hit<3, FLAT>(r, b, (int)qc.w, limit, tt)) { d[3] = tt; c[3] = (int)qc.w; } else { d[3] = INFINITY; c[3] = -1; }
 #define RT_CSWAP(i, j)                                                                  \
     {                                                                                   \
         const bool s = d[i] > d[j];                                                     \
