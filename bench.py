@@ -570,7 +570,7 @@ def bench_denoiser(pkg, scenes, dev, args, pt_tuning=None):
 # for the incoherent class it alternates between close candidates from run to
 # run, so each of them gets its own PMC frames.
 SCHED_NAMES = ("batch", "while-while", "packet", "refill")
-PMC_FRAME_CONFIG = (("CLASS_COHERENT", 0, 12, 0), ("CLASS_ANY", 0, 8, 4))
+PMC_FRAME_CONFIG = (("CLASS_COHERENT", 0, 12, 0), ("CLASS_ANY", 0, 12, 0))
 PMC_INCOHERENT = ((3, 8, 0), (3, 12, 0), (1, 12, 0))
 
 

@@ -355,6 +355,36 @@ __global__ __launch_bounds__(BLOCK) void k_trace_refill(TraceArgs a) {
     int sp = 0, ref = -1, bid = -1, bti = -1, lts = 0, ltc = 0;
     float tn = 0.f, best = 0.f, bu = 0.f, bv = 0.f;
 
+    // Closest hits of explicit rays are gathered per pool (the 64 consecutive
+    // rays of one work chunk) in LDS and written out, coalesced, when the
+    // pool's last ray finishes. Lanes finish rays at different times, and
+    // written one by one, a ray's four results each dirtied an L2 sector of
+    // its own. Two pools per wave are held; a third evicts the older one
+    // (its gathered results go out, its later rays are written directly).
+    constexpr bool POOL = MODE == kModeClosest && !PIN;
+    constexpr int kSlots = 2;
+    __shared__ float4 s_rec[POOL ? (BLOCK / 64) * kSlots * 64 : 1];
+    const int rbase = (tid >> 6) * kSlots * 64;
+    int sbase[kSlots] = {-1, -1}, scount[kSlots] = {0, 0};   // wave-uniform: pool base, results gathered
+    int sold = 0;                                             // wave-uniform: the slot assigned earlier
+    int fs = -1;                                              // slot that gathered this lane's last result
+    // writes the gathered results of slot sl (entries with t not NaN)
+    auto flush = [&](int sl) {
+        const int pb = sbase[sl];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const float4 rec = s_rec[rbase + sl * 64 + lane];
+        if (lane < n - pb && rec.x == rec.x) {
+            const int q = pb + lane;
+            a.t[q] = rec.x;
+            if (a.id) a.id[q] = __float_as_int(rec.y);
+            if (a.u) a.u[q] = rec.z;
+            if (a.v) a.v[q] = rec.w;
+        }
+        sbase[sl] = -1;
+        scount[sl] = 0;
+    };
+
     auto push = [&](int rf, float t) {
         if (sp < LDSD) {
             s_ref[sp * BLOCK + tid] = rf;
@@ -369,6 +399,17 @@ __global__ __launch_bounds__(BLOCK) void k_trace_refill(TraceArgs a) {
     };
 
     for (;;) {
+        if constexpr (POOL) {
+            // the previous step's finished rays: a pool whose rays are all
+            // gathered goes out (every lane is active here)
+#pragma unroll
+            for (int sl = 0; sl < kSlots; sl++) {
+                if (sbase[sl] < 0) continue;
+                scount[sl] += __popcll(__ballot(fs == sl));
+                if (scount[sl] == min(64, n - sbase[sl])) flush(sl);
+            }
+            fs = -1;
+        }
         if (!exhausted || poolNext < poolEnd) {
             const uint64_t idle = __ballot(!has);
             if (idle && (__popcll(idle) >= kRefillMin || !__any(has))) {
@@ -378,6 +419,15 @@ __global__ __launch_bounds__(BLOCK) void k_trace_refill(TraceArgs a) {
                 if (cnt > avail && !exhausted) {
                     fresh = rtd::take_chunk(a.counter, n, band);
                     if (fresh >= n) exhausted = true;
+                    if constexpr (POOL) {
+                        if (fresh < n) {
+                            const int sl = sbase[0] < 0 ? 0 : (sbase[1] < 0 ? 1 : sold);
+                            if (sbase[sl] >= 0) flush(sl);
+                            sbase[sl] = fresh;
+                            sold = 1 - sl;
+                            s_rec[rbase + sl * 64 + lane].x = __int_as_float(0x7fc00000);   // NaN: not yet gathered
+                        }
+                    }
                 }
                 const int k = __popcll(idle & ((1ull << lane) - 1ull));
                 int w = n;
@@ -481,6 +531,20 @@ __global__ __launch_bounds__(BLOCK) void k_trace_refill(TraceArgs a) {
         if (fin) {
             if constexpr (MODE == kModeAny) {
                 a.occ[ri] = found ? 1 : 0;
+            } else if constexpr (POOL) {
+                const bool hit = bid >= 0;
+                const float4 rec = make_float4(hit ? best : RT_MISS_T, __int_as_float(bid), hit ? bu : 0.f,
+                                               hit ? bv : 0.f);
+                const int pb = ri & ~63;
+                fs = pb == sbase[0] ? 0 : (pb == sbase[1] ? 1 : -1);
+                if (fs >= 0) {
+                    s_rec[rbase + fs * 64 + (ri & 63)] = rec;
+                } else {
+                    a.t[ri] = rec.x;
+                    if (a.id) a.id[ri] = bid;
+                    if (a.u) a.u[ri] = rec.z;
+                    if (a.v) a.v[ri] = rec.w;
+                }
             } else if constexpr (MODE == kModeClosest) {
                 put_hit(a, ri, bid >= 0, best, bid, bu, bv);
             } else {
@@ -488,6 +552,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_refill(TraceArgs a) {
             }
             has = false;
         }
+
     }
 }
 
@@ -1001,7 +1066,9 @@ static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a
     auto kern = SCHED == rt::kScheduleWhileWhile ? k_trace_ww<MODE, MT, PIN, COUNT, LDSD, kBlock>
               : (SCHED == rt::kScheduleRefill && !COUNT) ? k_trace_refill<MODE, MT, PIN, LDSD, kBlock>
                                                          : k_trace_batch<MODE, MT, PIN, COUNT, LDSD, kBlock>;
-    constexpr bool stage = MODE == kModeClosest && !PIN && !COUNT && SCHED != rt::kScheduleBatch;
+    // the while-while kernel stages hit records for k_hits_split; the refill
+    // kernel gathers them per pool in LDS
+    constexpr bool stage = MODE == kModeClosest && !PIN && !COUNT && SCHED == rt::kScheduleWhileWhile;
     return launch_kernel(h, sc, a, kern, LDSD, false, blocksPerCU, ki, s, stage);
 }
 
