@@ -318,7 +318,7 @@ def main():
                            "occluded": round(occ_rate, 4)},
         },
         "roofline": {
-            "kernel": "k_trace closest-hit, explicit rays (incoherent diffuse class), with its hit-record split",
+            "kernel": "k_trace closest-hit, explicit rays (incoherent diffuse class)",
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": prof_traffic,

@@ -282,7 +282,7 @@ int cudaBVH_traceAnyQueued(CudaBVH_t bvh, int capacity, const int* d_count,
  * refilled from the wave-local pool),
  * LDS stack depth per lane (4, 8, 12 or 16 entries; deeper entries spill to
  * HBM) and resident workgroups per CU (0 = occupancy limit). Defaults after a
- * build: batch / 12 / 0 for classes 0 and 2, refill / 12 / 0 for class 1. */
+ * build: batch / 12 / 0 for classes 0 and 2, refill / 8 / 0 for class 1. */
 int cudaBVH_setTraceConfig(CudaBVH_t bvh, int rayClass, int schedule, int ldsDepth, int blocksPerCU);
 int cudaBVH_getTraceConfig(CudaBVH_t bvh, int rayClass, int* schedule, int* ldsDepth, int* blocksPerCU);
 

@@ -155,10 +155,10 @@ static CudaBVHHandle* make_handle(CudaBVHHandle* h, const float* tri9, int n) {
         return nullptr;
     }
     // defaults: the configurations the autotuner picks most often on the
-    // 1080p terrain frame (refill / 12 also traces the 256-scene forest 10 %
-    // faster than while-while / 8)
+    // 1080p terrain frame (refill / 8: its 16 KB of stack plus 8 KB of
+    // gathered results per workgroup leave room for six workgroups per CU)
     h->tune[rt::kClassIncoherent].schedule = rt::kScheduleRefill;
-    h->tune[rt::kClassIncoherent].ldsDepth = 12;
+    h->tune[rt::kClassIncoherent].ldsDepth = 8;
     h->tune[rt::kClassCoherent].ldsDepth = 12;
     h->tune[rt::kClassAny].ldsDepth = 12;
     if (rt::verbose())
