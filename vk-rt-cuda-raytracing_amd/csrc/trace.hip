@@ -558,9 +558,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace_refill(TraceArgs a) {
 
 // Batch schedule: each wave takes 64 rays (one 8x8 pixel tile for pinhole
 // rays) with one atomicAdd and runs until all 64 are done. Best for coherent
-// rays, whose lanes stay in step.
+// rays, whose lanes stay in step. Compiled for seven waves per SIMD (72
+// VGPRs, no scratch): with an LDS stack of 8 entries seven fit, and the
+// 1080p primaries ran ~2 % faster than at six.
 template <int MODE, bool MT, bool PIN, bool COUNT, int LDSD, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_trace_batch(TraceArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void k_trace_batch(TraceArgs a) {
     __shared__ int s_ref[LDSD * BLOCK];
     __shared__ float s_tn[LDSD * BLOCK];
     const int tid = threadIdx.x;
