@@ -443,6 +443,39 @@ def test_fused_pinhole_odd_sizes(pkg, oracle, scenes, parity_scene, parity_bvh, 
     parity_bvh.set_trace_config(pkg.CLASS_COHERENT, *saved)
 
 
+def test_refill_gathered_results_with_evictions(pkg, oracle, dev):
+    """The refill kernel gathers closest-hit results per 64-ray work chunk and
+    holds two chunks per wave. Here one ray in 64 crosses a dense triangle
+    soup while the other 63 miss the scene at the root, so a wave runs through
+    many chunks while each chunk's long ray is still walking: older chunks are
+    evicted and their long rays write their results directly. Every result
+    equals the oracle's, for both stack depths of the refill kernel."""
+    rng = np.random.default_rng(44)
+    nt = 20000
+    c = rng.uniform(-1, 1, (nt, 1, 3)).astype(np.float32)
+    tris = (c + rng.normal(scale=0.08, size=(nt, 3, 3))).astype(np.float32).reshape(nt, 9)
+    n = 64 * 4096 + 13          # with one workgroup per CU: about four chunks per wave
+    rays = np.zeros((8, n), np.float32)
+    rays[0:3] = np.array([[0.0], [0.0], [-5.0]], np.float32)      # short rays: look away from the soup
+    rays[5] = -1.0
+    rays[7] = 1e30
+    long_ = np.arange(n) % 64 == 17
+    k = int(long_.sum())
+    o = rng.uniform(-0.9, 0.9, (k, 3)).astype(np.float32)
+    o[:, 2] = -3.0
+    d = rng.normal(size=(k, 3)) * 0.02 + np.array([0.0, 0.0, 1.0])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays[0:3, long_] = o.T
+    rays[3:6, long_] = d.T.astype(np.float32)
+    with pkg.BVH.build(tris) as b:
+        nodes, lt, pid = _oracle_inputs(b)
+        ref = oracle.trace_closest(nodes, lt, pid, rays)
+        assert (ref["id"][long_] >= 0).mean() > 0.9 and (ref["id"][~long_] < 0).all()
+        for depth in (8, 12):
+            b.set_trace_config(pkg.CLASS_INCOHERENT, pkg.SCHEDULE_REFILL, depth, 1)
+            _assert_closest_equal(_gpu_closest(b, rays, dev), ref)
+
+
 @pytest.mark.parametrize("n", [1, 7, 64, 65, 511, 4097])
 def test_closest_small_ray_counts(oracle, parity_scene, parity_bvh, dev, n):
     """Ray counts below, at and just past the 64-ray chunk and the eight
