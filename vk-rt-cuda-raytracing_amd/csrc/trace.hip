@@ -135,13 +135,14 @@ struct Result {
     int visits, tests;
 };
 
-// Closest-hit result of ray ri. The refill and while-while kernels, whose
-// lanes finish rays at different times, write it as one 16-byte record into
-// a staging array when the launch provides one, split into the four output
-// arrays by k_hits_split afterwards: four 4-byte stores from a few lanes at
-// a time each dirtied a 32-byte sector of its own (233 MB written per
-// launch for 27 MB of hits on the 1080p diffuse rays; 73 MB staged, MI355X
-// PMC). Appending the records in finishing order instead (one reservation
+// Closest-hit result of ray ri. The while-while kernel, whose lanes finish
+// rays at different times, writes it as one 16-byte record into a staging
+// array when the launch provides one, split into the four output arrays by
+// k_hits_split afterwards: four 4-byte stores from a few lanes at a time
+// each dirtied a 32-byte sector of its own (233 MB written per launch for
+// 27 MB of hits on the 1080p diffuse rays; 73 MB staged, MI355X PMC). The
+// refill kernel gathers its results per work chunk in LDS instead (its
+// pool staging). Appending the records in finishing order (one reservation
 // of 64 slots per wave) wrote more in total and ran 10 % slower.
 __device__ __forceinline__ void put_hit(const TraceArgs& a, int ri, bool hit, float t, int id, float u, float v) {
     if (a.hits) {
