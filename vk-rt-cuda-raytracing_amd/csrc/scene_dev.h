@@ -162,9 +162,12 @@ struct LdsStack {
 // (same culling and tie rule; any hit may visit children nearest first,
 // ORD). Closest hit: returns whether something was hit and (t, inst, id, u,
 // v); any hit: whether the ray is occluded. Each loop iteration is one short
-// step — a node visit, an instance entry, or one triangle of the current
-// BLAS leaf (leaf cursor) — so lanes of a wave at different kinds of step do
-// not wait for whole leaf loops.
+// step — a node visit, an instance entry, or the next one or two triangles
+// of the current BLAS leaf (leaf cursor; the two fetched together) — so
+// lanes of a wave at different kinds of step do not wait for whole leaf
+// loops. As in the one-level batch kernel, the wave's priority rises one
+// level per 32 steps, so that long walks (grazing rays) issue ahead of short
+// ones; it starts at 0 with each call.
 template <bool ANY, bool ORD, class Stack>
 __device__ __forceinline__ bool trace2l_ray(const uint4* tlasNodes, const float4* tlasTris, const InstanceDev* insts,
                                             const BlasDev* blas, const Ray& rw, Stack& st, float& bt, int& binst,
@@ -179,7 +182,16 @@ __device__ __forceinline__ bool trace2l_ray(const uint4* tlasNodes, const float4
     const float4* tris = nullptr;
     bid = -1; binst = -1; bt = rw.tmax; bu = 0.f; bv = 0.f;
     auto push = [&](int rf, float t) { st.push(rf, t); };
+    constexpr int kPrio2 = 32;
+    __builtin_amdgcn_s_setprio(0);
+    int steps = 0;
     for (;;) {
+        if ((++steps & 15) == 0) {
+            const int s = __builtin_amdgcn_readfirstlane(steps);
+            if (s == kPrio2) __builtin_amdgcn_s_setprio(1);
+            else if (s == 2 * kPrio2) __builtin_amdgcn_s_setprio(2);
+            else if (s == 3 * kPrio2) __builtin_amdgcn_s_setprio(3);
+        }
         if (ltc == 0) {
             while (ref == -1 || ref == kReturnMark || (!ANY && tn > best)) {
                 if (ref == kReturnMark) { inst = -1; nodes = tlasNodes; ref = -1; continue; }
@@ -193,20 +205,30 @@ __device__ __forceinline__ bool trace2l_ray(const uint4* tlasNodes, const float4
         }
         if (ltc > 0) {
             const int ti = lts;
-            lts++; ltc--;
-            const float4 p0 = rtd::ldg_f4(tris + 3 * ti);
-            float t, u, v;
-            if (rtd::isect_watertight(ro, p0, rtd::ldg_f4(tris + 3 * ti + 1), rtd::ldg_f4(tris + 3 * ti + 2), t, u, v) && t > ro.tmin &&
-                t < ro.tmax) {
-                if constexpr (ANY) {
-                    return true;
-                } else {
-                    const int id = __float_as_int(p0.w);
-                    if (t < best || (t == best && (inst < binst || (inst == binst && id < bid)))) {
-                        best = t; bid = id; binst = inst; bt = t; bu = u; bv = v;
+            const bool two = ltc > 1;
+            const int tj = two ? ti + 1 : ti;
+            const float4 p0 = rtd::ldg_f4(tris + 3 * ti), p1 = rtd::ldg_f4(tris + 3 * ti + 1), p2 = rtd::ldg_f4(tris + 3 * ti + 2);
+            const float4 q0 = rtd::ldg_f4(tris + 3 * tj), q1 = rtd::ldg_f4(tris + 3 * tj + 1), q2 = rtd::ldg_f4(tris + 3 * tj + 2);
+            lts += two ? 2 : 1;
+            ltc -= two ? 2 : 1;
+            // leaf order: the first triangle, then the second (a closest-hit
+            // update of the first is seen by the second's tie rule)
+            auto test = [&](const float4& a0, const float4& a1, const float4& a2) {
+                float t, u, v;
+                if (rtd::isect_watertight(ro, a0, a1, a2, t, u, v) && t > ro.tmin && t < ro.tmax) {
+                    if constexpr (ANY) {
+                        return true;
+                    } else {
+                        const int id = __float_as_int(a0.w);
+                        if (t < best || (t == best && (inst < binst || (inst == binst && id < bid)))) {
+                            best = t; bid = id; binst = inst; bt = t; bu = u; bv = v;
+                        }
                     }
                 }
-            }
+                return false;
+            };
+            if (test(p0, p1, p2)) return true;
+            if (two && test(q0, q1, q2)) return true;
         } else if (ref >= 0) {
             const bool w = inst < 0;
             const Ray& r = w ? rw : ro;
