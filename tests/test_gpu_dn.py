@@ -49,7 +49,7 @@ def _rel(a, b):
 # (measured on the CPU between the two oracle modes), hence the loose bound
 # there. The gradient check uses targets far below the prediction, so the L1
 # loss gradient has the same sign everywhere on both sides.
-@pytest.mark.parametrize("B,H,W", [(3, 8, 8), (1, 16, 16), (1, 24, 40)])
+@pytest.mark.parametrize("B,H,W", [(3, 8, 8), (1, 16, 16), (1, 24, 40), (1, 16, 136)])
 def test_forward_and_gradients(pkg, dn_ref, dev, B, H, W):
     import torch
     worst, fwd = [], []

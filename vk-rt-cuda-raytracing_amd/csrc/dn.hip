@@ -16,6 +16,8 @@
 //     slice of a wider buffer, so skip concatenation costs nothing; optional
 //     fusions: 2x2 max pooling of the output, 2x nearest upsampling of part
 //     of the input, and the 1x1 head with the residual and expm1.
+//   * k_conv01: the first two 3x3 convolutions in one kernel, the first
+//     one's output kept in LDS (recomputed on the halo).
 //   * k_conv: the generic implicit GEMM, kept for the 1x1 head in training
 //     and its input gradient.
 //   * k_wgrad: weight gradients, M = output channels, N = input channels,
@@ -48,6 +50,34 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// Network input channels 0-2: log(1 + max(rgb, 0)), rounded to BF16 (also
+// the residual that the head's output is added to).
+__device__ __forceinline__ uint16_t log_rgb_bf(float v) { return f2bf(log1pf(fmaxf(v, 0.0f))); }
+
+// One 16-channel BF16 input pixel (two 16-byte halves) from its plane values
+// v = rgb[3], albedo[3], normal[3], depth: log(1 + rgb), albedo, normal,
+// log(1 + depth) / 4, zeros.
+__device__ __forceinline__ void pack_pixel(const float (&v)[10], uint4& o0, uint4& o1) {
+    uint32_t h[10];
+#pragma unroll
+    for (int c = 0; c < 3; c++) h[c] = log_rgb_bf(v[c]);
+#pragma unroll
+    for (int c = 3; c < 9; c++) h[c] = f2bf(v[c]);
+    h[9] = f2bf(0.25f * log1pf(fmaxf(v[9], 0.0f)));
+    o0 = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+    o1 = make_uint4(h[8] | (h[9] << 16), 0, 0, 0);
+}
+__device__ __forceinline__ void load_pixel(const float* rgb, const float* alb, const float* nrm, const float* dep,
+                                           size_t P, size_t p, float (&v)[10]) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        v[c] = rgb[c * P + p];
+        v[3 + c] = alb[c * P + p];
+        v[6 + c] = nrm[c * P + p];
+    }
+    v[9] = dep[p];
+}
+
 // ---------------------------------------------------------------- conv ----
 struct ConvArgs {
     const uint16_t* x; int ldx; int cin;     // input pixels (channel stride ldx), channels per tap
@@ -67,8 +97,11 @@ struct ConvArgs {
     // (3 x 32 BF16 weights hw, bias hbias) applied to the BF16-rounded output,
     // written as FP32 head values (head, P x 3; nullable) and/or as denoised
     // planes rgbOut[c][p] = max(expm1(head + residual), 0) (nullable), the
-    // residual being channel c of the 16-channel BF16 input pixels xres
-    const uint16_t* hw; const float* hbias; float* head; const uint16_t* xres; float* rgbOut;
+    // residual being log(1 + rgb) rounded to BF16 (input channel c) from the
+    // noisy radiance planes rgbIn[c][p]
+    const uint16_t* hw; const float* hbias; float* head; const float* rgbIn; float* rgbOut;
+    // k_conv3r only: x stores input channels [xc0, cin) (channel c at c - xc0)
+    int xc0;
 };
 
 // Block: 4 waves x (16*WM) pixels; each wave (16*WM) pixels x (16*WN)
@@ -258,7 +291,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) v
                 const uint16_t* src =
                     up ? a.xu + (size_t)((bb * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1)) * a.ldxu
                        : a.x + (size_t)((bb * a.H + y) * a.W + x) * a.ldx;
-                ra[k] = *(const uint4*)(src + c0 + 8 * (e % QP));
+                ra[k] = *(const uint4*)(src + (up ? c0 : c0 - a.xc0) + 8 * (e % QP));
             }
         }
     };
@@ -417,8 +450,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) v
                 if (kg < 3 && y < a.H && x < a.W) {
                     const float hv = d[0] + a.hbias[kg];
                     if (a.head) a.head[p * 3 + kg] = hv;
-                    if (a.rgbOut)
-                        a.rgbOut[(size_t)kg * a.B * a.H * a.W + p] = fmaxf(expm1f(hv + bf2f(a.xres[p * 16 + kg])), 0.0f);
+                    if (a.rgbOut) {
+                        const size_t pc = (size_t)kg * a.B * a.H * a.W + p;
+                        a.rgbOut[pc] = fmaxf(expm1f(hv + bf2f(log_rgb_bf(a.rgbIn[pc]))), 0.0f);
+                    }
                 }
             }
         }
@@ -433,6 +468,217 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC))) v
         if (c0 + CS < a.cin) chunk(c0 + CS, b1, b0);
     }
     store_tile();
+}
+
+// The first two 3x3 convolutions (16 -> 32 and 32 -> 32 channels, ReLU after
+// each) in one kernel: the first layer's output goes from registers to LDS
+// and never to memory (at 1920x1080 it is 133 MB written and read back, with
+// the second layer's halo, 211 MB). A workgroup produces the second layer's
+// output rows y0 .. y0 + 3, pixels x0 .. x0 + 61 (x0 = 62 tx), 32 channels,
+// with the 2x2 max-pooled copy. Stage 0 computes the first layer on the
+// 6 x 64 pixels those outputs read (rows y0 - 1 .. y0 + 4, pixels x0 - 1 ..
+// x0 + 62; wave w the 16-pixel column block w) from the 8 x 66-pixel input
+// halo, with the MFMAs of k_conv3r's 16-channel path in the same order, and
+// writes it (bias, ReLU, BF16; zero outside the image, which is the second
+// layer's zero padding) into the second layer's halo in LDS in k_conv3r's
+// swizzled 64-byte rows. Halo columns 64 and 65 feed only outputs 62 and 63
+// of each row, which are not stored; they are zero. Stage 1 runs k_conv3r's
+// 32-channel path on that halo. Each output is bit-identical to the two
+// separate convolutions (same MFMA sequence per output, same rounding). The
+// halo recomputes the first layer on 1.55x its pixels, a sixth more MFMAs for
+// the two layers. With y0out (training) stage 0 also stores the first
+// layer's output on the workgroup's own 4 x 62 pixels.
+struct Conv01Args {
+    const uint16_t* x;                      // B x H x W x 16 BF16 pixels
+    const uint16_t* wf0; const float* b0;   // first layer: weights in 16-channel fragment order, bias
+    const uint16_t* wf1; const float* b1;   // second layer: weights in fragment order, bias
+    uint16_t* y0out;                        // first layer output (32 channels, dense), nullable
+    uint16_t* y; int ldy;                   // second layer output (32 channels at stride ldy)
+    uint16_t* yp; int ldyp;                 // its 2x2 max-pooled copy
+    int B, H, W;
+};
+constexpr int kC01Rows = 4, kC01Px = 62;
+
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_conv01(Conv01Args a) {
+    constexpr int R = kC01Rows, R0 = R + 2;        // output rows of stage 1 / stage 0
+    constexpr int HX = 66, HY0 = R0 + 2, HY1 = R + 2;
+    constexpr int NP0 = HY0 * HX * 2;              // 16-byte pieces of the input halo (32-byte pixels)
+    constexpr int NA = (NP0 + kBlock - 1) / kBlock;
+    constexpr int PD = 2;
+    __shared__ uint4 h0[NP0];                      // input halo: pixel hp's pieces at 2 hp, 2 hp + 1
+    __shared__ uint4 h1[HY1 * HX * 4];             // first layer output halo: quarter q of pixel hp at 4 hp + (q ^ (hp & 2))
+    const int tid = threadIdx.x, lane = tid & 63, wx = tid >> 6;
+    const int row = lane & 15, kg = lane >> 4;
+    const int tilesX = (a.W + kC01Px - 1) / kC01Px, tilesY = (a.H + R - 1) / R;
+    const int tx = blockIdx.x % tilesX, tq = blockIdx.x / tilesX;
+    const int ty = tq % tilesY, bb = tq / tilesY;
+    const int x0 = tx * kC01Px, y0 = ty * R;
+    const uint16_t* w00 = a.wf0 + lane * 8;        // 16-channel blocks 0 and 1: [3 dy][2 tap pairs][lane][8]
+    const uint16_t* w01 = w00 + 6 * 512;
+    const uint16_t* w10 = a.wf1 + lane * 8;        // [9 taps][lane][8]
+    const uint16_t* w11 = w10 + 9 * 512;
+    auto load0 = [&](bf16x8 (&b)[3][2], int pr) {
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++) {
+            b[dy][0] = __builtin_bit_cast(bf16x8, *(const uint4*)(w00 + (dy * 2 + pr) * 512));
+            b[dy][1] = __builtin_bit_cast(bf16x8, *(const uint4*)(w01 + (dy * 2 + pr) * 512));
+        }
+    };
+    auto load1 = [&](bf16x8 (&b)[3][2], int dx) {
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++) {
+            b[dy][0] = __builtin_bit_cast(bf16x8, *(const uint4*)(w10 + (dy * 3 + dx) * 512));
+            b[dy][1] = __builtin_bit_cast(bf16x8, *(const uint4*)(w11 + (dy * 3 + dx) * 512));
+        }
+    };
+    bf16x8 bx[3][2], by[3][2];
+    load0(bx, 0);
+    // input halo: pixel (y0 - 2 + hy, x0 - 2 + hx), zero outside the image
+    uint4 ra[NA];
+#pragma unroll
+    for (int k = 0; k < NA; k++) {
+        const int e = tid + k * kBlock, hp = e >> 1;
+        const int hy = hp / HX, hx = hp - hy * HX;
+        const int y = y0 - 2 + hy, x = x0 - 2 + hx;
+        ra[k] = make_uint4(0, 0, 0, 0);
+        if (e < NP0 && y >= 0 && y < a.H && x >= 0 && x < a.W)
+            ra[k] = *(const uint4*)(a.x + (size_t)((bb * a.H + y) * a.W + x) * 16 + 8 * (e & 1));
+    }
+    float bias0[2][4], bias1[2][4];
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            bias0[j][i] = a.b0[j * 16 + 4 * kg + i];
+            bias1[j][i] = a.b1[j * 16 + 4 * kg + i];
+        }
+    if (tid < HY1 * 2 * 4) h1[((tid >> 3) * HX + 64 + ((tid >> 2) & 1)) * 4 + (tid & 3)] = make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < NA; k++) {
+        const int e = tid + k * kBlock;
+        if (e < NP0) h0[e] = ra[k];
+    }
+    __syncthreads();
+
+    // stage 0: tap pair pr (taps 2 pr, 2 pr + 1; tap 3 is zero weight and a
+    // zero operand) of the first layer, as k_conv3r's 16-channel path
+    f32x4 acc0[R0][2];
+#pragma unroll
+    for (int r = 0; r < R0; r++) acc0[r][0] = acc0[r][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    auto step0 = [&](const bf16x8 (&bc)[3][2], int pr) {
+        const bool pad = pr == 1 && kg >= 2;
+        const int hp0 = wx * 16 + row + 2 * pr + (pad ? 0 : kg >> 1);
+        const int s0 = hp0 * 2 + (kg & 1);
+#pragma unroll
+        for (int hy = 0; hy < HY0; hy++) {
+            const bf16x8 av = __builtin_bit_cast(bf16x8, pad ? make_uint4(0, 0, 0, 0) : h0[hy * HX * 2 + s0]);
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++) {
+                const int ro = hy - dy;
+                if (ro < 0 || ro >= R0) continue;
+                acc0[ro][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bc[dy][0], av, acc0[ro][0], 0, 0, 0);
+                acc0[ro][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bc[dy][1], av, acc0[ro][1], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_sched_group_barrier(0x100, PD, 0);
+        halo_row_order<R0, PD, 0>();
+    };
+    load0(by, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    step0(bx, 0);
+    load1(bx, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    step0(by, 1);
+    // stage 0 epilogue: first layer row r, pixel c = x0 - 1 + (16 wx + row)
+    // -> halo pixel r * HX + 16 wx + row of the second layer
+    {
+        const int c = wx * 16 + row, x = x0 - 1 + c;
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            const int y = y0 - 1 + r;
+            const bool in = y >= 0 && y < a.H && x >= 0 && x < a.W;
+            const int hp = r * HX + c;
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                float v[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) v[i] = fmaxf(acc0[r][j][i] + bias0[j][i], 0.0f);
+                const uint2 w = in ? make_uint2(f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                                                f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16))
+                                   : make_uint2(0, 0);
+                const int q = 2 * j + (kg >> 1);
+                ((uint2*)h1)[(hp * 4 + (q ^ (hp & 2))) * 2 + (kg & 1)] = w;
+                if (a.y0out && in && r >= 1 && r <= R && c >= 1 && c <= kC01Px)
+                    *(uint2*)(a.y0out + ((size_t)(bb * a.H + y) * a.W + x) * 32 + j * 16 + 4 * kg) = w;
+            }
+        }
+    }
+    __syncthreads();
+
+    // stage 1: k_conv3r's 32-channel path over h1
+    f32x4 acc[R][2];
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r][0] = acc[r][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    auto step1 = [&](const bf16x8 (&bc)[3][2], int dx) {
+        const int hp0 = wx * 16 + row + dx;
+        const int s0 = hp0 * 4 + (kg ^ (hp0 & 2));
+        const int s1 = hp0 * 4 + (kg ^ (hp0 & 2) ^ 2);
+#pragma unroll
+        for (int hy = 0; hy < HY1; hy++) {
+            const int slot = hy * HX * 4 + (((hy * HX) & 2) ? s1 : s0);
+            const bf16x8 av = __builtin_bit_cast(bf16x8, h1[slot]);
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++) {
+                const int ro = hy - dy;
+                if (ro < 0 || ro >= R) continue;
+                acc[ro][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bc[dy][0], av, acc[ro][0], 0, 0, 0);
+                acc[ro][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bc[dy][1], av, acc[ro][1], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_sched_group_barrier(0x100, PD, 0);
+        halo_row_order<R, PD, 0>();
+    };
+    load1(by, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    step1(bx, 0);
+    load1(bx, 2);
+    __builtin_amdgcn_sched_barrier(0);
+    step1(by, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    step1(bx, 2);
+
+    // stage 1 epilogue (k_conv3r's, with outputs 62 and 63 of the row masked)
+    const int x = x0 + wx * 16 + row;
+    const bool colOk = x < a.W && wx * 16 + row < kC01Px;
+    float pool[2][4];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + r;
+        const size_t p = (size_t)(bb * a.H + y) * a.W + x;
+        const bool ok = colOk && y < a.H;
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int col = j * 16 + 4 * kg;
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) v[i] = fmaxf(acc[r][j][i] + bias1[j][i], 0.0f);
+            if (ok)
+                *(uint2*)(a.y + p * a.ldy + col) = make_uint2(f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                                                              f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
+            float m[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float h = fmaxf(v[i], swap_pair(v[i]));
+                m[i] = (r & 1) ? fmaxf(h, pool[j][i]) : h;
+                pool[j][i] = m[i];
+            }
+            if ((r & 1) && ok && !(row & 1)) {
+                const size_t q = (size_t)(bb * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
+                *(uint2*)(a.yp + q * a.ldyp + col) = make_uint2(f2bf(m[0]) | ((uint32_t)f2bf(m[1]) << 16),
+                                                                f2bf(m[2]) | ((uint32_t)f2bf(m[3]) << 16));
+            }
+        }
+    }
 }
 
 // --------------------------------------------------------------- wgrad ----
@@ -708,17 +954,10 @@ __global__ __launch_bounds__(kBlock) void k_wfrag16(const uint16_t* w, int rows,
 __global__ __launch_bounds__(kBlock) void k_pack(const float* rgb, const float* alb, const float* nrm, const float* dep,
                                                  int P, uint16_t* x) {
     for (int p = blockIdx.x * kBlock + threadIdx.x; p < P; p += gridDim.x * kBlock) {
-        uint4 o0, o1 = make_uint4(0, 0, 0, 0);
-        uint16_t* e = (uint16_t*)&o0;
-        for (int c = 0; c < 3; c++) {
-            e[c] = f2bf(log1pf(fmaxf(rgb[(size_t)c * P + p], 0.0f)));
-            e[3 + c] = f2bf(alb[(size_t)c * P + p]);
-        }
-        e[6] = f2bf(nrm[p]);
-        e[7] = f2bf(nrm[(size_t)P + p]);
-        uint16_t* f = (uint16_t*)&o1;
-        f[0] = f2bf(nrm[(size_t)2 * P + p]);
-        f[1] = f2bf(0.25f * log1pf(fmaxf(dep[p], 0.0f)));
+        float v[10];
+        uint4 o0, o1;
+        load_pixel(rgb, alb, nrm, dep, (size_t)P, (size_t)p, v);
+        pack_pixel(v, o0, o1);
         *(uint4*)(x + (size_t)p * 16) = o0;
         *(uint4*)(x + (size_t)p * 16 + 8) = o1;
     }
@@ -784,11 +1023,14 @@ static void launch_conv(const ConvArgs& a, int P, int n, hipStream_t s) {
 // MI355X: 128-channel multiples take 8 rows x 16 pixels x 128 channels (each
 // wave 32 channels; no two waves share weights, so nothing is gained by
 // staging them); 64-channel multiples with inputs of 128 channels or more
-// 8 rows x 32 pixels x 64 channels; the rest 4 rows x 64 pixels x 32
-// channels at three waves per SIMD (the full-resolution layers, where the
-// smaller accumulator set buys occupancy), with the weights staged once per
-// workgroup in LDS when there is more than one channel chunk (all four waves
-// use the same 32 channels; 12 % faster on the 96-channel decoder layer).
+// 8 rows x 32 pixels x 64 channels, with 64-channel inputs 4 rows x 32 pixels
+// x 64 channels at three waves per SIMD (one pass over the input instead of
+// one per 32 output channels: 1.5 % of the 1080p frame); the rest 4 rows x
+// 64 pixels x 32 channels at three waves per SIMD (the full-resolution
+// layers, where the smaller accumulator set buys occupancy), with the weights
+// staged once per workgroup in LDS when there is more than one channel chunk
+// (all four waves use the same 32 channels; 12 % faster on the 96-channel
+// decoder layer).
 template <int R, int WGX, int WGC, int OCC, int PD, bool BL, int CS = 32, bool HD = false>
 static void launch_r(const ConvArgs& a, int chanTiles, hipStream_t s) {
     constexpr int TX = 16 * WGX;
@@ -799,6 +1041,7 @@ static void launch_r(const ConvArgs& a, int chanTiles, hipStream_t s) {
 static void launch_conv3(const ConvArgs& a, int n, hipStream_t s) {
     if (n % 128 == 0) launch_r<8, 1, 4, 2, 3, false>(a, n / 128, s);
     else if (n % 64 == 0 && a.cin >= 128) launch_r<8, 2, 2, 2, 3, false>(a, n / 64, s);
+    else if (n % 64 == 0 && a.cin == 64) launch_r<4, 2, 2, 3, 2, false>(a, n / 64, s);
     else if (a.cin > 32) launch_r<4, 4, 1, 3, 2, true>(a, (n + 31) / 32, s);
     else if (a.hw) launch_r<4, 4, 1, 2, 2, false, 32, true>(a, 1, s);
     else launch_r<4, 4, 1, 3, 2, false>(a, (n + 31) / 32, s);
@@ -808,8 +1051,9 @@ static void launch_conv3(const ConvArgs& a, int n, hipStream_t s) {
 // channels [0, up.cup) 2x upsampled from a half-resolution tensor; `yp`
 // receives the 2x2 max-pooled output.
 // `head` applies layer `head.li` (the 1x1 head) in the epilogue.
-struct UpSrc { const uint16_t* x = nullptr; int ld = 0, cup = 0; };
-struct HeadFuse { int li = -1; float* head = nullptr; const uint16_t* xres = nullptr; float* rgbOut = nullptr; };
+// With `skipOnly` the layer's x holds only its channels [up.cup, cin).
+struct UpSrc { const uint16_t* x = nullptr; int ld = 0, cup = 0; bool skipOnly = false; };
+struct HeadFuse { int li = -1; float* head = nullptr; const float* rgbIn = nullptr; float* rgbOut = nullptr; };
 static void conv_fwd(Net& N, int li, const uint16_t* x, int ldx, uint16_t* y, int ldy, int B, int H, int W, bool relu,
                      float* yf = nullptr, int ldyf = 0, UpSrc up = UpSrc(), uint16_t* yp = nullptr, int ldyp = 0,
                      HeadFuse hf = HeadFuse()) {
@@ -817,7 +1061,8 @@ static void conv_fwd(Net& N, int li, const uint16_t* x, int ldx, uint16_t* y, in
     const Layer* Lh = hf.li >= 0 ? &N.L[hf.li] : nullptr;
     ConvArgs a{x, ldx, L.d.cin, L.wf, L.wfF, N.w + L.bOff, y, ldy, yf, ldyf, L.d.cout, B, H, W, relu ? 1 : 0,
                up.x, up.ld, up.cup, yp, ldyp,
-               Lh ? Lh->wf : nullptr, Lh ? N.w + Lh->bOff : nullptr, hf.head, hf.xres, hf.rgbOut};
+               Lh ? Lh->wf : nullptr, Lh ? N.w + Lh->bOff : nullptr, hf.head, hf.rgbIn, hf.rgbOut,
+               up.skipOnly ? up.cup : 0};
     if (Lh && !(L.d.taps == 9 && L.d.cin == 32 && L.d.cout == 32 && Lh->d.cin == 32 && Lh->d.cout == 3)) {
         rt::set_error("denoiser: head fusion needs a 32 -> 32 3x3 layer and a 32 -> 3 head");
         return;
@@ -837,6 +1082,16 @@ static void conv_fwd(Net& N, int li, const uint16_t* x, int ldx, uint16_t* y, in
         if (WN == 2) launch_conv<9, 16, 2>(a, P, L.d.cout, s);
         else launch_conv<9, 16, 4>(a, P, L.d.cout, s);
     } else launch_conv3(a, L.d.cout, s);
+}
+
+// The first two layers (16 -> 32 -> 32 channels, ReLU after each) in one
+// kernel from N.X; y0out (training) receives the first layer's output.
+static void conv01_fwd(Net& N, uint16_t* y0out, uint16_t* y, int ldy, uint16_t* yp, int ldyp, int B, int H, int W) {
+    const Layer& L0 = N.L[0];
+    const Layer& L1 = N.L[1];
+    const Conv01Args a{N.X, L0.wfF, N.w + L0.bOff, L1.wfF, N.w + L1.bOff, y0out, y, ldy, yp, ldyp, B, H, W};
+    const dim3 grid(((W + kC01Px - 1) / kC01Px) * ((H + kC01Rows - 1) / kC01Rows) * B);
+    hipLaunchKernelGGL(k_conv01, grid, dim3(kBlock), 0, N.stream, a);
 }
 
 // input gradient: dx = conv(g, flipped transposed weights)
@@ -906,31 +1161,38 @@ static void update_layouts(Net& N) {
     }
 }
 
-// forward from N.X (B x H x W x 16) to N.head (P x 3 FP32, before the residual)
 // forward from N.X (B x H x W x 16) to N.head (P x 3 FP32, before the
-// residual), or in inference with rgbOut straight to the denoised planes. The pooled encoder outputs come from the producing
-// convolutions' epilogues; the decoder convolutions read their upsampled
-// inputs straight from the half-resolution tensors. Training also stores the
-// upsampled channels in the concatenation buffers (the weight gradients read
-// them).
-static void forward(Net& N, int B, int H, int W, bool train, float* rgbOut = nullptr) {
+// residual), or in inference with rgbOut straight to the denoised planes
+// (residual from the noisy radiance planes rgbIn). The pooled encoder outputs come from the producing convolutions' epilogues; the
+// decoder convolutions read their upsampled inputs straight from the
+// half-resolution tensors. Training also stores the upsampled channels in the
+// concatenation buffers (the weight gradients read them), with the skip
+// channels after them in each pixel's row. Inference stores only the skip
+// channels, each pixel's row dense (32, 64, 128 channels): a 32-channel skip
+// row inside a 96-channel pixel (64 bytes at byte 128 of 192) fills half of
+// a 128-byte line, and the decoder layer fetched twice its bytes.
+static void forward(Net& N, int B, int H, int W, bool train, float* rgbOut = nullptr, const float* rgbIn = nullptr) {
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
-    conv_fwd(N, 0, N.X, 16, N.A0, 32, B, H, W, true);
-    conv_fwd(N, 1, N.A0, 32, N.CAT1 + 64, 96, B, H, W, true, nullptr, 0, UpSrc(), N.P1, 32);        // E1, P1
+    // skip tensors E1, E2, E3: pointer and channel stride
+    uint16_t* const e1 = train ? N.CAT1 + 64 : N.CAT1;
+    uint16_t* const e2 = train ? N.CAT2 + 128 : N.CAT2;
+    uint16_t* const e3 = train ? N.CAT3 + 256 : N.CAT3;
+    const int l1 = train ? 96 : 32, l2 = train ? 192 : 64, l3 = train ? 384 : 128;
+    conv01_fwd(N, train ? N.A0 : nullptr, e1, l1, N.P1, 32, B, H, W);                         // A0, E1, P1
     conv_fwd(N, 2, N.P1, 32, N.A2, 64, B, H1, W1, true);
-    conv_fwd(N, 3, N.A2, 64, N.CAT2 + 128, 192, B, H1, W1, true, nullptr, 0, UpSrc(), N.P2, 64);    // E2, P2
+    conv_fwd(N, 3, N.A2, 64, e2, l2, B, H1, W1, true, nullptr, 0, UpSrc(), N.P2, 64);      // E2, P2
     conv_fwd(N, 4, N.P2, 64, N.A4, 128, B, H2, W2, true);
-    conv_fwd(N, 5, N.A4, 128, N.CAT3 + 256, 384, B, H2, W2, true, nullptr, 0, UpSrc(), N.P3, 128);  // E3, P3
+    conv_fwd(N, 5, N.A4, 128, e3, l3, B, H2, W2, true, nullptr, 0, UpSrc(), N.P3, 128);    // E3, P3
     conv_fwd(N, 6, N.P3, 128, N.A6, 256, B, H3, W3, true);
     conv_fwd(N, 7, N.A6, 256, N.E4, 256, B, H3, W3, true);
     if (train) up(N, N.E4, 256, N.CAT3, 384, 256, B, H3, W3);
-    conv_fwd(N, 8, N.CAT3, 384, N.A8, 128, B, H2, W2, true, nullptr, 0, UpSrc{N.E4, 256, 256});
+    conv_fwd(N, 8, train ? N.CAT3 : e3, l3, N.A8, 128, B, H2, W2, true, nullptr, 0, UpSrc{N.E4, 256, 256, !train});
     conv_fwd(N, 9, N.A8, 128, N.D3, 128, B, H2, W2, true);
     if (train) up(N, N.D3, 128, N.CAT2, 192, 128, B, H2, W2);
-    conv_fwd(N, 10, N.CAT2, 192, N.A10, 64, B, H1, W1, true, nullptr, 0, UpSrc{N.D3, 128, 128});
+    conv_fwd(N, 10, train ? N.CAT2 : e2, l2, N.A10, 64, B, H1, W1, true, nullptr, 0, UpSrc{N.D3, 128, 128, !train});
     conv_fwd(N, 11, N.A10, 64, N.D2, 64, B, H1, W1, true);
     if (train) up(N, N.D2, 64, N.CAT1, 96, 64, B, H1, W1);
-    conv_fwd(N, 12, N.CAT1, 96, N.A12, 32, B, H, W, true, nullptr, 0, UpSrc{N.D2, 64, 64});
+    conv_fwd(N, 12, train ? N.CAT1 : e1, l1, N.A12, 32, B, H, W, true, nullptr, 0, UpSrc{N.D2, 64, 64, !train});
     if (train) {
         conv_fwd(N, 13, N.A12, 32, N.D1, 32, B, H, W, true);
         conv_fwd(N, 14, N.D1, 32, nullptr, 0, B, H, W, false, N.head, 3);
@@ -940,7 +1202,7 @@ static void forward(Net& N, int B, int H, int W, bool train, float* rgbOut = nul
         HeadFuse hf;
         hf.li = 14;
         hf.head = rgbOut ? nullptr : N.head;
-        hf.xres = N.X;
+        hf.rgbIn = rgbIn;
         hf.rgbOut = rgbOut;
         conv_fwd(N, 13, N.A12, 32, nullptr, 0, B, H, W, true, nullptr, 0, UpSrc(), nullptr, 0, hf);
     }
@@ -1207,7 +1469,7 @@ int cudaBVH_denoise(CudaDenoiser_t d, int width, int height, const float* rgb, c
     const int P = width * height;
     hipLaunchKernelGGL(dn::k_pack, dim3(dn::grid_for(P)), dim3(dn::kBlock), 0, N.stream, rgb, albedo, normal, depth, P,
                        N.X);
-    dn::forward(N, 1, height, width, false, out);
+    dn::forward(N, 1, height, width, false, out, rgb);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
