@@ -68,3 +68,17 @@ def test_null_handle_conventions(lib, pkg):
     assert lib.cudaBVH_build(None, 10) is None
     assert lib.cudaBVH_buildFromAABBs(None, 0) is None
     assert lib.cudaBVH_buildTLASFast(None, 0, None, None) == 0
+
+
+def test_fd_calls_reject_bad_arguments_on_the_host(lib, pkg):
+    """importBufferFd / readGPUMemoryFd refuse a negative fd, an empty
+    import and a range outside the allocation before any HIP call."""
+    assert lib.cudaBVH_importBufferFd(-1, 4096) is None
+    assert "fd -1" in pkg.last_error()
+    assert lib.cudaBVH_importBufferFd(3, 0) is None
+    host = np.zeros(4, np.float32)
+    assert lib.cudaBVH_readGPUMemoryFd(-1, 4096, 0, 16, host.ctypes.data) == -1
+    assert "outside" in pkg.last_error()
+    assert lib.cudaBVH_readGPUMemoryFd(3, 4096, 4090, 16, host.ctypes.data) == -1
+    assert lib.cudaBVH_readGPUMemoryFd(3, 4096, 8192, 0, host.ctypes.data) == -1
+    assert lib.cudaBVH_readGPUMemoryFd(3, 4096, 0, 16, None) == -1

@@ -803,6 +803,7 @@ void cudaBVH_traceToGPUAsync(CudaBVH_t bvh, int width, int height, float camOx, 
 
 void* cudaBVH_importBufferFd(int fd, uint64_t size) {
     rt::clear_error();
+    if (fd < 0 || size == 0) { set_error("cannot import fd %d of %llu bytes", fd, (unsigned long long)size); return nullptr; }
     hipExternalMemoryHandleDesc desc;
     memset(&desc, 0, sizeof desc);
     desc.type = hipExternalMemoryHandleTypeOpaqueFd;
@@ -823,6 +824,11 @@ void* cudaBVH_importBufferFd(int fd, uint64_t size) {
 int cudaBVH_readGPUMemoryFd(int fd, uint64_t allocationSize, uint64_t offset, uint64_t size, void* dst) {
     rt::clear_error();
     if (!dst) { set_error("cudaBVH_readGPUMemoryFd: NULL destination"); return -1; }
+    if (fd < 0 || offset > allocationSize || size > allocationSize - offset) {
+        set_error("cudaBVH_readGPUMemoryFd: fd %d, bytes [%llu, %llu) outside an allocation of %llu", fd,
+                  (unsigned long long)offset, (unsigned long long)(offset + size), (unsigned long long)allocationSize);
+        return -1;
+    }
     hipExternalMemoryHandleDesc desc;
     memset(&desc, 0, sizeof desc);
     desc.type = hipExternalMemoryHandleTypeOpaqueFd;
@@ -831,12 +837,16 @@ int cudaBVH_readGPUMemoryFd(int fd, uint64_t allocationSize, uint64_t offset, ui
     hipExternalMemory_t ext = nullptr;
     hipError_t e = hipImportExternalMemory(&ext, &desc);
     if (e != hipSuccess) { set_error("cannot import fd %d: %s", fd, hipGetErrorString(e)); return -1; }
+    // The range is mapped from the start of the allocation and the copy
+    // starts at `offset`: this HIP runtime returns the allocation's base for
+    // a mapped buffer whatever its descriptor's offset (measured on a
+    // dma-buf import), so the offset is applied here.
     hipExternalMemoryBufferDesc bd;
     memset(&bd, 0, sizeof bd);
-    bd.offset = offset; bd.size = size;
+    bd.offset = 0; bd.size = offset + size;
     void* p = nullptr;
     e = hipExternalMemoryGetMappedBuffer(&p, ext, &bd);
-    if (e == hipSuccess) e = hipMemcpy(dst, p, size, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(dst, (const char*)p + offset, size, hipMemcpyDeviceToHost);
     if (p) hipFree(p);
     hipDestroyExternalMemory(ext);
     if (e != hipSuccess) { set_error("readGPUMemoryFd failed: %s", hipGetErrorString(e)); return -1; }
