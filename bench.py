@@ -1,7 +1,10 @@
 """Benchmark: closest-hit Mrays/s on the 3,407,872-triangle terrain scene.
 
 One step = one frame of the closest-hit chain on one MI355X, inputs resident
-in HBM: a fused 1920x1080 pinhole primary trace (coherent class), secondary-ray
+in HBM: a fused 1920x1080 pinhole primary trace (coherent class; its 8x8
+tiles start slowest first by the previous launch's per-tile step counts, as
+a renderer tracing consecutive frames would run it; the plain order and a
+moving camera are timed beside it under order_comparison), secondary-ray
 generation from the primary hits (one cosine-weighted diffuse bounce and one
 shadow ray per hit, stable wave64 compaction), and the closest-hit trace of
 the diffuse rays (incoherent class), read off a device-side ray count. The
@@ -36,6 +39,49 @@ RAY_KEYS = ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def coherent_order_comparison(pkg, scenes, bvh, W, H, cam, out, stream, steps=20, orbit_frames=30):
+    """Coherent-class launch times (ms, HIP events) in the plain tile order
+    and in the cost order: on the bench camera, and on a camera orbiting the
+    target by 1 and 3 degrees per frame, where each cost-ordered launch uses
+    the previous frame's costs."""
+    import math
+    import torch
+
+    def timed(cams, flags):
+        evs = []
+        for cc in cams:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            bvh.trace_pinhole_closest(W, H, cc, *out, flags=flags)
+            e1.record(stream)
+            evs.append((e0, e1))
+        torch.cuda.synchronize()
+        return np.array([a.elapsed_time(b) for a, b in evs])
+
+    res = {}
+    timed([cam] * 3, 0)
+    res["static_plain_ms"] = round(float(timed([cam] * steps, 0).mean()), 4)
+    timed([cam] * 3, pkg.FLAG_COST_ORDER)
+    res["static_cost_ordered_ms"] = round(float(timed([cam] * steps, pkg.FLAG_COST_ORDER).mean()), 4)
+    eye, tgt = np.array(scenes.TERRAIN_EYE, np.float64), np.array(scenes.TERRAIN_TARGET, np.float64)
+    for deg in (1.0, 3.0):
+        cams = []
+        for k in range(orbit_frames):
+            a, d = math.radians(deg * k), eye - tgt
+            e = tgt + np.array([d[0] * math.cos(a) - d[2] * math.sin(a), d[1], d[0] * math.sin(a) + d[2] * math.cos(a)])
+            cc = scenes.look_at(tuple(e), scenes.TERRAIN_TARGET, W, H, 60.0)
+            cams.append(pkg.camera(cc["o"], cc["f"], cc["r"], cc["u"], cc["scale_x"], cc["scale_y"]))
+        timed(cams[:2], 0)
+        plain = timed(cams, 0)[1:].mean()
+        timed(cams[:1], pkg.FLAG_COST_ORDER)
+        ordered = timed(cams, pkg.FLAG_COST_ORDER)[1:].mean()
+        res[f"orbit_{deg:g}deg_per_frame"] = {"plain_ms": round(float(plain), 4),
+                                              "cost_ordered_ms": round(float(ordered), 4)}
+    # leave the bench camera's costs in place for the launches that follow
+    timed([cam] * 2, pkg.FLAG_COST_ORDER)
+    return res
 
 
 def main():
@@ -102,7 +148,8 @@ def main():
 
     def step(ev=None):
         if ev: ev[0].record(stream)
-        bvh.trace_pinhole_closest(W, H, c, pt, pid, pu, pv)
+        # tiles ordered by the previous launch's per-tile step counts (slowest first)
+        bvh.trace_pinhole_closest(W, H, c, pt, pid, pu, pv, flags=pkg.FLAG_COST_ORDER)
         if ev: ev[1].record(stream)
         bvh.gen_bounce_rays(W, H, c, pt, pid, 1234, light, D, S, src, cnt, flags=pkg.FLAG_TILE_ORDER)
         if ev: ev[2].record(stream)
@@ -156,6 +203,13 @@ def main():
     occ_rate = float(occ[:m].float().mean().item())
     log(f"primary {prim_ms:.3f} ms  bounce {bounce_ms:.3f} ms  diffuse {diff_ms:.3f} ms  shadow {shadow_ms:.3f} ms"
         f"  (wall {wall_s / args.steps * 1e3:.3f} ms/step incl. shadow)")
+
+    # The coherent class without the cost order, and both orders on a moving
+    # camera (orbiting the target, so the previous launch's costs are a
+    # prediction, not the same frame's).
+    with torch.cuda.stream(stream):
+        order_cmp = coherent_order_comparison(pkg, scenes, bvh, W, H, c, (pt, pid, pu, pv), stream)
+    log("coherent tile order: " + json.dumps(order_cmp))
 
     # Occupancy of each class's launch (VGPRs, LDS, workgroups and waves per
     # SIMD), read back after one more untimed launch of each.
@@ -307,7 +361,11 @@ def main():
             "coherent_closest": {"rays": n, "ms": round(prim_ms, 4), "mrays_s": round(n / (prim_ms * 1e3), 1),
                                  "hit_rate": round(hit_rate, 4), "nodes_per_ray": round(p_nodes / n, 2),
                                  "tris_per_ray": round(p_tris / n, 2),
-                                 "alg_GBps": round(prim_bytes / (prim_ms * 1e-3) / 1e9, 1)},
+                                 "alg_GBps": round(prim_bytes / (prim_ms * 1e-3) / 1e9, 1),
+                                 "tile_order": "cost: each XCD band's 8x8 tiles predicted at >= 48 steps first, "
+                                               "slowest first, by the previous launch's per-tile step counts "
+                                               "dilated by 2 tiles (CUDABVH_FLAG_COST_ORDER); the rest in strip order",
+                                 "order_comparison": order_cmp},
             "incoherent_closest": {"rays": m, "ms": round(diff_ms, 4), "mrays_s": round(m / (diff_ms * 1e3), 1),
                                    "hit_rate": round(diff_hit, 4), "nodes_per_ray": round(d_nodes / max(m, 1), 2),
                                    "tris_per_ray": round(d_tris / max(m, 1), 2)},

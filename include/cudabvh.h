@@ -171,7 +171,9 @@ enum {
     CUDABVH_FLAG_SYNC          = 1u << 6, /* wait for completion before returning */
     CUDABVH_FLAG_GRAPH         = 1u << 7, /* capture the launches in a hipGraph once, replay them */
     CUDABVH_FLAG_CWBVH         = 1u << 8, /* trace the compressed 8-wide BVH (built on first use) */
-    CUDABVH_FLAG_TILE_ORDER    = 1u << 9  /* genBounceRays: compact in 8x8-tile order, not pixel order */
+    CUDABVH_FLAG_TILE_ORDER    = 1u << 9, /* genBounceRays: compact in 8x8-tile order, not pixel order */
+    CUDABVH_FLAG_COST_ORDER    = 1u << 10 /* tracePinholeClosest: start the 8x8 tiles that were slowest
+                                             in the previous launch of the same size first */
 };
 
 /* Build from geometry already in device memory (no counterpart in the
@@ -246,7 +248,13 @@ int cudaBVH_genPinholeRays(int width, int height, const CudaCamera* cam,
                            const CudaRayPacket* out, void* stream);
 
 /* Fused generation + closest hit for a pinhole camera (no ray buffers:
- * coherent primary rays). Device hit packet of W*H entries. */
+ * coherent primary rays). Device hit packet of W*H entries.
+ * CUDABVH_FLAG_COST_ORDER (batch schedule): the handle keeps each 8x8 tile's
+ * step count of the last launch of this image size, and the next launch
+ * starts the tiles predicted slowest first (a renderer tracing consecutive
+ * frames; the hits are the same in any order). Extension of this build:
+ * the reference's primary trace (layer/cuda_bvh_backend.cu:761-800) has
+ * no tile order. */
 int cudaBVH_tracePinholeClosest(CudaBVH_t bvh, int width, int height, const CudaCamera* cam,
                                 const CudaHitPacket* hits, unsigned flags);
 

@@ -7,7 +7,8 @@
   cosine-weighted diffuse bounce and the shadow ray of every hit (secondary
   rays equal the numpy restatement), diffuse closest hit and shadow any hit
   under the tuned launch schedules, the ray-sorted trace, and the
-  reference's host-pointer calls.
+  reference's host-pointer calls; the cost-ordered primary launches equal
+  the plain ones.
 * Instanced path tracer: 64 instances of 4 meshes (10K / 50K / 200K / 500K
   triangles) at 1920 x 1080, 1 spp, depth 8, Russian roulette from ray 3 —
   image, first-hit buffers and per-bounce ray counts bit-exact against the
@@ -146,6 +147,47 @@ def test_terrain_frame_parity(pkg, oracle, scenes, dev, terrain):
     assert _eq(hit, rp["t"])
     assert bvh.trace_status() == 0
 
+
+
+def test_terrain_cost_ordered_primaries(pkg, oracle, scenes, dev, terrain):
+    """CUDABVH_FLAG_COST_ORDER only reorders the 8x8 tiles: three launches in
+    a row (the first in the plain order, the next two in the order of the
+    previous launch's step counts), then a ragged image size (its costs
+    start again from zero) and the 1080p frame again, each into outputs
+    filled with a sentinel first, equal the oracle's hits (or the plain-order
+    trace at the ragged size) bit for bit, so every tile is traced exactly
+    once."""
+    import torch
+    tris, bvh, ob = terrain
+    nodes, lt, pid = ob.nodes(), ob.leaf_tris(), ob.order()
+
+    def run(w, h, flags):
+        cam = scenes.terrain_camera(w, h)
+        c = pkg.camera(cam["o"], cam["f"], cam["r"], cam["u"], cam["scale_x"], cam["scale_y"])
+        out = [torch.full((w * h,), -7.0, device=dev), torch.full((w * h,), -7, dtype=torch.int32, device=dev),
+               torch.full((w * h,), -7.0, device=dev), torch.full((w * h,), -7.0, device=dev)]
+        bvh.trace_pinhole_closest(w, h, c, *out, flags=flags)
+        torch.cuda.synchronize()
+        return [x.cpu().numpy() for x in out], scenes.cam_array(cam)
+
+    def same(a, b):
+        return np.array_equal(a[1], b[1]) and all(_eq(a[k], b[k]) for k in (0, 2, 3))
+
+    print("cost-ordered primaries: 1080p x 3", flush=True)
+    first, cam14 = run(W, H, 0)
+    ref = oracle.trace_closest(nodes, lt, pid, oracle.gen_pinhole(cam14, W, H), counts=False)
+    assert same(first, [ref["t"], ref["id"], ref["u"], ref["v"]])
+    for k in range(3):
+        got, _ = run(W, H, pkg.FLAG_COST_ORDER)
+        assert same(got, first), f"cost-ordered launch {k} differs"
+    print("cost-ordered primaries: ragged 1001 x 563, then 1080p", flush=True)
+    plain, _ = run(1001, 563, 0)
+    for k in range(2):
+        got, _ = run(1001, 563, pkg.FLAG_COST_ORDER)
+        assert same(got, plain)
+    got, _ = run(W, H, pkg.FLAG_COST_ORDER)
+    assert same(got, first)
+    assert bvh.trace_status() == 0
 
 def test_instanced_path_tracer_full_frame(pkg, oracle, scenes, dev):
     """The instanced path tracer's 1080p frame, bit-exact, eager and graph."""

@@ -33,6 +33,7 @@ FLAG_SYNC = 1 << 6
 FLAG_GRAPH = 1 << 7
 FLAG_CWBVH = 1 << 8
 FLAG_TILE_ORDER = 1 << 9
+FLAG_COST_ORDER = 1 << 10
 PT_MAX_DEPTH = 16
 MISS_T = np.float32(1e30)
 

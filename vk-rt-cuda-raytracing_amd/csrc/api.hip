@@ -1171,6 +1171,7 @@ int cudaBVH_tracePinholeClosest(CudaBVH_t bvh, int width, int height, const Cuda
     a.n = width * height; a.W = width; a.H = height;
     a.cam = to_cam(*cam);
     a.t = hits->t; a.id = hits->primId; a.u = hits->u; a.v = hits->v;
+    a.costOrder = (flags & CUDABVH_FLAG_COST_ORDER) != 0;
     int rc = rtk::launch_trace(bvh, bvh->scratch, a, rtk::kModeClosest, isect == 1, true, false, hstream(bvh));
     if (rc == 0 && (flags & CUDABVH_FLAG_SYNC)) return finish_sync(bvh->scratch, hstream(bvh), "cudaBVH_tracePinholeClosest");
     return rc;

@@ -318,6 +318,13 @@ __device__ __forceinline__ float4 ldg_f4(const float4* p) {
 constexpr int kBands = 8;
 constexpr int kBandStride = 32;
 
+// Maximum of v over the 64 lanes of a wave (every lane active).
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 __device__ __forceinline__ int take_chunk(uint32_t* ctr, int n, int& band) {
     const int nc = (n + 63) >> 6;
     for (int k = 0; k < kBands; ++k) {

@@ -108,6 +108,13 @@ struct TraceScratch {
     size_t tileCap = 0;
     void* hits = nullptr;               // staged closest-hit records (16 bytes per ray)
     size_t hitsBytes = 0;
+    // cost-ordered pinhole launches: per-tile step counts of the last launch
+    // of a costW x costH image (row-major by tile position), the predicted
+    // costs in chunk order and the order built from them
+    int* tileCost = nullptr;
+    int* tileOrder = nullptr;
+    int* tilePred = nullptr;
+    int costCap = 0, costW = 0, costH = 0;
     // Bumped whenever a buffer above is freed or reallocated: a captured graph
     // whose kernels hold these addresses is valid only while gen is unchanged.
     uint64_t gen = 0;

@@ -29,6 +29,7 @@ using rtd::Ray;
 // Zeroes the kBands work-distribution counters (launched with kBands threads).
 __global__ void k_reset_counter(uint32_t* c) { c[threadIdx.x * rtd::kBandStride] = 0u; }
 
+
 // Face-normal Lambert + rim shading of a primary hit, sky gradient on a miss
 // (backend :923-984; IEEE 1/sqrt and an explicit cube in place of rsqrtf and
 // powf).
@@ -133,6 +134,7 @@ struct Result {
     int id, ti;        // original primitive ID and leaf-order index (-1: miss)
     bool occluded;
     int visits, tests;
+    int steps;         // loop steps of the ray (batch schedule)
 };
 
 // Closest-hit result of ray ri. The while-while kernel, whose lanes finish
@@ -197,13 +199,98 @@ __device__ __forceinline__ int ray_root(const TraceArgs& a, int ri) {
 // coherent traces than row-major order on the 1080p terrain frame.
 constexpr int kStripTiles = 8;
 
-__device__ __forceinline__ void pin_tile(const TraceArgs& a, int tile, int& tx, int& ty) {
-    const int tilesY = a.nTiles / a.tilesX;
-    const int full = a.tilesX / kStripTiles, per = kStripTiles * tilesY;
+__device__ __forceinline__ void strip_tile(int tilesX, int nTiles, int tile, int& tx, int& ty) {
+    const int tilesY = nTiles / tilesX;
+    const int full = tilesX / kStripTiles, per = kStripTiles * tilesY;
     int strip = tile / per, r = tile - strip * per, w = kStripTiles;
-    if (strip >= full) { strip = full; r = tile - full * per; w = a.tilesX - full * kStripTiles; }
+    if (strip >= full) { strip = full; r = tile - full * per; w = tilesX - full * kStripTiles; }
     ty = r / w;
     tx = strip * kStripTiles + (r - ty * w);
+}
+__device__ __forceinline__ void pin_tile(const TraceArgs& a, int tile, int& tx, int& ty) {
+    strip_tile(a.tilesX, a.nTiles, tile, tx, ty);
+}
+
+// Cost order of the pinhole tiles (batch schedule). The launch's end is set
+// by its slowest 8x8 tiles (grazing rays: up to 7x the mean step count), and
+// in the plain order they start wherever they fall in their band. Ordering
+// each XCD band's tiles by the step counts the previous launch of the same
+// image size recorded, slowest first (longest-processing-time-first list
+// scheduling), starts those chains at the launch's beginning, where they
+// overlap the rest of the work. A tile's predicted cost is the largest
+// recorded count within kCostReach tiles of it (k_cost_dilate), so a camera
+// that moved a few tiles since the last launch still finds its slow tiles
+// in front: the slow tiles are thin features (silhouettes, grazing slopes)
+// that a shift of one tile misses entirely. The order itself: one workgroup
+// per band (the bands of rtd::take_chunk, so a band's tiles stay on its
+// XCD), a stable counting sort, descending, on kCostBuckets buckets of
+// kCostShift-bit predicted costs for the tiles predicted at kCostHeavy steps
+// or more; every other tile follows in its strip order, which keeps the L2
+// locality of the cheap majority. Only the schedule changes: every tile is
+// traced once and each ray's result is the same.
+constexpr int kCostBuckets = 32, kCostShift = 3, kCostHeavy = 48, kCostReach = 2, kOrderBlock = 1024;
+
+// pred[c] = the largest cost (row-major by tile position) within kCostReach
+// tiles of chunk c's tile
+__global__ void k_cost_dilate(const int* cost, int tilesX, int nTiles, int* pred) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nTiles) return;
+    const int tilesY = nTiles / tilesX;
+    int tx, ty;
+    strip_tile(tilesX, nTiles, c, tx, ty);
+    int m = 0;
+    for (int y = max(ty - kCostReach, 0); y <= min(ty + kCostReach, tilesY - 1); y++)
+        for (int x = max(tx - kCostReach, 0); x <= min(tx + kCostReach, tilesX - 1); x++) m = max(m, cost[y * tilesX + x]);
+    pred[c] = m;
+}
+
+__global__ __launch_bounds__(kOrderBlock) void k_tile_order(const int* cost, int nTiles, int* order) {
+    __shared__ int base[kCostBuckets];            // first slot of each bucket in the band
+    __shared__ int run[kCostBuckets];             // tiles of each bucket placed so far
+    __shared__ int wcnt[kOrderBlock / 64][kCostBuckets];
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int lo = (int)(((long long)nTiles * b) / rtd::kBands);
+    const int hi = (int)(((long long)nTiles * (b + 1)) / rtd::kBands);
+    // bucket 0 first: heavy tiles by descending cost, the rest in bucket kCostBuckets - 1
+    auto bucket = [&](int c) {
+        const int v = cost[c];
+        return v >= kCostHeavy ? kCostBuckets - 2 - min(v >> kCostShift, kCostBuckets - 2) : kCostBuckets - 1;
+    };
+    if (tid < kCostBuckets) run[tid] = 0;
+    __syncthreads();
+    for (int c = lo + tid; c < hi; c += kOrderBlock) atomicAdd(&run[bucket(c)], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int k = 0; k < kCostBuckets; k++) { base[k] = acc; acc += run[k]; run[k] = 0; }
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1;
+    for (int c0 = lo; c0 < hi; c0 += kOrderBlock) {
+        const int c = c0 + tid;
+        const bool ok = c < hi;
+        const int k = ok ? bucket(c) : -1;
+        int rank = 0;
+        for (int j = 0; j < kCostBuckets; j++) {
+            const uint64_t m = __ballot(k == j);
+            if (k == j) rank = __popcll(m & below);
+            if (lane == 0) wcnt[w][j] = __popcll(m);
+        }
+        __syncthreads();
+        if (tid < kCostBuckets) {                  // per bucket: exclusive prefix over the waves
+            int r = run[tid];
+            for (int v = 0; v < kOrderBlock / 64; v++) { const int x = wcnt[v][tid]; wcnt[v][tid] = r; r += x; }
+            run[tid] = r;
+        }
+        __syncthreads();
+        if (ok) order[lo + base[k] + wcnt[w][k] + rank] = c;
+        __syncthreads();
+    }
+}
+
+__global__ void k_fill_i32(int* p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // Batch schedule: traverses one ray in one loop. Each iteration is one short
@@ -326,7 +413,7 @@ __device__ __forceinline__ void trace_ray(const TraceArgs& a, const Ray& r, int*
     }
     __builtin_amdgcn_s_setprio(0);
     res.t = best; res.u = bu; res.v = bv; res.id = bid; res.ti = bti;
-    res.occluded = found; res.visits = visits; res.tests = tests;
+    res.occluded = found; res.visits = visits; res.tests = tests; res.steps = steps;
 }
 
 // Refill schedule: the batch schedule's one-short-step loop (a node visit or
@@ -580,13 +667,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void
         int ri = base + lane;
         int py = 0;
         bool live;
+        int tile = 0, tx = 0, ty = 0;
         if constexpr (PIN) {
-            const int tile = (int)base >> 6;
-            int tx, ty;
+            // chunk c traces tile c, or tileOrder[c] in a cost-ordered launch
+            tile = (int)base >> 6;
+            if (a.tileOrder) tile = a.tileOrder[tile];
             pin_tile(a, tile, tx, ty);
             const int px = tx * 8 + (lane & 7);
             py = ty * 8 + (lane >> 3);
-            live = px < a.W && py < a.H;
+            live = (unsigned)tile < (unsigned)a.nTiles && px < a.W && py < a.H;
             ri = py * a.W + px;
         } else {
             live = ri < n;
@@ -608,26 +697,37 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void
         const bool anyFlat = __any(live && rtd::ray_flat(r));
         // the live lanes' shared watertight axis permutation (-1: mixed)
         const int pc = MT ? -1 : rtd::wave_perm(r, live);
-        if (!live) continue;
-        Result res;
-        const int root = ray_root(a, ri);
-        if (anyFlat) trace_ray<MODE, MT, true, LDSD, BLOCK>(a, r, s_ref, s_tn, tid, gtid, res, root, pc);
-        else trace_ray<MODE, MT, false, LDSD, BLOCK>(a, r, s_ref, s_tn, tid, gtid, res, root, pc);
+        int bsteps = 0;
+        if (live) {
+            Result res;
+            const int root = ray_root(a, ri);
+            if (anyFlat) trace_ray<MODE, MT, true, LDSD, BLOCK>(a, r, s_ref, s_tn, tid, gtid, res, root, pc);
+            else trace_ray<MODE, MT, false, LDSD, BLOCK>(a, r, s_ref, s_tn, tid, gtid, res, root, pc);
 
-        if constexpr (MODE == kModeAny) {
-            a.occ[ri] = res.occluded ? 1 : 0;
-        } else if constexpr (MODE == kModeClosest) {
-            const bool hit = res.id >= 0;
-            a.t[ri] = hit ? res.t : RT_MISS_T;
-            if (a.id) a.id[ri] = res.id;
-            if (a.u) a.u[ri] = hit ? res.u : 0.f;
-            if (a.v) a.v[ri] = hit ? res.v : 0.f;
-        } else {
-            shade_pixel(a, r, ri, py, res.id >= 0, res.ti);
+            if constexpr (MODE == kModeAny) {
+                a.occ[ri] = res.occluded ? 1 : 0;
+            } else if constexpr (MODE == kModeClosest) {
+                const bool hit = res.id >= 0;
+                a.t[ri] = hit ? res.t : RT_MISS_T;
+                if (a.id) a.id[ri] = res.id;
+                if (a.u) a.u[ri] = hit ? res.u : 0.f;
+                if (a.v) a.v[ri] = hit ? res.v : 0.f;
+            } else {
+                shade_pixel(a, r, ri, py, res.id >= 0, res.ti);
+            }
+            if constexpr (COUNT) {
+                a.cntNodes[ri] = res.visits;
+                a.cntTris[ri] = res.tests;
+            }
+            bsteps = res.steps;
         }
-        if constexpr (COUNT) {
-            a.cntNodes[ri] = res.visits;
-            a.cntTris[ri] = res.tests;
+        // a cost-ordered launch records the batch's step count (its slowest
+        // lane's) for the next launch's order, row-major by tile position
+        if constexpr (PIN && !COUNT) {
+            if (a.tileCost) {
+                const int m = rtd::wave_max(bsteps);
+                if (lane == 0 && (unsigned)tile < (unsigned)a.nTiles) a.tileCost[ty * a.tilesX + tx] = m;
+            }
         }
     }
 }
@@ -1038,6 +1138,35 @@ static int launch_kernel(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs
         a.tilesX = (a.W + 7) / 8;
         a.nTiles = a.tilesX * ((a.H + 7) / 8);
     }
+    a.tileOrder = nullptr;
+    a.tileCost = nullptr;
+    if (a.costOrder && a.W > 0 && !sc.fixed) {
+        if (a.nTiles > sc.costCap) {
+            hipFree(sc.tileCost); hipFree(sc.tileOrder); hipFree(sc.tilePred);
+            sc.tileCost = sc.tileOrder = sc.tilePred = nullptr; sc.costCap = 0; sc.costW = sc.costH = 0;
+            sc.gen++;
+            if (hipMalloc((void**)&sc.tileCost, (size_t)a.nTiles * sizeof(int)) != hipSuccess ||
+                hipMalloc((void**)&sc.tileOrder, (size_t)a.nTiles * sizeof(int)) != hipSuccess ||
+                hipMalloc((void**)&sc.tilePred, (size_t)a.nTiles * sizeof(int)) != hipSuccess) {
+                hipFree(sc.tileCost); hipFree(sc.tileOrder); hipFree(sc.tilePred);
+                sc.tileCost = sc.tileOrder = sc.tilePred = nullptr;
+                rt::set_error("cannot allocate the tile cost order of %d tiles", a.nTiles);
+                return -1;
+            }
+            sc.costCap = a.nTiles;
+        }
+        if (sc.costW != a.W || sc.costH != a.H) {
+            // no costs for this image size yet: the first launch takes the plain order
+            hipLaunchKernelGGL(k_fill_i32, dim3((a.nTiles + 255) / 256), dim3(256), 0, s, sc.tileCost, a.nTiles, 0);
+            sc.costW = a.W; sc.costH = a.H;
+        }
+        hipLaunchKernelGGL(k_cost_dilate, dim3((a.nTiles + 255) / 256), dim3(256), 0, s, sc.tileCost, a.tilesX,
+                           a.nTiles, sc.tilePred);
+        hipLaunchKernelGGL(k_tile_order, dim3(rtd::kBands), dim3(kOrderBlock), 0, s, sc.tilePred, a.nTiles,
+                           sc.tileOrder);
+        a.tileOrder = sc.tileOrder;
+        a.tileCost = sc.tileCost;
+    }
     a.spill = (int2*)sc.spill;
     a.spillStride = perWave ? threads / 64 : threads;
     a.counter = sc.counter;
@@ -1072,12 +1201,15 @@ static int launch_one(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a
     // the while-while kernel stages hit records for k_hits_split; the refill
     // kernel gathers them per pool in LDS
     constexpr bool stage = MODE == kModeClosest && !PIN && !COUNT && SCHED == rt::kScheduleWhileWhile;
+    // the cost order applies to uncounted pinhole launches of the batch schedule
+    if (!(PIN && !COUNT && SCHED == rt::kScheduleBatch)) a.costOrder = false;
     return launch_kernel(h, sc, a, kern, LDSD, false, blocksPerCU, ki, s, stage);
 }
 
 template <int MODE, bool MT, int LDSD>
 static int launch_packet(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int blocksPerCU, hipStream_t s) {
     static rt::KernelInfo ki;
+    a.costOrder = false;
     return launch_kernel(h, sc, a, k_trace_packet<MODE, MT, LDSD, kBlock>, LDSD, true, blocksPerCU, ki, s);
 }
 
@@ -1094,8 +1226,10 @@ int alloc_scratch(rt::TraceScratch& sc) {
 
 void free_scratch(rt::TraceScratch& sc) {
     hipFree(sc.spill); hipFree(sc.counter); hipFree(sc.status); hipFree(sc.tiles); hipFree(sc.hits);
+    hipFree(sc.tileCost); hipFree(sc.tileOrder); hipFree(sc.tilePred);
     sc.spill = nullptr; sc.spillBytes = 0; sc.counter = nullptr; sc.status = nullptr;
     sc.tiles = nullptr; sc.tileCap = 0; sc.hits = nullptr; sc.hitsBytes = 0;
+    sc.tileCost = sc.tileOrder = sc.tilePred = nullptr; sc.costCap = 0; sc.costW = sc.costH = 0;
     sc.gen++;
 }
 

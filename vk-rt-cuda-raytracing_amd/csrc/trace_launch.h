@@ -46,6 +46,10 @@ struct TraceArgs {
     rtd::Cam cam{};
     int W = 0, H = 0;
     int tilesX = 0, nTiles = 0;         // pinhole: 8x8 pixel tiles, one per wave
+    // pinhole batch launches with a cost order: chunk c traces tile
+    // tileOrder[c], and each tile's step count goes to tileCost[tile]
+    const int* tileOrder = nullptr; int* tileCost = nullptr;
+    bool costOrder = false;             // requested by the caller (set before launch_trace)
     // outputs
     float* t = nullptr; int* id = nullptr; float* u = nullptr; float* v = nullptr;
     uint4* hits = nullptr;              // closest hits staged as one (t, id, u, v) record per ray
