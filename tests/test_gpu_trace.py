@@ -423,7 +423,10 @@ def test_launch_info_reports_configuration(pkg, parity_scene, parity_bvh, dev):
 def test_fused_pinhole_odd_sizes(pkg, oracle, scenes, parity_scene, parity_bvh, dev, W, H):
     """Image sizes that leave partial 8x8 tiles, a partial last tile strip,
     fewer tiles than XCD bands or a single pixel: every schedule writes
-    every pixel exactly as the oracle traces it, and nothing past W*H."""
+    every pixel exactly as the oracle traces it, and nothing past W*H; also
+    with the cost order (the batch schedule's second cost-ordered launch
+    takes the order of the first one's step counts; other schedules ignore
+    the flag)."""
     import torch
     _, cam = parity_scene
     c = pkg.camera(cam["o"], cam["f"], cam["r"], cam["u"], cam["scale_x"], cam["scale_y"])
@@ -433,13 +436,14 @@ def test_fused_pinhole_odd_sizes(pkg, oracle, scenes, parity_scene, parity_bvh, 
     saved = parity_bvh.trace_config(pkg.CLASS_COHERENT)
     for sched in (pkg.SCHEDULE_BATCH, pkg.SCHEDULE_WHILE_WHILE, pkg.SCHEDULE_REFILL, pkg.SCHEDULE_PACKET):
         parity_bvh.set_trace_config(pkg.CLASS_COHERENT, sched, 8, 0)
-        hb = {k: torch.full((W * H + 64,), -3, dtype=torch.int32 if k == "prim_id" else torch.float32, device=dev)
-              for k in ("t", "prim_id", "u", "v")}
-        parity_bvh.trace_pinhole_closest(W, H, c, hb["t"], hb["prim_id"], hb["u"], hb["v"])
-        torch.cuda.synchronize()
-        got = {k: v.cpu().numpy() for k, v in hb.items()}
-        _assert_closest_equal({k: v[:W * H] for k, v in got.items()}, ref)
-        assert (got["prim_id"][W * H:] == -3).all(), sched
+        for flags in (0, pkg.FLAG_COST_ORDER, pkg.FLAG_COST_ORDER):
+            hb = {k: torch.full((W * H + 64,), -3, dtype=torch.int32 if k == "prim_id" else torch.float32, device=dev)
+                  for k in ("t", "prim_id", "u", "v")}
+            parity_bvh.trace_pinhole_closest(W, H, c, hb["t"], hb["prim_id"], hb["u"], hb["v"], flags=flags)
+            torch.cuda.synchronize()
+            got = {k: v.cpu().numpy() for k, v in hb.items()}
+            _assert_closest_equal({k: v[:W * H] for k, v in got.items()}, ref)
+            assert (got["prim_id"][W * H:] == -3).all(), (sched, flags)
     parity_bvh.set_trace_config(pkg.CLASS_COHERENT, *saved)
 
 
