@@ -298,7 +298,9 @@ int cudaBVH_getTraceConfig(CudaBVH_t bvh, int rayClass, int* schedule, int* ldsD
  * caller's workload (coherent: a W x H pinhole frame; otherwise numRays
  * device rays), reps timed launches each, and keeps the fastest in the
  * handle. results (nullable, room for 192 floats: schedule, ldsDepth,
- * blocksPerCU, ms per candidate) and *numResults report the sweep. */
+ * blocksPerCU, ms per candidate) and *numResults report the sweep. The
+ * coherent class's batch candidates are timed in the tile cost order of
+ * CUDABVH_FLAG_COST_ORDER. */
 int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const CudaCamera* cam,
                      int numRays, const CudaRayPacket* rays, int reps, float* results, int* numResults);
 

@@ -1283,7 +1283,9 @@ int cudaBVH_autotune(CudaBVH_t bvh, int rayClass, int width, int height, const C
     HIP_TRY(hipMalloc(&out, size_t(n) * 16), return -1);
     rtk::TraceArgs base;
     base.n = n;
-    if (coherent) { base.W = width; base.H = height; base.cam = to_cam(*cam); }
+    // coherent candidates of the batch schedule run in the cost order, as
+    // the frame-after-frame callers of tracePinholeClosest run them
+    if (coherent) { base.W = width; base.H = height; base.cam = to_cam(*cam); base.costOrder = true; }
     else {
         base.ox = rays->ox; base.oy = rays->oy; base.oz = rays->oz; base.dx = rays->dx; base.dy = rays->dy;
         base.dz = rays->dz; base.tmin = rays->tmin; base.tmax = rays->tmax;

@@ -223,10 +223,10 @@ __device__ __forceinline__ void pin_tile(const TraceArgs& a, int tile, int& tx, 
 // in front: the slow tiles are thin features (silhouettes, grazing slopes)
 // that a shift of one tile misses entirely. The order itself: one workgroup
 // per band (the bands of rtd::take_chunk, so a band's tiles stay on its
-// XCD), a stable counting sort, descending, on kCostBuckets buckets of
-// kCostShift-bit predicted costs for the tiles predicted at kCostHeavy steps
-// or more; every other tile follows in its strip order, which keeps the L2
-// locality of the cheap majority. Only the schedule changes: every tile is
+// XCD): the tiles predicted at kCostHeavy steps or more first, by
+// descending cost in kCostBuckets buckets of kCostShift-bit costs; every
+// other tile follows in its strip order (a stable partition), which keeps
+// the L2 locality of the cheap majority. Only the schedule changes: every tile is
 // traced once and each ray's result is the same.
 constexpr int kCostBuckets = 32, kCostShift = 3, kCostHeavy = 48, kCostReach = 2, kOrderBlock = 1024;
 
@@ -245,46 +245,57 @@ __global__ void k_cost_dilate(const int* cost, int tilesX, int nTiles, int* pred
 }
 
 __global__ __launch_bounds__(kOrderBlock) void k_tile_order(const int* cost, int nTiles, int* order) {
-    __shared__ int base[kCostBuckets];            // first slot of each bucket in the band
-    __shared__ int run[kCostBuckets];             // tiles of each bucket placed so far
-    __shared__ int wcnt[kOrderBlock / 64][kCostBuckets];
+    constexpr int NW = kOrderBlock / 64, NH = kCostBuckets;
+    __shared__ int hbase[NH];                     // first slot of each heavy bucket in the band
+    __shared__ int hfill[NH];                     // heavy tiles of each bucket placed so far
+    __shared__ int wl[2][NW];                     // light tiles per wave in a round (alternating buffers)
+    __shared__ int nheavy, nlight;                // heavy tiles of the band; light tiles placed so far
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int lo = (int)(((long long)nTiles * b) / rtd::kBands);
     const int hi = (int)(((long long)nTiles * (b + 1)) / rtd::kBands);
-    // bucket 0 first: heavy tiles by descending cost, the rest in bucket kCostBuckets - 1
-    auto bucket = [&](int c) {
+    // heavy bucket of tile c (0: slowest), -1 for a light tile
+    auto hbucket = [&](int c) {
         const int v = cost[c];
-        return v >= kCostHeavy ? kCostBuckets - 2 - min(v >> kCostShift, kCostBuckets - 2) : kCostBuckets - 1;
+        return v >= kCostHeavy ? NH - 1 - min(v >> kCostShift, NH - 1) : -1;
     };
-    if (tid < kCostBuckets) run[tid] = 0;
+    if (tid < NH) hfill[tid] = 0;
+    if (tid == 0) nlight = 0;
     __syncthreads();
-    for (int c = lo + tid; c < hi; c += kOrderBlock) atomicAdd(&run[bucket(c)], 1);
+    for (int c = lo + tid; c < hi; c += kOrderBlock) {
+        const int k = hbucket(c);
+        if (k >= 0) atomicAdd(&hfill[k], 1);
+    }
     __syncthreads();
     if (tid == 0) {
         int acc = 0;
-        for (int k = 0; k < kCostBuckets; k++) { base[k] = acc; acc += run[k]; run[k] = 0; }
+        for (int k = 0; k < NH; k++) { hbase[k] = acc; acc += hfill[k]; hfill[k] = 0; }
+        nheavy = acc;
     }
     __syncthreads();
+    // heavy tiles: slots [0, nheavy) by bucket (any order inside a bucket);
+    // light tiles: slots [nheavy, band size) in strip order (a stable
+    // partition: one ballot per round, ranks from the waves before)
     const uint64_t below = (1ull << lane) - 1;
-    for (int c0 = lo; c0 < hi; c0 += kOrderBlock) {
+    int buf = 0;
+    for (int c0 = lo; c0 < hi; c0 += kOrderBlock, buf ^= 1) {
         const int c = c0 + tid;
-        const bool ok = c < hi;
-        const int k = ok ? bucket(c) : -1;
-        int rank = 0;
-        for (int j = 0; j < kCostBuckets; j++) {
-            const uint64_t m = __ballot(k == j);
-            if (k == j) rank = __popcll(m & below);
-            if (lane == 0) wcnt[w][j] = __popcll(m);
+        const int k = c < hi ? hbucket(c) : -2;
+        const uint64_t m = __ballot(k == -1);
+        if (lane == 0) wl[buf][w] = __popcll(m);
+        __syncthreads();
+        if (k == -1) {
+            int before = nlight + __popcll(m & below);
+            for (int v = 0; v < w; v++) before += wl[buf][v];
+            order[lo + nheavy + before] = c;
+        } else if (k >= 0) {
+            order[lo + hbase[k] + atomicAdd(&hfill[k], 1)] = c;
         }
-        __syncthreads();
-        if (tid < kCostBuckets) {                  // per bucket: exclusive prefix over the waves
-            int r = run[tid];
-            for (int v = 0; v < kOrderBlock / 64; v++) { const int x = wcnt[v][tid]; wcnt[v][tid] = r; r += x; }
-            run[tid] = r;
+        __syncthreads();                          // nlight and wl[buf] read by every wave
+        if (tid == 0) {
+            int t = 0;
+            for (int v = 0; v < NW; v++) t += wl[buf][v];
+            nlight += t;
         }
-        __syncthreads();
-        if (ok) order[lo + base[k] + wcnt[w][k] + rank] = c;
-        __syncthreads();
     }
 }
 
