@@ -60,7 +60,10 @@ float cudaBVH_traceDiffuse(CudaBVH_t bvh, int numRays,
                            float* outHitT);
 
 /* layer/cuda_bvh_backend.h:45-47. Look-at camera toward the scene centre,
- * shaded RGBA8 into a host buffer. 0 on success. */
+ * shaded RGBA8 into a host buffer. 0 on success. This and the other display
+ * calls (traceToGPUPtr, traceToGPUAsync, traceToHostPtr) start their 8x8
+ * tiles in the cost order of the last frame of the same size (as
+ * CUDABVH_FLAG_COST_ORDER); the image does not depend on the order. */
 int cudaBVH_traceToRGBA(CudaBVH_t bvh, int width, int height,
                         float camOx, float camOy, float camOz,
                         uint32_t* outRGBA_host);

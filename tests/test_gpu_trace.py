@@ -336,6 +336,13 @@ def test_shading_formats(pkg, oracle, parity_bvh, dev):
     assert np.array_equal(r8, (rgb[..., 0] * 255).astype(np.uint8).astype(np.int32))
     b8 = (bgra.cpu().numpy().reshape(H, W).view(np.uint32) & 0xFF).astype(np.int32)
     assert np.array_equal(b8, (rgb[..., 2] * 255).astype(np.uint8).astype(np.int32))
+    # the display calls take their tiles in the cost order of the previous
+    # frame of the same size: later frames are the same image bit for bit
+    for _ in range(2):
+        again = torch.full((W * H * 4,), -1.0, device=dev)
+        parity_bvh.trace_to_gpu_ptr(W, H, cam_o, again.data_ptr(), 3)
+        torch.cuda.synchronize()
+        assert np.array_equal(again.cpu().numpy().view(np.uint32), f32.cpu().numpy().view(np.uint32))
 
 
 def test_determinism_and_stream(pkg, oracle, scenes, parity_bvh, dev):

@@ -350,6 +350,10 @@ static int shade_to_device(CudaBVHHandle* h, rt::TraceScratch& sc, hipStream_t s
     a.n = width * height; a.W = width; a.H = height;
     a.cam = lookat_cam(h, width, height, cx, cy, cz);
     a.img = dimg; a.fmt = fmt;
+    // a frame-after-frame display call: its tiles start in the cost order of
+    // the last frame of this size traced with sc (the image is the same in
+    // any order)
+    a.costOrder = true;
     return rtk::launch_trace(h, sc, a, rtk::kModeShade, false, true, false, s);
 }
 
