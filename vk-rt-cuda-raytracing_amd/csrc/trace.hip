@@ -244,7 +244,9 @@ __global__ void k_cost_dilate(const int* cost, int tilesX, int nTiles, int* pred
     pred[c] = m;
 }
 
-__global__ __launch_bounds__(kOrderBlock) void k_tile_order(const int* cost, int nTiles, int* order) {
+// Also zeroes band b's work counter for the trace launch that follows (in
+// place of k_reset_counter).
+__global__ __launch_bounds__(kOrderBlock) void k_tile_order(const int* cost, int nTiles, int* order, uint32_t* counter) {
     constexpr int NW = kOrderBlock / 64, NH = kCostBuckets;
     __shared__ int hbase[NH];                     // first slot of each heavy bucket in the band
     __shared__ int hfill[NH];                     // heavy tiles of each bucket placed so far
@@ -259,7 +261,10 @@ __global__ __launch_bounds__(kOrderBlock) void k_tile_order(const int* cost, int
         return v >= kCostHeavy ? NH - 1 - min(v >> kCostShift, NH - 1) : -1;
     };
     if (tid < NH) hfill[tid] = 0;
-    if (tid == 0) nlight = 0;
+    if (tid == 0) {
+        nlight = 0;
+        counter[b * rtd::kBandStride] = 0u;
+    }
     __syncthreads();
     for (int c = lo + tid; c < hi; c += kOrderBlock) {
         const int k = hbucket(c);
@@ -1174,7 +1179,7 @@ static int launch_kernel(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs
         hipLaunchKernelGGL(k_cost_dilate, dim3((a.nTiles + 255) / 256), dim3(256), 0, s, sc.tileCost, a.tilesX,
                            a.nTiles, sc.tilePred);
         hipLaunchKernelGGL(k_tile_order, dim3(rtd::kBands), dim3(kOrderBlock), 0, s, sc.tilePred, a.nTiles,
-                           sc.tileOrder);
+                           sc.tileOrder, sc.counter);
         a.tileOrder = sc.tileOrder;
         a.tileCost = sc.tileCost;
     }
@@ -1190,7 +1195,8 @@ static int launch_kernel(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs
         if (grow_hits(sc, (size_t)a.n * sizeof(uint4)) != 0) return -1;
         a.hits = (uint4*)sc.hits;
     }
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(rtd::kBands), 0, s, sc.counter);
+    // (a cost-ordered launch's k_tile_order has zeroed the counters)
+    if (!a.tileOrder) hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(rtd::kBands), 0, s, sc.counter);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s, a);
     if (a.hits) {
         const int g = std::min((a.n + 255) / 256, 4096);
