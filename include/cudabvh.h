@@ -61,7 +61,8 @@ float cudaBVH_traceDiffuse(CudaBVH_t bvh, int numRays,
 
 /* layer/cuda_bvh_backend.h:45-47. Look-at camera toward the scene centre,
  * shaded RGBA8 into a host buffer. 0 on success. This and the other display
- * calls (traceToGPUPtr, traceToGPUAsync, traceToHostPtr) start their 8x8
+ * calls (traceToGPUPtr, traceToGPUAsync, traceToHostPtr) and
+ * cudaBVH_tracePrimary start their 8x8
  * tiles in the cost order of the last frame of the same size (as
  * CUDABVH_FLAG_COST_ORDER); the image does not depend on the order. */
 int cudaBVH_traceToRGBA(CudaBVH_t bvh, int width, int height,

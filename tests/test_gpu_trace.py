@@ -269,6 +269,11 @@ def test_reference_trace_primary(pkg, oracle, parity_scene, parity_bvh):
     ref = oracle.trace_closest(nodes, lt, pid, rays)
     assert _same_bits(out, ref["t"])
     assert (out < 1e20).mean() > 0.1
+    # later frames run in the cost order of the previous one: same distances
+    for _ in range(2):
+        again = np.full(side * side, -1.0, np.float32)
+        assert parity_bvh.trace_primary(side, cam_o, again) > 0
+        assert _same_bits(again, ref["t"])
 
 
 def test_reference_trace_diffuse(pkg, oracle, parity_bvh):

@@ -572,6 +572,7 @@ float cudaBVH_tracePrimary(CudaBVH_t bvh, int side, float camOx, float camOy, fl
     a.cam.f[2] = 1.f; a.cam.r[0] = 1.f; a.cam.u[1] = 1.f;   // looking +z, 90 degrees (backend :69-74)
     a.cam.sx = 1.f; a.cam.sy = 1.f;
     a.t = dt;
+    a.costOrder = true;              // repeated per frame: tiles in the last frame's cost order
     hipEvent_t e0, e1;
     hipEventCreate(&e0); hipEventCreate(&e1);
     hipEventRecord(e0, hstream(bvh));
