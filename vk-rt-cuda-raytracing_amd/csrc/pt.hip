@@ -49,6 +49,8 @@ struct PtArgs {
     PtQueue q[2];         // extension queues (ping-pong by bounce parity)
     PtQueue t;            // staging queue of the octant sort (shade appends here, the sort scatters to q)
     int* oct;             // [b * 8 + o] rays of bounce b + 1 in direction octant o; [128 + b * 8 + o] sort cursors
+    int* camCost;         // camera-ray trace: step count of each 64-ray chunk in the last sample
+    int* camOrder;        // and the chunk order built from them (slowest first)
     int sortOct;          // sort each bounce's next extension rays by direction octant
     float* s[8];          // shadow rays
     float* sc[3];         // shadow contribution
@@ -444,6 +446,17 @@ static int pt_extend(CudaSceneHandle* sc, PtArgs& a, int b, hipStream_t s) {
     t.tmin = q.r[6]; t.tmax = q.r[7];
     t.t = a.ht; t.inst_out = a.hinst; t.id = a.hprim;
     t.gate = &a.cnt[b]; t.gateMax = a.tailPaths;
+    if (b == 0) {
+        // camera rays: every pixel, slot i holding its 8x8-tile-strip pixel,
+        // so a 64-ray chunk is a tile; the batch kernels take the chunks in
+        // the cost order of the previous sample's camera-ray trace
+        if (launch_chunk_order(a.camCost, (a.N + 63) / 64, a.camOrder, s) != 0) {
+            rt::set_error("path tracer: chunk order launch failed");
+            return -1;
+        }
+        t.chunkOrder = a.camOrder;
+        t.chunkCost = a.camCost;
+    }
     return launch_trace2l(sc, sc->scratch, t, b == 0 ? 0 : 2, s);
 }
 
@@ -492,7 +505,8 @@ static int pt_alloc(PtResources& R, int N) {
     // per ray: 3 queues (two ping-pong, one octant-sort staging) x (8 + 3 floats, pix, rng 8 B)
     // + shadow (8 + 3 floats, pix, occ) + hit (t, inst, prim)
     const size_t per = 3 * (11 * 4 + 4 + 8) + (11 * 4 + 4 + 1) + 12;
-    const size_t bytes = per * (size_t)N + 4096;
+    const size_t chunkBytes = 2 * ((size_t)(N + 63) / 64 * 4 + 256);     // camera-ray chunk costs and order
+    const size_t bytes = per * (size_t)N + chunkBytes + 4096;
     if (hipMalloc(&R.block, bytes) != hipSuccess) {
         rt::set_error("path tracer: cannot allocate %zu bytes of queues", bytes);
         return -1;
@@ -523,6 +537,11 @@ static void pt_layout(PtResources& R, PtArgs& a) {
     a.hinst = (int*)take(4 * N);
     a.hprim = (int*)take(4 * N);
     a.occ = (uint8_t*)take(N);
+    // the camera rays of one sample are traced slowest chunk first by the
+    // previous sample's step counts (same camera, new jitter): any values
+    // give a valid order, so a fresh allocation needs no clearing
+    a.camCost = (int*)take((N + 63) / 64 * 4);
+    a.camOrder = (int*)take((N + 63) / 64 * 4);
     a.cnt = R.small;                                           // 2 * kPtMaxDepth ints
     a.frame = R.small + 2 * kPtMaxDepth;                       // 2 ints
     a.totals = (unsigned long long*)(R.small + 4 * kPtMaxDepth);   // 8-byte aligned

@@ -54,6 +54,9 @@ struct Trace2LArgs {
     int2* spill = nullptr; int spillStride = 0; int maxStack = 0;
     int* status = nullptr;
     const int* gate = nullptr; int gateMax = -1;   // skip the launch when *gate <= gateMax
+    // batch schedule: chunk c of 64 rays traces chunk chunkOrder[c], and each
+    // chunk's step count (its slowest lane's) goes to chunkCost[chunk]
+    const int* chunkOrder = nullptr; int* chunkCost = nullptr;
 };
 
 }  // namespace rtk

@@ -208,28 +208,40 @@ __global__ __launch_bounds__(BLOCK) void k_trace2l_batch(Trace2LArgs a) {
     if (a.gate && *a.gate <= a.gateMax) return;
     const int n = a.nDev ? min(*a.nDev, a.n) : a.n;
     int band = blockIdx.x % rtd::kBands;
+    const int nChunks = (n + 63) >> 6;
     for (;;) {
         const int base = rtd::take_chunk(a.counter, n, band);
         if (base >= n) break;
-        const int ri = base + lane;
-        if (ri >= n) continue;
-        Ray r;
-        r.ox = a.ox[ri]; r.oy = a.oy[ri]; r.oz = a.oz[ri];
-        r.dx = a.dx[ri]; r.dy = a.dy[ri]; r.dz = a.dz[ri];
-        r.tmin = a.tmin[ri]; r.tmax = a.tmax[ri];
-        rtd::ray_setup(r);
-        LdsStack<LDSD, BLOCK> st{s_ref, s_tn, a.spill, (size_t)a.spillStride, tid, gtid, a.maxStack, 0, a.status};
-        float t, u, v;
-        int inst, id;
-        const bool hit = trace2l_ray<ANY, ORD>(a.tlasNodes, a.tlasTris, a.inst, a.blas, r, st, t, inst, id, u, v);
-        if constexpr (ANY) {
-            a.occ[ri] = hit ? 1 : 0;
-        } else {
-            a.t[ri] = hit ? t : RT_MISS_T;
-            if (a.inst_out) a.inst_out[ri] = hit ? inst : -1;
-            if (a.id) a.id[ri] = hit ? id : -1;
-            if (a.u) a.u[ri] = hit ? u : 0.f;
-            if (a.v) a.v[ri] = hit ? v : 0.f;
+        // chunk c traces chunk c, or chunkOrder[c] in a cost-ordered launch
+        int chunk = base >> 6;
+        if (a.chunkOrder) chunk = a.chunkOrder[chunk];
+        const bool chunkOk = (unsigned)chunk < (unsigned)nChunks;
+        const int ri = chunk * 64 + lane;
+        int steps = 0;
+        if (chunkOk && ri < n) {
+            Ray r;
+            r.ox = a.ox[ri]; r.oy = a.oy[ri]; r.oz = a.oz[ri];
+            r.dx = a.dx[ri]; r.dy = a.dy[ri]; r.dz = a.dz[ri];
+            r.tmin = a.tmin[ri]; r.tmax = a.tmax[ri];
+            rtd::ray_setup(r);
+            LdsStack<LDSD, BLOCK> st{s_ref, s_tn, a.spill, (size_t)a.spillStride, tid, gtid, a.maxStack, 0, a.status};
+            float t, u, v;
+            int inst, id;
+            const bool hit =
+                trace2l_ray<ANY, ORD>(a.tlasNodes, a.tlasTris, a.inst, a.blas, r, st, t, inst, id, u, v, &steps);
+            if constexpr (ANY) {
+                a.occ[ri] = hit ? 1 : 0;
+            } else {
+                a.t[ri] = hit ? t : RT_MISS_T;
+                if (a.inst_out) a.inst_out[ri] = hit ? inst : -1;
+                if (a.id) a.id[ri] = hit ? id : -1;
+                if (a.u) a.u[ri] = hit ? u : 0.f;
+                if (a.v) a.v[ri] = hit ? v : 0.f;
+            }
+        }
+        if (a.chunkCost) {
+            const int m = rtd::wave_max(steps);
+            if (lane == 0 && chunkOk) a.chunkCost[chunk] = m;
         }
     }
 }
@@ -406,6 +418,8 @@ static int launch2(Trace2LArgs& a, rt::TraceScratch& sc, int maxStack, int block
     const int grid = perCU * cus, threads = grid * kBlock2;
     rt::note_launch(kern, kBlock2, ki, perCU, grid, sc.launchInfo);
     a.maxStack = maxStack;
+    // only the batch kernels take a chunk order
+    if (SCHED != 1 && SCHED != 2) { a.chunkOrder = nullptr; a.chunkCost = nullptr; }
     const int spillDepth = maxStack > LDSD ? maxStack - LDSD : 0;
     const size_t need = (size_t)spillDepth * threads * sizeof(int2);
     if (alloc_scratch(sc) != 0) return -1;

@@ -171,7 +171,7 @@ struct LdsStack {
 template <bool ANY, bool ORD, class Stack>
 __device__ __forceinline__ bool trace2l_ray(const uint4* tlasNodes, const float4* tlasTris, const InstanceDev* insts,
                                             const BlasDev* blas, const Ray& rw, Stack& st, float& bt, int& binst,
-                                            int& bid, float& bu, float& bv) {
+                                            int& bid, float& bu, float& bv, int* stepsOut = nullptr) {
     Ray ro;
     int ref = 0, inst = -1;
     int lts = 0, ltc = 0;                   // leaf cursor: next triangle, triangles left
@@ -195,7 +195,10 @@ __device__ __forceinline__ bool trace2l_ray(const uint4* tlasNodes, const float4
         if (ltc == 0) {
             while (ref == -1 || ref == kReturnMark || (!ANY && tn > best)) {
                 if (ref == kReturnMark) { inst = -1; nodes = tlasNodes; ref = -1; continue; }
-                if (!st.pop(ref, tn)) return ANY ? false : bid >= 0;
+                if (!st.pop(ref, tn)) {
+                    if (stepsOut) *stepsOut = steps;
+                    return ANY ? false : bid >= 0;
+                }
             }
             if (ref < 0 && inst >= 0) {         // BLAS leaf: start its cursor
                 const int enc = -(ref + 2);
@@ -227,8 +230,10 @@ __device__ __forceinline__ bool trace2l_ray(const uint4* tlasNodes, const float4
                 }
                 return false;
             };
-            if (test(p0, p1, p2)) return true;
-            if (two && test(q0, q1, q2)) return true;
+            if ((test(p0, p1, p2)) || (two && test(q0, q1, q2))) {
+                if (stepsOut) *stepsOut = steps;
+                return true;
+            }
         } else if (ref >= 0) {
             const bool w = inst < 0;
             const Ray& r = w ? rw : ro;

@@ -245,7 +245,7 @@ __global__ void k_cost_dilate(const int* cost, int tilesX, int nTiles, int* pred
 }
 
 // Also zeroes band b's work counter for the trace launch that follows (in
-// place of k_reset_counter).
+// place of k_reset_counter), unless counter is null.
 __global__ __launch_bounds__(kOrderBlock) void k_tile_order(const int* cost, int nTiles, int* order, uint32_t* counter) {
     constexpr int NW = kOrderBlock / 64, NH = kCostBuckets;
     __shared__ int hbase[NH];                     // first slot of each heavy bucket in the band
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kOrderBlock) void k_tile_order(const int* cost, int
     if (tid < NH) hfill[tid] = 0;
     if (tid == 0) {
         nlight = 0;
-        counter[b * rtd::kBandStride] = 0u;
+        if (counter) counter[b * rtd::kBandStride] = 0u;
     }
     __syncthreads();
     for (int c = lo + tid; c < hi; c += kOrderBlock) {
@@ -1228,6 +1228,12 @@ static int launch_packet(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs
     static rt::KernelInfo ki;
     a.costOrder = false;
     return launch_kernel(h, sc, a, k_trace_packet<MODE, MT, LDSD, kBlock>, LDSD, true, blocksPerCU, ki, s);
+}
+
+int launch_chunk_order(const int* cost, int nChunks, int* order, hipStream_t s) {
+    if (nChunks <= 0) return 0;
+    hipLaunchKernelGGL(k_tile_order, dim3(rtd::kBands), dim3(kOrderBlock), 0, s, cost, nChunks, order, nullptr);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int alloc_scratch(rt::TraceScratch& sc) {

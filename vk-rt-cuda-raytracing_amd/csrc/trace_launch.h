@@ -68,6 +68,11 @@ struct TraceArgs {
 int launch_trace(const CudaBVHHandle* h, rt::TraceScratch& sc, TraceArgs& a, int mode, bool mt, bool pinhole,
                  bool counted, hipStream_t s);
 int alloc_scratch(rt::TraceScratch& sc);
+// Orders nChunks work chunks of 64 items slowest first by their recorded
+// step counts (cost, any int values), per XCD band as the pinhole tile
+// order does: order[c] is the chunk to trace c-th. Stream-ordered, no
+// allocation (graph-capturable).
+int launch_chunk_order(const int* cost, int nChunks, int* order, hipStream_t s);
 void free_scratch(rt::TraceScratch& sc);
 // Grows the staged hit records of sc to at least need bytes (bumps sc.gen
 // when it reallocates); 0 on success.
